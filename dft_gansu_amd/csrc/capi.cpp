@@ -1,0 +1,204 @@
+// extern "C" boundary (include/gansu_fock.h): every entry point converts C++
+// exceptions into an error code plus a thread-local message.
+#include <cstring>
+#include <string>
+
+#include "engine.hpp"
+#include "gansu_fock.h"
+
+namespace gansu {
+struct ScfResult {
+  double e_elec = 0, e_total = 0;
+  int iterations = 0, fock_builds = 0, converged = 0;
+  std::vector<double> D;
+};
+ScfResult rhf_run(const Molecule& mol, const Molecule* aux, const Params& prm);
+void compute_oneint(const Molecule& mol, double* dS, double* dT, double* dV, hipStream_t s);
+
+namespace {
+thread_local std::string g_last_error;
+}
+void set_last_error(const std::string& msg) { g_last_error = msg; }
+}  // namespace gansu
+
+struct gansu_molecule { gansu::Molecule m; };
+struct gansu_params { gansu::Params p; };
+struct gansu_fock_engine { std::unique_ptr<gansu::FockEngine> e; };
+
+namespace {
+template <class F>
+int guarded(F&& f) {
+  try {
+    f();
+    return GANSU_OK;
+  } catch (const gansu::Error& e) {
+    gansu::set_last_error(e.what());
+    return e.code;
+  } catch (const std::exception& e) {
+    gansu::set_last_error(e.what());
+    return GANSU_E_ARG;
+  } catch (...) {
+    gansu::set_last_error("unknown error");
+    return GANSU_E_ARG;
+  }
+}
+}  // namespace
+
+extern "C" {
+
+const char* gansu_last_error(void) { return gansu::g_last_error.c_str(); }
+const char* gansu_version(void) { return "gansu-mi355x 0.1 (gfx950)"; }
+
+int gansu_molecule_from_xyz(const char* xyz_path, const char* gbs_path, int charge, int beta_to_alpha,
+                            int normalization, gansu_molecule** out) {
+  return guarded([&] {
+    if (!xyz_path || !gbs_path || !out) throw gansu::Error(GANSU_E_ARG, "null argument");
+    auto atoms = gansu::read_xyz(xyz_path);
+    *out = new gansu_molecule{gansu::build_molecule(atoms, gbs_path, charge, beta_to_alpha, normalization)};
+  });
+}
+
+int gansu_molecule_from_atoms(int n_atoms, const int* Z, const double* xyz_bohr, const char* gbs_path, int charge,
+                              int beta_to_alpha, int normalization, gansu_molecule** out) {
+  return guarded([&] {
+    if (n_atoms <= 0 || !Z || !xyz_bohr || !gbs_path || !out) throw gansu::Error(GANSU_E_ARG, "null argument");
+    std::vector<gansu::Atom> atoms(n_atoms);
+    for (int i = 0; i < n_atoms; ++i) {
+      atoms[i].Z = Z[i];
+      for (int k = 0; k < 3; ++k) atoms[i].xyz[k] = xyz_bohr[3 * i + k];
+    }
+    *out = new gansu_molecule{gansu::build_molecule(atoms, gbs_path, charge, beta_to_alpha, normalization)};
+  });
+}
+
+int gansu_molecule_sizes(const gansu_molecule* mol, int* sizes) {
+  return guarded([&] {
+    if (!mol || !sizes) throw gansu::Error(GANSU_E_ARG, "null argument");
+    const auto& m = mol->m;
+    sizes[0] = (int)m.prims.size(); sizes[1] = m.nbasis; sizes[2] = (int)m.types.size();
+    sizes[3] = (int)m.atoms.size(); sizes[4] = m.n_electrons; sizes[5] = m.n_alpha; sizes[6] = m.n_beta;
+  });
+}
+
+int gansu_molecule_get(const gansu_molecule* mol, gansu_primitive_shell* prims, gansu_shell_type_info* types,
+                       double* norm, int* Z, double* xyz_bohr) {
+  return guarded([&] {
+    if (!mol) throw gansu::Error(GANSU_E_ARG, "null molecule");
+    const auto& m = mol->m;
+    if (prims) std::memcpy(prims, m.prims.data(), m.prims.size() * sizeof(gansu_primitive_shell));
+    if (types) std::memcpy(types, m.types.data(), m.types.size() * sizeof(gansu_shell_type_info));
+    if (norm) std::memcpy(norm, m.norm.data(), m.norm.size() * sizeof(double));
+    for (size_t i = 0; i < m.atoms.size(); ++i) {
+      if (Z) Z[i] = m.atoms[i].Z;
+      if (xyz_bohr) for (int k = 0; k < 3; ++k) xyz_bohr[3 * i + k] = m.atoms[i].xyz[k];
+    }
+  });
+}
+
+double gansu_molecule_nuclear_repulsion(const gansu_molecule* mol) { return mol ? mol->m.nuclear_repulsion() : 0.0; }
+void gansu_molecule_destroy(gansu_molecule* mol) { delete mol; }
+
+int gansu_params_create(gansu_params** out) {
+  return guarded([&] {
+    if (!out) throw gansu::Error(GANSU_E_ARG, "null argument");
+    *out = new gansu_params{};
+  });
+}
+int gansu_params_load_recipe(gansu_params* p, const char* path) {
+  return guarded([&] {
+    if (!p || !path) throw gansu::Error(GANSU_E_ARG, "null argument");
+    p->p.load_recipe(path);
+  });
+}
+int gansu_params_set(gansu_params* p, const char* key, const char* value) {
+  return guarded([&] {
+    if (!p || !key || !value) throw gansu::Error(GANSU_E_ARG, "null argument");
+    p->p.set(key, value, false);
+  });
+}
+int gansu_params_get(const gansu_params* p, const char* key, char* buf, int buflen) {
+  return guarded([&] {
+    if (!p || !key || !buf || buflen <= 0) throw gansu::Error(GANSU_E_ARG, "null argument");
+    const std::string v = p->p.get(key);
+    if ((int)v.size() + 1 > buflen) throw gansu::Error(GANSU_E_ARG, "buffer too small");
+    std::memcpy(buf, v.c_str(), v.size() + 1);
+  });
+}
+void gansu_params_destroy(gansu_params* p) { delete p; }
+
+int gansu_fock_create(const gansu_primitive_shell* prims, size_t n_prims, const gansu_shell_type_info* types,
+                      int n_types, const double* cgto_norm, int num_basis, const gansu_primitive_shell* aux,
+                      size_t n_aux, const gansu_shell_type_info* aux_types, int n_aux_types, const double* aux_norm,
+                      int num_aux, const char* options, gansu_fock_engine** out) {
+  return guarded([&] {
+    if (!out) throw gansu::Error(GANSU_E_ARG, "null output");
+    auto opts = gansu::parse_options(options);
+    auto e = std::make_unique<gansu::FockEngine>(prims, n_prims, types, n_types, cgto_norm, num_basis, aux, n_aux,
+                                                 aux_types, n_aux_types, aux_norm, num_aux, opts);
+    *out = new gansu_fock_engine{std::move(e)};
+  });
+}
+
+int gansu_fock_precompute(gansu_fock_engine* e) {
+  return guarded([&] {
+    if (!e) throw gansu::Error(GANSU_E_ARG, "null engine");
+    e->e->precompute();
+  });
+}
+
+int gansu_fock_build_rhf(gansu_fock_engine* e, const double* d_D, const double* d_H, double* d_F, void* stream) {
+  return guarded([&] {
+    if (!e || !d_D || !d_H || !d_F) throw gansu::Error(GANSU_E_ARG, "null argument");
+    e->e->build_jk(d_D, nullptr, nullptr, d_H, d_F, (hipStream_t)stream);
+  });
+}
+
+int gansu_fock_build_jk(gansu_fock_engine* e, const double* d_D, double* d_J, double* d_K, void* stream) {
+  return guarded([&] {
+    if (!e || !d_D || !d_J || !d_K) throw gansu::Error(GANSU_E_ARG, "null argument");
+    e->e->build_jk(d_D, d_J, d_K, nullptr, nullptr, (hipStream_t)stream);
+  });
+}
+
+const char* gansu_fock_algorithm_name(const gansu_fock_engine* e) { return e ? e->e->algorithm_name() : ""; }
+
+int gansu_fock_schwarz(gansu_fock_engine* e, double* host_Q, size_t n) {
+  return guarded([&] {
+    if (!e || !host_Q) throw gansu::Error(GANSU_E_ARG, "null argument");
+    auto q = e->e->schwarz_ref();
+    if (n != q.size()) throw gansu::Error(GANSU_E_ARG, "size mismatch: engine has " + std::to_string(q.size()));
+    std::memcpy(host_Q, q.data(), n * sizeof(double));
+  });
+}
+
+size_t gansu_fock_num_primitive_pairs(const gansu_fock_engine* e) { return e ? e->e->num_primitive_pairs() : 0; }
+
+int gansu_fock_stats(const gansu_fock_engine* e, double* stats) {
+  return guarded([&] {
+    if (!e || !stats) throw gansu::Error(GANSU_E_ARG, "null argument");
+    e->e->stats(stats);
+  });
+}
+
+void gansu_fock_destroy(gansu_fock_engine* e) { delete e; }
+
+int gansu_oneint(const gansu_molecule* mol, double* d_S, double* d_T, double* d_V, void* stream) {
+  return guarded([&] {
+    if (!mol || !d_S || !d_T || !d_V) throw gansu::Error(GANSU_E_ARG, "null argument");
+    gansu::compute_oneint(mol->m, d_S, d_T, d_V, (hipStream_t)stream);
+  });
+}
+
+int gansu_rhf_run(const gansu_molecule* mol, const gansu_molecule* aux_mol, const gansu_params* p, double* energies,
+                  double* D_out, int* info) {
+  return guarded([&] {
+    if (!mol || !p || !energies) throw gansu::Error(GANSU_E_ARG, "null argument");
+    auto r = gansu::rhf_run(mol->m, aux_mol ? &aux_mol->m : nullptr, p->p);
+    energies[0] = r.e_elec;
+    energies[1] = r.e_total;
+    if (D_out) std::memcpy(D_out, r.D.data(), r.D.size() * sizeof(double));
+    if (info) { info[0] = r.iterations; info[1] = r.fock_builds; info[2] = r.converged; info[3] = 0; }
+  });
+}
+
+}  // extern "C"

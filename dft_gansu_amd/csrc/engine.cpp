@@ -1,0 +1,288 @@
+// Fock engine host side: screening, work decomposition and build orchestration
+// for the four-centre paths (stored / direct).  The RI path lives in ri.cpp.
+//
+// Reference counterparts: ERI_Stored::precomputation (eri.cu:60-116),
+// computeSchwarzUpperBounds (gpu_manager.cu:2154-2179), makeShellPairTypeInfo
+// (gpu_manager.cu:413-432), computeERIMatrix (gpu_manager.cu:434-517) and
+// computeFockMatrix_RHF (gpu_manager.cu:678-691).
+#include "engine.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace gansu {
+
+void hip_check(hipError_t e, const char* what) {
+  if (e != hipSuccess) throw Error(GANSU_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+namespace {
+
+double dfact(int n) {
+  double r = 1.0;
+  for (int k = n; k > 1; k -= 2) r *= k;
+  return r;
+}
+
+// 1/sqrt((2l-1)!!(2m-1)!!(2n-1)!!): the component-dependent part of the
+// primitive norm (int2e.hpp:127-130), folded into the raw-basis scaling.
+double component_factor(int L, int c, bool aux) {
+  int e[3];
+  // orbital order (types.hpp:197-228); aux f uses the RI order (int2c2e.hpp:39-47)
+  static const int d2[6][3] = {{2, 0, 0}, {0, 2, 0}, {0, 0, 2}, {1, 1, 0}, {1, 0, 1}, {0, 1, 1}};
+  static const int f3o[10][3] = {{3, 0, 0}, {0, 3, 0}, {0, 0, 3}, {1, 2, 0}, {2, 1, 0},
+                                 {2, 0, 1}, {1, 0, 2}, {0, 1, 2}, {0, 2, 1}, {1, 1, 1}};
+  static const int f3r[10][3] = {{3, 0, 0}, {0, 3, 0}, {0, 0, 3}, {2, 1, 0}, {2, 0, 1},
+                                 {0, 2, 1}, {1, 2, 0}, {0, 1, 2}, {1, 0, 2}, {1, 1, 1}};
+  for (int d = 0; d < 3; ++d) {
+    if (L == 0) e[d] = 0;
+    else if (L == 1) e[d] = (c == d);
+    else if (L == 2) e[d] = d2[c][d];
+    else e[d] = aux ? f3r[c][d] : f3o[c][d];
+  }
+  return 1.0 / std::sqrt(dfact(2 * e[0] - 1) * dfact(2 * e[1] - 1) * dfact(2 * e[2] - 1));
+}
+
+int pair_class_id(int la, int lb) { return la * (la + 1) / 2 + lb; }
+
+}  // namespace
+
+FockEngine::FockEngine(const gansu_primitive_shell* prims, size_t n_prims, const gansu_shell_type_info* types,
+                       int n_types, const double* norm, int num_basis, const gansu_primitive_shell* aux, size_t n_aux,
+                       const gansu_shell_type_info* aux_types, int n_aux_types, const double* aux_norm, int num_aux,
+                       const std::map<std::string, std::string>& opts) {
+  if (!prims || n_prims == 0 || !types || n_types <= 0 || !norm || num_basis <= 0)
+    throw Error(GANSU_E_ARG, "empty primitive list or basis");
+  method_ = "stored";
+  for (const auto& kv : opts) {
+    if (kv.first == "eri_method") method_ = kv.second;
+    else if (kv.first == "schwarz_screening_threshold") thr_ = std::stod(kv.second);
+    else if (kv.first == "precision") {
+      if (kv.second != "fp64" && kv.second != "mixed") throw Error(GANSU_E_PARAM, "precision must be fp64|mixed");
+      mixed_ = kv.second == "mixed";
+    } else if (kv.first == "fp32_bound_threshold") fp32_thr_ = std::stod(kv.second);
+    else throw Error(GANSU_E_PARAM, "Unknown engine option: " + kv.first);
+  }
+  if (method_ != "stored" && method_ != "direct" && method_ != "ri")
+    throw Error(GANSU_E_PARAM, "Unknown eri_method: " + method_ + " (stored|direct|ri)");
+  prims_.assign(prims, prims + n_prims);
+  types_.assign(types, types + n_types);
+  norm_.assign(norm, norm + num_basis);
+  M_ = num_basis;
+  size_t tot = 0;
+  for (int t = 0; t < n_types; ++t) {
+    if ((size_t)types[t].start_index != tot) throw Error(GANSU_E_ARG, "shell type infos are not contiguous");
+    tot += types[t].count;
+    for (int k = 0; k < types[t].count; ++k)
+      if (prims[types[t].start_index + k].shell_type != t) throw Error(GANSU_E_ARG, "primitives not sorted by type");
+  }
+  if (tot != n_prims) throw Error(GANSU_E_ARG, "shell type counts do not cover the primitive list");
+  shells_ = contracted_shells(prims, n_prims, kMaxL);
+  w_h_.assign(M_, 0.0);
+  for (const auto& s : shells_) {
+    if (s.basis + ncart(s.L) > M_) throw Error(GANSU_E_ARG, "basis_index out of range");
+    for (int c = 0; c < ncart(s.L); ++c) w_h_[s.basis + c] = norm_[s.basis + c] * component_factor(s.L, c, false);
+  }
+  if (method_ == "ri") {
+    if (!aux || n_aux == 0 || !aux_types || n_aux_types <= 0 || !aux_norm || num_aux <= 0)
+      throw Error(GANSU_E_ARG, "eri_method=ri needs an auxiliary basis");
+    aux_prims_.assign(aux, aux + n_aux);
+    aux_types_.assign(aux_types, aux_types + n_aux_types);
+    aux_norm_.assign(aux_norm, aux_norm + num_aux);
+    Maux_ = num_aux;
+  }
+}
+
+FockEngine::~FockEngine() { ri_destroy(ri_); }
+
+const char* FockEngine::algorithm_name() const {
+  if (method_ == "stored") return "Stored";
+  if (method_ == "direct") return "Direct";
+  return "RI";
+}
+
+void FockEngine::precompute() {
+  GANSU_HIP(hipSetDevice(0));
+  if (!boys_.ptr) boys_.upload(boys_grid_xn());
+  d_w_.upload(w_h_);
+  d_Dp_.alloc((size_t)M_ * M_);
+  d_Jr_.alloc((size_t)M_ * M_);
+  d_Kr_.alloc((size_t)M_ * M_);
+  precompute_four_center();
+  if (method_ == "ri") precompute_ri();
+  GANSU_HIP(hipDeviceSynchronize());
+  precomputed_ = true;
+}
+
+void FockEngine::precompute_four_center() {
+  d_prims_.upload(prims_);
+  const int nt = (int)types_.size();
+  // reference primitive pair list (gpu_manager.cu:413-432 order)
+  std::vector<int> pa, pb;
+  std::vector<size_t> type_off((size_t)nt * nt, 0);
+  for (int s0 = 0; s0 < nt; ++s0)
+    for (int s1 = s0; s1 < nt; ++s1) {
+      type_off[(size_t)s0 * nt + s1] = pa.size();
+      const int c0 = types_[s0].count, c1 = types_[s1].count;
+      const int b0 = (int)types_[s0].start_index, b1 = (int)types_[s1].start_index;
+      if (s0 == s1) {
+        for (int r2 = 0; r2 < c0; ++r2)
+          for (int r1 = 0; r1 <= r2; ++r1) { pa.push_back(b0 + r1); pb.push_back(b0 + r2); }
+      } else {
+        for (int r1 = 0; r1 < c0; ++r1)
+          for (int r2 = 0; r2 < c1; ++r2) { pa.push_back(b0 + r1); pb.push_back(b1 + r2); }
+      }
+    }
+  ref_pairs_ = pa.size();
+  {
+    DevBuf<int> dpa, dpb;
+    dpa.upload(pa);
+    dpb.upload(pb);
+    d_Qref_.alloc(ref_pairs_);
+    GANSU_HIP(launch_schwarz_ref(d_prims_.ptr, dpa.ptr, dpb.ptr, ref_pairs_, boys_.ptr, 0, d_Qref_.ptr, 0));
+    GANSU_HIP(hipDeviceSynchronize());
+  }
+  Qref_h_ = d_Qref_.download();
+  auto refpos = [&](int x, int y) -> size_t {
+    int tx = prims_[x].shell_type, ty = prims_[y].shell_type;
+    if (tx > ty || (tx == ty && x > y)) { std::swap(x, y); std::swap(tx, ty); }
+    const size_t off = type_off[(size_t)tx * nt + ty];
+    const int rx = x - (int)types_[tx].start_index, ry = y - (int)types_[ty].start_index;
+    if (tx == ty) return off + (size_t)ry * (ry + 1) / 2 + rx;
+    return off + (size_t)rx * types_[ty].count + ry;
+  };
+  double qglob = 0.0;
+  for (double q : Qref_h_) qglob = std::max(qglob, q);
+
+  // contracted shell pairs, L_i >= L_j (i >= j for equal L)
+  std::vector<int> ppa, ppb;
+  std::vector<double> pq;
+  sp_h_.clear();
+  const int ns = (int)shells_.size();
+  for (int i = 0; i < ns; ++i)
+    for (int j = 0; j <= i; ++j) {
+      // shells_ is sorted by L, so L_i >= L_j here
+      const CShell &si = shells_[i], &sj = shells_[j];
+      SPair sp;
+      std::memset(&sp, 0, sizeof sp);
+      sp.prim_begin = (int)ppa.size();
+      sp.bi = si.basis; sp.bj = sj.basis; sp.si = i; sp.sj = j; sp.la = si.L; sp.lb = sj.L;
+      double qmax = 0.0;
+      for (int a : si.prims)
+        for (int b : sj.prims) {
+          const double q = Qref_h_[refpos(a, b)];
+          ppa.push_back(a); ppb.push_back(b); pq.push_back(q);
+          qmax = std::max(qmax, q);
+        }
+      sp.nprim = (int)ppa.size() - sp.prim_begin;
+      sp.qmax = qmax;
+      if (qmax * qglob < thr_) {  // no ket can rescue this pair
+        ppa.resize(sp.prim_begin); ppb.resize(sp.prim_begin); pq.resize(sp.prim_begin);
+        continue;
+      }
+      sp_h_.push_back(sp);
+    }
+  {
+    DevBuf<int> dpa, dpb;
+    DevBuf<double> dq;
+    dpa.upload(ppa); dpb.upload(ppb); dq.upload(pq);
+    d_pr_.alloc(ppa.size());
+    GANSU_HIP(launch_pair_data(d_prims_.ptr, dpa.ptr, dpb.ptr, dq.ptr, ppa.size(), d_pr_.ptr, 0));
+    GANSU_HIP(hipDeviceSynchronize());
+  }
+  d_sp_.upload(sp_h_);
+
+  // pair classes sorted by qmax (descending; ties by id)
+  const int npc = (kMaxL + 1) * (kMaxL + 2) / 2;
+  pclass_.clear();
+  pclass_.resize(npc);
+  for (int la = 0; la <= kMaxL; ++la)
+    for (int lb = 0; lb <= la; ++lb) { pclass_[pair_class_id(la, lb)].la = la; pclass_[pair_class_id(la, lb)].lb = lb; }
+  for (int id = 0; id < (int)sp_h_.size(); ++id) pclass_[pair_class_id(sp_h_[id].la, sp_h_[id].lb)].sorted.push_back(id);
+  for (auto& pc : pclass_) {
+    std::stable_sort(pc.sorted.begin(), pc.sorted.end(),
+                     [&](int x, int y) { return sp_h_[x].qmax > sp_h_[y].qmax; });
+    pc.list.upload(pc.sorted);
+  }
+  // quartet classes (bra class A >= ket class B) and their work blocks
+  qclass_.clear();
+  n_quartets_ = 0;
+  for (int A = 0; A < npc; ++A)
+    for (int B = 0; B <= A; ++B) {
+      auto qc = std::make_unique<QuartetClass>();
+      qc->A = A; qc->B = B;
+      qc->LA = pclass_[A].la; qc->LB = pclass_[A].lb; qc->LC = pclass_[B].la; qc->LD = pclass_[B].lb;
+      const auto& bl = pclass_[A].sorted;
+      const auto& kl = pclass_[B].sorted;
+      for (int s = 0; s < (int)bl.size(); ++s) {
+        const double qb = sp_h_[bl[s]].qmax;
+        // kets with qb * qk >= thr form a prefix of the descending list
+        int lo = 0, hi = (int)kl.size();
+        while (lo < hi) {
+          const int mid = (lo + hi) / 2;
+          if (qb * sp_h_[kl[mid]].qmax >= thr_) lo = mid + 1; else hi = mid;
+        }
+        int count = lo;
+        if (A == B) count = std::min(count, s + 1);
+        for (int k0 = 0; k0 < count; k0 += 64)
+          qc->blocks_h.push_back(BlockRec{s, k0, std::min(64, count - k0), 0});
+        qc->n_quartets += count;
+      }
+      n_quartets_ += qc->n_quartets;
+      qc->blocks.upload(qc->blocks_h);
+      qclass_.push_back(std::move(qc));
+    }
+
+  stored_bytes_ = 0;
+  if (method_ == "stored") {
+    EriArgs g{d_sp_.ptr, d_pr_.ptr, boys_.ptr, thr_, d_Dp_.ptr, M_, d_Jr_.ptr, d_Kr_.ptr};
+    for (auto& qc : qclass_) {
+      const size_t ncomp = (size_t)ncart(qc->LA) * ncart(qc->LB) * ncart(qc->LC) * ncart(qc->LD);
+      qc->store.alloc(qc->blocks_h.size() * 64 * ncomp);
+      stored_bytes_ += double(qc->store.n) * sizeof(double);
+      ClassLaunch c{qc->LA, qc->LB, qc->LC, qc->LD, qc->blocks.ptr, (int)qc->blocks_h.size(),
+                    pclass_[qc->A].list.ptr, pclass_[qc->B].list.ptr, qc->store.ptr};
+      GANSU_HIP(launch_eri_class(c, g, kModeStore, 0));
+    }
+    GANSU_HIP(hipDeviceSynchronize());
+  }
+}
+
+void FockEngine::build_four_center(const double* d_D, hipStream_t s) {
+  GANSU_HIP(launch_scale_density(d_D, d_w_.ptr, M_, d_Dp_.ptr, s));
+  GANSU_HIP(launch_zero(d_Jr_.ptr, (size_t)M_ * M_, s));
+  GANSU_HIP(launch_zero(d_Kr_.ptr, (size_t)M_ * M_, s));
+  EriArgs g{d_sp_.ptr, d_pr_.ptr, boys_.ptr, thr_, d_Dp_.ptr, M_, d_Jr_.ptr, d_Kr_.ptr};
+  const int mode = method_ == "stored" ? kModeStoredDigest : kModeDirect;
+  for (auto& qc : qclass_) {
+    ClassLaunch c{qc->LA, qc->LB, qc->LC, qc->LD, qc->blocks.ptr, (int)qc->blocks_h.size(),
+                  pclass_[qc->A].list.ptr, pclass_[qc->B].list.ptr, qc->store.ptr};
+    GANSU_HIP(launch_eri_class(c, g, mode, s));
+  }
+}
+
+void FockEngine::build_jk(const double* d_D, double* d_J, double* d_K, const double* d_H, double* d_F,
+                          hipStream_t s) {
+  if (!precomputed_) throw Error(GANSU_E_STATE, "build before precompute");
+  if (!d_D) throw Error(GANSU_E_ARG, "null density");
+  if (method_ == "ri") { build_ri(d_D, d_J, d_K, d_H, d_F, s); return; }
+  build_four_center(d_D, s);
+  GANSU_HIP(launch_finalize(d_Jr_.ptr, d_Kr_.ptr, d_w_.ptr, M_, d_H, d_J, d_K, d_F, s));
+}
+
+std::vector<double> FockEngine::schwarz_ref() const {
+  if (!precomputed_) throw Error(GANSU_E_STATE, "schwarz factors before precompute");
+  return Qref_h_;
+}
+
+void FockEngine::stats(double* out) const {
+  for (int k = 0; k < 8; ++k) out[k] = 0.0;
+  out[0] = n_quartets_;
+  out[1] = n_prim_quartets_;
+  out[2] = stored_bytes_;
+  const double P = double(M_) * (M_ + 1) / 2;
+  out[4] = P * (P + 1) / 2;
+}
+
+}  // namespace gansu

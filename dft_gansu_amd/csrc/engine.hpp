@@ -1,0 +1,133 @@
+// Fock engine: owns integral-side state (pair data, work blocks, stored
+// integrals, RI tensors) and runs precompute / Fock builds on one GPU.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "fock_kernels.hpp"
+#include "gansu_internal.hpp"
+
+namespace gansu {
+
+void hip_check(hipError_t e, const char* what);
+#define GANSU_HIP(x) ::gansu::hip_check((x), #x)
+
+// RAII device buffer.
+template <typename T>
+struct DevBuf {
+  T* ptr = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  explicit DevBuf(size_t count) { alloc(count); }
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : ptr(o.ptr), n(o.n) { o.ptr = nullptr; o.n = 0; }
+  DevBuf& operator=(DevBuf&& o) noexcept {
+    if (this != &o) { release(); ptr = o.ptr; n = o.n; o.ptr = nullptr; o.n = 0; }
+    return *this;
+  }
+  ~DevBuf() { release(); }
+  void alloc(size_t count) {
+    release();
+    n = count;
+    if (count) GANSU_HIP(hipMalloc((void**)&ptr, count * sizeof(T)));
+  }
+  void release() {
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    n = 0;
+  }
+  void upload(const T* h, size_t count) {
+    alloc(count);
+    if (count) GANSU_HIP(hipMemcpy(ptr, h, count * sizeof(T), hipMemcpyHostToDevice));
+  }
+  void upload(const std::vector<T>& v) { upload(v.data(), v.size()); }
+  std::vector<T> download() const {
+    std::vector<T> h(n);
+    if (n) GANSU_HIP(hipMemcpy(h.data(), ptr, n * sizeof(T), hipMemcpyDeviceToHost));
+    return h;
+  }
+};
+
+struct QuartetClass {
+  int A, B;  // pair class ids (bra >= ket)
+  int LA, LB, LC, LD;
+  std::vector<BlockRec> blocks_h;
+  DevBuf<BlockRec> blocks;
+  DevBuf<double> store;  // stored integrals of this class (eri_method=stored)
+  double n_quartets = 0;
+};
+
+struct PairClass {
+  int la, lb;
+  std::vector<int> sorted;  // shell-pair ids sorted by qmax desc
+  DevBuf<int> list;
+};
+
+class FockEngine {
+ public:
+  FockEngine(const gansu_primitive_shell* prims, size_t n_prims, const gansu_shell_type_info* types, int n_types,
+             const double* norm, int num_basis, const gansu_primitive_shell* aux, size_t n_aux,
+             const gansu_shell_type_info* aux_types, int n_aux_types, const double* aux_norm, int num_aux,
+             const std::map<std::string, std::string>& opts);
+  ~FockEngine();
+
+  void precompute();
+  void build_jk(const double* d_D, double* d_J, double* d_K, const double* d_H, double* d_F, hipStream_t s);
+  const char* algorithm_name() const;
+  std::vector<double> schwarz_ref() const;  // host copy, reference pair order
+  size_t num_primitive_pairs() const { return ref_pairs_; }
+  void stats(double* out) const;
+  int num_basis() const { return M_; }
+
+ private:
+  void precompute_four_center();
+  void precompute_ri();
+  void build_four_center(const double* d_D, hipStream_t s);
+  void build_ri(const double* d_D, double* d_J, double* d_K, const double* d_H, double* d_F, hipStream_t s);
+
+  std::string method_;  // stored | direct | ri
+  double thr_ = 1e-12;
+  bool mixed_ = false;
+  double fp32_thr_ = 1e-6;
+  int M_ = 0;
+  std::vector<gansu_primitive_shell> prims_;
+  std::vector<gansu_shell_type_info> types_;
+  std::vector<double> norm_;
+  std::vector<double> w_h_;  // raw-basis scaling per basis function
+  std::vector<CShell> shells_;
+  bool precomputed_ = false;
+
+  // device state
+  DevBuf<double> boys_;
+  DevBuf<gansu_primitive_shell> d_prims_;
+  DevBuf<double> d_Qref_;
+  size_t ref_pairs_ = 0;
+  std::vector<double> Qref_h_;
+  DevBuf<PairRec> d_pr_;
+  DevBuf<SPair> d_sp_;
+  std::vector<SPair> sp_h_;
+  std::vector<PairClass> pclass_;
+  std::vector<std::unique_ptr<QuartetClass>> qclass_;
+  DevBuf<double> d_w_, d_Dp_, d_Jr_, d_Kr_;
+  double n_quartets_ = 0, n_prim_quartets_ = 0, stored_bytes_ = 0;
+
+  // RI state
+  std::vector<gansu_primitive_shell> aux_prims_;
+  std::vector<gansu_shell_type_info> aux_types_;
+  std::vector<double> aux_norm_;
+  int Maux_ = 0;
+
+ public:
+  struct RIState;
+
+ private:
+  RIState* ri_ = nullptr;  // owned; released by ri_destroy
+};
+
+void ri_destroy(FockEngine::RIState*);
+
+}  // namespace gansu

@@ -1,0 +1,623 @@
+// Fock-build kernels for gfx950: Schwarz factors, primitive-pair data and the
+// shell-quartet ERI kernels with fused J/K digestion (or stored blocks).
+//
+// Work decomposition.  Contracted shell pairs of one angular-momentum class are
+// sorted by their largest primitive Schwarz factor (descending).  For a bra
+// pair at sorted position s, the kets whose bound survives screening form a
+// prefix of the ket class list, further cut to positions <= s when bra and
+// ket class coincide (8-fold symmetry).  Each 64-lane wavefront takes one bra
+// pair and 64 consecutive kets: bra data is wave-uniform, the bra block of J
+// is reduced across the wave, and every lane digests its own quartet.
+//
+// Screening keeps the reference's semantics: a primitive quartet is skipped
+// when Q_ab * Q_cd < threshold (int2e.cu:1681), with Q the primitive-pair
+// factor of schwarz.txt:239-478.
+//
+// Digestion.  A lane holding the contracted block v = (ab|cd) of a unique
+// shell quartet (i j | k l) adds, in the raw (un-normalised) basis,
+//   Jr[a][b] += f_i f_k v D[c][d]        Jr[c][d] += f_i f_k v D[a][b]  (ij != kl)
+//   Kr[a][c] += f_o v D[b][d]   Kr[b][c] += f_o v D[a][d] (i != j)
+//   Kr[a][d] += f_o v D[b][c] (k != l)   Kr[b][d] += f_o v D[a][c] (i != j, k != l)
+// with f_i = 2 if i != j, f_k = 2 if k != l, f_o = 2 if (ij) != (kl); the
+// final J, K are the symmetrised accumulators (gpu_kernels.cu:230-287 defines
+// F = H + J - K/2 on the dense tensor).
+#include <hip/hip_runtime.h>
+
+#include "fock_kernels.hpp"
+#include "md_device.hpp"
+
+namespace gansu {
+
+using namespace dev;
+
+namespace {
+
+__device__ __forceinline__ double dfact2(int n) {  // (n)!! for n >= -1
+  double r = 1.0;
+  for (int k = n; k > 1; k -= 2) r *= k;
+  return r;
+}
+
+// primitive norm of component (l,m,n), int2e.hpp:127-130
+__device__ double prim_norm(double a, int l, int m, int n) {
+  const int L = l + m + n;
+  return double(1 << L) / sqrt(dfact2(2 * l - 1) * dfact2(2 * m - 1) * dfact2(2 * n - 1)) *
+         0.71270547035499016035 * pow(a, (2.0 * L + 3.0) / 4.0);
+}
+
+struct DPrim {  // == gansu_primitive_shell layout
+  double exponent, coefficient, x, y, z;
+  int shell_type;
+  size_t basis_index;
+  int atom_index;
+};
+
+// ---------------------------------------------------------------- Schwarz --
+// Q = sqrt(max_comp c_a^2 c_b^2 N_a^2 N_b^2 [ab|ab]) per primitive pair in the
+// reference's pair order; pair_b < 0 marks an auxiliary primitive (a|a).
+__global__ void k_schwarz_ref(const DPrim* __restrict__ prims, const int* __restrict__ pa,
+                              const int* __restrict__ pb, size_t n, const double* __restrict__ boys, int aux,
+                              double* __restrict__ Q) {
+  const size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= n) return;
+  const DPrim a = prims[pa[id]];
+  const bool single = pb[id] < 0;
+  const DPrim b = single ? a : prims[pb[id]];
+  const int la = a.shell_type, lb = single ? 0 : b.shell_type;
+  const double ea = a.exponent, eb = single ? 0.0 : b.exponent;
+  const double p = ea + eb;
+  const double A[3] = {a.x, a.y, a.z}, B[3] = {b.x, b.y, b.z};
+  double PA[3], PB[3], r2 = 0.0;
+  for (int d = 0; d < 3; ++d) {
+    const double P = (ea * A[d] + eb * B[d]) / p;
+    PA[d] = P - A[d];
+    PB[d] = P - B[d];
+    r2 += (A[d] - B[d]) * (A[d] - B[d]);
+  }
+  const double K = single ? 1.0 : exp(-ea * eb / p * r2);
+  double E[3][4 * 4 * 7];
+  for (int d = 0; d < 3; ++d) hermite_e_rt<double>(la, lb, p, PA[d], PB[d], E[d]);
+  const int L = 2 * (la + lb);
+  double R[nherm(12)];
+  // [ab|ab]: q = p, PQ = 0, reduced exponent p/2
+  const double pref = kTwoPi25 / (p * p * sqrt(2.0 * p)) * K * K;
+  hermite_r_rt<double>(L, 0.5 * p, 0.0, 0.0, 0.0, pref, boys, R);
+  const int nt = la + lb + 1;
+  double best = 0.0;
+  const double ca2 = a.coefficient * a.coefficient * (single ? 1.0 : b.coefficient * b.coefficient);
+  for (int ia = 0; ia < ncart(la); ++ia)
+    for (int ib = 0; ib < ncart(lb); ++ib) {
+      int ax[3], bx[3];
+      for (int d = 0; d < 3; ++d) { ax[d] = cart_exp(la, ia, d, aux); bx[d] = cart_exp(lb, ib, d, aux); }
+      const double* Ex = E[0] + (ax[0] * (lb + 1) + bx[0]) * nt;
+      const double* Ey = E[1] + (ax[1] * (lb + 1) + bx[1]) * nt;
+      const double* Ez = E[2] + (ax[2] * (lb + 1) + bx[2]) * nt;
+      double s = 0.0;
+      for (int t = 0; t <= ax[0] + bx[0]; ++t)
+        for (int u = 0; u <= ax[1] + bx[1]; ++u)
+          for (int v = 0; v <= ax[2] + bx[2]; ++v) {
+            const double e1 = Ex[t] * Ey[u] * Ez[v];
+            for (int t2 = 0; t2 <= ax[0] + bx[0]; ++t2)
+              for (int u2 = 0; u2 <= ax[1] + bx[1]; ++u2)
+                for (int v2 = 0; v2 <= ax[2] + bx[2]; ++v2) {
+                  const double sg = ((t2 + u2 + v2) & 1) ? -1.0 : 1.0;
+                  s += e1 * sg * Ex[t2] * Ey[u2] * Ez[v2] * R[hidx(t + t2, u + u2, v + v2)];
+                }
+          }
+      double nrm = prim_norm(ea, ax[0], ax[1], ax[2]);
+      if (!single) nrm *= prim_norm(eb, bx[0], bx[1], bx[2]);
+      const double val = nrm * nrm * ca2 * s;
+      if (fabs(best) < fabs(val)) best = val;
+    }
+  Q[id] = sqrt(best);
+}
+
+// --------------------------------------------------------- pair records ----
+__global__ void k_pair_data(const DPrim* __restrict__ prims, const int* __restrict__ pa, const int* __restrict__ pb,
+                            const double* __restrict__ Qg, size_t n, PairRec* __restrict__ out) {
+  const size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= n) return;
+  const DPrim a = prims[pa[id]], b = prims[pb[id]];
+  const double p = a.exponent + b.exponent;
+  const double A[3] = {a.x, a.y, a.z}, B[3] = {b.x, b.y, b.z};
+  PairRec r;
+  double r2 = 0.0;
+  for (int d = 0; d < 3; ++d) {
+    const double P = (a.exponent * A[d] + b.exponent * B[d]) / p;
+    r.P[d] = P;
+    r.PA[d] = P - A[d];
+    r.PB[d] = P - B[d];
+    r2 += (A[d] - B[d]) * (A[d] - B[d]);
+  }
+  r.p = p;
+  const int la = a.shell_type, lb = b.shell_type;
+  const double na = 0.71270547035499016035 * double(1 << la) * pow(a.exponent, (2.0 * la + 3.0) / 4.0);
+  const double nb = 0.71270547035499016035 * double(1 << lb) * pow(b.exponent, (2.0 * lb + 3.0) / 4.0);
+  r.Kc = exp(-a.exponent * b.exponent / p * r2) * a.coefficient * b.coefficient * na * nb;
+  r.Q = Qg[id];
+  out[id] = r;
+}
+
+// ------------------------------------------------------------- digestion --
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Digest one contracted block (compile-time shape).  `out` is [ab][cd].
+template <int NA, int NB, int NC, int ND>
+__device__ __forceinline__ void digest_block(const double* out, bool active, const SPair& bra, const SPair& ket,
+                                             bool pair_diff, const EriArgs& g) {
+  constexpr int NAB = NA * NB, NCD = NC * ND;
+  const int M = g.M;
+  const bool ij_diff = bra.si != bra.sj, kl_diff = ket.si != ket.sj;
+  const double fbk = (ij_diff ? 2.0 : 1.0) * (kl_diff ? 2.0 : 1.0);
+  const double fo = pair_diff ? 2.0 : 1.0;
+  const int bi = bra.bi, bj = bra.bj, bk = ket.bi, bl = ket.bj;
+  const double* D = g.Dp;
+  // J bra block: reduced over the wave (the bra is wave-uniform)
+  {
+    double dcd[NCD];
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
+#pragma unroll
+      for (int d = 0; d < ND; ++d) dcd[c * ND + d] = active ? D[(size_t)(bk + c) * M + bl + d] : 0.0;
+#pragma unroll
+    for (int ab = 0; ab < NAB; ++ab) {
+      double s = 0.0;
+#pragma unroll
+      for (int cd = 0; cd < NCD; ++cd) s += out[ab * NCD + cd] * dcd[cd];
+      s = wave_sum(s * fbk);
+      if (threadIdx.x == 0) unsafeAtomicAdd(&g.Jr[(size_t)(bi + ab / NB) * M + bj + ab % NB], s);
+    }
+  }
+  if (!active) return;
+  if (pair_diff) {
+    double dab[NAB];
+#pragma unroll
+    for (int a = 0; a < NA; ++a)
+#pragma unroll
+      for (int b = 0; b < NB; ++b) dab[a * NB + b] = D[(size_t)(bi + a) * M + bj + b];
+#pragma unroll
+    for (int cd = 0; cd < NCD; ++cd) {
+      double s = 0.0;
+#pragma unroll
+      for (int ab = 0; ab < NAB; ++ab) s += out[ab * NCD + cd] * dab[ab];
+      unsafeAtomicAdd(&g.Jr[(size_t)(bk + cd / ND) * M + bl + cd % ND], s * fbk);
+    }
+  }
+  // K: [a][c] with D[b][d]
+  {
+    double dbd[NB * ND];
+#pragma unroll
+    for (int b = 0; b < NB; ++b)
+#pragma unroll
+      for (int d = 0; d < ND; ++d) dbd[b * ND + d] = D[(size_t)(bj + b) * M + bl + d];
+#pragma unroll
+    for (int a = 0; a < NA; ++a)
+#pragma unroll
+      for (int c = 0; c < NC; ++c) {
+        double s = 0.0;
+#pragma unroll
+        for (int b = 0; b < NB; ++b)
+#pragma unroll
+          for (int d = 0; d < ND; ++d) s += out[(a * NB + b) * NCD + c * ND + d] * dbd[b * ND + d];
+        unsafeAtomicAdd(&g.Kr[(size_t)(bi + a) * M + bk + c], s * fo);
+      }
+  }
+  if (ij_diff) {  // [b][c] with D[a][d]
+    double dad[NA * ND];
+#pragma unroll
+    for (int a = 0; a < NA; ++a)
+#pragma unroll
+      for (int d = 0; d < ND; ++d) dad[a * ND + d] = D[(size_t)(bi + a) * M + bl + d];
+#pragma unroll
+    for (int b = 0; b < NB; ++b)
+#pragma unroll
+      for (int c = 0; c < NC; ++c) {
+        double s = 0.0;
+#pragma unroll
+        for (int a = 0; a < NA; ++a)
+#pragma unroll
+          for (int d = 0; d < ND; ++d) s += out[(a * NB + b) * NCD + c * ND + d] * dad[a * ND + d];
+        unsafeAtomicAdd(&g.Kr[(size_t)(bj + b) * M + bk + c], s * fo);
+      }
+  }
+  if (kl_diff) {  // [a][d] with D[b][c]
+    double dbc[NB * NC];
+#pragma unroll
+    for (int b = 0; b < NB; ++b)
+#pragma unroll
+      for (int c = 0; c < NC; ++c) dbc[b * NC + c] = D[(size_t)(bj + b) * M + bk + c];
+#pragma unroll
+    for (int a = 0; a < NA; ++a)
+#pragma unroll
+      for (int d = 0; d < ND; ++d) {
+        double s = 0.0;
+#pragma unroll
+        for (int b = 0; b < NB; ++b)
+#pragma unroll
+          for (int c = 0; c < NC; ++c) s += out[(a * NB + b) * NCD + c * ND + d] * dbc[b * NC + c];
+        unsafeAtomicAdd(&g.Kr[(size_t)(bi + a) * M + bl + d], s * fo);
+      }
+  }
+  if (ij_diff && kl_diff) {  // [b][d] with D[a][c]
+    double dac[NA * NC];
+#pragma unroll
+    for (int a = 0; a < NA; ++a)
+#pragma unroll
+      for (int c = 0; c < NC; ++c) dac[a * NC + c] = D[(size_t)(bi + a) * M + bk + c];
+#pragma unroll
+    for (int b = 0; b < NB; ++b)
+#pragma unroll
+      for (int d = 0; d < ND; ++d) {
+        double s = 0.0;
+#pragma unroll
+        for (int a = 0; a < NA; ++a)
+#pragma unroll
+          for (int c = 0; c < NC; ++c) s += out[(a * NB + b) * NCD + c * ND + d] * dac[a * NC + c];
+        unsafeAtomicAdd(&g.Kr[(size_t)(bj + b) * M + bl + d], s * fo);
+      }
+  }
+}
+
+// ------------------------------------------------- class-templated kernel --
+// One lane = one contracted shell quartet; full register tiles.  Used for the
+// classes whose block and Hermite tables fit in registers.
+template <int LA, int LB, int LC, int LD, int MODE>
+__global__ __launch_bounds__(64) void k_eri_class(ClassLaunch c, EriArgs g) {
+  constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
+  constexpr int NAB = NA * NB, NCD = NC * ND, NCOMP = NAB * NCD;
+  constexpr int LAB = LA + LB, LCD = LC + LD, L = LAB + LCD;
+  const BlockRec blk = c.blocks[blockIdx.x];
+  const int lane = threadIdx.x;
+  const bool active = lane < blk.count;
+  const int bra_id = c.bra_list[blk.bra_pos];
+  const int ket_id = c.ket_list[blk.ket_start + (active ? lane : 0)];
+  const SPair bra = g.sp[bra_id];
+  const SPair ket = g.sp[ket_id];
+
+  double out[NCOMP];
+#pragma unroll
+  for (int k = 0; k < NCOMP; ++k) out[k] = 0.0;
+
+  if constexpr (MODE == kModeStoredDigest) {
+    const double* src = c.store + (size_t)blockIdx.x * NCOMP * 64 + lane;
+#pragma unroll
+    for (int k = 0; k < NCOMP; ++k) out[k] = active ? src[(size_t)k * 64] : 0.0;
+  } else {
+    const int nk = active ? ket.nprim : 0;
+    for (int ib = 0; ib < bra.nprim; ++ib) {
+      const PairRec pb = g.pr[bra.prim_begin + ib];
+      double Eb[3][LA + 1][LB + 1][LAB + 1];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) hermite_e_t<LA, LB, double>(pb.p, pb.PA[d], pb.PB[d], Eb[d]);
+      for (int ik = 0; ik < nk; ++ik) {
+        const PairRec pk = g.pr[ket.prim_begin + ik];
+        if (pb.Q * pk.Q < g.thr) continue;
+        double Ek[3][LC + 1][LD + 1][LCD + 1];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) hermite_e_t<LC, LD, double>(pk.p, pk.PA[d], pk.PB[d], Ek[d]);
+        const double p = pb.p, q = pk.p;
+        const double alpha = p * q / (p + q);
+        const double pref = kTwoPi25 / (p * q * sqrt(p + q)) * pb.Kc * pk.Kc;
+        double R[nherm(L)];
+        hermite_r_t<L, double>(alpha, pb.P[0] - pk.P[0], pb.P[1] - pk.P[1], pb.P[2] - pk.P[2], pref, g.boys, R);
+#pragma unroll
+        for (int cd = 0; cd < NCD; ++cd) {
+          const int ci = cd / ND, di = cd % ND;
+          const int cx = cart_exp(LC, ci, 0), cy = cart_exp(LC, ci, 1), cz = cart_exp(LC, ci, 2);
+          const int dx = cart_exp(LD, di, 0), dy = cart_exp(LD, di, 1), dz = cart_exp(LD, di, 2);
+          // X[tuv] = sum_{tau nu phi} (-1)^(tau+nu+phi) E^cd_tau E^cd_nu E^cd_phi R[t+tau, u+nu, v+phi]
+          double X[nherm(LAB)];
+#pragma unroll
+          for (int t = 0; t <= LAB; ++t)
+#pragma unroll
+            for (int u = 0; u <= LAB; ++u)
+#pragma unroll
+              for (int v = 0; v <= LAB; ++v) {
+                if (t + u + v > LAB) continue;
+                double s = 0.0;
+#pragma unroll
+                for (int tt = 0; tt <= LCD; ++tt) {
+                  if (tt > cx + dx) continue;
+#pragma unroll
+                  for (int uu = 0; uu <= LCD; ++uu) {
+                    if (uu > cy + dy) continue;
+                    const double exy = Ek[0][cx][dx][tt] * Ek[1][cy][dy][uu];
+#pragma unroll
+                    for (int vv = 0; vv <= LCD; ++vv) {
+                      if (vv > cz + dz) continue;
+                      const double e = ((tt + uu + vv) & 1) ? -exy * Ek[2][cz][dz][vv] : exy * Ek[2][cz][dz][vv];
+                      s += e * R[hidx(t + tt, u + uu, v + vv)];
+                    }
+                  }
+                }
+                X[hidx(t, u, v)] = s;
+              }
+#pragma unroll
+          for (int ab = 0; ab < NAB; ++ab) {
+            const int ai = ab / NB, bi_ = ab % NB;
+            const int ax = cart_exp(LA, ai, 0), ay = cart_exp(LA, ai, 1), az = cart_exp(LA, ai, 2);
+            const int bx = cart_exp(LB, bi_, 0), by = cart_exp(LB, bi_, 1), bz = cart_exp(LB, bi_, 2);
+            double s = 0.0;
+#pragma unroll
+            for (int t = 0; t <= LAB; ++t) {
+              if (t > ax + bx) continue;
+#pragma unroll
+              for (int u = 0; u <= LAB; ++u) {
+                if (u > ay + by) continue;
+                const double exy = Eb[0][ax][bx][t] * Eb[1][ay][by][u];
+#pragma unroll
+                for (int v = 0; v <= LAB; ++v) {
+                  if (v > az + bz) continue;
+                  s += exy * Eb[2][az][bz][v] * X[hidx(t, u, v)];
+                }
+              }
+            }
+            out[ab * NCD + cd] += s;
+          }
+        }
+      }
+    }
+  }
+
+  if constexpr (MODE == kModeStore) {
+    double* dst = c.store + (size_t)blockIdx.x * NCOMP * 64 + lane;
+#pragma unroll
+    for (int k = 0; k < NCOMP; ++k) dst[(size_t)k * 64] = out[k];
+  } else {
+    digest_block<NA, NB, NC, ND>(out, active, bra, ket, bra_id != ket_id, g);
+  }
+}
+
+// ------------------------------------------------------ generic kernel ----
+// Runtime angular momenta (d-containing classes with large blocks): the same
+// algorithm with private arrays.
+constexpr int kGenMaxComp = 36 * 36;
+constexpr int kGenMaxE = 3 * 3 * 5;  // (l+1)(l+1)(2l+1) for l <= 2
+
+__global__ __launch_bounds__(64) void k_eri_generic(ClassLaunch c, EriArgs g, int mode) {
+  const int LA = c.LA, LB = c.LB, LC = c.LC, LD = c.LD;
+  const int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
+  const int NAB = NA * NB, NCD = NC * ND, NCOMP = NAB * NCD;
+  const int LAB = LA + LB, LCD = LC + LD, L = LAB + LCD;
+  const BlockRec blk = c.blocks[blockIdx.x];
+  const int lane = threadIdx.x;
+  const bool active = lane < blk.count;
+  const int bra_id = c.bra_list[blk.bra_pos];
+  const int ket_id = c.ket_list[blk.ket_start + (active ? lane : 0)];
+  const SPair bra = g.sp[bra_id];
+  const SPair ket = g.sp[ket_id];
+  double out[kGenMaxComp];
+  for (int k = 0; k < NCOMP; ++k) out[k] = 0.0;
+  if (mode == kModeStoredDigest) {
+    const double* src = c.store + (size_t)blockIdx.x * NCOMP * 64 + lane;
+    for (int k = 0; k < NCOMP; ++k) out[k] = active ? src[(size_t)k * 64] : 0.0;
+  } else {
+    const int ntb = LAB + 1, ntk = LCD + 1;
+    const int nk = active ? ket.nprim : 0;
+    for (int ib = 0; ib < bra.nprim; ++ib) {
+      const PairRec pb = g.pr[bra.prim_begin + ib];
+      double Eb[3][kGenMaxE];
+      for (int d = 0; d < 3; ++d) hermite_e_rt<double>(LA, LB, pb.p, pb.PA[d], pb.PB[d], Eb[d]);
+      for (int ik = 0; ik < nk; ++ik) {
+        const PairRec pk = g.pr[ket.prim_begin + ik];
+        if (pb.Q * pk.Q < g.thr) continue;
+        double Ek[3][kGenMaxE];
+        for (int d = 0; d < 3; ++d) hermite_e_rt<double>(LC, LD, pk.p, pk.PA[d], pk.PB[d], Ek[d]);
+        const double p = pb.p, q = pk.p;
+        const double alpha = p * q / (p + q);
+        const double pref = kTwoPi25 / (p * q * sqrt(p + q)) * pb.Kc * pk.Kc;
+        double R[nherm(8)];
+        hermite_r_rt<double>(L, alpha, pb.P[0] - pk.P[0], pb.P[1] - pk.P[1], pb.P[2] - pk.P[2], pref, g.boys, R);
+        for (int cd = 0; cd < NCD; ++cd) {
+          const int ci = cd / ND, di = cd % ND;
+          int ce[3], de[3];
+          for (int d = 0; d < 3; ++d) { ce[d] = cart_exp(LC, ci, d); de[d] = cart_exp(LD, di, d); }
+          const double* Ekx = Ek[0] + (ce[0] * (LD + 1) + de[0]) * ntk;
+          const double* Eky = Ek[1] + (ce[1] * (LD + 1) + de[1]) * ntk;
+          const double* Ekz = Ek[2] + (ce[2] * (LD + 1) + de[2]) * ntk;
+          double X[nherm(4)];
+          for (int t = 0; t <= LAB; ++t)
+            for (int u = 0; u + t <= LAB; ++u)
+              for (int v = 0; v + u + t <= LAB; ++v) {
+                double s = 0.0;
+                for (int tt = 0; tt <= ce[0] + de[0]; ++tt)
+                  for (int uu = 0; uu <= ce[1] + de[1]; ++uu) {
+                    const double exy = Ekx[tt] * Eky[uu];
+                    for (int vv = 0; vv <= ce[2] + de[2]; ++vv) {
+                      const double e = ((tt + uu + vv) & 1) ? -exy * Ekz[vv] : exy * Ekz[vv];
+                      s += e * R[hidx(t + tt, u + uu, v + vv)];
+                    }
+                  }
+                X[hidx(t, u, v)] = s;
+              }
+          for (int ab = 0; ab < NAB; ++ab) {
+            const int ai = ab / NB, bi_ = ab % NB;
+            int ae[3], be[3];
+            for (int d = 0; d < 3; ++d) { ae[d] = cart_exp(LA, ai, d); be[d] = cart_exp(LB, bi_, d); }
+            const double* Ebx = Eb[0] + (ae[0] * (LB + 1) + be[0]) * ntb;
+            const double* Eby = Eb[1] + (ae[1] * (LB + 1) + be[1]) * ntb;
+            const double* Ebz = Eb[2] + (ae[2] * (LB + 1) + be[2]) * ntb;
+            double s = 0.0;
+            for (int t = 0; t <= ae[0] + be[0]; ++t)
+              for (int u = 0; u <= ae[1] + be[1]; ++u) {
+                const double exy = Ebx[t] * Eby[u];
+                for (int v = 0; v <= ae[2] + be[2]; ++v) s += exy * Ebz[v] * X[hidx(t, u, v)];
+              }
+            out[ab * NCD + cd] += s;
+          }
+        }
+      }
+    }
+  }
+  if (mode == kModeStore) {
+    double* dst = c.store + (size_t)blockIdx.x * NCOMP * 64 + lane;
+    for (int k = 0; k < NCOMP; ++k) dst[(size_t)k * 64] = out[k];
+    return;
+  }
+  // digestion with runtime shapes
+  const int M = g.M;
+  const bool ij_diff = bra.si != bra.sj, kl_diff = ket.si != ket.sj, pair_diff = bra_id != ket_id;
+  const double fbk = (ij_diff ? 2.0 : 1.0) * (kl_diff ? 2.0 : 1.0);
+  const double fo = pair_diff ? 2.0 : 1.0;
+  const int bi = bra.bi, bj = bra.bj, bk = ket.bi, bl = ket.bj;
+  const double* D = g.Dp;
+  for (int ab = 0; ab < NAB; ++ab) {
+    double s = 0.0;
+    if (active)
+      for (int cd = 0; cd < NCD; ++cd) s += out[ab * NCD + cd] * D[(size_t)(bk + cd / ND) * M + bl + cd % ND];
+    s = wave_sum(s * fbk);
+    if (lane == 0) unsafeAtomicAdd(&g.Jr[(size_t)(bi + ab / NB) * M + bj + ab % NB], s);
+  }
+  if (!active) return;
+  if (pair_diff)
+    for (int cd = 0; cd < NCD; ++cd) {
+      double s = 0.0;
+      for (int ab = 0; ab < NAB; ++ab) s += out[ab * NCD + cd] * D[(size_t)(bi + ab / NB) * M + bj + ab % NB];
+      unsafeAtomicAdd(&g.Jr[(size_t)(bk + cd / ND) * M + bl + cd % ND], s * fbk);
+    }
+  for (int a = 0; a < NA; ++a)
+    for (int cc = 0; cc < NC; ++cc) {
+      double s = 0.0;
+      for (int b = 0; b < NB; ++b)
+        for (int d = 0; d < ND; ++d) s += out[(a * NB + b) * NCD + cc * ND + d] * D[(size_t)(bj + b) * M + bl + d];
+      unsafeAtomicAdd(&g.Kr[(size_t)(bi + a) * M + bk + cc], s * fo);
+    }
+  if (ij_diff)
+    for (int b = 0; b < NB; ++b)
+      for (int cc = 0; cc < NC; ++cc) {
+        double s = 0.0;
+        for (int a = 0; a < NA; ++a)
+          for (int d = 0; d < ND; ++d) s += out[(a * NB + b) * NCD + cc * ND + d] * D[(size_t)(bi + a) * M + bl + d];
+        unsafeAtomicAdd(&g.Kr[(size_t)(bj + b) * M + bk + cc], s * fo);
+      }
+  if (kl_diff)
+    for (int a = 0; a < NA; ++a)
+      for (int d = 0; d < ND; ++d) {
+        double s = 0.0;
+        for (int b = 0; b < NB; ++b)
+          for (int cc = 0; cc < NC; ++cc) s += out[(a * NB + b) * NCD + cc * ND + d] * D[(size_t)(bj + b) * M + bk + cc];
+        unsafeAtomicAdd(&g.Kr[(size_t)(bi + a) * M + bl + d], s * fo);
+      }
+  if (ij_diff && kl_diff)
+    for (int b = 0; b < NB; ++b)
+      for (int d = 0; d < ND; ++d) {
+        double s = 0.0;
+        for (int a = 0; a < NA; ++a)
+          for (int cc = 0; cc < NC; ++cc) s += out[(a * NB + b) * NCD + cc * ND + d] * D[(size_t)(bi + a) * M + bk + cc];
+        unsafeAtomicAdd(&g.Kr[(size_t)(bj + b) * M + bl + d], s * fo);
+      }
+}
+
+// ---------------------------------------------------------- elementwise ----
+__global__ void k_scale_density(const double* __restrict__ D, const double* __restrict__ w, int M,
+                                double* __restrict__ Dp) {
+  const size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= (size_t)M * M) return;
+  const int i = (int)(id / M), j = (int)(id % M);
+  Dp[id] = D[id] * w[i] * w[j];
+}
+
+__global__ void k_zero(double* __restrict__ p, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    p[i] = 0.0;
+}
+
+__global__ void k_finalize(const double* __restrict__ Jr, const double* __restrict__ Kr, const double* __restrict__ w,
+                           int M, const double* __restrict__ H, double* J, double* K, double* F) {
+  const size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= (size_t)M * M) return;
+  const int i = (int)(id / M), j = (int)(id % M);
+  const size_t tr = (size_t)j * M + i;
+  const double ww = 0.5 * w[i] * w[j];
+  const double jv = ww * (Jr[id] + Jr[tr]);
+  const double kv = ww * (Kr[id] + Kr[tr]);
+  if (J) J[id] = jv;
+  if (K) K[id] = kv;
+  if (F) F[id] = H[id] + jv - 0.5 * kv;
+}
+
+template <int LA, int LB, int LC, int LD>
+hipError_t launch_t(const ClassLaunch& c, const EriArgs& a, int mode, hipStream_t s) {
+  if (mode == kModeDirect)
+    hipLaunchKernelGGL((k_eri_class<LA, LB, LC, LD, kModeDirect>), dim3(c.nblocks), dim3(64), 0, s, c, a);
+  else if (mode == kModeStore)
+    hipLaunchKernelGGL((k_eri_class<LA, LB, LC, LD, kModeStore>), dim3(c.nblocks), dim3(64), 0, s, c, a);
+  else
+    hipLaunchKernelGGL((k_eri_class<LA, LB, LC, LD, kModeStoredDigest>), dim3(c.nblocks), dim3(64), 0, s, c, a);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+// --------------------------------------------------------------- launchers --
+hipError_t launch_schwarz_ref(const void* prims, const int* pair_a, const int* pair_b, size_t n, const double* boys,
+                              int aux_order, double* Q, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  const unsigned nb = (unsigned)((n + 255) / 256);
+  hipLaunchKernelGGL(k_schwarz_ref, dim3(nb), dim3(256), 0, s, (const DPrim*)prims, pair_a, pair_b, n, boys, aux_order,
+                     Q);
+  return hipGetLastError();
+}
+
+hipError_t launch_pair_data(const void* prims, const int* pa, const int* pb, const double* Qg, size_t n, PairRec* out,
+                            hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  const unsigned nb = (unsigned)((n + 255) / 256);
+  hipLaunchKernelGGL(k_pair_data, dim3(nb), dim3(256), 0, s, (const DPrim*)prims, pa, pb, Qg, n, out);
+  return hipGetLastError();
+}
+
+// Classes whose block and Hermite tables fit a lane's registers run the
+// templated kernel; the rest the generic one.
+hipError_t launch_eri_class(const ClassLaunch& c, const EriArgs& a, int mode, hipStream_t s) {
+  if (c.nblocks == 0) return hipSuccess;
+  const int key = ((c.LA * 3 + c.LB) * 3 + c.LC) * 3 + c.LD;
+#define GANSU_CASE(a_, b_, c_, d_) \
+  case ((a_ * 3 + b_) * 3 + c_) * 3 + d_: return launch_t<a_, b_, c_, d_>(c, a, mode, s);
+  switch (key) {
+    GANSU_CASE(0, 0, 0, 0)
+    GANSU_CASE(1, 0, 0, 0)
+    GANSU_CASE(1, 0, 1, 0)
+    GANSU_CASE(1, 1, 0, 0)
+    GANSU_CASE(1, 1, 1, 0)
+    GANSU_CASE(1, 1, 1, 1)
+    GANSU_CASE(2, 0, 0, 0)
+    GANSU_CASE(2, 0, 1, 0)
+    GANSU_CASE(2, 0, 1, 1)
+    GANSU_CASE(2, 0, 2, 0)
+    GANSU_CASE(2, 1, 0, 0)
+    GANSU_CASE(2, 1, 1, 0)
+    GANSU_CASE(2, 2, 0, 0)
+    default:
+      hipLaunchKernelGGL(k_eri_generic, dim3(c.nblocks), dim3(64), 0, s, c, a, mode);
+      return hipGetLastError();
+  }
+#undef GANSU_CASE
+}
+
+hipError_t launch_scale_density(const double* D, const double* w, int M, double* Dp, hipStream_t s) {
+  const size_t n = (size_t)M * M;
+  hipLaunchKernelGGL(k_scale_density, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, D, w, M, Dp);
+  return hipGetLastError();
+}
+
+hipError_t launch_zero(double* p, size_t n, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  size_t nb = (n + 255) / 256;
+  if (nb > 4096) nb = 4096;
+  hipLaunchKernelGGL(k_zero, dim3((unsigned)nb), dim3(256), 0, s, p, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_finalize(const double* Jr, const double* Kr, const double* w, int M, const double* H, double* J,
+                           double* K, double* F, hipStream_t s) {
+  const size_t n = (size_t)M * M;
+  hipLaunchKernelGGL(k_finalize, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, Jr, Kr, w, M, H, J, K, F);
+  return hipGetLastError();
+}
+
+}  // namespace gansu

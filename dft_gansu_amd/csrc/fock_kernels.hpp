@@ -1,0 +1,65 @@
+// Device data layouts and launch entry points of the Fock-build kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstddef>
+
+namespace gansu {
+
+// One primitive pair (a, b) of a contracted shell pair, L_a >= L_b.
+// Kc = exp(-ab/p |AB|^2) c_a c_b N'_a N'_b with N'(x, L) = (2/pi)^3/4 2^L x^((2L+3)/4),
+// the component-independent part of the primitive norm (int2e.hpp:127-130).
+struct PairRec {
+  double p, P[3], PA[3], PB[3], Kc, Q;
+};
+
+// One contracted shell pair (i, j), L_i >= L_j (i >= j when L_i == L_j).
+struct SPair {
+  int prim_begin, nprim;  // range in the PairRec array
+  int bi, bj;             // first basis function of shells i, j
+  int si, sj;             // shell ids
+  int la, lb;
+  double qmax;            // max over primitive pairs of Q
+  double pad;
+};
+
+// A wavefront's work: one bra pair (position in the bra class list) and up to
+// 64 consecutive kets of the ket class list starting at ket_start.
+struct BlockRec {
+  int bra_pos, ket_start, count, pad;
+};
+
+enum EriMode { kModeDirect = 0, kModeStore = 1, kModeStoredDigest = 2 };
+
+struct ClassLaunch {
+  int LA, LB, LC, LD;
+  const BlockRec* blocks;
+  int nblocks;
+  const int* bra_list;
+  const int* ket_list;
+  double* store;  // stored blocks of this class (kModeStore / kModeStoredDigest)
+};
+
+struct EriArgs {
+  const SPair* sp;
+  const PairRec* pr;
+  const double* boys;  // [1024][32]
+  double thr;
+  const double* Dp;  // density scaled to the raw basis (M x M)
+  int M;
+  double* Jr;  // raw-basis J accumulator (M x M)
+  double* Kr;  // raw-basis K accumulator (M x M)
+};
+
+// Host-callable launchers (fock_kernels.hip).
+hipError_t launch_schwarz_ref(const void* prims, const int* pair_a, const int* pair_b, size_t n, const double* boys,
+                              int aux_order, double* Q, hipStream_t s);
+hipError_t launch_pair_data(const void* prims, const int* pa, const int* pb, const double* Qg, size_t n, PairRec* out,
+                            hipStream_t s);
+hipError_t launch_eri_class(const ClassLaunch& c, const EriArgs& a, int mode, hipStream_t s);
+hipError_t launch_scale_density(const double* D, const double* w, int M, double* Dp, hipStream_t s);
+hipError_t launch_zero(double* p, size_t n, hipStream_t s);
+// J = w w sym(Jr), K = w w sym(Kr); if F: F = H + J - K/2 (J, K may be null then)
+hipError_t launch_finalize(const double* Jr, const double* Kr, const double* w, int M, const double* H, double* J,
+                           double* K, double* F, hipStream_t s);
+
+}  // namespace gansu

@@ -1,0 +1,172 @@
+// One-electron integrals S, T, V (McMurchie-Davidson) and small dense helpers
+// used by the RHF driver.  The reference computes S/T by MD and V by
+// Obara-Saika per shell-type pair (int1e.cu, MD_kernel.txt, OS_kernel.txt);
+// here one lane evaluates one (mu, nu) basis-function pair.
+#include <hip/hip_runtime.h>
+
+#include "md_device.hpp"
+#include "scf_kernels.hpp"
+
+namespace gansu {
+
+using namespace dev;
+
+namespace {
+
+struct DPrim {
+  double exponent, coefficient, x, y, z;
+  int shell_type;
+  size_t basis_index;
+  int atom_index;
+};
+
+__global__ void k_oneint(OneIntArgs a) {
+  const size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int M = a.M;
+  if (id >= (size_t)M * M) return;
+  const int mu = (int)(id / M), nu = (int)(id % M);
+  if (nu > mu) return;  // lower triangle, mirrored at the end
+  const int sa = a.bf_shell[mu], sb = a.bf_shell[nu];
+  const int ca = mu - a.shell_basis[sa], cb = nu - a.shell_basis[sb];
+  const int la = a.shell_L[sa], lb = a.shell_L[sb];
+  int ea[3], eb[3];
+  for (int d = 0; d < 3; ++d) { ea[d] = cart_exp(la, ca, d); eb[d] = cart_exp(lb, cb, d); }
+  double S = 0.0, T = 0.0, V = 0.0;
+  for (int ia = a.shell_off[sa]; ia < a.shell_off[sa + 1]; ++ia) {
+    const DPrim pa = ((const DPrim*)a.prims)[a.shell_prims[ia]];
+    for (int ib = a.shell_off[sb]; ib < a.shell_off[sb + 1]; ++ib) {
+      const DPrim pb = ((const DPrim*)a.prims)[a.shell_prims[ib]];
+      const double al = pa.exponent, be = pb.exponent, p = al + be;
+      const double A[3] = {pa.x, pa.y, pa.z}, B[3] = {pb.x, pb.y, pb.z};
+      double P[3], r2 = 0.0;
+      double E[3][3 * 5 * 8];  // (la+1) x (lb+3) x (la+lb+3), la, lb <= 2
+      const int nj = lb + 3, ntt = la + lb + 3;
+      for (int d = 0; d < 3; ++d) {
+        P[d] = (al * A[d] + be * B[d]) / p;
+        r2 += (A[d] - B[d]) * (A[d] - B[d]);
+        hermite_e_rt<double>(la, lb + 2, p, P[d] - A[d], P[d] - B[d], E[d]);
+      }
+      const double K = exp(-al * be / p * r2);
+      const double na = 0.71270547035499016035 * double(1 << la) * pow(al, (2.0 * la + 3.0) / 4.0);
+      const double nb = 0.71270547035499016035 * double(1 << lb) * pow(be, (2.0 * lb + 3.0) / 4.0);
+      const double pf = pa.coefficient * pb.coefficient * na * nb * K;
+      double s1[3], t1[3];
+      for (int d = 0; d < 3; ++d) {
+        const int i = ea[d], j = eb[d];
+        const double sij = E[d][(i * nj + j) * ntt];
+        const double sp2 = E[d][(i * nj + j + 2) * ntt];
+        const double sm2 = j >= 2 ? E[d][(i * nj + j - 2) * ntt] : 0.0;
+        s1[d] = sij;
+        t1[d] = -2.0 * be * be * sp2 + be * (2 * j + 1) * sij - 0.5 * j * (j - 1) * sm2;
+      }
+      const double ovl = pow(kPi / p, 1.5);
+      S += pf * ovl * s1[0] * s1[1] * s1[2];
+      T += pf * ovl * (t1[0] * s1[1] * s1[2] + s1[0] * t1[1] * s1[2] + s1[0] * s1[1] * t1[2]);
+      const int L = la + lb;
+      const double* Ex = E[0] + (ea[0] * nj + eb[0]) * ntt;
+      const double* Ey = E[1] + (ea[1] * nj + eb[1]) * ntt;
+      const double* Ez = E[2] + (ea[2] * nj + eb[2]) * ntt;
+      double vsum = 0.0;
+      for (int c = 0; c < a.natoms; ++c) {
+        double R[nherm(4)];
+        hermite_r_rt<double>(L, p, P[0] - a.atom_xyz[3 * c], P[1] - a.atom_xyz[3 * c + 1],
+                             P[2] - a.atom_xyz[3 * c + 2], 1.0, a.boys, R);
+        double acc = 0.0;
+        for (int t = 0; t <= ea[0] + eb[0]; ++t)
+          for (int u = 0; u <= ea[1] + eb[1]; ++u)
+            for (int v = 0; v <= ea[2] + eb[2]; ++v) acc += Ex[t] * Ey[u] * Ez[v] * R[hidx(t, u, v)];
+        vsum -= a.Z[c] * acc;
+      }
+      V += pf * 2.0 * kPi / p * vsum;
+    }
+  }
+  const double ww = a.w[mu] * a.w[nu];
+  a.S[(size_t)mu * M + nu] = a.S[(size_t)nu * M + mu] = S * ww;
+  a.T[(size_t)mu * M + nu] = a.T[(size_t)nu * M + mu] = T * ww;
+  a.V[(size_t)mu * M + nu] = a.V[(size_t)nu * M + mu] = V * ww;
+}
+
+// X[i][k] = U_col[i + k n] * (s_k < thr ? 0 : s_k^-1/2)   (hf.cu:171-242,
+// gpu_kernels.cu:28-43: eigenvalues below 1e-6 are dropped, M columns kept)
+__global__ void k_transform(const double* __restrict__ Ucol, const double* __restrict__ s, int n, double thr,
+                            double* __restrict__ X) {
+  const size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= (size_t)n * n) return;
+  const int i = (int)(id / n), k = (int)(id % n);
+  const double sk = s[k];
+  X[id] = Ucol[(size_t)k * n + i] * (sk < thr ? 0.0 : 1.0 / sqrt(sk));
+}
+
+// out = sum_i c_i in_i (DIIS extrapolation) or out = a*x + b*y
+__global__ void k_lincomb(int nmat, const double* __restrict__ coef, const double* const* __restrict__ mats,
+                          size_t n, double* __restrict__ out) {
+  const size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= n) return;
+  double s = 0.0;
+  for (int m = 0; m < nmat; ++m) s += coef[m] * mats[m][id];
+  out[id] = s;
+}
+
+__global__ void k_axpby(double a, const double* __restrict__ x, double b, const double* __restrict__ y, size_t n,
+                        double* __restrict__ out) {
+  const size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id < n) out[id] = a * x[id] + b * y[id];
+}
+
+// partial sums of sum_i x_i * y_i (deterministic two-pass reduction)
+__global__ void k_dot_partial(const double* __restrict__ x, const double* __restrict__ y, size_t n,
+                              double* __restrict__ part) {
+  __shared__ double sh[256];
+  double s = 0.0;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    s += x[i] * y[i];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+__global__ void k_has_nan(const double* __restrict__ x, size_t n, int* flag) {
+  const size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id < n && !(x[id] == x[id])) *flag = 1;
+}
+
+}  // namespace
+
+hipError_t launch_oneint(const OneIntArgs& a, hipStream_t s) {
+  const size_t n = (size_t)a.M * a.M;
+  hipLaunchKernelGGL(k_oneint, dim3((unsigned)((n + 127) / 128)), dim3(128), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_transform(const double* Ucol, const double* evals, int n, double thr, double* X, hipStream_t s) {
+  const size_t m = (size_t)n * n;
+  hipLaunchKernelGGL(k_transform, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, Ucol, evals, n, thr, X);
+  return hipGetLastError();
+}
+
+hipError_t launch_lincomb(int nmat, const double* d_coef, const double* const* d_mats, size_t n, double* out,
+                          hipStream_t s) {
+  hipLaunchKernelGGL(k_lincomb, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, nmat, d_coef, d_mats, n, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_axpby(double a, const double* x, double b, const double* y, size_t n, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_axpby, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a, x, b, y, n, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_dot_partial(const double* x, const double* y, size_t n, double* part, int nblocks, hipStream_t s) {
+  hipLaunchKernelGGL(k_dot_partial, dim3(nblocks), dim3(256), 0, s, x, y, n, part);
+  return hipGetLastError();
+}
+
+hipError_t launch_has_nan(const double* x, size_t n, int* flag, hipStream_t s) {
+  hipLaunchKernelGGL(k_has_nan, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, x, n, flag);
+  return hipGetLastError();
+}
+
+}  // namespace gansu

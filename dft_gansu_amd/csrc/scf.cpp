@@ -1,0 +1,307 @@
+// RHF SCF driver around the Fock engine, on the device in FP64.
+//
+// Semantics of HF::single_point_energy (hf.cu:245-314):
+//   core Hamiltonian, ERI precompute, X = U s^-1/2 (eigenvalues < 1e-6 dropped,
+//   hf.cu:171-242), initial guess (core: rhf.hpp:384-396), then per iteration
+//   C <- F (X^T F X, eigen, X C'), D = 2 C_occ C_occ^T, F <- D through the
+//   engine, E = 1/2 sum D (H + F); stop when |E - E_prev| < threshold, else
+//   update F by DIIS (rhf.hpp:298-327, gpu_manager.cu:1020-1194) or damping
+//   (rhf.hpp:182-262, gpu_manager.cu:898-1007).
+// Dense linear algebra: rocBLAS dgemm, rocSOLVER dsyevd; small DIIS systems
+// on the host.
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <limits>
+
+#include "engine.hpp"
+#include "scf_kernels.hpp"
+
+namespace gansu {
+
+namespace {
+
+void blas_check(rocblas_status st, const char* what) {
+  if (st != rocblas_status_success) throw Error(GANSU_E_HIP, std::string(what) + ": rocblas status " + std::to_string(st));
+}
+
+struct Blas {
+  rocblas_handle h = nullptr;
+  Blas() { blas_check(rocblas_create_handle(&h), "rocblas_create_handle"); }
+  ~Blas() { if (h) rocblas_destroy_handle(h); }
+  // row-major C = op(A) op(B), all n x n
+  void gemm(bool ta, bool tb, int n, const double* A, const double* B, double* C, double alpha = 1.0,
+            double beta = 0.0) {
+    blas_check(rocblas_dgemm(h, tb ? rocblas_operation_transpose : rocblas_operation_none,
+                             ta ? rocblas_operation_transpose : rocblas_operation_none, n, n, n, &alpha, B, n, A, n,
+                             &beta, C, n),
+               "rocblas_dgemm");
+  }
+};
+
+// Device workspace of the driver.
+struct Work {
+  int n;
+  DevBuf<double> S, T, V, H, X, F, D, C, tmp, tmp2, evec, eval, part;
+  DevBuf<int> info;
+  explicit Work(int n_) : n(n_) {
+    const size_t nn = (size_t)n * n;
+    for (auto* b : {&S, &T, &V, &H, &X, &F, &D, &C, &tmp, &tmp2, &evec}) b->alloc(nn);
+    eval.alloc(n);
+    part.alloc(256);
+    info.alloc(1);
+  }
+};
+
+double dot(const double* x, const double* y, size_t n, DevBuf<double>& part) {
+  GANSU_HIP(launch_dot_partial(x, y, n, part.ptr, 256, 0));
+  double h[256];
+  GANSU_HIP(hipMemcpy(h, part.ptr, sizeof h, hipMemcpyDeviceToHost));
+  double s = 0.0;
+  for (double v : h) s += v;
+  return s;
+}
+
+// symmetric eigendecomposition: evec (column-major eigenvectors), ascending eval
+void syevd(rocblas_handle h, int n, const double* A, Work& w) {
+  GANSU_HIP(hipMemcpy(w.evec.ptr, A, sizeof(double) * n * n, hipMemcpyDeviceToDevice));
+  blas_check(rocsolver_dsyevd(h, rocblas_evect_original, rocblas_fill_upper, n, w.evec.ptr, n, w.eval.ptr,
+                              w.tmp2.ptr, w.info.ptr),
+             "rocsolver_dsyevd");
+  int info = 0;
+  GANSU_HIP(hipMemcpy(&info, w.info.ptr, sizeof(int), hipMemcpyDeviceToHost));
+  if (info != 0) throw Error(GANSU_E_NUMERIC, "dsyevd failed, info=" + std::to_string(info));
+}
+
+// Gaussian elimination with partial pivoting (the getrf/getrs of the reference's DIIS).
+std::vector<double> solve_small(int n, std::vector<double> A, std::vector<double> b) {
+  for (int c = 0; c < n; ++c) {
+    int piv = c;
+    for (int r = c + 1; r < n; ++r)
+      if (std::fabs(A[r * n + c]) > std::fabs(A[piv * n + c])) piv = r;
+    if (A[piv * n + c] == 0.0) throw Error(GANSU_E_NUMERIC, "singular DIIS system");
+    if (piv != c) {
+      for (int k = 0; k < n; ++k) std::swap(A[c * n + k], A[piv * n + k]);
+      std::swap(b[c], b[piv]);
+    }
+    for (int r = c + 1; r < n; ++r) {
+      const double f = A[r * n + c] / A[c * n + c];
+      for (int k = c; k < n; ++k) A[r * n + k] -= f * A[c * n + k];
+      b[r] -= f * b[c];
+    }
+  }
+  for (int r = n - 1; r >= 0; --r) {
+    double s = b[r];
+    for (int k = r + 1; k < n; ++k) s -= A[r * n + k] * b[k];
+    b[r] = s / A[r * n + r];
+  }
+  return b;
+}
+
+}  // namespace
+
+// Build S, T, V of a molecule on the device.
+void compute_oneint(const Molecule& mol, double* dS, double* dT, double* dV, hipStream_t s) {
+  const auto shells = contracted_shells(mol.prims.data(), mol.prims.size(), kMaxL);
+  std::vector<int> off{0}, sp, sl, sb, bfs(mol.nbasis, -1);
+  std::vector<double> w(mol.nbasis);
+  for (size_t i = 0; i < shells.size(); ++i) {
+    for (int p : shells[i].prims) sp.push_back(p);
+    off.push_back((int)sp.size());
+    sl.push_back(shells[i].L);
+    sb.push_back(shells[i].basis);
+    for (int c = 0; c < ncart(shells[i].L); ++c) bfs[shells[i].basis + c] = (int)i;
+  }
+  for (int mu = 0; mu < mol.nbasis; ++mu) {
+    if (bfs[mu] < 0) throw Error(GANSU_E_ARG, "basis function without shell");
+    const int L = sl[bfs[mu]], c = mu - sb[bfs[mu]];
+    // same factor as the engine: cgto norm / sqrt((2l-1)!!(2m-1)!!(2n-1)!!)
+    int e[3];
+    static const int d2[6][3] = {{2, 0, 0}, {0, 2, 0}, {0, 0, 2}, {1, 1, 0}, {1, 0, 1}, {0, 1, 1}};
+    for (int d = 0; d < 3; ++d) e[d] = L == 0 ? 0 : (L == 1 ? (c == d) : d2[c][d]);
+    double df = 1.0;
+    for (int d = 0; d < 3; ++d) df *= (e[d] == 2 ? 3.0 : 1.0);
+    w[mu] = mol.norm[mu] / std::sqrt(df);
+  }
+  std::vector<double> xyz;
+  std::vector<int> Z;
+  for (const auto& a : mol.atoms) { xyz.insert(xyz.end(), a.xyz, a.xyz + 3); Z.push_back(a.Z); }
+  DevBuf<gansu_primitive_shell> dp;
+  dp.upload(mol.prims);
+  DevBuf<int> doff, dsp, dsl, dsb, dbf, dZ;
+  DevBuf<double> dw, dxyz, boys;
+  doff.upload(off); dsp.upload(sp); dsl.upload(sl); dsb.upload(sb); dbf.upload(bfs); dZ.upload(Z);
+  dw.upload(w); dxyz.upload(xyz);
+  boys.upload(boys_grid_xn());
+  OneIntArgs a{dp.ptr, doff.ptr, dsp.ptr, dsl.ptr, dsb.ptr, dbf.ptr, dw.ptr, mol.nbasis, (int)Z.size(),
+               dxyz.ptr, dZ.ptr, boys.ptr, dS, dT, dV};
+  GANSU_HIP(launch_oneint(a, s));
+  GANSU_HIP(hipStreamSynchronize(s));
+}
+
+struct ScfResult {
+  double e_elec = 0, e_total = 0;
+  int iterations = 0, fock_builds = 0, converged = 0;
+  std::vector<double> D;
+};
+
+ScfResult rhf_run(const Molecule& mol, const Molecule* aux, const Params& prm) {
+  if (prm.get("method") != "rhf") throw Error(GANSU_E_PARAM, "only method=rhf is provided by this engine");
+  if (mol.n_electrons % 2 != 0) throw Error(GANSU_E_ELECTRONS, "RHF requires an even number of electrons");
+  const int n = mol.nbasis;
+  const int nocc = mol.n_electrons / 2;
+  if (nocc > n) throw Error(GANSU_E_ELECTRONS, "more occupied orbitals than basis functions");
+  const std::string guess = prm.get("initial_guess");
+  if (guess != "core") throw Error(GANSU_E_UNSUPPORTED, "initial_guess=" + guess + " is not provided (core only)");
+  const std::string conv_method = prm.get("convergence_method");
+  if (conv_method != "diis" && conv_method != "damping" && conv_method != "optimaldamping")
+    throw Error(GANSU_E_PARAM, "Unknown convergence method: " + conv_method);
+  const int maxiter = prm.geti("maxiter");
+  const double conv = prm.getd("convergence_energy_threshold");
+  const int diis_size = prm.geti("diis_size");
+  const bool diis_x = prm.geti("diis_include_transform") != 0;
+  const double damp = prm.getd("damping_factor");
+
+  GANSU_HIP(hipSetDevice(0));
+  Work w(n);
+  Blas blas;
+  const size_t nn = (size_t)n * n;
+  compute_oneint(mol, w.S.ptr, w.T.ptr, w.V.ptr, 0);
+  GANSU_HIP(launch_axpby(1.0, w.T.ptr, 1.0, w.V.ptr, nn, w.H.ptr, 0));
+
+  std::map<std::string, std::string> opts{{"eri_method", prm.get("eri_method")},
+                                          {"schwarz_screening_threshold", prm.get("schwarz_screening_threshold")},
+                                          {"precision", prm.get("precision")},
+                                          {"fp32_bound_threshold", prm.get("fp32_bound_threshold")}};
+  const bool ri = prm.get("eri_method") == "ri";
+  if (ri && !aux) throw Error(GANSU_E_ARG, "eri_method=ri needs auxiliary_gbsfilename");
+  FockEngine eng(mol.prims.data(), mol.prims.size(), mol.types.data(), (int)mol.types.size(), mol.norm.data(), n,
+                 ri ? aux->prims.data() : nullptr, ri ? aux->prims.size() : 0, ri ? aux->types.data() : nullptr,
+                 ri ? (int)aux->types.size() : 0, ri ? aux->norm.data() : nullptr, ri ? aux->nbasis : 0, opts);
+  eng.precompute();
+
+  // X = U s^-1/2
+  syevd(blas.h, n, w.S.ptr, w);
+  GANSU_HIP(launch_transform(w.evec.ptr, w.eval.ptr, n, 1e-6, w.X.ptr, 0));
+
+  ScfResult res;
+  auto coefficients = [&](const double* Fm) {
+    blas.gemm(true, false, n, w.X.ptr, Fm, w.tmp.ptr);
+    blas.gemm(false, false, n, w.tmp.ptr, w.X.ptr, w.tmp2.ptr);
+    syevd(blas.h, n, w.tmp2.ptr, w);
+    // evec column-major: row view is V^T, so C = X V = X (row view)^T
+    blas.gemm(false, true, n, w.X.ptr, w.evec.ptr, w.C.ptr);
+  };
+  auto density = [&]() {
+    // D = 2 C_occ C_occ^T (gpu_kernels.cu:54-69).  The column-major view of
+    // row-major C is C^T, whose first nocc rows are the occupied orbitals.
+    const double two = 2.0, zero = 0.0;
+    blas_check(rocblas_dgemm(blas.h, rocblas_operation_transpose, rocblas_operation_none, n, n, nocc, &two, w.C.ptr,
+                             n, w.C.ptr, n, &zero, w.D.ptr, n),
+               "rocblas_dgemm(D)");
+  };
+  auto fock = [&]() {
+    eng.build_jk(w.D.ptr, nullptr, nullptr, w.H.ptr, w.F.ptr, 0);
+    ++res.fock_builds;
+    if (ri) {  // rhf.hpp:665-674 checks F for NaN on the RI path
+      DevBuf<int> flag(1);
+      int zero = 0;
+      GANSU_HIP(hipMemcpy(flag.ptr, &zero, sizeof(int), hipMemcpyHostToDevice));
+      GANSU_HIP(launch_has_nan(w.F.ptr, nn, flag.ptr, 0));
+      int h = 0;
+      GANSU_HIP(hipMemcpy(&h, flag.ptr, sizeof(int), hipMemcpyDeviceToHost));
+      if (h) throw Error(GANSU_E_NUMERIC, "NaN detected in the Fock matrix");
+    }
+  };
+  auto energy = [&]() { return 0.5 * (dot(w.D.ptr, w.H.ptr, nn, w.part) + dot(w.D.ptr, w.F.ptr, nn, w.part)); };
+
+  coefficients(w.H.ptr);
+  density();
+  fock();
+
+  // convergence state
+  DevBuf<double> errs, focks, prevF, prevD;
+  if (conv_method == "diis") { errs.alloc(nn * diis_size); focks.alloc(nn * diis_size); }
+  else { prevF.alloc(nn); prevD.alloc(nn); }
+  int diis_iter = 0;
+  bool first_damp = true;
+  double prev = std::numeric_limits<double>::max(), E = 0.0;
+  int it = 0;
+  while (it < maxiter) {
+    coefficients(w.F.ptr);
+    density();
+    fock();
+    E = energy();
+    if (std::fabs(E - prev) < conv) { res.converged = 1; break; }
+    ++it;
+    prev = E;
+    if (conv_method == "diis") {
+      // e = F D S - S D F (optionally X e X^T)
+      double* e = w.tmp2.ptr;
+      blas.gemm(false, false, n, w.F.ptr, w.D.ptr, w.tmp.ptr);
+      blas.gemm(false, false, n, w.tmp.ptr, w.S.ptr, e);
+      blas.gemm(false, false, n, w.S.ptr, w.D.ptr, w.tmp.ptr);
+      blas.gemm(false, false, n, w.tmp.ptr, w.F.ptr, e, -1.0, 1.0);
+      if (diis_x) {
+        blas.gemm(false, false, n, w.X.ptr, e, w.tmp.ptr);
+        blas.gemm(false, true, n, w.tmp.ptr, w.X.ptr, e);
+      }
+      const int slot = diis_iter % diis_size;
+      GANSU_HIP(hipMemcpy(errs.ptr + slot * nn, e, nn * sizeof(double), hipMemcpyDeviceToDevice));
+      GANSU_HIP(hipMemcpy(focks.ptr + slot * nn, w.F.ptr, nn * sizeof(double), hipMemcpyDeviceToDevice));
+      const int np = std::min(diis_iter + 1, diis_size);
+      if (np > 1) {
+        const int ns = np + 1;
+        std::vector<double> B((size_t)ns * ns, 0.0), rhs(ns, 0.0);
+        for (int i = 0; i < np; ++i) {
+          for (int j = i; j < np; ++j)
+            B[i * ns + j] = B[j * ns + i] = dot(errs.ptr + i * nn, errs.ptr + j * nn, nn, w.part);
+          B[i * ns + np] = B[np * ns + i] = -1.0;
+        }
+        rhs[np] = -1.0;
+        const auto c = solve_small(ns, B, rhs);
+        std::vector<const double*> mats(np);
+        for (int i = 0; i < np; ++i) mats[i] = focks.ptr + i * nn;
+        DevBuf<double> dc;
+        dc.upload(c.data(), np);
+        DevBuf<const double*> dm;
+        dm.upload(mats);
+        GANSU_HIP(launch_lincomb(np, dc.ptr, dm.ptr, nn, w.F.ptr, 0));
+      }
+      ++diis_iter;
+    } else {
+      if (first_damp) {
+        first_damp = false;
+        GANSU_HIP(hipMemcpy(prevD.ptr, w.D.ptr, nn * sizeof(double), hipMemcpyDeviceToDevice));
+        GANSU_HIP(hipMemcpy(prevF.ptr, w.F.ptr, nn * sizeof(double), hipMemcpyDeviceToDevice));
+      } else {
+        double alpha = damp;
+        if (conv_method == "optimaldamping") {
+          // s = tr(Fp dD), c = tr(dF dD)
+          GANSU_HIP(launch_axpby(1.0, w.F.ptr, -1.0, prevF.ptr, nn, w.tmp.ptr, 0));
+          GANSU_HIP(launch_axpby(1.0, w.D.ptr, -1.0, prevD.ptr, nn, w.tmp2.ptr, 0));
+          // for symmetric matrices tr(AB) = sum A .* B
+          const double sv = dot(prevF.ptr, w.tmp2.ptr, nn, w.part);
+          const double cv = dot(w.tmp.ptr, w.tmp2.ptr, nn, w.part);
+          alpha = (cv <= -sv / 2.0) ? 1.0 : -0.5 * sv / cv;
+          GANSU_HIP(launch_axpby(1.0 - alpha, prevD.ptr, alpha, w.D.ptr, nn, prevD.ptr, 0));
+          GANSU_HIP(hipMemcpy(w.D.ptr, prevD.ptr, nn * sizeof(double), hipMemcpyDeviceToDevice));
+        }
+        GANSU_HIP(launch_axpby(1.0 - alpha, prevF.ptr, alpha, w.F.ptr, nn, prevF.ptr, 0));
+        GANSU_HIP(hipMemcpy(w.F.ptr, prevF.ptr, nn * sizeof(double), hipMemcpyDeviceToDevice));
+      }
+    }
+  }
+  GANSU_HIP(hipDeviceSynchronize());
+  res.e_elec = E;
+  res.e_total = E + mol.nuclear_repulsion();
+  res.iterations = it;
+  res.D = w.D.download();
+  return res;
+}
+
+}  // namespace gansu

@@ -1,0 +1,32 @@
+// Launchers for one-electron integrals and the dense helpers of the RHF driver.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstddef>
+
+namespace gansu {
+
+struct OneIntArgs {
+  const void* prims;       // gansu_primitive_shell[]
+  const int* shell_off;    // [nshell+1] ranges into shell_prims
+  const int* shell_prims;  // primitive indices grouped by contracted shell
+  const int* shell_L;
+  const int* shell_basis;
+  const int* bf_shell;     // basis function -> shell
+  const double* w;         // cgto norm x component factor per basis function
+  int M;
+  int natoms;
+  const double* atom_xyz;
+  const int* Z;
+  const double* boys;
+  double *S, *T, *V;
+};
+
+hipError_t launch_oneint(const OneIntArgs& a, hipStream_t s);
+hipError_t launch_transform(const double* Ucol, const double* evals, int n, double thr, double* X, hipStream_t s);
+hipError_t launch_lincomb(int nmat, const double* d_coef, const double* const* d_mats, size_t n, double* out,
+                          hipStream_t s);
+hipError_t launch_axpby(double a, const double* x, double b, const double* y, size_t n, double* out, hipStream_t s);
+hipError_t launch_dot_partial(const double* x, const double* y, size_t n, double* part, int nblocks, hipStream_t s);
+hipError_t launch_has_nan(const double* x, size_t n, int* flag, hipStream_t s);
+
+}  // namespace gansu

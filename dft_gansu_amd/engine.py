@@ -1,0 +1,154 @@
+"""Host-side mirror of the reference's ERI strategy interface and HF driver.
+
+  FockEngine.precomputation()        == ERI::precomputation()      (eri.hpp:36-63)
+  FockEngine.compute_fock_matrix()   == ERI::compute_fock_matrix() (RHF: F = H + J - K/2)
+  FockEngine.get_algorithm_name()    == ERI::get_algorithm_name()
+  Parameters                         == ParameterManager (parameter_manager.cpp)
+  rhf(...)                           == HF::single_point_energy (hf.cu:245-314)
+
+Matrices are torch tensors on the GPU (M x M, row-major FP64): torch supplies
+device memory and streams; every computation runs in libgansu_fock.so.
+"""
+import ctypes
+
+import numpy as np
+
+from ._lib import PRIM_DTYPE, check, lib, ptr
+
+
+class Parameters:
+    """Same keys, defaults and recipe syntax as the reference's ParameterManager."""
+
+    def __init__(self, recipe=None, **kv):
+        self._h = ctypes.c_void_p()
+        check(lib().gansu_params_create(ctypes.byref(self._h)))
+        if recipe:
+            check(lib().gansu_params_load_recipe(self._h, recipe.encode()))
+        for k, v in kv.items():
+            self[k] = v
+
+    def __setitem__(self, key, value):
+        if isinstance(value, bool):
+            value = "1" if value else "0"
+        check(lib().gansu_params_set(self._h, str(key).encode(), str(value).encode()))
+
+    def __getitem__(self, key):
+        buf = ctypes.create_string_buffer(4096)
+        check(lib().gansu_params_get(self._h, str(key).encode(), buf, len(buf)))
+        return buf.value.decode()
+
+    @property
+    def handle(self):
+        return self._h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                lib().gansu_params_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+
+class FockEngine:
+    """Fock-build engine over a molecule (and an auxiliary basis for RI)."""
+
+    def __init__(self, mol, aux=None, eri_method="stored", schwarz_screening_threshold=1e-12, precision="fp64",
+                 fp32_bound_threshold=1e-6):
+        self.mol = mol
+        self.aux = aux
+        self.M = mol.n_basis
+        opts = (f"eri_method={eri_method};schwarz_screening_threshold={schwarz_screening_threshold!r};"
+                f"precision={precision};fp32_bound_threshold={fp32_bound_threshold!r}")
+        self._h = ctypes.c_void_p()
+        a = aux
+        check(lib().gansu_fock_create(
+            ptr(mol.prims), len(mol.prims), ptr(mol.types), len(mol.types), ptr(mol.norm), mol.n_basis,
+            ptr(a.prims) if a else None, len(a.prims) if a else 0, ptr(a.types) if a else None,
+            len(a.types) if a else 0, ptr(a.norm) if a else None, a.n_basis if a else 0, opts.encode(),
+            ctypes.byref(self._h)))
+        self._precomputed = False
+
+    def precomputation(self):
+        check(lib().gansu_fock_precompute(self._h))
+        self._precomputed = True
+
+    @staticmethod
+    def _stream(stream):
+        if stream is None:
+            return None
+        return ctypes.c_void_p(stream.cuda_stream if hasattr(stream, "cuda_stream") else int(stream))
+
+    def _check_mat(self, t, name):
+        import torch
+        if not (isinstance(t, torch.Tensor) and t.is_cuda and t.dtype == torch.float64 and t.is_contiguous()
+                and tuple(t.shape) == (self.M, self.M)):
+            raise ValueError(f"{name} must be a contiguous float64 CUDA tensor of shape ({self.M}, {self.M})")
+
+    def compute_fock_matrix(self, D, H, F=None, stream=None):
+        """F = H + J[D] - K[D]/2 (RHF); returns F."""
+        import torch
+        self._check_mat(D, "D")
+        self._check_mat(H, "H")
+        if F is None:
+            F = torch.empty_like(D)
+        self._check_mat(F, "F")
+        check(lib().gansu_fock_build_rhf(self._h, ptr(D), ptr(H), ptr(F), self._stream(stream)))
+        return F
+
+    def compute_jk(self, D, stream=None):
+        import torch
+        self._check_mat(D, "D")
+        J = torch.empty_like(D)
+        K = torch.empty_like(D)
+        check(lib().gansu_fock_build_jk(self._h, ptr(D), ptr(J), ptr(K), self._stream(stream)))
+        return J, K
+
+    def get_algorithm_name(self):
+        return lib().gansu_fock_algorithm_name(self._h).decode()
+
+    def schwarz_factors(self):
+        """Primitive-pair Schwarz factors in the reference's pair order."""
+        n = lib().gansu_fock_num_primitive_pairs(self._h)
+        q = np.zeros(n)
+        check(lib().gansu_fock_schwarz(self._h, ptr(q), n))
+        return q
+
+    def stats(self):
+        s = np.zeros(8)
+        check(lib().gansu_fock_stats(self._h, ptr(s)))
+        return dict(contracted_quartets=s[0], primitive_quartets=s[1], stored_bytes=s[2], fp32_quartets=s[3],
+                    unique_quartets=s[4], ri_bytes=s[5])
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                lib().gansu_fock_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+
+def oneint(mol):
+    """S, T, V on the GPU (torch float64 tensors)."""
+    import torch
+    M = mol.n_basis
+    S, T, V = (torch.empty((M, M), dtype=torch.float64, device="cuda") for _ in range(3))
+    check(lib().gansu_oneint(mol.handle, ptr(S), ptr(T), ptr(V), None))
+    torch.cuda.synchronize()
+    return S, T, V
+
+
+def rhf(mol, params, aux=None):
+    """Run the RHF SCF; returns dict(e_elec, e_total, iterations, fock_builds, converged, D)."""
+    en = np.zeros(2)
+    D = np.zeros((mol.n_basis, mol.n_basis))
+    info = np.zeros(4, dtype=np.int32)
+    check(lib().gansu_rhf_run(mol.handle, aux.handle if aux else None, params.handle, ptr(en), ptr(D), ptr(info)))
+    return dict(e_elec=float(en[0]), e_total=float(en[1]), iterations=int(info[0]), fock_builds=int(info[1]),
+                converged=bool(info[2]), D=D)
+
+
+__all__ = ["FockEngine", "Parameters", "oneint", "rhf", "PRIM_DTYPE"]

@@ -1,0 +1,145 @@
+/*
+ * gansu_fock.h -- C ABI of the MI355X (gfx950) Fock-build engine.
+ *
+ * Drop-in boundary for the reference's ERI strategy interface
+ * (zhaokaifan02/DFT_GANSU include/eri.hpp:36-63):
+ *
+ *   ERI::precomputation()        -> gansu_fock_precompute      (called at hf.cu:265
+ *                                                               via rhf.cu:110-114)
+ *   ERI::compute_fock_matrix()   -> gansu_fock_build_rhf       (hf.cu:284, rhf.hpp:394,
+ *                                                               :422, :450, :578, :583)
+ *   ERI::get_algorithm_name()    -> gansu_fock_algorithm_name  (hf.cu:424-429, :460-472)
+ *
+ * plus the input path it is fed from (basis_set.cpp:32-124, parseXYZ.cpp:44-115,
+ * molecular.hpp:117-190), the parameter recipes (parameter_manager.cpp:42-225)
+ * and the RHF SCF driver around it (hf.cu:171-314).
+ *
+ * Conventions
+ *  - Every function returns 0 on success and a non-zero GANSU_E_* code on
+ *    failure; gansu_last_error() then holds a message (thread-local).  No C++
+ *    exception crosses this boundary (the reference throws gansu::Exception,
+ *    utils.hpp THROW_EXCEPTION; each throw site becomes an error code here).
+ *  - Matrix pointers named d_* are caller-owned device pointers to M x M
+ *    row-major FP64 matrices (the reference's DeviceHostMatrix layout).  The
+ *    engine never frees caller memory.
+ *  - `stream` is a hipStream_t passed as void* (NULL = the null stream); build
+ *    calls are ordered on it and do not synchronise the host.
+ *  - One engine per host thread.
+ */
+#ifndef GANSU_FOCK_H
+#define GANSU_FOCK_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  GANSU_OK = 0,
+  GANSU_E_ARG = 1,        /* invalid argument / size                            */
+  GANSU_E_IO = 2,         /* file cannot be opened or parsed                    */
+  GANSU_E_BASIS = 3,      /* element missing from basis / bad shell label       */
+  GANSU_E_ELECTRONS = 4,  /* electron count invalid (molecular.hpp:100-109)     */
+  GANSU_E_PARAM = 5,      /* unknown key or bad value                           */
+  GANSU_E_HIP = 6,        /* HIP runtime / library failure                      */
+  GANSU_E_STATE = 7,      /* call out of order (e.g. build before precompute)   */
+  GANSU_E_NUMERIC = 8,    /* NaN in F (rhf.hpp:665-674), SPD failure, DIIS      */
+  GANSU_E_UNSUPPORTED = 9 /* angular momentum beyond the compiled classes       */
+};
+
+/* == gansu::PrimitiveShell (types.hpp:116-125), 64 bytes. */
+typedef struct gansu_primitive_shell {
+  double exponent;
+  double coefficient;
+  double x, y, z;       /* bohr */
+  int shell_type;       /* 0=s 1=p 2=d 3=f */
+  size_t basis_index;   /* first basis function of the contracted shell */
+  int atom_index;
+} gansu_primitive_shell;
+
+/* == gansu::ShellTypeInfo (types.hpp:150-154). */
+typedef struct gansu_shell_type_info {
+  int count;
+  size_t start_index;
+} gansu_shell_type_info;
+
+const char* gansu_last_error(void);
+const char* gansu_version(void);
+
+/* ---------------------------------------------------------------- input --
+ * Molecule = atoms + basis -> primitive list sorted by shell type, contracted
+ * normalisation factors, shell type infos (molecular.hpp:117-190).
+ * normalization: 0 = reference (keeps the integer-division quirk of
+ * basis_set.hpp:123 for odd L), 1 = unit-normalised contracted functions.   */
+typedef struct gansu_molecule gansu_molecule;
+
+/* From an .xyz file (Angstrom, parseXYZ.cpp). */
+int gansu_molecule_from_xyz(const char* xyz_path, const char* gbs_path, int charge, int beta_to_alpha,
+                            int normalization, gansu_molecule** out);
+/* From atomic numbers and coordinates in bohr. */
+int gansu_molecule_from_atoms(int n_atoms, const int* Z, const double* xyz_bohr, const char* gbs_path, int charge,
+                              int beta_to_alpha, int normalization, gansu_molecule** out);
+/* sizes[7] = {n_prims, n_basis, n_types, n_atoms, n_electrons, n_alpha, n_beta} */
+int gansu_molecule_sizes(const gansu_molecule* mol, int* sizes);
+/* Any output pointer may be NULL.  norm has n_basis entries, xyz 3*n_atoms. */
+int gansu_molecule_get(const gansu_molecule* mol, gansu_primitive_shell* prims, gansu_shell_type_info* types,
+                       double* norm, int* Z, double* xyz_bohr);
+double gansu_molecule_nuclear_repulsion(const gansu_molecule* mol);
+void gansu_molecule_destroy(gansu_molecule* mol);
+
+/* ----------------------------------------------------------- parameters --
+ * Same keys, defaults and recipe syntax as ParameterManager
+ * (parameter_manager.cpp:42-88, :164-225); unknown keys are rejected.       */
+typedef struct gansu_params gansu_params;
+int gansu_params_create(gansu_params** out);
+int gansu_params_load_recipe(gansu_params* p, const char* path);
+int gansu_params_set(gansu_params* p, const char* key, const char* value);
+/* Copies the value (NUL-terminated) into buf. */
+int gansu_params_get(const gansu_params* p, const char* key, char* buf, int buflen);
+void gansu_params_destroy(gansu_params* p);
+
+/* ---------------------------------------------------------- Fock engine --
+ * `options` is a ";"-separated "key=value" list using parameter keys
+ * (eri_method=stored|direct|ri, schwarz_screening_threshold=..,
+ * precision=fp64|mixed, fp32_bound_threshold=..).  aux* may be NULL/0 unless
+ * eri_method=ri.                                                            */
+typedef struct gansu_fock_engine gansu_fock_engine;
+
+int gansu_fock_create(const gansu_primitive_shell* prims, size_t n_prims, const gansu_shell_type_info* types,
+                      int n_types, const double* cgto_norm, int num_basis, const gansu_primitive_shell* aux,
+                      size_t n_aux, const gansu_shell_type_info* aux_types, int n_aux_types, const double* aux_norm,
+                      int num_aux, const char* options, gansu_fock_engine** out);
+/* == ERI::precomputation (eri.cu:60-116 stored, :134-236 RI). */
+int gansu_fock_precompute(gansu_fock_engine* e);
+/* == ERI::compute_fock_matrix for RHF: F = H + J - K/2 (gpu_kernels.cu:230-287). */
+int gansu_fock_build_rhf(gansu_fock_engine* e, const double* d_D, const double* d_H, double* d_F, void* stream);
+/* J and K separately: J_ij = sum_kl (ij|kl) D_kl, K_ij = sum_kl (ik|jl) D_kl. */
+int gansu_fock_build_jk(gansu_fock_engine* e, const double* d_D, double* d_J, double* d_K, void* stream);
+/* == ERI::get_algorithm_name ("Stored", "Direct", "RI"). */
+const char* gansu_fock_algorithm_name(const gansu_fock_engine* e);
+/* Primitive-pair Schwarz factors in the reference's pair order
+ * (gpu_manager.cu:413-432, schwarz.txt): host copy of n entries.            */
+int gansu_fock_schwarz(gansu_fock_engine* e, double* host_Q, size_t n);
+size_t gansu_fock_num_primitive_pairs(const gansu_fock_engine* e);
+/* stats[8] = {surviving contracted quartets, surviving primitive quartets,
+ *             stored bytes, fp32 quartets, unique contracted quartets,
+ *             ri B bytes, 0, 0}                                             */
+int gansu_fock_stats(const gansu_fock_engine* e, double* stats);
+void gansu_fock_destroy(gansu_fock_engine* e);
+
+/* ------------------------------------------------ one-electron / SCF -----
+ * S, T, V on the device (M x M row-major, caller-owned device pointers).    */
+int gansu_oneint(const gansu_molecule* mol, double* d_S, double* d_T, double* d_V, void* stream);
+
+/* RHF SCF loop of hf.cu:245-314 (core/DIIS/damping as the recipe says) with
+ * this engine as the ERI strategy.  energies[2] = {E_electronic, E_total};
+ * D_out (host, M*M) may be NULL.  info[4] = {iterations, fock builds,
+ * converged, 0}.                                                            */
+int gansu_rhf_run(const gansu_molecule* mol, const gansu_molecule* aux_mol, const gansu_params* p, double* energies,
+                  double* D_out, int* info);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GANSU_FOCK_H */

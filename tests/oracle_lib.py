@@ -1,0 +1,220 @@
+"""ctypes binding of the FP64 host oracle (oracle/liboracle.so).
+
+Test infrastructure only: the oracle is the checker for the HIP library and is
+loaded from tests, the smoke check and the CPU-baseline leg of the benchmark.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+_LIB = None
+
+PRIM_DTYPE = np.dtype([("exponent", "f8"), ("coefficient", "f8"), ("x", "f8"), ("y", "f8"), ("z", "f8"),
+                       ("shell_type", "i4"), ("_pad0", "i4"), ("basis_index", "u8"), ("atom_index", "i4"),
+                       ("_pad1", "i4")])
+TYPE_DTYPE = np.dtype([("count", "i4"), ("_pad", "i4"), ("start_index", "u8")])
+assert PRIM_DTYPE.itemsize == 64 and TYPE_DTYPE.itemsize == 16
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        so = os.path.join(ORACLE_DIR, "liboracle.so")
+        if not os.path.exists(so):
+            subprocess.check_call(["make", "-s", "-C", ORACLE_DIR])
+        _LIB = ctypes.CDLL(so)
+        L = _LIB
+        L.orc_cgto_norm.restype = ctypes.c_double
+        L.orc_schwarz.restype = ctypes.c_size_t
+        L.orc_optimal_damping.restype = ctypes.c_double
+        L.orc_eri_prim_os.restype = ctypes.c_double
+        L.orc_eri_prim_md.restype = ctypes.c_double
+        L.orc_last_error.restype = ctypes.c_char_p
+    return _LIB
+
+
+def boys(n, x, reference_style=False, table=None):
+    out = np.zeros(n + 1)
+    tab = table if table is not None else np.zeros(1)
+    lib().orc_boys(ctypes.c_int(n), ctypes.c_double(x), ctypes.c_int(int(reference_style)), _ptr(tab), _ptr(out))
+    return out
+
+
+def boys_table(round15=True):
+    out = np.zeros(30 * 1024)
+    lib().orc_boys_table(ctypes.c_int(int(round15)), _ptr(out))
+    return out
+
+
+def build_molecule(gbs_path, Z, xyz_bohr, quirk=True):
+    Z = np.ascontiguousarray(Z, dtype=np.int32)
+    xyz = np.ascontiguousarray(xyz_bohr, dtype=np.float64).reshape(-1)
+    npr, nb, nt = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    rc = lib().orc_build_molecule(gbs_path.encode(), ctypes.c_int(len(Z)), _ptr(Z), _ptr(xyz), ctypes.c_int(int(quirk)),
+                                  ctypes.byref(npr), ctypes.byref(nb), ctypes.byref(nt))
+    if rc:
+        raise RuntimeError(lib().orc_last_error().decode())
+    prims = np.zeros(npr.value, PRIM_DTYPE)
+    types = np.zeros(nt.value, TYPE_DTYPE)
+    norm = np.zeros(nb.value)
+    lib().orc_get_molecule(_ptr(prims), _ptr(types), _ptr(norm))
+    return prims, types, norm
+
+
+def n_pairs(types):
+    c = [int(t["count"]) for t in types]
+    n = 0
+    for i in range(len(c)):
+        for j in range(i, len(c)):
+            n += c[i] * (c[i] + 1) // 2 if i == j else c[i] * c[j]
+    return n
+
+
+def schwarz(prims, types, aux_order=False):
+    Q = np.zeros(n_pairs(types))
+    lib().orc_schwarz(_ptr(prims), _ptr(types), ctypes.c_int(len(types)), ctypes.c_int(int(aux_order)), _ptr(Q))
+    return Q
+
+
+def schwarz_aux(prims):
+    Q = np.zeros(len(prims))
+    lib().orc_schwarz_aux(_ptr(prims), ctypes.c_int(len(prims)), _ptr(Q))
+    return Q
+
+
+def eri_dense(prims, types, norm, Q, thr=1e-12):
+    M = len(norm)
+    out = np.zeros((M, M, M, M))
+    lib().orc_eri_dense(_ptr(prims), _ptr(types), ctypes.c_int(len(types)), _ptr(norm), ctypes.c_int(M), _ptr(Q),
+                        ctypes.c_double(thr), _ptr(out))
+    return out
+
+
+def jk(prims, types, norm, Q, D, thr=1e-12, sample_stride=1):
+    M = len(norm)
+    D = np.ascontiguousarray(D, dtype=np.float64)
+    J = np.zeros((M, M))
+    K = np.zeros((M, M))
+    lib().orc_jk(_ptr(prims), _ptr(types), ctypes.c_int(len(types)), _ptr(norm), ctypes.c_int(M), _ptr(Q),
+                 ctypes.c_double(thr), _ptr(D), _ptr(J), _ptr(K), ctypes.c_int(sample_stride))
+    return J, K
+
+
+def oneint(prims, norm, Z, xyz_bohr):
+    M = len(norm)
+    Z = np.ascontiguousarray(Z, dtype=np.int32)
+    xyz = np.ascontiguousarray(xyz_bohr, dtype=np.float64).reshape(-1)
+    S, T, V = np.zeros((M, M)), np.zeros((M, M)), np.zeros((M, M))
+    lib().orc_oneint(_ptr(prims), ctypes.c_int(len(prims)), _ptr(norm), ctypes.c_int(M), ctypes.c_int(len(Z)), _ptr(Z),
+                     _ptr(xyz), _ptr(S), _ptr(T), _ptr(V))
+    return S, T, V
+
+
+def eigh(A):
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    n = A.shape[0]
+    w = np.zeros(n)
+    V = np.zeros((n, n))
+    lib().orc_eigh(ctypes.c_int(n), _ptr(A), _ptr(w), _ptr(V))
+    return w, V
+
+
+def cholesky(A):
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    n = A.shape[0]
+    L = np.zeros((n, n))
+    if lib().orc_cholesky(ctypes.c_int(n), _ptr(A), _ptr(L)):
+        raise np.linalg.LinAlgError("not SPD")
+    return L
+
+
+def optimal_damping(F, Fp, D, Dp):
+    arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (F, Fp, D, Dp)]
+    return lib().orc_optimal_damping(ctypes.c_int(arrs[0].shape[0]), *[_ptr(a) for a in arrs])
+
+
+def nuclear_repulsion(Z, xyz_bohr):
+    xyz = np.asarray(xyz_bohr, dtype=np.float64).reshape(-1, 3)
+    e = 0.0
+    for a in range(len(Z)):
+        for b in range(a):
+            e += Z[a] * Z[b] / np.linalg.norm(xyz[a] - xyz[b])
+    return e
+
+
+def rhf(prims, types, norm, Z, xyz_bohr, nocc, thr=1e-12, conv=1e-10, maxiter=100, diis_size=8, dense=True,
+        diis_include_transform=False):
+    """Full FP64 RHF on the host.  Returns (E_elec, E_total, D, F, iterations)."""
+    M = len(norm)
+    S, T, V = oneint(prims, norm, Z, xyz_bohr)
+    H = np.ascontiguousarray(T + V)
+    Q = schwarz(prims, types)
+    eri = eri_dense(prims, types, norm, Q, thr) if dense else None
+    enuc = nuclear_repulsion(Z, xyz_bohr)
+    en = np.zeros(2)
+    D = np.zeros((M, M))
+    F = np.zeros((M, M))
+    it = lib().orc_rhf(ctypes.c_int(M), ctypes.c_int(nocc), _ptr(S), _ptr(H),
+                       _ptr(eri) if eri is not None else None, _ptr(prims), _ptr(types), ctypes.c_int(len(types)),
+                       _ptr(norm), _ptr(Q), ctypes.c_double(thr), ctypes.c_double(enuc), ctypes.c_int(maxiter),
+                       ctypes.c_double(conv), ctypes.c_int(diis_size), ctypes.c_int(int(diis_include_transform)),
+                       _ptr(en), _ptr(D), _ptr(F))
+    return en[0], en[1], D, F, it
+
+
+def eri_prim_os(ctr, ex, ang):
+    ctr = np.ascontiguousarray(ctr, dtype=np.float64).reshape(-1)
+    ex = np.ascontiguousarray(ex, dtype=np.float64)
+    ang = np.ascontiguousarray(ang, dtype=np.int32)
+    return lib().orc_eri_prim_os(_ptr(ctr), _ptr(ex), _ptr(ang))
+
+
+def eri_prim_md(ctr, ex, ang):
+    ctr = np.ascontiguousarray(ctr, dtype=np.float64).reshape(-1)
+    ex = np.ascontiguousarray(ex, dtype=np.float64)
+    ang = np.ascontiguousarray(ang, dtype=np.int32)
+    return lib().orc_eri_prim_md(_ptr(ctr), _ptr(ex), _ptr(ang))
+
+
+def int2c2e(aux_prims, aux_norm, Qaux, thr=1e-12):
+    Ma = len(aux_norm)
+    out = np.zeros((Ma, Ma))
+    lib().orc_int2c2e(_ptr(aux_prims), ctypes.c_int(len(aux_prims)), _ptr(aux_norm), ctypes.c_int(Ma), _ptr(Qaux),
+                      ctypes.c_double(thr), _ptr(out))
+    return out
+
+
+def int3c2e(prims, types, norm, Q, aux_prims, aux_norm, Qaux, thr=1e-12):
+    M, Ma = len(norm), len(aux_norm)
+    out = np.zeros((Ma, M, M))
+    lib().orc_int3c2e(_ptr(prims), _ptr(types), ctypes.c_int(len(types)), _ptr(norm), ctypes.c_int(M), _ptr(Q),
+                      _ptr(aux_prims), ctypes.c_int(len(aux_prims)), _ptr(aux_norm), ctypes.c_int(Ma), _ptr(Qaux),
+                      ctypes.c_double(thr), _ptr(out))
+    return out
+
+
+def ri_B(L, T):
+    T = np.ascontiguousarray(T, dtype=np.float64).copy()
+    Ma = L.shape[0]
+    ncols = T.size // Ma
+    lib().orc_ri_B(ctypes.c_int(Ma), _ptr(np.ascontiguousarray(L)), ctypes.c_size_t(ncols), _ptr(T))
+    return T
+
+
+def ri_jk(B, D):
+    Ma, M = B.shape[0], B.shape[1]
+    J, K = np.zeros((M, M)), np.zeros((M, M))
+    lib().orc_ri_jk(ctypes.c_int(Ma), ctypes.c_int(M), _ptr(np.ascontiguousarray(B)),
+                    _ptr(np.ascontiguousarray(D, dtype=np.float64)), _ptr(J), _ptr(K))
+    return J, K

@@ -1,0 +1,133 @@
+"""GPU parity of the Fock-build path against the FP64 host oracle.
+
+Tolerances: integer/index work (pair order, screening masks) bit-exact;
+FP64 integrals and J/K within 1e-10 of max|J|, max|K| (summation order of
+FP64 atomics differs from the oracle's deterministic sums); SCF energies
+within 1e-9 Eh.
+"""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from helpers import both, ethane, methane_water, random_density
+from dft_gansu_amd import FockEngine, Parameters, oneint, rhf
+from dft_gansu_amd import molecule as gm
+
+pytestmark = pytest.mark.gpu
+
+THR = 1e-12
+
+
+def _torch(a, cuda):
+    import torch
+    return torch.as_tensor(np.ascontiguousarray(a), dtype=torch.float64, device=cuda)
+
+
+SYSTEMS = {
+    "h2o_sto3g": lambda: (gm.water(), "sto-3g"),
+    "h2o_ccpvdz": lambda: (gm.water(), "cc-pvdz"),
+    "ch4h2o_ccpvdz": lambda: (methane_water(), "cc-pvdz"),
+    "ethane_631gs": lambda: (ethane(), "6-31g_st"),
+}
+
+
+@pytest.mark.parametrize("name", list(SYSTEMS))
+def test_schwarz_factors_and_masks(name, cuda):
+    (Z, xyz), basis = SYSTEMS[name]()
+    mol, (p, t, n) = both(Z, xyz, basis)
+    eng = FockEngine(mol, eri_method="direct")
+    eng.precomputation()
+    q_gpu = eng.schwarz_factors()
+    q_ref = O.schwarz(p, t)
+    assert q_gpu.shape == q_ref.shape
+    np.testing.assert_allclose(q_gpu, q_ref, rtol=1e-13, atol=1e-300)
+    # screening mask Q_x Q_y >= thr: bit-exact outside a 1e-12 relative band
+    prod_g = np.outer(q_gpu, q_gpu)
+    prod_r = np.outer(q_ref, q_ref)
+    band = np.abs(prod_r / THR - 1.0) < 1e-12
+    assert np.array_equal((prod_g >= THR)[~band], (prod_r >= THR)[~band])
+    assert band.sum() == 0
+
+
+@pytest.mark.parametrize("name", list(SYSTEMS))
+@pytest.mark.parametrize("method", ["direct", "stored"])
+def test_jk_matches_oracle(name, method, cuda):
+    (Z, xyz), basis = SYSTEMS[name]()
+    mol, (p, t, n) = both(Z, xyz, basis)
+    M = mol.n_basis
+    D = random_density(M)
+    eng = FockEngine(mol, eri_method=method)
+    eng.precomputation()
+    J, K = eng.compute_jk(_torch(D, cuda))
+    J, K = J.cpu().numpy(), K.cpu().numpy()
+    Q = O.schwarz(p, t)
+    Jr, Kr = O.jk(p, t, n, Q, D, THR)
+    assert np.abs(J - Jr).max() <= 1e-10 * max(1.0, np.abs(Jr).max())
+    assert np.abs(K - Kr).max() <= 1e-10 * max(1.0, np.abs(Kr).max())
+    np.testing.assert_allclose(J, J.T, atol=1e-13 * np.abs(J).max())
+
+
+def test_fock_rhf_matches_dense_oracle(cuda):
+    (Z, xyz), basis = SYSTEMS["h2o_ccpvdz"]()
+    mol, (p, t, n) = both(Z, xyz, basis)
+    M = mol.n_basis
+    D = random_density(M, seed=11)
+    H = random_density(M, seed=12) - 1.0
+    eng = FockEngine(mol, eri_method="stored")
+    eng.precomputation()
+    F = eng.compute_fock_matrix(_torch(D, cuda), _torch(H, cuda)).cpu().numpy()
+    Q = O.schwarz(p, t)
+    eri = O.eri_dense(p, t, n, Q, THR)
+    F_ref = np.zeros((M, M))
+    O.lib().orc_fock_from_dense(O._ptr(eri), O._ptr(np.ascontiguousarray(D)), O._ptr(np.ascontiguousarray(H)), M,
+                                O._ptr(F_ref))
+    assert np.abs(F - F_ref).max() <= 1e-10 * np.abs(F_ref).max()
+    assert eng.get_algorithm_name() == "Stored"
+
+
+@pytest.mark.parametrize("name", ["h2o_sto3g", "ch4h2o_ccpvdz"])
+def test_oneint_matches_oracle(name, cuda):
+    (Z, xyz), basis = SYSTEMS[name]()
+    mol, (p, t, n) = both(Z, xyz, basis)
+    S, T, V = (x.cpu().numpy() for x in oneint(mol))
+    Sr, Tr, Vr = O.oneint(p, n, Z, xyz)
+    for a, b in ((S, Sr), (T, Tr), (V, Vr)):
+        assert np.abs(a - b).max() <= 1e-11 * max(1.0, np.abs(b).max())
+
+
+@pytest.mark.parametrize("name,method", [("h2o_sto3g", "stored"), ("h2o_sto3g", "direct"), ("h2o_ccpvdz", "stored"),
+                                         ("ethane_631gs", "direct")])
+def test_scf_energy_matches_oracle(name, method, cuda):
+    (Z, xyz), basis = SYSTEMS[name]()
+    mol, (p, t, n) = both(Z, xyz, basis)
+    prm = Parameters(eri_method=method, convergence_energy_threshold=1e-10, maxiter=100)
+    r = rhf(mol, prm)
+    e_elec, e_tot, _, _, _ = O.rhf(p, t, n, Z, xyz, mol.n_electrons // 2, thr=THR, conv=1e-10)
+    assert r["converged"]
+    assert abs(r["e_total"] - e_tot) < 1e-9, (r["e_total"], e_tot)
+
+
+def test_config0_energy(cuda):
+    mol, aux = gm.config_molecule(0)
+    p, t, n = O.build_molecule(gm.basis_path("sto-3g"), mol.Z, mol.xyz)
+    r = rhf(mol, Parameters(convergence_energy_threshold=1e-10))
+    _, e_tot, _, _, _ = O.rhf(p, t, n, mol.Z, mol.xyz, 5, conv=1e-10)
+    assert abs(r["e_total"] - e_tot) < 1e-9
+    assert -75.05 < r["e_total"] < -74.85
+
+
+def test_stored_equals_direct_config1(cuda):
+    """Full-size C10H22/cc-pVDZ: the stored and direct builds agree and J, K are
+    symmetric (size-independent properties; the oracle is too slow here)."""
+    mol, _ = gm.config_molecule(1)
+    M = mol.n_basis
+    D = _torch(random_density(M, seed=5), cuda)
+    res = {}
+    for m in ("direct", "stored"):
+        eng = FockEngine(mol, eri_method=m)
+        eng.precomputation()
+        res[m] = [x.cpu().numpy() for x in eng.compute_jk(D)]
+        del eng
+    for a, b in zip(res["direct"], res["stored"]):
+        assert np.abs(a - b).max() <= 1e-11 * np.abs(a).max()
+        np.testing.assert_allclose(a, a.T, atol=1e-12 * np.abs(a).max())
