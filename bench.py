@@ -1,0 +1,209 @@
+#!/usr/bin/env python3
+"""Fock-build benchmark on one MI355X.
+
+Workload (default): all-trans C10H22 (seeded +-0.02 A jitter), cc-pVDZ, 260
+Cartesian basis functions, RHF, conventional (stored) ERIs, Schwarz 1e-12.
+A step is one Fock build F = H + J[D] - K[D]/2 with the integrals already
+resident in HBM (the reference's ERI::compute_fock_matrix on the stored path).
+
+Prints one JSON line with the measured Fock builds/s, the roofline figures of
+the dominant kernel (timed live with HIP events on the build stream), the SCF
+wall time, the direct (integral-recomputing) build rate, and a CPU baseline:
+the FP64 host oracle's direct J/K build on a bounded sample of the same
+workload, extrapolated.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def kernel_name(kid):
+    kid = int(kid)
+    mode = {0: "direct", 1: "store", 2: "stored-digest"}.get(kid // 10000, "?")
+    l = "spdf"
+    c = kid % 10000
+    la, lb, lc, ld = c // 1000, (c // 100) % 10, (c // 10) % 10, c % 10
+    return f"eri_class({l[la]}{l[lb]}|{l[lc]}{l[ld]}) {mode}"
+
+
+def cpu_baseline(cfg, target_s):
+    """FP64 host oracle direct J/K on a strided sample of bra primitive-pair rows."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    from dft_gansu_amd import molecule as gm
+    c = gm.CONFIGS[cfg]
+    Z, xyz = c["geom"]()
+    p, t, n = O.build_molecule(gm.basis_path(c["basis"]), Z, xyz)
+    Q = O.schwarz(p, t)
+    rng = np.random.default_rng(5)
+    C = rng.normal(size=(len(n), max(1, len(n) // 3))) / np.sqrt(len(n))
+    D = 2.0 * C @ C.T
+    stride = 4096
+    while True:
+        t0 = time.perf_counter()
+        O.jk(p, t, n, Q, D, 1e-12, sample_stride=stride)
+        dt = time.perf_counter() - t0
+        if dt * 4 > target_s or stride == 1:
+            break
+        stride = max(1, int(stride * max(dt, 1e-3) / target_s * 0.9)) if dt > 0.05 else max(1, stride // 8)
+    full_s = dt * stride
+    return dict(value=1.0 / full_s, unit="Fock builds/s", cores=O.lib().orc_num_threads(), kind="port",
+                sample=f"direct J/K (MD integrals + digestion) over every {stride}-th bra primitive-pair row "
+                       f"({dt:.1f} s measured, x{stride} extrapolated; full build ~{full_s:.0f} s)")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--config", type=int, default=1)
+    ap.add_argument("--method", default="stored", choices=["stored", "direct"])
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-scf", action="store_true")
+    ap.add_argument("--no-direct", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: the Fock build runs on a single MI355X in this build "
+              "(multi-GPU sharding is described, not run); use --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    from dft_gansu_amd import FockEngine, Parameters, oneint, rhf
+    from dft_gansu_amd import molecule as gm
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    dev = torch.device("cuda:0")
+    cfgd = gm.CONFIGS[args.config]
+    mol, aux = gm.config_molecule(args.config)
+    M = mol.n_basis
+    log(f"[bench] {cfgd['name']}: M={M} prims={mol.n_prims} method={args.method} precision={args.precision}")
+
+    eng = FockEngine(mol, eri_method=args.method, precision=args.precision)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    eng.precomputation()
+    torch.cuda.synchronize()
+    precompute_s = time.perf_counter() - t0
+    st = eng.stats()
+    log(f"[bench] precompute {precompute_s:.3f} s, quartets {st['contracted_quartets']:.3e}, "
+        f"stored {st['stored_bytes'] / 1e9:.2f} GB")
+
+    S, T, V = oneint(mol)
+    H = (T + V).contiguous()
+    rng = np.random.default_rng(5)
+    Cm = rng.normal(size=(M, mol.n_electrons // 2)) / np.sqrt(M)
+    D = torch.tensor(2.0 * Cm @ Cm.T, dtype=torch.float64, device=dev)
+    F = torch.empty_like(D)
+    stream = torch.cuda.current_stream()
+
+    for _ in range(args.warmup):
+        eng.compute_fock_matrix(D, H, F, stream=stream)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        eng.compute_fock_matrix(D, H, F, stream=stream)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    ms_per_step = elapsed / args.steps * 1e3
+    log(f"[bench] {args.steps} builds in {elapsed:.3f} s -> {ms_per_step:.3f} ms/build")
+
+    # live per-kernel timing (HIP events on the build stream), averaged over builds
+    eng.set_profiling(True)
+    agg = {}
+    nprof = max(3, min(10, args.steps))
+    for _ in range(nprof):
+        eng.compute_fock_matrix(D, H, F, stream=stream)
+        for kid, ms, by, fl, un in eng.last_profile():
+            a = agg.setdefault(int(kid), [0.0, by, fl, un])
+            a[0] += ms / nprof
+    eng.set_profiling(False)
+    torch.cuda.synchronize()
+    kernels = sorted(((k, v[0], v[1], v[2], v[3]) for k, v in agg.items()), key=lambda r: -r[1])
+    dom = kernels[0]
+    build_kernel_ms = sum(r[1] for r in kernels)
+    all_bytes = sum(r[2] for r in kernels)
+    if args.method == "stored":
+        roof = dict(bound="hbm", kernel=kernel_name(dom[0]), achieved=round(dom[2] / (dom[1] * 1e-3) / 1e9, 1),
+                    peak=HBM_PEAK_GBS, unit="GB/s", frac=round(dom[2] / (dom[1] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                    traffic=None, bytes_per_launch=dom[2], avg_launch_ms=round(dom[1], 4),
+                    all_digest_kernels=dict(ms=round(build_kernel_ms, 4), bytes=all_bytes,
+                                            GBs=round(all_bytes / (build_kernel_ms * 1e-3) / 1e9, 1)))
+    else:
+        roof = dict(bound="fp64-valu", kernel=kernel_name(dom[0]), achieved=None, peak=78.6, unit="TFLOP/s",
+                    frac=None, traffic=None, avg_launch_ms=round(dom[1], 4))
+    per_kernel = [dict(kernel=kernel_name(k), ms=round(ms, 4), quartets=un) for k, ms, by, fl, un in kernels[:8]]
+    log(f"[bench] dominant kernel {roof['kernel']}: {dom[1]:.4f} ms")
+
+    extra = {}
+    del eng
+    torch.cuda.empty_cache()
+    if not args.no_direct and args.method != "direct":
+        e2 = FockEngine(mol, eri_method="direct", precision=args.precision)
+        e2.precomputation()
+        e2.compute_fock_matrix(D, H, F, stream=stream)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        nd = 3
+        for _ in range(nd):
+            e2.compute_fock_matrix(D, H, F, stream=stream)
+        torch.cuda.synchronize()
+        extra["direct_fock_builds_per_s"] = round(nd / (time.perf_counter() - t0), 3)
+        log(f"[bench] direct build rate {extra['direct_fock_builds_per_s']} /s")
+        del e2
+    if not args.no_scf:
+        t0 = time.perf_counter()
+        r = rhf(mol, Parameters(eri_method=args.method, precision=args.precision,
+                                convergence_energy_threshold=1e-8, maxiter=100), aux=aux)
+        extra.update(scf_wall_s=round(time.perf_counter() - t0, 3), scf_energy=r["e_total"],
+                     scf_iterations=r["iterations"], scf_converged=r["converged"])
+        log(f"[bench] SCF {extra['scf_wall_s']} s, E = {r['e_total']:.10f} ({r['iterations']} it)")
+
+    cpu = None
+    if not args.no_cpu:
+        log("[bench] CPU baseline (oracle) ...")
+        cpu = cpu_baseline(args.config, args.cpu_seconds)
+        log(f"[bench] CPU baseline {cpu['value']:.3e} builds/s on {cpu['cores']} threads")
+
+    out = {
+        "metric": "Fock builds/s (SCF wall s, mixed vs FP64 on GPU)",
+        "value": round(args.steps / elapsed, 3),
+        "unit": "Fock builds/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f64+f32",
+        "data": "synthetic (seeded geometry generator; random PSD density of rank nocc)",
+        "config": {"workload": f"{cfgd['name']} RHF {args.method} ERI, Schwarz 1e-12", "n_basis_cartesian": M,
+                   "n_primitives": mol.n_prims, "eri_method": args.method, "precision": args.precision,
+                   "precompute_s": round(precompute_s, 4), "contracted_quartets": st["contracted_quartets"],
+                   "stored_bytes": st["stored_bytes"]},
+        "roofline": roof,
+        "kernels": per_kernel,
+        "cpu_baseline": cpu,
+    }
+    out.update(extra)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -28,7 +28,7 @@ EXPORTS = [
     "gansu_params_create", "gansu_params_load_recipe", "gansu_params_set", "gansu_params_get", "gansu_params_destroy",
     "gansu_fock_create", "gansu_fock_precompute", "gansu_fock_build_rhf", "gansu_fock_build_jk",
     "gansu_fock_algorithm_name", "gansu_fock_schwarz", "gansu_fock_num_primitive_pairs", "gansu_fock_stats",
-    "gansu_fock_destroy", "gansu_oneint", "gansu_rhf_run",
+    "gansu_fock_set_profiling", "gansu_fock_last_profile", "gansu_fock_destroy", "gansu_oneint", "gansu_rhf_run",
 ]
 
 
@@ -74,6 +74,8 @@ def lib():
         L.gansu_fock_num_primitive_pairs.argtypes = [vp]
         L.gansu_fock_num_primitive_pairs.restype = sz
         L.gansu_fock_stats.argtypes = [vp, vp]
+        L.gansu_fock_set_profiling.argtypes = [vp, ip]
+        L.gansu_fock_last_profile.argtypes = [vp, vp, ip, vp]
         L.gansu_fock_destroy.argtypes = [vp]
         L.gansu_oneint.argtypes = [vp, vp, vp, vp, vp]
         L.gansu_rhf_run.argtypes = [vp, vp, vp, vp, vp, vp]

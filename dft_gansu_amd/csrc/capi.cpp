@@ -180,6 +180,23 @@ int gansu_fock_stats(const gansu_fock_engine* e, double* stats) {
   });
 }
 
+int gansu_fock_set_profiling(gansu_fock_engine* e, int enable) {
+  return guarded([&] {
+    if (!e) throw gansu::Error(GANSU_E_ARG, "null engine");
+    e->e->set_profiling(enable != 0);
+  });
+}
+
+int gansu_fock_last_profile(gansu_fock_engine* e, double* rows, int max_rows, int* n_rows) {
+  return guarded([&] {
+    if (!e || !n_rows) throw gansu::Error(GANSU_E_ARG, "null argument");
+    auto v = e->e->last_profile();
+    const int n = (int)(v.size() / 5);
+    *n_rows = n;
+    if (rows) std::memcpy(rows, v.data(), sizeof(double) * 5 * std::min(n, max_rows));
+  });
+}
+
 void gansu_fock_destroy(gansu_fock_engine* e) { delete e; }
 
 int gansu_oneint(const gansu_molecule* mol, double* d_S, double* d_T, double* d_V, void* stream) {

@@ -94,7 +94,48 @@ FockEngine::FockEngine(const gansu_primitive_shell* prims, size_t n_prims, const
   }
 }
 
-FockEngine::~FockEngine() { ri_destroy(ri_); }
+FockEngine::~FockEngine() {
+  prof_clear();
+  ri_destroy(ri_);
+}
+
+void FockEngine::set_profiling(bool on) {
+  prof_clear();
+  profiling_ = on;
+}
+
+void FockEngine::prof_clear() {
+  for (auto& r : prof_) {
+    (void)hipEventDestroy(r.a);
+    (void)hipEventDestroy(r.b);
+  }
+  prof_.clear();
+}
+
+void FockEngine::prof_begin(hipStream_t s, double id, double bytes, double flops, double units) {
+  if (!profiling_) return;
+  ProfRow r{id, bytes, flops, units, nullptr, nullptr};
+  GANSU_HIP(hipEventCreate(&r.a));
+  GANSU_HIP(hipEventCreate(&r.b));
+  GANSU_HIP(hipEventRecord(r.a, s));
+  prof_.push_back(r);
+}
+
+void FockEngine::prof_end(hipStream_t s) {
+  if (!profiling_) return;
+  GANSU_HIP(hipEventRecord(prof_.back().b, s));
+}
+
+std::vector<double> FockEngine::last_profile() {
+  std::vector<double> out;
+  for (auto& r : prof_) {
+    GANSU_HIP(hipEventSynchronize(r.b));
+    float ms = 0.f;
+    GANSU_HIP(hipEventElapsedTime(&ms, r.a, r.b));
+    out.insert(out.end(), {r.id, (double)ms, r.bytes, r.flops, r.units});
+  }
+  return out;
+}
 
 const char* FockEngine::algorithm_name() const {
   if (method_ == "stored") return "Stored";
@@ -250,15 +291,23 @@ void FockEngine::precompute_four_center() {
 }
 
 void FockEngine::build_four_center(const double* d_D, hipStream_t s) {
+  prof_clear();
   GANSU_HIP(launch_scale_density(d_D, d_w_.ptr, M_, d_Dp_.ptr, s));
   GANSU_HIP(launch_zero(d_Jr_.ptr, (size_t)M_ * M_, s));
   GANSU_HIP(launch_zero(d_Kr_.ptr, (size_t)M_ * M_, s));
   EriArgs g{d_sp_.ptr, d_pr_.ptr, boys_.ptr, thr_, d_Dp_.ptr, M_, d_Jr_.ptr, d_Kr_.ptr};
   const int mode = method_ == "stored" ? kModeStoredDigest : kModeDirect;
   for (auto& qc : qclass_) {
+    if (qc->blocks_h.empty()) continue;
     ClassLaunch c{qc->LA, qc->LB, qc->LC, qc->LD, qc->blocks.ptr, (int)qc->blocks_h.size(),
                   pclass_[qc->A].list.ptr, pclass_[qc->B].list.ptr, qc->store.ptr};
+    const double ncomp = double(ncart(qc->LA) * ncart(qc->LB) * ncart(qc->LC) * ncart(qc->LD));
+    // algorithmic traffic of the stored digestion: each surviving integral read once
+    const double bytes = mode == kModeStoredDigest ? qc->n_quartets * ncomp * 8.0 : 0.0;
+    prof_begin(s, 10000.0 * mode + 1000 * qc->LA + 100 * qc->LB + 10 * qc->LC + qc->LD, bytes, 0.0,
+               qc->n_quartets);
     GANSU_HIP(launch_eri_class(c, g, mode, s));
+    prof_end(s);
   }
 }
 

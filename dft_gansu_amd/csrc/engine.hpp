@@ -82,6 +82,8 @@ class FockEngine {
   size_t num_primitive_pairs() const { return ref_pairs_; }
   void stats(double* out) const;
   int num_basis() const { return M_; }
+  void set_profiling(bool on);
+  std::vector<double> last_profile();  // 5 values per launch
 
  private:
   void precompute_four_center();
@@ -114,6 +116,14 @@ class FockEngine {
   std::vector<std::unique_ptr<QuartetClass>> qclass_;
   DevBuf<double> d_w_, d_Dp_, d_Jr_, d_Kr_;
   double n_quartets_ = 0, n_prim_quartets_ = 0, stored_bytes_ = 0;
+
+  // profiling: events around each launch of a build
+  struct ProfRow { double id, bytes, flops, units; hipEvent_t a, b; };
+  bool profiling_ = false;
+  std::vector<ProfRow> prof_;
+  void prof_begin(hipStream_t s, double id, double bytes, double flops, double units);
+  void prof_end(hipStream_t s);
+  void prof_clear();
 
   // RI state
   std::vector<gansu_primitive_shell> aux_prims_;

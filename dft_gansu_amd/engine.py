@@ -121,6 +121,17 @@ class FockEngine:
         return dict(contracted_quartets=s[0], primitive_quartets=s[1], stored_bytes=s[2], fp32_quartets=s[3],
                     unique_quartets=s[4], ri_bytes=s[5])
 
+    def set_profiling(self, enable=True):
+        check(lib().gansu_fock_set_profiling(self._h, int(enable)))
+
+    def last_profile(self):
+        """Per-launch rows of the last build: (kernel id, ms, bytes, flops, units)."""
+        n = ctypes.c_int()
+        check(lib().gansu_fock_last_profile(self._h, None, 0, ctypes.byref(n)))
+        rows = np.zeros((max(n.value, 1), 5))
+        check(lib().gansu_fock_last_profile(self._h, ptr(rows), n.value, ctypes.byref(n)))
+        return rows[:n.value]
+
     def __del__(self):
         h = getattr(self, "_h", None)
         if h is not None and h.value:

@@ -126,6 +126,14 @@ size_t gansu_fock_num_primitive_pairs(const gansu_fock_engine* e);
  *             stored bytes, fp32 quartets, unique contracted quartets,
  *             ri B bytes, 0, 0}                                             */
 int gansu_fock_stats(const gansu_fock_engine* e, double* stats);
+/* Per-launch timing of the builds that follow, with HIP events recorded on the
+ * build stream (no counterpart in the reference, whose profiler.hpp times host
+ * wall clock only).  After a build, rows[5*i + k] for launch i hold
+ * {kernel id, milliseconds, algorithmic bytes, algorithmic flops, work units}
+ * where kernel id = 1000*LA + 100*LB + 10*LC + LD + 10000*mode for ERI class
+ * kernels (mode 0 direct, 2 stored digestion) and 90000+ for helper kernels. */
+int gansu_fock_set_profiling(gansu_fock_engine* e, int enable);
+int gansu_fock_last_profile(gansu_fock_engine* e, double* rows, int max_rows, int* n_rows);
 void gansu_fock_destroy(gansu_fock_engine* e);
 
 /* ------------------------------------------------ one-electron / SCF -----
