@@ -30,11 +30,13 @@ def log(msg):
 
 def kernel_name(kid):
     kid = int(kid)
+    slab = kid >= 100000
+    kid %= 100000
     mode = {0: "direct", 1: "store", 2: "stored-digest"}.get(kid // 10000, "?")
     l = "spdf"
     c = kid % 10000
     la, lb, lc, ld = c // 1000, (c // 100) % 10, (c // 10) % 10, c % 10
-    return f"eri_class({l[la]}{l[lb]}|{l[lc]}{l[ld]}) {mode}"
+    return f"{'fock_slab' if slab else 'eri_class'}({l[la]}{l[lb]}|{l[lc]}{l[ld]}) {mode}"
 
 
 def cpu_baseline(cfg, target_s):

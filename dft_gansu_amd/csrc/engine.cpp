@@ -266,12 +266,14 @@ void FockEngine::precompute_four_center() {
         }
         int count = lo;
         if (A == B) count = std::min(count, s + 1);
+        if (count > 0) qc->bras_h.push_back(BraRec{s, (int)qc->blocks_h.size(), count, 0});
         for (int k0 = 0; k0 < count; k0 += 64)
           qc->blocks_h.push_back(BlockRec{s, k0, std::min(64, count - k0), 0});
         qc->n_quartets += count;
       }
       n_quartets_ += qc->n_quartets;
       qc->blocks.upload(qc->blocks_h);
+      qc->bras.upload(qc->bras_h);
       qclass_.push_back(std::move(qc));
     }
 
@@ -297,16 +299,30 @@ void FockEngine::build_four_center(const double* d_D, hipStream_t s) {
   GANSU_HIP(launch_zero(d_Kr_.ptr, (size_t)M_ * M_, s));
   EriArgs g{d_sp_.ptr, d_pr_.ptr, boys_.ptr, thr_, d_Dp_.ptr, M_, d_Jr_.ptr, d_Kr_.ptr};
   const int mode = method_ == "stored" ? kModeStoredDigest : kModeDirect;
+  if (max_lds_ == 0) {
+    int dev = 0, v = 0;
+    GANSU_HIP(hipGetDevice(&dev));
+    GANSU_HIP(hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    max_lds_ = (size_t)v;
+  }
   for (auto& qc : qclass_) {
     if (qc->blocks_h.empty()) continue;
-    ClassLaunch c{qc->LA, qc->LB, qc->LC, qc->LD, qc->blocks.ptr, (int)qc->blocks_h.size(),
-                  pclass_[qc->A].list.ptr, pclass_[qc->B].list.ptr, qc->store.ptr};
     const double ncomp = double(ncart(qc->LA) * ncart(qc->LB) * ncart(qc->LC) * ncart(qc->LD));
     // algorithmic traffic of the stored digestion: each surviving integral read once
     const double bytes = mode == kModeStoredDigest ? qc->n_quartets * ncomp * 8.0 : 0.0;
-    prof_begin(s, 10000.0 * mode + 1000 * qc->LA + 100 * qc->LB + 10 * qc->LC + qc->LD, bytes, 0.0,
-               qc->n_quartets);
-    GANSU_HIP(launch_eri_class(c, g, mode, s));
+    const bool slab = slab_available(qc->LA, qc->LB, qc->LC, qc->LD, mode) &&
+                      slab_lds_bytes(qc->LA, qc->LB, M_) <= max_lds_;
+    prof_begin(s, 10000.0 * mode + 1000 * qc->LA + 100 * qc->LB + 10 * qc->LC + qc->LD + (slab ? 100000.0 : 0.0),
+               bytes, 0.0, qc->n_quartets);
+    if (slab) {
+      SlabLaunch c{qc->LA, qc->LB, qc->LC, qc->LD, qc->bras.ptr, (int)qc->bras_h.size(),
+                   pclass_[qc->A].list.ptr, pclass_[qc->B].list.ptr, qc->store.ptr};
+      GANSU_HIP(launch_fock_slab(c, g, mode, s));
+    } else {
+      ClassLaunch c{qc->LA, qc->LB, qc->LC, qc->LD, qc->blocks.ptr, (int)qc->blocks_h.size(),
+                    pclass_[qc->A].list.ptr, pclass_[qc->B].list.ptr, qc->store.ptr};
+      GANSU_HIP(launch_eri_class(c, g, mode, s));
+    }
     prof_end(s);
   }
 }

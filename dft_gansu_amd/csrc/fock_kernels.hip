@@ -262,32 +262,16 @@ __device__ __forceinline__ void digest_block(const double* out, bool active, con
   }
 }
 
-// ------------------------------------------------- class-templated kernel --
-// One lane = one contracted shell quartet; full register tiles.  Used for the
-// classes whose block and Hermite tables fit in registers.
-template <int LA, int LB, int LC, int LD, int MODE>
-__global__ __launch_bounds__(64) void k_eri_class(ClassLaunch c, EriArgs g) {
+// Contracted block (ab|cd), [ab][cd] order, accumulated into `out` over the
+// screened primitive quartets of one bra and one ket shell pair.  nk is the
+// number of ket primitive pairs to visit (0 for an idle lane).
+template <int LA, int LB, int LC, int LD>
+__device__ __forceinline__ void eri_block_t(const SPair& bra, const SPair& ket, int nk, const EriArgs& g,
+                                            double* out) {
   constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
-  constexpr int NAB = NA * NB, NCD = NC * ND, NCOMP = NAB * NCD;
+  constexpr int NAB = NA * NB, NCD = NC * ND;
   constexpr int LAB = LA + LB, LCD = LC + LD, L = LAB + LCD;
-  const BlockRec blk = c.blocks[blockIdx.x];
-  const int lane = threadIdx.x;
-  const bool active = lane < blk.count;
-  const int bra_id = c.bra_list[blk.bra_pos];
-  const int ket_id = c.ket_list[blk.ket_start + (active ? lane : 0)];
-  const SPair bra = g.sp[bra_id];
-  const SPair ket = g.sp[ket_id];
-
-  double out[NCOMP];
-#pragma unroll
-  for (int k = 0; k < NCOMP; ++k) out[k] = 0.0;
-
-  if constexpr (MODE == kModeStoredDigest) {
-    const double* src = c.store + (size_t)blockIdx.x * NCOMP * 64 + lane;
-#pragma unroll
-    for (int k = 0; k < NCOMP; ++k) out[k] = active ? src[(size_t)k * 64] : 0.0;
-  } else {
-    const int nk = active ? ket.nprim : 0;
+  (void)NA; (void)NC;
     for (int ib = 0; ib < bra.nprim; ++ib) {
       const PairRec pb = g.pr[bra.prim_begin + ib];
       double Eb[3][LA + 1][LB + 1][LAB + 1];
@@ -361,6 +345,34 @@ __global__ __launch_bounds__(64) void k_eri_class(ClassLaunch c, EriArgs g) {
         }
       }
     }
+  }
+
+// ------------------------------------------------- class-templated kernel --
+// One lane = one contracted shell quartet; full register tiles.  Used for the
+// classes whose block and Hermite tables fit in registers.
+template <int LA, int LB, int LC, int LD, int MODE>
+__global__ __launch_bounds__(64) void k_eri_class(ClassLaunch c, EriArgs g) {
+  constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
+  constexpr int NAB = NA * NB, NCD = NC * ND, NCOMP = NAB * NCD;
+  constexpr int LAB = LA + LB, LCD = LC + LD, L = LAB + LCD;
+  const BlockRec blk = c.blocks[blockIdx.x];
+  const int lane = threadIdx.x;
+  const bool active = lane < blk.count;
+  const int bra_id = c.bra_list[blk.bra_pos];
+  const int ket_id = c.ket_list[blk.ket_start + (active ? lane : 0)];
+  const SPair bra = g.sp[bra_id];
+  const SPair ket = g.sp[ket_id];
+
+  double out[NCOMP];
+#pragma unroll
+  for (int k = 0; k < NCOMP; ++k) out[k] = 0.0;
+
+  if constexpr (MODE == kModeStoredDigest) {
+    const double* src = c.store + (size_t)blockIdx.x * NCOMP * 64 + lane;
+#pragma unroll
+    for (int k = 0; k < NCOMP; ++k) out[k] = active ? src[(size_t)k * 64] : 0.0;
+  } else {
+    eri_block_t<LA, LB, LC, LD>(bra, ket, active ? ket.nprim : 0, g, out);
   }
 
   if constexpr (MODE == kModeStore) {
@@ -551,6 +563,147 @@ hipError_t launch_t(const ClassLaunch& c, const EriArgs& a, int mode, hipStream_
   return hipGetLastError();
 }
 
+
+// --------------------------------------------------- slab digestion kernel --
+// One 256-thread workgroup per bra shell pair (i j); each thread takes one ket
+// (k l) at a time.  Every K update of a quartet lands in rows of shell i or j:
+//   K[a][c] += v D[b][d], K[b][c] += v D[a][d], K[a][d] += v D[b][c],
+//   K[b][d] += v D[a][c]      (a in i, b in j, c in k, d in l)
+// so the workgroup accumulates K[i rows][:] and K[j rows][:] in LDS (with the
+// matching D rows beside them) and flushes them once.  J of the bra block is
+// reduced over the workgroup; J of each ket block goes out with one atomic
+// per ket component.
+constexpr int kSlabThreads = 256;
+
+template <int LA, int LB, int LC, int LD, int MODE>
+__global__ __launch_bounds__(kSlabThreads) void k_fock_slab(SlabLaunch c, EriArgs g) {
+  extern __shared__ double lds[];
+  constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
+  constexpr int NAB = NA * NB, NCD = NC * ND, NCOMP = NAB * NCD;
+  const BraRec br = c.bras[blockIdx.x];
+  const int bra_id = c.bra_list[br.bra_pos];
+  const SPair bra = g.sp[bra_id];
+  const int M = g.M;
+  const int tid = threadIdx.x;
+  const bool ij_diff = bra.si != bra.sj;
+  const int bi = bra.bi, bj = bra.bj;
+  double* Ki = lds;           // [NA][M]
+  double* Kj = lds + NA * M;  // [NB][M]
+  double* Di = Kj + NB * M;   // [NA][M]: D rows of shell i
+  double* Dj = Di + NA * M;   // [NB][M]: D rows of shell j
+  for (int x = tid; x < (NA + NB) * M; x += kSlabThreads) {
+    lds[x] = 0.0;
+    const int r = x / M, col = x - r * M;
+    const int row = r < NA ? bi + r : bj + (r - NA);
+    Di[x] = g.Dp[(size_t)row * M + col];
+  }
+  __syncthreads();
+  double jb[NAB];
+#pragma unroll
+  for (int k = 0; k < NAB; ++k) jb[k] = 0.0;
+
+  for (int k0 = 0; k0 < br.count; k0 += kSlabThreads) {
+    const int kk = k0 + tid;
+    if (kk < br.count) {
+      const int ket_id = c.ket_list[kk];
+      const SPair ket = g.sp[ket_id];
+      const bool kl_diff = ket.si != ket.sj;
+      const bool pair_diff = bra_id != ket_id;
+      const double fbk = (ij_diff ? 2.0 : 1.0) * (kl_diff ? 2.0 : 1.0);
+      const double fo = pair_diff ? 2.0 : 1.0;
+      const int bk = ket.bi, bl = ket.bj;
+      [[maybe_unused]] double out[MODE == kModeDirect ? NCOMP : 1];
+      [[maybe_unused]] const double* src = nullptr;
+      if constexpr (MODE == kModeDirect) {
+#pragma unroll
+        for (int k = 0; k < NCOMP; ++k) out[k] = 0.0;
+        eri_block_t<LA, LB, LC, LD>(bra, ket, ket.nprim, g, out);
+      } else {
+        src = c.store + ((size_t)(br.first_block + kk / 64) * NCOMP) * 64 + (kk & 63);
+      }
+      auto V = [&](int comp) -> double {
+        if constexpr (MODE == kModeDirect) return out[comp];
+        else return src[(size_t)comp * 64];
+      };
+      double dcd[NCD], jk[NCD];
+#pragma unroll
+      for (int cc = 0; cc < NC; ++cc)
+#pragma unroll
+        for (int d = 0; d < ND; ++d) {
+          dcd[cc * ND + d] = fbk * g.Dp[(size_t)(bk + cc) * M + bl + d];
+          jk[cc * ND + d] = 0.0;
+        }
+#pragma unroll
+      for (int a = 0; a < NA; ++a)
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+          const double dab = Di[a * M + bj + b];
+          double kad[ND], kbd[ND];
+#pragma unroll
+          for (int d = 0; d < ND; ++d) { kad[d] = 0.0; kbd[d] = 0.0; }
+#pragma unroll
+          for (int cc = 0; cc < NC; ++cc) {
+            const double dbc = Dj[b * M + bk + cc];
+            const double dac = Di[a * M + bk + cc];
+            double kac = 0.0, kbc = 0.0;
+#pragma unroll
+            for (int d = 0; d < ND; ++d) {
+              const double v = V(((a * NB + b) * NC + cc) * ND + d);
+              jb[a * NB + b] += v * dcd[cc * ND + d];
+              jk[cc * ND + d] += v * dab;
+              kac += v * Dj[b * M + bl + d];
+              kbc += v * Di[a * M + bl + d];
+              kad[d] += v * dbc;
+              kbd[d] += v * dac;
+            }
+            unsafeAtomicAdd(&Ki[a * M + bk + cc], fo * kac);
+            if (ij_diff) unsafeAtomicAdd(&Kj[b * M + bk + cc], fo * kbc);
+          }
+          if (kl_diff) {
+#pragma unroll
+            for (int d = 0; d < ND; ++d) {
+              unsafeAtomicAdd(&Ki[a * M + bl + d], fo * kad[d]);
+              if (ij_diff) unsafeAtomicAdd(&Kj[b * M + bl + d], fo * kbd[d]);
+            }
+          }
+        }
+      if (pair_diff) {
+#pragma unroll
+        for (int cd = 0; cd < NCD; ++cd)
+          unsafeAtomicAdd(&g.Jr[(size_t)(bk + cd / ND) * M + bl + cd % ND], fbk * jk[cd]);
+      }
+    }
+  }
+  __syncthreads();
+  for (int x = tid; x < NA * M; x += kSlabThreads) {
+    const double v = Ki[x];
+    if (v != 0.0) unsafeAtomicAdd(&g.Kr[(size_t)(bi + x / M) * M + x % M], v);
+  }
+  if (ij_diff)
+    for (int x = tid; x < NB * M; x += kSlabThreads) {
+      const double v = Kj[x];
+      if (v != 0.0) unsafeAtomicAdd(&g.Kr[(size_t)(bj + x / M) * M + x % M], v);
+    }
+  // J of the bra block: wave sums, then one atomic per component per wave
+#pragma unroll
+  for (int ab = 0; ab < NAB; ++ab) {
+    const double sum = wave_sum(jb[ab]);
+    if ((tid & 63) == 0) unsafeAtomicAdd(&g.Jr[(size_t)(bi + ab / NB) * M + bj + ab % NB], sum);
+  }
+}
+
+template <int LA, int LB, int LC, int LD>
+hipError_t launch_slab_t(const SlabLaunch& c, const EriArgs& a, int mode, size_t lds, hipStream_t s) {
+  if (mode == kModeDirect)
+    hipLaunchKernelGGL((k_fock_slab<LA, LB, LC, LD, kModeDirect>), dim3(c.nbras), dim3(kSlabThreads), lds, s, c, a);
+  else
+    hipLaunchKernelGGL((k_fock_slab<LA, LB, LC, LD, kModeStoredDigest>), dim3(c.nbras), dim3(kSlabThreads), lds, s,
+                       c, a);
+  return hipGetLastError();
+}
+
+size_t g_max_lds = 0;
+
 }  // namespace
 
 // --------------------------------------------------------------- launchers --
@@ -597,6 +750,58 @@ hipError_t launch_eri_class(const ClassLaunch& c, const EriArgs& a, int mode, hi
       return hipGetLastError();
   }
 #undef GANSU_CASE
+}
+
+
+size_t slab_lds_bytes(int LA, int LB, int M) {
+  return 2ull * (size_t)(ncart(LA) + ncart(LB)) * (size_t)M * sizeof(double);
+}
+
+// Register-resident blocks (the classes of launch_eri_class's templated list)
+// can also be computed inside the slab kernel; every class can be digested
+// from stored blocks.
+bool slab_available(int LA, int LB, int LC, int LD, int mode) {
+  if (mode == kModeStoredDigest) return true;
+  const int L = LA + LB + LC + LD;
+  const int ncomp = ncart(LA) * ncart(LB) * ncart(LC) * ncart(LD);
+  return mode == kModeDirect && L <= 4 && ncomp <= 81;
+}
+
+hipError_t launch_fock_slab(const SlabLaunch& c, const EriArgs& a, int mode, hipStream_t s) {
+  if (c.nbras == 0) return hipSuccess;
+  if (!slab_available(c.LA, c.LB, c.LC, c.LD, mode)) return hipErrorInvalidValue;
+  if (g_max_lds == 0) {
+    int dev = 0, v = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    e = hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+    if (e != hipSuccess) return e;
+    g_max_lds = (size_t)v;
+  }
+  const size_t lds = slab_lds_bytes(c.LA, c.LB, a.M);
+  if (lds > g_max_lds) return hipErrorInvalidValue;
+  const int key = ((c.LA * 3 + c.LB) * 3 + c.LC) * 3 + c.LD;
+#define GANSU_SLAB(a_, b_, c_, d_) \
+  case ((a_ * 3 + b_) * 3 + c_) * 3 + d_: return launch_slab_t<a_, b_, c_, d_>(c, a, mode, lds, s);
+  if (mode == kModeDirect) {
+    switch (key) {
+      GANSU_SLAB(0, 0, 0, 0) GANSU_SLAB(1, 0, 0, 0) GANSU_SLAB(1, 0, 1, 0) GANSU_SLAB(1, 1, 0, 0)
+      GANSU_SLAB(1, 1, 1, 0) GANSU_SLAB(1, 1, 1, 1) GANSU_SLAB(2, 0, 0, 0) GANSU_SLAB(2, 0, 1, 0)
+      GANSU_SLAB(2, 0, 1, 1) GANSU_SLAB(2, 0, 2, 0) GANSU_SLAB(2, 1, 0, 0) GANSU_SLAB(2, 1, 1, 0)
+      GANSU_SLAB(2, 2, 0, 0)
+      default: return hipErrorInvalidValue;
+    }
+  }
+  switch (key) {
+    GANSU_SLAB(0, 0, 0, 0) GANSU_SLAB(1, 0, 0, 0) GANSU_SLAB(1, 0, 1, 0) GANSU_SLAB(1, 1, 0, 0)
+    GANSU_SLAB(1, 1, 1, 0) GANSU_SLAB(1, 1, 1, 1) GANSU_SLAB(2, 0, 0, 0) GANSU_SLAB(2, 0, 1, 0)
+    GANSU_SLAB(2, 0, 1, 1) GANSU_SLAB(2, 0, 2, 0) GANSU_SLAB(2, 1, 0, 0) GANSU_SLAB(2, 1, 1, 0)
+    GANSU_SLAB(2, 1, 1, 1) GANSU_SLAB(2, 1, 2, 0) GANSU_SLAB(2, 1, 2, 1) GANSU_SLAB(2, 2, 0, 0)
+    GANSU_SLAB(2, 2, 1, 0) GANSU_SLAB(2, 2, 1, 1) GANSU_SLAB(2, 2, 2, 0) GANSU_SLAB(2, 2, 2, 1)
+    GANSU_SLAB(2, 2, 2, 2)
+    default: return hipErrorInvalidValue;
+  }
+#undef GANSU_SLAB
 }
 
 hipError_t launch_scale_density(const double* D, const double* w, int M, double* Dp, hipStream_t s) {

@@ -30,6 +30,21 @@ struct BlockRec {
 
 enum EriMode { kModeDirect = 0, kModeStore = 1, kModeStoredDigest = 2 };
 
+// One bra pair with all of its kets (positions 0..count-1 of the ket class
+// list); its stored blocks start at first_block (64 kets per block).
+struct BraRec {
+  int bra_pos, first_block, count, pad;
+};
+
+struct SlabLaunch {
+  int LA, LB, LC, LD;
+  const BraRec* bras;
+  int nbras;
+  const int* bra_list;
+  const int* ket_list;
+  const double* store;
+};
+
 struct ClassLaunch {
   int LA, LB, LC, LD;
   const BlockRec* blocks;
@@ -56,6 +71,13 @@ hipError_t launch_schwarz_ref(const void* prims, const int* pair_a, const int* p
 hipError_t launch_pair_data(const void* prims, const int* pa, const int* pb, const double* Qg, size_t n, PairRec* out,
                             hipStream_t s);
 hipError_t launch_eri_class(const ClassLaunch& c, const EriArgs& a, int mode, hipStream_t s);
+// Slab digestion: one workgroup per bra pair, K rows of the bra shells
+// accumulated in LDS.  Returns hipErrorInvalidValue when the class/mode has no
+// slab variant or its LDS footprint exceeds the device limit (callers then
+// use launch_eri_class).
+hipError_t launch_fock_slab(const SlabLaunch& c, const EriArgs& a, int mode, hipStream_t s);
+size_t slab_lds_bytes(int LA, int LB, int M);
+bool slab_available(int LA, int LB, int LC, int LD, int mode);
 hipError_t launch_scale_density(const double* D, const double* w, int M, double* Dp, hipStream_t s);
 hipError_t launch_zero(double* p, size_t n, hipStream_t s);
 // J = w w sym(Jr), K = w w sym(Kr); if F: F = H + J - K/2 (J, K may be null then)

@@ -83,8 +83,9 @@ double dfact(int n) { double r = 1; for (int k = n; k > 1; k -= 2) r *= k; retur
 //   F_n(x) = exp(-x) * sum_k (2x)^k / ((2n+1)(2n+3)...(2n+2k+1))
 // (all terms positive; no cancellation), valid for every x >= 0 used here.
 double boys_series(int n, double x) {
-  if (x > 60.0) {
-    // asymptote: (2n-1)!!/2^(n+1) sqrt(pi/x^(2n+1)); relative error < 1e-25
+  if (x > 200.0) {
+    // asymptote (2n-1)!!/2^(n+1) sqrt(pi/x^(2n+1)); its relative error is the
+    // Poisson tail e^-x x^k/k! summed over k <= n, below 1e-50 here for n <= 32
     return dfact(2 * n - 1) / std::pow(2.0, n + 1) * std::sqrt(M_PI / std::pow(x, 2 * n + 1));
   }
   double term = 1.0 / (2 * n + 1), sum = term;

@@ -1,0 +1,186 @@
+"""CPU tests of the product's host side: input path, parameters, geometry
+generators, error codes and the C ABI surface (no GPU calls)."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from dft_gansu_amd import GansuError, Molecule, Parameters, basis_path, lib
+from dft_gansu_amd import molecule as gm
+from dft_gansu_amd._lib import EXPORTS, LIB_PATH
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "gansu_fock.h")
+
+# Cartesian sizes per workload (M, primitive shells, aux functions)
+SIZES = {0: (7, 15, None), 1: (260, 330, None), 2: (420, 588, 1764), 3: (194, 238, None), 4: (1600, 2048, 6144)}
+
+
+def test_library_exports_every_declared_symbol():
+    decl = set(re.findall(r"\b(gansu_[a-z_0-9]+)\s*\(", open(HEADER).read()))
+    assert decl, "no declarations parsed"
+    L = ctypes.CDLL(LIB_PATH)
+    missing = [s for s in decl if not hasattr(L, s)]
+    assert not missing, missing
+    assert decl == set(EXPORTS), decl ^ set(EXPORTS)
+
+
+@pytest.mark.parametrize("cfg", [0, 1, 2, 3, 4])
+def test_workload_sizes(cfg):
+    mol, aux = gm.config_molecule(cfg)
+    M, np_, Ma = SIZES[cfg]
+    assert (mol.n_basis, mol.n_prims) == (M, np_)
+    assert (aux.n_basis if aux else None) == Ma
+
+
+@pytest.mark.parametrize("cfg", [0, 1, 2, 3])
+@pytest.mark.parametrize("normalization", ["reference", "unit"])
+def test_molecule_bit_exact_vs_oracle(cfg, normalization):
+    c = gm.CONFIGS[cfg]
+    Z, xyz = c["geom"]()
+    mol = Molecule.from_atoms(Z, xyz, basis_path(c["basis"]), normalization=normalization)
+    p, t, n = O.build_molecule(basis_path(c["basis"]), Z, xyz, quirk=(normalization == "reference"))
+    for f in ("exponent", "coefficient", "x", "y", "z", "shell_type", "basis_index", "atom_index"):
+        assert np.array_equal(mol.prims[f], p[f]), f
+    assert np.array_equal(mol.types["count"], t["count"])
+    assert np.array_equal(mol.types["start_index"], t["start_index"])
+    assert np.array_equal(mol.norm, n)
+
+
+def test_quirk_only_changes_odd_l_norms():
+    Z, xyz = gm.water()
+    a = Molecule.from_atoms(Z, xyz, basis_path("cc-pvdz"), normalization="reference")
+    b = Molecule.from_atoms(Z, xyz, basis_path("cc-pvdz"), normalization="unit")
+    # basis functions of p shells differ, s and d agree
+    lof = np.zeros(a.n_basis, dtype=int)
+    for pr in a.prims:
+        L = int(pr["shell_type"])
+        lof[int(pr["basis_index"]):int(pr["basis_index"]) + (L + 1) * (L + 2) // 2] = L
+    assert np.array_equal(a.norm[lof % 2 == 0], b.norm[lof % 2 == 0])
+    assert np.all(a.norm[lof == 1] != b.norm[lof == 1])
+
+
+def test_xyz_reader_and_errors(tmp_path):
+    f = tmp_path / "w.xyz"
+    f.write_text("3\nwater\nO 0.0 0.0 0.11779\nH 0.0 0.75545 -0.47116\n\nH 0.0 -0.75545 -0.47116\n")
+    m = Molecule.from_xyz(str(f), basis_path("sto-3g"))
+    assert m.n_atoms == 3 and m.n_basis == 7 and m.n_electrons == 10
+    np.testing.assert_allclose(m.xyz[1], np.array([0.0, 0.75545, -0.47116]) * 1.8897259886, rtol=0, atol=1e-15)
+    bad = tmp_path / "bad.xyz"
+    bad.write_text("2\nx\nH 0 0 0\nH 0 0 0.00001\n")
+    with pytest.raises(GansuError, match="overlapping"):
+        Molecule.from_xyz(str(bad), basis_path("sto-3g"))
+    bad.write_text("3\nx\nH 0 0 0\nH 0 0 1\n")
+    with pytest.raises(GansuError, match="count mismatch"):
+        Molecule.from_xyz(str(bad), basis_path("sto-3g"))
+    bad.write_text("1\nx\nH 0 0\n")
+    with pytest.raises(GansuError, match="Invalid line"):
+        Molecule.from_xyz(str(bad), basis_path("sto-3g"))
+    with pytest.raises(GansuError, match="GANSU_E_IO"):
+        Molecule.from_xyz(str(tmp_path / "missing.xyz"), basis_path("sto-3g"))
+
+
+def test_basis_and_electron_errors(tmp_path):
+    with pytest.raises(GansuError, match="GANSU_E_BASIS"):
+        Molecule.from_atoms([54], [[0, 0, 0]], basis_path("cc-pvdz"))  # Xe not in cc-pVDZ file
+    with pytest.raises(GansuError, match="GANSU_E_ELECTRONS"):
+        Molecule.from_atoms([1], [[0, 0, 0]], basis_path("sto-3g"), charge=1)
+    g = tmp_path / "bad.gbs"
+    g.write_text("!\n\nH     0\nSPD   1   1.00\n  1.0  1.0 1.0 1.0\n****\n")
+    with pytest.raises(GansuError, match="Invalid basis function name"):
+        Molecule.from_atoms([1], [[0, 0, 0]], str(g))
+
+
+def test_parameters_defaults_recipe_and_rejection(tmp_path):
+    p = Parameters()
+    assert p["eri_method"] == "stored" and p["schwarz_screening_threshold"] == "1e-12"
+    assert p["diis_size"] == "8" and p["convergence_method"] == "diis" and p["maxiter"] == "100"
+    r = tmp_path / "r.txt"
+    r.write_text("# recipe\nxyzfilename = ../xyz/H2O.xyz\ngbsfilename = ../Basis/STO-3G.gbs\nmethod = RHF\r\n"
+                 "convergence_method = DIIS\ndiis_size = 6  # six\ndiis_include_transform = false\n\n")
+    p = Parameters(recipe=str(r))
+    assert p["method"] == "rhf" and p["convergence_method"] == "diis" and p["diis_size"] == "6"
+    assert p["gbsfilename"] == "../Basis/STO-3G.gbs"  # paths keep their case
+    assert p["diis_include_transform"] == "0"
+    with pytest.raises(GansuError, match="Unknown parameter"):
+        p["no_such_key"] = 1
+    r.write_text("novalue\n")
+    with pytest.raises(GansuError, match="Invalid line"):
+        Parameters(recipe=str(r))
+
+
+def test_engine_option_errors():
+    from dft_gansu_amd import FockEngine
+    mol = Molecule.from_atoms(*gm.water(), basis_path("sto-3g"))
+    with pytest.raises(GansuError, match="eri_method"):
+        FockEngine(mol, eri_method="hash")
+    with pytest.raises(GansuError, match="precision"):
+        FockEngine(mol, precision="fp16")
+    with pytest.raises(GansuError, match="auxiliary"):
+        FockEngine(mol, eri_method="ri")
+    f = mol.prims.copy()
+    f["shell_type"][-1] = 3  # an f primitive: beyond the compiled orbital classes
+    mol.prims = f
+    with pytest.raises(GansuError):
+        FockEngine(mol)
+
+
+# ---------------------------------------------------------- geometries ----
+def _dist(xyz, i, j):
+    return np.linalg.norm(xyz[i] - xyz[j]) / gm.ANGSTROM_TO_BOHR
+
+
+def test_water_geometry():
+    Z, xyz = gm.water(jitter=0.0)
+    assert list(Z) == [8, 1, 1]
+    assert _dist(xyz, 0, 1) == pytest.approx(0.9572) and _dist(xyz, 0, 2) == pytest.approx(0.9572)
+    v1, v2 = xyz[1] - xyz[0], xyz[2] - xyz[0]
+    ang = np.degrees(np.arccos(v1 @ v2 / np.linalg.norm(v1) / np.linalg.norm(v2)))
+    assert ang == pytest.approx(104.52)
+    Zj, xj = gm.water()
+    assert np.abs(xj - xyz).max() / gm.ANGSTROM_TO_BOHR <= 0.01 + 1e-12
+    assert np.array_equal(xj, gm.water()[1])  # seeded
+
+
+def test_alkane_geometry():
+    Z, xyz = gm.alkane(10, jitter=0.0)
+    assert list(Z).count(6) == 10 and list(Z).count(1) == 22
+    C = xyz[:10]
+    for i in range(9):
+        assert _dist(C, i, i + 1) == pytest.approx(1.54)
+    for h in range(10, 32):
+        d = min(_dist(xyz, h, c) for c in range(10))
+        assert d == pytest.approx(1.09)
+    # tetrahedral angles at every carbon
+    for i in range(1, 9):
+        a, b = C[i - 1] - C[i], C[i + 1] - C[i]
+        assert np.degrees(np.arccos(a @ b / np.linalg.norm(a) / np.linalg.norm(b))) == pytest.approx(109.4712, abs=1e-3)
+    # no two atoms closer than 1.0 A
+    D = np.linalg.norm(xyz[:, None] - xyz[None], axis=-1) / gm.ANGSTROM_TO_BOHR
+    assert (D + 10 * np.eye(len(Z))).min() > 1.0
+
+
+def test_coronene_geometry():
+    Z, xyz = gm.coronene()
+    assert list(Z).count(6) == 24 and list(Z).count(1) == 12
+    assert np.allclose(xyz[:, 2], 0.0)
+    C = xyz[:24]
+    D = np.linalg.norm(C[:, None] - C[None], axis=-1) / gm.ANGSTROM_TO_BOHR
+    nb = ((D > 1.0) & (D < 1.5)).sum(1)
+    assert sorted(set(nb)) == [2, 3] and (nb == 2).sum() == 12
+    assert np.allclose(D[(D > 1.0) & (D < 1.5)], 1.42)
+
+
+def test_water_box_geometry():
+    Z, xyz = gm.water_box()
+    assert len(Z) == 192 and list(Z).count(8) == 64
+    O_ = xyz[Z == 8] / gm.ANGSTROM_TO_BOHR
+    D = np.linalg.norm(O_[:, None] - O_[None], axis=-1) + 100 * np.eye(64)
+    assert D.min() >= 2.7
+    assert O_.min() >= 0.0 and O_.max() <= 12.4
+    for k in range(64):
+        o, h1, h2 = xyz[3 * k: 3 * k + 3] / gm.ANGSTROM_TO_BOHR
+        assert np.linalg.norm(h1 - o) == pytest.approx(0.9572) and np.linalg.norm(h2 - o) == pytest.approx(0.9572)

@@ -1,0 +1,257 @@
+"""Pins the FP64 host oracle (CPU only).
+
+Every oracle function is checked against something that does not share its
+code: the reference's own Boys grid (golden fixture), closed forms, an
+independent Obara-Saika recursion, textbook energies, symmetry/invariance
+properties and the reference's unit-test known answers.
+"""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+from scipy import special
+
+import oracle_lib as O
+from helpers import both, random_density
+from dft_gansu_amd import molecule as gm
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+# ------------------------------------------------------------------ Boys ----
+def dfact(n):
+    r = 1.0
+    while n > 1:
+        r *= n
+        n -= 2
+    return r
+
+
+def boys_closed(n, x):
+    """F_n(x) = 1F1(n+1/2; n+3/2; -x)/(2n+1) = Gamma(n+1/2) P(n+1/2, x) / (2 x^(n+1/2))."""
+    if x < 30.0:  # hyp1f1 is the more accurate scipy routine below ~30
+        return special.hyp1f1(n + 0.5, n + 1.5, -x) / (2 * n + 1)
+    return special.gamma(n + 0.5) * special.gammainc(n + 0.5, x) / (2.0 * x ** (n + 0.5))
+
+
+@pytest.mark.parametrize("x", [0.0, 1e-8, 0.03, 0.5, 1.7, 5.0, 12.3, 25.0, 31.9, 40.0, 80.0, 150.0])
+def test_boys_exact_matches_closed_form(x):
+    F = O.boys(24, x)
+    for n in range(25):
+        ref = boys_closed(n, x)
+        assert abs(F[n] - ref) <= 2e-15 * max(ref, 1e-300) + 1e-300, (n, x, F[n], ref)
+
+
+def test_boys_zero_and_asymptote():
+    F = O.boys(20, 0.0)
+    np.testing.assert_allclose(F, [1.0 / (2 * n + 1) for n in range(21)], rtol=0, atol=1e-16)
+    x = 200.0
+    F = O.boys(10, x)
+    for n in range(11):
+        assert F[n] == pytest.approx(dfact(2 * n - 1) / 2 ** (n + 1) * math.sqrt(math.pi / x ** (2 * n + 1)),
+                                     rel=1e-14)
+
+
+def test_boys_table_matches_reference_grid():
+    """The oracle's F_n(i/32) grid, rounded to 15 decimals, against the
+    reference's boys_30720.h (fixture: digest + 640 sampled entries)."""
+    fx = json.load(open(os.path.join(GOLDEN, "boys_ref_samples.json")))
+    s = np.array(fx["samples"])
+    idx = (s[:, 0] * 1024 + s[:, 1]).astype(int)
+    t15 = O.boys_table(round15=True)
+    exact = O.boys_table(round15=False)
+    assert np.abs(t15[idx] - s[:, 2]).max() <= 1.01e-15
+    assert np.mean(t15[idx] == s[:, 2]) > 0.99
+    assert np.abs(exact[idx] - s[:, 2]).max() <= 1e-15
+
+
+@pytest.mark.parametrize("x", [1e-6, 0.004, 0.2, 0.9, 3.3, 9.99, 18.0, 27.0, 29.5, 35.0])
+def test_boys_reference_style_close_to_exact(x):
+    tab = O.boys_table(round15=True)
+    Fr = O.boys(16, x, reference_style=True, table=tab)
+    Fe = O.boys(16, x)
+    np.testing.assert_allclose(Fr, Fe, rtol=5e-12, atol=1e-16)
+
+
+# --------------------------------------------------------------- ERIs ------
+def test_ssss_closed_form():
+    rng = np.random.default_rng(0)
+    for _ in range(20):
+        ctr = rng.uniform(-2, 2, (4, 3))
+        ex = rng.uniform(0.1, 5.0, 4)
+        p, q = ex[0] + ex[1], ex[2] + ex[3]
+        P = (ex[0] * ctr[0] + ex[1] * ctr[1]) / p
+        Q = (ex[2] * ctr[2] + ex[3] * ctr[3]) / q
+        Kab = math.exp(-ex[0] * ex[1] / p * np.sum((ctr[0] - ctr[1]) ** 2))
+        Kcd = math.exp(-ex[2] * ex[3] / q * np.sum((ctr[2] - ctr[3]) ** 2))
+        T = p * q / (p + q) * np.sum((P - Q) ** 2)
+        F0 = 0.5 * math.sqrt(math.pi / T) * math.erf(math.sqrt(T)) if T > 0 else 1.0
+        ref = 2 * math.pi ** 2.5 / (p * q * math.sqrt(p + q)) * Kab * Kcd * F0
+        assert O.eri_prim_md(ctr, ex, [0] * 12) == pytest.approx(ref, rel=1e-13)
+
+
+def _components(L):
+    return [(i, j, L - i - j) for i in range(L, -1, -1) for j in range(L - i, -1, -1)]
+
+
+@pytest.mark.parametrize("ls", [(1, 0, 0, 0), (1, 1, 0, 0), (1, 0, 1, 0), (1, 1, 1, 1), (2, 0, 0, 0), (2, 1, 1, 0),
+                                (2, 2, 0, 0), (2, 0, 2, 0), (2, 1, 2, 1), (2, 2, 2, 2), (3, 0, 2, 0), (3, 2, 0, 0)])
+def test_md_matches_obara_saika(ls):
+    """Two unrelated recursions (McMurchie-Davidson vs Obara-Saika/HGP)."""
+    rng = np.random.default_rng(sum(ls) * 7 + ls[0])
+    ctr = rng.uniform(-1.5, 1.5, (4, 3))
+    ex = rng.uniform(0.2, 3.0, 4)
+    comps = [_components(l) for l in ls]
+    picks = rng.integers(0, 10 ** 6, size=(24, 4))
+    for pk in picks:
+        ang = []
+        for k in range(4):
+            ang.extend(comps[k][pk[k] % len(comps[k])])
+        md = O.eri_prim_md(ctr, ex, ang)
+        os_ = O.eri_prim_os(ctr, ex, ang)
+        assert md == pytest.approx(os_, rel=1e-11, abs=1e-14), ang
+
+
+@pytest.fixture(scope="module")
+def water_ccpvdz():
+    (Z, xyz) = gm.water()
+    mol, (p, t, n) = both(Z, xyz, "cc-pvdz", normalization="unit")
+    Q = O.schwarz(p, t)
+    eri = O.eri_dense(p, t, n, Q, 0.0)
+    return Z, xyz, p, t, n, eri
+
+
+def test_eri_eightfold_symmetry(water_ccpvdz):
+    eri = water_ccpvdz[-1]
+    for perm in [(1, 0, 2, 3), (0, 1, 3, 2), (2, 3, 0, 1), (3, 2, 1, 0)]:
+        np.testing.assert_allclose(eri, eri.transpose(perm), rtol=0, atol=1e-13)
+
+
+def test_eri_schwarz_inequality_and_psd(water_ccpvdz):
+    eri = water_ccpvdz[-1]
+    M = eri.shape[0]
+    G = eri.reshape(M * M, M * M)
+    d = np.sqrt(np.clip(np.diag(G), 0, None))
+    assert np.all(np.abs(G) <= np.outer(d, d) * (1 + 1e-12) + 1e-14)
+    w = np.linalg.eigvalsh(G)
+    assert w.min() > -1e-10 * w.max()
+
+
+def test_overlap_unit_normalisation(water_ccpvdz):
+    Z, xyz, p, t, n, _ = water_ccpvdz
+    S, T, V = O.oneint(p, n, Z, xyz)
+    np.testing.assert_allclose(np.diag(S), 1.0, atol=1e-12)
+    assert np.all(np.linalg.eigvalsh(S) > 0)
+
+
+def test_overlap_ss_closed_form():
+    """Two normalised s Gaussians: S = (2 sqrt(ab)/(a+b))^(3/2) exp(-ab/(a+b) R^2)."""
+    import tempfile
+    a, b, R = 0.8, 1.9, 1.3
+    gbs = "!\n\nH     0\nS    1   1.00\n      %.10f       1.0\n****\nHe     0\nS    1   1.00\n      %.10f       1.0\n****\n" % (a, b)
+    with tempfile.NamedTemporaryFile("w", suffix=".gbs", delete=False) as f:
+        f.write(gbs)
+        path = f.name
+    try:
+        p, t, n = O.build_molecule(path, [1, 2], [[0, 0, 0], [0, 0, R]], quirk=False)
+        S, _, _ = O.oneint(p, n, [1, 2], [[0, 0, 0], [0, 0, R]])
+        ref = (2 * math.sqrt(a * b) / (a + b)) ** 1.5 * math.exp(-a * b / (a + b) * R * R)
+        assert S[0, 1] == pytest.approx(ref, rel=1e-13)
+    finally:
+        os.unlink(path)
+
+
+# ---------------------------------------------------------------- SCF ------
+def test_h2_sto3g_textbook_energy():
+    """Szabo & Ostlund, H2/STO-3G at R = 1.4 bohr: E = -1.1167 Eh."""
+    p, t, n = O.build_molecule(gm.basis_path("sto-3g"), [1, 1], [[0, 0, 0], [0, 0, 1.4]])
+    e_elec, e_tot, D, F, it = O.rhf(p, t, n, [1, 1], [[0, 0, 0], [0, 0, 1.4]], 1)
+    assert e_tot == pytest.approx(-1.1167, abs=1e-4)
+    assert e_elec == pytest.approx(-1.8310, abs=1e-4)
+
+
+def test_h2o_sto3g_textbook_energy():
+    """Szabo & Ostlund table 3.13, H2O/STO-3G at R = 1.809 bohr, 104.52 deg: -74.963 Eh."""
+    th = math.radians(104.52)
+    R = 1.809
+    xyz = [[0, 0, 0], [R, 0, 0], [R * math.cos(th), R * math.sin(th), 0]]
+    p, t, n = O.build_molecule(gm.basis_path("sto-3g"), [8, 1, 1], xyz)
+    e = O.rhf(p, t, n, [8, 1, 1], xyz, 5, conv=1e-10)[1]
+    assert e == pytest.approx(-74.963, abs=1e-3)
+
+
+def test_energy_invariant_to_normalisation_and_rotation():
+    Z, xyz = gm.water(jitter=0.0)
+    e = []
+    for q in (True, False):
+        p, t, n = O.build_molecule(gm.basis_path("cc-pvdz"), Z, xyz, quirk=q)
+        e.append(O.rhf(p, t, n, Z, xyz, 5, conv=1e-11)[1])
+    R = gm._random_rotation(np.random.default_rng(3))
+    xr = (R @ xyz.T).T + np.array([0.3, -1.1, 2.0])
+    p, t, n = O.build_molecule(gm.basis_path("cc-pvdz"), Z, xr, quirk=True)
+    e.append(O.rhf(p, t, n, Z, xr, 5, conv=1e-11)[1])
+    assert abs(e[0] - e[1]) < 1e-9
+    assert abs(e[0] - e[2]) < 1e-8
+    assert -76.1 < e[0] < -75.9
+
+
+def test_jk_direct_equals_dense(water_ccpvdz):
+    Z, xyz, p, t, n, eri = water_ccpvdz
+    D = random_density(len(n))
+    J, K = O.jk(p, t, n, O.schwarz(p, t), D, 0.0)
+    np.testing.assert_allclose(J, np.einsum("ijkl,kl->ij", eri, D), atol=1e-11)
+    np.testing.assert_allclose(K, np.einsum("ikjl,kl->ij", eri, D), atol=1e-11)
+
+
+# ----------------------------------------------- reference known answers ----
+def test_reference_cholesky_known_answer():
+    """test/test_gpu_manager.cu:332-343."""
+    A = np.array([[25, 15, -5], [15, 18, 0], [-5, 0, 11]], dtype=float)
+    np.testing.assert_allclose(O.cholesky(A), [[5, 0, 0], [3, 3, 0], [-1, 1, 3]], atol=1e-12)
+
+
+def test_reference_eigen_known_answer():
+    """test/test_gpu_manager.cu:32-36."""
+    w, V = O.eigh(np.array([[4.0, 1.0], [1.0, 3.0]]))
+    np.testing.assert_allclose(w, [2.381966, 4.618034], atol=1e-6)
+    A = random_density(30, seed=9) - 0.3
+    w, V = O.eigh(A)
+    np.testing.assert_allclose(w, np.linalg.eigvalsh(A), atol=1e-12)
+    np.testing.assert_allclose(V @ np.diag(w) @ V.T, A, atol=1e-12)
+
+
+def test_reference_optimal_damping_known_answer():
+    """test/test_gpu_manager.cu:468-492: alpha = 0.613207547."""
+    F = np.array([[2.0, 2.0, 1.0], [2.0, 3.0, 1.0], [1.0, 1.0, 2.0]])
+    Fp = np.array([[1.0, 2.0, 3.0], [4.0, 6.0, 6.0], [7.0, 8.0, 10.0]])
+    D = np.full((3, 3), 3.0)
+    Dp = np.array([[2.0, 4.0, 6.0], [4.0, 1.0, 1.0], [6.0, 1.0, 10.0]])
+    assert O.optimal_damping(F, Fp, D, Dp) == pytest.approx(0.613207547, abs=1e-8)
+
+
+# ----------------------------------------------------------------- RI ------
+def test_ri_B_reproduces_fitted_integrals():
+    """sum_P B^P_mn B^P_ls == (mn|P) (P|Q)^-1 (Q|ls), and RI-J/K approach exact J/K."""
+    Z, xyz = gm.water(jitter=0.0)
+    mol, (p, t, n) = both(Z, xyz, "cc-pvdz")
+    aux, (pa, ta, na) = both(Z, xyz, "cc-pvdz-rifit")
+    Q = O.schwarz(p, t)
+    Qa = O.schwarz_aux(pa)
+    V = O.int2c2e(pa, na, Qa, 0.0)
+    T = O.int3c2e(p, t, n, Q, pa, na, Qa, 0.0)
+    np.testing.assert_allclose(V, V.T, atol=1e-12)
+    np.testing.assert_allclose(T, T.transpose(0, 2, 1), atol=1e-12)
+    L = O.cholesky(V)
+    B = O.ri_B(L, T)
+    M, Ma = len(n), len(na)
+    lhs = B.reshape(Ma, -1).T @ B.reshape(Ma, -1)
+    Tm = T.reshape(Ma, -1)
+    rhs = Tm.T @ np.linalg.solve(V, Tm)
+    assert np.abs(lhs - rhs).max() < 1e-9 * np.abs(rhs).max()
+    D = random_density(M, seed=4)
+    Jri, Kri = O.ri_jk(B, D)
+    J, K = O.jk(p, t, n, Q, D, 0.0)
+    assert np.abs(Jri - J).max() < 2e-3 * np.abs(J).max()
+    assert np.abs(Kri - K).max() < 2e-2 * np.abs(K).max()
