@@ -150,13 +150,14 @@ void FockEngine::precompute() {
   d_Dp_.alloc((size_t)M_ * M_);
   d_Jr_.alloc((size_t)M_ * M_);
   d_Kr_.alloc((size_t)M_ * M_);
-  precompute_four_center();
+  precompute_pairs();
   if (method_ == "ri") precompute_ri();
+  else precompute_quartets();
   GANSU_HIP(hipDeviceSynchronize());
   precomputed_ = true;
 }
 
-void FockEngine::precompute_four_center() {
+void FockEngine::precompute_pairs() {
   d_prims_.upload(prims_);
   const int nt = (int)types_.size();
   // reference primitive pair list (gpu_manager.cu:413-432 order)
@@ -233,7 +234,9 @@ void FockEngine::precompute_four_center() {
     GANSU_HIP(hipDeviceSynchronize());
   }
   d_sp_.upload(sp_h_);
+}
 
+void FockEngine::precompute_quartets() {
   // pair classes sorted by qmax (descending; ties by id)
   const int npc = (kMaxL + 1) * (kMaxL + 2) / 2;
   pclass_.clear();
@@ -348,6 +351,7 @@ void FockEngine::stats(double* out) const {
   out[2] = stored_bytes_;
   const double P = double(M_) * (M_ + 1) / 2;
   out[4] = P * (P + 1) / 2;
+  if (method_ == "ri") out[5] = double(M_) * Maux_ * M_ * sizeof(double);
 }
 
 }  // namespace gansu

@@ -88,7 +88,8 @@ class FockEngine {
   std::vector<double> last_profile();  // 5 values per launch
 
  private:
-  void precompute_four_center();
+  void precompute_pairs();     // Schwarz factors, shell pairs, primitive-pair records
+  void precompute_quartets();  // quartet classes, work blocks, stored integrals
   void precompute_ri();
   void build_four_center(const double* d_D, hipStream_t s);
   void build_ri(const double* d_D, double* d_J, double* d_K, const double* d_H, double* d_F, hipStream_t s);

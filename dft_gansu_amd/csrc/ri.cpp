@@ -1,15 +1,190 @@
 // RI (density fitting) path of the Fock engine.
+//
+// Precompute (ERI_RI::precomputation, eri.cu:134-236, and
+// compute_RI_IntermediateMatrixB, gpu_manager.cu:1448-1526):
+//   aux Schwarz factors -> (P|Q) -> Cholesky (P|Q) = L L^T (rocSOLVER dpotrf,
+//   gpu_manager.cu:1311-1349) -> (mn|P) -> B = L^-1 (P|mn) (rocBLAS dtrsm).
+// B is kept as B[m][P][n]: for each m an Maux x M slab.  On that layout
+//   J = sum_P (B^P . D) B^P            two HBM-bound passes (ri_kernels.hip)
+//   K = sum_P B^P D B^P^T              T[m][P][l] = B[m][P][:] D  (batched DGEMM)
+//                                      K = T_cat B_cat^T          (one DGEMM, k = Maux M)
+// which is the reference's D-form K (gpu_manager.cu:1589-1618: 2 Maux small
+// DGEMMs per build) as two large GEMMs, chunked over P to bound the workspace.
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
 #include "engine.hpp"
+#include "ri_kernels.hpp"
 
 namespace gansu {
 
-struct FockEngine::RIState {};
+namespace {
+void blas_ok(rocblas_status st, const char* what) {
+  if (st != rocblas_status_success) throw Error(GANSU_E_HIP, std::string(what) + ": rocblas status " + std::to_string(st));
+}
+
+double dfact(int n) {
+  double r = 1.0;
+  for (int k = n; k > 1; k -= 2) r *= k;
+  return r;
+}
+
+// cgto norm x 1/sqrt((2l-1)!!(2m-1)!!(2n-1)!!) in the RI component order
+double aux_component_factor(int L, int c) {
+  static const int d2[6][3] = {{2, 0, 0}, {0, 2, 0}, {0, 0, 2}, {1, 1, 0}, {1, 0, 1}, {0, 1, 1}};
+  static const int f3r[10][3] = {{3, 0, 0}, {0, 3, 0}, {0, 0, 3}, {2, 1, 0}, {2, 0, 1},
+                                 {0, 2, 1}, {1, 2, 0}, {0, 1, 2}, {1, 0, 2}, {1, 1, 1}};
+  int e[3];
+  for (int d = 0; d < 3; ++d) {
+    if (L == 0) e[d] = 0;
+    else if (L == 1) e[d] = (c == d);
+    else if (L == 2) e[d] = d2[c][d];
+    else e[d] = f3r[c][d];
+  }
+  return 1.0 / std::sqrt(dfact(2 * e[0] - 1) * dfact(2 * e[1] - 1) * dfact(2 * e[2] - 1));
+}
+}  // namespace
+
+struct FockEngine::RIState {
+  rocblas_handle h = nullptr;
+  std::vector<AuxShell> shells_h;
+  DevBuf<AuxShell> shells;
+  DevBuf<AuxPrim> prims;
+  DevBuf<double> waux, L, B, W, J, K, T;
+  int pchunk = 0;
+  ~RIState() {
+    if (h) rocblas_destroy_handle(h);
+  }
+};
+
 void ri_destroy(FockEngine::RIState* p) { delete p; }
 
-void FockEngine::precompute_ri() { throw Error(GANSU_E_UNSUPPORTED, "RI path not available in this build"); }
+void FockEngine::precompute_ri() {
+  ri_destroy(ri_);
+  ri_ = new RIState();
+  RIState& r = *ri_;
+  blas_ok(rocblas_create_handle(&r.h), "rocblas_create_handle");
+  const int M = M_, Ma = Maux_;
+  // auxiliary shells and primitive Schwarz factors (schwarz.txt:479+)
+  const auto ash = contracted_shells(aux_prims_.data(), aux_prims_.size(), kMaxLAux);
+  DevBuf<gansu_primitive_shell> dprims;
+  dprims.upload(aux_prims_);
+  std::vector<int> pa(aux_prims_.size()), pb(aux_prims_.size(), -1);
+  for (size_t i = 0; i < pa.size(); ++i) pa[i] = (int)i;
+  std::vector<double> qaux;
+  {
+    DevBuf<int> dpa, dpb;
+    DevBuf<double> dq(aux_prims_.size());
+    dpa.upload(pa);
+    dpb.upload(pb);
+    GANSU_HIP(launch_schwarz_ref(dprims.ptr, dpa.ptr, dpb.ptr, pa.size(), boys_.ptr, 1, dq.ptr, 0));
+    GANSU_HIP(hipDeviceSynchronize());
+    qaux = dq.download();
+  }
+  std::vector<AuxPrim> aprims;
+  std::vector<double> waux(Ma, 0.0);
+  for (const auto& s : ash) {
+    AuxShell as{};
+    as.L = s.L;
+    as.basis = s.basis;
+    as.prim_begin = (int)aprims.size();
+    as.nprim = (int)s.prims.size();
+    double qmax = 0.0;
+    for (int ip : s.prims) {
+      const auto& p = aux_prims_[ip];
+      AuxPrim ap{};
+      ap.alpha = p.exponent;
+      ap.C[0] = p.x; ap.C[1] = p.y; ap.C[2] = p.z;
+      ap.Kc = p.coefficient * std::pow(2.0 / M_PI, 0.75) * std::pow(2.0, s.L) *
+              std::pow(p.exponent, (2.0 * s.L + 3.0) / 4.0);
+      ap.Q = qaux[ip];
+      qmax = std::max(qmax, ap.Q);
+      aprims.push_back(ap);
+    }
+    as.qmax = qmax;
+    r.shells_h.push_back(as);
+    if (s.basis + ncart(s.L) > Ma) throw Error(GANSU_E_ARG, "aux basis_index out of range");
+    for (int c = 0; c < ncart(s.L); ++c) waux[s.basis + c] = aux_norm_[s.basis + c] * aux_component_factor(s.L, c);
+  }
+  r.shells.upload(r.shells_h);
+  r.prims.upload(aprims);
+  r.waux.upload(waux);
 
-void FockEngine::build_ri(const double*, double*, double*, const double*, double*, hipStream_t) {
-  throw Error(GANSU_E_UNSUPPORTED, "RI path not available in this build");
+  // (P|Q) and its Cholesky factor.  Row-major L (lower) is the column-major
+  // upper factor of dpotrf (the reference's convention, gpu_manager.cu:1311-1349).
+  r.L.alloc((size_t)Ma * Ma);
+  GANSU_HIP(launch_zero(r.L.ptr, r.L.n, 0));
+  const size_t nsh = r.shells_h.size();
+  RI2cArgs a2{r.shells.ptr, r.prims.ptr, nsh * (nsh + 1) / 2, r.waux.ptr, Ma, thr_, boys_.ptr, r.L.ptr};
+  GANSU_HIP(launch_int2c2e(a2, 0));
+  DevBuf<int> info(1);
+  blas_ok(rocsolver_dpotrf(r.h, rocblas_fill_upper, Ma, r.L.ptr, Ma, info.ptr), "rocsolver_dpotrf");
+  int hinfo = 0;
+  GANSU_HIP(hipMemcpy(&hinfo, info.ptr, sizeof(int), hipMemcpyDeviceToHost));
+  if (hinfo != 0) throw Error(GANSU_E_NUMERIC, "(P|Q) is not positive definite (potrf info " + std::to_string(hinfo) + ")");
+
+  // (mn|P) into B[m][P][n], then B_m = L^-1 T_m for every m
+  r.B.alloc((size_t)M * Ma * M);
+  GANSU_HIP(launch_zero(r.B.ptr, r.B.n, 0));
+  DevBuf<double> dw;
+  dw.upload(w_h_);
+  RI3cArgs a3{d_sp_.ptr, d_pr_.ptr, (int)sp_h_.size(), r.shells.ptr, r.prims.ptr, (int)nsh, dw.ptr, r.waux.ptr,
+              M, Ma, thr_, boys_.ptr, r.B.ptr};
+  GANSU_HIP(launch_int3c2e(a3, 0));
+  const double one = 1.0;
+  // column-major view of slab m is T_m^T (M x Maux, ld M): T_m^T <- T_m^T U^-1, U = L^T
+  blas_ok(rocblas_dtrsm_strided_batched(r.h, rocblas_side_right, rocblas_fill_upper, rocblas_operation_none,
+                                        rocblas_diagonal_non_unit, M, Ma, &one, r.L.ptr, Ma, 0, r.B.ptr, M,
+                                        (rocblas_stride)Ma * M, M),
+          "rocblas_dtrsm_strided_batched");
+  r.W.alloc(Ma);
+  r.J.alloc((size_t)M * M);
+  r.K.alloc((size_t)M * M);
+  // K workspace: a P-chunk of T[m][P][l], at most ~4 GB
+  const size_t budget = size_t(4) << 30;
+  r.pchunk = (int)std::max<size_t>(1, std::min<size_t>(Ma, budget / ((size_t)M * M * sizeof(double))));
+  r.T.alloc((size_t)M * r.pchunk * M);
+  GANSU_HIP(hipDeviceSynchronize());
+}
+
+void FockEngine::build_ri(const double* d_D, double* d_J, double* d_K, const double* d_H, double* d_F,
+                          hipStream_t s) {
+  RIState& r = *ri_;
+  const int M = M_, Ma = Maux_;
+  blas_ok(rocblas_set_stream(r.h, s), "rocblas_set_stream");
+  prof_clear();
+  double* J = d_J ? d_J : r.J.ptr;
+  double* K = d_K ? d_K : r.K.ptr;
+  const double bytesB = double(M) * Ma * M * sizeof(double);
+  prof_begin(s, 80001, bytesB, 2.0 * M * Ma * M, (double)Ma);
+  GANSU_HIP(launch_ri_w(r.B.ptr, d_D, M, Ma, r.W.ptr, s));
+  prof_end(s);
+  prof_begin(s, 80002, bytesB, 2.0 * M * Ma * M, (double)M);
+  GANSU_HIP(launch_ri_j(r.B.ptr, r.W.ptr, M, Ma, J, s));
+  prof_end(s);
+  const double one = 1.0, zero = 0.0;
+  for (int p0 = 0; p0 < Ma; p0 += r.pchunk) {
+    const int pc = std::min(r.pchunk, Ma - p0);
+    // T_m (pc x M) = B_m[p0:p0+pc] D  for every m; column-major: T_m^T = D B_m^T
+    prof_begin(s, 80003, 0.0, 2.0 * M * (double)pc * M * M, (double)pc);
+    blas_ok(rocblas_dgemm_strided_batched(r.h, rocblas_operation_none, rocblas_operation_none, M, pc, M, &one, d_D, M,
+                                          0, r.B.ptr + (size_t)p0 * M, M, (rocblas_stride)Ma * M, &zero, r.T.ptr, M,
+                                          (rocblas_stride)pc * M, M),
+            "rocblas_dgemm_strided_batched(T)");
+    prof_end(s);
+    // K += T_cat B_cat^T (row-major); column-major: K^T = B_cat T_cat^T
+    prof_begin(s, 80004, 0.0, 2.0 * M * (double)M * pc * M, (double)pc);
+    const double beta = p0 == 0 ? 0.0 : 1.0;
+    blas_ok(rocblas_dgemm(r.h, rocblas_operation_transpose, rocblas_operation_none, M, M, pc * M, &one,
+                          r.B.ptr + (size_t)p0 * M, Ma * M, r.T.ptr, pc * M, &beta, K, M),
+            "rocblas_dgemm(K)");
+    prof_end(s);
+  }
+  if (d_F) GANSU_HIP(launch_fock_ri(d_H, J, K, (size_t)M * M, d_F, s));
 }
 
 }  // namespace gansu

@@ -1,0 +1,208 @@
+// RI (density fitting) kernels for gfx950.
+//
+//   (P|Q)    two-centre Coulomb metric over auxiliary shell pairs
+//            (int2c2e.cu:468-687; screening Qaux_a Qaux_b < thr, int2c2e.cu:506)
+//   (mn|P)   three-centre integrals, written straight into the B layout
+//            B[m][P][n] (full m, n), screening Q_ab Qaux_c < thr (int3c2e.cu:1139)
+//   W_P  = sum_mn B[m][P][n] D[m][n]           (gpu_manager.cu:1573, cublasDgemv)
+//   J_mn = sum_P W_P B[m][P][n]                (weighted_sum_matrices_kernel)
+// Both J passes stream B once from HBM.  K is a pair of large GEMMs on the
+// same layout (ri.cpp).
+#include <hip/hip_runtime.h>
+
+#include "md_device.hpp"
+#include "ri_kernels.hpp"
+
+namespace gansu {
+
+using namespace dev;
+
+namespace {
+
+// Raw integral block between a bra (pair or single) and a ket single, both
+// expressed as Hermite pairs: out[(a*NB+b)*NC+c].
+__device__ void block_pair_single(int la, int lb, int lc, double p, const double* P, const double* PA,
+                                  const double* PB, double q, const double* Q, double pref, bool aux_a, bool aux_c,
+                                  const double* boys, double* out) {
+  const int nt = la + lb + 1;
+  double Eb[3][4 * 3 * 6];  // (la+1)(lb+1)(la+lb+1), la <= 3 (aux bra), lb <= 2
+  double Ek[3][4 * 1 * 4];
+  for (int d = 0; d < 3; ++d) {
+    hermite_e_rt<double>(la, lb, p, PA[d], PB[d], Eb[d]);
+    hermite_e_rt<double>(lc, 0, q, 0.0, 0.0, Ek[d]);
+  }
+  const int L = la + lb + lc;
+  double R[nherm(7)];
+  hermite_r_rt<double>(L, p * q / (p + q), P[0] - Q[0], P[1] - Q[1], P[2] - Q[2], pref, boys, R);
+  const int NA = ncart(la), NB = ncart(lb), NC = ncart(lc), ntk = lc + 1;
+  for (int c = 0; c < NC; ++c) {
+    int ce[3];
+    for (int d = 0; d < 3; ++d) ce[d] = cart_exp(lc, c, d, aux_c);
+    for (int a = 0; a < NA; ++a)
+      for (int b = 0; b < NB; ++b) {
+        int ae[3], be[3];
+        for (int d = 0; d < 3; ++d) { ae[d] = cart_exp(la, a, d, aux_a); be[d] = cart_exp(lb, b, d); }
+        double s = 0.0;
+        for (int t = 0; t <= ae[0] + be[0]; ++t)
+          for (int u = 0; u <= ae[1] + be[1]; ++u)
+            for (int v = 0; v <= ae[2] + be[2]; ++v) {
+              const double eab = Eb[0][(ae[0] * (lb + 1) + be[0]) * nt + t] * Eb[1][(ae[1] * (lb + 1) + be[1]) * nt + u] *
+                                 Eb[2][(ae[2] * (lb + 1) + be[2]) * nt + v];
+              for (int tt = 0; tt <= ce[0]; ++tt)
+                for (int uu = 0; uu <= ce[1]; ++uu)
+                  for (int vv = 0; vv <= ce[2]; ++vv) {
+                    const double ec = Ek[0][ce[0] * ntk + tt] * Ek[1][ce[1] * ntk + uu] * Ek[2][ce[2] * ntk + vv];
+                    const double sg = ((tt + uu + vv) & 1) ? -1.0 : 1.0;
+                    s += eab * sg * ec * R[hidx(t + tt, u + uu, v + vv)];
+                  }
+            }
+        out[(a * NB + b) * NC + c] += s;
+      }
+  }
+}
+
+__global__ void k_int2c2e(RI2cArgs g) {
+  const size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= g.npairs) return;
+  // shell pair (P >= Q) of the triangular index
+  size_t r = (size_t)((sqrt(8.0 * (double)id + 1.0) - 1.0) / 2.0);
+  while ((r + 1) * (r + 2) / 2 <= id) ++r;
+  while (r * (r + 1) / 2 > id) --r;
+  const int sP = (int)r, sQ = (int)(id - r * (r + 1) / 2);
+  const AuxShell A = g.shells[sP], B = g.shells[sQ];
+  if (A.qmax * B.qmax < g.thr) return;
+  const int NA = ncart(A.L), NB = ncart(B.L);
+  double out[10 * 10];
+  for (int k = 0; k < NA * NB; ++k) out[k] = 0.0;
+  for (int ia = 0; ia < A.nprim; ++ia) {
+    const AuxPrim pa = g.prims[A.prim_begin + ia];
+    for (int ib = 0; ib < B.nprim; ++ib) {
+      const AuxPrim pb = g.prims[B.prim_begin + ib];
+      if (pa.Q * pb.Q < g.thr) continue;
+      const double zero3[3] = {0.0, 0.0, 0.0};
+      const double pref = kTwoPi25 / (pa.alpha * pb.alpha * sqrt(pa.alpha + pb.alpha)) * pa.Kc * pb.Kc;
+      // (a| as a pair (a, s-dummy) with PA = PB = 0
+      double tmp[10 * 10];
+      for (int k = 0; k < NA * NB; ++k) tmp[k] = 0.0;
+      block_pair_single(A.L, 0, B.L, pa.alpha, pa.C, zero3, zero3, pb.alpha, pb.C, pref, true, true, g.boys, tmp);
+      for (int k = 0; k < NA * NB; ++k) out[k] += tmp[k];
+    }
+  }
+  // aux Cartesian order on the bra side too
+  for (int a = 0; a < NA; ++a)
+    for (int b = 0; b < NB; ++b) {
+      const int p = A.basis + a, q = B.basis + b;
+      const double v = out[a * NB + b] * g.w[p] * g.w[q];
+      g.V[(size_t)p * g.Maux + q] = v;
+      g.V[(size_t)q * g.Maux + p] = v;
+    }
+}
+
+__global__ void k_int3c2e(RI3cArgs g) {
+  const size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= (size_t)g.nsp * g.naux_shells) return;
+  const int isp = (int)(id / g.naux_shells), iaux = (int)(id % g.naux_shells);
+  const SPair sp = g.sp[isp];
+  const AuxShell C = g.shells[iaux];
+  if (sp.qmax * C.qmax < g.thr) return;
+  const int NA = ncart(sp.la), NB = ncart(sp.lb), NC = ncart(C.L);
+  double out[6 * 6 * 10];
+  for (int k = 0; k < NA * NB * NC; ++k) out[k] = 0.0;
+  for (int ip = 0; ip < sp.nprim; ++ip) {
+    const PairRec pr = g.pr[sp.prim_begin + ip];
+    for (int ic = 0; ic < C.nprim; ++ic) {
+      const AuxPrim pc = g.prims[C.prim_begin + ic];
+      if (pr.Q * pc.Q < g.thr) continue;
+      const double pref = kTwoPi25 / (pr.p * pc.alpha * sqrt(pr.p + pc.alpha)) * pr.Kc * pc.Kc;
+      block_pair_single(sp.la, sp.lb, C.L, pr.p, pr.P, pr.PA, pr.PB, pc.alpha, pc.C, pref, false, true, g.boys, out);
+    }
+  }
+  const size_t M = g.M, Maux = g.Maux;
+  const bool diff = sp.si != sp.sj;
+  for (int a = 0; a < NA; ++a)
+    for (int b = 0; b < NB; ++b) {
+      const int m = sp.bi + a, n = sp.bj + b;
+      const double wmn = g.w[m] * g.w[n];
+      for (int c = 0; c < NC; ++c) {
+        const int P = C.basis + c;
+        const double v = out[(a * NB + b) * NC + c] * wmn * g.waux[P];
+        g.B[((size_t)m * Maux + P) * M + n] = v;
+        if (diff) g.B[((size_t)n * Maux + P) * M + m] = v;
+      }
+    }
+}
+
+// W_P = sum_{m,n} B[m][P][n] D[m][n]: one workgroup per P.
+template <typename T>
+__global__ void k_ri_w(const T* __restrict__ B, const double* __restrict__ D, int M, int Maux, double* __restrict__ W) {
+  const int P = blockIdx.x;
+  double s = 0.0;
+  for (int m = 0; m < M; ++m) {
+    const T* row = B + ((size_t)m * Maux + P) * M;
+    const double* drow = D + (size_t)m * M;
+    for (int n = threadIdx.x; n < M; n += blockDim.x) s += (double)row[n] * drow[n];
+  }
+  __shared__ double sh[256];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) W[P] = sh[0];
+}
+
+// J[m][n] = sum_P W_P B[m][P][n]: one workgroup per m.
+template <typename T>
+__global__ void k_ri_j(const T* __restrict__ B, const double* __restrict__ W, int M, int Maux, double* __restrict__ J) {
+  const int m = blockIdx.x;
+  for (int n = threadIdx.x; n < M; n += blockDim.x) {
+    double s = 0.0;
+    const T* col = B + (size_t)m * Maux * M + n;
+    for (int P = 0; P < Maux; ++P) s += W[P] * (double)col[(size_t)P * M];
+    J[(size_t)m * M + n] = s;
+  }
+}
+
+__global__ void k_fock_ri(const double* __restrict__ H, const double* __restrict__ J, const double* __restrict__ K,
+                          size_t n, double* __restrict__ F) {
+  const size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id < n) F[id] = H[id] + J[id] - 0.5 * K[id];
+}
+
+__global__ void k_to_f32(const double* __restrict__ x, size_t n, float* __restrict__ y) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    y[i] = (float)x[i];
+}
+
+}  // namespace
+
+hipError_t launch_int2c2e(const RI2cArgs& a, hipStream_t s) {
+  if (a.npairs == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_int2c2e, dim3((unsigned)((a.npairs + 127) / 128)), dim3(128), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_int3c2e(const RI3cArgs& a, hipStream_t s) {
+  const size_t n = (size_t)a.nsp * a.naux_shells;
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_int3c2e, dim3((unsigned)((n + 127) / 128)), dim3(128), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_ri_w(const double* B, const double* D, int M, int Maux, double* W, hipStream_t s) {
+  hipLaunchKernelGGL(k_ri_w<double>, dim3(Maux), dim3(256), 0, s, B, D, M, Maux, W);
+  return hipGetLastError();
+}
+
+hipError_t launch_ri_j(const double* B, const double* W, int M, int Maux, double* J, hipStream_t s) {
+  hipLaunchKernelGGL(k_ri_j<double>, dim3(M), dim3(256), 0, s, B, W, M, Maux, J);
+  return hipGetLastError();
+}
+
+hipError_t launch_fock_ri(const double* H, const double* J, const double* K, size_t n, double* F, hipStream_t s) {
+  hipLaunchKernelGGL(k_fock_ri, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, H, J, K, n, F);
+  return hipGetLastError();
+}
+
+}  // namespace gansu

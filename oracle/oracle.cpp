@@ -761,10 +761,13 @@ double orc_optimal_damping(int n, const double* F, const double* Fp, const doubl
 // ---- RHF SCF with DIIS (hf.cu:245-314, rhf.hpp:298-327) ------------------
 // fock(D, F) is provided as a dense ERI tensor (M small) or computed by orc_jk.
 // Returns the number of iterations; energies in out[0]=E_elec, out[1]=E_total.
+int orc_ri_jk(int Maux, int M, const double* B, const double* D, double* J, double* K);
+
+// riB (Maux x M x M, B^P_mn) selects RI-J/K Fock builds (ERI_RI_RHF, rhf.hpp:650).
 int orc_rhf(int M, int nocc, const double* S, const double* H, const double* eri, const void* vprims,
             const void* vtypes, int ntypes, const double* norm, const double* Q, double thr, double enuc, int maxiter,
             double conv, int diis_size, int diis_include_transform, double* out_energy, double* out_D,
-            double* out_F) {
+            double* out_F, const double* riB, int Maux) {
   size_t MM = (size_t)M * M;
   std::vector<double> w(M), U(MM), X(MM), tmp(MM), Fp(MM), Cp(MM), C(MM), D(MM), F(MM), J(MM), K(MM);
   orc_eigh(M, S, w.data(), U.data());
@@ -786,6 +789,11 @@ int orc_rhf(int M, int nocc, const double* S, const double* H, const double* eri
       }
   };
   auto fock = [&]() {
+    if (riB) {
+      orc_ri_jk(Maux, M, riB, D.data(), J.data(), K.data());
+      for (size_t i = 0; i < MM; ++i) F[i] = H[i] + J[i] - 0.5 * K[i];
+      return;
+    }
     if (eri) { orc_fock_from_dense(eri, D.data(), H, M, F.data()); return; }
     orc_jk(vprims, vtypes, ntypes, norm, M, Q, thr, D.data(), J.data(), K.data(), 1);
     for (size_t i = 0; i < MM; ++i) F[i] = H[i] + J[i] - 0.5 * K[i];

@@ -154,13 +154,16 @@ def nuclear_repulsion(Z, xyz_bohr):
 
 
 def rhf(prims, types, norm, Z, xyz_bohr, nocc, thr=1e-12, conv=1e-10, maxiter=100, diis_size=8, dense=True,
-        diis_include_transform=False):
-    """Full FP64 RHF on the host.  Returns (E_elec, E_total, D, F, iterations)."""
+        diis_include_transform=False, riB=None):
+    """Full FP64 RHF on the host.  Returns (E_elec, E_total, D, F, iterations).
+    riB (Maux x M x M) selects RI-J/K Fock builds."""
     M = len(norm)
     S, T, V = oneint(prims, norm, Z, xyz_bohr)
     H = np.ascontiguousarray(T + V)
     Q = schwarz(prims, types)
-    eri = eri_dense(prims, types, norm, Q, thr) if dense else None
+    eri = eri_dense(prims, types, norm, Q, thr) if (dense and riB is None) else None
+    if riB is not None:
+        riB = np.ascontiguousarray(riB)
     enuc = nuclear_repulsion(Z, xyz_bohr)
     en = np.zeros(2)
     D = np.zeros((M, M))
@@ -169,7 +172,8 @@ def rhf(prims, types, norm, Z, xyz_bohr, nocc, thr=1e-12, conv=1e-10, maxiter=10
                        _ptr(eri) if eri is not None else None, _ptr(prims), _ptr(types), ctypes.c_int(len(types)),
                        _ptr(norm), _ptr(Q), ctypes.c_double(thr), ctypes.c_double(enuc), ctypes.c_int(maxiter),
                        ctypes.c_double(conv), ctypes.c_int(diis_size), ctypes.c_int(int(diis_include_transform)),
-                       _ptr(en), _ptr(D), _ptr(F))
+                       _ptr(en), _ptr(D), _ptr(F), _ptr(riB) if riB is not None else None,
+                       ctypes.c_int(riB.shape[0] if riB is not None else 0))
     return en[0], en[1], D, F, it
 
 
@@ -218,3 +222,12 @@ def ri_jk(B, D):
     lib().orc_ri_jk(ctypes.c_int(Ma), ctypes.c_int(M), _ptr(np.ascontiguousarray(B)),
                     _ptr(np.ascontiguousarray(D, dtype=np.float64)), _ptr(J), _ptr(K))
     return J, K
+
+
+def ri_tensor(prims, types, norm, aux_prims, aux_norm, thr=1e-12):
+    """B (Maux x M x M) = L^-1 (P|mn) with (P|Q) = L L^T."""
+    Q = schwarz(prims, types)
+    Qa = schwarz_aux(aux_prims)
+    V = int2c2e(aux_prims, aux_norm, Qa, thr)
+    T = int3c2e(prims, types, norm, Q, aux_prims, aux_norm, Qa, thr)
+    return ri_B(cholesky(V), T).reshape(len(aux_norm), len(norm), len(norm))
