@@ -9,7 +9,8 @@ import os
 import numpy as np
 
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(PKG_DIR, "libgansu_fock.so")
+# GANSU_FOCK_LIB points at another build of the same library (e.g. an experiment).
+LIB_PATH = os.environ.get("GANSU_FOCK_LIB") or os.path.join(PKG_DIR, "libgansu_fock.so")
 
 PRIM_DTYPE = np.dtype([("exponent", "f8"), ("coefficient", "f8"), ("x", "f8"), ("y", "f8"), ("z", "f8"),
                        ("shell_type", "i4"), ("_pad0", "i4"), ("basis_index", "u8"), ("atom_index", "i4"),

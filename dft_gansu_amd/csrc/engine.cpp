@@ -277,6 +277,22 @@ void FockEngine::precompute_quartets() {
       n_quartets_ += qc->n_quartets;
       qc->blocks.upload(qc->blocks_h);
       qc->bras.upload(qc->bras_h);
+      for (const auto& br : qc->bras_h) qc->max_count = std::max(qc->max_count, br.count);
+      // slab work records: tiles of bras, each bra's kets cut into chunks of
+      // a few passes so that heavy classes with few bras still fill the GPU
+      for (int mode : {(int)kModeDirect, (int)kModeStoredDigest}) {
+        int bpw = 1, qpp = 256;
+        slab_geometry(qc->LA, qc->LB, qc->LC, qc->LD, mode, &bpw, &qpp);
+        const int kch = 4 * qpp;
+        auto& w = qc->work_h[mode];
+        for (int b0 = 0; b0 < (int)qc->bras_h.size(); b0 += bpw) {
+          const int nb = std::min(bpw, (int)qc->bras_h.size() - b0);
+          int maxc = 0;
+          for (int b = 0; b < nb; ++b) maxc = std::max(maxc, qc->bras_h[b0 + b].count);
+          for (int k0 = 0; k0 < maxc; k0 += kch) w.push_back(SlabWork{b0, nb, k0, std::min(maxc, k0 + kch)});
+        }
+        qc->work[mode].upload(w);
+      }
       qclass_.push_back(std::move(qc));
     }
 
@@ -302,26 +318,32 @@ void FockEngine::build_four_center(const double* d_D, hipStream_t s) {
   GANSU_HIP(launch_zero(d_Kr_.ptr, (size_t)M_ * M_, s));
   EriArgs g{d_sp_.ptr, d_pr_.ptr, boys_.ptr, thr_, d_Dp_.ptr, M_, d_Jr_.ptr, d_Kr_.ptr};
   const int mode = method_ == "stored" ? kModeStoredDigest : kModeDirect;
-  if (max_lds_ == 0) {
-    int dev = 0, v = 0;
-    GANSU_HIP(hipGetDevice(&dev));
-    GANSU_HIP(hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
-    max_lds_ = (size_t)v;
-  }
   for (auto& qc : qclass_) {
     if (qc->blocks_h.empty()) continue;
     const double ncomp = double(ncart(qc->LA) * ncart(qc->LB) * ncart(qc->LC) * ncart(qc->LD));
     // algorithmic traffic of the stored digestion: each surviving integral read once
     const double bytes = mode == kModeStoredDigest ? qc->n_quartets * ncomp * 8.0 : 0.0;
-    const bool slab = slab_available(qc->LA, qc->LB, qc->LC, qc->LD, mode) &&
-                      slab_lds_bytes(qc->LA, qc->LB, M_) <= max_lds_;
+    const bool slab = slab_available(qc->LA, qc->LB, qc->LC, qc->LD, mode) && qc->slab_ok;
     prof_begin(s, 10000.0 * mode + 1000 * qc->LA + 100 * qc->LB + 10 * qc->LC + qc->LD + (slab ? 100000.0 : 0.0),
                bytes, 0.0, qc->n_quartets);
+    bool launched = false;
     if (slab) {
-      SlabLaunch c{qc->LA, qc->LB, qc->LC, qc->LD, qc->bras.ptr, (int)qc->bras_h.size(),
-                   pclass_[qc->A].list.ptr, pclass_[qc->B].list.ptr, qc->store.ptr};
-      GANSU_HIP(launch_fock_slab(c, g, mode, s));
-    } else {
+      SlabLaunch c{qc->LA, qc->LB, qc->LC, qc->LD, qc->bras.ptr, (int)qc->bras_h.size(), qc->work[mode].ptr,
+                   (int)qc->work_h[mode].size(), pclass_[qc->A].list.ptr, pclass_[qc->B].list.ptr, qc->store.ptr,
+                   qc->max_count};
+      const hipError_t e = launch_fock_slab(c, g, mode, s);
+      if (e == hipErrorInvalidValue) qc->slab_ok = false;  // LDS footprint too large: use the per-lane kernel
+      else {
+        GANSU_HIP(e);
+        launched = true;
+        if (mode == kModeStoredDigest) {
+          prof_end(s);
+          prof_begin(s, 300000.0 + 10000.0 * mode + 1000 * qc->LA + 100 * qc->LB + 10 * qc->LC + qc->LD, bytes, 0.0, qc->n_quartets);
+          GANSU_HIP(launch_jket(c, g, s));
+        }
+      }
+    }
+    if (!launched) {
       ClassLaunch c{qc->LA, qc->LB, qc->LC, qc->LD, qc->blocks.ptr, (int)qc->blocks_h.size(),
                     pclass_[qc->A].list.ptr, pclass_[qc->B].list.ptr, qc->store.ptr};
       GANSU_HIP(launch_eri_class(c, g, mode, s));

@@ -59,6 +59,10 @@ struct QuartetClass {
   DevBuf<BlockRec> blocks;
   std::vector<BraRec> bras_h;  // one entry per bra with kets (slab digestion)
   DevBuf<BraRec> bras;
+  bool slab_ok = true;  // cleared when the slab kernel's LDS does not fit
+  std::vector<SlabWork> work_h[3];  // per mode (direct, -, stored digest)
+  DevBuf<SlabWork> work[3];
+  int max_count = 0;
   DevBuf<double> store;  // stored integrals of this class (eri_method=stored)
   double n_quartets = 0;
 };
@@ -119,7 +123,6 @@ class FockEngine {
   std::vector<std::unique_ptr<QuartetClass>> qclass_;
   DevBuf<double> d_w_, d_Dp_, d_Jr_, d_Kr_;
   double n_quartets_ = 0, n_prim_quartets_ = 0, stored_bytes_ = 0;
-  size_t max_lds_ = 0;  // device limit of dynamic LDS per workgroup
 
   // profiling: events around each launch of a build
   struct ProfRow { double id, bytes, flops, units; hipEvent_t a, b; };

@@ -565,140 +565,275 @@ hipError_t launch_t(const ClassLaunch& c, const EriArgs& a, int mode, hipStream_
 
 
 // --------------------------------------------------- slab digestion kernel --
-// One 256-thread workgroup per bra shell pair (i j); each thread takes one ket
-// (k l) at a time.  Every K update of a quartet lands in rows of shell i or j:
+// A 256-thread workgroup owns BPW consecutive bra shell pairs (i j) of one
+// quartet class and walks all of their kets (k l).  Every K update of a
+// quartet lands in a row of shell i or j:
 //   K[a][c] += v D[b][d], K[b][c] += v D[a][d], K[a][d] += v D[b][c],
 //   K[b][d] += v D[a][c]      (a in i, b in j, c in k, d in l)
-// so the workgroup accumulates K[i rows][:] and K[j rows][:] in LDS (with the
-// matching D rows beside them) and flushes them once.  J of the bra block is
-// reduced over the workgroup; J of each ket block goes out with one atomic
-// per ket component.
+// so the workgroup accumulates K[i rows][:] and K[j rows][:] of each owned bra
+// in LDS (with the matching D rows beside them) and flushes them once.  A
+// thread reads the ket's metadata and D block once for all BPW bras, and its
+// J contribution to the ket block leaves with one atomic per component per
+// workgroup.  For the large bra blocks (d-p, d-d) the NA rows of a quartet are
+// split over NA lanes instead (SPLIT), which keeps per-lane work short.
 constexpr int kSlabThreads = 256;
+
+template <int LA, int LB, int LC, int LD, int MODE>
+struct SlabShape {
+  static constexpr int NA = ncart(LA), NB = ncart(LB), NAB = NA * NB;
+  static constexpr bool SPLIT = MODE == kModeStoredDigest && NAB >= 18;
+  static constexpr int BPW = MODE != kModeStoredDigest ? 1 : (NAB == 1 ? 8 : NAB <= 3 ? 4 : NAB <= 9 ? 2 : 1);
+  static constexpr int LPQ = SPLIT ? NA : 1;          // lanes per quartet
+  static constexpr int QPP = kSlabThreads / LPQ;      // quartets per pass
+};
 
 template <int LA, int LB, int LC, int LD, int MODE>
 __global__ __launch_bounds__(kSlabThreads) void k_fock_slab(SlabLaunch c, EriArgs g) {
   extern __shared__ double lds[];
+  using S = SlabShape<LA, LB, LC, LD, MODE>;
   constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
   constexpr int NAB = NA * NB, NCD = NC * ND, NCOMP = NAB * NCD;
-  const BraRec br = c.bras[blockIdx.x];
-  const int bra_id = c.bra_list[br.bra_pos];
-  const SPair bra = g.sp[bra_id];
+  constexpr int BPW = S::BPW, LPQ = S::LPQ, QPP = S::QPP;
+  constexpr bool SPLIT = S::SPLIT;
   const int M = g.M;
   const int tid = threadIdx.x;
-  const bool ij_diff = bra.si != bra.sj;
-  const int bi = bra.bi, bj = bra.bj;
-  double* Ki = lds;           // [NA][M]
-  double* Kj = lds + NA * M;  // [NB][M]
-  double* Di = Kj + NB * M;   // [NA][M]: D rows of shell i
-  double* Dj = Di + NA * M;   // [NB][M]: D rows of shell j
-  for (int x = tid; x < (NA + NB) * M; x += kSlabThreads) {
-    lds[x] = 0.0;
-    const int r = x / M, col = x - r * M;
-    const int row = r < NA ? bi + r : bj + (r - NA);
-    Di[x] = g.Dp[(size_t)row * M + col];
-  }
-  __syncthreads();
-  double jb[NAB];
+  const SlabWork wk = c.work[blockIdx.x];
+  const int b0 = wk.bra0;
+  const int nb = wk.nb;
+  // per owned bra: [Ki NA*M][Kj NB*M][Di NA*M][Dj NB*M], then J of the bra block
+  const int slab = 2 * (NA + NB) * M;
+  double* jbl = lds + BPW * slab;  // [BPW][NAB]
+  int bid[BPW], bbi[BPW], bbj[BPW], cnt[BPW], fblk[BPW];
+  bool bdiff[BPW];
+  int maxc = 0;
 #pragma unroll
-  for (int k = 0; k < NAB; ++k) jb[k] = 0.0;
+  for (int b = 0; b < BPW; ++b) {
+    if (b < nb) {
+      const BraRec br = c.bras[b0 + b];
+      bid[b] = c.bra_list[br.bra_pos];
+      const SPair bra = g.sp[bid[b]];
+      bbi[b] = bra.bi; bbj[b] = bra.bj; bdiff[b] = bra.si != bra.sj;
+      cnt[b] = min(br.count, wk.k_end); fblk[b] = br.first_block;
+      maxc = max(maxc, cnt[b]);
+    } else {
+      bid[b] = -1; bbi[b] = bbj[b] = 0; bdiff[b] = false; cnt[b] = 0; fblk[b] = 0;
+    }
+  }
+  for (int b = 0; b < nb; ++b) {
+    double* base = lds + b * slab;
+    for (int x = tid; x < (NA + NB) * M; x += kSlabThreads) {
+      base[x] = 0.0;
+      const int r = x / M, col = x - r * M;
+      const int row = r < NA ? bbi[b] + r : bbj[b] + (r - NA);
+      base[(NA + NB) * M + x] = g.Dp[(size_t)row * M + col];
+    }
+  }
+  for (int x = tid; x < BPW * NAB; x += kSlabThreads) jbl[x] = 0.0;
+  __syncthreads();
 
-  for (int k0 = 0; k0 < br.count; k0 += kSlabThreads) {
-    const int kk = k0 + tid;
-    if (kk < br.count) {
+  // SPLIT: lane -> (row a, ket); consecutive lanes read consecutive kets
+  const int sub = SPLIT ? tid / QPP : 0;
+  const int qi = SPLIT ? tid % QPP : tid;
+  constexpr int NJB = SPLIT ? NB : NAB;  // J bra components a lane owns
+  double jb[BPW][NJB];
+#pragma unroll
+  for (int b = 0; b < BPW; ++b)
+#pragma unroll
+    for (int k = 0; k < NJB; ++k) jb[b][k] = 0.0;
+
+  for (int k0 = wk.k_begin; k0 < maxc; k0 += QPP) {
+    const int kk = k0 + qi;
+    if (sub < LPQ && kk < maxc) {
       const int ket_id = c.ket_list[kk];
       const SPair ket = g.sp[ket_id];
       const bool kl_diff = ket.si != ket.sj;
-      const bool pair_diff = bra_id != ket_id;
-      const double fbk = (ij_diff ? 2.0 : 1.0) * (kl_diff ? 2.0 : 1.0);
-      const double fo = pair_diff ? 2.0 : 1.0;
       const int bk = ket.bi, bl = ket.bj;
-      [[maybe_unused]] double out[MODE == kModeDirect ? NCOMP : 1];
-      [[maybe_unused]] const double* src = nullptr;
-      if constexpr (MODE == kModeDirect) {
-#pragma unroll
-        for (int k = 0; k < NCOMP; ++k) out[k] = 0.0;
-        eri_block_t<LA, LB, LC, LD>(bra, ket, ket.nprim, g, out);
-      } else {
-        src = c.store + ((size_t)(br.first_block + kk / 64) * NCOMP) * 64 + (kk & 63);
-      }
-      auto V = [&](int comp) -> double {
-        if constexpr (MODE == kModeDirect) return out[comp];
-        else return src[(size_t)comp * 64];
-      };
       double dcd[NCD], jk[NCD];
 #pragma unroll
       for (int cc = 0; cc < NC; ++cc)
 #pragma unroll
         for (int d = 0; d < ND; ++d) {
-          dcd[cc * ND + d] = fbk * g.Dp[(size_t)(bk + cc) * M + bl + d];
+          dcd[cc * ND + d] = g.Dp[(size_t)(bk + cc) * M + bl + d];
           jk[cc * ND + d] = 0.0;
         }
 #pragma unroll
-      for (int a = 0; a < NA; ++a)
+      for (int b = 0; b < BPW; ++b) {
+        if (b >= nb || kk >= cnt[b]) continue;
+        const double* Ki = lds + b * slab;
+        const double* Kj_c = Ki + NA * M;
+        double* KiW = lds + b * slab;
+        double* KjW = KiW + NA * M;
+        const double* Di = Kj_c + NB * M;
+        const double* Dj = Di + NA * M;
+        (void)Ki;
+        const bool ij_diff = bdiff[b];
+        const bool pair_diff = bid[b] != ket_id;
+        const double fbk = (ij_diff ? 2.0 : 1.0) * (kl_diff ? 2.0 : 1.0);
+        const double fo = pair_diff ? 2.0 : 1.0;
+        const double fjk = pair_diff ? fbk : 0.0;
+        const int bj = bbj[b];
+        [[maybe_unused]] double out[MODE == kModeDirect ? NCOMP : 1];
+        [[maybe_unused]] const double* src = nullptr;
+        if constexpr (MODE == kModeDirect) {
 #pragma unroll
-        for (int b = 0; b < NB; ++b) {
-          const double dab = Di[a * M + bj + b];
-          double kad[ND], kbd[ND];
+          for (int k = 0; k < NCOMP; ++k) out[k] = 0.0;
+          eri_block_t<LA, LB, LC, LD>(g.sp[bid[b]], ket, ket.nprim, g, out);
+        } else {
+          src = c.store + ((size_t)(fblk[b] + kk / 64) * NCOMP) * 64 + (kk & 63);
+        }
+        auto V = [&](int comp) -> double {
+          if constexpr (MODE == kModeDirect) return out[comp];
+          else return src[(size_t)comp * 64];
+        };
 #pragma unroll
-          for (int d = 0; d < ND; ++d) { kad[d] = 0.0; kbd[d] = 0.0; }
+        for (int aa = 0; aa < (SPLIT ? 1 : NA); ++aa) {
+          const int a = SPLIT ? sub : aa;
 #pragma unroll
-          for (int cc = 0; cc < NC; ++cc) {
-            const double dbc = Dj[b * M + bk + cc];
-            const double dac = Di[a * M + bk + cc];
-            double kac = 0.0, kbc = 0.0;
+          for (int bb = 0; bb < NB; ++bb) {
+            const double dab = Di[a * M + bj + bb];
+            double kad[ND], kbd[ND];
 #pragma unroll
-            for (int d = 0; d < ND; ++d) {
-              const double v = V(((a * NB + b) * NC + cc) * ND + d);
-              jb[a * NB + b] += v * dcd[cc * ND + d];
-              jk[cc * ND + d] += v * dab;
-              kac += v * Dj[b * M + bl + d];
-              kbc += v * Di[a * M + bl + d];
-              kad[d] += v * dbc;
-              kbd[d] += v * dac;
+            for (int d = 0; d < ND; ++d) { kad[d] = 0.0; kbd[d] = 0.0; }
+            double jbs = 0.0;
+#pragma unroll
+            for (int cc = 0; cc < NC; ++cc) {
+              const double dbc = Dj[bb * M + bk + cc];
+              const double dac = Di[a * M + bk + cc];
+              double kac = 0.0, kbc = 0.0;
+#pragma unroll
+              for (int d = 0; d < ND; ++d) {
+                const double v = V(((a * NB + bb) * NC + cc) * ND + d);
+                jbs += v * dcd[cc * ND + d];
+                if constexpr (MODE == kModeDirect) jk[cc * ND + d] += fjk * v * dab;
+                kac += v * Dj[bb * M + bl + d];
+                kbc += v * Di[a * M + bl + d];
+                kad[d] += v * dbc;
+                kbd[d] += v * dac;
+              }
+              unsafeAtomicAdd(&KiW[a * M + bk + cc], fo * kac);
+              if (ij_diff) unsafeAtomicAdd(&KjW[bb * M + bk + cc], fo * kbc);
             }
-            unsafeAtomicAdd(&Ki[a * M + bk + cc], fo * kac);
-            if (ij_diff) unsafeAtomicAdd(&Kj[b * M + bk + cc], fo * kbc);
-          }
-          if (kl_diff) {
+            jb[b][SPLIT ? bb : a * NB + bb] += fbk * jbs;
+            if (kl_diff) {
 #pragma unroll
-            for (int d = 0; d < ND; ++d) {
-              unsafeAtomicAdd(&Ki[a * M + bl + d], fo * kad[d]);
-              if (ij_diff) unsafeAtomicAdd(&Kj[b * M + bl + d], fo * kbd[d]);
+              for (int d = 0; d < ND; ++d) {
+                unsafeAtomicAdd(&KiW[a * M + bl + d], fo * kad[d]);
+                if (ij_diff) unsafeAtomicAdd(&KjW[bb * M + bl + d], fo * kbd[d]);
+              }
             }
           }
         }
-      if (pair_diff) {
+      }
+      // stored blocks: J of the ket block comes from the ket-major pass (k_jket)
+      if constexpr (MODE == kModeDirect) {
 #pragma unroll
         for (int cd = 0; cd < NCD; ++cd)
-          unsafeAtomicAdd(&g.Jr[(size_t)(bk + cd / ND) * M + bl + cd % ND], fbk * jk[cd]);
+          if (jk[cd] != 0.0) unsafeAtomicAdd(&g.Jr[(size_t)(bk + cd / ND) * M + bl + cd % ND], jk[cd]);
       }
     }
   }
-  __syncthreads();
-  for (int x = tid; x < NA * M; x += kSlabThreads) {
-    const double v = Ki[x];
-    if (v != 0.0) unsafeAtomicAdd(&g.Kr[(size_t)(bi + x / M) * M + x % M], v);
-  }
-  if (ij_diff)
-    for (int x = tid; x < NB * M; x += kSlabThreads) {
-      const double v = Kj[x];
-      if (v != 0.0) unsafeAtomicAdd(&g.Kr[(size_t)(bj + x / M) * M + x % M], v);
-    }
-  // J of the bra block: wave sums, then one atomic per component per wave
+  // J of the bra blocks: LDS accumulation, then one atomic per component
 #pragma unroll
-  for (int ab = 0; ab < NAB; ++ab) {
-    const double sum = wave_sum(jb[ab]);
-    if ((tid & 63) == 0) unsafeAtomicAdd(&g.Jr[(size_t)(bi + ab / NB) * M + bj + ab % NB], sum);
+  for (int b = 0; b < BPW; ++b) {
+    if (b >= nb) break;
+#pragma unroll
+    for (int k = 0; k < NJB; ++k)
+      if (jb[b][k] != 0.0) unsafeAtomicAdd(&jbl[b * NAB + (SPLIT ? sub * NB + k : k)], jb[b][k]);
+  }
+  __syncthreads();
+  for (int b = 0; b < nb; ++b) {
+    const double* Ki = lds + b * slab;
+    const double* Kj = Ki + NA * M;
+    for (int x = tid; x < NA * M; x += kSlabThreads) {
+      const double v = Ki[x];
+      if (v != 0.0) unsafeAtomicAdd(&g.Kr[(size_t)(bbi[b] + x / M) * M + x % M], v);
+    }
+    if (bdiff[b])
+      for (int x = tid; x < NB * M; x += kSlabThreads) {
+        const double v = Kj[x];
+        if (v != 0.0) unsafeAtomicAdd(&g.Kr[(size_t)(bbj[b] + x / M) * M + x % M], v);
+      }
+    for (int ab = tid; ab < NAB; ab += kSlabThreads)
+      unsafeAtomicAdd(&g.Jr[(size_t)(bbi[b] + ab / NB) * M + bbj[b] + ab % NB], jbl[b * NAB + ab]);
+  }
+}
+
+template <int LA, int LB, int LC, int LD, int MODE>
+size_t slab_lds_t(int M) {
+  using S = SlabShape<LA, LB, LC, LD, MODE>;
+  return (size_t)S::BPW * (2ull * (S::NA + S::NB) * (size_t)M + S::NAB) * sizeof(double);
+}
+
+template <int LA, int LB, int LC, int LD>
+hipError_t launch_slab_t(const SlabLaunch& c, const EriArgs& a, int mode, size_t max_lds, hipStream_t s) {
+  if (mode == kModeDirect) {
+    const size_t lds = slab_lds_t<LA, LB, LC, LD, kModeDirect>(a.M);
+    if (lds > max_lds) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((k_fock_slab<LA, LB, LC, LD, kModeDirect>), dim3(c.nwork), dim3(kSlabThreads), lds, s, c, a);
+  } else {
+    const size_t lds = slab_lds_t<LA, LB, LC, LD, kModeStoredDigest>(a.M);
+    if (lds > max_lds) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((k_fock_slab<LA, LB, LC, LD, kModeStoredDigest>), dim3(c.nwork), dim3(kSlabThreads), lds, s,
+                       c, a);
+  }
+  return hipGetLastError();
+}
+
+
+// ---------------------------------------------- ket-major J of stored blocks --
+// J[k+c][l+d] += f sum_ab (ab|cd) D[i+a][j+b] for every stored quartet.  A
+// 64-lane workgroup takes 64 consecutive ket positions, one ket component cd
+// (blockIdx.y) and a range of bras (blockIdx.z): the bra's D block is uniform
+// across the wave, the lanes read 64 consecutive kets of one stored chunk
+// (coalesced), and each lane adds its sum to J once.
+template <int LA, int LB, int LC, int LD>
+__global__ __launch_bounds__(64) void k_jket(SlabLaunch c, EriArgs g, int bras_per_block) {
+  constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
+  constexpr int NAB = NA * NB, NCD = NC * ND, NCOMP = NAB * NCD;
+  const int tfirst = blockIdx.x * 64;
+  const int t = tfirst + threadIdx.x;
+  const int cd = blockIdx.y;
+  const int s0 = blockIdx.z * bras_per_block;
+  const int s1 = min(c.nbras, s0 + bras_per_block);
+  const int M = g.M;
+  double jk = 0.0;
+  int ket_id = -1;
+  bool kl_diff = false;
+  if (t < c.max_count) {
+    ket_id = c.ket_list[t];
+    const SPair ket = g.sp[ket_id];
+    kl_diff = ket.si != ket.sj;
+  }
+  for (int s = s0; s < s1; ++s) {
+    const BraRec br = c.bras[s];
+    if (br.count <= tfirst) continue;  // uniform: no ket of this workgroup
+    if (t >= br.count) continue;
+    const int bra_id = c.bra_list[br.bra_pos];
+    if (bra_id == ket_id) continue;  // diagonal quartet: its J is complete in the bra pass
+    const SPair bra = g.sp[bra_id];
+    const double f = (bra.si != bra.sj ? 2.0 : 1.0) * (kl_diff ? 2.0 : 1.0);
+    const double* src = c.store + ((size_t)(br.first_block + t / 64) * NCOMP) * 64 + (t & 63) + (size_t)cd * 64;
+    double acc = 0.0;
+#pragma unroll
+    for (int a = 0; a < NA; ++a)
+#pragma unroll
+      for (int b = 0; b < NB; ++b)
+        acc += g.Dp[(size_t)(bra.bi + a) * M + bra.bj + b] * src[(size_t)(a * NB + b) * NCD * 64];
+    jk += f * acc;
+  }
+  if (ket_id >= 0 && jk != 0.0) {
+    const SPair ket = g.sp[ket_id];
+    unsafeAtomicAdd(&g.Jr[(size_t)(ket.bi + cd / ND) * M + ket.bj + cd % ND], jk);
   }
 }
 
 template <int LA, int LB, int LC, int LD>
-hipError_t launch_slab_t(const SlabLaunch& c, const EriArgs& a, int mode, size_t lds, hipStream_t s) {
-  if (mode == kModeDirect)
-    hipLaunchKernelGGL((k_fock_slab<LA, LB, LC, LD, kModeDirect>), dim3(c.nbras), dim3(kSlabThreads), lds, s, c, a);
-  else
-    hipLaunchKernelGGL((k_fock_slab<LA, LB, LC, LD, kModeStoredDigest>), dim3(c.nbras), dim3(kSlabThreads), lds, s,
-                       c, a);
+hipError_t launch_jket_t(const SlabLaunch& c, const EriArgs& a, hipStream_t s) {
+  constexpr int NCD = ncart(LC) * ncart(LD);
+  // few long bra loops for light classes, more parallel ones for heavy ones
+  const int bpb = ncart(LA) * ncart(LB) * NCD <= 9 ? 64 : 16;
+  dim3 grid((c.max_count + 63) / 64, NCD, (c.nbras + bpb - 1) / bpb);
+  hipLaunchKernelGGL((k_jket<LA, LB, LC, LD>), grid, dim3(64), 0, s, c, a, bpb);
   return hipGetLastError();
 }
 
@@ -753,8 +888,31 @@ hipError_t launch_eri_class(const ClassLaunch& c, const EriArgs& a, int mode, hi
 }
 
 
-size_t slab_lds_bytes(int LA, int LB, int M) {
-  return 2ull * (size_t)(ncart(LA) + ncart(LB)) * (size_t)M * sizeof(double);
+
+void slab_geometry(int LA, int LB, int LC, int LD, int mode, int* bpw, int* qpp) {
+  const int nab = ncart(LA) * ncart(LB);
+  const bool stored = mode == kModeStoredDigest;
+  const bool split = stored && nab >= 18;
+  *bpw = !stored ? 1 : (nab == 1 ? 8 : nab <= 3 ? 4 : nab <= 9 ? 2 : 1);
+  *qpp = split ? kSlabThreads / ncart(LA) : kSlabThreads;
+  (void)LC; (void)LD;
+}
+
+hipError_t launch_jket(const SlabLaunch& c, const EriArgs& a, hipStream_t s) {
+  if (c.nbras == 0 || c.max_count == 0) return hipSuccess;
+  const int key = ((c.LA * 3 + c.LB) * 3 + c.LC) * 3 + c.LD;
+#define GANSU_JK(a_, b_, c_, d_) \
+  case ((a_ * 3 + b_) * 3 + c_) * 3 + d_: return launch_jket_t<a_, b_, c_, d_>(c, a, s);
+  switch (key) {
+    GANSU_JK(0, 0, 0, 0) GANSU_JK(1, 0, 0, 0) GANSU_JK(1, 0, 1, 0) GANSU_JK(1, 1, 0, 0)
+    GANSU_JK(1, 1, 1, 0) GANSU_JK(1, 1, 1, 1) GANSU_JK(2, 0, 0, 0) GANSU_JK(2, 0, 1, 0)
+    GANSU_JK(2, 0, 1, 1) GANSU_JK(2, 0, 2, 0) GANSU_JK(2, 1, 0, 0) GANSU_JK(2, 1, 1, 0)
+    GANSU_JK(2, 1, 1, 1) GANSU_JK(2, 1, 2, 0) GANSU_JK(2, 1, 2, 1) GANSU_JK(2, 2, 0, 0)
+    GANSU_JK(2, 2, 1, 0) GANSU_JK(2, 2, 1, 1) GANSU_JK(2, 2, 2, 0) GANSU_JK(2, 2, 2, 1)
+    GANSU_JK(2, 2, 2, 2)
+    default: return hipErrorInvalidValue;
+  }
+#undef GANSU_JK
 }
 
 // Register-resident blocks (the classes of launch_eri_class's templated list)
@@ -767,6 +925,8 @@ bool slab_available(int LA, int LB, int LC, int LD, int mode) {
   return mode == kModeDirect && L <= 4 && ncomp <= 81;
 }
 
+// Launches the slab kernel; hipErrorInvalidValue (nothing launched) when the
+// class has no slab variant or its LDS footprint exceeds the device limit.
 hipError_t launch_fock_slab(const SlabLaunch& c, const EriArgs& a, int mode, hipStream_t s) {
   if (c.nbras == 0) return hipSuccess;
   if (!slab_available(c.LA, c.LB, c.LC, c.LD, mode)) return hipErrorInvalidValue;
@@ -778,11 +938,9 @@ hipError_t launch_fock_slab(const SlabLaunch& c, const EriArgs& a, int mode, hip
     if (e != hipSuccess) return e;
     g_max_lds = (size_t)v;
   }
-  const size_t lds = slab_lds_bytes(c.LA, c.LB, a.M);
-  if (lds > g_max_lds) return hipErrorInvalidValue;
   const int key = ((c.LA * 3 + c.LB) * 3 + c.LC) * 3 + c.LD;
 #define GANSU_SLAB(a_, b_, c_, d_) \
-  case ((a_ * 3 + b_) * 3 + c_) * 3 + d_: return launch_slab_t<a_, b_, c_, d_>(c, a, mode, lds, s);
+  case ((a_ * 3 + b_) * 3 + c_) * 3 + d_: return launch_slab_t<a_, b_, c_, d_>(c, a, mode, g_max_lds, s);
   if (mode == kModeDirect) {
     switch (key) {
       GANSU_SLAB(0, 0, 0, 0) GANSU_SLAB(1, 0, 0, 0) GANSU_SLAB(1, 0, 1, 0) GANSU_SLAB(1, 1, 0, 0)

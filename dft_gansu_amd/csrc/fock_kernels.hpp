@@ -36,14 +36,27 @@ struct BraRec {
   int bra_pos, first_block, count, pad;
 };
 
+// A slab workgroup's share: nb consecutive bras starting at bras[bra0] and
+// the ket positions [k_begin, k_end) of each (clipped to the bra's count).
+struct SlabWork {
+  int bra0, nb, k_begin, k_end;
+};
+
 struct SlabLaunch {
   int LA, LB, LC, LD;
   const BraRec* bras;
   int nbras;
+  const SlabWork* work;
+  int nwork;
   const int* bra_list;
   const int* ket_list;
   const double* store;
+  int max_count;  // largest ket count of any bra (ket-major J pass)
 };
+
+// Geometry of the slab kernel of a class: bras per workgroup and kets a
+// workgroup visits per pass (used by the host to cut SlabWork records).
+void slab_geometry(int LA, int LB, int LC, int LD, int mode, int* bras_per_wg, int* kets_per_pass);
 
 struct ClassLaunch {
   int LA, LB, LC, LD;
@@ -76,8 +89,9 @@ hipError_t launch_eri_class(const ClassLaunch& c, const EriArgs& a, int mode, hi
 // slab variant or its LDS footprint exceeds the device limit (callers then
 // use launch_eri_class).
 hipError_t launch_fock_slab(const SlabLaunch& c, const EriArgs& a, int mode, hipStream_t s);
-size_t slab_lds_bytes(int LA, int LB, int M);
 bool slab_available(int LA, int LB, int LC, int LD, int mode);
+// J of the ket blocks from stored integrals, ket-major (no atomics per quartet).
+hipError_t launch_jket(const SlabLaunch& c, const EriArgs& a, hipStream_t s);
 hipError_t launch_scale_density(const double* D, const double* w, int M, double* Dp, hipStream_t s);
 hipError_t launch_zero(double* p, size_t n, hipStream_t s);
 // J = w w sym(Jr), K = w w sym(Kr); if F: F = H + J - K/2 (J, K may be null then)
