@@ -27,7 +27,7 @@ EXPORTS = [
     "gansu_last_error", "gansu_version", "gansu_molecule_from_xyz", "gansu_molecule_from_atoms",
     "gansu_molecule_sizes", "gansu_molecule_get", "gansu_molecule_nuclear_repulsion", "gansu_molecule_destroy",
     "gansu_params_create", "gansu_params_load_recipe", "gansu_params_set", "gansu_params_get", "gansu_params_destroy",
-    "gansu_fock_create", "gansu_fock_precompute", "gansu_fock_build_rhf", "gansu_fock_build_jk",
+    "gansu_fock_create", "gansu_fock_precompute", "gansu_fock_build_rhf", "gansu_fock_build_jk", "gansu_fock_build_rhf_occ",
     "gansu_fock_algorithm_name", "gansu_fock_schwarz", "gansu_fock_num_primitive_pairs", "gansu_fock_stats",
     "gansu_fock_set_profiling", "gansu_fock_last_profile", "gansu_fock_destroy", "gansu_oneint", "gansu_rhf_run",
 ]
@@ -69,6 +69,7 @@ def lib():
         L.gansu_fock_precompute.argtypes = [vp]
         L.gansu_fock_build_rhf.argtypes = [vp, vp, vp, vp, vp]
         L.gansu_fock_build_jk.argtypes = [vp, vp, vp, vp, vp]
+        L.gansu_fock_build_rhf_occ.argtypes = [vp, vp, vp, ip, vp, vp, vp]
         L.gansu_fock_algorithm_name.argtypes = [vp]
         L.gansu_fock_algorithm_name.restype = cp
         L.gansu_fock_schwarz.argtypes = [vp, vp, sz]

@@ -160,6 +160,14 @@ int gansu_fock_build_jk(gansu_fock_engine* e, const double* d_D, double* d_J, do
   });
 }
 
+int gansu_fock_build_rhf_occ(gansu_fock_engine* e, const double* d_D, const double* d_C, int nocc, const double* d_H,
+                             double* d_F, void* stream) {
+  return guarded([&] {
+    if (!e || !d_D || !d_C || !d_H || !d_F) throw gansu::Error(GANSU_E_ARG, "null argument");
+    e->e->build_jk(d_D, nullptr, nullptr, d_H, d_F, (hipStream_t)stream, d_C, nocc);
+  });
+}
+
 const char* gansu_fock_algorithm_name(const gansu_fock_engine* e) { return e ? e->e->algorithm_name() : ""; }
 
 int gansu_fock_schwarz(gansu_fock_engine* e, double* host_Q, size_t n) {

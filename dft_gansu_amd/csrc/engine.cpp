@@ -62,6 +62,10 @@ FockEngine::FockEngine(const gansu_primitive_shell* prims, size_t n_prims, const
       if (kv.second != "fp64" && kv.second != "mixed") throw Error(GANSU_E_PARAM, "precision must be fp64|mixed");
       mixed_ = kv.second == "mixed";
     } else if (kv.first == "fp32_bound_threshold") fp32_thr_ = std::stod(kv.second);
+    else if (kv.first == "ri_k") {
+      if (kv.second != "density" && kv.second != "occupied") throw Error(GANSU_E_PARAM, "ri_k must be density|occupied");
+      ri_k_occ_ = kv.second == "occupied";
+    }
     else throw Error(GANSU_E_PARAM, "Unknown engine option: " + kv.first);
   }
   if (method_ != "stored" && method_ != "direct" && method_ != "ri")
@@ -353,10 +357,11 @@ void FockEngine::build_four_center(const double* d_D, hipStream_t s) {
 }
 
 void FockEngine::build_jk(const double* d_D, double* d_J, double* d_K, const double* d_H, double* d_F,
-                          hipStream_t s) {
+                          hipStream_t s, const double* d_C, int nocc) {
   if (!precomputed_) throw Error(GANSU_E_STATE, "build before precompute");
   if (!d_D) throw Error(GANSU_E_ARG, "null density");
-  if (method_ == "ri") { build_ri(d_D, d_J, d_K, d_H, d_F, s); return; }
+  if (d_C && (nocc <= 0 || nocc > M_)) throw Error(GANSU_E_ARG, "nocc out of range");
+  if (method_ == "ri") { build_ri(d_D, d_J, d_K, d_H, d_F, s, d_C, nocc); return; }
   build_four_center(d_D, s);
   GANSU_HIP(launch_finalize(d_Jr_.ptr, d_Kr_.ptr, d_w_.ptr, M_, d_H, d_J, d_K, d_F, s));
 }

@@ -82,7 +82,9 @@ class FockEngine {
   ~FockEngine();
 
   void precompute();
-  void build_jk(const double* d_D, double* d_J, double* d_K, const double* d_H, double* d_F, hipStream_t s);
+  // d_C/nocc (optional): occupied orbitals, D = 2 C_occ C_occ^T
+  void build_jk(const double* d_D, double* d_J, double* d_K, const double* d_H, double* d_F, hipStream_t s,
+                const double* d_C = nullptr, int nocc = 0);
   const char* algorithm_name() const;
   std::vector<double> schwarz_ref() const;  // host copy, reference pair order
   size_t num_primitive_pairs() const { return ref_pairs_; }
@@ -95,13 +97,16 @@ class FockEngine {
   void precompute_pairs();     // Schwarz factors, shell pairs, primitive-pair records
   void precompute_quartets();  // quartet classes, work blocks, stored integrals
   void precompute_ri();
+  void build_ri_k_occ(const double* d_C, int nocc, double* K, hipStream_t s);
   void build_four_center(const double* d_D, hipStream_t s);
-  void build_ri(const double* d_D, double* d_J, double* d_K, const double* d_H, double* d_F, hipStream_t s);
+  void build_ri(const double* d_D, double* d_J, double* d_K, const double* d_H, double* d_F, hipStream_t s,
+                const double* d_C, int nocc);
 
   std::string method_;  // stored | direct | ri
   double thr_ = 1e-12;
   bool mixed_ = false;
   double fp32_thr_ = 1e-6;
+  bool ri_k_occ_ = false;  // RI-K from occupied orbitals when they are given
   int M_ = 0;
   std::vector<gansu_primitive_shell> prims_;
   std::vector<gansu_shell_type_info> types_;

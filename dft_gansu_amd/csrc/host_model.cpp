@@ -287,7 +287,8 @@ Params::Params() {
         // engine keys (no reference counterpart)
         {"precision", "fp64"},
         {"fp32_bound_threshold", "1e-6"},
-        {"normalization", "reference"}};
+        {"normalization", "reference"},
+        {"ri_k", "density"}};
 }
 
 static std::string lower(std::string s) {

@@ -152,7 +152,7 @@ void FockEngine::precompute_ri() {
 }
 
 void FockEngine::build_ri(const double* d_D, double* d_J, double* d_K, const double* d_H, double* d_F,
-                          hipStream_t s) {
+                          hipStream_t s, const double* d_C, int nocc) {
   RIState& r = *ri_;
   const int M = M_, Ma = Maux_;
   blas_ok(rocblas_set_stream(r.h, s), "rocblas_set_stream");
@@ -167,6 +167,9 @@ void FockEngine::build_ri(const double* d_D, double* d_J, double* d_K, const dou
   GANSU_HIP(launch_ri_j(r.B.ptr, r.W.ptr, M, Ma, J, s));
   prof_end(s);
   const double one = 1.0, zero = 0.0;
+  if (d_C && (ri_k_occ_ || mixed_)) {
+    build_ri_k_occ(d_C, nocc, K, s);
+  } else {
   for (int p0 = 0; p0 < Ma; p0 += r.pchunk) {
     const int pc = std::min(r.pchunk, Ma - p0);
     // T_m (pc x M) = B_m[p0:p0+pc] D  for every m; column-major: T_m^T = D B_m^T
@@ -184,7 +187,37 @@ void FockEngine::build_ri(const double* d_D, double* d_J, double* d_K, const dou
             "rocblas_dgemm(K)");
     prof_end(s);
   }
+  }
   if (d_F) GANSU_HIP(launch_fock_ri(d_H, J, K, (size_t)M * M, d_F, s));
+}
+
+// K = sum_P (B^P X)(B^P X)^T with X = sqrt(2) C_occ (M x nocc, row-major,
+// leading dimension M).  Per P-chunk: T_m = B_m[p0:p0+pc] X for every m
+// (batched DGEMM), then K += T_cat T_cat^T with T_cat = M x (pc nocc).
+void FockEngine::build_ri_k_occ(const double* d_C, int nocc, double* K, hipStream_t s) {
+  RIState& r = *ri_;
+  const int M = M_, Ma = Maux_;
+  const size_t need = (size_t)M * Ma * nocc;
+  const int pc_max = (int)std::max<size_t>(1, std::min<size_t>(Ma, r.T.n / ((size_t)M * nocc)));
+  (void)need;
+  const double two = std::sqrt(2.0), zero = 0.0;
+  for (int p0 = 0; p0 < Ma; p0 += pc_max) {
+    const int pc = std::min(pc_max, Ma - p0);
+    // column-major: T_m^T (nocc x pc) = X^T (nocc x M) B_m^T (M x pc); X^T is the
+    // column-major view of row-major C restricted to its first nocc columns
+    prof_begin(s, 80005, 0.0, 2.0 * M * (double)pc * M * nocc, (double)pc);
+    blas_ok(rocblas_dgemm_strided_batched(r.h, rocblas_operation_none, rocblas_operation_none, nocc, pc, M, &two, d_C,
+                                          M, 0, r.B.ptr + (size_t)p0 * M, M, (rocblas_stride)Ma * M, &zero, r.T.ptr,
+                                          nocc, (rocblas_stride)pc * nocc, M),
+            "rocblas_dgemm_strided_batched(T occ)");
+    prof_end(s);
+    prof_begin(s, 80006, 0.0, 2.0 * M * (double)M * pc * nocc, (double)pc);
+    const double one = 1.0, beta = p0 == 0 ? 0.0 : 1.0;
+    blas_ok(rocblas_dgemm(r.h, rocblas_operation_transpose, rocblas_operation_none, M, M, pc * nocc, &one, r.T.ptr,
+                          pc * nocc, r.T.ptr, pc * nocc, &beta, K, M),
+            "rocblas_dgemm(K occ)");
+    prof_end(s);
+  }
 }
 
 }  // namespace gansu

@@ -176,7 +176,8 @@ ScfResult rhf_run(const Molecule& mol, const Molecule* aux, const Params& prm) {
   std::map<std::string, std::string> opts{{"eri_method", prm.get("eri_method")},
                                           {"schwarz_screening_threshold", prm.get("schwarz_screening_threshold")},
                                           {"precision", prm.get("precision")},
-                                          {"fp32_bound_threshold", prm.get("fp32_bound_threshold")}};
+                                          {"fp32_bound_threshold", prm.get("fp32_bound_threshold")},
+                                          {"ri_k", prm.get("ri_k")}};
   const bool ri = prm.get("eri_method") == "ri";
   if (ri && !aux) throw Error(GANSU_E_ARG, "eri_method=ri needs auxiliary_gbsfilename");
   FockEngine eng(mol.prims.data(), mol.prims.size(), mol.types.data(), (int)mol.types.size(), mol.norm.data(), n,
@@ -205,7 +206,7 @@ ScfResult rhf_run(const Molecule& mol, const Molecule* aux, const Params& prm) {
                "rocblas_dgemm(D)");
   };
   auto fock = [&]() {
-    eng.build_jk(w.D.ptr, nullptr, nullptr, w.H.ptr, w.F.ptr, 0);
+    eng.build_jk(w.D.ptr, nullptr, nullptr, w.H.ptr, w.F.ptr, 0, w.C.ptr, nocc);
     ++res.fock_builds;
     if (ri) {  // rhf.hpp:665-674 checks F for NaN on the RI path
       DevBuf<int> flag(1);

@@ -55,12 +55,12 @@ class FockEngine:
     """Fock-build engine over a molecule (and an auxiliary basis for RI)."""
 
     def __init__(self, mol, aux=None, eri_method="stored", schwarz_screening_threshold=1e-12, precision="fp64",
-                 fp32_bound_threshold=1e-6):
+                 fp32_bound_threshold=1e-6, ri_k="density"):
         self.mol = mol
         self.aux = aux
         self.M = mol.n_basis
         opts = (f"eri_method={eri_method};schwarz_screening_threshold={schwarz_screening_threshold!r};"
-                f"precision={precision};fp32_bound_threshold={fp32_bound_threshold!r}")
+                f"precision={precision};fp32_bound_threshold={fp32_bound_threshold!r};ri_k={ri_k}")
         self._h = ctypes.c_void_p()
         a = aux
         check(lib().gansu_fock_create(
@@ -95,6 +95,16 @@ class FockEngine:
             F = torch.empty_like(D)
         self._check_mat(F, "F")
         check(lib().gansu_fock_build_rhf(self._h, ptr(D), ptr(H), ptr(F), self._stream(stream)))
+        return F
+
+    def compute_fock_matrix_occ(self, D, C, nocc, H, F=None, stream=None):
+        """F = H + J - K/2 with D = 2 C_occ C_occ^T given as well (C: M x M, first nocc columns)."""
+        import torch
+        for t, n in ((D, "D"), (C, "C"), (H, "H")):
+            self._check_mat(t, n)
+        if F is None:
+            F = torch.empty_like(D)
+        check(lib().gansu_fock_build_rhf_occ(self._h, ptr(D), ptr(C), int(nocc), ptr(H), ptr(F), self._stream(stream)))
         return F
 
     def compute_jk(self, D, stream=None):

@@ -116,6 +116,15 @@ int gansu_fock_precompute(gansu_fock_engine* e);
 int gansu_fock_build_rhf(gansu_fock_engine* e, const double* d_D, const double* d_H, double* d_F, void* stream);
 /* J and K separately: J_ij = sum_kl (ij|kl) D_kl, K_ij = sum_kl (ik|jl) D_kl. */
 int gansu_fock_build_jk(gansu_fock_engine* e, const double* d_D, double* d_J, double* d_K, void* stream);
+/* RHF build given the occupied orbitals as well: D = 2 C_occ C_occ^T with C
+ * the M x M row-major coefficient matrix (first nocc columns occupied), as
+ * the reference's RHF holds it (rhf.hpp coefficient_matrix).  The RI path
+ * then forms K from X = sqrt(2) C_occ, K = sum_P (B^P X)(B^P X)^T, in
+ * 4 Maux M^2 nocc flops instead of the density form's 4 Maux M^3; the
+ * four-centre paths use D only.  No counterpart in the reference, whose RI
+ * K is the density form (gpu_manager.cu:1589-1618).                          */
+int gansu_fock_build_rhf_occ(gansu_fock_engine* e, const double* d_D, const double* d_C, int nocc, const double* d_H,
+                             double* d_F, void* stream);
 /* == ERI::get_algorithm_name ("Stored", "Direct", "RI"). */
 const char* gansu_fock_algorithm_name(const gansu_fock_engine* e);
 /* Primitive-pair Schwarz factors in the reference's pair order

@@ -164,3 +164,36 @@ def test_ri_scf_energy_matches_oracle(cuda):
     # RI error against the exact 4-centre energy stays small
     e_exact = O.rhf(p, t, n, Z, xyz, 5, conv=1e-10)[1]
     assert abs(e_tot - e_exact) < 2e-3  # cc-pVDZ-RIFIT is an MP2 fitting set: ~1 mEh in HF
+
+
+def test_ri_occupied_form_equals_density_form(cuda):
+    import torch
+    Z, xyz = methane_water()
+    mol, _ = both(Z, xyz, "cc-pvdz")
+    aux, _ = both(Z, xyz, "cc-pvdz-rifit")
+    M, nocc = mol.n_basis, mol.n_electrons // 2
+    rng = np.random.default_rng(8)
+    C = rng.normal(size=(M, M)) / np.sqrt(M)
+    D = 2.0 * C[:, :nocc] @ C[:, :nocc].T
+    H = random_density(M, seed=2)
+    Dt, Ct, Ht = _torch(D, cuda), _torch(C, cuda), _torch(H, cuda)
+    e_den = FockEngine(mol, aux=aux, eri_method="ri")
+    e_den.precomputation()
+    F_den = e_den.compute_fock_matrix(Dt, Ht).cpu().numpy()
+    e_occ = FockEngine(mol, aux=aux, eri_method="ri", ri_k="occupied")
+    e_occ.precomputation()
+    F_occ = e_occ.compute_fock_matrix_occ(Dt, Ct, nocc, Ht).cpu().numpy()
+    assert np.abs(F_occ - F_den).max() <= 1e-12 * np.abs(F_den).max()
+    # without orbitals the occupied-form engine falls back to the density form
+    F_fallback = e_occ.compute_fock_matrix(Dt, Ht).cpu().numpy()
+    assert np.abs(F_fallback - F_den).max() <= 1e-12 * np.abs(F_den).max()
+    del torch
+
+
+def test_ri_scf_occupied_form_energy(cuda):
+    Z, xyz = gm.water()
+    mol, _ = both(Z, xyz, "cc-pvdz")
+    aux, _ = both(Z, xyz, "cc-pvdz-rifit")
+    r1 = rhf(mol, Parameters(eri_method="ri", convergence_energy_threshold=1e-10), aux=aux)
+    r2 = rhf(mol, Parameters(eri_method="ri", ri_k="occupied", convergence_energy_threshold=1e-10), aux=aux)
+    assert abs(r1["e_total"] - r2["e_total"]) < 1e-10
