@@ -55,6 +55,9 @@ struct FockEngine::RIState {
   DevBuf<AuxShell> shells;
   DevBuf<AuxPrim> prims;
   DevBuf<double> waux, L, B, W, J, K, T;
+  DevBuf<float> B32;  // FP32 copy of B for the J increments of precision=mixed
+  DevBuf<double> Wprev, dW, Jprev, Kpart;
+  DevBuf<int> jstate;  // {builds since FP64 J (-1: none yet), FP64 this build}
   int pchunk = 0;
   ~RIState() {
     if (h) rocblas_destroy_handle(h);
@@ -141,6 +144,16 @@ void FockEngine::precompute_ri() {
                                         rocblas_diagonal_non_unit, M, Ma, &one, r.L.ptr, Ma, 0, r.B.ptr, M,
                                         (rocblas_stride)Ma * M, M),
           "rocblas_dtrsm_strided_batched");
+  if (mixed_) {
+    r.B32.alloc(r.B.n);
+    GANSU_HIP(launch_to_f32(r.B.ptr, r.B.n, r.B32.ptr, 0));
+    r.Wprev.alloc(Ma);
+    r.dW.alloc(Ma);
+    r.Jprev.alloc((size_t)M * M);
+    GANSU_HIP(launch_zero(r.Wprev.ptr, Ma, 0));
+    const int init[2] = {-1, 1};
+    r.jstate.upload(init, 2);
+  }
   r.W.alloc(Ma);
   r.J.alloc((size_t)M * M);
   r.K.alloc((size_t)M * M);
@@ -160,16 +173,26 @@ void FockEngine::build_ri(const double* d_D, double* d_J, double* d_K, const dou
   double* J = d_J ? d_J : r.J.ptr;
   double* K = d_K ? d_K : r.K.ptr;
   const double bytesB = double(M) * Ma * M * sizeof(double);
-  prof_begin(s, 80001, bytesB, 2.0 * M * Ma * M, (double)Ma);
-  GANSU_HIP(launch_ri_w(r.B.ptr, d_D, M, Ma, r.W.ptr, s));
-  prof_end(s);
-  prof_begin(s, 80002, bytesB, 2.0 * M * Ma * M, (double)M);
-  GANSU_HIP(launch_ri_j(r.B.ptr, r.W.ptr, M, Ma, J, s));
-  prof_end(s);
-  const double one = 1.0, zero = 0.0;
-  if (d_C && (ri_k_occ_ || mixed_)) {
+  const bool occ = d_C && (ri_k_occ_ || mixed_);
+  if (occ) {
+    // K and W from the occupied-orbital intermediate T = B X
     build_ri_k_occ(d_C, nocc, K, s);
   } else {
+    prof_begin(s, 80001, bytesB, 2.0 * M * Ma * M, (double)Ma);
+    GANSU_HIP(launch_ri_w(r.B.ptr, d_D, M, Ma, r.W.ptr, s));
+    prof_end(s);
+  }
+  if (mixed_) {
+    prof_begin(s, 80012, bytesB / 2, 2.0 * M * Ma * M, (double)M);
+    GANSU_HIP(launch_ri_j_incremental(r.B.ptr, r.B32.ptr, r.W.ptr, r.Wprev.ptr, r.dW.ptr, M, Ma, 1e-2, 8,
+                                      r.jstate.ptr, r.Jprev.ptr, J, s));
+  } else {
+    prof_begin(s, 80002, bytesB, 2.0 * M * Ma * M, (double)M);
+    GANSU_HIP(launch_ri_j(r.B.ptr, r.W.ptr, M, Ma, J, s));
+  }
+  prof_end(s);
+  const double one = 1.0, zero = 0.0;
+  if (!occ) {
   for (int p0 = 0; p0 < Ma; p0 += r.pchunk) {
     const int pc = std::min(r.pchunk, Ma - p0);
     // T_m (pc x M) = B_m[p0:p0+pc] D  for every m; column-major: T_m^T = D B_m^T
@@ -201,6 +224,7 @@ void FockEngine::build_ri_k_occ(const double* d_C, int nocc, double* K, hipStrea
   const int pc_max = (int)std::max<size_t>(1, std::min<size_t>(Ma, r.T.n / ((size_t)M * nocc)));
   (void)need;
   const double two = std::sqrt(2.0), zero = 0.0;
+  bool split_mirror = false;  // K holds one SYRK triangle that still needs mirroring
   for (int p0 = 0; p0 < Ma; p0 += pc_max) {
     const int pc = std::min(pc_max, Ma - p0);
     // column-major: T_m^T (nocc x pc) = X^T (nocc x M) B_m^T (M x pc); X^T is the
@@ -211,13 +235,50 @@ void FockEngine::build_ri_k_occ(const double* d_C, int nocc, double* K, hipStrea
                                           nocc, (rocblas_stride)pc * nocc, M),
             "rocblas_dgemm_strided_batched(T occ)");
     prof_end(s);
-    prof_begin(s, 80006, 0.0, 2.0 * M * (double)M * pc * nocc, (double)pc);
-    const double one = 1.0, beta = p0 == 0 ? 0.0 : 1.0;
-    blas_ok(rocblas_dgemm(r.h, rocblas_operation_transpose, rocblas_operation_none, M, M, pc * nocc, &one, r.T.ptr,
-                          pc * nocc, r.T.ptr, pc * nocc, &beta, K, M),
-            "rocblas_dgemm(K occ)");
+    prof_begin(s, 80007, 0.0, 2.0 * M * (double)pc * nocc, (double)pc);
+    GANSU_HIP(launch_ri_w_from_t(r.T.ptr, d_C, M, pc, nocc, p0, r.W.ptr, s));
     prof_end(s);
+    const double one = 1.0, beta = p0 == 0 ? 0.0 : 1.0;
+    const int kdim = pc * nocc;
+    // K += T_cat T_cat^T.  Large M: one SYRK (half the flops).  Small M has too
+    // few output tiles for one long-k product, so k is split over a batched
+    // GEMM into partial products that a last kernel sums.
+    const int tiles = ((M + 63) / 64) * ((M + 63) / 64);
+    int nsplit = std::min(64, std::max(1, 1024 / tiles));
+    nsplit = std::min(nsplit, std::max(1, kdim / 2048));
+    if (nsplit <= 1) {
+      prof_begin(s, 80006, 0.0, 1.0 * M * (double)M * kdim, (double)pc);
+      blas_ok(rocblas_dsyrk(r.h, rocblas_fill_upper, rocblas_operation_transpose, M, kdim, &one, r.T.ptr, kdim, &beta,
+                            K, M),
+              "rocblas_dsyrk(K occ)");
+      prof_end(s);
+      split_mirror = true;
+    } else {
+      const int kb = kdim / nsplit;  // the last split takes the remainder below
+      if (r.Kpart.n < (size_t)(nsplit + 1) * M * M) r.Kpart.alloc((size_t)(nsplit + 1) * M * M);
+      prof_begin(s, 80008, 0.0, 2.0 * M * (double)M * kdim, (double)pc);
+      blas_ok(rocblas_dgemm_strided_batched(r.h, rocblas_operation_transpose, rocblas_operation_none, M, M, kb, &one,
+                                            r.T.ptr, kdim, kb, r.T.ptr, kdim, kb, &zero, r.Kpart.ptr, M,
+                                            (rocblas_stride)M * M, nsplit),
+              "rocblas_dgemm_strided_batched(K split)");
+      int nparts = nsplit;
+      const int rem = kdim - kb * nsplit;
+      if (rem > 0) {
+        blas_ok(rocblas_dgemm(r.h, rocblas_operation_transpose, rocblas_operation_none, M, M, rem, &one,
+                              r.T.ptr + (size_t)kb * nsplit, kdim, r.T.ptr + (size_t)kb * nsplit, kdim, &zero,
+                              r.Kpart.ptr + (size_t)nsplit * M * M, M),
+                "rocblas_dgemm(K split rest)");
+        nparts += 1;
+      }
+      if (split_mirror) {  // an earlier chunk wrote only one triangle
+        GANSU_HIP(launch_mirror_upper(K, M, s));
+        split_mirror = false;
+      }
+      GANSU_HIP(launch_sum_parts(r.Kpart.ptr, nparts, (size_t)M * M, p0 == 0 ? 0 : 1, K, s));
+      prof_end(s);
+    }
   }
+  if (split_mirror) GANSU_HIP(launch_mirror_upper(K, M, s));
 }
 
 }  // namespace gansu

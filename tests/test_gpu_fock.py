@@ -197,3 +197,16 @@ def test_ri_scf_occupied_form_energy(cuda):
     r1 = rhf(mol, Parameters(eri_method="ri", convergence_energy_threshold=1e-10), aux=aux)
     r2 = rhf(mol, Parameters(eri_method="ri", ri_k="occupied", convergence_energy_threshold=1e-10), aux=aux)
     assert abs(r1["e_total"] - r2["e_total"]) < 1e-10
+
+
+def test_ri_mixed_precision_energy(cuda):
+    """precision=mixed (occupied-form FP64 K; J as FP64 J of an earlier build
+    plus an FP32-B increment over the fitted-density change) stays within
+    1 micro-Eh of the FP64 density-form RI energy."""
+    Z, xyz = methane_water()
+    mol, _ = both(Z, xyz, "cc-pvdz")
+    aux, _ = both(Z, xyz, "cc-pvdz-rifit")
+    r64 = rhf(mol, Parameters(eri_method="ri", convergence_energy_threshold=1e-10), aux=aux)
+    rmx = rhf(mol, Parameters(eri_method="ri", precision="mixed", convergence_energy_threshold=1e-10), aux=aux)
+    assert rmx["converged"]
+    assert abs(rmx["e_total"] - r64["e_total"]) < 1e-6
