@@ -98,6 +98,7 @@ class FockEngine {
   void precompute_quartets();  // quartet classes, work blocks, stored integrals
   void precompute_ri();
   void build_ri_k_occ(const double* d_C, int nocc, double* K, hipStream_t s);
+  void ri_k_product(const double* A, int kdim, bool accumulate, double* K, hipStream_t s);
   void build_four_center(const double* d_D, hipStream_t s);
   void build_ri(const double* d_D, double* d_J, double* d_K, const double* d_H, double* d_F, hipStream_t s,
                 const double* d_C, int nocc);

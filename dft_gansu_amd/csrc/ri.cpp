@@ -66,6 +66,19 @@ struct FockEngine::RIState {
 
 void ri_destroy(FockEngine::RIState* p) { delete p; }
 
+// Output blocking of K = A^T A (A = kdim x M column-major): nb x nb blocks of
+// side bs, of which the upper ones (bi <= bj) are computed; each block's k is
+// split so that one product has about 1024 64x64 output tiles in flight.
+static void ri_k_blocking(int M, int* nb, int* bs) {
+  *nb = M >= 1024 ? (M + 799) / 800 : 1;
+  *bs = (M + *nb - 1) / *nb;
+}
+
+static int ri_k_max_split(int nr, int nc) {
+  const int tiles = ((nr + 63) / 64) * ((nc + 63) / 64);
+  return std::min(64, std::max(1, 1024 / tiles));
+}
+
 void FockEngine::precompute_ri() {
   ri_destroy(ri_);
   ri_ = new RIState();
@@ -161,6 +174,11 @@ void FockEngine::precompute_ri() {
   const size_t budget = size_t(4) << 30;
   r.pchunk = (int)std::max<size_t>(1, std::min<size_t>(Ma, budget / ((size_t)M * M * sizeof(double))));
   r.T.alloc((size_t)M * r.pchunk * M);
+  {
+    int nb, bs;
+    ri_k_blocking(M, &nb, &bs);
+    r.Kpart.alloc((size_t)(ri_k_max_split(bs, bs) + 1) * bs * bs);
+  }
   GANSU_HIP(hipDeviceSynchronize());
 }
 
@@ -220,11 +238,8 @@ void FockEngine::build_ri(const double* d_D, double* d_J, double* d_K, const dou
 void FockEngine::build_ri_k_occ(const double* d_C, int nocc, double* K, hipStream_t s) {
   RIState& r = *ri_;
   const int M = M_, Ma = Maux_;
-  const size_t need = (size_t)M * Ma * nocc;
   const int pc_max = (int)std::max<size_t>(1, std::min<size_t>(Ma, r.T.n / ((size_t)M * nocc)));
-  (void)need;
   const double two = std::sqrt(2.0), zero = 0.0;
-  bool split_mirror = false;  // K holds one SYRK triangle that still needs mirroring
   for (int p0 = 0; p0 < Ma; p0 += pc_max) {
     const int pc = std::min(pc_max, Ma - p0);
     // column-major: T_m^T (nocc x pc) = X^T (nocc x M) B_m^T (M x pc); X^T is the
@@ -238,47 +253,53 @@ void FockEngine::build_ri_k_occ(const double* d_C, int nocc, double* K, hipStrea
     prof_begin(s, 80007, 0.0, 2.0 * M * (double)pc * nocc, (double)pc);
     GANSU_HIP(launch_ri_w_from_t(r.T.ptr, d_C, M, pc, nocc, p0, r.W.ptr, s));
     prof_end(s);
-    const double one = 1.0, beta = p0 == 0 ? 0.0 : 1.0;
-    const int kdim = pc * nocc;
-    // K += T_cat T_cat^T.  Large M: one SYRK (half the flops).  Small M has too
-    // few output tiles for one long-k product, so k is split over a batched
-    // GEMM into partial products that a last kernel sums.
-    const int tiles = ((M + 63) / 64) * ((M + 63) / 64);
-    int nsplit = std::min(64, std::max(1, 1024 / tiles));
-    nsplit = std::min(nsplit, std::max(1, kdim / 2048));
-    if (nsplit <= 1) {
-      prof_begin(s, 80006, 0.0, 1.0 * M * (double)M * kdim, (double)pc);
-      blas_ok(rocblas_dsyrk(r.h, rocblas_fill_upper, rocblas_operation_transpose, M, kdim, &one, r.T.ptr, kdim, &beta,
-                            K, M),
-              "rocblas_dsyrk(K occ)");
-      prof_end(s);
-      split_mirror = true;
-    } else {
-      const int kb = kdim / nsplit;  // the last split takes the remainder below
-      if (r.Kpart.n < (size_t)(nsplit + 1) * M * M) r.Kpart.alloc((size_t)(nsplit + 1) * M * M);
-      prof_begin(s, 80008, 0.0, 2.0 * M * (double)M * kdim, (double)pc);
-      blas_ok(rocblas_dgemm_strided_batched(r.h, rocblas_operation_transpose, rocblas_operation_none, M, M, kb, &one,
-                                            r.T.ptr, kdim, kb, r.T.ptr, kdim, kb, &zero, r.Kpart.ptr, M,
-                                            (rocblas_stride)M * M, nsplit),
-              "rocblas_dgemm_strided_batched(K split)");
-      int nparts = nsplit;
-      const int rem = kdim - kb * nsplit;
-      if (rem > 0) {
-        blas_ok(rocblas_dgemm(r.h, rocblas_operation_transpose, rocblas_operation_none, M, M, rem, &one,
-                              r.T.ptr + (size_t)kb * nsplit, kdim, r.T.ptr + (size_t)kb * nsplit, kdim, &zero,
-                              r.Kpart.ptr + (size_t)nsplit * M * M, M),
-                "rocblas_dgemm(K split rest)");
-        nparts += 1;
+    ri_k_product(r.T.ptr, pc * nocc, p0 > 0, K, s);
+  }
+  GANSU_HIP(launch_mirror_upper(K, M, s));
+}
+
+// K (+)= A^T A into the column-major upper block triangle of K (the diagonal
+// blocks in full); the caller mirrors.  A = T_cat, kdim x M, leading dim kdim.
+void FockEngine::ri_k_product(const double* A, int kdim, bool accumulate, double* K, hipStream_t s) {
+  RIState& r = *ri_;
+  const int M = M_;
+  int nb, bs;
+  ri_k_blocking(M, &nb, &bs);
+  const double one = 1.0, zero = 0.0;
+  for (int bi = 0; bi < nb; ++bi)
+    for (int bj = bi; bj < nb; ++bj) {
+      const int r0 = bi * bs, c0 = bj * bs;
+      const int nr = std::min(bs, M - r0), nc = std::min(bs, M - c0);
+      if (nr <= 0 || nc <= 0) continue;
+      double* Kb = K + r0 + (size_t)c0 * M;
+      const int nsplit = std::min(ri_k_max_split(nr, nc), std::max(1, kdim / 2048));
+      prof_begin(s, 80008, 0.0, 2.0 * nr * (double)nc * kdim, 1.0);
+      if (nsplit <= 1) {
+        const double beta = accumulate ? 1.0 : 0.0;
+        blas_ok(rocblas_dgemm(r.h, rocblas_operation_transpose, rocblas_operation_none, nr, nc, kdim, &one,
+                              A + (size_t)r0 * kdim, kdim, A + (size_t)c0 * kdim, kdim, &beta, Kb, M),
+                "rocblas_dgemm(K block)");
+      } else {
+        const int kb = kdim / nsplit;  // the remainder goes into one more part
+        const size_t blk = (size_t)nr * nc;
+        blas_ok(rocblas_dgemm_strided_batched(r.h, rocblas_operation_transpose, rocblas_operation_none, nr, nc, kb,
+                                              &one, A + (size_t)r0 * kdim, kdim, kb, A + (size_t)c0 * kdim, kdim, kb,
+                                              &zero, r.Kpart.ptr, nr, (rocblas_stride)blk, nsplit),
+                "rocblas_dgemm_strided_batched(K split)");
+        int nparts = nsplit;
+        const int rem = kdim - kb * nsplit;
+        if (rem > 0) {
+          const size_t ko = (size_t)kb * nsplit;
+          blas_ok(rocblas_dgemm(r.h, rocblas_operation_transpose, rocblas_operation_none, nr, nc, rem, &one,
+                                A + (size_t)r0 * kdim + ko, kdim, A + (size_t)c0 * kdim + ko, kdim, &zero,
+                                r.Kpart.ptr + (size_t)nsplit * blk, nr),
+                  "rocblas_dgemm(K split rest)");
+          nparts += 1;
+        }
+        GANSU_HIP(launch_sum_parts(r.Kpart.ptr, nparts, nr, nc, accumulate ? 1 : 0, Kb, M, s));
       }
-      if (split_mirror) {  // an earlier chunk wrote only one triangle
-        GANSU_HIP(launch_mirror_upper(K, M, s));
-        split_mirror = false;
-      }
-      GANSU_HIP(launch_sum_parts(r.Kpart.ptr, nparts, (size_t)M * M, p0 == 0 ? 0 : 1, K, s));
       prof_end(s);
     }
-  }
-  if (split_mirror) GANSU_HIP(launch_mirror_upper(K, M, s));
 }
 
 }  // namespace gansu

@@ -196,13 +196,19 @@ __global__ void k_mirror_upper(double* __restrict__ K, int M) {
   if (j > i) K[id] = K[(size_t)j * M + i];
 }
 
-// Sum of split-k partial products: K = (accumulate ? K : 0) + sum_b part[b].
-__global__ void k_sum_parts(const double* __restrict__ part, int nparts, size_t n, int accumulate, double* K) {
+// Sum of split-k partial products into a column-major nr x nc block of K
+// (leading dimension ldk): K = (accumulate ? K : 0) + sum_b part[b], with
+// part[b] column-major nr x nc.
+__global__ void k_sum_parts(const double* __restrict__ part, int nparts, int nr, int nc, int accumulate, double* K,
+                            int ldk) {
+  const size_t n = (size_t)nr * nc;
   const size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (id >= n) return;
-  double s = accumulate ? K[id] : 0.0;
+  const int i = (int)(id % nr), j = (int)(id / nr);
+  double* k = K + i + (size_t)j * ldk;
+  double s = accumulate ? *k : 0.0;
   for (int b = 0; b < nparts; ++b) s += part[(size_t)b * n + id];
-  K[id] = s;
+  *k = s;
 }
 
 // Incremental J state for precision=mixed, kept on the device so a build never
@@ -324,8 +330,11 @@ hipError_t launch_to_f32(const double* x, size_t n, float* y, hipStream_t s) {
   return hipGetLastError();
 }
 
-hipError_t launch_sum_parts(const double* part, int nparts, size_t n, int accumulate, double* K, hipStream_t s) {
-  hipLaunchKernelGGL(k_sum_parts, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, part, nparts, n, accumulate, K);
+hipError_t launch_sum_parts(const double* part, int nparts, int nr, int nc, int accumulate, double* K, int ldk,
+                            hipStream_t s) {
+  const size_t n = (size_t)nr * nc;
+  hipLaunchKernelGGL(k_sum_parts, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, part, nparts, nr, nc, accumulate,
+                     K, ldk);
   return hipGetLastError();
 }
 

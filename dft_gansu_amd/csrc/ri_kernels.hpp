@@ -52,7 +52,8 @@ hipError_t launch_ri_w_from_t(const double* T, const double* C, int M, int pc, i
                               hipStream_t s);
 hipError_t launch_mirror_upper(double* K, int M, hipStream_t s);
 hipError_t launch_to_f32(const double* x, size_t n, float* y, hipStream_t s);
-hipError_t launch_sum_parts(const double* part, int nparts, size_t n, int accumulate, double* K, hipStream_t s);
+hipError_t launch_sum_parts(const double* part, int nparts, int nr, int nc, int accumulate, double* K, int ldk,
+                            hipStream_t s);
 // precision=mixed J: FP64 J when the fitted density moved by more than rel
 // (max-norm of W) or after `refresh` increments, else Jprev + FP32-B increment.
 hipError_t launch_ri_j_incremental(const double* B, const float* B32, const double* W, double* Wprev, double* dW,
