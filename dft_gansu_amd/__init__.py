@@ -5,5 +5,6 @@ include/gansu_fock.h); this package binds it with ctypes and provides the
 workload geometry generators.
 """
 from ._lib import GansuError, LIB_PATH, lib  # noqa: F401
-from .engine import FockEngine, Parameters, oneint, rhf  # noqa: F401
+from .engine import (FockEngine, Parameters, nuclear_repulsion_gradient, oneint, oneint_gradient, rhf,  # noqa: F401
+                     rhf_gradient)
 from .molecule import CONFIGS, Molecule, basis_path, config_molecule  # noqa: F401

@@ -5,15 +5,9 @@
 
 #include "engine.hpp"
 #include "gansu_fock.h"
+#include "scf.hpp"
 
 namespace gansu {
-struct ScfResult {
-  double e_elec = 0, e_total = 0;
-  int iterations = 0, fock_builds = 0, converged = 0;
-  std::vector<double> D;
-};
-ScfResult rhf_run(const Molecule& mol, const Molecule* aux, const Params& prm);
-void compute_oneint(const Molecule& mol, double* dS, double* dT, double* dV, hipStream_t s);
 
 namespace {
 thread_local std::string g_last_error;
@@ -222,6 +216,41 @@ int gansu_rhf_run(const gansu_molecule* mol, const gansu_molecule* aux_mol, cons
     energies[0] = r.e_elec;
     energies[1] = r.e_total;
     if (D_out) std::memcpy(D_out, r.D.data(), r.D.size() * sizeof(double));
+    if (info) { info[0] = r.iterations; info[1] = r.fock_builds; info[2] = r.converged; info[3] = 0; }
+  });
+}
+
+int gansu_fock_gradient_rhf(gansu_fock_engine* e, const double* d_D, int n_atoms, double* d_grad, void* stream) {
+  return guarded([&] {
+    if (!e) throw gansu::Error(GANSU_E_ARG, "null engine");
+    e->e->gradient_2e(d_D, n_atoms, d_grad, (hipStream_t)stream);
+  });
+}
+
+int gansu_oneint_gradient(const gansu_molecule* mol, const double* d_D, const double* d_W, double* d_grad,
+                          void* stream) {
+  return guarded([&] {
+    if (!mol || !d_D || !d_W || !d_grad) throw gansu::Error(GANSU_E_ARG, "null argument");
+    gansu::compute_oneint_gradient(mol->m, d_D, d_W, d_grad, (hipStream_t)stream);
+  });
+}
+
+int gansu_molecule_nuclear_repulsion_gradient(const gansu_molecule* mol, double* grad) {
+  return guarded([&] {
+    if (!mol || !grad) throw gansu::Error(GANSU_E_ARG, "null argument");
+    const auto g = gansu::nuclear_repulsion_gradient(mol->m);
+    std::memcpy(grad, g.data(), g.size() * sizeof(double));
+  });
+}
+
+int gansu_rhf_gradient(const gansu_molecule* mol, const gansu_molecule* aux_mol, const gansu_params* p,
+                       double* energies, double* grad, int* info) {
+  return guarded([&] {
+    if (!mol || !p || !energies || !grad) throw gansu::Error(GANSU_E_ARG, "null argument");
+    auto r = gansu::rhf_run(mol->m, aux_mol ? &aux_mol->m : nullptr, p->p, true);
+    energies[0] = r.e_elec;
+    energies[1] = r.e_total;
+    std::memcpy(grad, r.grad.data(), r.grad.size() * sizeof(double));
     if (info) { info[0] = r.iterations; info[1] = r.fock_builds; info[2] = r.converged; info[3] = 0; }
   });
 }

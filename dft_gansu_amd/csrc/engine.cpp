@@ -237,6 +237,8 @@ void FockEngine::precompute_pairs() {
     GANSU_HIP(launch_pair_data(d_prims_.ptr, dpa.ptr, dpb.ptr, dq.ptr, ppa.size(), d_pr_.ptr, 0));
     GANSU_HIP(hipDeviceSynchronize());
   }
+  pair_alpha_h_.resize(ppa.size());
+  for (size_t i = 0; i < ppa.size(); ++i) pair_alpha_h_[i] = prims_[ppa[i]].exponent;
   d_sp_.upload(sp_h_);
 }
 

@@ -92,6 +92,9 @@ class FockEngine {
   int num_basis() const { return M_; }
   void set_profiling(bool on);
   std::vector<double> last_profile();  // 5 values per launch
+  // Two-electron part of the RHF nuclear gradient for density D (device,
+  // M x M): d_grad (device, 3 natoms) = sum Gamma(D) d(ab|cd)/dX, overwritten.
+  void gradient_2e(const double* d_D, int natoms, double* d_grad, hipStream_t s);
 
  private:
   void precompute_pairs();     // Schwarz factors, shell pairs, primitive-pair records
@@ -123,6 +126,9 @@ class FockEngine {
   size_t ref_pairs_ = 0;
   std::vector<double> Qref_h_;
   DevBuf<PairRec> d_pr_;
+  std::vector<double> pair_alpha_h_;  // exponent of the first primitive of each pair record
+  DevBuf<double> d_alpha_;
+  DevBuf<int> d_shell_atom_;
   DevBuf<SPair> d_sp_;
   std::vector<SPair> sp_h_;
   std::vector<PairClass> pclass_;

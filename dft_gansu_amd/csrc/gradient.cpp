@@ -1,0 +1,41 @@
+// Two-electron part of the RHF nuclear gradient on the engine's four-centre
+// work lists (the direct Fock build's class-sorted blocks and Schwarz bounds).
+#include <algorithm>
+#include <string>
+
+#include "engine.hpp"
+#include "gradient_kernels.hpp"
+
+namespace gansu {
+
+void FockEngine::gradient_2e(const double* d_D, int natoms, double* d_grad, hipStream_t s) {
+  if (!precomputed_) throw Error(GANSU_E_STATE, "gradient before precompute");
+  if (method_ == "ri") throw Error(GANSU_E_UNSUPPORTED, "the nuclear gradient is provided for eri_method=stored|direct");
+  if (!d_D || !d_grad) throw Error(GANSU_E_ARG, "null device pointer");
+  int max_atom = -1;
+  for (const auto& sh : shells_) max_atom = std::max(max_atom, sh.atom);
+  if (natoms <= max_atom || natoms <= 0) throw Error(GANSU_E_ARG, "n_atoms smaller than the atoms the basis sits on");
+  if (natoms > kGradMaxAtoms)
+    throw Error(GANSU_E_UNSUPPORTED, "gradient limited to " + std::to_string(kGradMaxAtoms) + " atoms");
+  if (!d_alpha_.ptr) d_alpha_.upload(pair_alpha_h_);
+  if (!d_shell_atom_.ptr) {
+    std::vector<int> sa(shells_.size());
+    for (size_t i = 0; i < shells_.size(); ++i) sa[i] = shells_[i].atom;
+    d_shell_atom_.upload(sa);
+  }
+  prof_clear();
+  GANSU_HIP(launch_zero(d_grad, 3 * (size_t)natoms, s));
+  GANSU_HIP(launch_scale_density(d_D, d_w_.ptr, M_, d_Dp_.ptr, s));
+  const Grad2eArgs g{d_sp_.ptr, d_pr_.ptr, d_alpha_.ptr, boys_.ptr, thr_, d_Dp_.ptr, M_,
+                     d_shell_atom_.ptr, natoms, d_grad};
+  for (auto& qc : qclass_) {
+    if (qc->blocks_h.empty()) continue;
+    prof_begin(s, 400000.0 + 1000 * qc->LA + 100 * qc->LB + 10 * qc->LC + qc->LD, 0.0, 0.0, qc->n_quartets);
+    const ClassLaunch c{qc->LA, qc->LB, qc->LC, qc->LD, qc->blocks.ptr, (int)qc->blocks_h.size(),
+                        pclass_[qc->A].list.ptr, pclass_[qc->B].list.ptr, nullptr};
+    GANSU_HIP(launch_grad_2e(c, g, s));
+    prof_end(s);
+  }
+}
+
+}  // namespace gansu

@@ -1,0 +1,36 @@
+// Backward (nuclear-gradient) kernels: device layouts and launchers.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstddef>
+
+#include "fock_kernels.hpp"
+#include "scf_kernels.hpp"
+
+namespace gansu {
+
+// Two-electron part: grad_X += sum_quartets deg * sum_abcd Gamma_abcd d(ab|cd)/dX
+// with Gamma = 1/2 D_ab D_cd - 1/8 (D_ac D_bd + D_ad D_bc) on the raw-basis
+// density.
+struct Grad2eArgs {
+  const SPair* sp;
+  const PairRec* pr;
+  const double* alpha;  // exponent of the first primitive of each pair record
+  const double* boys;
+  double thr;
+  const double* Dp;  // raw-basis density (M x M)
+  int M;
+  const int* shell_atom;
+  int natoms;
+  double* grad;  // 3 natoms, accumulated
+};
+
+// Largest atom count whose gradient accumulator fits the LDS of one workgroup.
+constexpr int kGradMaxAtoms = 2048;  // 48 KiB of LDS
+
+hipError_t launch_grad_2e(const ClassLaunch& c, const Grad2eArgs& a, hipStream_t s);
+
+// One-electron part: grad += sum D d(T+V)/dX - sum W dS/dX, both matrices in
+// the caller's basis (the kernel applies the raw-basis factors w).
+hipError_t launch_grad_1e(const OneIntArgs& a, const double* D, const double* W, double* grad, hipStream_t s);
+
+}  // namespace gansu

@@ -18,8 +18,8 @@
 #include <cstring>
 #include <limits>
 
-#include "engine.hpp"
-#include "scf_kernels.hpp"
+#include "gradient_kernels.hpp"
+#include "scf.hpp"
 
 namespace gansu {
 
@@ -104,52 +104,73 @@ std::vector<double> solve_small(int n, std::vector<double> A, std::vector<double
 
 }  // namespace
 
-// Build S, T, V of a molecule on the device.
-void compute_oneint(const Molecule& mol, double* dS, double* dT, double* dV, hipStream_t s) {
+OneIntHost::OneIntHost(const Molecule& mol) : M(mol.nbasis), natoms((int)mol.atoms.size()) {
   const auto shells = contracted_shells(mol.prims.data(), mol.prims.size(), kMaxL);
-  std::vector<int> off{0}, sp, sl, sb, bfs(mol.nbasis, -1);
-  std::vector<double> w(mol.nbasis);
+  std::vector<int> o{0}, sp, l, bs, bf(mol.nbasis, -1);
+  std::vector<double> wv(mol.nbasis);
   for (size_t i = 0; i < shells.size(); ++i) {
     for (int p : shells[i].prims) sp.push_back(p);
-    off.push_back((int)sp.size());
-    sl.push_back(shells[i].L);
-    sb.push_back(shells[i].basis);
-    for (int c = 0; c < ncart(shells[i].L); ++c) bfs[shells[i].basis + c] = (int)i;
+    o.push_back((int)sp.size());
+    l.push_back(shells[i].L);
+    bs.push_back(shells[i].basis);
+    for (int c = 0; c < ncart(shells[i].L); ++c) bf[shells[i].basis + c] = (int)i;
   }
   for (int mu = 0; mu < mol.nbasis; ++mu) {
-    if (bfs[mu] < 0) throw Error(GANSU_E_ARG, "basis function without shell");
-    const int L = sl[bfs[mu]], c = mu - sb[bfs[mu]];
+    if (bf[mu] < 0) throw Error(GANSU_E_ARG, "basis function without shell");
+    const int L = l[bf[mu]], c = mu - bs[bf[mu]];
     // same factor as the engine: cgto norm / sqrt((2l-1)!!(2m-1)!!(2n-1)!!)
     int e[3];
     static const int d2[6][3] = {{2, 0, 0}, {0, 2, 0}, {0, 0, 2}, {1, 1, 0}, {1, 0, 1}, {0, 1, 1}};
     for (int d = 0; d < 3; ++d) e[d] = L == 0 ? 0 : (L == 1 ? (c == d) : d2[c][d]);
     double df = 1.0;
     for (int d = 0; d < 3; ++d) df *= (e[d] == 2 ? 3.0 : 1.0);
-    w[mu] = mol.norm[mu] / std::sqrt(df);
+    wv[mu] = mol.norm[mu] / std::sqrt(df);
   }
-  std::vector<double> xyz;
-  std::vector<int> Z;
-  for (const auto& a : mol.atoms) { xyz.insert(xyz.end(), a.xyz, a.xyz + 3); Z.push_back(a.Z); }
-  DevBuf<gansu_primitive_shell> dp;
-  dp.upload(mol.prims);
-  DevBuf<int> doff, dsp, dsl, dsb, dbf, dZ;
-  DevBuf<double> dw, dxyz, boys;
-  doff.upload(off); dsp.upload(sp); dsl.upload(sl); dsb.upload(sb); dbf.upload(bfs); dZ.upload(Z);
-  dw.upload(w); dxyz.upload(xyz);
+  std::vector<double> x;
+  std::vector<int> z;
+  for (const auto& a : mol.atoms) { x.insert(x.end(), a.xyz, a.xyz + 3); z.push_back(a.Z); }
+  prims.upload(mol.prims);
+  off.upload(o); sprims.upload(sp); sl.upload(l); sb.upload(bs); bfs.upload(bf); Z.upload(z);
+  w.upload(wv); xyz.upload(x);
   boys.upload(boys_grid_xn());
-  OneIntArgs a{dp.ptr, doff.ptr, dsp.ptr, dsl.ptr, dsb.ptr, dbf.ptr, dw.ptr, mol.nbasis, (int)Z.size(),
-               dxyz.ptr, dZ.ptr, boys.ptr, dS, dT, dV};
-  GANSU_HIP(launch_oneint(a, s));
+}
+
+OneIntArgs OneIntHost::args(double* S, double* T, double* V) const {
+  return OneIntArgs{prims.ptr, off.ptr, sprims.ptr, sl.ptr, sb.ptr, bfs.ptr, w.ptr, M, natoms,
+                    xyz.ptr, Z.ptr, boys.ptr, S, T, V};
+}
+
+// Build S, T, V of a molecule on the device.
+void compute_oneint(const Molecule& mol, double* dS, double* dT, double* dV, hipStream_t s) {
+  OneIntHost h(mol);
+  GANSU_HIP(launch_oneint(h.args(dS, dT, dV), s));
   GANSU_HIP(hipStreamSynchronize(s));
 }
 
-struct ScfResult {
-  double e_elec = 0, e_total = 0;
-  int iterations = 0, fock_builds = 0, converged = 0;
-  std::vector<double> D;
-};
+void compute_oneint_gradient(const Molecule& mol, const double* dD, const double* dW, double* dgrad, hipStream_t s) {
+  OneIntHost h(mol);
+  if (h.natoms > kGradMaxAtoms)
+    throw Error(GANSU_E_UNSUPPORTED, "gradient limited to " + std::to_string(kGradMaxAtoms) + " atoms");
+  GANSU_HIP(launch_zero(dgrad, 3 * (size_t)h.natoms, s));
+  GANSU_HIP(launch_grad_1e(h.args(nullptr, nullptr, nullptr), dD, dW, dgrad, s));
+  GANSU_HIP(hipStreamSynchronize(s));
+}
 
-ScfResult rhf_run(const Molecule& mol, const Molecule* aux, const Params& prm) {
+std::vector<double> nuclear_repulsion_gradient(const Molecule& mol) {
+  const size_t n = mol.atoms.size();
+  std::vector<double> g(3 * n, 0.0);
+  for (size_t a = 0; a < n; ++a)
+    for (size_t b = 0; b < n; ++b) {
+      if (a == b) continue;
+      double r[3], r2 = 0.0;
+      for (int d = 0; d < 3; ++d) { r[d] = mol.atoms[a].xyz[d] - mol.atoms[b].xyz[d]; r2 += r[d] * r[d]; }
+      const double f = double(mol.atoms[a].Z) * mol.atoms[b].Z / (r2 * std::sqrt(r2));
+      for (int d = 0; d < 3; ++d) g[3 * a + d] -= f * r[d];
+    }
+  return g;
+}
+
+ScfResult rhf_run(const Molecule& mol, const Molecule* aux, const Params& prm, bool gradient) {
   if (prm.get("method") != "rhf") throw Error(GANSU_E_PARAM, "only method=rhf is provided by this engine");
   if (mol.n_electrons % 2 != 0) throw Error(GANSU_E_ELECTRONS, "RHF requires an even number of electrons");
   const int n = mol.nbasis;
@@ -302,6 +323,19 @@ ScfResult rhf_run(const Molecule& mol, const Molecule* aux, const Params& prm) {
   res.e_total = E + mol.nuclear_repulsion();
   res.iterations = it;
   res.D = w.D.download();
+  if (gradient) {
+    // W = 2 C_occ eps C_occ^T = D F D / 2 for the converged pair (D, F(D))
+    blas.gemm(false, false, n, w.F.ptr, w.D.ptr, w.tmp.ptr);
+    blas.gemm(false, false, n, w.D.ptr, w.tmp.ptr, w.tmp2.ptr, 0.5);
+    const int na = (int)mol.atoms.size();
+    DevBuf<double> g1(3 * (size_t)na), g2(3 * (size_t)na);
+    compute_oneint_gradient(mol, w.D.ptr, w.tmp2.ptr, g1.ptr, 0);
+    eng.gradient_2e(w.D.ptr, na, g2.ptr, 0);
+    GANSU_HIP(hipDeviceSynchronize());
+    const auto h1 = g1.download(), h2 = g2.download();
+    res.grad = nuclear_repulsion_gradient(mol);
+    for (size_t i = 0; i < res.grad.size(); ++i) res.grad[i] += h1[i] + h2[i];
+  }
   return res;
 }
 

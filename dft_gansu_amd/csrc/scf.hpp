@@ -1,0 +1,39 @@
+// RHF driver, one-electron integrals and the nuclear gradient on the host side.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "engine.hpp"
+#include "scf_kernels.hpp"
+
+namespace gansu {
+
+struct ScfResult {
+  double e_elec = 0, e_total = 0;
+  int iterations = 0, fock_builds = 0, converged = 0;
+  std::vector<double> D;     // final density (M x M)
+  std::vector<double> grad;  // dE/dX (3 natoms), when requested
+};
+
+// HF::single_point_energy (hf.cu:245-314) for RHF; with gradient = true the
+// analytic nuclear gradient of the converged energy follows.
+ScfResult rhf_run(const Molecule& mol, const Molecule* aux, const Params& prm, bool gradient = false);
+
+// Device copies of a molecule's shell tables for the one-electron kernels.
+struct OneIntHost {
+  DevBuf<gansu_primitive_shell> prims;
+  DevBuf<int> off, sprims, sl, sb, bfs, Z;
+  DevBuf<double> w, xyz, boys;
+  int M = 0, natoms = 0;
+  explicit OneIntHost(const Molecule& mol);
+  OneIntArgs args(double* S, double* T, double* V) const;
+};
+
+void compute_oneint(const Molecule& mol, double* dS, double* dT, double* dV, hipStream_t s);
+// d_grad (device, 3 natoms) = sum D d(T+V)/dX - sum W dS/dX, overwritten.
+void compute_oneint_gradient(const Molecule& mol, const double* dD, const double* dW, double* dgrad, hipStream_t s);
+// dV_nn/dX (host, 3 natoms)
+std::vector<double> nuclear_repulsion_gradient(const Molecule& mol);
+
+}  // namespace gansu

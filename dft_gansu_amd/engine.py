@@ -142,6 +142,16 @@ class FockEngine:
         check(lib().gansu_fock_last_profile(self._h, ptr(rows), n.value, ctypes.byref(n)))
         return rows[:n.value]
 
+    def gradient_2e(self, D, n_atoms=None, stream=None):
+        """Two-electron term of the RHF nuclear gradient, sum Gamma(D) d(ab|cd)/dX,
+        as an (n_atoms, 3) float64 CUDA tensor."""
+        import torch
+        self._check_mat(D, "D")
+        n = int(n_atoms if n_atoms is not None else self.mol.n_atoms)
+        g = torch.empty((n, 3), dtype=torch.float64, device=D.device)
+        check(lib().gansu_fock_gradient_rhf(self._h, ptr(D), n, ptr(g), self._stream(stream)))
+        return g
+
     def __del__(self):
         h = getattr(self, "_h", None)
         if h is not None and h.value:
@@ -162,6 +172,34 @@ def oneint(mol):
     return S, T, V
 
 
+def oneint_gradient(mol, D, W, stream=None):
+    """sum D d(T+V)/dX - sum W dS/dX as an (n_atoms, 3) float64 CUDA tensor."""
+    import torch
+    g = torch.empty((mol.n_atoms, 3), dtype=torch.float64, device=D.device)
+    for t in (D, W):
+        if not (t.is_cuda and t.dtype == torch.float64 and t.is_contiguous() and tuple(t.shape) == (mol.n_basis,) * 2):
+            raise ValueError("D and W must be contiguous float64 CUDA tensors of shape (M, M)")
+    check(lib().gansu_oneint_gradient(mol.handle, ptr(D), ptr(W), ptr(g), FockEngine._stream(stream)))
+    return g
+
+
+def nuclear_repulsion_gradient(mol):
+    g = np.zeros((mol.n_atoms, 3))
+    check(lib().gansu_molecule_nuclear_repulsion_gradient(mol.handle, ptr(g)))
+    return g
+
+
+def rhf_gradient(mol, params, aux=None):
+    """SCF then the analytic nuclear gradient; returns dict(e_elec, e_total,
+    iterations, fock_builds, converged, grad (n_atoms x 3, hartree/bohr))."""
+    en = np.zeros(2)
+    g = np.zeros((mol.n_atoms, 3))
+    info = np.zeros(4, dtype=np.int32)
+    check(lib().gansu_rhf_gradient(mol.handle, aux.handle if aux else None, params.handle, ptr(en), ptr(g), ptr(info)))
+    return dict(e_elec=float(en[0]), e_total=float(en[1]), iterations=int(info[0]), fock_builds=int(info[1]),
+                converged=bool(info[2]), grad=g)
+
+
 def rhf(mol, params, aux=None):
     """Run the RHF SCF; returns dict(e_elec, e_total, iterations, fock_builds, converged, D)."""
     en = np.zeros(2)
@@ -172,4 +210,5 @@ def rhf(mol, params, aux=None):
                 converged=bool(info[2]), D=D)
 
 
-__all__ = ["FockEngine", "Parameters", "oneint", "rhf", "PRIM_DTYPE"]
+__all__ = ["FockEngine", "Parameters", "oneint", "rhf", "oneint_gradient", "nuclear_repulsion_gradient",
+           "rhf_gradient", "PRIM_DTYPE"]

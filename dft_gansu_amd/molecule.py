@@ -49,7 +49,7 @@ class Molecule:
         h = ctypes.c_void_p()
         check(lib().gansu_molecule_from_xyz(xyz_path.encode(), gbs_path.encode(), charge, beta_to_alpha,
                                             _norm_code(normalization), ctypes.byref(h)))
-        return cls(h)
+        return cls(h)._remember(gbs_path, charge, beta_to_alpha, normalization)
 
     @classmethod
     def from_atoms(cls, Z, xyz_bohr, gbs_path, charge=0, beta_to_alpha=0, normalization="reference"):
@@ -58,7 +58,17 @@ class Molecule:
         h = ctypes.c_void_p()
         check(lib().gansu_molecule_from_atoms(len(Z), ptr(Z), ptr(xyz), gbs_path.encode(), charge, beta_to_alpha,
                                               _norm_code(normalization), ctypes.byref(h)))
-        return cls(h)
+        return cls(h)._remember(gbs_path, charge, beta_to_alpha, normalization)
+
+    def _remember(self, gbs_path, charge, beta_to_alpha, normalization):
+        self.gbs_path, self.charge = gbs_path, charge
+        self.beta_to_alpha, self.normalization = beta_to_alpha, normalization
+        return self
+
+    def with_coordinates(self, xyz_bohr):
+        """The same atoms, basis and charge at new positions (bohr, n_atoms x 3)."""
+        return Molecule.from_atoms(self.Z, xyz_bohr, self.gbs_path, self.charge, self.beta_to_alpha,
+                                   self.normalization)
 
     @property
     def handle(self):

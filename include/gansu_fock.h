@@ -156,6 +156,31 @@ int gansu_oneint(const gansu_molecule* mol, double* d_S, double* d_T, double* d_
 int gansu_rhf_run(const gansu_molecule* mol, const gansu_molecule* aux_mol, const gansu_params* p, double* energies,
                   double* D_out, int* info);
 
+/* ------------------------------------------------------ nuclear gradient --
+ * Analytic RHF gradient dE/dX (hartree/bohr, atom-major x,y,z).  No
+ * counterpart in the reference, which has no gradient; the terms follow the
+ * RHF energy of hf.cu:245-314:
+ *   dE/dX = sum D d(T+V)/dX - sum W dS/dX
+ *         + sum Gamma_abcd d(ab|cd)/dX + dV_nn/dX,
+ *   Gamma_abcd = 1/2 D_ab D_cd - 1/8 (D_ac D_bd + D_ad D_bc),
+ *   W = 2 C_occ eps_occ C_occ^T (= D F D / 2 at convergence).
+ * Device outputs are overwritten; calls are ordered on `stream`.            */
+/* Two-electron term for an engine with eri_method=stored|direct, after
+ * precompute; d_grad has 3*n_atoms entries.                                 */
+int gansu_fock_gradient_rhf(gansu_fock_engine* e, const double* d_D, int n_atoms, double* d_grad, void* stream);
+/* One-electron and overlap terms (kinetic, nuclear attraction including the
+ * Hellmann-Feynman term of every nucleus, -W dS).                          */
+int gansu_oneint_gradient(const gansu_molecule* mol, const double* d_D, const double* d_W, double* d_grad,
+                          void* stream);
+/* dV_nn/dX into host grad[3*n_atoms]. */
+int gansu_molecule_nuclear_repulsion_gradient(const gansu_molecule* mol, double* grad);
+/* SCF (as gansu_rhf_run) then the full gradient into host grad[3*n_atoms];
+ * energies[2] and info[4] as gansu_rhf_run.  Converge tightly
+ * (convergence_energy_threshold <= 1e-10): the gradient is exact only at the
+ * SCF solution.                                                             */
+int gansu_rhf_gradient(const gansu_molecule* mol, const gansu_molecule* aux_mol, const gansu_params* p,
+                       double* energies, double* grad, int* info);
+
 #ifdef __cplusplus
 }
 #endif

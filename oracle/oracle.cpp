@@ -1078,3 +1078,211 @@ double orc_eri_prim_md(const double* ctr, const double* ex, const int* ang) {
 int orc_num_threads() { return omp_get_max_threads(); }
 
 }  // extern "C"
+
+// ---- analytic RHF nuclear gradient (FP64 host) ---------------------------
+// The reference has no gradient; this is the oracle of the engine's backward
+// kernels.  Every centre derivative of a Cartesian Gaussian is a two-term
+// combination of Gaussians with one exponent raised or lowered:
+//   d/dA_x [x_A^i exp(-a x_A^2)] = 2a x_A^(i+1) exp(..) - i x_A^(i-1) exp(..)
+// with x_A = x - A_x, so each derivative integral is formed from integrals of
+// the same primitive pair with shifted angular momentum (E tables built one
+// order higher, R one order higher).  Energy (RHF, D = 2 C_occ C_occ^T):
+//   E = sum D h + 1/2 sum D_ab D_cd (ab|cd) - 1/4 sum D_ac D_bd (ab|cd) + V_nn
+//   dE/dX = sum D dh/dX + sum Gamma_abcd d(ab|cd)/dX - sum W dS/dX + dV_nn/dX
+// with Gamma_abcd = 1/2 D_ab D_cd - 1/8 (D_ac D_bd + D_ad D_bc) summed over all
+// ordered index quadruples, W = 2 C_occ eps C_occ^T (= D F D / 2 at convergence).
+
+// Raw primitive (ab|cd) for explicit exponent vectors (E tables of ab/cd must
+// reach the requested orders; R must reach sum of all orders).
+static double md_raw4(const PairData& ab, const PairData& cd, const Rtab& R, const int* A, const int* B, const int* C,
+                      const int* D) {
+  double s = 0;
+  for (int t = 0; t <= A[0] + B[0]; ++t)
+    for (int u = 0; u <= A[1] + B[1]; ++u)
+      for (int v = 0; v <= A[2] + B[2]; ++v) {
+        const double eab = ab.E[0].e[A[0]][B[0]][t] * ab.E[1].e[A[1]][B[1]][u] * ab.E[2].e[A[2]][B[2]][v];
+        if (eab == 0.0) continue;
+        for (int tt = 0; tt <= C[0] + D[0]; ++tt)
+          for (int uu = 0; uu <= C[1] + D[1]; ++uu)
+            for (int vv = 0; vv <= C[2] + D[2]; ++vv) {
+              const double ecd = cd.E[0].e[C[0]][D[0]][tt] * cd.E[1].e[C[1]][D[1]][uu] * cd.E[2].e[C[2]][D[2]][vv];
+              const double sg = ((tt + uu + vv) & 1) ? -1.0 : 1.0;
+              s += eab * sg * ecd * R.r[t + tt][u + uu][v + vv];
+            }
+      }
+  return s;
+}
+
+extern "C" {
+
+// Two-electron part: grad[3*natoms] = sum Gamma(D) d(ab|cd)/dX over the same
+// screened primitive quartets as orc_jk (thr applies to Q_x Q_y).
+int orc_grad_2e(const void* vprims, const void* vtypes, int ntypes, const double* norm, int M, const double* Q,
+                double thr, const double* D, int natoms, double* grad) {
+  const Prim* P = (const Prim*)vprims;
+  auto pl = pair_list(P, (const TypeInfo*)vtypes, ntypes, Q);
+  const size_t n = pl.size(), MM = M;
+  const int nt = omp_get_max_threads();
+  std::vector<std::vector<double>> gt(nt, std::vector<double>((size_t)3 * natoms, 0.0));
+#pragma omp parallel
+  {
+    double* gl = gt[omp_get_thread_num()].data();
+#pragma omp for schedule(dynamic, 1)
+    for (long long xx = 0; xx < (long long)n; ++xx) {
+      const size_t x = (size_t)xx;
+      for (size_t y = x; y < n; ++y) {
+        if (pl[x].Q * pl[y].Q < thr) continue;
+        const Prim* f[4] = {&P[pl[x].a], &P[pl[x].b], &P[pl[y].a], &P[pl[y].b]};
+        int l[4];
+        for (int k = 0; k < 4; ++k) l[k] = f[k]->shell_type;
+        PairData ab, cd;
+        make_pair(*f[0], l[0] + 1, *f[1], l[1] + 1, ab);
+        make_pair(*f[2], l[2] + 1, *f[3], l[3] + 1, cd);
+        const double p = ab.p, q = cd.p;
+        Rtab R;
+        hermite_r(l[0] + l[1] + l[2] + l[3] + 1, p * q / (p + q), ab.P[0] - cd.P[0], ab.P[1] - cd.P[1],
+                  ab.P[2] - cd.P[2], R);
+        const double pref = 2.0 * std::pow(M_PI, 2.5) / (p * q * std::sqrt(p + q)) * ab.K * cd.K;
+        const bool sb = pl[x].a == pl[x].b, sk = pl[y].a == pl[y].b, sbk = x == y;
+        const double deg = (sb ? 1.0 : 2.0) * (sk ? 1.0 : 2.0) * (sbk ? 1.0 : 2.0);
+        const double cf = f[0]->coefficient * f[1]->coefficient * f[2]->coefficient * f[3]->coefficient;
+        for (int i0 = 0; i0 < ncart(l[0]); ++i0)
+          for (int i1 = 0; i1 < ncart(l[1]); ++i1)
+            for (int i2 = 0; i2 < ncart(l[2]); ++i2)
+              for (int i3 = 0; i3 < ncart(l[3]); ++i3) {
+                const int ic[4] = {i0, i1, i2, i3};
+                size_t bf[4];
+                int e[4][3];
+                double nrm = 1.0;
+                for (int k = 0; k < 4; ++k) {
+                  const int* c = cart(l[k], ic[k], false);
+                  for (int d = 0; d < 3; ++d) e[k][d] = c[d];
+                  bf[k] = f[k]->basis_index + ic[k];
+                  nrm *= prim_norm(f[k]->exponent, c[0], c[1], c[2]) * norm[bf[k]];
+                }
+                const size_t pp = bf[0], qq = bf[1], rr = bf[2], ss = bf[3];
+                const double gam = 0.5 * D[pp * MM + qq] * D[rr * MM + ss] -
+                                   0.125 * (D[pp * MM + rr] * D[qq * MM + ss] + D[pp * MM + ss] * D[qq * MM + rr]);
+                if (gam == 0.0) continue;
+                const double fac = deg * cf * nrm * gam * pref;
+                for (int k = 0; k < 4; ++k)
+                  for (int d = 0; d < 3; ++d) {
+                    const int e0 = e[k][d];
+                    e[k][d] = e0 + 1;
+                    double v = 2.0 * f[k]->exponent * md_raw4(ab, cd, R, e[0], e[1], e[2], e[3]);
+                    if (e0 > 0) {
+                      e[k][d] = e0 - 1;
+                      v -= e0 * md_raw4(ab, cd, R, e[0], e[1], e[2], e[3]);
+                    }
+                    e[k][d] = e0;
+                    gl[3 * f[k]->atom_index + d] += fac * v;
+                  }
+              }
+      }
+    }
+  }
+  for (int i = 0; i < 3 * natoms; ++i) {
+    double s = 0;
+    for (int t = 0; t < nt; ++t) s += gt[t][i];
+    grad[i] = s;
+  }
+  return 0;
+}
+
+// One-electron part: grad = sum D d(T+V)/dX - sum W dS/dX, including the
+// Hellmann-Feynman term of each nucleus (by translational invariance of every
+// single-nucleus attraction integral, d/dC = -(d/dA + d/dB)).
+int orc_grad_1e(const void* vprims, int nprims, const double* norm, int M, int natoms, const int* Z, const double* xyz,
+                const double* D, const double* W, double* grad) {
+  const Prim* P = (const Prim*)vprims;
+  const size_t MM = M;
+  std::fill(grad, grad + 3 * natoms, 0.0);
+  std::vector<Rtab> Rn(natoms);
+  std::vector<double> vc(natoms);
+  for (int ia = 0; ia < nprims; ++ia)
+    for (int ib = 0; ib < nprims; ++ib) {
+      const Prim &a = P[ia], &b = P[ib];
+      const int la = a.shell_type, lb = b.shell_type;
+      PairData pd;
+      make_pair(a, la + 1, b, lb + 3, pd);  // shifted orders; +2 more on b for the kinetic term
+      const double p = pd.p, bb = b.exponent;
+      for (int c = 0; c < natoms; ++c)
+        hermite_r(la + lb + 1, p, pd.P[0] - xyz[3 * c], pd.P[1] - xyz[3 * c + 1], pd.P[2] - xyz[3 * c + 2], Rn[c]);
+      const double sp = std::pow(M_PI / p, 1.5) * pd.K, vp = 2.0 * M_PI / p * pd.K;
+      // S, T and the per-nucleus V of one (possibly shifted) component pair
+      auto stv = [&](const int* A, const int* B, double& s, double& t) {
+        double s1[3], t1[3];
+        for (int d = 0; d < 3; ++d) {
+          const int i = A[d], j = B[d];
+          const double sij = pd.E[d].e[i][j][0];
+          const double sp2 = pd.E[d].e[i][j + 2][0];
+          const double sm2 = j >= 2 ? pd.E[d].e[i][j - 2][0] : 0.0;
+          s1[d] = sij;
+          t1[d] = -2.0 * bb * bb * sp2 + bb * (2 * j + 1) * sij - 0.5 * j * (j - 1) * sm2;
+        }
+        s = s1[0] * s1[1] * s1[2] * sp;
+        t = (t1[0] * s1[1] * s1[2] + s1[0] * t1[1] * s1[2] + s1[0] * s1[1] * t1[2]) * sp;
+        for (int c = 0; c < natoms; ++c) {
+          double acc = 0;
+          for (int tt = 0; tt <= A[0] + B[0]; ++tt)
+            for (int u = 0; u <= A[1] + B[1]; ++u)
+              for (int w = 0; w <= A[2] + B[2]; ++w)
+                acc += pd.E[0].e[A[0]][B[0]][tt] * pd.E[1].e[A[1]][B[1]][u] * pd.E[2].e[A[2]][B[2]][w] *
+                       Rn[c].r[tt][u][w];
+          vc[c] = -Z[c] * acc * vp;
+        }
+      };
+      for (int ca = 0; ca < ncart(la); ++ca)
+        for (int cb = 0; cb < ncart(lb); ++cb) {
+          int e[2][3];
+          const int *A = cart(la, ca, 0), *B = cart(lb, cb, 0);
+          for (int d = 0; d < 3; ++d) { e[0][d] = A[d]; e[1][d] = B[d]; }
+          const size_t p_ = a.basis_index + ca, q_ = b.basis_index + cb;
+          const double nn = prim_norm(a.exponent, A[0], A[1], A[2]) * prim_norm(b.exponent, B[0], B[1], B[2]) *
+                            a.coefficient * b.coefficient * norm[p_] * norm[q_];
+          const double dpq = D[p_ * MM + q_] * nn, wpq = W[p_ * MM + q_] * nn;
+          const Prim* fk[2] = {&a, &b};
+          for (int k = 0; k < 2; ++k)
+            for (int d = 0; d < 3; ++d) {
+              const int e0 = e[k][d];
+              double s, t, ds, dt;
+              e[k][d] = e0 + 1;
+              stv(e[0], e[1], s, t);
+              const double up = 2.0 * fk[k]->exponent;
+              ds = up * s; dt = up * t;
+              std::vector<double> dv(natoms);
+              for (int c = 0; c < natoms; ++c) dv[c] = up * vc[c];
+              if (e0 > 0) {
+                e[k][d] = e0 - 1;
+                stv(e[0], e[1], s, t);
+                ds -= e0 * s; dt -= e0 * t;
+                for (int c = 0; c < natoms; ++c) dv[c] -= e0 * vc[c];
+              }
+              e[k][d] = e0;
+              double vsum = 0;
+              for (int c = 0; c < natoms; ++c) {
+                vsum += dv[c];
+                grad[3 * c + d] -= dpq * dv[c];  // nucleus c moves opposite to the basis centres
+              }
+              grad[3 * fk[k]->atom_index + d] += dpq * (dt + vsum) - wpq * ds;
+            }
+        }
+    }
+  return 0;
+}
+
+// Nuclear repulsion gradient.
+void orc_grad_nuc(int natoms, const int* Z, const double* xyz, double* grad) {
+  for (int a = 0; a < natoms; ++a)
+    for (int d = 0; d < 3; ++d) grad[3 * a + d] = 0.0;
+  for (int a = 0; a < natoms; ++a)
+    for (int b = 0; b < natoms; ++b) {
+      if (a == b) continue;
+      double r[3], r2 = 0;
+      for (int d = 0; d < 3; ++d) { r[d] = xyz[3 * a + d] - xyz[3 * b + d]; r2 += r[d] * r[d]; }
+      const double f = Z[a] * Z[b] / (r2 * std::sqrt(r2));
+      for (int d = 0; d < 3; ++d) grad[3 * a + d] -= f * r[d];
+    }
+}
+
+}  // extern "C"

@@ -231,3 +231,44 @@ def ri_tensor(prims, types, norm, aux_prims, aux_norm, thr=1e-12):
     V = int2c2e(aux_prims, aux_norm, Qa, thr)
     T = int3c2e(prims, types, norm, Q, aux_prims, aux_norm, Qa, thr)
     return ri_B(cholesky(V), T).reshape(len(aux_norm), len(norm), len(norm))
+
+
+def grad_2e(prims, types, norm, Q, D, natoms, thr=1e-12):
+    """sum Gamma(D) d(ab|cd)/dX, Gamma = 1/2 D_ab D_cd - 1/8 (D_ac D_bd + D_ad D_bc)."""
+    M = len(norm)
+    D = np.ascontiguousarray(D, dtype=np.float64)
+    g = np.zeros(3 * natoms)
+    lib().orc_grad_2e(_ptr(prims), _ptr(types), ctypes.c_int(len(types)), _ptr(norm), ctypes.c_int(M), _ptr(Q),
+                      ctypes.c_double(thr), _ptr(D), ctypes.c_int(natoms), _ptr(g))
+    return g.reshape(natoms, 3)
+
+
+def grad_1e(prims, norm, Z, xyz_bohr, D, W):
+    """sum D d(T+V)/dX - sum W dS/dX (basis-centre and Hellmann-Feynman terms)."""
+    M = len(norm)
+    Z = np.ascontiguousarray(Z, dtype=np.int32)
+    xyz = np.ascontiguousarray(xyz_bohr, dtype=np.float64).reshape(-1)
+    D = np.ascontiguousarray(D, dtype=np.float64)
+    W = np.ascontiguousarray(W, dtype=np.float64)
+    g = np.zeros(3 * len(Z))
+    lib().orc_grad_1e(_ptr(prims), ctypes.c_int(len(prims)), _ptr(norm), ctypes.c_int(M), ctypes.c_int(len(Z)), _ptr(Z),
+                      _ptr(xyz), _ptr(D), _ptr(W), _ptr(g))
+    return g.reshape(len(Z), 3)
+
+
+def grad_nuc(Z, xyz_bohr):
+    Z = np.ascontiguousarray(Z, dtype=np.int32)
+    xyz = np.ascontiguousarray(xyz_bohr, dtype=np.float64).reshape(-1)
+    g = np.zeros(3 * len(Z))
+    lib().orc_grad_nuc(ctypes.c_int(len(Z)), _ptr(Z), _ptr(xyz), _ptr(g))
+    return g.reshape(len(Z), 3)
+
+
+def rhf_gradient(gbs_path, Z, xyz_bohr, nocc, quirk=True, conv=1e-12, thr=1e-12):
+    """FP64 host RHF energy and analytic nuclear gradient (W = D F D / 2)."""
+    prims, types, norm = build_molecule(gbs_path, Z, xyz_bohr, quirk=quirk)
+    _, E, D, F, _ = rhf(prims, types, norm, Z, xyz_bohr, nocc, thr=thr, conv=conv, maxiter=200)
+    W = 0.5 * D @ F @ D
+    Q = schwarz(prims, types)
+    g = grad_1e(prims, norm, Z, xyz_bohr, D, W) + grad_2e(prims, types, norm, Q, D, len(Z), thr) + grad_nuc(Z, xyz_bohr)
+    return E, g

@@ -1,0 +1,121 @@
+"""Pins the FP64 host gradient (oracle/oracle.cpp orc_grad_*), which checks the
+engine's backward kernels.
+
+The reference has no analytic gradient, so the oracle is pinned by what the
+mathematics fixes:
+ - central finite differences of a fixed-density functional
+   f(R) = sum D (T+V) - sum W S + 1/2 D.J(D) - 1/4 D.K(D) + V_nn
+   built from the oracle's own integral routines at displaced geometries
+   (independent of SCF convergence; pins every derivative-integral path);
+ - translational invariance of each part (sum over atoms = 0);
+ - central finite differences of converged RHF energies (pins W = D F D / 2
+   and the assembly), h = 1e-4 bohr, tolerance 1e-6 Eh/bohr.
+"""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from dft_gansu_amd import molecule as gm
+from dft_gansu_amd.molecule import basis_path
+
+
+def _water():
+    return gm.water(jitter=0.01, seed=11)
+
+
+def _functional(gbs, Z, xyz, D, W, thr=0.0):
+    prims, types, norm = O.build_molecule(gbs, Z, xyz)
+    S, T, V = O.oneint(prims, norm, Z, xyz)
+    Q = O.schwarz(prims, types)
+    J, K = O.jk(prims, types, norm, Q, D, thr=thr)
+    return float(np.sum(D * (T + V)) - np.sum(W * S) + 0.5 * np.sum(D * J) - 0.25 * np.sum(D * K)
+                 + O.nuclear_repulsion(Z, xyz))
+
+
+def _analytic(gbs, Z, xyz, D, W, thr=0.0):
+    prims, types, norm = O.build_molecule(gbs, Z, xyz)
+    Q = O.schwarz(prims, types)
+    return (O.grad_1e(prims, norm, Z, xyz, D, W), O.grad_2e(prims, types, norm, Q, D, len(Z), thr),
+            O.grad_nuc(Z, xyz))
+
+
+def _sym(M, seed):
+    rng = np.random.default_rng(seed)
+    A = rng.normal(size=(M, M)) * 0.3
+    return A + A.T
+
+
+@pytest.mark.parametrize("basis", ["sto-3g", "6-31g_st"])
+def test_fixed_density_functional_matches_finite_differences(basis):
+    Z, xyz = _water()
+    gbs = basis_path(basis)
+    M = len(O.build_molecule(gbs, Z, xyz)[2])
+    D, W = _sym(M, 1), _sym(M, 2)
+    g1, g2, gn = _analytic(gbs, Z, xyz, D, W)
+    g = g1 + g2 + gn
+    h = 1e-4
+    fd = np.zeros_like(g)
+    for a in range(len(Z)):
+        for d in range(3):
+            xp, xm = xyz.copy(), xyz.copy()
+            xp[a, d] += h
+            xm[a, d] -= h
+            fd[a, d] = (_functional(gbs, Z, xp, D, W) - _functional(gbs, Z, xm, D, W)) / (2 * h)
+    assert np.max(np.abs(fd - g)) < 2e-7 * max(1.0, np.max(np.abs(g)))
+
+
+def test_each_part_is_translation_invariant():
+    Z, xyz = _water()
+    gbs = basis_path("6-31g_st")
+    M = len(O.build_molecule(gbs, Z, xyz)[2])
+    for part in _analytic(gbs, Z, xyz, _sym(M, 4), _sym(M, 5)):
+        assert np.max(np.abs(part.sum(axis=0))) < 1e-10 * max(1.0, np.max(np.abs(part)))
+
+
+def test_two_electron_part_alone_matches_finite_differences():
+    """Only the ERI derivative path, on a system with d shells on two centres."""
+    Z, xyz = gm.alkane(2, jitter=0.02, seed=3)  # C2H6
+    gbs = basis_path("6-31g_st")
+    prims, types, norm = O.build_molecule(gbs, Z, xyz)
+    D = _sym(len(norm), 7)
+    g2 = _analytic(gbs, Z, xyz, D, np.zeros_like(D))[1]
+
+    def e2(x):
+        pr, ty, nm = O.build_molecule(gbs, Z, x)
+        J, K = O.jk(pr, ty, nm, O.schwarz(pr, ty), D, thr=0.0)
+        return 0.5 * np.sum(D * J) - 0.25 * np.sum(D * K)
+
+    h = 1e-4
+    for a, d in [(0, 0), (0, 2), (1, 1), (3, 0), (4, 2)]:
+        xp, xm = xyz.copy(), xyz.copy()
+        xp[a, d] += h
+        xm[a, d] -= h
+        fd = (e2(xp) - e2(xm)) / (2 * h)
+        assert abs(fd - g2[a, d]) < 2e-7 * max(1.0, abs(g2[a, d]))
+
+
+def test_rhf_gradient_matches_energy_finite_differences():
+    Z, xyz = _water()
+    gbs = basis_path("sto-3g")
+    nocc = int(sum(Z)) // 2
+    E, g = O.rhf_gradient(gbs, Z, xyz, nocc)
+    h = 1e-4
+    for a in range(len(Z)):
+        for d in range(3):
+            xp, xm = xyz.copy(), xyz.copy()
+            xp[a, d] += h
+            xm[a, d] -= h
+            fd = (O.rhf_gradient(gbs, Z, xp, nocc)[0] - O.rhf_gradient(gbs, Z, xm, nocc)[0]) / (2 * h)
+            assert abs(fd - g[a, d]) < 1e-6, (a, d, fd, g[a, d])
+    assert np.max(np.abs(g.sum(axis=0))) < 1e-8
+
+
+def test_rhf_gradient_vanishes_by_symmetry():
+    """H2 along z: the x and y components vanish, the z components are equal
+    and opposite."""
+    Z = np.array([1, 1])
+    xyz = np.array([[0.0, 0.0, 0.0], [0.0, 0.0, 1.4]])
+    E, g = O.rhf_gradient(basis_path("sto-3g"), Z, xyz, 1)
+    assert abs(E - (-1.1167)) < 1e-4  # Szabo-Ostlund H2/STO-3G at 1.4 bohr
+    assert np.max(np.abs(g[:, :2])) < 1e-12
+    assert abs(g[0, 2] + g[1, 2]) < 1e-12
