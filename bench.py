@@ -30,7 +30,9 @@ def log(msg):
 
 def kernel_name(kid):
     kid = int(kid)
-    kind = {1: "fock_slab", 3: "jket"}.get(kid // 100000, "eri_class")
+    f32 = kid >= 1000000  # class with an FP32-stored part (precision=mixed)
+    kid %= 1000000
+    kind = {1: "fock_slab", 3: "jket"}.get(kid // 100000, "eri_class") + ("[fp64+fp32 store]" if f32 else "")
     kid %= 100000
     mode = {0: "direct", 1: "store", 2: "stored-digest"}.get(kid // 10000, "?")
     l = "spdf"
@@ -141,7 +143,7 @@ def main():
     build_kernel_ms = sum(r[1] for r in kernels)
     # algorithmic bytes of a build: every stored integral once (the ket-major J
     # pass re-reads them; that is extra traffic, not algorithmic work)
-    all_bytes = sum(r[2] for r in kernels if int(r[0]) // 100000 != 3)
+    all_bytes = sum(r[2] for r in kernels if (int(r[0]) % 1000000) // 100000 != 3)
     if args.method == "stored":
         roof = dict(bound="hbm", kernel=kernel_name(dom[0]), achieved=round(dom[2] / (dom[1] * 1e-3) / 1e9, 1),
                     peak=HBM_PEAK_GBS, unit="GB/s", frac=round(dom[2] / (dom[1] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),

@@ -8,6 +8,7 @@
 #include "engine.hpp"
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstring>
 
@@ -100,7 +101,28 @@ FockEngine::FockEngine(const gansu_primitive_shell* prims, size_t n_prims, const
 
 FockEngine::~FockEngine() {
   prof_clear();
+  streams_destroy();
   ri_destroy(ri_);
+}
+
+void FockEngine::streams_create() {
+  if (ev_fork_) return;
+  for (int i = 0; i < kSideStreams; ++i) {
+    GANSU_HIP(hipStreamCreateWithFlags(&side_[i], hipStreamNonBlocking));
+    GANSU_HIP(hipEventCreateWithFlags(&ev_join_[i], hipEventDisableTiming));
+  }
+  GANSU_HIP(hipEventCreateWithFlags(&ev_fork_, hipEventDisableTiming));
+}
+
+void FockEngine::streams_destroy() {
+  for (int i = 0; i < kSideStreams; ++i) {
+    if (side_[i]) (void)hipStreamDestroy(side_[i]);
+    if (ev_join_[i]) (void)hipEventDestroy(ev_join_[i]);
+    side_[i] = nullptr;
+    ev_join_[i] = nullptr;
+  }
+  if (ev_fork_) (void)hipEventDestroy(ev_fork_);
+  ev_fork_ = nullptr;
 }
 
 void FockEngine::set_profiling(bool on) {
@@ -157,6 +179,7 @@ void FockEngine::precompute() {
   precompute_pairs();
   if (method_ == "ri") precompute_ri();
   else precompute_quartets();
+  streams_create();
   GANSU_HIP(hipDeviceSynchronize());
   precomputed_ = true;
 }
@@ -255,9 +278,14 @@ void FockEngine::precompute_quartets() {
                      [&](int x, int y) { return sp_h_[x].qmax > sp_h_[y].qmax; });
     pc.list.upload(pc.sorted);
   }
-  // quartet classes (bra class A >= ket class B) and their work blocks
+  // quartet classes (bra class A >= ket class B) and their work blocks; with
+  // precision=mixed on the stored path each bra's kets split into an FP64
+  // prefix (qmax_bra * qmax_ket >= fp32_bound_threshold; the ket list is
+  // sorted by qmax) and an FP32 suffix
   qclass_.clear();
   n_quartets_ = 0;
+  n_fp32_quartets_ = 0;
+  const bool split32 = mixed_ && method_ == "stored";
   for (int A = 0; A < npc; ++A)
     for (int B = 0; B <= A; ++B) {
       auto qc = std::make_unique<QuartetClass>();
@@ -265,24 +293,35 @@ void FockEngine::precompute_quartets() {
       qc->LA = pclass_[A].la; qc->LB = pclass_[A].lb; qc->LC = pclass_[B].la; qc->LD = pclass_[B].lb;
       const auto& bl = pclass_[A].sorted;
       const auto& kl = pclass_[B].sorted;
-      for (int s = 0; s < (int)bl.size(); ++s) {
-        const double qb = sp_h_[bl[s]].qmax;
-        // kets with qb * qk >= thr form a prefix of the descending list
+      // kets with qb * qk >= t form a prefix of the descending list
+      auto prefix = [&](double qb, double t) {
         int lo = 0, hi = (int)kl.size();
         while (lo < hi) {
           const int mid = (lo + hi) / 2;
-          if (qb * sp_h_[kl[mid]].qmax >= thr_) lo = mid + 1; else hi = mid;
+          if (qb * sp_h_[kl[mid]].qmax >= t) lo = mid + 1; else hi = mid;
         }
-        int count = lo;
+        return lo;
+      };
+      for (int s = 0; s < (int)bl.size(); ++s) {
+        const double qb = sp_h_[bl[s]].qmax;
+        int count = prefix(qb, thr_);
         if (A == B) count = std::min(count, s + 1);
-        if (count > 0) qc->bras_h.push_back(BraRec{s, (int)qc->blocks_h.size(), count, 0});
-        for (int k0 = 0; k0 < count; k0 += 64)
-          qc->blocks_h.push_back(BlockRec{s, k0, std::min(64, count - k0), 0});
+        if (count <= 0) continue;
+        const int split = split32 ? std::min(count, prefix(qb, fp32_thr_)) : count;
+        qc->bras_h.push_back(BraRec{s, qc->n_blocks64, count, split, qc->n_blocks32, 0, 0, 0});
+        for (int k0 = 0; k0 < split; k0 += 64)
+          qc->blocks_h.push_back(BlockRec{s, k0, std::min(64, split - k0), qc->n_blocks64++});
+        for (int k0 = split; k0 < count; k0 += 64)
+          qc->blocks_h.push_back(BlockRec{s, k0, std::min(64, count - k0), -1 - qc->n_blocks32++});
         qc->n_quartets += count;
+        qc->n_fp32 += count - split;
       }
       n_quartets_ += qc->n_quartets;
+      n_fp32_quartets_ += qc->n_fp32;
       qc->blocks.upload(qc->blocks_h);
       qc->bras.upload(qc->bras_h);
+      for (int mode : {(int)kModeDirect, (int)kModeStoredDigest})
+        qc->slab_ok[mode] = slab_fits(qc->LA, qc->LB, qc->LC, qc->LD, mode, M_);
       for (const auto& br : qc->bras_h) qc->max_count = std::max(qc->max_count, br.count);
       // slab work records: tiles of bras, each bra's kets cut into chunks of
       // a few passes so that heavy classes with few bras still fill the GPU
@@ -307,10 +346,11 @@ void FockEngine::precompute_quartets() {
     EriArgs g{d_sp_.ptr, d_pr_.ptr, boys_.ptr, thr_, d_Dp_.ptr, M_, d_Jr_.ptr, d_Kr_.ptr};
     for (auto& qc : qclass_) {
       const size_t ncomp = (size_t)ncart(qc->LA) * ncart(qc->LB) * ncart(qc->LC) * ncart(qc->LD);
-      qc->store.alloc(qc->blocks_h.size() * 64 * ncomp);
-      stored_bytes_ += double(qc->store.n) * sizeof(double);
+      qc->store.alloc((size_t)qc->n_blocks64 * 64 * ncomp);
+      qc->store32.alloc((size_t)qc->n_blocks32 * 64 * ncomp);
+      stored_bytes_ += double(qc->store.n) * sizeof(double) + double(qc->store32.n) * sizeof(float);
       ClassLaunch c{qc->LA, qc->LB, qc->LC, qc->LD, qc->blocks.ptr, (int)qc->blocks_h.size(),
-                    pclass_[qc->A].list.ptr, pclass_[qc->B].list.ptr, qc->store.ptr};
+                    pclass_[qc->A].list.ptr, pclass_[qc->B].list.ptr, qc->store.ptr, qc->store32.ptr};
       GANSU_HIP(launch_eri_class(c, g, kModeStore, 0));
     }
     GANSU_HIP(hipDeviceSynchronize());
@@ -324,38 +364,71 @@ void FockEngine::build_four_center(const double* d_D, hipStream_t s) {
   GANSU_HIP(launch_zero(d_Kr_.ptr, (size_t)M_ * M_, s));
   EriArgs g{d_sp_.ptr, d_pr_.ptr, boys_.ptr, thr_, d_Dp_.ptr, M_, d_Jr_.ptr, d_Kr_.ptr};
   const int mode = method_ == "stored" ? kModeStoredDigest : kModeDirect;
+  // launch items: (class, 0 = digestion, 1 = ket-J pass), costed by the
+  // integrals they visit; profiling runs them in order on the build stream
+  struct Item { QuartetClass* qc; int kind; double cost; };
+  std::vector<Item> items;
   for (auto& qc : qclass_) {
     if (qc->blocks_h.empty()) continue;
     const double ncomp = double(ncart(qc->LA) * ncart(qc->LB) * ncart(qc->LC) * ncart(qc->LD));
-    // algorithmic traffic of the stored digestion: each surviving integral read once
-    const double bytes = mode == kModeStoredDigest ? qc->n_quartets * ncomp * 8.0 : 0.0;
-    const bool slab = slab_available(qc->LA, qc->LB, qc->LC, qc->LD, mode) && qc->slab_ok;
-    prof_begin(s, 10000.0 * mode + 1000 * qc->LA + 100 * qc->LB + 10 * qc->LC + qc->LD + (slab ? 100000.0 : 0.0),
-               bytes, 0.0, qc->n_quartets);
-    bool launched = false;
-    if (slab) {
-      SlabLaunch c{qc->LA, qc->LB, qc->LC, qc->LD, qc->bras.ptr, (int)qc->bras_h.size(), qc->work[mode].ptr,
-                   (int)qc->work_h[mode].size(), pclass_[qc->A].list.ptr, pclass_[qc->B].list.ptr, qc->store.ptr,
-                   qc->max_count};
-      const hipError_t e = launch_fock_slab(c, g, mode, s);
-      if (e == hipErrorInvalidValue) qc->slab_ok = false;  // LDS footprint too large: use the per-lane kernel
-      else {
-        GANSU_HIP(e);
-        launched = true;
-        if (mode == kModeStoredDigest) {
-          prof_end(s);
-          prof_begin(s, 300000.0 + 10000.0 * mode + 1000 * qc->LA + 100 * qc->LB + 10 * qc->LC + qc->LD, bytes, 0.0, qc->n_quartets);
-          GANSU_HIP(launch_jket(c, g, s));
-        }
-      }
-    }
-    if (!launched) {
-      ClassLaunch c{qc->LA, qc->LB, qc->LC, qc->LD, qc->blocks.ptr, (int)qc->blocks_h.size(),
-                    pclass_[qc->A].list.ptr, pclass_[qc->B].list.ptr, qc->store.ptr};
-      GANSU_HIP(launch_eri_class(c, g, mode, s));
-    }
-    prof_end(s);
+    const double w = qc->n_quartets * ncomp * (mode == kModeDirect ? 10.0 : 1.0);
+    items.push_back({qc.get(), 0, w});
+    if (mode == kModeStoredDigest && qc->slab_ok[mode]) items.push_back({qc.get(), 1, 0.5 * w});
   }
+  const bool concurrent = !profiling_ && ev_fork_ != nullptr;
+  std::vector<hipStream_t> st{s};
+  if (concurrent) {
+    std::stable_sort(items.begin(), items.end(), [](const Item& x, const Item& y) { return x.cost > y.cost; });
+    GANSU_HIP(hipEventRecord(ev_fork_, s));
+    for (int i = 0; i < kSideStreams; ++i) {
+      GANSU_HIP(hipStreamWaitEvent(side_[i], ev_fork_, 0));
+      st.push_back(side_[i]);
+    }
+  }
+  // per-lane (non-slab) class kernels can need a large private segment;
+  // they stay on the build stream so that only its queue holds such scratch
+  std::vector<double> load(st.size(), 0.0);
+  for (const Item& it : items)
+    if (it.kind == 0 && !it.qc->slab_ok[mode]) load[0] += it.cost;
+  for (const Item& it : items) {
+    QuartetClass* qc = it.qc;
+    const bool slab = qc->slab_ok[mode];
+    size_t k = 0;
+    if (it.kind == 1 || slab) {
+      k = std::min_element(load.begin(), load.end()) - load.begin();
+      load[k] += it.cost;
+    }
+    hipStream_t q = st[k];
+    const double ncomp = double(ncart(qc->LA) * ncart(qc->LB) * ncart(qc->LC) * ncart(qc->LD));
+    // algorithmic traffic of the stored digestion: each surviving integral read once
+    const double bytes =
+        mode == kModeStoredDigest ? ((qc->n_quartets - qc->n_fp32) * 8.0 + qc->n_fp32 * 4.0) * ncomp : 0.0;
+    const double f32id = qc->n_fp32 > 0 && mode == kModeStoredDigest ? 1000000.0 : 0.0;
+    const double cls = 1000 * qc->LA + 100 * qc->LB + 10 * qc->LC + qc->LD;
+    SlabLaunch c{qc->LA, qc->LB, qc->LC, qc->LD, qc->bras.ptr, (int)qc->bras_h.size(), qc->work[mode].ptr,
+                 (int)qc->work_h[mode].size(), pclass_[qc->A].list.ptr, pclass_[qc->B].list.ptr, qc->store.ptr,
+                 qc->max_count, qc->store32.ptr};
+    if (it.kind == 1) {  // ket-major J of the stored blocks
+      prof_begin(q, f32id + 300000.0 + 10000.0 * mode + cls, bytes, 0.0, qc->n_quartets);
+      GANSU_HIP(launch_jket(c, g, q));
+      prof_end(q);
+      continue;
+    }
+    prof_begin(q, f32id + 10000.0 * mode + cls + (slab ? 100000.0 : 0.0), bytes, 0.0, qc->n_quartets);
+    if (slab) {
+      GANSU_HIP(launch_fock_slab(c, g, mode, q));
+    } else {  // per-lane kernel: digests J of bra and ket completely
+      ClassLaunch cl{qc->LA, qc->LB, qc->LC, qc->LD, qc->blocks.ptr, (int)qc->blocks_h.size(),
+                     pclass_[qc->A].list.ptr, pclass_[qc->B].list.ptr, qc->store.ptr, qc->store32.ptr};
+      GANSU_HIP(launch_eri_class(cl, g, mode, q));
+    }
+    prof_end(q);
+  }
+  if (concurrent)
+    for (int i = 0; i < kSideStreams; ++i) {
+      GANSU_HIP(hipEventRecord(ev_join_[i], side_[i]));
+      GANSU_HIP(hipStreamWaitEvent(s, ev_join_[i], 0));
+    }
 }
 
 void FockEngine::build_jk(const double* d_D, double* d_J, double* d_K, const double* d_H, double* d_F,
@@ -378,6 +451,7 @@ void FockEngine::stats(double* out) const {
   out[0] = n_quartets_;
   out[1] = n_prim_quartets_;
   out[2] = stored_bytes_;
+  out[3] = n_fp32_quartets_;
   const double P = double(M_) * (M_ + 1) / 2;
   out[4] = P * (P + 1) / 2;
   if (method_ == "ri") out[5] = double(M_) * Maux_ * M_ * sizeof(double);

@@ -59,12 +59,16 @@ struct QuartetClass {
   DevBuf<BlockRec> blocks;
   std::vector<BraRec> bras_h;  // one entry per bra with kets (slab digestion)
   DevBuf<BraRec> bras;
-  bool slab_ok = true;  // cleared when the slab kernel's LDS does not fit
+  bool slab_ok[3] = {true, true, true};  // per mode: the slab kernel has a variant whose LDS fits
   std::vector<SlabWork> work_h[3];  // per mode (direct, -, stored digest)
   DevBuf<SlabWork> work[3];
   int max_count = 0;
   DevBuf<double> store;  // stored integrals of this class (eri_method=stored)
-  double n_quartets = 0;
+  // precision=mixed: the quartets whose bound qmax_bra * qmax_ket is below
+  // fp32_bound_threshold (a suffix of each bra's kets) are stored in FP32
+  DevBuf<float> store32;
+  int n_blocks64 = 0, n_blocks32 = 0;
+  double n_quartets = 0, n_fp32 = 0;
 };
 
 struct PairClass {
@@ -134,7 +138,17 @@ class FockEngine {
   std::vector<PairClass> pclass_;
   std::vector<std::unique_ptr<QuartetClass>> qclass_;
   DevBuf<double> d_w_, d_Dp_, d_Jr_, d_Kr_;
-  double n_quartets_ = 0, n_prim_quartets_ = 0, stored_bytes_ = 0;
+  double n_quartets_ = 0, n_prim_quartets_ = 0, stored_bytes_ = 0, n_fp32_quartets_ = 0;
+
+  // Concurrent class launches: the build stream forks to kSideStreams more
+  // streams (default priority) through an event and joins them at the end.
+  // Each launch item (a class's digestion or ket-J pass) goes to the stream
+  // with the least assigned work, largest items first.
+  static constexpr int kSideStreams = 3;
+  hipStream_t side_[kSideStreams] = {};
+  hipEvent_t ev_fork_ = nullptr, ev_join_[kSideStreams] = {};
+  void streams_create();
+  void streams_destroy();
 
   // profiling: events around each launch of a build
   struct ProfRow { double id, bytes, flops, units; hipEvent_t a, b; };

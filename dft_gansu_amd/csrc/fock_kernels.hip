@@ -367,18 +367,28 @@ __global__ __launch_bounds__(64) void k_eri_class(ClassLaunch c, EriArgs g) {
 #pragma unroll
   for (int k = 0; k < NCOMP; ++k) out[k] = 0.0;
 
+  const bool f32 = blk.slot < 0;
+  const size_t at = (size_t)(f32 ? -blk.slot - 1 : blk.slot) * NCOMP * 64 + lane;
   if constexpr (MODE == kModeStoredDigest) {
-    const double* src = c.store + (size_t)blockIdx.x * NCOMP * 64 + lane;
+    if (f32) {
 #pragma unroll
-    for (int k = 0; k < NCOMP; ++k) out[k] = active ? src[(size_t)k * 64] : 0.0;
+      for (int k = 0; k < NCOMP; ++k) out[k] = active ? (double)c.store32[at + (size_t)k * 64] : 0.0;
+    } else {
+#pragma unroll
+      for (int k = 0; k < NCOMP; ++k) out[k] = active ? c.store[at + (size_t)k * 64] : 0.0;
+    }
   } else {
     eri_block_t<LA, LB, LC, LD>(bra, ket, active ? ket.nprim : 0, g, out);
   }
 
   if constexpr (MODE == kModeStore) {
-    double* dst = c.store + (size_t)blockIdx.x * NCOMP * 64 + lane;
+    if (f32) {
 #pragma unroll
-    for (int k = 0; k < NCOMP; ++k) dst[(size_t)k * 64] = out[k];
+      for (int k = 0; k < NCOMP; ++k) c.store32[at + (size_t)k * 64] = (float)out[k];
+    } else {
+#pragma unroll
+      for (int k = 0; k < NCOMP; ++k) c.store[at + (size_t)k * 64] = out[k];
+    }
   } else {
     digest_block<NA, NB, NC, ND>(out, active, bra, ket, bra_id != ket_id, g);
   }
@@ -404,9 +414,11 @@ __global__ __launch_bounds__(64) void k_eri_generic(ClassLaunch c, EriArgs g, in
   const SPair ket = g.sp[ket_id];
   double out[kGenMaxComp];
   for (int k = 0; k < NCOMP; ++k) out[k] = 0.0;
+  const bool f32 = blk.slot < 0;
+  const size_t at = (size_t)(f32 ? -blk.slot - 1 : blk.slot) * NCOMP * 64 + lane;
   if (mode == kModeStoredDigest) {
-    const double* src = c.store + (size_t)blockIdx.x * NCOMP * 64 + lane;
-    for (int k = 0; k < NCOMP; ++k) out[k] = active ? src[(size_t)k * 64] : 0.0;
+    for (int k = 0; k < NCOMP; ++k)
+      out[k] = active ? (f32 ? (double)c.store32[at + (size_t)k * 64] : c.store[at + (size_t)k * 64]) : 0.0;
   } else {
     const int ntb = LAB + 1, ntk = LCD + 1;
     const int nk = active ? ket.nprim : 0;
@@ -466,8 +478,10 @@ __global__ __launch_bounds__(64) void k_eri_generic(ClassLaunch c, EriArgs g, in
     }
   }
   if (mode == kModeStore) {
-    double* dst = c.store + (size_t)blockIdx.x * NCOMP * 64 + lane;
-    for (int k = 0; k < NCOMP; ++k) dst[(size_t)k * 64] = out[k];
+    if (f32)
+      for (int k = 0; k < NCOMP; ++k) c.store32[at + (size_t)k * 64] = (float)out[k];
+    else
+      for (int k = 0; k < NCOMP; ++k) c.store[at + (size_t)k * 64] = out[k];
     return;
   }
   // digestion with runtime shapes
@@ -603,7 +617,7 @@ __global__ __launch_bounds__(kSlabThreads) void k_fock_slab(SlabLaunch c, EriArg
   // per owned bra: [Ki NA*M][Kj NB*M][Di NA*M][Dj NB*M], then J of the bra block
   const int slab = 2 * (NA + NB) * M;
   double* jbl = lds + BPW * slab;  // [BPW][NAB]
-  int bid[BPW], bbi[BPW], bbj[BPW], cnt[BPW], fblk[BPW];
+  int bid[BPW], bbi[BPW], bbj[BPW], cnt[BPW], fblk[BPW], spl[BPW], fblk32[BPW];
   bool bdiff[BPW];
   int maxc = 0;
 #pragma unroll
@@ -614,9 +628,10 @@ __global__ __launch_bounds__(kSlabThreads) void k_fock_slab(SlabLaunch c, EriArg
       const SPair bra = g.sp[bid[b]];
       bbi[b] = bra.bi; bbj[b] = bra.bj; bdiff[b] = bra.si != bra.sj;
       cnt[b] = min(br.count, wk.k_end); fblk[b] = br.first_block;
+      spl[b] = br.split; fblk32[b] = br.first_block32;
       maxc = max(maxc, cnt[b]);
     } else {
-      bid[b] = -1; bbi[b] = bbj[b] = 0; bdiff[b] = false; cnt[b] = 0; fblk[b] = 0;
+      bid[b] = -1; bbi[b] = bbj[b] = 0; bdiff[b] = false; cnt[b] = 0; fblk[b] = 0; spl[b] = 0; fblk32[b] = 0;
     }
   }
   for (int b = 0; b < nb; ++b) {
@@ -673,55 +688,59 @@ __global__ __launch_bounds__(kSlabThreads) void k_fock_slab(SlabLaunch c, EriArg
         const double fjk = pair_diff ? fbk : 0.0;
         const int bj = bbj[b];
         [[maybe_unused]] double out[MODE == kModeDirect ? NCOMP : 1];
-        [[maybe_unused]] const double* src = nullptr;
+        // digestion of one quartet; V(comp) yields its integrals
+        auto digest = [&](auto&& V) {
+#pragma unroll
+          for (int aa = 0; aa < (SPLIT ? 1 : NA); ++aa) {
+            const int a = SPLIT ? sub : aa;
+#pragma unroll
+            for (int bb = 0; bb < NB; ++bb) {
+              const double dab = Di[a * M + bj + bb];
+              double kad[ND], kbd[ND];
+#pragma unroll
+              for (int d = 0; d < ND; ++d) { kad[d] = 0.0; kbd[d] = 0.0; }
+              double jbs = 0.0;
+#pragma unroll
+              for (int cc = 0; cc < NC; ++cc) {
+                const double dbc = Dj[bb * M + bk + cc];
+                const double dac = Di[a * M + bk + cc];
+                double kac = 0.0, kbc = 0.0;
+#pragma unroll
+                for (int d = 0; d < ND; ++d) {
+                  const double v = V(((a * NB + bb) * NC + cc) * ND + d);
+                  jbs += v * dcd[cc * ND + d];
+                  if constexpr (MODE == kModeDirect) jk[cc * ND + d] += fjk * v * dab;
+                  kac += v * Dj[bb * M + bl + d];
+                  kbc += v * Di[a * M + bl + d];
+                  kad[d] += v * dbc;
+                  kbd[d] += v * dac;
+                }
+                unsafeAtomicAdd(&KiW[a * M + bk + cc], fo * kac);
+                if (ij_diff) unsafeAtomicAdd(&KjW[bb * M + bk + cc], fo * kbc);
+              }
+              jb[b][SPLIT ? bb : a * NB + bb] += fbk * jbs;
+              if (kl_diff) {
+#pragma unroll
+                for (int d = 0; d < ND; ++d) {
+                  unsafeAtomicAdd(&KiW[a * M + bl + d], fo * kad[d]);
+                  if (ij_diff) unsafeAtomicAdd(&KjW[bb * M + bl + d], fo * kbd[d]);
+                }
+              }
+            }
+          }
+        };
         if constexpr (MODE == kModeDirect) {
 #pragma unroll
           for (int k = 0; k < NCOMP; ++k) out[k] = 0.0;
           eri_block_t<LA, LB, LC, LD>(g.sp[bid[b]], ket, ket.nprim, g, out);
+          digest([&](int comp) -> double { return out[comp]; });
+        } else if (kk < spl[b]) {
+          const double* src = c.store + ((size_t)(fblk[b] + kk / 64) * NCOMP) * 64 + (kk & 63);
+          digest([&](int comp) -> double { return src[(size_t)comp * 64]; });
         } else {
-          src = c.store + ((size_t)(fblk[b] + kk / 64) * NCOMP) * 64 + (kk & 63);
-        }
-        auto V = [&](int comp) -> double {
-          if constexpr (MODE == kModeDirect) return out[comp];
-          else return src[(size_t)comp * 64];
-        };
-#pragma unroll
-        for (int aa = 0; aa < (SPLIT ? 1 : NA); ++aa) {
-          const int a = SPLIT ? sub : aa;
-#pragma unroll
-          for (int bb = 0; bb < NB; ++bb) {
-            const double dab = Di[a * M + bj + bb];
-            double kad[ND], kbd[ND];
-#pragma unroll
-            for (int d = 0; d < ND; ++d) { kad[d] = 0.0; kbd[d] = 0.0; }
-            double jbs = 0.0;
-#pragma unroll
-            for (int cc = 0; cc < NC; ++cc) {
-              const double dbc = Dj[bb * M + bk + cc];
-              const double dac = Di[a * M + bk + cc];
-              double kac = 0.0, kbc = 0.0;
-#pragma unroll
-              for (int d = 0; d < ND; ++d) {
-                const double v = V(((a * NB + bb) * NC + cc) * ND + d);
-                jbs += v * dcd[cc * ND + d];
-                if constexpr (MODE == kModeDirect) jk[cc * ND + d] += fjk * v * dab;
-                kac += v * Dj[bb * M + bl + d];
-                kbc += v * Di[a * M + bl + d];
-                kad[d] += v * dbc;
-                kbd[d] += v * dac;
-              }
-              unsafeAtomicAdd(&KiW[a * M + bk + cc], fo * kac);
-              if (ij_diff) unsafeAtomicAdd(&KjW[bb * M + bk + cc], fo * kbc);
-            }
-            jb[b][SPLIT ? bb : a * NB + bb] += fbk * jbs;
-            if (kl_diff) {
-#pragma unroll
-              for (int d = 0; d < ND; ++d) {
-                unsafeAtomicAdd(&KiW[a * M + bl + d], fo * kad[d]);
-                if (ij_diff) unsafeAtomicAdd(&KjW[bb * M + bl + d], fo * kbd[d]);
-              }
-            }
-          }
+          const int kr = kk - spl[b];
+          const float* src32 = c.store32 + ((size_t)(fblk32[b] + kr / 64) * NCOMP) * 64 + (kr & 63);
+          digest([&](int comp) -> double { return (double)src32[(size_t)comp * 64]; });
         }
       }
       // stored blocks: J of the ket block comes from the ket-major pass (k_jket)
@@ -812,13 +831,23 @@ __global__ __launch_bounds__(64) void k_jket(SlabLaunch c, EriArgs g, int bras_p
     if (bra_id == ket_id) continue;  // diagonal quartet: its J is complete in the bra pass
     const SPair bra = g.sp[bra_id];
     const double f = (bra.si != bra.sj ? 2.0 : 1.0) * (kl_diff ? 2.0 : 1.0);
-    const double* src = c.store + ((size_t)(br.first_block + t / 64) * NCOMP) * 64 + (t & 63) + (size_t)cd * 64;
     double acc = 0.0;
+    if (t < br.split) {
+      const double* src = c.store + ((size_t)(br.first_block + t / 64) * NCOMP) * 64 + (t & 63) + (size_t)cd * 64;
 #pragma unroll
-    for (int a = 0; a < NA; ++a)
+      for (int a = 0; a < NA; ++a)
 #pragma unroll
-      for (int b = 0; b < NB; ++b)
-        acc += g.Dp[(size_t)(bra.bi + a) * M + bra.bj + b] * src[(size_t)(a * NB + b) * NCD * 64];
+        for (int b = 0; b < NB; ++b)
+          acc += g.Dp[(size_t)(bra.bi + a) * M + bra.bj + b] * src[(size_t)(a * NB + b) * NCD * 64];
+    } else {
+      const int tr = t - br.split;
+      const float* src = c.store32 + ((size_t)(br.first_block32 + tr / 64) * NCOMP) * 64 + (tr & 63) + (size_t)cd * 64;
+#pragma unroll
+      for (int a = 0; a < NA; ++a)
+#pragma unroll
+        for (int b = 0; b < NB; ++b)
+          acc += g.Dp[(size_t)(bra.bi + a) * M + bra.bj + b] * (double)src[(size_t)(a * NB + b) * NCD * 64];
+    }
     jk += f * acc;
   }
   if (ket_id >= 0 && jk != 0.0) {
@@ -923,6 +952,24 @@ bool slab_available(int LA, int LB, int LC, int LD, int mode) {
   const int L = LA + LB + LC + LD;
   const int ncomp = ncart(LA) * ncart(LB) * ncart(LC) * ncart(LD);
   return mode == kModeDirect && L <= 4 && ncomp <= 81;
+}
+
+// LDS bytes of the slab kernel of a class at basis size M (0: no variant).
+size_t slab_lds_bytes(int LA, int LB, int LC, int LD, int mode, int M) {
+  if (!slab_available(LA, LB, LC, LD, mode)) return 0;
+  const int NA = ncart(LA), NB = ncart(LB);
+  int bpw = 1, qpp = 0;
+  slab_geometry(LA, LB, LC, LD, mode, &bpw, &qpp);
+  return (size_t)bpw * (2ull * (NA + NB) * (size_t)M + (size_t)NA * NB) * sizeof(double);
+}
+
+bool slab_fits(int LA, int LB, int LC, int LD, int mode, int M) {
+  const size_t need = slab_lds_bytes(LA, LB, LC, LD, mode, M);
+  if (need == 0) return false;
+  int dev = 0, v = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return false;
+  if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) return false;
+  return need <= (size_t)v;
 }
 
 // Launches the slab kernel; hipErrorInvalidValue (nothing launched) when the

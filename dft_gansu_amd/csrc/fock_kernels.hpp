@@ -23,17 +23,22 @@ struct SPair {
 };
 
 // A wavefront's work: one bra pair (position in the bra class list) and up to
-// 64 consecutive kets of the ket class list starting at ket_start.
+// 64 consecutive kets of the ket class list starting at ket_start.  slot
+// locates its stored integrals: >= 0 in the FP64 store, -slot-1 in the FP32
+// store (precision=mixed).
 struct BlockRec {
-  int bra_pos, ket_start, count, pad;
+  int bra_pos, ket_start, count, slot;
 };
 
 enum EriMode { kModeDirect = 0, kModeStore = 1, kModeStoredDigest = 2 };
 
-// One bra pair with all of its kets (positions 0..count-1 of the ket class
-// list); its stored blocks start at first_block (64 kets per block).
+// One bra pair with its kets at positions 0..count-1 of the ket class list.
+// Kets below `split` are stored in FP64 from block first_block on, the rest
+// (precision=mixed: bound below fp32_bound_threshold) in FP32 from block
+// first_block32 on, 64 kets per block counted from the start of each range.
 struct BraRec {
-  int bra_pos, first_block, count, pad;
+  int bra_pos, first_block, count, split;
+  int first_block32, pad0, pad1, pad2;
 };
 
 // A slab workgroup's share: nb consecutive bras starting at bras[bra0] and
@@ -52,11 +57,15 @@ struct SlabLaunch {
   const int* ket_list;
   const double* store;
   int max_count;  // largest ket count of any bra (ket-major J pass)
+  const float* store32;  // FP32 blocks of the kets at or above each bra's split
 };
 
 // Geometry of the slab kernel of a class: bras per workgroup and kets a
 // workgroup visits per pass (used by the host to cut SlabWork records).
 void slab_geometry(int LA, int LB, int LC, int LD, int mode, int* bras_per_wg, int* kets_per_pass);
+// The class has a slab variant in this mode whose LDS footprint at basis
+// size M fits one workgroup of the current device.
+bool slab_fits(int LA, int LB, int LC, int LD, int mode, int M);
 
 struct ClassLaunch {
   int LA, LB, LC, LD;
@@ -65,6 +74,7 @@ struct ClassLaunch {
   const int* bra_list;
   const int* ket_list;
   double* store;  // stored blocks of this class (kModeStore / kModeStoredDigest)
+  float* store32 = nullptr;  // FP32 stored blocks (BlockRec::slot < 0)
 };
 
 struct EriArgs {

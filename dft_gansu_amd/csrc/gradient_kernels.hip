@@ -28,8 +28,6 @@ using namespace dev;
 
 namespace {
 
-constexpr int kGMaxE = 4 * 4 * 7;  // (l+2)^2 (2l+3) entries per direction, l <= 2
-constexpr int kGMaxComp = 36 * 36;
 constexpr int kGradThreads = 256;
 
 __device__ __forceinline__ double wave_sum_g(double v) {
@@ -78,15 +76,18 @@ __device__ inline double pair_deriv(const ETab& T, int which, int d, double ex, 
   return v;
 }
 
+// Per-class instance: private arrays sized for the class (the largest,
+// (dd|dd), needs about 10 KB per lane; the s/p classes stay in registers).
+template <int LA, int LB, int LC, int LD>
 __global__ __launch_bounds__(kGradThreads) void k_grad_2e(ClassLaunch c, Grad2eArgs g) {
   extern __shared__ double s_grad[];
+  constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
+  constexpr int NAB = NA * NB, NCD = NC * ND;
+  constexpr int LAB = LA + LB, LCD = LC + LD, L = LAB + LCD;
+  constexpr int NEB = (LA + 2) * (LB + 2) * (LAB + 3), NEK = (LC + 2) * (LD + 2) * (LCD + 3);
   const int n3 = 3 * g.natoms;
   for (int i = threadIdx.x; i < n3; i += blockDim.x) s_grad[i] = 0.0;
   __syncthreads();
-  const int LA = c.LA, LB = c.LB, LC = c.LC, LD = c.LD;
-  const int NB = ncart(LB), ND = ncart(LD);
-  const int NAB = ncart(LA) * NB, NCD = ncart(LC) * ND;
-  const int LAB = LA + LB, LCD = LC + LD, L = LAB + LCD;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int nw = blockDim.x >> 6;
   const int M = g.M;
@@ -99,33 +100,44 @@ __global__ __launch_bounds__(kGradThreads) void k_grad_2e(ClassLaunch c, Grad2eA
     const SPair ket = g.sp[ket_id];
     double gA[3] = {0.0, 0.0, 0.0}, gB[3] = {0.0, 0.0, 0.0}, gC[3] = {0.0, 0.0, 0.0};
     if (active) {
-      double gam[kGMaxComp];
-      const double* D = g.Dp;
-      for (int ab = 0; ab < NAB; ++ab) {
-        const size_t a = bra.bi + ab / NB, bb = bra.bj + ab % NB;
-        for (int cd = 0; cd < NCD; ++cd) {
-          const size_t cc = ket.bi + cd / ND, dd = ket.bj + cd % ND;
-          gam[ab * NCD + cd] = 0.5 * D[a * M + bb] * D[cc * M + dd] -
-                               0.125 * (D[a * M + cc] * D[bb * M + dd] + D[a * M + dd] * D[bb * M + cc]);
+      // density blocks of Gamma_abcd = 1/2 D_ab D_cd - 1/8 (D_ac D_bd + D_ad D_bc)
+      double dab[NAB], dcd[NCD], dac[NA * NC], dbd[NB * ND], dad[NA * ND], dbc[NB * NC];
+      {
+        const double* D = g.Dp;
+        const size_t ra = bra.bi, rb = bra.bj, rc = ket.bi, rd = ket.bj;
+        for (int a = 0; a < NA; ++a) {
+          for (int bb = 0; bb < NB; ++bb) dab[a * NB + bb] = D[(ra + a) * M + rb + bb];
+          for (int cc = 0; cc < NC; ++cc) dac[a * NC + cc] = D[(ra + a) * M + rc + cc];
+          for (int dd = 0; dd < ND; ++dd) dad[a * ND + dd] = D[(ra + a) * M + rd + dd];
         }
+        for (int bb = 0; bb < NB; ++bb) {
+          for (int dd = 0; dd < ND; ++dd) dbd[bb * ND + dd] = D[(rb + bb) * M + rd + dd];
+          for (int cc = 0; cc < NC; ++cc) dbc[bb * NC + cc] = D[(rb + bb) * M + rc + cc];
+        }
+        for (int cc = 0; cc < NC; ++cc)
+          for (int dd = 0; dd < ND; ++dd) dcd[cc * ND + dd] = D[(rc + cc) * M + rd + dd];
       }
+      auto gamma = [&](int ab, int cd) {
+        const int a = ab / NB, bb = ab % NB, cc = cd / ND, dd = cd % ND;
+        return 0.5 * dab[ab] * dcd[cd] - 0.125 * (dac[a * NC + cc] * dbd[bb * ND + dd] + dad[a * ND + dd] * dbc[bb * NC + cc]);
+      };
       for (int ib = 0; ib < bra.nprim; ++ib) {
         const PairRec pb = g.pr[bra.prim_begin + ib];
         const double exa = g.alpha[bra.prim_begin + ib], exb = pb.p - exa;
-        double Eb[3][kGMaxE];
+        double Eb[3][NEB];
         for (int d = 0; d < 3; ++d) hermite_e_rt<double>(LA + 1, LB + 1, pb.p, pb.PA[d], pb.PB[d], Eb[d]);
         const ETab TB{{Eb[0], Eb[1], Eb[2]}, LB + 2, LAB + 3};
         for (int ik = 0; ik < ket.nprim; ++ik) {
           const PairRec pk = g.pr[ket.prim_begin + ik];
           if (pb.Q * pk.Q < g.thr) continue;
           const double exc = g.alpha[ket.prim_begin + ik];
-          double Ek[3][kGMaxE];
+          double Ek[3][NEK];
           for (int d = 0; d < 3; ++d) hermite_e_rt<double>(LC + 1, LD + 1, pk.p, pk.PA[d], pk.PB[d], Ek[d]);
           const ETab TK{{Ek[0], Ek[1], Ek[2]}, LD + 2, LCD + 3};
           const double p = pb.p, q = pk.p;
           const double alpha = p * q / (p + q);
           const double pref = kTwoPi25 / (p * q * sqrt(p + q)) * pb.Kc * pk.Kc;
-          double R[nherm(9)];
+          double R[nherm(L + 1)];
           hermite_r_rt<double>(L + 1, alpha, pb.P[0] - pk.P[0], pb.P[1] - pk.P[1], pb.P[2] - pk.P[2], pref, g.boys, R);
           // bra centres
           for (int cd = 0; cd < NCD; ++cd) {
@@ -134,23 +146,23 @@ __global__ __launch_bounds__(kGradThreads) void k_grad_2e(ClassLaunch c, Grad2eA
             const double* Kx = TK.row(0, ce[0], de[0]);
             const double* Ky = TK.row(1, ce[1], de[1]);
             const double* Kz = TK.row(2, ce[2], de[2]);
-            double G[nherm(5)];
+            double G[nherm(LAB + 1)];
             for (int t = 0; t <= LAB + 1; ++t)
               for (int u = 0; u + t <= LAB + 1; ++u)
                 for (int v = 0; v + u + t <= LAB + 1; ++v) {
-                  double s = 0.0;
+                  double sacc = 0.0;
                   for (int tt = 0; tt <= ce[0] + de[0]; ++tt)
                     for (int uu = 0; uu <= ce[1] + de[1]; ++uu) {
                       const double exy = Kx[tt] * Ky[uu];
                       for (int vv = 0; vv <= ce[2] + de[2]; ++vv) {
                         const double e = ((tt + uu + vv) & 1) ? -exy * Kz[vv] : exy * Kz[vv];
-                        s += e * R[hidx(t + tt, u + uu, v + vv)];
+                        sacc += e * R[hidx(t + tt, u + uu, v + vv)];
                       }
                     }
-                  G[hidx(t, u, v)] = s;
+                  G[hidx(t, u, v)] = sacc;
                 }
             for (int ab = 0; ab < NAB; ++ab) {
-              const double gm = gam[ab * NCD + cd];
+              const double gm = gamma(ab, cd);
               if (gm == 0.0) continue;
               int ae[3], be[3];
               for (int d = 0; d < 3; ++d) { ae[d] = cart_exp(LA, ab / NB, d); be[d] = cart_exp(LB, ab % NB, d); }
@@ -167,20 +179,20 @@ __global__ __launch_bounds__(kGradThreads) void k_grad_2e(ClassLaunch c, Grad2eA
             const double* Bx = TB.row(0, ae[0], be[0]);
             const double* By = TB.row(1, ae[1], be[1]);
             const double* Bz = TB.row(2, ae[2], be[2]);
-            double H[nherm(5)];
+            double H[nherm(LCD + 1)];
             for (int t = 0; t <= LCD + 1; ++t)
               for (int u = 0; u + t <= LCD + 1; ++u)
                 for (int v = 0; v + u + t <= LCD + 1; ++v) {
-                  double s = 0.0;
+                  double sacc = 0.0;
                   for (int tt = 0; tt <= ae[0] + be[0]; ++tt)
                     for (int uu = 0; uu <= ae[1] + be[1]; ++uu) {
                       const double exy = Bx[tt] * By[uu];
-                      for (int vv = 0; vv <= ae[2] + be[2]; ++vv) s += exy * Bz[vv] * R[hidx(t + tt, u + uu, v + vv)];
+                      for (int vv = 0; vv <= ae[2] + be[2]; ++vv) sacc += exy * Bz[vv] * R[hidx(t + tt, u + uu, v + vv)];
                     }
-                  H[hidx(t, u, v)] = ((t + u + v) & 1) ? -s : s;
+                  H[hidx(t, u, v)] = ((t + u + v) & 1) ? -sacc : sacc;
                 }
             for (int cd = 0; cd < NCD; ++cd) {
-              const double gm = gam[ab * NCD + cd];
+              const double gm = gamma(ab, cd);
               if (gm == 0.0) continue;
               int ce[3], de[3];
               for (int d = 0; d < 3; ++d) { ce[d] = cart_exp(LC, cd / ND, d); de[d] = cart_exp(LD, cd % ND, d); }
@@ -211,6 +223,15 @@ __global__ __launch_bounds__(kGradThreads) void k_grad_2e(ClassLaunch c, Grad2eA
   __syncthreads();
   for (int i = threadIdx.x; i < n3; i += blockDim.x)
     if (s_grad[i] != 0.0) unsafeAtomicAdd(&g.grad[i], s_grad[i]);
+}
+
+template <int LA, int LB, int LC, int LD>
+hipError_t launch_grad_2e_t(const ClassLaunch& c, const Grad2eArgs& a, hipStream_t s) {
+  const int waves = kGradThreads / 64;
+  const int grid = std::min(2048, (c.nblocks + waves - 1) / waves);
+  hipLaunchKernelGGL((k_grad_2e<LA, LB, LC, LD>), dim3(grid), dim3(kGradThreads), 3 * a.natoms * sizeof(double), s,
+                     c, a);
+  return hipGetLastError();
 }
 
 struct DPrimG {  // == gansu_primitive_shell layout
@@ -339,10 +360,19 @@ __global__ __launch_bounds__(kGradThreads) void k_grad_1e(OneIntArgs a, const do
 hipError_t launch_grad_2e(const ClassLaunch& c, const Grad2eArgs& a, hipStream_t s) {
   if (c.nblocks <= 0) return hipSuccess;
   if (a.natoms <= 0 || a.natoms > kGradMaxAtoms) return hipErrorInvalidValue;
-  const int waves = kGradThreads / 64;
-  const int grid = std::min(2048, (c.nblocks + waves - 1) / waves);
-  hipLaunchKernelGGL(k_grad_2e, dim3(grid), dim3(kGradThreads), 3 * a.natoms * sizeof(double), s, c, a);
-  return hipGetLastError();
+  const int key = ((c.LA * 3 + c.LB) * 3 + c.LC) * 3 + c.LD;
+#define GANSU_G(a_, b_, c_, d_) \
+  case ((a_ * 3 + b_) * 3 + c_) * 3 + d_: return launch_grad_2e_t<a_, b_, c_, d_>(c, a, s);
+  switch (key) {
+    GANSU_G(0, 0, 0, 0) GANSU_G(1, 0, 0, 0) GANSU_G(1, 0, 1, 0) GANSU_G(1, 1, 0, 0)
+    GANSU_G(1, 1, 1, 0) GANSU_G(1, 1, 1, 1) GANSU_G(2, 0, 0, 0) GANSU_G(2, 0, 1, 0)
+    GANSU_G(2, 0, 1, 1) GANSU_G(2, 0, 2, 0) GANSU_G(2, 1, 0, 0) GANSU_G(2, 1, 1, 0)
+    GANSU_G(2, 1, 1, 1) GANSU_G(2, 1, 2, 0) GANSU_G(2, 1, 2, 1) GANSU_G(2, 2, 0, 0)
+    GANSU_G(2, 2, 1, 0) GANSU_G(2, 2, 1, 1) GANSU_G(2, 2, 2, 0) GANSU_G(2, 2, 2, 1)
+    GANSU_G(2, 2, 2, 2)
+    default: return hipErrorInvalidValue;
+  }
+#undef GANSU_G
 }
 
 hipError_t launch_grad_1e(const OneIntArgs& a, const double* D, const double* W, double* grad, hipStream_t s) {

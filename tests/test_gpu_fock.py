@@ -133,6 +133,38 @@ def test_stored_equals_direct_config1(cuda):
         np.testing.assert_allclose(a, a.T, atol=1e-12 * np.abs(a).max())
 
 
+@pytest.mark.parametrize("fp32_thr", [1e-6, 1e-2])
+def test_stored_mixed_precision_jk(fp32_thr, cuda):
+    """precision=mixed on the stored path: quartets bounded below fp32_thr are
+    kept in FP32.  Each such integral has absolute error <= fp32_thr * 2^-24,
+    so J and K move by a correspondingly small amount; the FP32 share grows
+    with the threshold."""
+    Z, xyz = methane_water()
+    mol, _ = both(Z, xyz, "cc-pvdz")
+    D = _torch(random_density(mol.n_basis), cuda)
+    e64 = FockEngine(mol, eri_method="stored")
+    e64.precomputation()
+    J64, K64 = [x.cpu().numpy() for x in e64.compute_jk(D)]
+    emx = FockEngine(mol, eri_method="stored", precision="mixed", fp32_bound_threshold=fp32_thr)
+    emx.precomputation()
+    J, K = [x.cpu().numpy() for x in emx.compute_jk(D)]
+    st, st64 = emx.stats(), e64.stats()
+    assert st["contracted_quartets"] == st64["contracted_quartets"]
+    assert 0 < st["fp32_quartets"] < st["contracted_quartets"]
+    tol = 1e-4 * fp32_thr
+    assert np.abs(J - J64).max() <= tol and np.abs(K - K64).max() <= tol
+    assert np.abs(J - J64).max() > 0.0  # the FP32 part is really used
+
+
+def test_stored_mixed_precision_scf_config1(cuda):
+    """C10H22/cc-pVDZ, FP32 for quartets bounded < 1e-6: |E_mixed - E_fp64| <= 1 uEh."""
+    mol, _ = gm.config_molecule(1)
+    r64 = rhf(mol, Parameters(eri_method="stored", convergence_energy_threshold=1e-9))
+    rmx = rhf(mol, Parameters(eri_method="stored", precision="mixed", convergence_energy_threshold=1e-9))
+    assert r64["converged"] and rmx["converged"]
+    assert abs(rmx["e_total"] - r64["e_total"]) < 1e-6
+
+
 # ------------------------------------------------------------------- RI ----
 RI_SYSTEMS = {"h2o": lambda: gm.water(), "ch4h2o": lambda: methane_water()}
 
