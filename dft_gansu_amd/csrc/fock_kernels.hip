@@ -688,46 +688,85 @@ __global__ __launch_bounds__(kSlabThreads) void k_fock_slab(SlabLaunch c, EriArg
         const double fjk = pair_diff ? fbk : 0.0;
         const int bj = bbj[b];
         [[maybe_unused]] double out[MODE == kModeDirect ? NCOMP : 1];
-        // digestion of one quartet; V(comp) yields its integrals
+        // digestion of one quartet; V(comp) yields its integrals.  The D
+        // elements this ket needs are gathered from LDS into registers first
+        // (the K rows share the LDS array, so reads cannot be hoisted across
+        // the atomics otherwise) and K partial sums stay in registers until
+        // one LDS atomic per (row, ket function).
         auto digest = [&](auto&& V) {
+          constexpr int NAL = SPLIT ? 1 : NA;  // bra rows a of this lane
+          double dal[NAL][ND], dak[NAL][NC], dbl[NB][ND], dbk[NB][NC];
 #pragma unroll
-          for (int aa = 0; aa < (SPLIT ? 1 : NA); ++aa) {
+          for (int aa = 0; aa < NAL; ++aa) {
+            const int a = SPLIT ? sub : aa;
+#pragma unroll
+            for (int d = 0; d < ND; ++d) dal[aa][d] = Di[a * M + bl + d];
+#pragma unroll
+            for (int cc = 0; cc < NC; ++cc) dak[aa][cc] = Di[a * M + bk + cc];
+          }
+#pragma unroll
+          for (int bb = 0; bb < NB; ++bb) {
+#pragma unroll
+            for (int d = 0; d < ND; ++d) dbl[bb][d] = Dj[bb * M + bl + d];
+#pragma unroll
+            for (int cc = 0; cc < NC; ++cc) dbk[bb][cc] = Dj[bb * M + bk + cc];
+          }
+          double kac[NAL][NC], kbc[NB][NC], kad[NAL][ND], kbd[NB][ND];
+#pragma unroll
+          for (int aa = 0; aa < NAL; ++aa) {
+#pragma unroll
+            for (int cc = 0; cc < NC; ++cc) kac[aa][cc] = 0.0;
+#pragma unroll
+            for (int d = 0; d < ND; ++d) kad[aa][d] = 0.0;
+          }
+#pragma unroll
+          for (int bb = 0; bb < NB; ++bb) {
+#pragma unroll
+            for (int cc = 0; cc < NC; ++cc) kbc[bb][cc] = 0.0;
+#pragma unroll
+            for (int d = 0; d < ND; ++d) kbd[bb][d] = 0.0;
+          }
+#pragma unroll
+          for (int aa = 0; aa < NAL; ++aa) {
             const int a = SPLIT ? sub : aa;
 #pragma unroll
             for (int bb = 0; bb < NB; ++bb) {
-              const double dab = Di[a * M + bj + bb];
-              double kad[ND], kbd[ND];
-#pragma unroll
-              for (int d = 0; d < ND; ++d) { kad[d] = 0.0; kbd[d] = 0.0; }
+              [[maybe_unused]] double dab = 0.0;
+              if constexpr (MODE == kModeDirect) dab = Di[a * M + bj + bb];
               double jbs = 0.0;
 #pragma unroll
-              for (int cc = 0; cc < NC; ++cc) {
-                const double dbc = Dj[bb * M + bk + cc];
-                const double dac = Di[a * M + bk + cc];
-                double kac = 0.0, kbc = 0.0;
+              for (int cc = 0; cc < NC; ++cc)
 #pragma unroll
                 for (int d = 0; d < ND; ++d) {
                   const double v = V(((a * NB + bb) * NC + cc) * ND + d);
                   jbs += v * dcd[cc * ND + d];
                   if constexpr (MODE == kModeDirect) jk[cc * ND + d] += fjk * v * dab;
-                  kac += v * Dj[bb * M + bl + d];
-                  kbc += v * Di[a * M + bl + d];
-                  kad[d] += v * dbc;
-                  kbd[d] += v * dac;
+                  kac[aa][cc] += v * dbl[bb][d];
+                  kbc[bb][cc] += v * dal[aa][d];
+                  kad[aa][d] += v * dbk[bb][cc];
+                  kbd[bb][d] += v * dak[aa][cc];
                 }
-                unsafeAtomicAdd(&KiW[a * M + bk + cc], fo * kac);
-                if (ij_diff) unsafeAtomicAdd(&KjW[bb * M + bk + cc], fo * kbc);
-              }
               jb[b][SPLIT ? bb : a * NB + bb] += fbk * jbs;
-              if (kl_diff) {
-#pragma unroll
-                for (int d = 0; d < ND; ++d) {
-                  unsafeAtomicAdd(&KiW[a * M + bl + d], fo * kad[d]);
-                  if (ij_diff) unsafeAtomicAdd(&KjW[bb * M + bl + d], fo * kbd[d]);
-                }
-              }
             }
           }
+#pragma unroll
+          for (int aa = 0; aa < NAL; ++aa) {
+            const int a = SPLIT ? sub : aa;
+#pragma unroll
+            for (int cc = 0; cc < NC; ++cc) unsafeAtomicAdd(&KiW[a * M + bk + cc], fo * kac[aa][cc]);
+            if (kl_diff)
+#pragma unroll
+              for (int d = 0; d < ND; ++d) unsafeAtomicAdd(&KiW[a * M + bl + d], fo * kad[aa][d]);
+          }
+          if (ij_diff)
+#pragma unroll
+            for (int bb = 0; bb < NB; ++bb) {
+#pragma unroll
+              for (int cc = 0; cc < NC; ++cc) unsafeAtomicAdd(&KjW[bb * M + bk + cc], fo * kbc[bb][cc]);
+              if (kl_diff)
+#pragma unroll
+                for (int d = 0; d < ND; ++d) unsafeAtomicAdd(&KjW[bb * M + bl + d], fo * kbd[bb][d]);
+            }
         };
         if constexpr (MODE == kModeDirect) {
 #pragma unroll
