@@ -63,6 +63,7 @@ FockEngine::FockEngine(const gansu_primitive_shell* prims, size_t n_prims, const
       if (kv.second != "fp64" && kv.second != "mixed") throw Error(GANSU_E_PARAM, "precision must be fp64|mixed");
       mixed_ = kv.second == "mixed";
     } else if (kv.first == "fp32_bound_threshold") fp32_thr_ = std::stod(kv.second);
+    else if (kv.first == "autotune") autotune_ = kv.second != "0";
     else if (kv.first == "ri_k") {
       if (kv.second != "density" && kv.second != "occupied") throw Error(GANSU_E_PARAM, "ri_k must be density|occupied");
       ri_k_occ_ = kv.second == "occupied";
@@ -277,6 +278,12 @@ void FockEngine::precompute_quartets() {
     std::stable_sort(pc.sorted.begin(), pc.sorted.end(),
                      [&](int x, int y) { return sp_h_[x].qmax > sp_h_[y].qmax; });
     pc.list.upload(pc.sorted);
+    std::vector<int4> meta(pc.sorted.size());
+    for (size_t k = 0; k < pc.sorted.size(); ++k) {
+      const SPair& p = sp_h_[pc.sorted[k]];
+      meta[k] = make_int4(p.bi, p.bj, p.si != p.sj ? 1 : 0, pc.sorted[k]);
+    }
+    pc.meta.upload(meta);
   }
   // quartet classes (bra class A >= ket class B) and their work blocks; with
   // precision=mixed on the stored path each bra's kets split into an FP64
@@ -323,20 +330,11 @@ void FockEngine::precompute_quartets() {
       for (int mode : {(int)kModeDirect, (int)kModeStoredDigest})
         qc->slab_ok[mode] = slab_fits(qc->LA, qc->LB, qc->LC, qc->LD, mode, M_);
       for (const auto& br : qc->bras_h) qc->max_count = std::max(qc->max_count, br.count);
-      // slab work records: tiles of bras, each bra's kets cut into chunks of
-      // a few passes so that heavy classes with few bras still fill the GPU
+      // default slab work records: the largest bra tile, 4 passes of kets
       for (int mode : {(int)kModeDirect, (int)kModeStoredDigest}) {
         int bpw = 1, qpp = 256;
         slab_geometry(qc->LA, qc->LB, qc->LC, qc->LD, mode, &bpw, &qpp);
-        const int kch = 4 * qpp;
-        auto& w = qc->work_h[mode];
-        for (int b0 = 0; b0 < (int)qc->bras_h.size(); b0 += bpw) {
-          const int nb = std::min(bpw, (int)qc->bras_h.size() - b0);
-          int maxc = 0;
-          for (int b = 0; b < nb; ++b) maxc = std::max(maxc, qc->bras_h[b0 + b].count);
-          for (int k0 = 0; k0 < maxc; k0 += kch) w.push_back(SlabWork{b0, nb, k0, std::min(maxc, k0 + kch)});
-        }
-        qc->work[mode].upload(w);
+        make_slab_work(*qc, mode, bpw, 4 * qpp);
       }
       qclass_.push_back(std::move(qc));
     }
@@ -355,6 +353,75 @@ void FockEngine::precompute_quartets() {
     }
     GANSU_HIP(hipDeviceSynchronize());
   }
+  // autotune the digestion tiles on this geometry (the stored path's
+  // per-build kernel; the direct path's cost is the integrals themselves)
+  if (method_ == "stored" && autotune_) {
+    GANSU_HIP(launch_fill(d_Dp_.ptr, (size_t)M_ * M_, 1e-3, 0));
+    for (auto& qc : qclass_)
+      if (!qc->blocks_h.empty() && qc->slab_ok[kModeStoredDigest]) tune_slab(*qc, kModeStoredDigest);
+    GANSU_HIP(launch_zero(d_Jr_.ptr, (size_t)M_ * M_, 0));
+    GANSU_HIP(launch_zero(d_Kr_.ptr, (size_t)M_ * M_, 0));
+    GANSU_HIP(hipDeviceSynchronize());
+  }
+}
+
+void FockEngine::make_slab_work(QuartetClass& qc, int mode, int bpw, int kch) {
+  auto& w = qc.work_h[mode];
+  w.clear();
+  for (int b0 = 0; b0 < (int)qc.bras_h.size(); b0 += bpw) {
+    const int nb = std::min(bpw, (int)qc.bras_h.size() - b0);
+    int maxc = 0;
+    for (int b = 0; b < nb; ++b) maxc = std::max(maxc, qc.bras_h[b0 + b].count);
+    for (int k0 = 0; k0 < maxc; k0 += kch) w.push_back(SlabWork{b0, nb, k0, std::min(maxc, k0 + kch)});
+  }
+  qc.work[mode].upload(w);
+  qc.bpw[mode] = bpw;
+  qc.kch[mode] = kch;
+}
+
+SlabLaunch FockEngine::slab_launch(QuartetClass& qc, int mode) {
+  return SlabLaunch{qc.LA, qc.LB, qc.LC, qc.LD, qc.bras.ptr, (int)qc.bras_h.size(), qc.work[mode].ptr,
+                    (int)qc.work_h[mode].size(), pclass_[qc.A].list.ptr, pclass_[qc.B].list.ptr, qc.store.ptr,
+                    qc.max_count, qc.store32.ptr, pclass_[qc.B].meta.ptr, qc.bpw[mode]};
+}
+
+void FockEngine::tune_slab(QuartetClass& qc, int mode) {
+  int bpw_max = 1, qpp = 256;
+  slab_geometry(qc.LA, qc.LB, qc.LC, qc.LD, mode, &bpw_max, &qpp);
+  const EriArgs g{d_sp_.ptr, d_pr_.ptr, boys_.ptr, thr_, d_Dp_.ptr, M_, d_Jr_.ptr, d_Kr_.ptr};
+  hipEvent_t e0, e1;
+  GANSU_HIP(hipEventCreate(&e0));
+  GANSU_HIP(hipEventCreate(&e1));
+  float best = 1e30f;
+  int best_bpw = qc.bpw[mode], best_kch = qc.kch[mode];
+  for (int bpw = bpw_max; bpw >= 1; bpw /= 2)
+    for (int mult : {1, 2, 4, 8}) {
+      make_slab_work(qc, mode, bpw, mult * qpp);
+      const SlabLaunch c = slab_launch(qc, mode);
+      GANSU_HIP(launch_fock_slab(c, g, mode, 0));  // warm-up
+      float t = 1e30f;
+      for (int rep = 0; rep < 3; ++rep) {
+        GANSU_HIP(hipEventRecord(e0, 0));
+        GANSU_HIP(launch_fock_slab(c, g, mode, 0));
+        GANSU_HIP(hipEventRecord(e1, 0));
+        GANSU_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        GANSU_HIP(hipEventElapsedTime(&ms, e0, e1));
+        t = std::min(t, ms);
+      }
+      if (t < best) { best = t; best_bpw = bpw; best_kch = mult * qpp; }
+    }
+  make_slab_work(qc, mode, best_bpw, best_kch);
+  qc.t_slab = best;
+  const SlabLaunch c = slab_launch(qc, mode);
+  GANSU_HIP(launch_jket(c, g, 0));
+  GANSU_HIP(hipEventRecord(e0, 0));
+  GANSU_HIP(launch_jket(c, g, 0));
+  GANSU_HIP(hipEventRecord(e1, 0));
+  GANSU_HIP(hipEventSynchronize(e1));
+  GANSU_HIP(hipEventElapsedTime(&qc.t_jket, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
 }
 
 void FockEngine::build_four_center(const double* d_D, hipStream_t s) {
@@ -371,9 +438,12 @@ void FockEngine::build_four_center(const double* d_D, hipStream_t s) {
   for (auto& qc : qclass_) {
     if (qc->blocks_h.empty()) continue;
     const double ncomp = double(ncart(qc->LA) * ncart(qc->LB) * ncart(qc->LC) * ncart(qc->LD));
-    const double w = qc->n_quartets * ncomp * (mode == kModeDirect ? 10.0 : 1.0);
+    // tuned kernel times where the autotuner ran, else the integral count
+    const bool timed = mode == kModeStoredDigest && qc->t_slab > 0.f;
+    const double w = timed ? 1e9 * qc->t_slab : qc->n_quartets * ncomp * (mode == kModeDirect ? 10.0 : 1.0);
     items.push_back({qc.get(), 0, w});
-    if (mode == kModeStoredDigest && qc->slab_ok[mode]) items.push_back({qc.get(), 1, 0.5 * w});
+    if (mode == kModeStoredDigest && qc->slab_ok[mode])
+      items.push_back({qc.get(), 1, timed ? 1e9 * qc->t_jket : 0.5 * w});
   }
   const bool concurrent = !profiling_ && ev_fork_ != nullptr;
   std::vector<hipStream_t> st{s};
@@ -405,9 +475,7 @@ void FockEngine::build_four_center(const double* d_D, hipStream_t s) {
         mode == kModeStoredDigest ? ((qc->n_quartets - qc->n_fp32) * 8.0 + qc->n_fp32 * 4.0) * ncomp : 0.0;
     const double f32id = qc->n_fp32 > 0 && mode == kModeStoredDigest ? 1000000.0 : 0.0;
     const double cls = 1000 * qc->LA + 100 * qc->LB + 10 * qc->LC + qc->LD;
-    SlabLaunch c{qc->LA, qc->LB, qc->LC, qc->LD, qc->bras.ptr, (int)qc->bras_h.size(), qc->work[mode].ptr,
-                 (int)qc->work_h[mode].size(), pclass_[qc->A].list.ptr, pclass_[qc->B].list.ptr, qc->store.ptr,
-                 qc->max_count, qc->store32.ptr};
+    const SlabLaunch c = slab_launch(*qc, mode);
     if (it.kind == 1) {  // ket-major J of the stored blocks
       prof_begin(q, f32id + 300000.0 + 10000.0 * mode + cls, bytes, 0.0, qc->n_quartets);
       GANSU_HIP(launch_jket(c, g, q));

@@ -62,6 +62,9 @@ struct QuartetClass {
   bool slab_ok[3] = {true, true, true};  // per mode: the slab kernel has a variant whose LDS fits
   std::vector<SlabWork> work_h[3];  // per mode (direct, -, stored digest)
   DevBuf<SlabWork> work[3];
+  int bpw[3] = {1, 1, 1};  // bras per workgroup of work[mode]
+  int kch[3] = {0, 0, 0};  // kets per workgroup of work[mode]
+  float t_slab = 0.f, t_jket = 0.f;  // tuned kernel times (ms), launch-order costs
   int max_count = 0;
   DevBuf<double> store;  // stored integrals of this class (eri_method=stored)
   // precision=mixed: the quartets whose bound qmax_bra * qmax_ket is below
@@ -75,6 +78,7 @@ struct PairClass {
   int la, lb;
   std::vector<int> sorted;  // shell-pair ids sorted by qmax desc
   DevBuf<int> list;
+  DevBuf<int4> meta;  // per sorted position: {bi, bj, si != sj, id}
 };
 
 class FockEngine {
@@ -103,6 +107,12 @@ class FockEngine {
  private:
   void precompute_pairs();     // Schwarz factors, shell pairs, primitive-pair records
   void precompute_quartets();  // quartet classes, work blocks, stored integrals
+  // Slab work records of a class for bpw bras and kch kets per workgroup.
+  void make_slab_work(QuartetClass& qc, int mode, int bpw, int kch);
+  // Autotuner: times the class's slab kernel over candidate (bpw, kch) and
+  // keeps the fastest.
+  void tune_slab(QuartetClass& qc, int mode);
+  SlabLaunch slab_launch(QuartetClass& qc, int mode);
   void precompute_ri();
   void build_ri_k_occ(const double* d_C, int nocc, double* K, hipStream_t s);
   void ri_k_product(const double* A, int kdim, bool accumulate, double* K, hipStream_t s);
@@ -115,6 +125,7 @@ class FockEngine {
   bool mixed_ = false;
   double fp32_thr_ = 1e-6;
   bool ri_k_occ_ = false;  // RI-K from occupied orbitals when they are given
+  bool autotune_ = true;   // time candidate digestion tiles at precompute
   int M_ = 0;
   std::vector<gansu_primitive_shell> prims_;
   std::vector<gansu_shell_type_info> types_;

@@ -616,7 +616,7 @@ __global__ __launch_bounds__(kSlabThreads) void k_fock_slab(SlabLaunch c, EriArg
   const int nb = wk.nb;
   // per owned bra: [Ki NA*M][Kj NB*M][Di NA*M][Dj NB*M], then J of the bra block
   const int slab = 2 * (NA + NB) * M;
-  double* jbl = lds + BPW * slab;  // [BPW][NAB]
+  double* jbl = lds + c.bpw * slab;  // [bpw][NAB]
   int bid[BPW], bbi[BPW], bbj[BPW], cnt[BPW], fblk[BPW], spl[BPW], fblk32[BPW];
   bool bdiff[BPW];
   int maxc = 0;
@@ -659,10 +659,10 @@ __global__ __launch_bounds__(kSlabThreads) void k_fock_slab(SlabLaunch c, EriArg
   for (int k0 = wk.k_begin; k0 < maxc; k0 += QPP) {
     const int kk = k0 + qi;
     if (sub < LPQ && kk < maxc) {
-      const int ket_id = c.ket_list[kk];
-      const SPair ket = g.sp[ket_id];
-      const bool kl_diff = ket.si != ket.sj;
-      const int bk = ket.bi, bl = ket.bj;
+      const int4 km = c.ket_meta[kk];
+      const int ket_id = km.w;
+      const bool kl_diff = km.z != 0;
+      const int bk = km.x, bl = km.y;
       double dcd[NCD], jk[NCD];
 #pragma unroll
       for (int cc = 0; cc < NC; ++cc)
@@ -771,6 +771,7 @@ __global__ __launch_bounds__(kSlabThreads) void k_fock_slab(SlabLaunch c, EriArg
         if constexpr (MODE == kModeDirect) {
 #pragma unroll
           for (int k = 0; k < NCOMP; ++k) out[k] = 0.0;
+          const SPair ket = g.sp[ket_id];
           eri_block_t<LA, LB, LC, LD>(g.sp[bid[b]], ket, ket.nprim, g, out);
           digest([&](int comp) -> double { return out[comp]; });
         } else if (kk < spl[b]) {
@@ -817,19 +818,22 @@ __global__ __launch_bounds__(kSlabThreads) void k_fock_slab(SlabLaunch c, EriArg
 }
 
 template <int LA, int LB, int LC, int LD, int MODE>
-size_t slab_lds_t(int M) {
+size_t slab_lds_t(int M, int bpw) {
   using S = SlabShape<LA, LB, LC, LD, MODE>;
-  return (size_t)S::BPW * (2ull * (S::NA + S::NB) * (size_t)M + S::NAB) * sizeof(double);
+  return (size_t)bpw * (2ull * (S::NA + S::NB) * (size_t)M + S::NAB) * sizeof(double);
 }
 
 template <int LA, int LB, int LC, int LD>
 hipError_t launch_slab_t(const SlabLaunch& c, const EriArgs& a, int mode, size_t max_lds, hipStream_t s) {
+  if (c.nwork == 0) return hipSuccess;
   if (mode == kModeDirect) {
-    const size_t lds = slab_lds_t<LA, LB, LC, LD, kModeDirect>(a.M);
+    if (c.bpw < 1 || c.bpw > SlabShape<LA, LB, LC, LD, kModeDirect>::BPW) return hipErrorInvalidValue;
+    const size_t lds = slab_lds_t<LA, LB, LC, LD, kModeDirect>(a.M, c.bpw);
     if (lds > max_lds) return hipErrorInvalidValue;
     hipLaunchKernelGGL((k_fock_slab<LA, LB, LC, LD, kModeDirect>), dim3(c.nwork), dim3(kSlabThreads), lds, s, c, a);
   } else {
-    const size_t lds = slab_lds_t<LA, LB, LC, LD, kModeStoredDigest>(a.M);
+    if (c.bpw < 1 || c.bpw > SlabShape<LA, LB, LC, LD, kModeStoredDigest>::BPW) return hipErrorInvalidValue;
+    const size_t lds = slab_lds_t<LA, LB, LC, LD, kModeStoredDigest>(a.M, c.bpw);
     if (lds > max_lds) return hipErrorInvalidValue;
     hipLaunchKernelGGL((k_fock_slab<LA, LB, LC, LD, kModeStoredDigest>), dim3(c.nwork), dim3(kSlabThreads), lds, s,
                        c, a);
@@ -857,10 +861,11 @@ __global__ __launch_bounds__(64) void k_jket(SlabLaunch c, EriArgs g, int bras_p
   double jk = 0.0;
   int ket_id = -1;
   bool kl_diff = false;
+  int4 km = make_int4(0, 0, 0, -1);
   if (t < c.max_count) {
-    ket_id = c.ket_list[t];
-    const SPair ket = g.sp[ket_id];
-    kl_diff = ket.si != ket.sj;
+    km = c.ket_meta[t];
+    ket_id = km.w;
+    kl_diff = km.z != 0;
   }
   for (int s = s0; s < s1; ++s) {
     const BraRec br = c.bras[s];
@@ -889,10 +894,7 @@ __global__ __launch_bounds__(64) void k_jket(SlabLaunch c, EriArgs g, int bras_p
     }
     jk += f * acc;
   }
-  if (ket_id >= 0 && jk != 0.0) {
-    const SPair ket = g.sp[ket_id];
-    unsafeAtomicAdd(&g.Jr[(size_t)(ket.bi + cd / ND) * M + ket.bj + cd % ND], jk);
-  }
+  if (ket_id >= 0 && jk != 0.0) unsafeAtomicAdd(&g.Jr[(size_t)(km.x + cd / ND) * M + km.y + cd % ND], jk);
 }
 
 template <int LA, int LB, int LC, int LD>
@@ -1002,6 +1004,12 @@ size_t slab_lds_bytes(int LA, int LB, int LC, int LD, int mode, int M) {
   return (size_t)bpw * (2ull * (NA + NB) * (size_t)M + (size_t)NA * NB) * sizeof(double);
 }
 
+int slab_max_bpw(int LA, int LB, int LC, int LD, int mode) {
+  int bpw = 1, qpp = 0;
+  slab_geometry(LA, LB, LC, LD, mode, &bpw, &qpp);
+  return bpw;
+}
+
 bool slab_fits(int LA, int LB, int LC, int LD, int mode, int M) {
   const size_t need = slab_lds_bytes(LA, LB, LC, LD, mode, M);
   if (need == 0) return false;
@@ -1051,6 +1059,18 @@ hipError_t launch_fock_slab(const SlabLaunch& c, const EriArgs& a, int mode, hip
 hipError_t launch_scale_density(const double* D, const double* w, int M, double* Dp, hipStream_t s) {
   const size_t n = (size_t)M * M;
   hipLaunchKernelGGL(k_scale_density, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, D, w, M, Dp);
+  return hipGetLastError();
+}
+
+__global__ void k_fill(double* __restrict__ p, size_t n, double v) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
+hipError_t launch_fill(double* p, size_t n, double v, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  size_t nb = (n + 255) / 256;
+  if (nb > 4096) nb = 4096;
+  hipLaunchKernelGGL(k_fill, dim3((unsigned)nb), dim3(256), 0, s, p, n, v);
   return hipGetLastError();
 }
 

@@ -58,7 +58,12 @@ struct SlabLaunch {
   const double* store;
   int max_count;  // largest ket count of any bra (ket-major J pass)
   const float* store32;  // FP32 blocks of the kets at or above each bra's split
+  const int4* ket_meta;  // per ket position: {first basis fn i, first basis fn j, i != j, shell-pair id}
+  int bpw;               // bras per workgroup of the SlabWork records (1 .. slab_max_bpw)
 };
+
+// Largest bras-per-workgroup the slab kernel of a class is compiled for.
+int slab_max_bpw(int LA, int LB, int LC, int LD, int mode);
 
 // Geometry of the slab kernel of a class: bras per workgroup and kets a
 // workgroup visits per pass (used by the host to cut SlabWork records).
@@ -104,6 +109,7 @@ bool slab_available(int LA, int LB, int LC, int LD, int mode);
 hipError_t launch_jket(const SlabLaunch& c, const EriArgs& a, hipStream_t s);
 hipError_t launch_scale_density(const double* D, const double* w, int M, double* Dp, hipStream_t s);
 hipError_t launch_zero(double* p, size_t n, hipStream_t s);
+hipError_t launch_fill(double* p, size_t n, double v, hipStream_t s);
 // J = w w sym(Jr), K = w w sym(Kr); if F: F = H + J - K/2 (J, K may be null then)
 hipError_t launch_finalize(const double* Jr, const double* Kr, const double* w, int M, const double* H, double* J,
                            double* K, double* F, hipStream_t s);

@@ -55,12 +55,13 @@ class FockEngine:
     """Fock-build engine over a molecule (and an auxiliary basis for RI)."""
 
     def __init__(self, mol, aux=None, eri_method="stored", schwarz_screening_threshold=1e-12, precision="fp64",
-                 fp32_bound_threshold=1e-6, ri_k="density"):
+                 fp32_bound_threshold=1e-6, ri_k="density", autotune=True):
         self.mol = mol
         self.aux = aux
         self.M = mol.n_basis
         opts = (f"eri_method={eri_method};schwarz_screening_threshold={schwarz_screening_threshold!r};"
-                f"precision={precision};fp32_bound_threshold={fp32_bound_threshold!r};ri_k={ri_k}")
+                f"precision={precision};fp32_bound_threshold={fp32_bound_threshold!r};ri_k={ri_k};"
+                f"autotune={int(bool(autotune))}")
         self._h = ctypes.c_void_p()
         a = aux
         check(lib().gansu_fock_create(
