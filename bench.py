@@ -68,6 +68,104 @@ def cpu_baseline(cfg, target_s):
                        f"({dt:.1f} s measured, x{stride} extrapolated; full build ~{full_s:.0f} s)")
 
 
+def ri_leg(log_fn):
+    """64-water cc-pVDZ + cc-pVDZ-RIFIT: the plain FP64 GPU path (four-centre
+    direct build, the drop-in) against the RI-JK forms, and the mixed RI SCF
+    against the FP64 RI SCF."""
+    import numpy as np
+    import torch
+    from dft_gansu_amd import FockEngine, Parameters, rhf
+    from dft_gansu_amd import molecule as gm
+    mol, aux = gm.config_molecule(4)
+    M, nocc = mol.n_basis, mol.n_electrons // 2
+    rng = np.random.default_rng(5)
+    C = torch.tensor(np.linalg.qr(rng.normal(size=(M, M)))[0], dtype=torch.float64, device="cuda")
+    D = (2.0 * C[:, :nocc] @ C[:, :nocc].T).contiguous()
+    H = torch.zeros_like(D)
+    out = {"workload": "64 H2O cc-pVDZ (M=%d) + cc-pVDZ-RIFIT (Maux=%d), nocc=%d" % (M, aux.n_basis, nocc)}
+
+    def timed(fn, n):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(n):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / n
+
+    e = FockEngine(mol, eri_method="direct")
+    e.precomputation()
+    out["fp64_direct_build_s"] = round(timed(lambda: e.compute_fock_matrix(D, H), 2), 4)
+    del e
+    log_fn(f"[bench] 64-water FP64 direct build {out['fp64_direct_build_s']} s")
+    for name, kw in (("ri_fp64_density", dict(precision="fp64", ri_k="density")),
+                     ("ri_fp64_occupied", dict(precision="fp64", ri_k="occupied")),
+                     ("ri_mixed", dict(precision="mixed"))):
+        e = FockEngine(mol, aux=aux, eri_method="ri", **kw)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        e.precomputation()
+        torch.cuda.synchronize()
+        pre = time.perf_counter() - t0
+        if name == "ri_fp64_density":
+            t = timed(lambda: e.compute_fock_matrix(D, H), 1)
+        else:
+            t = timed(lambda: e.compute_fock_matrix_occ(D, C, nocc, H), 3)
+        out[name + "_build_s"] = round(t, 4)
+        out[name + "_precompute_s"] = round(pre, 3)
+        del e
+        torch.cuda.empty_cache()
+        log_fn(f"[bench] 64-water {name} build {t:.4f} s (precompute {pre:.2f} s)")
+    out["speedup_mixed_ri_vs_fp64_direct"] = round(out["fp64_direct_build_s"] / out["ri_mixed_build_s"], 2)
+    out["speedup_mixed_ri_vs_fp64_ri_density"] = round(out["ri_fp64_density_build_s"] / out["ri_mixed_build_s"], 2)
+    scf = {}
+    for name, kw in (("fp64", dict(ri_k="occupied")), ("mixed", dict(precision="mixed"))):
+        t0 = time.perf_counter()
+        r = rhf(mol, Parameters(eri_method="ri", convergence_energy_threshold=1e-8, maxiter=60, **kw), aux=aux)
+        scf[name] = dict(wall_s=round(time.perf_counter() - t0, 3), energy=r["e_total"], iterations=r["iterations"],
+                         converged=r["converged"])
+        log_fn(f"[bench] 64-water RI SCF {name}: {scf[name]}")
+    scf["dE_mixed_vs_fp64"] = scf["mixed"]["energy"] - scf["fp64"]["energy"]
+    out["scf"] = scf
+    return out
+
+
+def gradient_leg(log_fn):
+    """C10H22/6-31G*: SCF then the analytic gradient; gradient s/call is the
+    backward work alone (two-electron + one-electron kernels) at the converged
+    density."""
+    import numpy as np
+    import torch
+    from dft_gansu_amd import FockEngine, Parameters, oneint_gradient, rhf, rhf_gradient
+    from dft_gansu_amd import molecule as gm
+    mol, _ = gm.config_molecule(3)
+    prm = Parameters(convergence_energy_threshold=1e-11, maxiter=200)
+    t0 = time.perf_counter()
+    r = rhf_gradient(mol, prm)
+    wall = time.perf_counter() - t0
+    D = torch.tensor(rhf(mol, prm)["D"], dtype=torch.float64, device="cuda")
+    W = (0.5 * D @ D).contiguous()  # any symmetric W: the kernel cost does not depend on it
+    eng = FockEngine(mol, eri_method="direct")
+    eng.precomputation()
+    eng.gradient_2e(D)
+    oneint_gradient(mol, D, W)
+    torch.cuda.synchronize()
+    n = 3
+    t0 = time.perf_counter()
+    for _ in range(n):
+        eng.gradient_2e(D)
+        oneint_gradient(mol, D, W)
+    torch.cuda.synchronize()
+    per = (time.perf_counter() - t0) / n
+    g = r["grad"]
+    out = dict(workload=f"C10H22 6-31G* RHF (M={mol.n_basis}, {mol.n_atoms} atoms)", gradient_s_per_call=round(per, 4),
+               scf_plus_gradient_wall_s=round(wall, 3), energy=r["e_total"],
+               max_abs_gradient=float(np.max(np.abs(g))),
+               translational_residual=float(np.max(np.abs(g.sum(axis=0)))))
+    log_fn(f"[bench] gradient C10H22/6-31G*: {per * 1e3:.1f} ms/call, SCF+gradient {wall:.2f} s")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -80,6 +178,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-scf", action="store_true")
     ap.add_argument("--no-direct", action="store_true")
+    ap.add_argument("--no-ri", action="store_true", help="skip the 64-water RI-JK leg")
+    ap.add_argument("--no-grad", action="store_true", help="skip the C10H22/6-31G* gradient leg")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision store leg")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the Fock build runs on a single MI355X in this build "
@@ -180,6 +281,40 @@ def main():
                      scf_iterations=r["iterations"], scf_converged=r["converged"])
         log(f"[bench] SCF {extra['scf_wall_s']} s, E = {r['e_total']:.10f} ({r['iterations']} it)")
 
+    if not args.no_mixed and args.precision == "fp64" and args.method == "stored":
+        em = FockEngine(mol, eri_method="stored", precision="mixed")
+        em.precomputation()
+        for _ in range(args.warmup):
+            em.compute_fock_matrix(D, H, F, stream=stream)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            em.compute_fock_matrix(D, H, F, stream=stream)
+        torch.cuda.synchronize()
+        tm = (time.perf_counter() - t0) / args.steps
+        sm = em.stats()
+        del em
+        torch.cuda.empty_cache()
+        mixed = dict(fock_builds_per_s=round(1.0 / tm, 3), ms_per_build=round(tm * 1e3, 4),
+                     stored_bytes=sm["stored_bytes"], fp32_quartets=sm["fp32_quartets"],
+                     fp32_bound_threshold=1e-6)
+        if not args.no_scf:
+            t0 = time.perf_counter()
+            r = rhf(mol, Parameters(eri_method="stored", precision="mixed", convergence_energy_threshold=1e-8,
+                                    maxiter=100))
+            mixed.update(scf_wall_s=round(time.perf_counter() - t0, 3), scf_energy=r["e_total"],
+                         scf_iterations=r["iterations"])
+            if "scf_energy" in extra:
+                mixed["dE_vs_fp64"] = r["e_total"] - extra["scf_energy"]
+        extra["mixed"] = mixed
+        log(f"[bench] mixed store: {mixed}")
+    if not args.no_grad:
+        extra["gradient"] = gradient_leg(log)
+        torch.cuda.empty_cache()
+    if not args.no_ri:
+        extra["ri_64water"] = ri_leg(log)
+        torch.cuda.empty_cache()
+
     cpu = None
     if not args.no_cpu:
         log("[bench] CPU baseline (oracle) ...")
@@ -187,7 +322,7 @@ def main():
         log(f"[bench] CPU baseline {cpu['value']:.3e} builds/s on {cpu['cores']} threads")
 
     out = {
-        "metric": "Fock builds/s (SCF wall s, mixed vs FP64 on GPU)",
+        "metric": "Fock builds/s + SCF wall s at |dE|<=1uEh, mixed vs FP64 on GPU; gradient s/call",
         "value": round(args.steps / elapsed, 3),
         "unit": "Fock builds/s",
         "n_gpus": 1,
