@@ -33,7 +33,9 @@ void FockEngine::gradient_2e(const double* d_D, int natoms, double* d_grad, hipS
     prof_begin(s, 400000.0 + 1000 * qc->LA + 100 * qc->LB + 10 * qc->LC + qc->LD, 0.0, 0.0, qc->n_quartets);
     const ClassLaunch c{qc->LA, qc->LB, qc->LC, qc->LD, qc->blocks.ptr, (int)qc->blocks_h.size(),
                         pclass_[qc->A].list.ptr, pclass_[qc->B].list.ptr, nullptr};
-    GANSU_HIP(launch_grad_2e(c, g, s));
+    int maxnp = 1;
+    for (const auto& br : qc->bras_h) maxnp = std::max(maxnp, sp_h_[pclass_[qc->A].sorted[br.bra_pos]].nprim);
+    GANSU_HIP(launch_grad_2e(c, g, maxnp, s));
     prof_end(s);
   }
 }

@@ -8,17 +8,22 @@
 // one order higher, every derivative integral is a contraction of the same
 // intermediates.
 //
-// Two-electron part: one lane = one contracted shell quartet, over the same
-// class-sorted work blocks as the direct Fock build.  Per primitive quartet:
+// Two-electron part (template in gradient_2e_impl.hpp, instances in
+// gradient_2e_*.hip): one wave = one work block of the direct Fock build (a
+// bra shell pair and up to 64 kets, one per lane) and one primitive pair of
+// the bra; classes with La+Lb+Lc+Ld <= 4 run a fully unrolled body whose
+// tables stay in registers.  Per primitive quartet:
 //   G_cd(tuv) = sum (-1)^(tau+nu+phi) E^cd_(tau nu phi) R_(t+tau, u+nu, v+phi)
 // gives both bra-centre derivatives, the mirrored H_ab(tau nu phi) gives the
 // first ket centre, and the last centre follows from translational invariance
 // (the four centre derivatives of any primitive integral sum to zero).  The
 // density enters through Gamma_abcd = 1/2 D_ab D_cd - 1/8 (D_ac D_bd + D_ad D_bc)
 // and the 8-fold degeneracy of the quartet.  Four waves per workgroup walk the
-// blocks; per-atom accumulators live in LDS and reach HBM once per workgroup.
+// work items; per-atom accumulators live in LDS and reach HBM once per
+// workgroup.  This file holds the one-electron kernel and the dispatch.
 #include <hip/hip_runtime.h>
 
+#include "gradient_2e_impl.hpp"
 #include "gradient_kernels.hpp"
 #include "md_device.hpp"
 
@@ -26,213 +31,9 @@ namespace gansu {
 
 using namespace dev;
 
+using namespace grad_detail;
+
 namespace {
-
-constexpr int kGradThreads = 256;
-
-__device__ __forceinline__ double wave_sum_g(double v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
-}
-
-// sum_{t<=nt, u<=nu, v<=nv} Ex[t] Ey[u] Ez[v] G[hidx(t,u,v)]
-__device__ inline double contract_e(const double* Ex, int nt, const double* Ey, int nu, const double* Ez, int nv,
-                                    const double* G) {
-  double s = 0.0;
-  for (int t = 0; t <= nt; ++t)
-    for (int u = 0; u <= nu; ++u) {
-      const double exy = Ex[t] * Ey[u];
-      for (int v = 0; v <= nv; ++v) s += exy * Ez[v] * G[hidx(t, u, v)];
-    }
-  return s;
-}
-
-// E tables of one primitive pair built to (la+1, lb+1): row (i, j) of
-// direction d starts at E[d] + (i nj + j) nt.
-struct ETab {
-  const double* E[3];
-  int nj, nt;
-  __device__ const double* row(int d, int i, int j) const { return E[d] + (i * nj + j) * nt; }
-};
-
-__device__ inline double pair_int(const ETab& T, const int* ea, const int* eb, const double* G) {
-  return contract_e(T.row(0, ea[0], eb[0]), ea[0] + eb[0], T.row(1, ea[1], eb[1]), ea[1] + eb[1],
-                    T.row(2, ea[2], eb[2]), ea[2] + eb[2], G);
-}
-
-// Derivative along direction d of the first (which = 0) or second centre of
-// the pair, exponent ex of that primitive.
-__device__ inline double pair_deriv(const ETab& T, int which, int d, double ex, int* ea, int* eb, const double* G) {
-  int* e = which == 0 ? ea : eb;
-  const int e0 = e[d];
-  e[d] = e0 + 1;
-  double v = 2.0 * ex * pair_int(T, ea, eb, G);
-  if (e0 > 0) {
-    e[d] = e0 - 1;
-    v -= e0 * pair_int(T, ea, eb, G);
-  }
-  e[d] = e0;
-  return v;
-}
-
-// Per-class instance: private arrays sized for the class (the largest,
-// (dd|dd), needs about 10 KB per lane; the s/p classes stay in registers).
-template <int LA, int LB, int LC, int LD>
-__global__ __launch_bounds__(kGradThreads) void k_grad_2e(ClassLaunch c, Grad2eArgs g) {
-  extern __shared__ double s_grad[];
-  constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
-  constexpr int NAB = NA * NB, NCD = NC * ND;
-  constexpr int LAB = LA + LB, LCD = LC + LD, L = LAB + LCD;
-  constexpr int NEB = (LA + 2) * (LB + 2) * (LAB + 3), NEK = (LC + 2) * (LD + 2) * (LCD + 3);
-  const int n3 = 3 * g.natoms;
-  for (int i = threadIdx.x; i < n3; i += blockDim.x) s_grad[i] = 0.0;
-  __syncthreads();
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int nw = blockDim.x >> 6;
-  const int M = g.M;
-  for (int b = blockIdx.x * nw + wave; b < c.nblocks; b += gridDim.x * nw) {  // wave-uniform loop
-    const BlockRec blk = c.blocks[b];
-    const bool active = lane < blk.count;
-    const int bra_id = c.bra_list[blk.bra_pos];
-    const int ket_id = c.ket_list[blk.ket_start + (active ? lane : 0)];
-    const SPair bra = g.sp[bra_id];
-    const SPair ket = g.sp[ket_id];
-    double gA[3] = {0.0, 0.0, 0.0}, gB[3] = {0.0, 0.0, 0.0}, gC[3] = {0.0, 0.0, 0.0};
-    if (active) {
-      // density blocks of Gamma_abcd = 1/2 D_ab D_cd - 1/8 (D_ac D_bd + D_ad D_bc)
-      double dab[NAB], dcd[NCD], dac[NA * NC], dbd[NB * ND], dad[NA * ND], dbc[NB * NC];
-      {
-        const double* D = g.Dp;
-        const size_t ra = bra.bi, rb = bra.bj, rc = ket.bi, rd = ket.bj;
-        for (int a = 0; a < NA; ++a) {
-          for (int bb = 0; bb < NB; ++bb) dab[a * NB + bb] = D[(ra + a) * M + rb + bb];
-          for (int cc = 0; cc < NC; ++cc) dac[a * NC + cc] = D[(ra + a) * M + rc + cc];
-          for (int dd = 0; dd < ND; ++dd) dad[a * ND + dd] = D[(ra + a) * M + rd + dd];
-        }
-        for (int bb = 0; bb < NB; ++bb) {
-          for (int dd = 0; dd < ND; ++dd) dbd[bb * ND + dd] = D[(rb + bb) * M + rd + dd];
-          for (int cc = 0; cc < NC; ++cc) dbc[bb * NC + cc] = D[(rb + bb) * M + rc + cc];
-        }
-        for (int cc = 0; cc < NC; ++cc)
-          for (int dd = 0; dd < ND; ++dd) dcd[cc * ND + dd] = D[(rc + cc) * M + rd + dd];
-      }
-      auto gamma = [&](int ab, int cd) {
-        const int a = ab / NB, bb = ab % NB, cc = cd / ND, dd = cd % ND;
-        return 0.5 * dab[ab] * dcd[cd] - 0.125 * (dac[a * NC + cc] * dbd[bb * ND + dd] + dad[a * ND + dd] * dbc[bb * NC + cc]);
-      };
-      for (int ib = 0; ib < bra.nprim; ++ib) {
-        const PairRec pb = g.pr[bra.prim_begin + ib];
-        const double exa = g.alpha[bra.prim_begin + ib], exb = pb.p - exa;
-        double Eb[3][NEB];
-        for (int d = 0; d < 3; ++d) hermite_e_rt<double>(LA + 1, LB + 1, pb.p, pb.PA[d], pb.PB[d], Eb[d]);
-        const ETab TB{{Eb[0], Eb[1], Eb[2]}, LB + 2, LAB + 3};
-        for (int ik = 0; ik < ket.nprim; ++ik) {
-          const PairRec pk = g.pr[ket.prim_begin + ik];
-          if (pb.Q * pk.Q < g.thr) continue;
-          const double exc = g.alpha[ket.prim_begin + ik];
-          double Ek[3][NEK];
-          for (int d = 0; d < 3; ++d) hermite_e_rt<double>(LC + 1, LD + 1, pk.p, pk.PA[d], pk.PB[d], Ek[d]);
-          const ETab TK{{Ek[0], Ek[1], Ek[2]}, LD + 2, LCD + 3};
-          const double p = pb.p, q = pk.p;
-          const double alpha = p * q / (p + q);
-          const double pref = kTwoPi25 / (p * q * sqrt(p + q)) * pb.Kc * pk.Kc;
-          double R[nherm(L + 1)];
-          hermite_r_rt<double>(L + 1, alpha, pb.P[0] - pk.P[0], pb.P[1] - pk.P[1], pb.P[2] - pk.P[2], pref, g.boys, R);
-          // bra centres
-          for (int cd = 0; cd < NCD; ++cd) {
-            int ce[3], de[3];
-            for (int d = 0; d < 3; ++d) { ce[d] = cart_exp(LC, cd / ND, d); de[d] = cart_exp(LD, cd % ND, d); }
-            const double* Kx = TK.row(0, ce[0], de[0]);
-            const double* Ky = TK.row(1, ce[1], de[1]);
-            const double* Kz = TK.row(2, ce[2], de[2]);
-            double G[nherm(LAB + 1)];
-            for (int t = 0; t <= LAB + 1; ++t)
-              for (int u = 0; u + t <= LAB + 1; ++u)
-                for (int v = 0; v + u + t <= LAB + 1; ++v) {
-                  double sacc = 0.0;
-                  for (int tt = 0; tt <= ce[0] + de[0]; ++tt)
-                    for (int uu = 0; uu <= ce[1] + de[1]; ++uu) {
-                      const double exy = Kx[tt] * Ky[uu];
-                      for (int vv = 0; vv <= ce[2] + de[2]; ++vv) {
-                        const double e = ((tt + uu + vv) & 1) ? -exy * Kz[vv] : exy * Kz[vv];
-                        sacc += e * R[hidx(t + tt, u + uu, v + vv)];
-                      }
-                    }
-                  G[hidx(t, u, v)] = sacc;
-                }
-            for (int ab = 0; ab < NAB; ++ab) {
-              const double gm = gamma(ab, cd);
-              if (gm == 0.0) continue;
-              int ae[3], be[3];
-              for (int d = 0; d < 3; ++d) { ae[d] = cart_exp(LA, ab / NB, d); be[d] = cart_exp(LB, ab % NB, d); }
-              for (int d = 0; d < 3; ++d) {
-                gA[d] += gm * pair_deriv(TB, 0, d, exa, ae, be, G);
-                gB[d] += gm * pair_deriv(TB, 1, d, exb, ae, be, G);
-              }
-            }
-          }
-          // first ket centre
-          for (int ab = 0; ab < NAB; ++ab) {
-            int ae[3], be[3];
-            for (int d = 0; d < 3; ++d) { ae[d] = cart_exp(LA, ab / NB, d); be[d] = cart_exp(LB, ab % NB, d); }
-            const double* Bx = TB.row(0, ae[0], be[0]);
-            const double* By = TB.row(1, ae[1], be[1]);
-            const double* Bz = TB.row(2, ae[2], be[2]);
-            double H[nherm(LCD + 1)];
-            for (int t = 0; t <= LCD + 1; ++t)
-              for (int u = 0; u + t <= LCD + 1; ++u)
-                for (int v = 0; v + u + t <= LCD + 1; ++v) {
-                  double sacc = 0.0;
-                  for (int tt = 0; tt <= ae[0] + be[0]; ++tt)
-                    for (int uu = 0; uu <= ae[1] + be[1]; ++uu) {
-                      const double exy = Bx[tt] * By[uu];
-                      for (int vv = 0; vv <= ae[2] + be[2]; ++vv) sacc += exy * Bz[vv] * R[hidx(t + tt, u + uu, v + vv)];
-                    }
-                  H[hidx(t, u, v)] = ((t + u + v) & 1) ? -sacc : sacc;
-                }
-            for (int cd = 0; cd < NCD; ++cd) {
-              const double gm = gamma(ab, cd);
-              if (gm == 0.0) continue;
-              int ce[3], de[3];
-              for (int d = 0; d < 3; ++d) { ce[d] = cart_exp(LC, cd / ND, d); de[d] = cart_exp(LD, cd % ND, d); }
-              for (int d = 0; d < 3; ++d) gC[d] += gm * pair_deriv(TK, 0, d, exc, ce, de, H);
-            }
-          }
-        }
-      }
-    }
-    const double deg = (bra.si != bra.sj ? 2.0 : 1.0) * (ket.si != ket.sj ? 2.0 : 1.0) * (bra_id != ket_id ? 2.0 : 1.0);
-    const int atA = g.shell_atom[bra.si], atB = g.shell_atom[bra.sj];
-    for (int d = 0; d < 3; ++d) {
-      const double sa = wave_sum_g(gA[d] * deg);  // the bra is wave-uniform
-      const double sb = wave_sum_g(gB[d] * deg);
-      if (lane == 0) {
-        atomicAdd(&s_grad[3 * atA + d], sa);
-        atomicAdd(&s_grad[3 * atB + d], sb);
-      }
-    }
-    if (active) {
-      const int atC = g.shell_atom[ket.si], atD = g.shell_atom[ket.sj];
-      for (int d = 0; d < 3; ++d) {
-        atomicAdd(&s_grad[3 * atC + d], gC[d] * deg);
-        atomicAdd(&s_grad[3 * atD + d], -(gA[d] + gB[d] + gC[d]) * deg);
-      }
-    }
-  }
-  __syncthreads();
-  for (int i = threadIdx.x; i < n3; i += blockDim.x)
-    if (s_grad[i] != 0.0) unsafeAtomicAdd(&g.grad[i], s_grad[i]);
-}
-
-template <int LA, int LB, int LC, int LD>
-hipError_t launch_grad_2e_t(const ClassLaunch& c, const Grad2eArgs& a, hipStream_t s) {
-  const int waves = kGradThreads / 64;
-  const int grid = std::min(2048, (c.nblocks + waves - 1) / waves);
-  hipLaunchKernelGGL((k_grad_2e<LA, LB, LC, LD>), dim3(grid), dim3(kGradThreads), 3 * a.natoms * sizeof(double), s,
-                     c, a);
-  return hipGetLastError();
-}
 
 struct DPrimG {  // == gansu_primitive_shell layout
   double exponent, coefficient, x, y, z;
@@ -357,22 +158,21 @@ __global__ __launch_bounds__(kGradThreads) void k_grad_1e(OneIntArgs a, const do
 
 }  // namespace
 
-hipError_t launch_grad_2e(const ClassLaunch& c, const Grad2eArgs& a, hipStream_t s) {
+bool grad_2e_dispatch_0(int, const ClassLaunch&, const Grad2eArgs&, int, hipStream_t, hipError_t*);
+bool grad_2e_dispatch_1(int, const ClassLaunch&, const Grad2eArgs&, int, hipStream_t, hipError_t*);
+bool grad_2e_dispatch_2(int, const ClassLaunch&, const Grad2eArgs&, int, hipStream_t, hipError_t*);
+bool grad_2e_dispatch_3(int, const ClassLaunch&, const Grad2eArgs&, int, hipStream_t, hipError_t*);
+
+hipError_t launch_grad_2e(const ClassLaunch& c, const Grad2eArgs& a, int max_bra_prims, hipStream_t s) {
   if (c.nblocks <= 0) return hipSuccess;
+  if (max_bra_prims <= 0) return hipErrorInvalidValue;
   if (a.natoms <= 0 || a.natoms > kGradMaxAtoms) return hipErrorInvalidValue;
   const int key = ((c.LA * 3 + c.LB) * 3 + c.LC) * 3 + c.LD;
-#define GANSU_G(a_, b_, c_, d_) \
-  case ((a_ * 3 + b_) * 3 + c_) * 3 + d_: return launch_grad_2e_t<a_, b_, c_, d_>(c, a, s);
-  switch (key) {
-    GANSU_G(0, 0, 0, 0) GANSU_G(1, 0, 0, 0) GANSU_G(1, 0, 1, 0) GANSU_G(1, 1, 0, 0)
-    GANSU_G(1, 1, 1, 0) GANSU_G(1, 1, 1, 1) GANSU_G(2, 0, 0, 0) GANSU_G(2, 0, 1, 0)
-    GANSU_G(2, 0, 1, 1) GANSU_G(2, 0, 2, 0) GANSU_G(2, 1, 0, 0) GANSU_G(2, 1, 1, 0)
-    GANSU_G(2, 1, 1, 1) GANSU_G(2, 1, 2, 0) GANSU_G(2, 1, 2, 1) GANSU_G(2, 2, 0, 0)
-    GANSU_G(2, 2, 1, 0) GANSU_G(2, 2, 1, 1) GANSU_G(2, 2, 2, 0) GANSU_G(2, 2, 2, 1)
-    GANSU_G(2, 2, 2, 2)
-    default: return hipErrorInvalidValue;
-  }
-#undef GANSU_G
+  hipError_t err = hipErrorInvalidValue;
+  if (grad_2e_dispatch_0(key, c, a, max_bra_prims, s, &err) || grad_2e_dispatch_1(key, c, a, max_bra_prims, s, &err) ||
+      grad_2e_dispatch_2(key, c, a, max_bra_prims, s, &err) || grad_2e_dispatch_3(key, c, a, max_bra_prims, s, &err))
+    return err;
+  return hipErrorInvalidValue;
 }
 
 hipError_t launch_grad_1e(const OneIntArgs& a, const double* D, const double* W, double* grad, hipStream_t s) {
