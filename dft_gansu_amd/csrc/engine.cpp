@@ -64,6 +64,7 @@ FockEngine::FockEngine(const gansu_primitive_shell* prims, size_t n_prims, const
       mixed_ = kv.second == "mixed";
     } else if (kv.first == "fp32_bound_threshold") fp32_thr_ = std::stod(kv.second);
     else if (kv.first == "autotune") autotune_ = kv.second != "0";
+    else if (kv.first == "graph") use_graph_ = kv.second != "0";
     else if (kv.first == "ri_k") {
       if (kv.second != "density" && kv.second != "occupied") throw Error(GANSU_E_PARAM, "ri_k must be density|occupied");
       ri_k_occ_ = kv.second == "occupied";
@@ -102,6 +103,8 @@ FockEngine::FockEngine(const gansu_primitive_shell* prims, size_t n_prims, const
 
 FockEngine::~FockEngine() {
   prof_clear();
+  graph_release();
+  if (cap_) (void)hipStreamDestroy(cap_);
   streams_destroy();
   ri_destroy(ri_);
 }
@@ -130,6 +133,8 @@ void FockEngine::set_profiling(bool on) {
   prof_clear();
   profiling_ = on;
 }
+
+int FockEngine::graph_captures() const { return n_graph_captures_; }
 
 void FockEngine::prof_clear() {
   for (auto& r : prof_) {
@@ -172,6 +177,7 @@ const char* FockEngine::algorithm_name() const {
 
 void FockEngine::precompute() {
   GANSU_HIP(hipSetDevice(0));
+  graph_release();  // a captured build refers to the buffers rebuilt below
   if (!boys_.ptr) boys_.upload(boys_grid_xn());
   d_w_.upload(w_h_);
   d_Dp_.alloc((size_t)M_ * M_);
@@ -505,8 +511,43 @@ void FockEngine::build_jk(const double* d_D, double* d_J, double* d_K, const dou
   if (!d_D) throw Error(GANSU_E_ARG, "null density");
   if (d_C && (nocc <= 0 || nocc > M_)) throw Error(GANSU_E_ARG, "nocc out of range");
   if (method_ == "ri") { build_ri(d_D, d_J, d_K, d_H, d_F, s, d_C, nocc); return; }
-  build_four_center(d_D, s);
-  GANSU_HIP(launch_finalize(d_Jr_.ptr, d_Kr_.ptr, d_w_.ptr, M_, d_H, d_J, d_K, d_F, s));
+  if (!use_graph_ || profiling_) {
+    build_four_center(d_D, s);
+    GANSU_HIP(launch_finalize(d_Jr_.ptr, d_Kr_.ptr, d_w_.ptr, M_, d_H, d_J, d_K, d_F, s));
+    return;
+  }
+  // Graph path: the build's launches (four streams joined by events) are
+  // captured once per set of caller buffers and replayed as one graph launch
+  // on the caller's stream; an SCF loop over fixed D/H/F buffers replays.
+  const GraphKey key{d_D, d_J, d_K, d_H, d_F};
+  if (!gexec_ || !(key == gkey_)) graph_capture(key);
+  GANSU_HIP(hipGraphLaunch(gexec_, s));
+}
+
+void FockEngine::graph_release() {
+  if (gexec_) (void)hipGraphExecDestroy(gexec_);
+  if (graph_) (void)hipGraphDestroy(graph_);
+  gexec_ = nullptr;
+  graph_ = nullptr;
+}
+
+void FockEngine::graph_capture(const GraphKey& k) {
+  graph_release();
+  if (!cap_) GANSU_HIP(hipStreamCreateWithFlags(&cap_, hipStreamNonBlocking));
+  GANSU_HIP(hipStreamBeginCapture(cap_, hipStreamCaptureModeThreadLocal));
+  try {
+    build_four_center(k.D, cap_);
+    GANSU_HIP(launch_finalize(d_Jr_.ptr, d_Kr_.ptr, d_w_.ptr, M_, k.H, k.J, k.K, k.F, cap_));
+  } catch (...) {
+    hipGraph_t g = nullptr;
+    (void)hipStreamEndCapture(cap_, &g);
+    if (g) (void)hipGraphDestroy(g);
+    throw;
+  }
+  GANSU_HIP(hipStreamEndCapture(cap_, &graph_));
+  GANSU_HIP(hipGraphInstantiate(&gexec_, graph_, nullptr, nullptr, 0));
+  gkey_ = k;
+  ++n_graph_captures_;
 }
 
 std::vector<double> FockEngine::schwarz_ref() const {
@@ -520,6 +561,7 @@ void FockEngine::stats(double* out) const {
   out[1] = n_prim_quartets_;
   out[2] = stored_bytes_;
   out[3] = n_fp32_quartets_;
+  out[6] = n_graph_captures_;
   const double P = double(M_) * (M_ + 1) / 2;
   out[4] = P * (P + 1) / 2;
   if (method_ == "ri") out[5] = double(M_) * Maux_ * M_ * sizeof(double);

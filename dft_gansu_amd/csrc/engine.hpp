@@ -99,6 +99,7 @@ class FockEngine {
   void stats(double* out) const;
   int num_basis() const { return M_; }
   void set_profiling(bool on);
+  int graph_captures() const;  // graphs captured so far (the build replays the last one)
   std::vector<double> last_profile();  // 5 values per launch
   // Two-electron part of the RHF nuclear gradient for density D (device,
   // M x M): d_grad (device, 3 natoms) = sum Gamma(D) d(ab|cd)/dX, overwritten.
@@ -160,6 +161,23 @@ class FockEngine {
   hipEvent_t ev_fork_ = nullptr, ev_join_[kSideStreams] = {};
   void streams_create();
   void streams_destroy();
+
+  // Graph of a four-centre build for one set of caller buffers.
+  struct GraphKey {
+    const double* D;
+    double *J, *K;
+    const double* H;
+    double* F;
+    bool operator==(const GraphKey& o) const { return D == o.D && J == o.J && K == o.K && H == o.H && F == o.F; }
+  };
+  bool use_graph_ = true;
+  hipStream_t cap_ = nullptr;
+  hipGraph_t graph_ = nullptr;
+  hipGraphExec_t gexec_ = nullptr;
+  GraphKey gkey_{};
+  int n_graph_captures_ = 0;
+  void graph_capture(const GraphKey& k);
+  void graph_release();
 
   // profiling: events around each launch of a build
   struct ProfRow { double id, bytes, flops, units; hipEvent_t a, b; };

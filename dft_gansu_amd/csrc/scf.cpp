@@ -14,6 +14,8 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cstdio>
+#include <cstdlib>
 #include <cmath>
 #include <cstring>
 #include <limits>
@@ -211,6 +213,15 @@ ScfResult rhf_run(const Molecule& mol, const Molecule* aux, const Params& prm, b
   GANSU_HIP(launch_transform(w.evec.ptr, w.eval.ptr, n, 1e-6, w.X.ptr, 0));
 
   ScfResult res;
+  // GANSU_SCF_TIMING=1 prints the wall time of each phase (with a device
+  // synchronisation at every phase boundary) to stderr.
+  const char* tenv = std::getenv("GANSU_SCF_TIMING");
+  const bool timing = tenv && tenv[0] == '1';
+  double t_diag = 0, t_dens = 0, t_fock = 0, t_energy = 0, t_conv = 0;
+  auto now = [&]() {
+    if (timing) GANSU_HIP(hipDeviceSynchronize());
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+  };
   auto coefficients = [&](const double* Fm) {
     blas.gemm(true, false, n, w.X.ptr, Fm, w.tmp.ptr);
     blas.gemm(false, false, n, w.tmp.ptr, w.X.ptr, w.tmp2.ptr);
@@ -254,13 +265,20 @@ ScfResult rhf_run(const Molecule& mol, const Molecule* aux, const Params& prm, b
   double prev = std::numeric_limits<double>::max(), E = 0.0;
   int it = 0;
   while (it < maxiter) {
+    double t0 = now();
     coefficients(w.F.ptr);
+    double t1 = now();
     density();
+    double t2 = now();
     fock();
+    double t3 = now();
     E = energy();
+    double t4 = now();
+    t_diag += t1 - t0; t_dens += t2 - t1; t_fock += t3 - t2; t_energy += t4 - t3;
     if (std::fabs(E - prev) < conv) { res.converged = 1; break; }
     ++it;
     prev = E;
+    const double tc0 = now();
     if (conv_method == "diis") {
       // e = F D S - S D F (optionally X e X^T)
       double* e = w.tmp2.ptr;
@@ -317,7 +335,12 @@ ScfResult rhf_run(const Molecule& mol, const Molecule* aux, const Params& prm, b
         GANSU_HIP(hipMemcpy(w.F.ptr, prevF.ptr, nn * sizeof(double), hipMemcpyDeviceToDevice));
       }
     }
+    t_conv += now() - tc0;
   }
+  if (timing)
+    std::fprintf(stderr, "[scf] %d iterations: diag %.2f ms, density %.2f ms, fock %.2f ms, energy %.2f ms, "
+                         "convergence %.2f ms (per iteration)\n", it + 1, 1e3 * t_diag / (it + 1),
+                 1e3 * t_dens / (it + 1), 1e3 * t_fock / (it + 1), 1e3 * t_energy / (it + 1), 1e3 * t_conv / (it + 1));
   GANSU_HIP(hipDeviceSynchronize());
   res.e_elec = E;
   res.e_total = E + mol.nuclear_repulsion();

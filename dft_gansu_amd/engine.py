@@ -55,13 +55,13 @@ class FockEngine:
     """Fock-build engine over a molecule (and an auxiliary basis for RI)."""
 
     def __init__(self, mol, aux=None, eri_method="stored", schwarz_screening_threshold=1e-12, precision="fp64",
-                 fp32_bound_threshold=1e-6, ri_k="density", autotune=True):
+                 fp32_bound_threshold=1e-6, ri_k="density", autotune=True, graph=True):
         self.mol = mol
         self.aux = aux
         self.M = mol.n_basis
         opts = (f"eri_method={eri_method};schwarz_screening_threshold={schwarz_screening_threshold!r};"
                 f"precision={precision};fp32_bound_threshold={fp32_bound_threshold!r};ri_k={ri_k};"
-                f"autotune={int(bool(autotune))}")
+                f"autotune={int(bool(autotune))};graph={int(bool(graph))}")
         self._h = ctypes.c_void_p()
         a = aux
         check(lib().gansu_fock_create(
@@ -130,7 +130,7 @@ class FockEngine:
         s = np.zeros(8)
         check(lib().gansu_fock_stats(self._h, ptr(s)))
         return dict(contracted_quartets=s[0], primitive_quartets=s[1], stored_bytes=s[2], fp32_quartets=s[3],
-                    unique_quartets=s[4], ri_bytes=s[5])
+                    unique_quartets=s[4], ri_bytes=s[5], graph_captures=s[6])
 
     def set_profiling(self, enable=True):
         check(lib().gansu_fock_set_profiling(self._h, int(enable)))

@@ -102,8 +102,11 @@ void gansu_params_destroy(gansu_params* p);
 /* ---------------------------------------------------------- Fock engine --
  * `options` is a ";"-separated "key=value" list using parameter keys
  * (eri_method=stored|direct|ri, schwarz_screening_threshold=..,
- * precision=fp64|mixed, fp32_bound_threshold=..).  aux* may be NULL/0 unless
- * eri_method=ri.                                                            */
+ * precision=fp64|mixed, fp32_bound_threshold=.., ri_k=density|occupied) and
+ * engine keys autotune=1|0 (time digestion tiles at precompute, stored path)
+ * and graph=1|0 (four-centre builds replay a HIP graph captured once per set
+ * of D/J/K/H/F pointers; the graph is re-captured when the pointers change).
+ * aux* may be NULL/0 unless eri_method=ri.                                  */
 typedef struct gansu_fock_engine gansu_fock_engine;
 
 int gansu_fock_create(const gansu_primitive_shell* prims, size_t n_prims, const gansu_shell_type_info* types,
@@ -133,7 +136,7 @@ int gansu_fock_schwarz(gansu_fock_engine* e, double* host_Q, size_t n);
 size_t gansu_fock_num_primitive_pairs(const gansu_fock_engine* e);
 /* stats[8] = {surviving contracted quartets, surviving primitive quartets,
  *             stored bytes, fp32 quartets, unique contracted quartets,
- *             ri B bytes, 0, 0}                                             */
+ *             ri B bytes, graphs captured, 0}                               */
 int gansu_fock_stats(const gansu_fock_engine* e, double* stats);
 /* Per-launch timing of the builds that follow, with HIP events recorded on the
  * build stream (no counterpart in the reference, whose profiler.hpp times host
