@@ -162,6 +162,21 @@ int gansu_fock_build_rhf_occ(gansu_fock_engine* e, const double* d_D, const doub
   });
 }
 
+int gansu_fock_build_uhf(gansu_fock_engine* e, const double* d_Da, const double* d_Db, const double* d_H, double* d_Fa,
+                         double* d_Fb, void* stream) {
+  return guarded([&] {
+    if (!e) throw gansu::Error(GANSU_E_ARG, "null engine");
+    e->e->build_uhf(d_Da, d_Db, d_H, d_Fa, d_Fb, (hipStream_t)stream);
+  });
+}
+
+int gansu_fock_update_geometry(gansu_fock_engine* e, const double* xyz_bohr, int n_atoms) {
+  return guarded([&] {
+    if (!e) throw gansu::Error(GANSU_E_ARG, "null engine");
+    e->e->update_geometry(xyz_bohr, n_atoms);
+  });
+}
+
 const char* gansu_fock_algorithm_name(const gansu_fock_engine* e) { return e ? e->e->algorithm_name() : ""; }
 
 int gansu_fock_schwarz(gansu_fock_engine* e, double* host_Q, size_t n) {

@@ -519,20 +519,28 @@ void FockEngine::build_jk(const double* d_D, double* d_J, double* d_K, const dou
   // Graph path: the build's launches (four streams joined by events) are
   // captured once per set of caller buffers and replayed as one graph launch
   // on the caller's stream; an SCF loop over fixed D/H/F buffers replays.
-  const GraphKey key{d_D, d_J, d_K, d_H, d_F};
-  if (!gexec_ || !(key == gkey_)) graph_capture(key);
-  GANSU_HIP(hipGraphLaunch(gexec_, s));
+  GANSU_HIP(hipGraphLaunch(graph_for(GraphKey{d_D, d_J, d_K, d_H, d_F}), s));
 }
 
 void FockEngine::graph_release() {
-  if (gexec_) (void)hipGraphExecDestroy(gexec_);
-  if (graph_) (void)hipGraphDestroy(graph_);
-  gexec_ = nullptr;
-  graph_ = nullptr;
+  for (auto& e : gcache_) {
+    if (e.exec) (void)hipGraphExecDestroy(e.exec);
+    if (e.graph) (void)hipGraphDestroy(e.graph);
+    e = GraphEntry{};
+  }
 }
 
-void FockEngine::graph_capture(const GraphKey& k) {
-  graph_release();
+hipGraphExec_t FockEngine::graph_for(const GraphKey& k) {
+  ++graph_clock_;
+  for (auto& e : gcache_)
+    if (e.exec && e.key == k) {
+      e.used = graph_clock_;
+      return e.exec;
+    }
+  GraphEntry& e = gcache_[gcache_[0].used <= gcache_[1].used ? 0 : 1];  // least recently used
+  if (e.exec) (void)hipGraphExecDestroy(e.exec);
+  if (e.graph) (void)hipGraphDestroy(e.graph);
+  e = GraphEntry{};
   if (!cap_) GANSU_HIP(hipStreamCreateWithFlags(&cap_, hipStreamNonBlocking));
   GANSU_HIP(hipStreamBeginCapture(cap_, hipStreamCaptureModeThreadLocal));
   try {
@@ -544,10 +552,48 @@ void FockEngine::graph_capture(const GraphKey& k) {
     if (g) (void)hipGraphDestroy(g);
     throw;
   }
-  GANSU_HIP(hipStreamEndCapture(cap_, &graph_));
-  GANSU_HIP(hipGraphInstantiate(&gexec_, graph_, nullptr, nullptr, 0));
-  gkey_ = k;
+  GANSU_HIP(hipStreamEndCapture(cap_, &e.graph));
+  GANSU_HIP(hipGraphInstantiate(&e.exec, e.graph, nullptr, nullptr, 0));
+  e.key = k;
+  e.used = graph_clock_;
   ++n_graph_captures_;
+  return e.exec;
+}
+
+void FockEngine::build_uhf(const double* d_Da, const double* d_Db, const double* d_H, double* d_Fa, double* d_Fb,
+                           hipStream_t s) {
+  if (!precomputed_) throw Error(GANSU_E_STATE, "build before precompute");
+  if (!d_Da || !d_Db || !d_H || !d_Fa || !d_Fb) throw Error(GANSU_E_ARG, "null device pointer");
+  const size_t nn = (size_t)M_ * M_;
+  if (uhf_.n < 4 * nn) uhf_.alloc(4 * nn);
+  double *Ja = uhf_.ptr, *Ka = Ja + nn, *Jb = Ka + nn, *Kb = Jb + nn;
+  build_jk(d_Da, Ja, Ka, nullptr, nullptr, s);
+  build_jk(d_Db, Jb, Kb, nullptr, nullptr, s);
+  GANSU_HIP(launch_uhf_combine(d_H, Ja, Ka, Jb, Kb, nn, d_Fa, d_Fb, s));
+}
+
+void FockEngine::update_geometry(const double* xyz, int n_atoms) {
+  if (!xyz || n_atoms <= 0) throw Error(GANSU_E_ARG, "empty geometry");
+  auto move = [&](std::vector<gansu_primitive_shell>& ps) {
+    for (auto& p : ps) {
+      if (p.atom_index < 0 || p.atom_index >= n_atoms) throw Error(GANSU_E_ARG, "atom_index outside n_atoms");
+      p.x = xyz[3 * p.atom_index];
+      p.y = xyz[3 * p.atom_index + 1];
+      p.z = xyz[3 * p.atom_index + 2];
+    }
+  };
+  move(prims_);
+  move(aux_prims_);
+  shells_ = contracted_shells(prims_.data(), prims_.size(), kMaxL);
+  graph_release();
+  if (precomputed_) {
+    precomputed_ = false;
+    ri_destroy(ri_);
+    ri_ = nullptr;
+    d_alpha_.release();
+    d_shell_atom_.release();
+    precompute();
+  }
 }
 
 std::vector<double> FockEngine::schwarz_ref() const {

@@ -93,6 +93,11 @@ class FockEngine {
   // d_C/nocc (optional): occupied orbitals, D = 2 C_occ C_occ^T
   void build_jk(const double* d_D, double* d_J, double* d_K, const double* d_H, double* d_F, hipStream_t s,
                 const double* d_C = nullptr, int nocc = 0);
+  // UHF (gpu_kernels.cu:289-340): F_a = H + J[D_a + D_b] - K[D_a], F_b likewise.
+  void build_uhf(const double* d_Da, const double* d_Db, const double* d_H, double* d_Fa, double* d_Fb, hipStream_t s);
+  // Moves every primitive (and auxiliary primitive) to the position of its
+  // atom and repeats the precompute if it had been done.
+  void update_geometry(const double* xyz_bohr, int n_atoms);
   const char* algorithm_name() const;
   std::vector<double> schwarz_ref() const;  // host copy, reference pair order
   size_t num_primitive_pairs() const { return ref_pairs_; }
@@ -172,12 +177,19 @@ class FockEngine {
   };
   bool use_graph_ = true;
   hipStream_t cap_ = nullptr;
-  hipGraph_t graph_ = nullptr;
-  hipGraphExec_t gexec_ = nullptr;
-  GraphKey gkey_{};
+  // two cached graphs (an RHF loop uses one, a UHF loop alternates two)
+  struct GraphEntry {
+    GraphKey key{};
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    unsigned long long used = 0;
+  };
+  GraphEntry gcache_[2];
+  unsigned long long graph_clock_ = 0;
   int n_graph_captures_ = 0;
-  void graph_capture(const GraphKey& k);
+  hipGraphExec_t graph_for(const GraphKey& k);
   void graph_release();
+  DevBuf<double> uhf_;  // Ja, Ka, Jb, Kb of a UHF build
 
   // profiling: events around each launch of a build
   struct ProfRow { double id, bytes, flops, units; hipEvent_t a, b; };

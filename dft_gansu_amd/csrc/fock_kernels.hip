@@ -552,6 +552,17 @@ __global__ void k_zero(double* __restrict__ p, size_t n) {
     p[i] = 0.0;
 }
 
+// UHF (gpu_kernels.cu:289-340): F_a = H + J[D_a] + J[D_b] - K[D_a], likewise F_b.
+__global__ void k_uhf_combine(const double* __restrict__ H, const double* __restrict__ Ja, const double* __restrict__ Ka,
+                              const double* __restrict__ Jb, const double* __restrict__ Kb, size_t n,
+                              double* __restrict__ Fa, double* __restrict__ Fb) {
+  const size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= n) return;
+  const double hj = H[id] + Ja[id] + Jb[id];
+  Fa[id] = hj - Ka[id];
+  Fb[id] = hj - Kb[id];
+}
+
 __global__ void k_finalize(const double* __restrict__ Jr, const double* __restrict__ Kr, const double* __restrict__ w,
                            int M, const double* __restrict__ H, double* J, double* K, double* F) {
   const size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1079,6 +1090,12 @@ hipError_t launch_zero(double* p, size_t n, hipStream_t s) {
   size_t nb = (n + 255) / 256;
   if (nb > 4096) nb = 4096;
   hipLaunchKernelGGL(k_zero, dim3((unsigned)nb), dim3(256), 0, s, p, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_uhf_combine(const double* H, const double* Ja, const double* Ka, const double* Jb, const double* Kb,
+                              size_t n, double* Fa, double* Fb, hipStream_t s) {
+  hipLaunchKernelGGL(k_uhf_combine, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, H, Ja, Ka, Jb, Kb, n, Fa, Fb);
   return hipGetLastError();
 }
 

@@ -111,6 +111,8 @@ hipError_t launch_scale_density(const double* D, const double* w, int M, double*
 hipError_t launch_zero(double* p, size_t n, hipStream_t s);
 hipError_t launch_fill(double* p, size_t n, double v, hipStream_t s);
 // J = w w sym(Jr), K = w w sym(Kr); if F: F = H + J - K/2 (J, K may be null then)
+hipError_t launch_uhf_combine(const double* H, const double* Ja, const double* Ka, const double* Jb, const double* Kb,
+                              size_t n, double* Fa, double* Fb, hipStream_t s);
 hipError_t launch_finalize(const double* Jr, const double* Kr, const double* w, int M, const double* H, double* J,
                            double* K, double* F, hipStream_t s);
 

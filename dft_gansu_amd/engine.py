@@ -108,6 +108,21 @@ class FockEngine:
         check(lib().gansu_fock_build_rhf_occ(self._h, ptr(D), ptr(C), int(nocc), ptr(H), ptr(F), self._stream(stream)))
         return F
 
+    def compute_fock_matrix_uhf(self, Da, Db, H, Fa=None, Fb=None, stream=None):
+        """UHF: F_a = H + J[Da + Db] - K[Da], F_b = H + J[Da + Db] - K[Db]; returns (Fa, Fb)."""
+        import torch
+        for t, n in ((Da, "Da"), (Db, "Db"), (H, "H")):
+            self._check_mat(t, n)
+        Fa = torch.empty_like(Da) if Fa is None else Fa
+        Fb = torch.empty_like(Da) if Fb is None else Fb
+        check(lib().gansu_fock_build_uhf(self._h, ptr(Da), ptr(Db), ptr(H), ptr(Fa), ptr(Fb), self._stream(stream)))
+        return Fa, Fb
+
+    def update_geometry(self, xyz_bohr):
+        """Move the atoms (n_atoms x 3, bohr); a precomputed engine precomputes again."""
+        xyz = np.ascontiguousarray(xyz_bohr, dtype=np.float64).reshape(-1)
+        check(lib().gansu_fock_update_geometry(self._h, ptr(xyz), len(xyz) // 3))
+
     def compute_jk(self, D, stream=None):
         import torch
         self._check_mat(D, "D")

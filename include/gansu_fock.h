@@ -128,6 +128,16 @@ int gansu_fock_build_jk(gansu_fock_engine* e, const double* d_D, double* d_J, do
  * K is the density form (gpu_manager.cu:1589-1618).                          */
 int gansu_fock_build_rhf_occ(gansu_fock_engine* e, const double* d_D, const double* d_C, int nocc, const double* d_H,
                              double* d_F, void* stream);
+/* UHF build == ERI::compute_fock_matrix for UHF (gpu_kernels.cu:289-340,
+ * computeFockMatrix_RI_UHF gpu_manager.cu:1634-1800):
+ *   F_a = H + J[D_a + D_b] - K[D_a],   F_b = H + J[D_a + D_b] - K[D_b]
+ * with D_a, D_b the alpha and beta densities (C_occ C_occ^T each).          */
+int gansu_fock_build_uhf(gansu_fock_engine* e, const double* d_Da, const double* d_Db, const double* d_H, double* d_Fa,
+                         double* d_Fb, void* stream);
+/* Moves every primitive to the position of its atom (xyz_bohr[3*atom_index])
+ * and, if the engine was precomputed, precomputes again (the reference's
+ * HF::update_geometry, hf.cu:460-472, rebuilds its ERI object).             */
+int gansu_fock_update_geometry(gansu_fock_engine* e, const double* xyz_bohr, int n_atoms);
 /* == ERI::get_algorithm_name ("Stored", "Direct", "RI"). */
 const char* gansu_fock_algorithm_name(const gansu_fock_engine* e);
 /* Primitive-pair Schwarz factors in the reference's pair order

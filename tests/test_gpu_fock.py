@@ -274,3 +274,48 @@ def test_graph_replay_matches_eager(method, cuda):
     F2 = graph.compute_fock_matrix(D, H)  # other output buffer: a second capture
     assert float((F2 - ref).abs().max()) <= 1e-12 * float(ref.abs().max())
     assert graph.stats()["graph_captures"] == 2
+
+
+@pytest.mark.parametrize("method", ["stored", "direct", "ri"])
+def test_uhf_fock_matches_oracle(method, cuda):
+    """UHF build (gpu_kernels.cu:289-340): F_a = H + J[Da] + J[Db] - K[Da]."""
+    Z, xyz = methane_water()
+    mol, (p, t, n) = both(Z, xyz, "cc-pvdz")
+    M = mol.n_basis
+    Da, Db, H = random_density(M, seed=31) / 2, random_density(M, seed=32) / 2, random_density(M, seed=33)
+    if method == "ri":
+        aux, (pa, ta, na) = both(Z, xyz, "cc-pvdz-rifit")
+        B = O.ri_tensor(p, t, n, pa, na, THR)
+        JKa, JKb = O.ri_jk(B, Da), O.ri_jk(B, Db)
+        eng = FockEngine(mol, aux=aux, eri_method="ri")
+    else:
+        Q = O.schwarz(p, t)
+        JKa, JKb = O.jk(p, t, n, Q, Da, THR), O.jk(p, t, n, Q, Db, THR)
+        eng = FockEngine(mol, eri_method=method)
+    eng.precomputation()
+    Fa, Fb = eng.compute_fock_matrix_uhf(_torch(Da, cuda), _torch(Db, cuda), _torch(H, cuda))
+    Fa_ref = H + JKa[0] + JKb[0] - JKa[1]
+    Fb_ref = H + JKa[0] + JKb[0] - JKb[1]
+    scale = max(1.0, np.abs(Fa_ref).max())
+    assert np.abs(Fa.cpu().numpy() - Fa_ref).max() <= 1e-10 * scale
+    assert np.abs(Fb.cpu().numpy() - Fb_ref).max() <= 1e-10 * scale
+
+
+@pytest.mark.parametrize("method", ["stored", "direct"])
+def test_update_geometry_equals_fresh_engine(method, cuda):
+    Z, xyz = methane_water()
+    mol, _ = both(Z, xyz, "cc-pvdz")
+    xyz2 = xyz + np.random.default_rng(4).normal(scale=0.05, size=xyz.shape)
+    mol2, _ = both(Z, xyz2, "cc-pvdz")
+    D = _torch(random_density(mol.n_basis, seed=8), cuda)
+    H = _torch(np.zeros((mol.n_basis, mol.n_basis)), cuda)
+    eng = FockEngine(mol, eri_method=method)
+    eng.precomputation()
+    F1 = eng.compute_fock_matrix(D, H).clone()
+    eng.update_geometry(xyz2)
+    F2 = eng.compute_fock_matrix(D, H)
+    fresh = FockEngine(mol2, eri_method=method)
+    fresh.precomputation()
+    Fr = fresh.compute_fock_matrix(D, H)
+    assert float((F2 - Fr).abs().max()) <= 1e-12 * float(Fr.abs().max())
+    assert float((F2 - F1).abs().max()) > 1e-6  # the geometry really moved
