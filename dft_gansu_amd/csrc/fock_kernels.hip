@@ -749,7 +749,7 @@ __global__ __launch_bounds__(kSlabThreads) void k_fock_slab(SlabLaunch c, EriArg
               for (int cc = 0; cc < NC; ++cc)
 #pragma unroll
                 for (int d = 0; d < ND; ++d) {
-                  const double v = V(((a * NB + bb) * NC + cc) * ND + d);
+                  const double v = V(aa, bb, cc, d);
                   jbs += v * dcd[cc * ND + d];
                   if constexpr (MODE == kModeDirect) jk[cc * ND + d] += fjk * v * dab;
                   kac[aa][cc] += v * dbl[bb][d];
@@ -784,14 +784,41 @@ __global__ __launch_bounds__(kSlabThreads) void k_fock_slab(SlabLaunch c, EriArg
           for (int k = 0; k < NCOMP; ++k) out[k] = 0.0;
           const SPair ket = g.sp[ket_id];
           eri_block_t<LA, LB, LC, LD>(g.sp[bid[b]], ket, ket.nprim, g, out);
-          digest([&](int comp) -> double { return out[comp]; });
-        } else if (kk < spl[b]) {
-          const double* src = c.store + ((size_t)(fblk[b] + kk / 64) * NCOMP) * 64 + (kk & 63);
-          digest([&](int comp) -> double { return src[(size_t)comp * 64]; });
+          digest([&](int aa, int bb, int cc, int d) -> double { return out[((aa * NB + bb) * NC + cc) * ND + d]; });
         } else {
-          const int kr = kk - spl[b];
-          const float* src32 = c.store32 + ((size_t)(fblk32[b] + kr / 64) * NCOMP) * 64 + (kr & 63);
-          digest([&](int comp) -> double { return (double)src32[(size_t)comp * 64]; });
+          // the lane's integrals of this quartet: rows a of the lane, all (b, c, d)
+          constexpr int NAL = SPLIT ? 1 : NA;
+          constexpr int NV = NAL * NB * NCD;
+          const int a0 = SPLIT ? sub : 0;
+          const bool lo = kk < spl[b];
+          const double* src = nullptr;
+          const float* src32 = nullptr;
+          if (lo) {
+            src = c.store + ((size_t)(fblk[b] + kk / 64) * NCOMP + (size_t)a0 * NB * NCD) * 64 + (kk & 63);
+          } else {
+            const int kr = kk - spl[b];
+            src32 = c.store32 + ((size_t)(fblk32[b] + kr / 64) * NCOMP + (size_t)a0 * NB * NCD) * 64 + (kr & 63);
+          }
+          if constexpr (NV <= 96) {
+            // all loads in flight before the first use
+            double vals[NV];
+            if (lo) {
+#pragma unroll
+              for (int k = 0; k < NV; ++k) vals[k] = src[(size_t)k * 64];
+            } else {
+#pragma unroll
+              for (int k = 0; k < NV; ++k) vals[k] = (double)src32[(size_t)k * 64];
+            }
+            digest([&](int aa, int bb, int cc, int d) -> double { return vals[((aa * NB + bb) * NC + cc) * ND + d]; });
+          } else if (lo) {
+            digest([&](int aa, int bb, int cc, int d) -> double {
+              return src[(size_t)(((aa * NB + bb) * NC + cc) * ND + d) * 64];
+            });
+          } else {
+            digest([&](int aa, int bb, int cc, int d) -> double {
+              return (double)src32[(size_t)(((aa * NB + bb) * NC + cc) * ND + d) * 64];
+            });
+          }
         }
       }
       // stored blocks: J of the ket block comes from the ket-major pass (k_jket)
