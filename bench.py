@@ -23,6 +23,13 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
 
+# HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
+# (scaled x2.0 / x1.0 by a 2 GiB copy), keyed by (workload, kernel); the
+# commands and raw counts are in profiles/stored_digest_traffic_c10h22_ccpvdz.md.
+MEASURED_TRAFFIC = {
+    ("C10H22/cc-pVDZ", "fock_slab(pp|ps) stored-digest"): 307.6e6,
+}
+
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
@@ -248,7 +255,8 @@ def main():
     if args.method == "stored":
         roof = dict(bound="hbm", kernel=kernel_name(dom[0]), achieved=round(dom[2] / (dom[1] * 1e-3) / 1e9, 1),
                     peak=HBM_PEAK_GBS, unit="GB/s", frac=round(dom[2] / (dom[1] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-                    traffic=None, bytes_per_launch=dom[2], avg_launch_ms=round(dom[1], 4),
+                    traffic=MEASURED_TRAFFIC.get((cfgd["name"], kernel_name(dom[0]))), bytes_per_launch=dom[2],
+                    avg_launch_ms=round(dom[1], 4),
                     all_digest_kernels=dict(ms=round(build_kernel_ms, 4), bytes=all_bytes,
                                             GBs=round(all_bytes / (build_kernel_ms * 1e-3) / 1e9, 1)))
     else:
