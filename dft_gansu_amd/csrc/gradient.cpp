@@ -33,9 +33,10 @@ void FockEngine::gradient_2e(const double* d_D, int natoms, double* d_grad, hipS
     prof_begin(s, 400000.0 + 1000 * qc->LA + 100 * qc->LB + 10 * qc->LC + qc->LD, 0.0, 0.0, qc->n_quartets);
     const ClassLaunch c{qc->LA, qc->LB, qc->LC, qc->LD, qc->blocks.ptr, (int)qc->blocks_h.size(),
                         pclass_[qc->A].list.ptr, pclass_[qc->B].list.ptr, nullptr};
-    int maxnp = 1;
+    int maxnp = 1, maxnk = 1;
     for (const auto& br : qc->bras_h) maxnp = std::max(maxnp, sp_h_[pclass_[qc->A].sorted[br.bra_pos]].nprim);
-    GANSU_HIP(launch_grad_2e(c, g, maxnp, s));
+    for (int k = 0; k < qc->max_count; ++k) maxnk = std::max(maxnk, sp_h_[pclass_[qc->B].sorted[k]].nprim);
+    GANSU_HIP(launch_grad_2e(c, g, maxnp, maxnk, s));
     prof_end(s);
   }
 }

@@ -4,8 +4,8 @@
 
 namespace gansu {
 
-bool grad_2e_dispatch_0(int key, const ClassLaunch& c, const Grad2eArgs& a, int maxnp, hipStream_t s,
-                          hipError_t* err) {
+bool grad_2e_dispatch_0(int key, const ClassLaunch& c, const Grad2eArgs& a, int maxnp, int maxnk,
+                          hipStream_t s, hipError_t* err) {
   using namespace grad_detail;
   switch (key) {
     GANSU_GRAD_CASE(0, 0, 0, 0)

@@ -154,10 +154,16 @@ __device__ __forceinline__ void pair_grad_u(const double (&E)[3][IA + 1][IB + 1]
   }
 }
 
+// Classes that run the fully unrolled body (La+Lb+Lc+Ld <= 4 and not a d-d
+// bra: measured faster even where it spills); the others run the
+// runtime-indexed body with one wave per ket primitive pair as well.
+template <int LA, int LB, int LC, int LD>
+constexpr bool kGradUnrolled = LA + LB + LC + LD <= 4 && ncart(LA) * ncart(LB) < 36;
+
 // Per-class instance: private arrays sized for the class (the largest,
 // (dd|dd), needs about 10 KB per lane; the s/p classes stay in registers).
 template <int LA, int LB, int LC, int LD>
-__global__ __launch_bounds__(kGradThreads) void k_grad_2e(ClassLaunch c, Grad2eArgs g, int maxnp) {
+__global__ __launch_bounds__(kGradThreads) void k_grad_2e(ClassLaunch c, Grad2eArgs g, int maxnp, int maxnk) {
   extern __shared__ double s_grad[];
   constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
   constexpr int NAB = NA * NB, NCD = NC * ND;
@@ -171,16 +177,22 @@ __global__ __launch_bounds__(kGradThreads) void k_grad_2e(ClassLaunch c, Grad2eA
   const int M = g.M;
   // one wave = one work block (a bra and up to 64 kets, one per lane) and one
   // primitive pair of the bra; the loop bounds are wave-uniform
-  const long long nvirt = (long long)c.nblocks * maxnp;
+  // the heavy (runtime-indexed) classes also give each ket primitive pair its
+  // own wave, for parallelism: their quartets are few and long
+  constexpr bool KSPLIT = !kGradUnrolled<LA, LB, LC, LD>;
+  const int nk = KSPLIT ? maxnk : 1;
+  const long long nvirt = (long long)c.nblocks * maxnp * nk;
   for (long long vb = (long long)blockIdx.x * nw + wave; vb < nvirt; vb += (long long)gridDim.x * nw) {
-    const int b = (int)(vb / maxnp), ibp = (int)(vb % maxnp);
+    const int ikp = (int)(vb % nk);
+    const int b = (int)(vb / nk / maxnp), ibp = (int)((vb / nk) % maxnp);
     const BlockRec blk = c.blocks[b];
     const int bra_id = c.bra_list[blk.bra_pos];
     const SPair bra = g.sp[bra_id];
     if (ibp >= bra.nprim) continue;  // wave-uniform
-    const bool active = lane < blk.count;
-    const int ket_id = c.ket_list[blk.ket_start + (active ? lane : 0)];
+    const bool in_block = lane < blk.count;
+    const int ket_id = c.ket_list[blk.ket_start + (in_block ? lane : 0)];
     const SPair ket = g.sp[ket_id];
+    const bool active = in_block && (!KSPLIT || ikp < ket.nprim);
     double gA[3] = {0.0, 0.0, 0.0}, gB[3] = {0.0, 0.0, 0.0}, gC[3] = {0.0, 0.0, 0.0};
     if (active) {
       // density blocks of Gamma_abcd = 1/2 D_ab D_cd - 1/8 (D_ac D_bd + D_ad D_bc)
@@ -213,7 +225,7 @@ __global__ __launch_bounds__(kGradThreads) void k_grad_2e(ClassLaunch c, Grad2eA
         const int a = ab / NB, bb = ab % NB, cc = cd / ND, dd = cd % ND;
         return 0.5 * dab[ab] * dcd[cd] - 0.125 * (dac[a * NC + cc] * dbd[bb * ND + dd] + dad[a * ND + dd] * dbc[bb * NC + cc]);
       };
-      if constexpr (L <= 4) {
+      if constexpr (kGradUnrolled<LA, LB, LC, LD>) {
         const PairRec pb = g.pr[bra.prim_begin + ibp];
         const double exa = g.alpha[bra.prim_begin + ibp], exb = pb.p - exa;
         double Eb[3][LA + 2][LB + 2][LAB + 3];
@@ -323,7 +335,7 @@ __global__ __launch_bounds__(kGradThreads) void k_grad_2e(ClassLaunch c, Grad2eA
         double Eb[3][NEB];
         for (int d = 0; d < 3; ++d) hermite_e_rt<double>(LA + 1, LB + 1, pb.p, pb.PA[d], pb.PB[d], Eb[d]);
         const ETab TB{{Eb[0], Eb[1], Eb[2]}, LB + 2, LAB + 3};
-        for (int ik = 0; ik < ket.nprim; ++ik) {
+        for (int ik = KSPLIT ? ikp : 0; ik < (KSPLIT ? ikp + 1 : ket.nprim); ++ik) {
           const PairRec pk = g.pr[ket.prim_begin + ik];
           if (pb.Q * pk.Q < g.thr) continue;
           const double exc = g.alpha[ket.prim_begin + ik];
@@ -426,12 +438,12 @@ __global__ __launch_bounds__(kGradThreads) void k_grad_2e(ClassLaunch c, Grad2eA
 }
 
 template <int LA, int LB, int LC, int LD>
-hipError_t launch_grad_2e_t(const ClassLaunch& c, const Grad2eArgs& a, int maxnp, hipStream_t s) {
+hipError_t launch_grad_2e_t(const ClassLaunch& c, const Grad2eArgs& a, int maxnp, int maxnk, hipStream_t s) {
   const int waves = kGradThreads / 64;
-  const long long nvirt = (long long)c.nblocks * maxnp;
+  const long long nvirt = (long long)c.nblocks * maxnp * (kGradUnrolled<LA, LB, LC, LD> ? 1 : maxnk);
   const int grid = (int)std::min<long long>(2048, (nvirt + waves - 1) / waves);
   hipLaunchKernelGGL((k_grad_2e<LA, LB, LC, LD>), dim3(grid), dim3(kGradThreads), 3 * a.natoms * sizeof(double), s,
-                     c, a, maxnp);
+                     c, a, maxnp, maxnk);
   return hipGetLastError();
 }
 
@@ -440,7 +452,7 @@ hipError_t launch_grad_2e_t(const ClassLaunch& c, const Grad2eArgs& a, int maxnp
 // false when this translation unit does not hold the class.
 #define GANSU_GRAD_CASE(a_, b_, c_, d_)                                               \
   case ((a_ * 3 + b_) * 3 + c_) * 3 + d_:                                            \
-    *err = launch_grad_2e_t<a_, b_, c_, d_>(c, a, maxnp, s);                         \
+    *err = launch_grad_2e_t<a_, b_, c_, d_>(c, a, maxnp, maxnk, s);                  \
     return true;
 
 }  // namespace grad_detail

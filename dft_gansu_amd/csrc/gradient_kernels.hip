@@ -11,8 +11,9 @@
 // Two-electron part (template in gradient_2e_impl.hpp, instances in
 // gradient_2e_*.hip): one wave = one work block of the direct Fock build (a
 // bra shell pair and up to 64 kets, one per lane) and one primitive pair of
-// the bra; classes with La+Lb+Lc+Ld <= 4 run a fully unrolled body whose
-// tables stay in registers.  Per primitive quartet:
+// the bra (and, for the heavy classes, one ket primitive pair per wave); the
+// classes with La+Lb+Lc+Ld <= 4 other than (dd|ss) run a fully unrolled body
+// whose tables live in registers.  Per primitive quartet:
 //   G_cd(tuv) = sum (-1)^(tau+nu+phi) E^cd_(tau nu phi) R_(t+tau, u+nu, v+phi)
 // gives both bra-centre derivatives, the mirrored H_ab(tau nu phi) gives the
 // first ket centre, and the last centre follows from translational invariance
@@ -158,19 +159,22 @@ __global__ __launch_bounds__(kGradThreads) void k_grad_1e(OneIntArgs a, const do
 
 }  // namespace
 
-bool grad_2e_dispatch_0(int, const ClassLaunch&, const Grad2eArgs&, int, hipStream_t, hipError_t*);
-bool grad_2e_dispatch_1(int, const ClassLaunch&, const Grad2eArgs&, int, hipStream_t, hipError_t*);
-bool grad_2e_dispatch_2(int, const ClassLaunch&, const Grad2eArgs&, int, hipStream_t, hipError_t*);
-bool grad_2e_dispatch_3(int, const ClassLaunch&, const Grad2eArgs&, int, hipStream_t, hipError_t*);
+bool grad_2e_dispatch_0(int, const ClassLaunch&, const Grad2eArgs&, int, int, hipStream_t, hipError_t*);
+bool grad_2e_dispatch_1(int, const ClassLaunch&, const Grad2eArgs&, int, int, hipStream_t, hipError_t*);
+bool grad_2e_dispatch_2(int, const ClassLaunch&, const Grad2eArgs&, int, int, hipStream_t, hipError_t*);
+bool grad_2e_dispatch_3(int, const ClassLaunch&, const Grad2eArgs&, int, int, hipStream_t, hipError_t*);
 
-hipError_t launch_grad_2e(const ClassLaunch& c, const Grad2eArgs& a, int max_bra_prims, hipStream_t s) {
+hipError_t launch_grad_2e(const ClassLaunch& c, const Grad2eArgs& a, int max_bra_prims, int max_ket_prims,
+                          hipStream_t s) {
   if (c.nblocks <= 0) return hipSuccess;
-  if (max_bra_prims <= 0) return hipErrorInvalidValue;
+  if (max_bra_prims <= 0 || max_ket_prims <= 0) return hipErrorInvalidValue;
   if (a.natoms <= 0 || a.natoms > kGradMaxAtoms) return hipErrorInvalidValue;
   const int key = ((c.LA * 3 + c.LB) * 3 + c.LC) * 3 + c.LD;
   hipError_t err = hipErrorInvalidValue;
-  if (grad_2e_dispatch_0(key, c, a, max_bra_prims, s, &err) || grad_2e_dispatch_1(key, c, a, max_bra_prims, s, &err) ||
-      grad_2e_dispatch_2(key, c, a, max_bra_prims, s, &err) || grad_2e_dispatch_3(key, c, a, max_bra_prims, s, &err))
+  if (grad_2e_dispatch_0(key, c, a, max_bra_prims, max_ket_prims, s, &err) ||
+      grad_2e_dispatch_1(key, c, a, max_bra_prims, max_ket_prims, s, &err) ||
+      grad_2e_dispatch_2(key, c, a, max_bra_prims, max_ket_prims, s, &err) ||
+      grad_2e_dispatch_3(key, c, a, max_bra_prims, max_ket_prims, s, &err))
     return err;
   return hipErrorInvalidValue;
 }

@@ -27,9 +27,12 @@ struct Grad2eArgs {
 // Largest atom count whose gradient accumulator fits the LDS of one workgroup.
 constexpr int kGradMaxAtoms = 2048;  // 48 KiB of LDS
 
-// max_bra_prims: largest primitive-pair count of a bra in the class (the
-// kernel gives each (work block, bra primitive pair) its own wave).
-hipError_t launch_grad_2e(const ClassLaunch& c, const Grad2eArgs& a, int max_bra_prims, hipStream_t s);
+// max_bra_prims / max_ket_prims: largest primitive-pair count of a bra / ket
+// of the class (the kernel gives each (work block, bra primitive pair) its own
+// wave, and for the classes without a register-resident body each ket
+// primitive pair too).
+hipError_t launch_grad_2e(const ClassLaunch& c, const Grad2eArgs& a, int max_bra_prims, int max_ket_prims,
+                          hipStream_t s);
 
 // One-electron part: grad += sum D d(T+V)/dX - sum W dS/dX, both matrices in
 // the caller's basis (the kernel applies the raw-basis factors w).
