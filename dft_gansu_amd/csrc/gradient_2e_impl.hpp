@@ -160,6 +160,12 @@ __device__ __forceinline__ void pair_grad_u(const double (&E)[3][IA + 1][IB + 1]
 template <int LA, int LB, int LC, int LD>
 constexpr bool kGradUnrolled = LA + LB + LC + LD <= 4 && ncart(LA) * ncart(LB) < 36;
 
+// Lanes sharing one quartet: 8 for (dd|dd) and (dd|dp), whose few, long
+// quartets leave most of the GPU idle otherwise; 1 elsewhere (measured: the
+// redundant per-lane G intermediates cost more than they gain there).
+template <int LA, int LB, int LC, int LD>
+constexpr int kGradLanesPerQuartet = LA + LB + LC + LD >= 7 ? 8 : 1;
+
 // Per-class instance: private arrays sized for the class (the largest,
 // (dd|dd), needs about 10 KB per lane; the s/p classes stay in registers).
 template <int LA, int LB, int LC, int LD>
@@ -180,17 +186,24 @@ __global__ __launch_bounds__(kGradThreads) void k_grad_2e(ClassLaunch c, Grad2eA
   // the heavy (runtime-indexed) classes also give each ket primitive pair its
   // own wave, for parallelism: their quartets are few and long
   constexpr bool KSPLIT = !kGradUnrolled<LA, LB, LC, LD>;
+  // ... and for the heaviest classes QL lanes share one quartet (its bra
+  // components ab = sub, sub + QL, ...): 64 / QL kets per wave
+  constexpr int QL = kGradLanesPerQuartet<LA, LB, LC, LD>;
   const int nk = KSPLIT ? maxnk : 1;
-  const long long nvirt = (long long)c.nblocks * maxnp * nk;
+  const long long nvirt = (long long)c.nblocks * maxnp * nk * QL;
   for (long long vb = (long long)blockIdx.x * nw + wave; vb < nvirt; vb += (long long)gridDim.x * nw) {
-    const int ikp = (int)(vb % nk);
-    const int b = (int)(vb / nk / maxnp), ibp = (int)((vb / nk) % maxnp);
+    const int chunk = (int)(vb % QL);
+    const long long vq = vb / QL;
+    const int ikp = (int)(vq % nk);
+    const int b = (int)(vq / nk / maxnp), ibp = (int)((vq / nk) % maxnp);
+    const int kq = chunk * (64 / QL) + lane / QL;  // this lane's ket in the block
+    const int sub = lane % QL;
     const BlockRec blk = c.blocks[b];
     const int bra_id = c.bra_list[blk.bra_pos];
     const SPair bra = g.sp[bra_id];
-    if (ibp >= bra.nprim) continue;  // wave-uniform
-    const bool in_block = lane < blk.count;
-    const int ket_id = c.ket_list[blk.ket_start + (in_block ? lane : 0)];
+    if (ibp >= bra.nprim || chunk * (64 / QL) >= blk.count) continue;  // wave-uniform
+    const bool in_block = kq < blk.count;
+    const int ket_id = c.ket_list[blk.ket_start + (in_block ? kq : 0)];
     const SPair ket = g.sp[ket_id];
     const bool active = in_block && (!KSPLIT || ikp < ket.nprim);
     double gA[3] = {0.0, 0.0, 0.0}, gB[3] = {0.0, 0.0, 0.0}, gC[3] = {0.0, 0.0, 0.0};
@@ -369,7 +382,7 @@ __global__ __launch_bounds__(kGradThreads) void k_grad_2e(ClassLaunch c, Grad2eA
                     }
                   G[hidx(t, u, v)] = sacc;
                 }
-            for (int ab = 0; ab < NAB; ++ab) {
+            for (int ab = sub; ab < NAB; ab += QL) {
               const double gm = gamma(ab, cd);
               if (gm == 0.0) continue;
               int ae[3], be[3];
@@ -383,7 +396,7 @@ __global__ __launch_bounds__(kGradThreads) void k_grad_2e(ClassLaunch c, Grad2eA
             }
           }
           // first ket centre
-          for (int ab = 0; ab < NAB; ++ab) {
+          for (int ab = sub; ab < NAB; ab += QL) {
             int ae[3], be[3];
             for (int d = 0; d < 3; ++d) { ae[d] = cart_exp(LA, ab / NB, d); be[d] = cart_exp(LB, ab % NB, d); }
             const double* Bx = TB.row(0, ae[0], be[0]);
@@ -440,7 +453,8 @@ __global__ __launch_bounds__(kGradThreads) void k_grad_2e(ClassLaunch c, Grad2eA
 template <int LA, int LB, int LC, int LD>
 hipError_t launch_grad_2e_t(const ClassLaunch& c, const Grad2eArgs& a, int maxnp, int maxnk, hipStream_t s) {
   const int waves = kGradThreads / 64;
-  const long long nvirt = (long long)c.nblocks * maxnp * (kGradUnrolled<LA, LB, LC, LD> ? 1 : maxnk);
+  const long long nvirt = (long long)c.nblocks * maxnp * (kGradUnrolled<LA, LB, LC, LD> ? 1 : maxnk) *
+                         kGradLanesPerQuartet<LA, LB, LC, LD>;
   const int grid = (int)std::min<long long>(2048, (nvirt + waves - 1) / waves);
   hipLaunchKernelGGL((k_grad_2e<LA, LB, LC, LD>), dim3(grid), dim3(kGradThreads), 3 * a.natoms * sizeof(double), s,
                      c, a, maxnp, maxnk);
