@@ -157,8 +157,11 @@ __device__ __forceinline__ void pair_grad_u(const double (&E)[3][IA + 1][IB + 1]
 // Classes that run the fully unrolled body (La+Lb+Lc+Ld <= 4 and not a d-d
 // bra: measured faster even where it spills); the others run the
 // runtime-indexed body with one wave per ket primitive pair as well.
+__host__ __device__ constexpr bool grad_unrolled(int LA, int LB, int LC, int LD) {
+  return LA + LB + LC + LD <= 4 && ncart(LA) * ncart(LB) < 36;
+}
 template <int LA, int LB, int LC, int LD>
-constexpr bool kGradUnrolled = LA + LB + LC + LD <= 4 && ncart(LA) * ncart(LB) < 36;
+constexpr bool kGradUnrolled = grad_unrolled(LA, LB, LC, LD);
 
 // Lanes sharing one quartet: 8 for (dd|dd) and (dd|dp), whose few, long
 // quartets leave most of the GPU idle otherwise; 1 elsewhere (measured: the

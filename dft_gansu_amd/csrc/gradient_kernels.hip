@@ -164,6 +164,8 @@ bool grad_2e_dispatch_1(int, const ClassLaunch&, const Grad2eArgs&, int, int, hi
 bool grad_2e_dispatch_2(int, const ClassLaunch&, const Grad2eArgs&, int, int, hipStream_t, hipError_t*);
 bool grad_2e_dispatch_3(int, const ClassLaunch&, const Grad2eArgs&, int, int, hipStream_t, hipError_t*);
 
+bool grad_2e_uses_scratch(int LA, int LB, int LC, int LD) { return !grad_unrolled(LA, LB, LC, LD); }
+
 hipError_t launch_grad_2e(const ClassLaunch& c, const Grad2eArgs& a, int max_bra_prims, int max_ket_prims,
                           hipStream_t s) {
   if (c.nblocks <= 0) return hipSuccess;

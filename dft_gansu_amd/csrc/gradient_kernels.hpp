@@ -34,6 +34,10 @@ constexpr int kGradMaxAtoms = 2048;  // 48 KiB of LDS
 hipError_t launch_grad_2e(const ClassLaunch& c, const Grad2eArgs& a, int max_bra_prims, int max_ket_prims,
                           hipStream_t s);
 
+// The class's kernel runs the runtime-indexed body (private arrays in scratch)
+// rather than the register-resident unrolled one.
+bool grad_2e_uses_scratch(int LA, int LB, int LC, int LD);
+
 // One-electron part: grad += sum D d(T+V)/dX - sum W dS/dX, both matrices in
 // the caller's basis (the kernel applies the raw-basis factors w).
 hipError_t launch_grad_1e(const OneIntArgs& a, const double* D, const double* W, double* grad, hipStream_t s);
