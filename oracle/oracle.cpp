@@ -44,6 +44,7 @@
 #include <vector>
 #include <algorithm>
 #include <fstream>
+#include <map>
 #include <sstream>
 #include <map>
 #include <omp.h>
@@ -895,8 +896,20 @@ int orc_int3c2e(const void* vprims, const void* vtypes, int ntypes, const double
   auto pl = pair_list(P, (const TypeInfo*)vtypes, ntypes, Q);
   size_t MM = M;
   std::memset(out, 0, sizeof(double) * (size_t)Maux * MM * MM);
-#pragma omp parallel for schedule(dynamic, 16)
-  for (long long c = 0; c < naux_prims; ++c) {
+  // primitives grouped by their contracted auxiliary function, so that one
+  // thread owns all writes to that function's rows
+  std::vector<std::vector<int>> groups;
+  {
+    std::map<size_t, int> gid;
+    for (int c = 0; c < naux_prims; ++c) {
+      auto it = gid.find(A[c].basis_index);
+      if (it == gid.end()) { gid[A[c].basis_index] = (int)groups.size(); groups.push_back({c}); }
+      else groups[it->second].push_back(c);
+    }
+  }
+#pragma omp parallel for schedule(dynamic, 4)
+  for (long long gi = 0; gi < (long long)groups.size(); ++gi)
+  for (int c : groups[gi]) {
     std::vector<double> buf;
     const Prim& pc = A[c];
     int lc = pc.shell_type, nc = ncart(lc);
@@ -917,7 +930,7 @@ int orc_int3c2e(const void* vprims, const void* vtypes, int ntypes, const double
             double v = buf[(ia * nb + ib) * nc + ic] * a.coefficient * b.coefficient * pc.coefficient *
                        prim_norm(a.exponent, CA[0], CA[1], CA[2]) * prim_norm(b.exponent, CB[0], CB[1], CB[2]) *
                        prim_norm(pc.exponent, CC[0], CC[1], CC[2]) * norm[p] * norm[q] * aux_norm[r];
-            // aux functions of one primitive are owned by this thread: no race
+            // every primitive of this auxiliary function is handled by this thread
             out[(r * MM + p) * MM + q] += v;
             if (pl[x].a != pl[x].b) out[(r * MM + q) * MM + p] += v;
           }
@@ -1283,6 +1296,134 @@ void orc_grad_nuc(int natoms, const int* Z, const double* xyz, double* grad) {
       const double f = Z[a] * Z[b] / (r2 * std::sqrt(r2));
       for (int d = 0; d < 3; ++d) grad[3 * a + d] -= f * r[d];
     }
+}
+
+}  // extern "C"
+
+// ---- RI-JK energy gradient: derivative 3-centre and 2-centre integrals ---
+// With A^P_mn = (mn|P), V_PQ = (P|Q), gamma = A.D, c = V^-1 gamma,
+// At^P = sum_Q V^-1_PQ A^Q and M_PQ = tr(A^P D A^Q D), the RI two-electron
+// energy 1/2 gamma^T V^-1 gamma - 1/4 sum V^-1_PQ M_PQ has the derivative
+//   sum_P sum_mn G^P_mn d(mn|P)/dX + sum_PQ Gam_PQ d(P|Q)/dX,
+//   G^P = c_P D - 1/2 D At^P D,   Gam = -1/2 c c^T + 1/4 V^-1 M V^-1,
+// which the two functions below contract (the caller forms G and Gam).
+// Centre derivatives use the E-shift of the orbital (or first auxiliary)
+// function; the remaining centre follows by translational invariance.
+extern "C" {
+
+// grad[3*natoms] += sum_P sum_mn G[P][m][n] d(mn|P)/dX over the screened set
+// of orc_int3c2e (G full in m, n).
+int orc_grad_3c(const void* vprims, const void* vtypes, int ntypes, const double* norm, int M, const double* Q,
+                const void* vaux, int naux_prims, const double* aux_norm, int Maux, const double* Qaux, double thr,
+                const double* G, int natoms, double* grad) {
+  const Prim* P = (const Prim*)vprims;
+  const Prim* A = (const Prim*)vaux;
+  auto pl = pair_list(P, (const TypeInfo*)vtypes, ntypes, Q);
+  const size_t MM = M;
+  (void)Maux;
+  const int nt = omp_get_max_threads();
+  std::vector<std::vector<double>> gt(nt, std::vector<double>((size_t)3 * natoms, 0.0));
+#pragma omp parallel for schedule(dynamic, 16)
+  for (long long ci = 0; ci < naux_prims; ++ci) {
+    double* gl = gt[omp_get_thread_num()].data();
+    const Prim& pc = A[ci];
+    const int lc = pc.shell_type;
+    Prim zc = s_dummy(pc);
+    PairData pcd;
+    make_pair(pc, lc, zc, 0, pcd);
+    for (size_t x = 0; x < pl.size(); ++x) {
+      if (pl[x].Q * Qaux[ci] < thr) continue;
+      const Prim &a = P[pl[x].a], &b = P[pl[x].b];
+      const int la = a.shell_type, lb = b.shell_type;
+      PairData ab;
+      make_pair(a, la + 1, b, lb + 1, ab);
+      const double p = ab.p, q = pcd.p;
+      Rtab R;
+      hermite_r(la + lb + lc + 1, p * q / (p + q), ab.P[0] - pcd.P[0], ab.P[1] - pcd.P[1], ab.P[2] - pcd.P[2], R);
+      const double pref = 2.0 * std::pow(M_PI, 2.5) / (p * q * std::sqrt(p + q)) * ab.K * pcd.K;
+      const int zero3[3] = {0, 0, 0};
+      for (int ia = 0; ia < ncart(la); ++ia)
+        for (int ib = 0; ib < ncart(lb); ++ib)
+          for (int ic = 0; ic < ncart(lc); ++ic) {
+            const int *CA = cart(la, ia, 0), *CB = cart(lb, ib, 0), *CC = cart(lc, ic, true);
+            const size_t pp = a.basis_index + ia, qq = b.basis_index + ib, r = pc.basis_index + ic;
+            double w = G[(r * MM + pp) * MM + qq];
+            if (pl[x].a != pl[x].b) w += G[(r * MM + qq) * MM + pp];
+            if (w == 0.0) continue;
+            const double fac = w * pref * a.coefficient * b.coefficient * pc.coefficient *
+                               prim_norm(a.exponent, CA[0], CA[1], CA[2]) * prim_norm(b.exponent, CB[0], CB[1], CB[2]) *
+                               prim_norm(pc.exponent, CC[0], CC[1], CC[2]) * norm[pp] * norm[qq] * aux_norm[r];
+            int e[2][3];
+            for (int d = 0; d < 3; ++d) { e[0][d] = CA[d]; e[1][d] = CB[d]; }
+            const Prim* f[2] = {&a, &b};
+            double dsum[3] = {0, 0, 0};
+            for (int k = 0; k < 2; ++k)
+              for (int d = 0; d < 3; ++d) {
+                const int e0 = e[k][d];
+                e[k][d] = e0 + 1;
+                double v = 2.0 * f[k]->exponent * md_raw4(ab, pcd, R, e[0], e[1], CC, zero3);
+                if (e0 > 0) {
+                  e[k][d] = e0 - 1;
+                  v -= e0 * md_raw4(ab, pcd, R, e[0], e[1], CC, zero3);
+                }
+                e[k][d] = e0;
+                gl[3 * f[k]->atom_index + d] += fac * v;
+                dsum[d] += fac * v;
+              }
+            for (int d = 0; d < 3; ++d) gl[3 * pc.atom_index + d] -= dsum[d];
+          }
+    }
+  }
+  for (int i = 0; i < 3 * natoms; ++i)
+    for (int t = 0; t < nt; ++t) grad[i] += gt[t][i];
+  return 0;
+}
+
+// grad[3*natoms] += sum_PQ Gam[P][Q] d(P|Q)/dX over the screened set of
+// orc_int2c2e.
+int orc_grad_2c(const void* vaux, int naux_prims, const double* aux_norm, int Maux, const double* Qaux, double thr,
+                const double* Gam, int natoms, double* grad) {
+  const Prim* P = (const Prim*)vaux;
+  (void)natoms;
+  const int zero3[3] = {0, 0, 0};
+  for (int i = 0; i < naux_prims; ++i)
+    for (int j = i; j < naux_prims; ++j) {
+      if (Qaux[i] * Qaux[j] < thr) continue;
+      const Prim &a = P[i], &b = P[j];
+      const int la = a.shell_type, lb = b.shell_type;
+      Prim za = s_dummy(a), zb = s_dummy(b);
+      PairData pa, pb;
+      make_pair(a, la + 1, za, 0, pa);
+      make_pair(b, lb, zb, 0, pb);
+      const double p = pa.p, q = pb.p;
+      Rtab R;
+      hermite_r(la + lb + 1, p * q / (p + q), pa.P[0] - pb.P[0], pa.P[1] - pb.P[1], pa.P[2] - pb.P[2], R);
+      const double pref = 2.0 * std::pow(M_PI, 2.5) / (p * q * std::sqrt(p + q)) * pa.K * pb.K;
+      for (int ca = 0; ca < ncart(la); ++ca)
+        for (int cb = 0; cb < ncart(lb); ++cb) {
+          const int *CA = cart(la, ca, true), *CB = cart(lb, cb, true);
+          const size_t pp = a.basis_index + ca, r = b.basis_index + cb;
+          double w = Gam[pp * Maux + r];
+          if (i != j) w += Gam[r * Maux + pp];
+          if (w == 0.0) continue;
+          const double fac = w * pref * a.coefficient * b.coefficient * prim_norm(a.exponent, CA[0], CA[1], CA[2]) *
+                             prim_norm(b.exponent, CB[0], CB[1], CB[2]) * aux_norm[pp] * aux_norm[r];
+          int e[3] = {CA[0], CA[1], CA[2]};
+          for (int d = 0; d < 3; ++d) {
+            const int e0 = e[d];
+            e[d] = e0 + 1;
+            double v = 2.0 * a.exponent * md_raw4(pa, pb, R, e, zero3, CB, zero3);
+            if (e0 > 0) {
+              e[d] = e0 - 1;
+              v -= e0 * md_raw4(pa, pb, R, e, zero3, CB, zero3);
+            }
+            e[d] = e0;
+            grad[3 * a.atom_index + d] += fac * v;
+            grad[3 * b.atom_index + d] -= fac * v;
+          }
+        }
+    }
+  return 0;
 }
 
 }  // extern "C"

@@ -272,3 +272,47 @@ def rhf_gradient(gbs_path, Z, xyz_bohr, nocc, quirk=True, conv=1e-12, thr=1e-12)
     Q = schwarz(prims, types)
     g = grad_1e(prims, norm, Z, xyz_bohr, D, W) + grad_2e(prims, types, norm, Q, D, len(Z), thr) + grad_nuc(Z, xyz_bohr)
     return E, g
+
+
+def grad_3c(prims, types, norm, aux_prims, aux_norm, G, natoms, thr=1e-12):
+    """sum_P sum_mn G[P][m][n] d(mn|P)/dX."""
+    M, Ma = len(norm), len(aux_norm)
+    g = np.zeros(3 * natoms)
+    lib().orc_grad_3c(_ptr(prims), _ptr(types), ctypes.c_int(len(types)), _ptr(norm), ctypes.c_int(M),
+                      _ptr(schwarz(prims, types)), _ptr(aux_prims), ctypes.c_int(len(aux_prims)), _ptr(aux_norm),
+                      ctypes.c_int(Ma), _ptr(schwarz_aux(aux_prims)), ctypes.c_double(thr),
+                      _ptr(np.ascontiguousarray(G, dtype=np.float64)), ctypes.c_int(natoms), _ptr(g))
+    return g.reshape(natoms, 3)
+
+
+def grad_2c(aux_prims, aux_norm, Gam, natoms, thr=1e-12):
+    """sum_PQ Gam[P][Q] d(P|Q)/dX."""
+    Ma = len(aux_norm)
+    g = np.zeros(3 * natoms)
+    lib().orc_grad_2c(_ptr(aux_prims), ctypes.c_int(len(aux_prims)), _ptr(aux_norm), ctypes.c_int(Ma),
+                      _ptr(schwarz_aux(aux_prims)), ctypes.c_double(thr),
+                      _ptr(np.ascontiguousarray(Gam, dtype=np.float64)), ctypes.c_int(natoms), _ptr(g))
+    return g.reshape(natoms, 3)
+
+
+def ri_grad_2e(prims, types, norm, aux_prims, aux_norm, D, natoms, thr=1e-12):
+    """Gradient of the RI two-electron energy 1/2 D.J_RI - 1/4 D.K_RI for a
+    fixed density: G^P = c_P D - 1/2 D At^P D into the 3-centre derivatives,
+    Gam = -1/2 c c^T + 1/4 V^-1 M V^-1 into the 2-centre ones, with
+    gamma = A.D, c = V^-1 gamma, At^P = sum_Q V^-1_PQ A^Q, M_PQ = tr(A^P D A^Q D)."""
+    Q = schwarz(prims, types)
+    Qa = schwarz_aux(aux_prims)
+    V = int2c2e(aux_prims, aux_norm, Qa, thr)
+    A3 = int3c2e(prims, types, norm, Q, aux_prims, aux_norm, Qa, thr)
+    Ma, M = A3.shape[0], A3.shape[1]
+    Vinv = np.linalg.inv(V)
+    gamma = np.einsum("pmn,mn->p", A3, D)
+    c = Vinv @ gamma
+    At = (Vinv @ A3.reshape(Ma, -1)).reshape(Ma, M, M)
+    DAD = np.einsum("mk,pkl,ln->pmn", D, At, D)
+    G = c[:, None, None] * D[None] - 0.5 * DAD
+    AD = np.einsum("pmk,kn->pmn", A3, D)
+    Mpq = np.einsum("pmn,qnm->pq", AD, AD)
+    Gam = -0.5 * np.outer(c, c) + 0.25 * Vinv @ Mpq @ Vinv
+    return grad_3c(prims, types, norm, aux_prims, aux_norm, G, natoms, thr) + grad_2c(aux_prims, aux_norm, Gam, natoms,
+                                                                                     thr)
