@@ -46,7 +46,6 @@
 #include <fstream>
 #include <map>
 #include <sstream>
-#include <map>
 #include <omp.h>
 
 namespace {
