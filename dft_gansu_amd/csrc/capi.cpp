@@ -242,6 +242,14 @@ int gansu_fock_gradient_rhf(gansu_fock_engine* e, const double* d_D, int n_atoms
   });
 }
 
+int gansu_fock_gradient_rhf_occ(gansu_fock_engine* e, const double* d_D, const double* d_C, int nocc, int n_atoms,
+                                double* d_grad, void* stream) {
+  return guarded([&] {
+    if (!e) throw gansu::Error(GANSU_E_ARG, "null engine");
+    e->e->gradient_2e(d_D, n_atoms, d_grad, (hipStream_t)stream, d_C, nocc);
+  });
+}
+
 int gansu_oneint_gradient(const gansu_molecule* mol, const double* d_D, const double* d_W, double* d_grad,
                           void* stream) {
   return guarded([&] {

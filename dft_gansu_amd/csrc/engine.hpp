@@ -108,7 +108,10 @@ class FockEngine {
   std::vector<double> last_profile();  // 5 values per launch
   // Two-electron part of the RHF nuclear gradient for density D (device,
   // M x M): d_grad (device, 3 natoms) = sum Gamma(D) d(ab|cd)/dX, overwritten.
-  void gradient_2e(const double* d_D, int natoms, double* d_grad, hipStream_t s);
+  // RI engines need the occupied orbitals: d_C (M x M row-major, first nocc
+  // columns occupied, D = 2 C_occ C_occ^T); the four-centre paths ignore them.
+  void gradient_2e(const double* d_D, int natoms, double* d_grad, hipStream_t s, const double* d_C = nullptr,
+                   int nocc = 0);
 
  private:
   void precompute_pairs();     // Schwarz factors, shell pairs, primitive-pair records
@@ -121,6 +124,7 @@ class FockEngine {
   SlabLaunch slab_launch(QuartetClass& qc, int mode);
   void precompute_ri();
   void build_ri_k_occ(const double* d_C, int nocc, double* K, hipStream_t s);
+  void gradient_ri(const double* d_D, const double* d_C, int nocc, int natoms, double* d_grad, hipStream_t s);
   void ri_k_product(const double* A, int kdim, bool accumulate, double* K, hipStream_t s);
   void build_four_center(const double* d_D, hipStream_t s);
   void build_ri(const double* d_D, double* d_J, double* d_K, const double* d_H, double* d_F, hipStream_t s,

@@ -8,15 +8,19 @@
 
 namespace gansu {
 
-void FockEngine::gradient_2e(const double* d_D, int natoms, double* d_grad, hipStream_t s) {
+void FockEngine::gradient_2e(const double* d_D, int natoms, double* d_grad, hipStream_t s, const double* d_C,
+                             int nocc) {
   if (!precomputed_) throw Error(GANSU_E_STATE, "gradient before precompute");
-  if (method_ == "ri") throw Error(GANSU_E_UNSUPPORTED, "the nuclear gradient is provided for eri_method=stored|direct");
   if (!d_D || !d_grad) throw Error(GANSU_E_ARG, "null device pointer");
   int max_atom = -1;
   for (const auto& sh : shells_) max_atom = std::max(max_atom, sh.atom);
   if (natoms <= max_atom || natoms <= 0) throw Error(GANSU_E_ARG, "n_atoms smaller than the atoms the basis sits on");
   if (natoms > kGradMaxAtoms)
     throw Error(GANSU_E_UNSUPPORTED, "gradient limited to " + std::to_string(kGradMaxAtoms) + " atoms");
+  if (method_ == "ri") {
+    gradient_ri(d_D, d_C, nocc, natoms, d_grad, s);
+    return;
+  }
   if (!d_alpha_.ptr) d_alpha_.upload(pair_alpha_h_);
   if (!d_shell_atom_.ptr) {
     std::vector<int> sa(shells_.size());

@@ -58,6 +58,8 @@ struct FockEngine::RIState {
   DevBuf<float> B32;  // FP32 copy of B for the J increments of precision=mixed
   DevBuf<double> Wprev, dW, Jprev, Kpart;
   DevBuf<int> jstate;  // {builds since FP64 J (-1: none yet), FP64 this build}
+  std::vector<int> aux_atom_h;  // atom of each auxiliary shell
+  DevBuf<int> aux_atom;
   int pchunk = 0;
   ~RIState() {
     if (h) rocblas_destroy_handle(h);
@@ -123,10 +125,12 @@ void FockEngine::precompute_ri() {
     }
     as.qmax = qmax;
     r.shells_h.push_back(as);
+    r.aux_atom_h.push_back(s.atom);
     if (s.basis + ncart(s.L) > Ma) throw Error(GANSU_E_ARG, "aux basis_index out of range");
     for (int c = 0; c < ncart(s.L); ++c) waux[s.basis + c] = aux_norm_[s.basis + c] * aux_component_factor(s.L, c);
   }
   r.shells.upload(r.shells_h);
+  r.aux_atom.upload(r.aux_atom_h);
   r.prims.upload(aprims);
   r.waux.upload(waux);
 
@@ -300,6 +304,117 @@ void FockEngine::ri_k_product(const double* A, int kdim, bool accumulate, double
       }
       prof_end(s);
     }
+}
+
+// Gradient of the RI-JK two-electron energy 1/2 D.J_RI - 1/4 D.K_RI
+// (B = L^-1 A, A^P_mn = (mn|P), V = (P|Q) = L L^T, column-major U = L^T):
+//   W = L^-1 gamma, c = V^-1 gamma = U^-1 W,
+//   S^P = X^T B^P X (X = sqrt2 C_occ, D = X X^T), N_RS = tr(S^R S^S),
+//   St = S U^-T along P (St^P = sum_Q (L^-T)_PQ S^Q),
+//   Gam = -1/2 c c^T + 1/4 U^-1 N U^-T         -> d(P|Q) contraction,
+//   G^P = c_P D - 1/2 X St^P X^T  (P-chunks)   -> d(mn|P) contraction,
+// which equal the oracle's G^P = c_P D - 1/2 D V^-1 A^P D and
+// Gam = -1/2 c c^T + 1/4 V^-1 M V^-1 with M_PQ = tr(A^P D A^Q D).
+void FockEngine::gradient_ri(const double* d_D, const double* d_C, int nocc, int natoms, double* d_grad,
+                             hipStream_t s) {
+  RIState& r = *ri_;
+  const int M = M_, Ma = Maux_;
+  if (!d_C || nocc <= 0 || nocc > M) throw Error(GANSU_E_ARG, "the RI gradient needs the occupied orbitals (C, nocc)");
+  blas_ok(rocblas_set_stream(r.h, s), "rocblas_set_stream");
+  if (!d_alpha_.ptr) d_alpha_.upload(pair_alpha_h_);
+  if (!d_shell_atom_.ptr) {
+    std::vector<int> sa(shells_.size());
+    for (size_t i = 0; i < shells_.size(); ++i) sa[i] = shells_[i].atom;
+    d_shell_atom_.upload(sa);
+  }
+  for (int a : r.aux_atom_h)
+    if (a < 0 || a >= natoms) throw Error(GANSU_E_ARG, "auxiliary basis on an atom beyond n_atoms");
+  GANSU_HIP(launch_zero(d_grad, 3 * (size_t)natoms, s));
+  const double one = 1.0, zero = 0.0, two = std::sqrt(2.0);
+  // W and c
+  GANSU_HIP(launch_ri_w(r.B.ptr, d_D, M, Ma, r.W.ptr, s));
+  DevBuf<double> c(Ma);
+  GANSU_HIP(hipMemcpyAsync(c.ptr, r.W.ptr, sizeof(double) * Ma, hipMemcpyDeviceToDevice, s));
+  blas_ok(rocblas_dtrsm(r.h, rocblas_side_left, rocblas_fill_upper, rocblas_operation_none, rocblas_diagonal_non_unit,
+                        Ma, 1, &one, r.L.ptr, Ma, c.ptr, Ma),
+          "rocblas_dtrsm(c)");
+  // S^P for all P, through T = B X in P-chunks
+  const size_t no2 = (size_t)nocc * nocc;
+  DevBuf<double> S(no2 * Ma);
+  const int pc_max = (int)std::max<size_t>(1, std::min<size_t>(Ma, r.T.n / ((size_t)M * nocc)));
+  for (int p0 = 0; p0 < Ma; p0 += pc_max) {
+    const int pc = std::min(pc_max, Ma - p0);
+    blas_ok(rocblas_dgemm_strided_batched(r.h, rocblas_operation_none, rocblas_operation_none, nocc, pc, M, &two, d_C,
+                                          M, 0, r.B.ptr + (size_t)p0 * M, M, (rocblas_stride)Ma * M, &zero, r.T.ptr,
+                                          nocc, (rocblas_stride)pc * nocc, M),
+            "rocblas_dgemm_strided_batched(T grad)");
+    blas_ok(rocblas_dgemm(r.h, rocblas_operation_none, rocblas_operation_transpose, nocc, pc * nocc, M, &two, d_C, M,
+                          r.T.ptr, pc * nocc, &zero, S.ptr + (size_t)p0 * no2, nocc),
+            "rocblas_dgemm(S)");
+  }
+  // N = S_flat^T S_flat, Gam = 1/4 U^-1 N U^-T - 1/2 c c^T
+  DevBuf<double> N((size_t)Ma * Ma);
+  blas_ok(rocblas_dgemm(r.h, rocblas_operation_transpose, rocblas_operation_none, Ma, Ma, (int)no2, &one, S.ptr,
+                        (int)no2, S.ptr, (int)no2, &zero, N.ptr, Ma),
+          "rocblas_dgemm(N)");
+  blas_ok(rocblas_dtrsm(r.h, rocblas_side_left, rocblas_fill_upper, rocblas_operation_none, rocblas_diagonal_non_unit,
+                        Ma, Ma, &one, r.L.ptr, Ma, N.ptr, Ma),
+          "rocblas_dtrsm(N left)");
+  blas_ok(rocblas_dtrsm(r.h, rocblas_side_right, rocblas_fill_upper, rocblas_operation_transpose,
+                        rocblas_diagonal_non_unit, Ma, Ma, &one, r.L.ptr, Ma, N.ptr, Ma),
+          "rocblas_dtrsm(N right)");
+  const double quarter = 0.25, mhalf = -0.5;
+  blas_ok(rocblas_dscal(r.h, Ma * Ma, &quarter, N.ptr, 1), "rocblas_dscal(Gam)");
+  blas_ok(rocblas_dger(r.h, Ma, Ma, &mhalf, c.ptr, 1, c.ptr, 1, N.ptr, Ma), "rocblas_dger(Gam)");
+  // St = S U^-T
+  blas_ok(rocblas_dtrsm(r.h, rocblas_side_right, rocblas_fill_upper, rocblas_operation_transpose,
+                        rocblas_diagonal_non_unit, (int)no2, Ma, &one, r.L.ptr, Ma, S.ptr, (int)no2),
+          "rocblas_dtrsm(St)");
+  RIGradArgs g{d_sp_.ptr, d_pr_.ptr, d_alpha_.ptr, (int)sp_h_.size(), r.shells.ptr, r.prims.ptr, 0,
+               (int)r.shells_h.size(), d_w_.ptr, r.waux.ptr, M, Ma, 0, d_shell_atom_.ptr, r.aux_atom.ptr, natoms, thr_,
+               boys_.ptr, N.ptr, d_grad};
+  GANSU_HIP(launch_ri_grad_2c(g, s));
+  // G^P chunks aligned to auxiliary shells, then the 3-centre derivatives
+  const size_t mm = (size_t)M * M;
+  const int nsh = (int)r.shells_h.size();
+  int maxc = 1;
+  for (const auto& as : r.shells_h) maxc = std::max(maxc, ncart(as.L));
+  // the K workspace holds a chunk of G unless one shell does not fit in it
+  int pg_max = (int)std::min<size_t>(Ma, r.T.n / mm);
+  DevBuf<double> Gown;
+  double* G = r.T.ptr;
+  if (pg_max < maxc) {
+    pg_max = maxc;
+    Gown.alloc((size_t)maxc * mm);
+    G = Gown.ptr;
+  }
+  DevBuf<double> Yt((size_t)pg_max * nocc * M);
+  for (int sh0 = 0; sh0 < nsh;) {
+    const int P0 = r.shells_h[sh0].basis;
+    int sh1 = sh0, pc = 0;
+    while (sh1 < nsh && r.shells_h[sh1].basis == P0 + pc && pc + ncart(r.shells_h[sh1].L) <= pg_max) {
+      pc += ncart(r.shells_h[sh1].L);
+      ++sh1;
+    }
+    if (sh1 == sh0) throw Error(GANSU_E_STATE, "auxiliary shells are not in basis-function order");
+    GANSU_HIP(launch_fill_cD(c.ptr, d_D, P0, pc, M, G, s));
+    blas_ok(rocblas_dgemm_strided_batched(r.h, rocblas_operation_none, rocblas_operation_none, nocc, M, nocc, &two,
+                                          S.ptr + (size_t)P0 * no2, nocc, (rocblas_stride)no2, d_C, M, 0, &zero, Yt.ptr,
+                                          nocc, (rocblas_stride)nocc * M, pc),
+            "rocblas_dgemm_strided_batched(Y)");
+    const double alpha = -0.5 * two;
+    blas_ok(rocblas_dgemm_strided_batched(r.h, rocblas_operation_transpose, rocblas_operation_none, M, M, nocc, &alpha,
+                                          Yt.ptr, nocc, (rocblas_stride)nocc * M, d_C, M, 0, &one, G, M,
+                                          (rocblas_stride)mm, pc),
+            "rocblas_dgemm_strided_batched(G)");
+    g.sh0 = sh0;
+    g.nsh = sh1 - sh0;
+    g.P0 = P0;
+    g.G = G;
+    GANSU_HIP(launch_ri_grad_3c(g, s));
+    sh0 = sh1;
+  }
+  GANSU_HIP(hipStreamSynchronize(s));  // the temporaries above are released on return
 }
 
 }  // namespace gansu

@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include "md_device.hpp"
+#include "gradient_2e_impl.hpp"
 #include "ri_kernels.hpp"
 
 namespace gansu {
@@ -130,6 +131,185 @@ __global__ void k_int3c2e(RI3cArgs g) {
         if (diff) g.B[((size_t)n * Maux + P) * M + m] = v;
       }
     }
+}
+
+// ------------------------------------------------- RI-JK energy gradient --
+// Centre derivatives by the E-shift 2a G(l+1) - l G(l-1) of the orbital pair's
+// functions (tables one order higher, R one order higher); the auxiliary
+// centre follows by translational invariance.  One lane per (shell pair,
+// auxiliary shell); per-atom sums in LDS, flushed once per workgroup.
+constexpr int kRIGradThreads = 256;
+
+__global__ __launch_bounds__(kRIGradThreads) void k_ri_grad_3c(RIGradArgs g) {
+  extern __shared__ double s_grad[];
+  const int n3 = 3 * g.natoms;
+  for (int i = threadIdx.x; i < n3; i += blockDim.x) s_grad[i] = 0.0;
+  __syncthreads();
+  const size_t nitems = (size_t)g.nsp * g.nsh;
+  const size_t M = g.M;
+  for (size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x; id < nitems; id += (size_t)gridDim.x * blockDim.x) {
+    const int isp = (int)(id / g.nsh), iaux = g.sh0 + (int)(id % g.nsh);
+    const SPair sp = g.sp[isp];
+    const AuxShell C = g.shells[iaux];
+    if (sp.qmax * C.qmax < g.thr) continue;
+    const int la = sp.la, lb = sp.lb, lc = C.L;
+    const int NB = ncart(lb), NAB = ncart(la) * NB, NC = ncart(lc);
+    const bool diff = sp.si != sp.sj;
+    double gA[3] = {0.0, 0.0, 0.0}, gB[3] = {0.0, 0.0, 0.0};
+    for (int ip = 0; ip < sp.nprim; ++ip) {
+      const PairRec pr = g.pr[sp.prim_begin + ip];
+      const double exa = g.alpha[sp.prim_begin + ip], exb = pr.p - exa;
+      double Eb[3][4 * 4 * 7];  // (la+2)(lb+2)(la+lb+3), la, lb <= 2
+      for (int d = 0; d < 3; ++d) hermite_e_rt<double>(la + 1, lb + 1, pr.p, pr.PA[d], pr.PB[d], Eb[d]);
+      const grad_detail::ETab TB{{Eb[0], Eb[1], Eb[2]}, lb + 2, la + lb + 3};
+      for (int ic = 0; ic < C.nprim; ++ic) {
+        const AuxPrim pc = g.prims[C.prim_begin + ic];
+        if (pr.Q * pc.Q < g.thr) continue;
+        double Ek[3][4 * 4];  // (lc+1) x (lc+1), lc <= 3
+        for (int d = 0; d < 3; ++d) hermite_e_rt<double>(lc, 0, pc.alpha, 0.0, 0.0, Ek[d]);
+        const double p = pr.p, q = pc.alpha;
+        const double pref = kTwoPi25 / (p * q * sqrt(p + q)) * pr.Kc * pc.Kc;
+        const int L = la + lb + lc + 1;
+        double R[nherm(8)];
+        hermite_r_rt<double>(L, p * q / (p + q), pr.P[0] - pc.C[0], pr.P[1] - pc.C[1], pr.P[2] - pc.C[2], pref, g.boys,
+                             R);
+        for (int c = 0; c < NC; ++c) {
+          int ce[3];
+          for (int d = 0; d < 3; ++d) ce[d] = cart_exp(lc, c, d, true);
+          const int P = C.basis + c;
+          const double* Gp = g.G + (size_t)(P - g.P0) * M * M;
+          // Hermite intermediate of the auxiliary function, levels up to la+lb+1
+          double Gh[nherm(5)];
+          const int lab1 = la + lb + 1;
+          for (int t = 0; t <= lab1; ++t)
+            for (int u = 0; u + t <= lab1; ++u)
+              for (int v = 0; v + u + t <= lab1; ++v) {
+                double sacc = 0.0;
+                for (int tt = 0; tt <= ce[0]; ++tt)
+                  for (int uu = 0; uu <= ce[1]; ++uu) {
+                    const double exy = Ek[0][ce[0] * (lc + 1) + tt] * Ek[1][ce[1] * (lc + 1) + uu];
+                    for (int vv = 0; vv <= ce[2]; ++vv) {
+                      const double e = exy * Ek[2][ce[2] * (lc + 1) + vv];
+                      sacc += (((tt + uu + vv) & 1) ? -e : e) * R[hidx(t + tt, u + uu, v + vv)];
+                    }
+                  }
+                Gh[hidx(t, u, v)] = sacc;
+              }
+          const double wp = g.waux[P];
+          for (int ab = 0; ab < NAB; ++ab) {
+            const size_t m = sp.bi + ab / NB, n = sp.bj + ab % NB;
+            double wt = Gp[m * M + n];
+            if (diff) wt += Gp[n * M + m];
+            if (wt == 0.0) continue;
+            wt *= g.w[m] * g.w[n] * wp;
+            int ae[3], be[3];
+            for (int d = 0; d < 3; ++d) { ae[d] = cart_exp(la, ab / NB, d); be[d] = cart_exp(lb, ab % NB, d); }
+            double dA[3], dB[3];
+            grad_detail::pair_grad(TB, ae, be, exa, exb, Gh, true, dA, dB);
+            for (int d = 0; d < 3; ++d) {
+              gA[d] += wt * dA[d];
+              gB[d] += wt * dB[d];
+            }
+          }
+        }
+      }
+    }
+    const int atA = g.shell_atom[sp.si], atB = g.shell_atom[sp.sj], atC = g.aux_atom[iaux];
+    for (int d = 0; d < 3; ++d) {
+      if (gA[d] != 0.0) atomicAdd(&s_grad[3 * atA + d], gA[d]);
+      if (gB[d] != 0.0) atomicAdd(&s_grad[3 * atB + d], gB[d]);
+      if (gA[d] + gB[d] != 0.0) atomicAdd(&s_grad[3 * atC + d], -(gA[d] + gB[d]));
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < n3; i += blockDim.x)
+    if (s_grad[i] != 0.0) unsafeAtomicAdd(&g.grad[i], s_grad[i]);
+}
+
+// sum_PQ Gam_PQ d(P|Q)/dX: one lane per auxiliary shell pair (P >= Q).
+__global__ __launch_bounds__(kRIGradThreads) void k_ri_grad_2c(RIGradArgs g) {
+  extern __shared__ double s_grad[];
+  const int n3 = 3 * g.natoms;
+  for (int i = threadIdx.x; i < n3; i += blockDim.x) s_grad[i] = 0.0;
+  __syncthreads();
+  const size_t ns = g.nsh;
+  const size_t npairs = ns * (ns + 1) / 2;
+  for (size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x; id < npairs; id += (size_t)gridDim.x * blockDim.x) {
+    size_t r = (size_t)((sqrt(8.0 * (double)id + 1.0) - 1.0) / 2.0);
+    while ((r + 1) * (r + 2) / 2 <= id) ++r;
+    while (r * (r + 1) / 2 > id) --r;
+    const int sP = (int)r, sQ = (int)(id - r * (r + 1) / 2);
+    const AuxShell A = g.shells[sP], B = g.shells[sQ];
+    if (A.qmax * B.qmax < g.thr) continue;
+    const int la = A.L, lb = B.L, NA = ncart(la), NB = ncart(lb);
+    double gP[3] = {0.0, 0.0, 0.0};
+    for (int ia = 0; ia < A.nprim; ++ia) {
+      const AuxPrim pa = g.prims[A.prim_begin + ia];
+      double Ea[3][5 * 3 * 6];  // (la+2)(1+2)(la+1+3) for the (a, s-dummy) pair built to (la+1, 1)
+      for (int d = 0; d < 3; ++d) hermite_e_rt<double>(la + 1, 1, pa.alpha, 0.0, 0.0, Ea[d]);
+      const grad_detail::ETab TA{{Ea[0], Ea[1], Ea[2]}, 2, la + 3};
+      for (int ib = 0; ib < B.nprim; ++ib) {
+        const AuxPrim pb = g.prims[B.prim_begin + ib];
+        if (pa.Q * pb.Q < g.thr) continue;
+        double Eb[3][4 * 4];
+        for (int d = 0; d < 3; ++d) hermite_e_rt<double>(lb, 0, pb.alpha, 0.0, 0.0, Eb[d]);
+        const double p = pa.alpha, q = pb.alpha;
+        const double pref = kTwoPi25 / (p * q * sqrt(p + q)) * pa.Kc * pb.Kc;
+        double R[nherm(7)];
+        hermite_r_rt<double>(la + lb + 1, p * q / (p + q), pa.C[0] - pb.C[0], pa.C[1] - pb.C[1], pa.C[2] - pb.C[2],
+                             pref, g.boys, R);
+        for (int b = 0; b < NB; ++b) {
+          int ce[3];
+          for (int d = 0; d < 3; ++d) ce[d] = cart_exp(lb, b, d, true);
+          double Gh[nherm(4)];
+          for (int t = 0; t <= la + 1; ++t)
+            for (int u = 0; u + t <= la + 1; ++u)
+              for (int v = 0; v + u + t <= la + 1; ++v) {
+                double sacc = 0.0;
+                for (int tt = 0; tt <= ce[0]; ++tt)
+                  for (int uu = 0; uu <= ce[1]; ++uu) {
+                    const double exy = Eb[0][ce[0] * (lb + 1) + tt] * Eb[1][ce[1] * (lb + 1) + uu];
+                    for (int vv = 0; vv <= ce[2]; ++vv) {
+                      const double e = exy * Eb[2][ce[2] * (lb + 1) + vv];
+                      sacc += (((tt + uu + vv) & 1) ? -e : e) * R[hidx(t + tt, u + uu, v + vv)];
+                    }
+                  }
+                Gh[hidx(t, u, v)] = sacc;
+              }
+          const int q_ = B.basis + b;
+          for (int a = 0; a < NA; ++a) {
+            const int p_ = A.basis + a;
+            double wt = g.G[(size_t)p_ * g.Maux + q_];
+            if (sP != sQ) wt += g.G[(size_t)q_ * g.Maux + p_];
+            if (wt == 0.0) continue;
+            wt *= g.waux[p_] * g.waux[q_];
+            int ae[3];
+            const int zero[3] = {0, 0, 0};
+            for (int d = 0; d < 3; ++d) ae[d] = cart_exp(la, a, d, true);
+            double dA[3], dB[3];
+            grad_detail::pair_grad(TA, ae, zero, p, 0.0, Gh, false, dA, dB);
+            for (int d = 0; d < 3; ++d) gP[d] += wt * dA[d];
+          }
+        }
+      }
+    }
+    const int atP = g.aux_atom[sP], atQ = g.aux_atom[sQ];
+    if (atP != atQ)
+      for (int d = 0; d < 3; ++d) {
+        atomicAdd(&s_grad[3 * atP + d], gP[d]);
+        atomicAdd(&s_grad[3 * atQ + d], -gP[d]);
+      }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < n3; i += blockDim.x)
+    if (s_grad[i] != 0.0) unsafeAtomicAdd(&g.grad[i], s_grad[i]);
+}
+
+__global__ void k_fill_cD(const double* __restrict__ c, const double* __restrict__ D, int P0, int pc, int M,
+                          double* __restrict__ G) {
+  const size_t n = (size_t)M * M, tot = n * pc;
+  for (size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x; id < tot; id += (size_t)gridDim.x * blockDim.x)
+    G[id] = c[P0 + id / n] * D[id % n];
 }
 
 // W_P = sum_{m,n} B[m][P][n] D[m][n]: one workgroup per P.
@@ -282,6 +462,32 @@ __global__ void k_to_f32(const double* __restrict__ x, size_t n, float* __restri
 }
 
 }  // namespace
+
+hipError_t launch_ri_grad_3c(const RIGradArgs& a, hipStream_t s) {
+  if (a.natoms <= 0 || a.natoms > kGradMaxAtoms) return hipErrorInvalidValue;
+  const size_t n = (size_t)a.nsp * a.nsh;
+  if (n == 0) return hipSuccess;
+  const unsigned grid = (unsigned)std::min<size_t>(8192, (n + kRIGradThreads - 1) / kRIGradThreads);
+  hipLaunchKernelGGL(k_ri_grad_3c, dim3(grid), dim3(kRIGradThreads), 3 * a.natoms * sizeof(double), s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_ri_grad_2c(const RIGradArgs& a, hipStream_t s) {
+  if (a.natoms <= 0 || a.natoms > kGradMaxAtoms) return hipErrorInvalidValue;
+  const size_t n = (size_t)a.nsh * (a.nsh + 1) / 2;
+  if (n == 0) return hipSuccess;
+  const unsigned grid = (unsigned)std::min<size_t>(4096, (n + kRIGradThreads - 1) / kRIGradThreads);
+  hipLaunchKernelGGL(k_ri_grad_2c, dim3(grid), dim3(kRIGradThreads), 3 * a.natoms * sizeof(double), s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_cD(const double* c, const double* D, int P0, int pc, int M, double* G, hipStream_t s) {
+  const size_t tot = (size_t)M * M * pc;
+  if (tot == 0) return hipSuccess;
+  const unsigned grid = (unsigned)std::min<size_t>(16384, (tot + 255) / 256);
+  hipLaunchKernelGGL(k_fill_cD, dim3(grid), dim3(256), 0, s, c, D, P0, pc, M, G);
+  return hipGetLastError();
+}
 
 hipError_t launch_int2c2e(const RI2cArgs& a, hipStream_t s) {
   if (a.npairs == 0) return hipSuccess;

@@ -43,6 +43,34 @@ struct RI3cArgs {
   double* B;  // [M][Maux][M]
 };
 
+// RI-JK energy gradient: sum_P sum_mn G^P_mn d(mn|P)/dX over the shell pairs
+// and the auxiliary shells [sh0, sh0 + nsh) whose functions form the chunk
+// [P0, P0 + pc) of G (G[p - P0][m][n], real basis, symmetric in m, n), and
+// sum_PQ Gam_PQ d(P|Q)/dX; per-atom sums are added into grad.
+struct RIGradArgs {
+  const SPair* sp;
+  const PairRec* pr;
+  const double* alpha;  // exponent of the first primitive of each pair record
+  int nsp;
+  const AuxShell* shells;
+  const AuxPrim* prims;
+  int sh0, nsh;
+  const double* w;     // orbital raw-basis factors
+  const double* waux;  // auxiliary raw-basis factors
+  int M, Maux, P0;
+  const int* shell_atom;
+  const int* aux_atom;
+  int natoms;
+  double thr;
+  const double* boys;
+  const double* G;    // 3-centre gradient density chunk, or Gam (Maux x Maux) for the 2-centre term
+  double* grad;       // 3 natoms, accumulated
+};
+hipError_t launch_ri_grad_3c(const RIGradArgs& a, hipStream_t s);
+hipError_t launch_ri_grad_2c(const RIGradArgs& a, hipStream_t s);
+// G[p][m][n] = c[P0 + p] D[m][n] for p < pc.
+hipError_t launch_fill_cD(const double* c, const double* D, int P0, int pc, int M, double* G, hipStream_t s);
+
 hipError_t launch_int2c2e(const RI2cArgs& a, hipStream_t s);
 hipError_t launch_int3c2e(const RI3cArgs& a, hipStream_t s);
 hipError_t launch_ri_w(const double* B, const double* D, int M, int Maux, double* W, hipStream_t s);

@@ -353,7 +353,7 @@ ScfResult rhf_run(const Molecule& mol, const Molecule* aux, const Params& prm, b
     const int na = (int)mol.atoms.size();
     DevBuf<double> g1(3 * (size_t)na), g2(3 * (size_t)na);
     compute_oneint_gradient(mol, w.D.ptr, w.tmp2.ptr, g1.ptr, 0);
-    eng.gradient_2e(w.D.ptr, na, g2.ptr, 0);
+    eng.gradient_2e(w.D.ptr, na, g2.ptr, 0, w.C.ptr, nocc);  // D = 2 C_occ C_occ^T at the converged step
     GANSU_HIP(hipDeviceSynchronize());
     const auto h1 = g1.download(), h2 = g2.download();
     res.grad = nuclear_repulsion_gradient(mol);

@@ -158,14 +158,21 @@ class FockEngine:
         check(lib().gansu_fock_last_profile(self._h, ptr(rows), n.value, ctypes.byref(n)))
         return rows[:n.value]
 
-    def gradient_2e(self, D, n_atoms=None, stream=None):
+    def gradient_2e(self, D, n_atoms=None, stream=None, C=None, nocc=None):
         """Two-electron term of the RHF nuclear gradient, sum Gamma(D) d(ab|cd)/dX,
-        as an (n_atoms, 3) float64 CUDA tensor."""
+        as an (n_atoms, 3) float64 CUDA tensor.  eri_method="ri" needs the
+        orbitals C (M x M, first nocc columns occupied, D = 2 C_occ C_occ^T) and
+        differentiates the RI energy 1/2 D.J_RI - 1/4 D.K_RI."""
         import torch
         self._check_mat(D, "D")
         n = int(n_atoms if n_atoms is not None else self.mol.n_atoms)
         g = torch.empty((n, 3), dtype=torch.float64, device=D.device)
-        check(lib().gansu_fock_gradient_rhf(self._h, ptr(D), n, ptr(g), self._stream(stream)))
+        if C is None:
+            check(lib().gansu_fock_gradient_rhf(self._h, ptr(D), n, ptr(g), self._stream(stream)))
+        else:
+            self._check_mat(C, "C")
+            k = int(nocc if nocc is not None else self.mol.n_electrons // 2)
+            check(lib().gansu_fock_gradient_rhf_occ(self._h, ptr(D), ptr(C), k, n, ptr(g), self._stream(stream)))
         return g
 
     def __del__(self):

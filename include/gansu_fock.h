@@ -181,6 +181,17 @@ int gansu_rhf_run(const gansu_molecule* mol, const gansu_molecule* aux_mol, cons
 /* Two-electron term for an engine with eri_method=stored|direct, after
  * precompute; d_grad has 3*n_atoms entries.                                 */
 int gansu_fock_gradient_rhf(gansu_fock_engine* e, const double* d_D, int n_atoms, double* d_grad, void* stream);
+/* The same for any eri_method, given the occupied orbitals as in
+ * gansu_fock_build_rhf_occ (d_C row-major M x M, the first nocc columns
+ * occupied, d_D = 2 C_occ C_occ^T).  eri_method=ri differentiates the RI
+ * energy 1/2 D.J_RI - 1/4 D.K_RI of the fitted integrals B = L^-1 (Q|mn)
+ * (rhf.cu / gpu_manager.cu:1311-1349 build them): the 3-centre and 2-centre
+ * derivative integrals contracted with
+ *   G^P = c_P D - 1/2 D (V^-1 A)^P D,   Gam = -1/2 c c^T + 1/4 V^-1 M V^-1,
+ *   c = V^-1 (A.D),  M_PQ = tr(A^P D A^Q D),  A^P_mn = (mn|P),  V = (P|Q);
+ * stored|direct ignore d_C and nocc.                                         */
+int gansu_fock_gradient_rhf_occ(gansu_fock_engine* e, const double* d_D, const double* d_C, int nocc, int n_atoms,
+                                double* d_grad, void* stream);
 /* One-electron and overlap terms (kinetic, nuclear attraction including the
  * Hellmann-Feynman term of every nucleus, -W dS).                          */
 int gansu_oneint_gradient(const gansu_molecule* mol, const double* d_D, const double* d_W, double* d_grad,

@@ -17,6 +17,7 @@ import pytest
 import oracle_lib as O
 from dft_gansu_amd import molecule as gm
 from dft_gansu_amd.molecule import basis_path
+from helpers import write_even_tempered_aux
 
 
 def _water():
@@ -121,15 +122,21 @@ def test_rhf_gradient_vanishes_by_symmetry():
     assert abs(g[0, 2] + g[1, 2]) < 1e-12
 
 
-def test_ri_gradient_matches_finite_differences():
+@pytest.mark.parametrize("aux_basis", ["6-31g_st", "even-tempered"])
+def test_ri_gradient_matches_finite_differences(tmp_path, aux_basis):
     """RI-JK two-electron energy for a fixed density: the 3-centre / 2-centre
     derivative contractions against central differences of
     1/2 D.J_RI - 1/4 D.K_RI built from the oracle's own RI tensor.  The fitting
-    basis is 6-31G* (cond(V) ~ 1e3): with cc-pVDZ-RIFIT (cond(V) ~ 4e8) the
-    difference quotient of the exchange part scatters by 1e-3 relative with the
-    step size, so it cannot pin the analytic value."""
+    bases are 6-31G* (cond(V) ~ 1e3) and an even-tempered set with f shells
+    (cond ~ 1e5): with cc-pVDZ-RIFIT (cond(V) ~ 4e8) the difference quotient of
+    the exchange part scatters by 1e-3 relative with the step size, so it
+    cannot pin the analytic value."""
     Z, xyz = _water()
-    gbs, abs_ = basis_path("cc-pvdz"), basis_path("6-31g_st")
+    gbs = basis_path("cc-pvdz")
+    if aux_basis == "even-tempered":
+        abs_ = write_even_tempered_aux(tmp_path / "aux.gbs")
+    else:
+        abs_ = basis_path(aux_basis)
     prims, types, norm = O.build_molecule(gbs, Z, xyz)
     D = _sym(len(norm), 12)
 
