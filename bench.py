@@ -81,7 +81,7 @@ def ri_leg(log_fn):
     against the FP64 RI SCF."""
     import numpy as np
     import torch
-    from dft_gansu_amd import FockEngine, Parameters, rhf
+    from dft_gansu_amd import FockEngine, Parameters, rhf, rhf_gradient
     from dft_gansu_amd import molecule as gm
     mol, aux = gm.config_molecule(4)
     M, nocc = mol.n_basis, mol.n_electrons // 2
@@ -120,17 +120,31 @@ def ri_leg(log_fn):
             t = timed(lambda: e.compute_fock_matrix_occ(D, C, nocc, H), 3)
         out[name + "_build_s"] = round(t, 4)
         out[name + "_precompute_s"] = round(pre, 3)
+        if name == "ri_mixed":  # RI-JK energy gradient for the same density and orbitals
+            g = [None]
+
+            def grad():
+                g[0] = e.gradient_2e(D, C=C, nocc=nocc)
+
+            out["ri_gradient_2e_s"] = round(timed(grad, 1), 4)
+            out["ri_gradient_translational_residual"] = float(g[0].sum(dim=0).abs().max())
+            log_fn(f"[bench] 64-water RI two-electron gradient {out['ri_gradient_2e_s']} s")
         del e
         torch.cuda.empty_cache()
         log_fn(f"[bench] 64-water {name} build {t:.4f} s (precompute {pre:.2f} s)")
     out["speedup_mixed_ri_vs_fp64_direct"] = round(out["fp64_direct_build_s"] / out["ri_mixed_build_s"], 2)
     out["speedup_mixed_ri_vs_fp64_ri_density"] = round(out["ri_fp64_density_build_s"] / out["ri_mixed_build_s"], 2)
     scf = {}
+    # the FP64 energy, then the mixed energy + analytic gradient (one SCF each)
     for name, kw in (("fp64", dict(ri_k="occupied")), ("mixed", dict(precision="mixed"))):
+        prm = Parameters(eri_method="ri", convergence_energy_threshold=1e-8, maxiter=60, **kw)
         t0 = time.perf_counter()
-        r = rhf(mol, Parameters(eri_method="ri", convergence_energy_threshold=1e-8, maxiter=60, **kw), aux=aux)
+        r = rhf(mol, prm, aux=aux) if name == "fp64" else rhf_gradient(mol, prm, aux=aux)
         scf[name] = dict(wall_s=round(time.perf_counter() - t0, 3), energy=r["e_total"], iterations=r["iterations"],
                          converged=r["converged"])
+        if "grad" in r:
+            scf[name]["with_gradient"] = True
+            scf[name]["max_abs_grad"] = float(np.max(np.abs(r["grad"])))
         log_fn(f"[bench] 64-water RI SCF {name}: {scf[name]}")
     scf["dE_mixed_vs_fp64"] = scf["mixed"]["energy"] - scf["fp64"]["energy"]
     out["scf"] = scf

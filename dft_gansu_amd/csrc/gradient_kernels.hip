@@ -44,10 +44,20 @@ struct DPrimG {  // == gansu_primitive_shell layout
 };
 
 constexpr int kG1MaxE = 4 * 6 * 9;  // (la+2)(lb+4)(la+lb+5), la, lb <= 2
+constexpr int kG1MaxEV = 4 * 4 * 7;  // (la+2)(lb+2)(la+lb+3): the attraction part
+constexpr int kG1NucGroup = 8;       // nuclei per pass of the attraction part
+// A primitive pair whose prefactor times the density weights falls below this
+// contributes nothing at FP64 precision to any component (the Gaussian
+// product factor exp(-ab/p r^2) makes most pairs of a large molecule vanish).
+constexpr double kG1Screen = 1e-17;
 
 // One lane = one basis-function pair mu >= nu; derivatives of S, T and V
 // with respect to both basis centres, and the Hellmann-Feynman term of every
-// nucleus (-(d/dA + d/dB) of that nucleus' attraction integral).
+// nucleus (-(d/dA + d/dB) of that nucleus' attraction integral).  The nuclei
+// are taken kG1NucGroup at a time in a wave-uniform loop: each lane keeps that
+// group's Hellmann-Feynman sums over its primitive pairs in registers, and a
+// wave reduction leaves one LDS atomic per nucleus and wave (an atomic per lane
+// and nucleus serialised 64 lanes on one address).
 __global__ __launch_bounds__(kGradThreads) void k_grad_1e(OneIntArgs a, const double* __restrict__ Dm,
                                                           const double* __restrict__ Wm, double* grad) {
   extern __shared__ double s_grad[];
@@ -57,46 +67,59 @@ __global__ __launch_bounds__(kGradThreads) void k_grad_1e(OneIntArgs a, const do
   const size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int M = a.M;
   const int mu = id < (size_t)M * M ? (int)(id / M) : 0, nu = id < (size_t)M * M ? (int)(id % M) : 1;
-  if (id < (size_t)M * M && nu <= mu) {
-    const int sa = a.bf_shell[mu], sb = a.bf_shell[nu];
-    const int ca = mu - a.shell_basis[sa], cb = nu - a.shell_basis[sb];
-    const int la = a.shell_L[sa], lb = a.shell_L[sb];
-    int ea[3], eb[3];
-    for (int d = 0; d < 3; ++d) { ea[d] = cart_exp(la, ca, d); eb[d] = cart_exp(lb, cb, d); }
-    const double ww = a.w[mu] * a.w[nu] * (mu == nu ? 1.0 : 2.0);
-    const double dd = Dm[(size_t)mu * M + nu] * ww, wd = Wm[(size_t)mu * M + nu] * ww;
-    double gA[3] = {0.0, 0.0, 0.0}, gB[3] = {0.0, 0.0, 0.0};
-    int atA = 0, atB = 0;
+  const bool act = id < (size_t)M * M && nu <= mu;
+  const int sa = act ? a.bf_shell[mu] : 0, sb = act ? a.bf_shell[nu] : 0;
+  const int ca = act ? mu - a.shell_basis[sa] : 0, cb = act ? nu - a.shell_basis[sb] : 0;
+  const int la = act ? a.shell_L[sa] : 0, lb = act ? a.shell_L[sb] : 0;
+  int ea[3], eb[3];
+  for (int d = 0; d < 3; ++d) { ea[d] = cart_exp(la, ca, d); eb[d] = cart_exp(lb, cb, d); }
+  const double ww = act ? a.w[mu] * a.w[nu] * (mu == nu ? 1.0 : 2.0) : 0.0;
+  const double dd = act ? Dm[(size_t)mu * M + nu] * ww : 0.0, wd = act ? Wm[(size_t)mu * M + nu] * ww : 0.0;
+  double gA[3] = {0.0, 0.0, 0.0}, gB[3] = {0.0, 0.0, 0.0};
+  int atA = 0, atB = 0;
+  const int pa0 = act ? a.shell_off[sa] : 0, pa1 = act ? a.shell_off[sa + 1] : 0;
+  const int pb0 = act ? a.shell_off[sb] : 0, pb1 = act ? a.shell_off[sb + 1] : 0;
+  auto prim_pair = [&](int ia, int ib, double* P, double& al, double& be, double& p, double& pf) {
+    const DPrimG pa = ((const DPrimG*)a.prims)[a.shell_prims[ia]];
+    const DPrimG pb = ((const DPrimG*)a.prims)[a.shell_prims[ib]];
+    atA = pa.atom_index;
+    atB = pb.atom_index;
+    al = pa.exponent;
+    be = pb.exponent;
+    p = al + be;
+    const double A[3] = {pa.x, pa.y, pa.z}, B[3] = {pb.x, pb.y, pb.z};
+    double r2 = 0.0;
+    for (int d = 0; d < 3; ++d) {
+      P[3 + d] = A[d];
+      P[6 + d] = B[d];
+      P[d] = (al * A[d] + be * B[d]) / p;
+      r2 += (A[d] - B[d]) * (A[d] - B[d]);
+    }
+    const double K = exp(-al * be / p * r2);
+    const double na = 0.71270547035499016035 * double(1 << la) * pow(al, (2.0 * la + 3.0) / 4.0);
+    const double nb = 0.71270547035499016035 * double(1 << lb) * pow(be, (2.0 * lb + 3.0) / 4.0);
+    pf = pa.coefficient * pb.coefficient * na * nb * K;
+  };
+  // overlap and kinetic
+  {
     const int nj = lb + 4, ntt = la + lb + 5;
-    for (int ia = a.shell_off[sa]; ia < a.shell_off[sa + 1]; ++ia) {
-      const DPrimG pa = ((const DPrimG*)a.prims)[a.shell_prims[ia]];
-      atA = pa.atom_index;
-      for (int ib = a.shell_off[sb]; ib < a.shell_off[sb + 1]; ++ib) {
-        const DPrimG pb = ((const DPrimG*)a.prims)[a.shell_prims[ib]];
-        atB = pb.atom_index;
-        const double al = pa.exponent, be = pb.exponent, p = al + be;
-        const double A[3] = {pa.x, pa.y, pa.z}, B[3] = {pb.x, pb.y, pb.z};
-        double P[3], r2 = 0.0;
+    for (int ia = pa0; ia < pa1; ++ia)
+      for (int ib = pb0; ib < pb1; ++ib) {
+        double P[9], al, be, p, pf;
+        prim_pair(ia, ib, P, al, be, p, pf);
+        const double ovl = pow(kPi / p, 1.5) * pf;
+        const double p1 = 1.0 + p;  // the derivatives carry up to three powers of the exponents
+        if (fabs(ovl) * (fabs(dd) + fabs(wd)) * p1 * p1 * p1 < kG1Screen) continue;
         double E[3][kG1MaxE];
-        for (int d = 0; d < 3; ++d) {
-          P[d] = (al * A[d] + be * B[d]) / p;
-          r2 += (A[d] - B[d]) * (A[d] - B[d]);
-          hermite_e_rt<double>(la + 1, lb + 3, p, P[d] - A[d], P[d] - B[d], E[d]);
-        }
+        for (int d = 0; d < 3; ++d) hermite_e_rt<double>(la + 1, lb + 3, p, P[d] - P[3 + d], P[d] - P[6 + d], E[d]);
         auto row = [&](int d, int i, int j) -> const double* { return E[d] + (i * nj + j) * ntt; };
         auto s1 = [&](int d, int i, int j) { return row(d, i, j)[0]; };
         auto t1 = [&](int d, int i, int j) {
           return -2.0 * be * be * row(d, i, j + 2)[0] + be * (2 * j + 1) * row(d, i, j)[0] -
                  (j >= 2 ? 0.5 * j * (j - 1) * row(d, i, j - 2)[0] : 0.0);
         };
-        const double K = exp(-al * be / p * r2);
-        const double na = 0.71270547035499016035 * double(1 << la) * pow(al, (2.0 * la + 3.0) / 4.0);
-        const double nb = 0.71270547035499016035 * double(1 << lb) * pow(be, (2.0 * lb + 3.0) / 4.0);
-        const double pf = pa.coefficient * pb.coefficient * na * nb * K;
-        const double ovl = pow(kPi / p, 1.5) * pf;
         double s0[3], t0[3];
         for (int d = 0; d < 3; ++d) { s0[d] = s1(d, ea[d], eb[d]); t0[d] = t1(d, ea[d], eb[d]); }
-        // overlap and kinetic
         for (int k = 0; k < 2; ++k) {
           const double ex = k == 0 ? al : be;
           for (int d = 0; d < 3; ++d) {
@@ -115,14 +138,32 @@ __global__ __launch_bounds__(kGradThreads) void k_grad_1e(OneIntArgs a, const do
             if (k == 0) gA[d] += v; else gB[d] += v;
           }
         }
-        // nuclear attraction
-        const int L = la + lb;
-        for (int c = 0; c < a.natoms; ++c) {
+      }
+  }
+  // nuclear attraction, kG1NucGroup nuclei per wave-uniform pass
+  const int L = la + lb;
+  const int nj = lb + 2, ntt = la + lb + 3;
+  for (int c0 = 0; c0 < a.natoms; c0 += kG1NucGroup) {
+    double hf[kG1NucGroup][3];
+#pragma unroll
+    for (int cc = 0; cc < kG1NucGroup; ++cc) hf[cc][0] = hf[cc][1] = hf[cc][2] = 0.0;
+    for (int ia = pa0; ia < pa1; ++ia)
+      for (int ib = pb0; ib < pb1; ++ib) {
+        double P[9], al, be, p, pf;
+        prim_pair(ia, ib, P, al, be, p, pf);
+        const double vpf = 2.0 * kPi / p * pf * dd;
+        if (fabs(vpf) * a.zsum * (1.0 + p) * (1.0 + p) < kG1Screen) continue;
+        double E[3][kG1MaxEV];
+        for (int d = 0; d < 3; ++d) hermite_e_rt<double>(la + 1, lb + 1, p, P[d] - P[3 + d], P[d] - P[6 + d], E[d]);
+        auto row = [&](int d, int i, int j) -> const double* { return E[d] + (i * nj + j) * ntt; };
+#pragma unroll
+        for (int cc = 0; cc < kG1NucGroup; ++cc) {
+          const int c = c0 + cc;
+          if (c >= a.natoms) break;
           double R[nherm(5)];
           hermite_r_rt<double>(L + 1, p, P[0] - a.atom_xyz[3 * c], P[1] - a.atom_xyz[3 * c + 1],
                                P[2] - a.atom_xyz[3 * c + 2], 1.0, a.boys, R);
-          const double vf = -a.Z[c] * 2.0 * kPi / p * pf * dd;
-          double hf[3] = {0.0, 0.0, 0.0};
+          const double vf = -a.Z[c] * vpf;
           for (int k = 0; k < 2; ++k) {
             const double ex = k == 0 ? al : be;
             for (int d = 0; d < 3; ++d) {
@@ -140,18 +181,25 @@ __global__ __launch_bounds__(kGradThreads) void k_grad_1e(OneIntArgs a, const do
               }
               v *= vf;
               if (k == 0) gA[d] += v; else gB[d] += v;
-              hf[d] -= v;
+              hf[cc][d] -= v;
             }
           }
-          for (int d = 0; d < 3; ++d) atomicAdd(&s_grad[3 * c + d], hf[d]);
         }
       }
-    }
+    // every lane of the wave reaches this point (the loop above is uniform)
+#pragma unroll
+    for (int cc = 0; cc < kG1NucGroup; ++cc)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const double v = wave_sum_g(hf[cc][d]);
+        if ((threadIdx.x & 63) == 0 && c0 + cc < a.natoms && v != 0.0) atomicAdd(&s_grad[3 * (c0 + cc) + d], v);
+      }
+  }
+  if (act)
     for (int d = 0; d < 3; ++d) {
       atomicAdd(&s_grad[3 * atA + d], gA[d]);
       atomicAdd(&s_grad[3 * atB + d], gB[d]);
     }
-  }
   __syncthreads();
   for (int i = threadIdx.x; i < n3; i += blockDim.x)
     if (s_grad[i] != 0.0) unsafeAtomicAdd(&grad[i], s_grad[i]);

@@ -370,49 +370,94 @@ void FockEngine::gradient_ri(const double* d_D, const double* d_C, int nocc, int
   blas_ok(rocblas_dtrsm(r.h, rocblas_side_right, rocblas_fill_upper, rocblas_operation_transpose,
                         rocblas_diagonal_non_unit, (int)no2, Ma, &one, r.L.ptr, Ma, S.ptr, (int)no2),
           "rocblas_dtrsm(St)");
-  RIGradArgs g{d_sp_.ptr, d_pr_.ptr, d_alpha_.ptr, (int)sp_h_.size(), r.shells.ptr, r.prims.ptr, 0,
-               (int)r.shells_h.size(), d_w_.ptr, r.waux.ptr, M, Ma, 0, d_shell_atom_.ptr, r.aux_atom.ptr, natoms, thr_,
+  // auxiliary shells in basis-function order, so that P-chunks are contiguous
+  const int nsh = (int)r.shells_h.size();
+  std::vector<int> order(nsh);
+  for (int i = 0; i < nsh; ++i) order[i] = i;
+  std::sort(order.begin(), order.end(), [&](int a, int b) { return r.shells_h[a].basis < r.shells_h[b].basis; });
+  RIGradArgs g{d_sp_.ptr, d_pr_.ptr, d_alpha_.ptr, (int)sp_h_.size(), r.shells.ptr, r.prims.ptr, nullptr,
+               nsh, d_w_.ptr, r.waux.ptr, M, Ma, 0, d_shell_atom_.ptr, r.aux_atom.ptr, natoms, thr_,
                boys_.ptr, N.ptr, d_grad};
   GANSU_HIP(launch_ri_grad_2c(g, s));
-  // G^P chunks aligned to auxiliary shells, then the 3-centre derivatives
+  // G^P chunks of whole auxiliary shells (basis order), then the 3-centre
+  // derivatives class by class: orbital pair class (la, lb) x auxiliary L
   const size_t mm = (size_t)M * M;
-  const int nsh = (int)r.shells_h.size();
   int maxc = 1;
   for (const auto& as : r.shells_h) maxc = std::max(maxc, ncart(as.L));
-  // the K workspace holds a chunk of G unless one shell does not fit in it
+  // the K workspace holds a chunk of G; a separate buffer of up to 12 GB
+  // (and a quarter of the free memory) when that gives larger chunks
   int pg_max = (int)std::min<size_t>(Ma, r.T.n / mm);
   DevBuf<double> Gown;
   double* G = r.T.ptr;
-  if (pg_max < maxc) {
-    pg_max = maxc;
-    Gown.alloc((size_t)maxc * mm);
-    G = Gown.ptr;
+  {
+    size_t fr = 0, tot = 0;
+    GANSU_HIP(hipMemGetInfo(&fr, &tot));
+    const size_t want = std::min<size_t>(Ma, std::min<size_t>(size_t(12) << 30, fr / 4) / (mm * sizeof(double)));
+    if ((int)want > pg_max || pg_max < maxc) {
+      pg_max = std::max<int>((int)want, maxc);
+      Gown.alloc((size_t)pg_max * mm);
+      G = Gown.ptr;
+    }
   }
   DevBuf<double> Yt((size_t)pg_max * nocc * M);
+  struct Part {
+    int P0, pc, off[kMaxLAux + 1], cnt[kMaxLAux + 1];
+  };
+  std::vector<Part> parts;
+  std::vector<int> grouped(nsh);
   for (int sh0 = 0; sh0 < nsh;) {
-    const int P0 = r.shells_h[sh0].basis;
+    const int P0 = r.shells_h[order[sh0]].basis;
     int sh1 = sh0, pc = 0;
-    while (sh1 < nsh && r.shells_h[sh1].basis == P0 + pc && pc + ncart(r.shells_h[sh1].L) <= pg_max) {
-      pc += ncart(r.shells_h[sh1].L);
+    while (sh1 < nsh && r.shells_h[order[sh1]].basis == P0 + pc && pc + ncart(r.shells_h[order[sh1]].L) <= pg_max) {
+      pc += ncart(r.shells_h[order[sh1]].L);
       ++sh1;
     }
-    if (sh1 == sh0) throw Error(GANSU_E_STATE, "auxiliary shells are not in basis-function order");
-    GANSU_HIP(launch_fill_cD(c.ptr, d_D, P0, pc, M, G, s));
+    if (sh1 == sh0) throw Error(GANSU_E_STATE, "auxiliary basis functions are not covered contiguously by shells");
+    Part pt{P0, pc, {}, {}};
+    int pos = sh0;
+    for (int l = 0; l <= kMaxLAux; ++l) {
+      pt.off[l] = pos;
+      for (int i = sh0; i < sh1; ++i)
+        if (r.shells_h[order[i]].L == l) grouped[pos++] = order[i];
+      pt.cnt[l] = pos - pt.off[l];
+    }
+    parts.push_back(pt);
+    sh0 = sh1;
+  }
+  DevBuf<int> d_grouped;
+  d_grouped.upload(grouped);
+  g.aux_list = d_grouped.ptr;
+  // orbital shell pairs by class (la >= lb)
+  std::vector<int> pairs_h;
+  int pcls[kMaxL + 1][kMaxL + 1][2] = {};
+  for (int la = 0; la <= kMaxL; ++la)
+    for (int lb = 0; lb <= la; ++lb) {
+      pcls[la][lb][0] = (int)pairs_h.size();
+      for (int id = 0; id < (int)sp_h_.size(); ++id)
+        if (sp_h_[id].la == la && sp_h_[id].lb == lb) pairs_h.push_back(id);
+      pcls[la][lb][1] = (int)pairs_h.size() - pcls[la][lb][0];
+    }
+  if (pairs_h.size() != sp_h_.size()) throw Error(GANSU_E_STATE, "shell pair with L_i < L_j");
+  DevBuf<int> d_pairs;
+  d_pairs.upload(pairs_h);
+  for (const Part& pt : parts) {
+    GANSU_HIP(launch_fill_cD(c.ptr, d_D, pt.P0, pt.pc, M, G, s));
     blas_ok(rocblas_dgemm_strided_batched(r.h, rocblas_operation_none, rocblas_operation_none, nocc, M, nocc, &two,
-                                          S.ptr + (size_t)P0 * no2, nocc, (rocblas_stride)no2, d_C, M, 0, &zero, Yt.ptr,
-                                          nocc, (rocblas_stride)nocc * M, pc),
+                                          S.ptr + (size_t)pt.P0 * no2, nocc, (rocblas_stride)no2, d_C, M, 0, &zero,
+                                          Yt.ptr, nocc, (rocblas_stride)nocc * M, pt.pc),
             "rocblas_dgemm_strided_batched(Y)");
     const double alpha = -0.5 * two;
     blas_ok(rocblas_dgemm_strided_batched(r.h, rocblas_operation_transpose, rocblas_operation_none, M, M, nocc, &alpha,
                                           Yt.ptr, nocc, (rocblas_stride)nocc * M, d_C, M, 0, &one, G, M,
-                                          (rocblas_stride)mm, pc),
+                                          (rocblas_stride)mm, pt.pc),
             "rocblas_dgemm_strided_batched(G)");
-    g.sh0 = sh0;
-    g.nsh = sh1 - sh0;
-    g.P0 = P0;
+    g.P0 = pt.P0;
     g.G = G;
-    GANSU_HIP(launch_ri_grad_3c(g, s));
-    sh0 = sh1;
+    for (int la = 0; la <= kMaxL; ++la)
+      for (int lb = 0; lb <= la; ++lb)
+        for (int l = 0; l <= kMaxLAux; ++l)
+          GANSU_HIP(launch_ri_grad_3c_class(la, lb, l, g, d_pairs.ptr + pcls[la][lb][0], pcls[la][lb][1], pt.off[l],
+                                            pt.cnt[l], s));
   }
   GANSU_HIP(hipStreamSynchronize(s));  // the temporaries above are released on return
 }

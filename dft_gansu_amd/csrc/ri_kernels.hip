@@ -134,97 +134,11 @@ __global__ void k_int3c2e(RI3cArgs g) {
 }
 
 // ------------------------------------------------- RI-JK energy gradient --
-// Centre derivatives by the E-shift 2a G(l+1) - l G(l-1) of the orbital pair's
-// functions (tables one order higher, R one order higher); the auxiliary
-// centre follows by translational invariance.  One lane per (shell pair,
-// auxiliary shell); per-atom sums in LDS, flushed once per workgroup.
+// 2-centre term (the 3-centre term has per-class kernels of its own).  The
+// centre derivatives take the E-shift 2a E(i+1) - i E(i-1) of the first
+// function's Hermite table; the second centre follows by translational
+// invariance.  Per-atom sums in LDS, flushed once per workgroup.
 constexpr int kRIGradThreads = 256;
-
-__global__ __launch_bounds__(kRIGradThreads) void k_ri_grad_3c(RIGradArgs g) {
-  extern __shared__ double s_grad[];
-  const int n3 = 3 * g.natoms;
-  for (int i = threadIdx.x; i < n3; i += blockDim.x) s_grad[i] = 0.0;
-  __syncthreads();
-  const size_t nitems = (size_t)g.nsp * g.nsh;
-  const size_t M = g.M;
-  for (size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x; id < nitems; id += (size_t)gridDim.x * blockDim.x) {
-    const int isp = (int)(id / g.nsh), iaux = g.sh0 + (int)(id % g.nsh);
-    const SPair sp = g.sp[isp];
-    const AuxShell C = g.shells[iaux];
-    if (sp.qmax * C.qmax < g.thr) continue;
-    const int la = sp.la, lb = sp.lb, lc = C.L;
-    const int NB = ncart(lb), NAB = ncart(la) * NB, NC = ncart(lc);
-    const bool diff = sp.si != sp.sj;
-    double gA[3] = {0.0, 0.0, 0.0}, gB[3] = {0.0, 0.0, 0.0};
-    for (int ip = 0; ip < sp.nprim; ++ip) {
-      const PairRec pr = g.pr[sp.prim_begin + ip];
-      const double exa = g.alpha[sp.prim_begin + ip], exb = pr.p - exa;
-      double Eb[3][4 * 4 * 7];  // (la+2)(lb+2)(la+lb+3), la, lb <= 2
-      for (int d = 0; d < 3; ++d) hermite_e_rt<double>(la + 1, lb + 1, pr.p, pr.PA[d], pr.PB[d], Eb[d]);
-      const grad_detail::ETab TB{{Eb[0], Eb[1], Eb[2]}, lb + 2, la + lb + 3};
-      for (int ic = 0; ic < C.nprim; ++ic) {
-        const AuxPrim pc = g.prims[C.prim_begin + ic];
-        if (pr.Q * pc.Q < g.thr) continue;
-        double Ek[3][4 * 4];  // (lc+1) x (lc+1), lc <= 3
-        for (int d = 0; d < 3; ++d) hermite_e_rt<double>(lc, 0, pc.alpha, 0.0, 0.0, Ek[d]);
-        const double p = pr.p, q = pc.alpha;
-        const double pref = kTwoPi25 / (p * q * sqrt(p + q)) * pr.Kc * pc.Kc;
-        const int L = la + lb + lc + 1;
-        double R[nherm(8)];
-        hermite_r_rt<double>(L, p * q / (p + q), pr.P[0] - pc.C[0], pr.P[1] - pc.C[1], pr.P[2] - pc.C[2], pref, g.boys,
-                             R);
-        for (int c = 0; c < NC; ++c) {
-          int ce[3];
-          for (int d = 0; d < 3; ++d) ce[d] = cart_exp(lc, c, d, true);
-          const int P = C.basis + c;
-          const double* Gp = g.G + (size_t)(P - g.P0) * M * M;
-          // Hermite intermediate of the auxiliary function, levels up to la+lb+1
-          double Gh[nherm(5)];
-          const int lab1 = la + lb + 1;
-          for (int t = 0; t <= lab1; ++t)
-            for (int u = 0; u + t <= lab1; ++u)
-              for (int v = 0; v + u + t <= lab1; ++v) {
-                double sacc = 0.0;
-                for (int tt = 0; tt <= ce[0]; ++tt)
-                  for (int uu = 0; uu <= ce[1]; ++uu) {
-                    const double exy = Ek[0][ce[0] * (lc + 1) + tt] * Ek[1][ce[1] * (lc + 1) + uu];
-                    for (int vv = 0; vv <= ce[2]; ++vv) {
-                      const double e = exy * Ek[2][ce[2] * (lc + 1) + vv];
-                      sacc += (((tt + uu + vv) & 1) ? -e : e) * R[hidx(t + tt, u + uu, v + vv)];
-                    }
-                  }
-                Gh[hidx(t, u, v)] = sacc;
-              }
-          const double wp = g.waux[P];
-          for (int ab = 0; ab < NAB; ++ab) {
-            const size_t m = sp.bi + ab / NB, n = sp.bj + ab % NB;
-            double wt = Gp[m * M + n];
-            if (diff) wt += Gp[n * M + m];
-            if (wt == 0.0) continue;
-            wt *= g.w[m] * g.w[n] * wp;
-            int ae[3], be[3];
-            for (int d = 0; d < 3; ++d) { ae[d] = cart_exp(la, ab / NB, d); be[d] = cart_exp(lb, ab % NB, d); }
-            double dA[3], dB[3];
-            grad_detail::pair_grad(TB, ae, be, exa, exb, Gh, true, dA, dB);
-            for (int d = 0; d < 3; ++d) {
-              gA[d] += wt * dA[d];
-              gB[d] += wt * dB[d];
-            }
-          }
-        }
-      }
-    }
-    const int atA = g.shell_atom[sp.si], atB = g.shell_atom[sp.sj], atC = g.aux_atom[iaux];
-    for (int d = 0; d < 3; ++d) {
-      if (gA[d] != 0.0) atomicAdd(&s_grad[3 * atA + d], gA[d]);
-      if (gB[d] != 0.0) atomicAdd(&s_grad[3 * atB + d], gB[d]);
-      if (gA[d] + gB[d] != 0.0) atomicAdd(&s_grad[3 * atC + d], -(gA[d] + gB[d]));
-    }
-  }
-  __syncthreads();
-  for (int i = threadIdx.x; i < n3; i += blockDim.x)
-    if (s_grad[i] != 0.0) unsafeAtomicAdd(&g.grad[i], s_grad[i]);
-}
 
 // sum_PQ Gam_PQ d(P|Q)/dX: one lane per auxiliary shell pair (P >= Q).
 __global__ __launch_bounds__(kRIGradThreads) void k_ri_grad_2c(RIGradArgs g) {
@@ -462,15 +376,6 @@ __global__ void k_to_f32(const double* __restrict__ x, size_t n, float* __restri
 }
 
 }  // namespace
-
-hipError_t launch_ri_grad_3c(const RIGradArgs& a, hipStream_t s) {
-  if (a.natoms <= 0 || a.natoms > kGradMaxAtoms) return hipErrorInvalidValue;
-  const size_t n = (size_t)a.nsp * a.nsh;
-  if (n == 0) return hipSuccess;
-  const unsigned grid = (unsigned)std::min<size_t>(8192, (n + kRIGradThreads - 1) / kRIGradThreads);
-  hipLaunchKernelGGL(k_ri_grad_3c, dim3(grid), dim3(kRIGradThreads), 3 * a.natoms * sizeof(double), s, a);
-  return hipGetLastError();
-}
 
 hipError_t launch_ri_grad_2c(const RIGradArgs& a, hipStream_t s) {
   if (a.natoms <= 0 || a.natoms > kGradMaxAtoms) return hipErrorInvalidValue;

@@ -43,10 +43,10 @@ struct RI3cArgs {
   double* B;  // [M][Maux][M]
 };
 
-// RI-JK energy gradient: sum_P sum_mn G^P_mn d(mn|P)/dX over the shell pairs
-// and the auxiliary shells [sh0, sh0 + nsh) whose functions form the chunk
+// RI-JK energy gradient: sum_P sum_mn G^P_mn d(mn|P)/dX over the chunk
 // [P0, P0 + pc) of G (G[p - P0][m][n], real basis, symmetric in m, n), and
-// sum_PQ Gam_PQ d(P|Q)/dX; per-atom sums are added into grad.
+// sum_PQ Gam_PQ d(P|Q)/dX over all nsh auxiliary shells; per-atom sums are
+// added into grad.
 struct RIGradArgs {
   const SPair* sp;
   const PairRec* pr;
@@ -54,9 +54,10 @@ struct RIGradArgs {
   int nsp;
   const AuxShell* shells;
   const AuxPrim* prims;
-  int sh0, nsh;
-  const double* w;     // orbital raw-basis factors
-  const double* waux;  // auxiliary raw-basis factors
+  const int* aux_list;  // 3-centre term: auxiliary shell ids, grouped per chunk and L
+  int nsh;              // 2-centre term: number of auxiliary shells
+  const double* w;      // orbital raw-basis factors
+  const double* waux;   // auxiliary raw-basis factors
   int M, Maux, P0;
   const int* shell_atom;
   const int* aux_atom;
@@ -66,7 +67,11 @@ struct RIGradArgs {
   const double* G;    // 3-centre gradient density chunk, or Gam (Maux x Maux) for the 2-centre term
   double* grad;       // 3 natoms, accumulated
 };
-hipError_t launch_ri_grad_3c(const RIGradArgs& a, hipStream_t s);
+// Per-class instance of the 3-centre term: orbital shell pairs plist[0..npairs)
+// of class (la, lb), la >= lb, and auxiliary shells aux_list[aoff .. aoff+naux)
+// of angular momentum lc, all inside the chunk of G.
+hipError_t launch_ri_grad_3c_class(int la, int lb, int lc, const RIGradArgs& a, const int* plist, int npairs,
+                                   int aoff, int naux, hipStream_t s);
 hipError_t launch_ri_grad_2c(const RIGradArgs& a, hipStream_t s);
 // G[p][m][n] = c[P0 + p] D[m][n] for p < pc.
 hipError_t launch_fill_cD(const double* c, const double* D, int P0, int pc, int M, double* G, hipStream_t s);

@@ -130,7 +130,11 @@ OneIntHost::OneIntHost(const Molecule& mol) : M(mol.nbasis), natoms((int)mol.ato
   }
   std::vector<double> x;
   std::vector<int> z;
-  for (const auto& a : mol.atoms) { x.insert(x.end(), a.xyz, a.xyz + 3); z.push_back(a.Z); }
+  for (const auto& a : mol.atoms) {
+    x.insert(x.end(), a.xyz, a.xyz + 3);
+    z.push_back(a.Z);
+    zsum += std::abs((double)a.Z);
+  }
   prims.upload(mol.prims);
   off.upload(o); sprims.upload(sp); sl.upload(l); sb.upload(bs); bfs.upload(bf); Z.upload(z);
   w.upload(wv); xyz.upload(x);
@@ -139,7 +143,7 @@ OneIntHost::OneIntHost(const Molecule& mol) : M(mol.nbasis), natoms((int)mol.ato
 
 OneIntArgs OneIntHost::args(double* S, double* T, double* V) const {
   return OneIntArgs{prims.ptr, off.ptr, sprims.ptr, sl.ptr, sb.ptr, bfs.ptr, w.ptr, M, natoms,
-                    xyz.ptr, Z.ptr, boys.ptr, S, T, V};
+                    xyz.ptr, Z.ptr, boys.ptr, S, T, V, zsum};
 }
 
 // Build S, T, V of a molecule on the device.

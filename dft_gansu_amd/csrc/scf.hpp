@@ -26,6 +26,7 @@ struct OneIntHost {
   DevBuf<int> off, sprims, sl, sb, bfs, Z;
   DevBuf<double> w, xyz, boys;
   int M = 0, natoms = 0;
+  double zsum = 0.0;
   explicit OneIntHost(const Molecule& mol);
   OneIntArgs args(double* S, double* T, double* V) const;
 };

@@ -19,6 +19,7 @@ struct OneIntArgs {
   const int* Z;
   const double* boys;
   double *S, *T, *V;
+  double zsum;             // sum of the nuclear charges
 };
 
 hipError_t launch_oneint(const OneIntArgs& a, hipStream_t s);
