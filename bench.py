@@ -151,6 +151,70 @@ def ri_leg(log_fn):
     return out
 
 
+def coronene_leg(log_fn):
+    """C24H12/cc-pVDZ + cc-pVDZ-RIFIT: every four-centre class up to (dd|dd)
+    through the batched class launches (stored FP64, stored mixed, direct) and
+    the RI-JK build (FP64 occupied form vs mixed), then the stored and RI SCF in
+    FP64 and mixed with their energy differences."""
+    import numpy as np
+    import torch
+    from dft_gansu_amd import FockEngine, Parameters, rhf
+    from dft_gansu_amd import molecule as gm
+    mol, aux = gm.config_molecule(2)
+    M, nocc = mol.n_basis, mol.n_electrons // 2
+    rng = np.random.default_rng(7)
+    C = torch.tensor(np.linalg.qr(rng.normal(size=(M, M)))[0], dtype=torch.float64, device="cuda")
+    D = (2.0 * C[:, :nocc] @ C[:, :nocc].T).contiguous()
+    H = torch.zeros_like(D)
+    out = {"workload": "C24H12 cc-pVDZ (M=%d) + cc-pVDZ-RIFIT (Maux=%d), nocc=%d" % (M, aux.n_basis, nocc)}
+
+    def timed(fn, n):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(n):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / n
+
+    legs = (("stored_fp64", dict(eri_method="stored"), False), ("stored_mixed", dict(eri_method="stored",
+            precision="mixed"), False), ("direct_fp64", dict(eri_method="direct"), False),
+            ("ri_fp64", dict(eri_method="ri", ri_k="occupied"), True), ("ri_mixed", dict(eri_method="ri",
+            precision="mixed"), True))
+    for name, kw, ri in legs:
+        e = FockEngine(mol, aux=aux if ri else None, **kw)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        e.precomputation()
+        torch.cuda.synchronize()
+        pre = time.perf_counter() - t0
+        t = timed((lambda: e.compute_fock_matrix_occ(D, C, nocc, H)) if ri else (lambda: e.compute_fock_matrix(D, H)),
+                  5)
+        out[name] = dict(build_s=round(t, 5), builds_per_s=round(1.0 / t, 2), precompute_s=round(pre, 3))
+        if not ri:
+            st = e.stats()
+            out[name]["stored_bytes"] = st["stored_bytes"]
+            out[name]["contracted_quartets"] = st["contracted_quartets"]
+        del e
+        torch.cuda.empty_cache()
+        log_fn(f"[bench] C24H12 {name}: {out[name]}")
+    out["speedup_ri_mixed_vs_direct_fp64"] = round(out["direct_fp64"]["build_s"] / out["ri_mixed"]["build_s"], 2)
+    scf = {}
+    for name, kw, ri in (("stored_fp64", dict(eri_method="stored"), False),
+                         ("stored_mixed", dict(eri_method="stored", precision="mixed"), False),
+                         ("ri_fp64", dict(eri_method="ri", ri_k="occupied"), True),
+                         ("ri_mixed", dict(eri_method="ri", precision="mixed"), True)):
+        t0 = time.perf_counter()
+        r = rhf(mol, Parameters(convergence_energy_threshold=1e-8, maxiter=100, **kw), aux=aux if ri else None)
+        scf[name] = dict(wall_s=round(time.perf_counter() - t0, 3), energy=r["e_total"], iterations=r["iterations"],
+                         converged=r["converged"])
+        log_fn(f"[bench] C24H12 SCF {name}: {scf[name]}")
+    scf["dE_stored_mixed_vs_fp64"] = scf["stored_mixed"]["energy"] - scf["stored_fp64"]["energy"]
+    scf["dE_ri_mixed_vs_fp64"] = scf["ri_mixed"]["energy"] - scf["ri_fp64"]["energy"]
+    out["scf"] = scf
+    return out
+
+
 def gradient_leg(log_fn):
     """C10H22/6-31G*: SCF then the analytic gradient; gradient s/call is the
     backward work alone (two-electron + one-electron kernels) at the converged
@@ -202,6 +266,7 @@ def main():
     ap.add_argument("--no-ri", action="store_true", help="skip the 64-water RI-JK leg")
     ap.add_argument("--no-grad", action="store_true", help="skip the C10H22/6-31G* gradient leg")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision store leg")
+    ap.add_argument("--no-coronene", action="store_true", help="skip the C24H12 four-centre + RI-JK leg")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the Fock build runs on a single MI355X in this build "
@@ -330,6 +395,9 @@ def main():
                 mixed["dE_vs_fp64"] = r["e_total"] - extra["scf_energy"]
         extra["mixed"] = mixed
         log(f"[bench] mixed store: {mixed}")
+    if not args.no_coronene:
+        extra["c24h12"] = coronene_leg(log)
+        torch.cuda.empty_cache()
     if not args.no_grad:
         extra["gradient"] = gradient_leg(log)
         torch.cuda.empty_cache()
