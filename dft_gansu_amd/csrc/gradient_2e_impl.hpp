@@ -210,6 +210,22 @@ __global__ __launch_bounds__(kGradThreads) void k_grad_2e(ClassLaunch c, Grad2eA
     const SPair ket = g.sp[ket_id];
     const bool active = in_block && (!KSPLIT || ikp < ket.nprim);
     double gA[3] = {0.0, 0.0, 0.0}, gB[3] = {0.0, 0.0, 0.0}, gC[3] = {0.0, 0.0, 0.0};
+    // runtime-indexed body: the bra primitive pair's E tables are wave-uniform,
+    // so three lanes build them once into the wave's LDS slot (read back by
+    // every lane instead of 64 private copies in scratch)
+    [[maybe_unused]] double* s_Eb = s_grad + 3 * g.natoms + wave * 3 * NEB;
+    if constexpr (!kGradUnrolled<LA, LB, LC, LD>) {
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      if (lane < 3) {
+        const PairRec pb = g.pr[bra.prim_begin + ibp];
+        const double PA = lane == 0 ? pb.PA[0] : lane == 1 ? pb.PA[1] : pb.PA[2];
+        const double PB = lane == 0 ? pb.PB[0] : lane == 1 ? pb.PB[1] : pb.PB[2];
+        hermite_e_rt<double>(LA + 1, LB + 1, pb.p, PA, PB, s_Eb + lane * NEB);
+      }
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+    }
     if (active) {
       // density blocks of Gamma_abcd = 1/2 D_ab D_cd - 1/8 (D_ac D_bd + D_ad D_bc)
       double dab[NAB], dcd[NCD], dac[NA * NC], dbd[NB * ND], dad[NA * ND], dbc[NB * NC];
@@ -348,9 +364,7 @@ __global__ __launch_bounds__(kGradThreads) void k_grad_2e(ClassLaunch c, Grad2eA
         const int ib = ibp;
         const PairRec pb = g.pr[bra.prim_begin + ib];
         const double exa = g.alpha[bra.prim_begin + ib], exb = pb.p - exa;
-        double Eb[3][NEB];
-        for (int d = 0; d < 3; ++d) hermite_e_rt<double>(LA + 1, LB + 1, pb.p, pb.PA[d], pb.PB[d], Eb[d]);
-        const ETab TB{{Eb[0], Eb[1], Eb[2]}, LB + 2, LAB + 3};
+        const ETab TB{{s_Eb, s_Eb + NEB, s_Eb + 2 * NEB}, LB + 2, LAB + 3};
         for (int ik = KSPLIT ? ikp : 0; ik < (KSPLIT ? ikp + 1 : ket.nprim); ++ik) {
           const PairRec pk = g.pr[ket.prim_begin + ik];
           if (pb.Q * pk.Q < g.thr) continue;
@@ -459,8 +473,10 @@ hipError_t launch_grad_2e_t(const ClassLaunch& c, const Grad2eArgs& a, int maxnp
   const long long nvirt = (long long)c.nblocks * maxnp * (kGradUnrolled<LA, LB, LC, LD> ? 1 : maxnk) *
                          kGradLanesPerQuartet<LA, LB, LC, LD>;
   const int grid = (int)std::min<long long>(2048, (nvirt + waves - 1) / waves);
-  hipLaunchKernelGGL((k_grad_2e<LA, LB, LC, LD>), dim3(grid), dim3(kGradThreads), 3 * a.natoms * sizeof(double), s,
-                     c, a, maxnp, maxnk);
+  constexpr int NEB = (LA + 2) * (LB + 2) * (LA + LB + 3);
+  const size_t lds = (3 * (size_t)a.natoms + (kGradUnrolled<LA, LB, LC, LD> ? 0 : (size_t)waves * 3 * NEB)) *
+                     sizeof(double);
+  hipLaunchKernelGGL((k_grad_2e<LA, LB, LC, LD>), dim3(grid), dim3(kGradThreads), lds, s, c, a, maxnp, maxnk);
   return hipGetLastError();
 }
 
