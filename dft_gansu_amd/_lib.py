@@ -31,7 +31,7 @@ EXPORTS = [
     "gansu_fock_algorithm_name", "gansu_fock_schwarz", "gansu_fock_num_primitive_pairs", "gansu_fock_stats",
     "gansu_fock_set_profiling", "gansu_fock_last_profile", "gansu_fock_destroy", "gansu_oneint", "gansu_rhf_run",
     "gansu_fock_gradient_rhf", "gansu_fock_gradient_rhf_occ", "gansu_oneint_gradient", "gansu_molecule_nuclear_repulsion_gradient",
-    "gansu_rhf_gradient", "gansu_fock_build_uhf", "gansu_fock_update_geometry",
+    "gansu_rhf_gradient", "gansu_fock_build_uhf", "gansu_fock_build_rohf", "gansu_fock_update_geometry",
 ]
 
 
@@ -89,6 +89,7 @@ def lib():
         L.gansu_molecule_nuclear_repulsion_gradient.argtypes = [vp, vp]
         L.gansu_rhf_gradient.argtypes = [vp, vp, vp, vp, vp, vp]
         L.gansu_fock_build_uhf.argtypes = [vp, vp, vp, vp, vp, vp, vp]
+        L.gansu_fock_build_rohf.argtypes = [vp, vp, vp, vp, vp, vp, ctypes.c_char_p, ip, ip, vp, vp, vp, vp]
         L.gansu_fock_update_geometry.argtypes = [vp, vp, ip]
         _LIB = L
         del dp

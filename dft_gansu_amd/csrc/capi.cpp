@@ -170,6 +170,18 @@ int gansu_fock_build_uhf(gansu_fock_engine* e, const double* d_Da, const double*
   });
 }
 
+int gansu_fock_build_rohf(gansu_fock_engine* e, const double* d_Dc, const double* d_Do, const double* d_H,
+                          const double* d_C, const double* d_S, const char* coupling, int num_closed, int num_open,
+                          double* d_Fc, double* d_Fo, double* d_F, void* stream) {
+  return guarded([&] {
+    if (!e) throw gansu::Error(GANSU_E_ARG, "null engine");
+    const std::string name = coupling ? coupling : "roothaan";
+    gansu::RohfCoupling cp{};
+    if (!gansu::rohf_coupling(name, &cp)) throw gansu::Error(GANSU_E_PARAM, "unknown ROHF coupling set: " + name);
+    e->e->build_rohf(d_Dc, d_Do, d_H, d_C, d_S, cp, num_closed, num_open, d_Fc, d_Fo, d_F, (hipStream_t)stream);
+  });
+}
+
 int gansu_fock_update_geometry(gansu_fock_engine* e, const double* xyz_bohr, int n_atoms) {
   return guarded([&] {
     if (!e) throw gansu::Error(GANSU_E_ARG, "null engine");

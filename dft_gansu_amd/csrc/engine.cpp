@@ -103,6 +103,7 @@ FockEngine::FockEngine(const gansu_primitive_shell* prims, size_t n_prims, const
 
 FockEngine::~FockEngine() {
   prof_clear();
+  rohf_release();
   graph_release();
   if (cap_) (void)hipStreamDestroy(cap_);
   streams_destroy();

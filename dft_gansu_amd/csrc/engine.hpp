@@ -81,6 +81,10 @@ struct PairClass {
   DevBuf<int4> meta;  // per sorted position: {bi, bj, si != sj, id}
 };
 
+// Coupling set of the unified ROHF Fock matrix by its parameter name
+// (rohf_parameter_name, rohf.cu:30-39); false for an unknown name.
+bool rohf_coupling(const std::string& name, RohfCoupling* out);
+
 class FockEngine {
  public:
   FockEngine(const gansu_primitive_shell* prims, size_t n_prims, const gansu_shell_type_info* types, int n_types,
@@ -95,6 +99,11 @@ class FockEngine {
                 const double* d_C = nullptr, int nocc = 0);
   // UHF (gpu_kernels.cu:289-340): F_a = H + J[D_a + D_b] - K[D_a], F_b likewise.
   void build_uhf(const double* d_Da, const double* d_Db, const double* d_H, double* d_Fa, double* d_Fb, hipStream_t s);
+  // ROHF (gpu_manager.cu:733-817): F_c, F_o from J/K of the closed and open
+  // densities, the unified MO-basis matrix R with coupling cp over nc closed
+  // and no open orbitals (columns of the row-major C), and F = S C R C^T S.
+  void build_rohf(const double* d_Dc, const double* d_Do, const double* d_H, const double* d_C, const double* d_S,
+                  const RohfCoupling& cp, int nc, int no, double* d_Fc, double* d_Fo, double* d_F, hipStream_t s);
   // Moves every primitive (and auxiliary primitive) to the position of its
   // atom and repeats the precompute if it had been done.
   void update_geometry(const double* xyz_bohr, int n_atoms);
@@ -193,7 +202,10 @@ class FockEngine {
   int n_graph_captures_ = 0;
   hipGraphExec_t graph_for(const GraphKey& k);
   void graph_release();
-  DevBuf<double> uhf_;  // Ja, Ka, Jb, Kb of a UHF build
+  DevBuf<double> uhf_;  // Ja, Ka, Jb, Kb of a UHF build (Jc, Kc, Jo, Ko of a ROHF build)
+  DevBuf<double> rohf_;  // ROHF MO-basis work matrices
+  void* rohf_blas_ = nullptr;  // rocBLAS handle of the ROHF transforms
+  void rohf_release();
 
   // profiling: events around each launch of a build
   struct ProfRow { double id, bytes, flops, units; hipEvent_t a, b; };

@@ -563,6 +563,40 @@ __global__ void k_uhf_combine(const double* __restrict__ H, const double* __rest
   Fb[id] = hj - Kb[id];
 }
 
+// ROHF (gpu_kernels.cu:351-418): F_c = H + J[D_c] + J[D_o] - (K[D_c] + K[D_o])/2,
+// F_o = (H + J[D_c] + J[D_o] - K[D_c]/2 - K[D_o])/2.
+__global__ void k_rohf_combine(const double* __restrict__ H, const double* __restrict__ Jc,
+                               const double* __restrict__ Kc, const double* __restrict__ Jo,
+                               const double* __restrict__ Ko, size_t n, double* __restrict__ Fc,
+                               double* __restrict__ Fo) {
+  const size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= n) return;
+  const double hj = H[id] + Jc[id] + Jo[id];
+  Fc[id] = hj - 0.5 * (Kc[id] + Ko[id]);
+  Fo[id] = 0.5 * (hj - 0.5 * Kc[id] - Ko[id]);
+}
+
+// Unified ROHF Fock matrix in the MO basis (gpu_kernels.cu:421-497): orbital
+// blocks closed (i < nc), open (nc <= i < nc + no), virtual; element (i, j)
+// takes the value of the upper element (min, max) of the MO matrices.
+__global__ void k_rohf_unified(const double* __restrict__ Fc, const double* __restrict__ Fo, RohfCoupling cp,
+                               int nc, int no, int M, double* __restrict__ R) {
+  const size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= (size_t)M * M) return;
+  const int r = (int)(id / M), c = (int)(id % M);
+  const int i = min(r, c), j = max(r, c);
+  const int ti = i < nc ? 0 : (i < nc + no ? 1 : 2), tj = j < nc ? 0 : (j < nc + no ? 1 : 2);
+  const double fc = Fc[(size_t)i * M + j], fo = Fo[(size_t)i * M + j];
+  double d;
+  if (ti == 0 && tj == 0) d = 2.0 * (cp.Acc * fo + cp.Bcc * (fc - fo));
+  else if (ti == 0 && tj == 1) d = 2.0 * (fc - fo);
+  else if (ti == 0) d = fc;
+  else if (ti == 1 && tj == 1) d = 2.0 * (cp.Aoo * fo + cp.Boo * (fc - fo));
+  else if (ti == 1) d = 2.0 * fo;
+  else d = 2.0 * (cp.Avv * fo + cp.Bvv * (fc - fo));
+  R[id] = d;
+}
+
 __global__ void k_finalize(const double* __restrict__ Jr, const double* __restrict__ Kr, const double* __restrict__ w,
                            int M, const double* __restrict__ H, double* J, double* K, double* F) {
   const size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1123,6 +1157,20 @@ hipError_t launch_zero(double* p, size_t n, hipStream_t s) {
 hipError_t launch_uhf_combine(const double* H, const double* Ja, const double* Ka, const double* Jb, const double* Kb,
                               size_t n, double* Fa, double* Fb, hipStream_t s) {
   hipLaunchKernelGGL(k_uhf_combine, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, H, Ja, Ka, Jb, Kb, n, Fa, Fb);
+  return hipGetLastError();
+}
+
+hipError_t launch_rohf_combine(const double* H, const double* Jc, const double* Kc, const double* Jo, const double* Ko,
+                               size_t n, double* Fc, double* Fo, hipStream_t s) {
+  hipLaunchKernelGGL(k_rohf_combine, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, H, Jc, Kc, Jo, Ko, n, Fc, Fo);
+  return hipGetLastError();
+}
+
+hipError_t launch_rohf_unified(const double* Fc_mo, const double* Fo_mo, const RohfCoupling& cp, int nc, int no, int M,
+                               double* R, hipStream_t s) {
+  const size_t n = (size_t)M * M;
+  hipLaunchKernelGGL(k_rohf_unified, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, Fc_mo, Fo_mo, cp, nc, no, M,
+                     R);
   return hipGetLastError();
 }
 

@@ -111,6 +111,14 @@ hipError_t launch_scale_density(const double* D, const double* w, int M, double*
 hipError_t launch_zero(double* p, size_t n, hipStream_t s);
 hipError_t launch_fill(double* p, size_t n, double v, hipStream_t s);
 // J = w w sym(Jr), K = w w sym(Kr); if F: F = H + J - K/2 (J, K may be null then)
+// ROHF coupling coefficients of the unified Fock matrix (types.hpp:165-173).
+struct RohfCoupling {
+  double Acc, Bcc, Aoo, Boo, Avv, Bvv;
+};
+hipError_t launch_rohf_combine(const double* H, const double* Jc, const double* Kc, const double* Jo, const double* Ko,
+                               size_t n, double* Fc, double* Fo, hipStream_t s);
+hipError_t launch_rohf_unified(const double* Fc_mo, const double* Fo_mo, const RohfCoupling& cp, int nc, int no, int M,
+                               double* R, hipStream_t s);
 hipError_t launch_uhf_combine(const double* H, const double* Ja, const double* Ka, const double* Jb, const double* Kb,
                               size_t n, double* Fa, double* Fb, hipStream_t s);
 hipError_t launch_finalize(const double* Jr, const double* Kr, const double* w, int M, const double* H, double* J,

@@ -118,6 +118,19 @@ class FockEngine:
         check(lib().gansu_fock_build_uhf(self._h, ptr(Da), ptr(Db), ptr(H), ptr(Fa), ptr(Fb), self._stream(stream)))
         return Fa, Fb
 
+    def compute_fock_matrix_rohf(self, Dc, Do, H, C, S, num_closed, num_open, coupling="roothaan", stream=None):
+        """ROHF: returns (Fc, Fo, F) with F_c = H + J[Dc + Do] - (K[Dc] + K[Do])/2,
+        F_o = (H + J[Dc + Do] - K[Dc]/2 - K[Do])/2 and the unified AO Fock matrix
+        F = S C R C^T S (C: MOs in columns, num_closed closed then num_open open)."""
+        import torch
+        for t, n in ((Dc, "Dc"), (Do, "Do"), (H, "H"), (C, "C"), (S, "S")):
+            self._check_mat(t, n)
+        Fc, Fo, F = (torch.empty_like(Dc) for _ in range(3))
+        check(lib().gansu_fock_build_rohf(self._h, ptr(Dc), ptr(Do), ptr(H), ptr(C), ptr(S),
+                                          coupling.encode() if coupling else None, int(num_closed), int(num_open),
+                                          ptr(Fc), ptr(Fo), ptr(F), self._stream(stream)))
+        return Fc, Fo, F
+
     def update_geometry(self, xyz_bohr):
         """Move the atoms (n_atoms x 3, bohr); a precomputed engine precomputes again."""
         xyz = np.ascontiguousarray(xyz_bohr, dtype=np.float64).reshape(-1)

@@ -134,6 +134,22 @@ int gansu_fock_build_rhf_occ(gansu_fock_engine* e, const double* d_D, const doub
  * with D_a, D_b the alpha and beta densities (C_occ C_occ^T each).          */
 int gansu_fock_build_uhf(gansu_fock_engine* e, const double* d_Da, const double* d_Db, const double* d_H, double* d_Fa,
                          double* d_Fb, void* stream);
+/* ROHF build == ERI::compute_fock_matrix for ROHF (rohf.hpp:660-760:
+ * computeFockMatrix_ROHF gpu_manager.cu:733-817, computeFockMatrix_RI_ROHF
+ * :1800-1961).  d_Dc, d_Do: closed- and open-shell densities; d_C: the MO
+ * coefficients (M x M row-major, MOs in columns: num_closed closed, then
+ * num_open open); d_S: overlap.  Outputs
+ *   F_c = H + J[Dc + Do] - (K[Dc] + K[Do])/2,
+ *   F_o = (H + J[Dc + Do] - K[Dc]/2 - K[Do])/2,
+ *   F   = S C R C^T S,
+ * R the unified MO-basis Fock matrix built blockwise from C^T F_c C and
+ * C^T F_o C with the coupling set `coupling` (rohf_parameter_name: roothaan
+ * when NULL, mcweeny-diercksen, davidson, guest-saunders,
+ * binkley-pople-dobosh, faegri-manne, goddard,
+ * plakhutin-gorelik-breslavskaya; GANSU_E_PARAM for another name).        */
+int gansu_fock_build_rohf(gansu_fock_engine* e, const double* d_Dc, const double* d_Do, const double* d_H,
+                          const double* d_C, const double* d_S, const char* coupling, int num_closed, int num_open,
+                          double* d_Fc, double* d_Fo, double* d_F, void* stream);
 /* Moves every primitive to the position of its atom (xyz_bohr[3*atom_index])
  * and, if the engine was precomputed, precomputes again (the reference's
  * HF::update_geometry, hf.cu:460-472, rebuilds its ERI object).             */

@@ -216,6 +216,54 @@ def ri_B(L, T):
     return T
 
 
+# ROHF coupling sets (Acc, Bcc, Aoo, Boo, Avv, Bvv) by rohf_parameter_name
+# (rohf.cu:30-39).
+ROHF_COUPLING = {
+    "roothaan": (-0.5, 1.5, 0.5, 0.5, 1.5, -0.5),
+    "mcweeny-diercksen": (1 / 3, 2 / 3, 1 / 3, 1 / 3, 2 / 3, 1 / 3),
+    "davidson": (0.5, 1.0, 0.0, 1.0, 1.0, 0.0),
+    "guest-saunders": (0.5, 0.5, 0.5, 0.5, 0.5, 0.5),
+    "binkley-pople-dobosh": (0.5, 0.5, 1.0, 0.0, 0.0, 1.0),
+    "faegri-manne": (0.5, 0.5, 1.0, 0.0, 0.5, 0.5),
+    "goddard": (0.5, 0.5, 0.5, 0.0, 1.0, 0.0),
+    "plakhutin-gorelik-breslavskaya": (0.0, 1.0, 1.0, 0.0, 1.0, 0.0),
+}
+
+
+def rohf_fock(H, JKc, JKo, C, S, nc, no, coupling="roothaan"):
+    """ROHF Fock matrices from the J/K of the closed and open densities:
+    F_c, F_o (gpu_kernels.cu:351-418), the unified MO-basis matrix built per
+    orbital block from C^T F_c C and C^T F_o C (gpu_kernels.cu:421-497, the
+    upper element serving both halves) and F = S C R C^T S
+    (gpu_manager.cu:733-817).  C holds the MOs in its columns."""
+    (Jc, Kc), (Jo, Ko) = JKc, JKo
+    Fc = H + Jc + Jo - 0.5 * (Kc + Ko)
+    Fo = 0.5 * (H + Jc + Jo - 0.5 * Kc - Ko)
+    A, B = C.T @ Fc @ C, C.T @ Fo @ C
+    Acc, Bcc, Aoo, Boo, Avv, Bvv = ROHF_COUPLING[coupling]
+    M = len(H)
+    kind = [0 if i < nc else (1 if i < nc + no else 2) for i in range(M)]
+    R = np.empty((M, M))
+    for i in range(M):
+        for j in range(i, M):
+            fc, fo = A[i, j], B[i, j]
+            blk = (kind[i], kind[j])
+            if blk == (0, 0):
+                d = 2.0 * (Acc * fo + Bcc * (fc - fo))
+            elif blk == (0, 1):
+                d = 2.0 * (fc - fo)
+            elif blk == (0, 2):
+                d = fc
+            elif blk == (1, 1):
+                d = 2.0 * (Aoo * fo + Boo * (fc - fo))
+            elif blk == (1, 2):
+                d = 2.0 * fo
+            else:
+                d = 2.0 * (Avv * fo + Bvv * (fc - fo))
+            R[i, j] = R[j, i] = d
+    return Fc, Fo, S @ C @ R @ C.T @ S
+
+
 def ri_jk(B, D):
     Ma, M = B.shape[0], B.shape[1]
     J, K = np.zeros((M, M)), np.zeros((M, M))

@@ -319,3 +319,42 @@ def test_update_geometry_equals_fresh_engine(method, cuda):
     Fr = fresh.compute_fock_matrix(D, H)
     assert float((F2 - Fr).abs().max()) <= 1e-12 * float(Fr.abs().max())
     assert float((F2 - F1).abs().max()) > 1e-6  # the geometry really moved
+
+
+@pytest.mark.parametrize("method", ["stored", "direct", "ri"])
+def test_rohf_fock_matches_oracle(method, cuda):
+    """ROHF build (gpu_manager.cu:733-817, RI :1800-1961): F_c, F_o and the
+    unified F = S C R C^T S against the oracle for two coupling sets, and the
+    closed-shell limit (no open shell, S-orthonormal C) equal to the engine's
+    own RHF Fock matrix."""
+    Z, xyz = methane_water()
+    mol, (p, t, n) = both(Z, xyz, "cc-pvdz")
+    M = mol.n_basis
+    S = O.oneint(p, n, Z, xyz)[0]
+    w, V = np.linalg.eigh(S)
+    C = V @ np.diag(w ** -0.5) @ V.T @ np.linalg.qr(np.random.default_rng(41).normal(size=(M, M)))[0]
+    Dc, Do, H = random_density(M, seed=34), 0.3 * random_density(M, seed=35), random_density(M, seed=36)
+    if method == "ri":
+        aux, (pa, ta, na) = both(Z, xyz, "cc-pvdz-rifit")
+        B = O.ri_tensor(p, t, n, pa, na, THR)
+        JKc, JKo = O.ri_jk(B, Dc), O.ri_jk(B, Do)
+        eng = FockEngine(mol, aux=aux, eri_method="ri")
+    else:
+        Q = O.schwarz(p, t)
+        JKc, JKo = O.jk(p, t, n, Q, Dc, THR), O.jk(p, t, n, Q, Do, THR)
+        eng = FockEngine(mol, eri_method=method)
+    eng.precomputation()
+    args = [_torch(x, cuda) for x in (Dc, Do, H, C, S)]
+    nc, no = mol.n_electrons // 2 - 1, 2
+    for coupling in ("roothaan", "guest-saunders"):
+        Fc, Fo, F = (x.cpu().numpy() for x in eng.compute_fock_matrix_rohf(*args, nc, no, coupling))
+        Fc_ref, Fo_ref, F_ref = O.rohf_fock(H, JKc, JKo, C, S, nc, no, coupling)
+        for got, ref in ((Fc, Fc_ref), (Fo, Fo_ref), (F, F_ref)):
+            assert np.abs(got - ref).max() <= 1e-10 * max(1.0, np.abs(ref).max()), coupling
+    # closed-shell limit: F = F_RHF of the same engine
+    zero = _torch(np.zeros((M, M)), cuda)
+    F_rhf = eng.compute_fock_matrix(args[0], args[2]).cpu().numpy()
+    F = eng.compute_fock_matrix_rohf(args[0], zero, args[2], args[3], args[4], mol.n_electrons // 2, 0)[2]
+    assert np.abs(F.cpu().numpy() - F_rhf).max() <= 1e-9 * np.abs(F_rhf).max()
+    with pytest.raises(RuntimeError):
+        eng.compute_fock_matrix_rohf(*args, nc, no, "no-such-set")

@@ -255,3 +255,71 @@ def test_ri_B_reproduces_fitted_integrals():
     J, K = O.jk(p, t, n, Q, D, 0.0)
     assert np.abs(Jri - J).max() < 2e-3 * np.abs(J).max()
     assert np.abs(Kri - K).max() < 2e-2 * np.abs(K).max()
+
+
+# ------------------------------------------------------------------ ROHF ----
+def _s_orthonormal(S, seed):
+    """Columns orthonormal in the S metric: C^T S C = 1, hence C C^T = S^-1."""
+    w, V = np.linalg.eigh(S)
+    X = V @ np.diag(w ** -0.5) @ V.T
+    Q, _ = np.linalg.qr(np.random.default_rng(seed).normal(size=S.shape))
+    return X @ Q
+
+
+def test_rohf_closed_shell_limit_is_rhf(water_ccpvdz):
+    """No open shell: F_o = F_c / 2, so the unified matrix is (Acc + Bcc) A in
+    the closed block, A between closed and virtual and (Avv + Bvv) A in the
+    virtual block, A = C^T F_RHF C; for S-orthonormal C (C C^T = S^-1) the
+    sets whose sums are 1 (all but davidson, whose table entry has
+    Acc + Bcc = 3/2) return F = F_RHF -- a transposed C breaks it."""
+    Z, xyz, p, t, n, _ = water_ccpvdz
+    S, T, V = O.oneint(p, n, Z, xyz)
+    H = T + V
+    D = random_density(len(n))
+    JK = O.jk(p, t, n, O.schwarz(p, t), D, 0.0)
+    C = _s_orthonormal(S, 3)
+    assert np.abs(C.T @ S @ C - np.eye(len(n))).max() < 1e-10
+    F_rhf = H + JK[0] - 0.5 * JK[1]
+    zero = (np.zeros_like(D), np.zeros_like(D))
+    A = C.T @ F_rhf @ C
+    nc = 5
+    c, v = slice(0, nc), slice(nc, len(n))
+    for name, (Acc, Bcc, Aoo, Boo, Avv, Bvv) in O.ROHF_COUPLING.items():
+        Fc, Fo, F = O.rohf_fock(H, JK, zero, C, S, nc, 0, name)
+        np.testing.assert_allclose(Fc, F_rhf, atol=1e-12)
+        np.testing.assert_allclose(Fo, 0.5 * F_rhf, atol=1e-12)
+        R = C.T @ F @ C
+        tol = 1e-8 * np.abs(A).max()
+        assert np.abs(R[c, c] - (Acc + Bcc) * A[c, c]).max() < tol, name
+        assert np.abs(R[c, v] - A[c, v]).max() < tol, name
+        assert np.abs(R[v, v] - (Avv + Bvv) * A[v, v]).max() < tol, name
+        if abs(Acc + Bcc - 1) < 1e-12 and abs(Avv + Bvv - 1) < 1e-12:
+            assert np.abs(F - F_rhf).max() < 1e-8 * np.abs(F_rhf).max(), name
+
+
+def test_rohf_couplings_only_change_the_diagonal_blocks(water_ccpvdz):
+    """The coupling set enters the closed-closed, open-open and virtual-virtual
+    blocks only: R = C^T F C (S-orthonormal C) agrees between sets in the
+    closed-open, closed-virtual and open-virtual blocks and differs in the
+    diagonal ones; R is symmetric."""
+    Z, xyz, p, t, n, _ = water_ccpvdz
+    S, T, V = O.oneint(p, n, Z, xyz)
+    H = T + V
+    Q = O.schwarz(p, t)
+    Dc, Do = random_density(len(n), seed=5), 0.3 * random_density(len(n), seed=6)
+    JKc, JKo = O.jk(p, t, n, Q, Dc, 0.0), O.jk(p, t, n, Q, Do, 0.0)
+    C = _s_orthonormal(S, 4)
+    nc, no = 4, 2
+    Rs = {}
+    for name in O.ROHF_COUPLING:
+        F = O.rohf_fock(H, JKc, JKo, C, S, nc, no, name)[2]
+        R = C.T @ F @ C
+        assert np.abs(R - R.T).max() < 1e-9 * np.abs(R).max()
+        Rs[name] = R
+    c, o, v = slice(0, nc), slice(nc, nc + no), slice(nc + no, len(n))
+    ref = Rs["roothaan"]
+    for name, R in Rs.items():
+        for a, b in ((c, o), (c, v), (o, v)):
+            assert np.abs(R[a, b] - ref[a, b]).max() < 1e-9 * np.abs(ref).max(), name
+    assert np.abs(Rs["guest-saunders"][c, c] - ref[c, c]).max() > 1e-3
+    assert np.abs(Rs["guest-saunders"][v, v] - ref[v, v]).max() > 1e-3
