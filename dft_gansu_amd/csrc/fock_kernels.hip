@@ -646,8 +646,19 @@ struct SlabShape {
   static constexpr int QPP = kSlabThreads / LPQ;      // quartets per pass
 };
 
+// Resident waves per SIMD the register allocation aims for in the stored
+// digestion, which is latency-bound (dependent integral and D loads per ket):
+// a third wave for (pp|ps) costs a 28-byte spill and cuts its time by 12 %
+// (measured); forcing more waves on the 200+-register classes spills and
+// loses (pp|pp) 15 %, (dd|dd) 75 %.
 template <int LA, int LB, int LC, int LD, int MODE>
-__global__ __launch_bounds__(kSlabThreads) void k_fock_slab(SlabLaunch c, EriArgs g) {
+constexpr int slab_waves_per_eu() {
+  return MODE == kModeStoredDigest && LA == 1 && LB == 1 && LC == 1 && LD == 0 ? 3 : 1;
+}
+
+template <int LA, int LB, int LC, int LD, int MODE>
+__global__ __launch_bounds__(kSlabThreads) __attribute__((amdgpu_waves_per_eu(slab_waves_per_eu<LA, LB, LC, LD, MODE>())))
+void k_fock_slab(SlabLaunch c, EriArgs g) {
   extern __shared__ double lds[];
   using S = SlabShape<LA, LB, LC, LD, MODE>;
   constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
