@@ -120,6 +120,28 @@ def ri_leg(log_fn):
             t = timed(lambda: e.compute_fock_matrix_occ(D, C, nocc, H), 3)
         out[name + "_build_s"] = round(t, 4)
         out[name + "_precompute_s"] = round(pre, 3)
+        if name != "ri_fp64_density":  # per-phase rates of one build from the engine's HIP-event rows
+            e.set_profiling(True)
+            e.compute_fock_matrix_occ(D, C, nocc, H)
+            rows = e.last_profile()
+            e.set_profiling(False)
+            phases = {}
+            for kid, label, unit in ((80005, "T=B*X dgemm", "TFLOP/s"), (80008, "K=T*T^T dgemm", "TFLOP/s"),
+                                     (80012, "J incremental (FP32 B)", "GB/s"), (80002, "J (FP64 B)", "GB/s")):
+                sel = rows[rows[:, 0] == kid]
+                if len(sel) == 0:
+                    continue
+                ms = float(sel[:, 1].sum())
+                if unit == "TFLOP/s":
+                    a = float(sel[:, 3].sum()) / (ms * 1e-3) / 1e12
+                    phases[label] = dict(ms=round(ms, 3), achieved=round(a, 2), peak=78.6, unit=unit,
+                                         frac=round(a / 78.6, 3))
+                else:
+                    a = float(sel[:, 2].sum()) / (ms * 1e-3) / 1e9
+                    phases[label] = dict(ms=round(ms, 3), achieved=round(a, 1), peak=HBM_PEAK_GBS, unit=unit,
+                                         frac=round(a / HBM_PEAK_GBS, 3))
+            out[name + "_phases"] = phases
+            log_fn(f"[bench] 64-water {name} phases: {phases}")
         if name == "ri_mixed":  # RI-JK energy gradient for the same density and orbitals
             g = [None]
 
