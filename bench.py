@@ -27,7 +27,7 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
 # (scaled x2.0 / x1.0 by a 2 GiB copy), keyed by (workload, kernel); the
 # commands and raw counts are in profiles/stored_digest_traffic_c10h22_ccpvdz.md.
 MEASURED_TRAFFIC = {
-    ("C10H22/cc-pVDZ", "fock_slab(pp|ps) stored-digest"): 307.6e6,
+    ("C10H22/cc-pVDZ", "fock_slab(pp|ps) stored-digest"): 336.8e6,
 }
 
 
