@@ -194,6 +194,15 @@ ScfResult rhf_run(const Molecule& mol, const Molecule* aux, const Params& prm, b
   const double damp = prm.getd("damping_factor");
 
   GANSU_HIP(hipSetDevice(0));
+  // GANSU_SCF_TIMING=1 prints the wall time of each phase (with a device
+  // synchronisation at every phase boundary) to stderr.
+  const char* tenv = std::getenv("GANSU_SCF_TIMING");
+  const bool timing = tenv && tenv[0] == '1';
+  auto wall = [&]() {
+    if (timing) GANSU_HIP(hipDeviceSynchronize());
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+  };
+  const double ts0 = wall();
   Work w(n);
   Blas blas;
   const size_t nn = (size_t)n * n;
@@ -210,17 +219,15 @@ ScfResult rhf_run(const Molecule& mol, const Molecule* aux, const Params& prm, b
   FockEngine eng(mol.prims.data(), mol.prims.size(), mol.types.data(), (int)mol.types.size(), mol.norm.data(), n,
                  ri ? aux->prims.data() : nullptr, ri ? aux->prims.size() : 0, ri ? aux->types.data() : nullptr,
                  ri ? (int)aux->types.size() : 0, ri ? aux->norm.data() : nullptr, ri ? aux->nbasis : 0, opts);
+  const double ts1 = wall();
   eng.precompute();
+  const double ts2 = wall();
 
   // X = U s^-1/2
   syevd(blas.h, n, w.S.ptr, w);
   GANSU_HIP(launch_transform(w.evec.ptr, w.eval.ptr, n, 1e-6, w.X.ptr, 0));
 
   ScfResult res;
-  // GANSU_SCF_TIMING=1 prints the wall time of each phase (with a device
-  // synchronisation at every phase boundary) to stderr.
-  const char* tenv = std::getenv("GANSU_SCF_TIMING");
-  const bool timing = tenv && tenv[0] == '1';
   double t_diag = 0, t_dens = 0, t_fock = 0, t_energy = 0, t_conv = 0;
   auto now = [&]() {
     if (timing) GANSU_HIP(hipDeviceSynchronize());
@@ -256,9 +263,14 @@ ScfResult rhf_run(const Molecule& mol, const Molecule* aux, const Params& prm, b
   };
   auto energy = [&]() { return 0.5 * (dot(w.D.ptr, w.H.ptr, nn, w.part) + dot(w.D.ptr, w.F.ptr, nn, w.part)); };
 
+  const double ts3 = wall();
   coefficients(w.H.ptr);
   density();
   fock();
+  if (timing)
+    std::fprintf(stderr, "[scf] setup: one-electron integrals + engine %.2f ms, precompute %.2f ms, X %.2f ms, "
+                         "core guess + first build %.2f ms\n", 1e3 * (ts1 - ts0), 1e3 * (ts2 - ts1),
+                 1e3 * (ts3 - ts2), 1e3 * (wall() - ts3));
 
   // convergence state
   DevBuf<double> errs, focks, prevF, prevD;
