@@ -24,6 +24,8 @@
 // workgroup.  This file holds the one-electron kernel and the dispatch.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "gradient_2e_impl.hpp"
 #include "gradient_kernels.hpp"
 #include "md_device.hpp"
@@ -44,20 +46,13 @@ struct DPrimG {  // == gansu_primitive_shell layout
 };
 
 constexpr int kG1MaxE = 4 * 6 * 9;  // (la+2)(lb+4)(la+lb+5), la, lb <= 2
-constexpr int kG1MaxEV = 4 * 4 * 7;  // (la+2)(lb+2)(la+lb+3): the attraction part
-constexpr int kG1NucGroup = 8;       // nuclei per pass of the attraction part
 // A primitive pair whose prefactor times the density weights falls below this
 // contributes nothing at FP64 precision to any component (the Gaussian
 // product factor exp(-ab/p r^2) makes most pairs of a large molecule vanish).
 constexpr double kG1Screen = 1e-17;
 
-// One lane = one basis-function pair mu >= nu; derivatives of S, T and V
-// with respect to both basis centres, and the Hellmann-Feynman term of every
-// nucleus (-(d/dA + d/dB) of that nucleus' attraction integral).  The nuclei
-// are taken kG1NucGroup at a time in a wave-uniform loop: each lane keeps that
-// group's Hellmann-Feynman sums over its primitive pairs in registers, and a
-// wave reduction leaves one LDS atomic per nucleus and wave (an atomic per lane
-// and nucleus serialised 64 lanes on one address).
+// Overlap and kinetic part: one lane = one basis-function pair mu >= nu;
+// derivatives of S and T with respect to both basis centres.
 __global__ __launch_bounds__(kGradThreads) void k_grad_1e(OneIntArgs a, const double* __restrict__ Dm,
                                                           const double* __restrict__ Wm, double* grad) {
   extern __shared__ double s_grad[];
@@ -140,61 +135,6 @@ __global__ __launch_bounds__(kGradThreads) void k_grad_1e(OneIntArgs a, const do
         }
       }
   }
-  // nuclear attraction, kG1NucGroup nuclei per wave-uniform pass
-  const int L = la + lb;
-  const int nj = lb + 2, ntt = la + lb + 3;
-  for (int c0 = 0; c0 < a.natoms; c0 += kG1NucGroup) {
-    double hf[kG1NucGroup][3];
-#pragma unroll
-    for (int cc = 0; cc < kG1NucGroup; ++cc) hf[cc][0] = hf[cc][1] = hf[cc][2] = 0.0;
-    for (int ia = pa0; ia < pa1; ++ia)
-      for (int ib = pb0; ib < pb1; ++ib) {
-        double P[9], al, be, p, pf;
-        prim_pair(ia, ib, P, al, be, p, pf);
-        const double vpf = 2.0 * kPi / p * pf * dd;
-        if (fabs(vpf) * a.zsum * (1.0 + p) * (1.0 + p) < kG1Screen) continue;
-        double E[3][kG1MaxEV];
-        for (int d = 0; d < 3; ++d) hermite_e_rt<double>(la + 1, lb + 1, p, P[d] - P[3 + d], P[d] - P[6 + d], E[d]);
-        auto row = [&](int d, int i, int j) -> const double* { return E[d] + (i * nj + j) * ntt; };
-#pragma unroll
-        for (int cc = 0; cc < kG1NucGroup; ++cc) {
-          const int c = c0 + cc;
-          if (c >= a.natoms) break;
-          double R[nherm(5)];
-          hermite_r_rt<double>(L + 1, p, P[0] - a.atom_xyz[3 * c], P[1] - a.atom_xyz[3 * c + 1],
-                               P[2] - a.atom_xyz[3 * c + 2], 1.0, a.boys, R);
-          const double vf = -a.Z[c] * vpf;
-          for (int k = 0; k < 2; ++k) {
-            const double ex = k == 0 ? al : be;
-            for (int d = 0; d < 3; ++d) {
-              int xa[3] = {ea[0], ea[1], ea[2]}, xb[3] = {eb[0], eb[1], eb[2]};
-              int* e = k == 0 ? xa : xb;
-              const int e0 = e[d];
-              e[d] = e0 + 1;
-              double v = 2.0 * ex *
-                         contract_e(row(0, xa[0], xb[0]), xa[0] + xb[0], row(1, xa[1], xb[1]), xa[1] + xb[1],
-                                    row(2, xa[2], xb[2]), xa[2] + xb[2], R);
-              if (e0 > 0) {
-                e[d] = e0 - 1;
-                v -= e0 * contract_e(row(0, xa[0], xb[0]), xa[0] + xb[0], row(1, xa[1], xb[1]), xa[1] + xb[1],
-                                     row(2, xa[2], xb[2]), xa[2] + xb[2], R);
-              }
-              v *= vf;
-              if (k == 0) gA[d] += v; else gB[d] += v;
-              hf[cc][d] -= v;
-            }
-          }
-        }
-      }
-    // every lane of the wave reaches this point (the loop above is uniform)
-#pragma unroll
-    for (int cc = 0; cc < kG1NucGroup; ++cc)
-#pragma unroll
-      for (int d = 0; d < 3; ++d) {
-        const double v = wave_sum_g(hf[cc][d]);
-        if ((threadIdx.x & 63) == 0 && c0 + cc < a.natoms && v != 0.0) atomicAdd(&s_grad[3 * (c0 + cc) + d], v);
-      }
-  }
   if (act)
     for (int d = 0; d < 3; ++d) {
       atomicAdd(&s_grad[3 * atA + d], gA[d]);
@@ -205,7 +145,181 @@ __global__ __launch_bounds__(kGradThreads) void k_grad_1e(OneIntArgs a, const do
     if (s_grad[i] != 0.0) unsafeAtomicAdd(&grad[i], s_grad[i]);
 }
 
+// Nuclear-attraction part, one instance per shell-pair class (LA >= LB): a
+// wave takes one shell pair and 64 nuclei, one per lane.  Per primitive pair
+// (wave-uniform) three lanes build the E tables (one order higher) into the
+// wave's LDS slot; each lane evaluates R for its nucleus and contracts every
+// component pair with compile-time offsets.  The basis-centre derivatives go
+// to the pair's atoms after a wave reduction; the Hellmann-Feynman term
+// -(dA + dB) goes to the lane's own nucleus (distinct LDS addresses).
+constexpr int kG1Waves = kGradThreads / 64;
+
+template <int LA, int LB>
+__global__ __launch_bounds__(kGradThreads) void k_grad_1e_nuc(OneIntArgs a, const int2* __restrict__ pairs,
+                                                              int npairs, const double* __restrict__ Dm,
+                                                              double* grad) {
+  constexpr int NB = ncart(LB), NAB = ncart(LA) * NB, LAB = LA + LB;
+  constexpr int NE = (LA + 2) * (LB + 2) * (LAB + 3);
+  extern __shared__ double smem[];
+  const int n3 = 3 * a.natoms;
+  double* s_grad = smem;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  double* s_E = smem + n3 + wave * 3 * NE;
+  for (int i = threadIdx.x; i < n3; i += blockDim.x) s_grad[i] = 0.0;
+  __syncthreads();
+  const int ngroups = (a.natoms + 63) >> 6;
+  const long long nvirt = (long long)npairs * ngroups;
+  const int M = a.M;
+  const DPrimG* prims = (const DPrimG*)a.prims;
+  for (long long vb = (long long)blockIdx.x * kG1Waves + wave; vb < nvirt; vb += (long long)gridDim.x * kG1Waves) {
+    const int2 sp = pairs[vb / ngroups];
+    const int c = (int)(vb % ngroups) * 64 + lane;
+    const bool act = c < a.natoms;
+    const int s1 = sp.x, s2 = sp.y;
+    const int b1 = a.shell_basis[s1], b2 = a.shell_basis[s2];
+    const double f = s1 == s2 ? 1.0 : 2.0;
+    double dd[NAB], ddmax = 0.0;
+#pragma unroll
+    for (int ab = 0; ab < NAB; ++ab) {
+      const int mu = b1 + ab / NB, nu = b2 + ab % NB;
+      dd[ab] = Dm[(size_t)mu * M + nu] * a.w[mu] * a.w[nu] * f;
+      ddmax = fmax(ddmax, fabs(dd[ab]));
+    }
+    const double Cx = act ? a.atom_xyz[3 * c] : 0.0, Cy = act ? a.atom_xyz[3 * c + 1] : 0.0,
+                 Cz = act ? a.atom_xyz[3 * c + 2] : 0.0;
+    const double Zc = act ? (double)a.Z[c] : 0.0;
+    double gA[3] = {0.0, 0.0, 0.0}, gB[3] = {0.0, 0.0, 0.0}, hf[3] = {0.0, 0.0, 0.0};
+    int atA = 0, atB = 0;
+    for (int ia = a.shell_off[s1]; ia < a.shell_off[s1 + 1]; ++ia)
+      for (int ib = a.shell_off[s2]; ib < a.shell_off[s2 + 1]; ++ib) {  // wave-uniform
+        const DPrimG pa = prims[a.shell_prims[ia]], pb = prims[a.shell_prims[ib]];
+        atA = pa.atom_index;
+        atB = pb.atom_index;
+        const double al = pa.exponent, be = pb.exponent, p = al + be;
+        const double A[3] = {pa.x, pa.y, pa.z}, B[3] = {pb.x, pb.y, pb.z};
+        double P[3], r2 = 0.0;
+        for (int d = 0; d < 3; ++d) {
+          P[d] = (al * A[d] + be * B[d]) / p;
+          r2 += (A[d] - B[d]) * (A[d] - B[d]);
+        }
+        const double na = 0.71270547035499016035 * double(1 << LA) * pow(al, (2.0 * LA + 3.0) / 4.0);
+        const double nb = 0.71270547035499016035 * double(1 << LB) * pow(be, (2.0 * LB + 3.0) / 4.0);
+        const double vpf = 2.0 * kPi / p * pa.coefficient * pb.coefficient * na * nb * exp(-al * be / p * r2);
+        if (fabs(vpf) * ddmax * a.zsum * (1.0 + p) * (1.0 + p) < kG1Screen) continue;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        if (lane < 3) {
+          double E[LA + 2][LB + 2][LAB + 3];
+          const double Pd = lane == 0 ? P[0] : lane == 1 ? P[1] : P[2];
+          const double Ad = lane == 0 ? A[0] : lane == 1 ? A[1] : A[2];
+          const double Bd = lane == 0 ? B[0] : lane == 1 ? B[1] : B[2];
+          hermite_e_t<LA + 1, LB + 1, double>(p, Pd - Ad, Pd - Bd, E);
+          double* dst = s_E + lane * NE;
+#pragma unroll
+          for (int i = 0; i < LA + 2; ++i)
+#pragma unroll
+            for (int j = 0; j < LB + 2; ++j)
+#pragma unroll
+              for (int t = 0; t < LAB + 3; ++t) dst[(i * (LB + 2) + j) * (LAB + 3) + t] = E[i][j][t];
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        if (!act) continue;
+        const auto& E = *reinterpret_cast<const double(*)[3][LA + 2][LB + 2][LAB + 3]>(s_E);
+        double R[nherm(LAB + 1)];
+        hermite_r_t<LAB + 1, double>(p, P[0] - Cx, P[1] - Cy, P[2] - Cz, 1.0, a.boys, R);
+        const double vf = -Zc * vpf;
+        // sum_tuv E_x[i0][j0][t] E_y[i1][j1][u] E_z[i2][j2][v] R[t,u,v]
+        auto V = [&](int i0, int j0, int i1, int j1, int i2, int j2) {
+          double sum = 0.0;
+#pragma unroll
+          for (int t = 0; t <= LAB + 1; ++t) {
+            if (t > i0 + j0) continue;
+#pragma unroll
+            for (int u = 0; u <= LAB + 1; ++u) {
+              if (u > i1 + j1) continue;
+              const double exy = E[0][i0][j0][t] * E[1][i1][j1][u];
+#pragma unroll
+              for (int v = 0; v <= LAB + 1; ++v) {
+                if (v > i2 + j2) continue;
+                sum += exy * E[2][i2][j2][v] * R[hidx(t, u, v)];
+              }
+            }
+          }
+          return sum;
+        };
+#pragma unroll
+        for (int ab = 0; ab < NAB; ++ab) {
+          const int e[6] = {cart_exp(LA, ab / NB, 0), cart_exp(LB, ab % NB, 0), cart_exp(LA, ab / NB, 1),
+                            cart_exp(LB, ab % NB, 1), cart_exp(LA, ab / NB, 2), cart_exp(LB, ab % NB, 2)};
+          const double w = vf * dd[ab];
+#pragma unroll
+          for (int d = 0; d < 3; ++d) {
+            int x[6] = {e[0], e[1], e[2], e[3], e[4], e[5]};
+            x[2 * d] = e[2 * d] + 1;
+            double vA = 2.0 * al * V(x[0], x[1], x[2], x[3], x[4], x[5]);
+            if (e[2 * d] > 0) {
+              x[2 * d] = e[2 * d] - 1;
+              vA -= e[2 * d] * V(x[0], x[1], x[2], x[3], x[4], x[5]);
+            }
+            x[2 * d] = e[2 * d];
+            x[2 * d + 1] = e[2 * d + 1] + 1;
+            double vB = 2.0 * be * V(x[0], x[1], x[2], x[3], x[4], x[5]);
+            if (e[2 * d + 1] > 0) {
+              x[2 * d + 1] = e[2 * d + 1] - 1;
+              vB -= e[2 * d + 1] * V(x[0], x[1], x[2], x[3], x[4], x[5]);
+            }
+            gA[d] += w * vA;
+            gB[d] += w * vB;
+            hf[d] -= w * (vA + vB);
+          }
+        }
+      }
+    if (act)
+#pragma unroll
+      for (int d = 0; d < 3; ++d)
+        if (hf[d] != 0.0) atomicAdd(&s_grad[3 * c + d], hf[d]);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const double sa = wave_sum_g(gA[d]), sb = wave_sum_g(gB[d]);  // every lane reaches this point
+      if (lane == 0 && (sa != 0.0 || sb != 0.0)) {
+        atomicAdd(&s_grad[3 * atA + d], sa);
+        atomicAdd(&s_grad[3 * atB + d], sb);
+      }
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < n3; i += blockDim.x)
+    if (s_grad[i] != 0.0) unsafeAtomicAdd(&grad[i], s_grad[i]);
+}
+
+template <int LA, int LB>
+hipError_t launch_grad_1e_nuc_t(const OneIntArgs& a, const int2* pairs, int npairs, const double* D, double* grad,
+                                hipStream_t s) {
+  const long long waves = (long long)npairs * ((a.natoms + 63) / 64);
+  const unsigned grid = (unsigned)std::min<long long>(8192, (waves + kG1Waves - 1) / kG1Waves);
+  const size_t lds = (3 * (size_t)a.natoms + (size_t)kG1Waves * 3 * (LA + 2) * (LB + 2) * (LA + LB + 3)) *
+                     sizeof(double);
+  hipLaunchKernelGGL((k_grad_1e_nuc<LA, LB>), dim3(grid), dim3(kGradThreads), lds, s, a, pairs, npairs, D, grad);
+  return hipGetLastError();
+}
+
 }  // namespace
+
+hipError_t launch_grad_1e_nuc(int la, int lb, const OneIntArgs& a, const int2* pairs, int npairs, const double* D,
+                              double* grad, hipStream_t s) {
+  if (a.natoms <= 0 || a.natoms > kGradMaxAtoms) return hipErrorInvalidValue;
+  if (npairs <= 0) return hipSuccess;
+  switch (la * 3 + lb) {
+    case 0: return launch_grad_1e_nuc_t<0, 0>(a, pairs, npairs, D, grad, s);
+    case 3: return launch_grad_1e_nuc_t<1, 0>(a, pairs, npairs, D, grad, s);
+    case 4: return launch_grad_1e_nuc_t<1, 1>(a, pairs, npairs, D, grad, s);
+    case 6: return launch_grad_1e_nuc_t<2, 0>(a, pairs, npairs, D, grad, s);
+    case 7: return launch_grad_1e_nuc_t<2, 1>(a, pairs, npairs, D, grad, s);
+    case 8: return launch_grad_1e_nuc_t<2, 2>(a, pairs, npairs, D, grad, s);
+    default: return hipErrorInvalidValue;
+  }
+}
 
 bool grad_2e_dispatch_0(int, const ClassLaunch&, const Grad2eArgs&, int, int, hipStream_t, hipError_t*);
 bool grad_2e_dispatch_1(int, const ClassLaunch&, const Grad2eArgs&, int, int, hipStream_t, hipError_t*);

@@ -38,8 +38,14 @@ hipError_t launch_grad_2e(const ClassLaunch& c, const Grad2eArgs& a, int max_bra
 // rather than the register-resident unrolled one.
 bool grad_2e_uses_scratch(int LA, int LB, int LC, int LD);
 
-// One-electron part: grad += sum D d(T+V)/dX - sum W dS/dX, both matrices in
-// the caller's basis (the kernel applies the raw-basis factors w).
+// One-electron part, grad += sum D d(T+V)/dX - sum W dS/dX, both matrices in
+// the caller's basis (the kernels apply the raw-basis factors w):
+// launch_grad_1e adds the overlap and kinetic terms, launch_grad_1e_nuc the
+// nuclear-attraction terms (basis centres and Hellmann-Feynman) of the shell
+// pairs pairs[0..npairs) = {s1, s2} of class (la, lb), la >= lb, each
+// unordered pair once.
 hipError_t launch_grad_1e(const OneIntArgs& a, const double* D, const double* W, double* grad, hipStream_t s);
+hipError_t launch_grad_1e_nuc(int la, int lb, const OneIntArgs& a, const int2* pairs, int npairs, const double* D,
+                              double* grad, hipStream_t s);
 
 }  // namespace gansu

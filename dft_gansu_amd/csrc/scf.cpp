@@ -135,6 +135,7 @@ OneIntHost::OneIntHost(const Molecule& mol) : M(mol.nbasis), natoms((int)mol.ato
     z.push_back(a.Z);
     zsum += std::abs((double)a.Z);
   }
+  shell_L = l;
   prims.upload(mol.prims);
   off.upload(o); sprims.upload(sp); sl.upload(l); sb.upload(bs); bfs.upload(bf); Z.upload(z);
   w.upload(wv); xyz.upload(x);
@@ -158,7 +159,26 @@ void compute_oneint_gradient(const Molecule& mol, const double* dD, const double
   if (h.natoms > kGradMaxAtoms)
     throw Error(GANSU_E_UNSUPPORTED, "gradient limited to " + std::to_string(kGradMaxAtoms) + " atoms");
   GANSU_HIP(launch_zero(dgrad, 3 * (size_t)h.natoms, s));
-  GANSU_HIP(launch_grad_1e(h.args(nullptr, nullptr, nullptr), dD, dW, dgrad, s));
+  const OneIntArgs args = h.args(nullptr, nullptr, nullptr);
+  GANSU_HIP(launch_grad_1e(args, dD, dW, dgrad, s));
+  // attraction terms by shell-pair class: every unordered pair once, the
+  // shell of higher angular momentum first
+  const int ns = (int)h.shell_L.size();
+  std::vector<int2> pairs;
+  int range[kMaxL + 1][kMaxL + 1][2] = {};
+  for (int la = 0; la <= kMaxL; ++la)
+    for (int lb = 0; lb <= la; ++lb) {
+      range[la][lb][0] = (int)pairs.size();
+      for (int s1 = 0; s1 < ns; ++s1)
+        for (int s2 = 0; s2 < ns; ++s2)
+          if (h.shell_L[s1] == la && h.shell_L[s2] == lb && (la > lb || s1 >= s2)) pairs.push_back(make_int2(s1, s2));
+      range[la][lb][1] = (int)pairs.size() - range[la][lb][0];
+    }
+  DevBuf<int2> dpairs;
+  dpairs.upload(pairs);
+  for (int la = 0; la <= kMaxL; ++la)
+    for (int lb = 0; lb <= la; ++lb)
+      GANSU_HIP(launch_grad_1e_nuc(la, lb, args, dpairs.ptr + range[la][lb][0], range[la][lb][1], dD, dgrad, s));
   GANSU_HIP(hipStreamSynchronize(s));
 }
 

@@ -27,6 +27,7 @@ struct OneIntHost {
   DevBuf<double> w, xyz, boys;
   int M = 0, natoms = 0;
   double zsum = 0.0;
+  std::vector<int> shell_L;  // angular momentum of each contracted shell (host copy)
   explicit OneIntHost(const Molecule& mol);
   OneIntArgs args(double* S, double* T, double* V) const;
 };
