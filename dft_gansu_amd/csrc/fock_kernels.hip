@@ -1033,12 +1033,6 @@ hipError_t launch_eri_class(const ClassLaunch& c, const EriArgs& a, int mode, hi
     GANSU_CASE(2, 1, 0, 0)
     GANSU_CASE(2, 1, 1, 0)
     GANSU_CASE(2, 2, 0, 0)
-    GANSU_CASE(2, 1, 1, 1)
-    GANSU_CASE(2, 1, 2, 0)
-    GANSU_CASE(2, 2, 1, 0)
-    GANSU_CASE(2, 1, 2, 1)
-    GANSU_CASE(2, 2, 1, 1)
-    GANSU_CASE(2, 2, 2, 0)
     default:
       hipLaunchKernelGGL(k_eri_generic, dim3(c.nblocks), dim3(64), 0, s, c, a, mode);
       return hipGetLastError();
