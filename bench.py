@@ -383,6 +383,9 @@ def main():
         log(f"[bench] direct build rate {extra['direct_fock_builds_per_s']} /s")
         del e2
     if not args.no_scf:
+        # one small SCF first (water STO-3G), so that the timed SCFs below all
+        # run with rocBLAS / rocSOLVER already initialised in this process
+        rhf(gm.config_molecule(0)[0], Parameters(convergence_energy_threshold=1e-8))
         t0 = time.perf_counter()
         r = rhf(mol, Parameters(eri_method=args.method, precision=args.precision,
                                 convergence_energy_threshold=1e-8, maxiter=100), aux=aux)
