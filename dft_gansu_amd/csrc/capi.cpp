@@ -278,6 +278,19 @@ int gansu_molecule_nuclear_repulsion_gradient(const gansu_molecule* mol, double*
   });
 }
 
+int gansu_uhf_run(const gansu_molecule* mol, const gansu_molecule* aux_mol, const gansu_params* p, double* energies,
+                  double* Da, double* Db, int* info) {
+  return guarded([&] {
+    if (!mol || !p || !energies) throw gansu::Error(GANSU_E_ARG, "null argument");
+    auto r = gansu::uhf_run(mol->m, aux_mol ? &aux_mol->m : nullptr, p->p);
+    energies[0] = r.e_elec;
+    energies[1] = r.e_total;
+    if (Da) std::memcpy(Da, r.Da.data(), r.Da.size() * sizeof(double));
+    if (Db) std::memcpy(Db, r.Db.data(), r.Db.size() * sizeof(double));
+    if (info) { info[0] = r.iterations; info[1] = r.fock_builds; info[2] = r.converged; info[3] = 0; }
+  });
+}
+
 int gansu_rhf_gradient(const gansu_molecule* mol, const gansu_molecule* aux_mol, const gansu_params* p,
                        double* energies, double* grad, int* info) {
   return guarded([&] {

@@ -398,4 +398,142 @@ ScfResult rhf_run(const Molecule& mol, const Molecule* aux, const Params& prm, b
   return res;
 }
 
+// UHF (uhf.cu:118-300, uhf.hpp:291-400): core-Hamiltonian guess; per spin
+// C from X^T F X, D_s = C_s,occ C_s,occ^T (n_alpha / n_beta orbitals), the
+// Fock pair from the engine's UHF build, E = 1/2 sum_s D_s.(H + F_s),
+// convergence on |dE|, and DIIS run separately for each spin with the error
+// F_s D_s S - S D_s F_s.
+UhfResult uhf_run(const Molecule& mol, const Molecule* aux, const Params& prm) {
+  const int n = mol.nbasis, na = mol.n_alpha, nb = mol.n_beta;
+  if (na > n || nb > n)
+    throw Error(GANSU_E_ELECTRONS, "more alpha or beta electrons than basis functions");
+  if (prm.get("initial_guess") != "core")
+    throw Error(GANSU_E_UNSUPPORTED, "initial_guess=" + prm.get("initial_guess") + " is not provided (core only)");
+  if (prm.get("convergence_method") != "diis")
+    throw Error(GANSU_E_UNSUPPORTED, "UHF is provided with convergence_method=diis");
+  const int maxiter = prm.geti("maxiter");
+  const double conv = prm.getd("convergence_energy_threshold");
+  const int diis_size = prm.geti("diis_size");
+  const bool diis_x = prm.geti("diis_include_transform") != 0;
+
+  GANSU_HIP(hipSetDevice(0));
+  Work w(n);
+  Blas blas;
+  const size_t nn = (size_t)n * n;
+  compute_oneint(mol, w.S.ptr, w.T.ptr, w.V.ptr, 0);
+  GANSU_HIP(launch_axpby(1.0, w.T.ptr, 1.0, w.V.ptr, nn, w.H.ptr, 0));
+  std::map<std::string, std::string> opts{{"eri_method", prm.get("eri_method")},
+                                          {"schwarz_screening_threshold", prm.get("schwarz_screening_threshold")},
+                                          {"precision", prm.get("precision")},
+                                          {"fp32_bound_threshold", prm.get("fp32_bound_threshold")},
+                                          {"ri_k", prm.get("ri_k")}};
+  const bool ri = prm.get("eri_method") == "ri";
+  if (ri && !aux) throw Error(GANSU_E_ARG, "eri_method=ri needs auxiliary_gbsfilename");
+  FockEngine eng(mol.prims.data(), mol.prims.size(), mol.types.data(), (int)mol.types.size(), mol.norm.data(), n,
+                 ri ? aux->prims.data() : nullptr, ri ? aux->prims.size() : 0, ri ? aux->types.data() : nullptr,
+                 ri ? (int)aux->types.size() : 0, ri ? aux->norm.data() : nullptr, ri ? aux->nbasis : 0, opts);
+  eng.precompute();
+  syevd(blas.h, n, w.S.ptr, w);
+  GANSU_HIP(launch_transform(w.evec.ptr, w.eval.ptr, n, 1e-6, w.X.ptr, 0));
+
+  // spin 0 = alpha (the driver's F, D, C), spin 1 = beta
+  DevBuf<double> Fb(nn), Db(nn), Cb(nn);
+  double* F[2] = {w.F.ptr, Fb.ptr};
+  double* D[2] = {w.D.ptr, Db.ptr};
+  double* C[2] = {w.C.ptr, Cb.ptr};
+  const int nocc[2] = {na, nb};
+  auto coefficients = [&](const double* Fm, double* Cm) {
+    blas.gemm(true, false, n, w.X.ptr, Fm, w.tmp.ptr);
+    blas.gemm(false, false, n, w.tmp.ptr, w.X.ptr, w.tmp2.ptr);
+    syevd(blas.h, n, w.tmp2.ptr, w);
+    blas.gemm(false, true, n, w.X.ptr, w.evec.ptr, Cm);
+  };
+  auto density = [&](int sp) {
+    // D_s = C_s,occ C_s,occ^T (gpu_kernels.cu computeDensityMatrix_UHF_kernel)
+    if (nocc[sp] == 0) {
+      GANSU_HIP(launch_zero(D[sp], nn, 0));
+      return;
+    }
+    const double one = 1.0, zero = 0.0;
+    blas_check(rocblas_dgemm(blas.h, rocblas_operation_transpose, rocblas_operation_none, n, n, nocc[sp], &one,
+                             C[sp], n, C[sp], n, &zero, D[sp], n),
+               "rocblas_dgemm(D_s)");
+  };
+  UhfResult res;
+  auto fock = [&]() {
+    eng.build_uhf(D[0], D[1], w.H.ptr, F[0], F[1], 0);
+    ++res.fock_builds;
+  };
+  auto energy = [&]() {
+    double e = 0.0;
+    for (int sp = 0; sp < 2; ++sp) e += dot(D[sp], w.H.ptr, nn, w.part) + dot(D[sp], F[sp], nn, w.part);
+    return 0.5 * e;
+  };
+  for (int sp = 0; sp < 2; ++sp) {
+    coefficients(w.H.ptr, C[sp]);
+    density(sp);
+  }
+  fock();
+  DevBuf<double> errs[2], focks[2];
+  for (int sp = 0; sp < 2; ++sp) {
+    errs[sp].alloc(nn * diis_size);
+    focks[sp].alloc(nn * diis_size);
+  }
+  int diis_iter = 0;
+  double prev = std::numeric_limits<double>::max(), E = 0.0;
+  int it = 0;
+  while (it < maxiter) {
+    for (int sp = 0; sp < 2; ++sp) {
+      coefficients(F[sp], C[sp]);
+      density(sp);
+    }
+    fock();
+    E = energy();
+    if (std::fabs(E - prev) < conv) { res.converged = 1; break; }
+    ++it;
+    prev = E;
+    const int slot = diis_iter % diis_size;
+    const int np = std::min(diis_iter + 1, diis_size);
+    for (int sp = 0; sp < 2; ++sp) {
+      double* e = w.tmp2.ptr;
+      blas.gemm(false, false, n, F[sp], D[sp], w.tmp.ptr);
+      blas.gemm(false, false, n, w.tmp.ptr, w.S.ptr, e);
+      blas.gemm(false, false, n, w.S.ptr, D[sp], w.tmp.ptr);
+      blas.gemm(false, false, n, w.tmp.ptr, F[sp], e, -1.0, 1.0);
+      if (diis_x) {
+        blas.gemm(false, false, n, w.X.ptr, e, w.tmp.ptr);
+        blas.gemm(false, true, n, w.tmp.ptr, w.X.ptr, e);
+      }
+      GANSU_HIP(hipMemcpy(errs[sp].ptr + slot * nn, e, nn * sizeof(double), hipMemcpyDeviceToDevice));
+      GANSU_HIP(hipMemcpy(focks[sp].ptr + slot * nn, F[sp], nn * sizeof(double), hipMemcpyDeviceToDevice));
+      if (np > 1) {
+        const int ns = np + 1;
+        std::vector<double> B((size_t)ns * ns, 0.0), rhs(ns, 0.0);
+        for (int i = 0; i < np; ++i) {
+          for (int j = i; j < np; ++j)
+            B[i * ns + j] = B[j * ns + i] = dot(errs[sp].ptr + i * nn, errs[sp].ptr + j * nn, nn, w.part);
+          B[i * ns + np] = B[np * ns + i] = -1.0;
+        }
+        rhs[np] = -1.0;
+        const auto c = solve_small(ns, B, rhs);
+        std::vector<const double*> mats(np);
+        for (int i = 0; i < np; ++i) mats[i] = focks[sp].ptr + i * nn;
+        DevBuf<double> dc;
+        dc.upload(c.data(), np);
+        DevBuf<const double*> dm;
+        dm.upload(mats);
+        GANSU_HIP(launch_lincomb(np, dc.ptr, dm.ptr, nn, F[sp], 0));
+      }
+    }
+    ++diis_iter;
+  }
+  GANSU_HIP(hipDeviceSynchronize());
+  res.e_elec = E;
+  res.e_total = E + mol.nuclear_repulsion();
+  res.iterations = it;
+  res.Da = w.D.download();
+  res.Db = Db.download();
+  return res;
+}
+
 }  // namespace gansu

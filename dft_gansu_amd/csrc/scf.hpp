@@ -20,6 +20,14 @@ struct ScfResult {
 // analytic nuclear gradient of the converged energy follows.
 ScfResult rhf_run(const Molecule& mol, const Molecule* aux, const Params& prm, bool gradient = false);
 
+struct UhfResult {
+  double e_elec = 0, e_total = 0;
+  int iterations = 0, fock_builds = 0, converged = 0;
+  std::vector<double> Da, Db;  // final alpha and beta densities (M x M)
+};
+// HF::single_point_energy for UHF (uhf.cu), core guess and per-spin DIIS.
+UhfResult uhf_run(const Molecule& mol, const Molecule* aux, const Params& prm);
+
 // Device copies of a molecule's shell tables for the one-electron kernels.
 struct OneIntHost {
   DevBuf<gansu_primitive_shell> prims;

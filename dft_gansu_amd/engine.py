@@ -236,6 +236,18 @@ def rhf_gradient(mol, params, aux=None):
                 converged=bool(info[2]), grad=g)
 
 
+def uhf(mol, params, aux=None):
+    """Run the UHF SCF; returns dict(e_elec, e_total, iterations, fock_builds, converged, Da, Db)."""
+    en = np.zeros(2)
+    Da = np.zeros((mol.n_basis, mol.n_basis))
+    Db = np.zeros((mol.n_basis, mol.n_basis))
+    info = np.zeros(4, dtype=np.int32)
+    check(lib().gansu_uhf_run(mol.handle, aux.handle if aux else None, params.handle, ptr(en), ptr(Da), ptr(Db),
+                              ptr(info)))
+    return dict(e_elec=float(en[0]), e_total=float(en[1]), iterations=int(info[0]), fock_builds=int(info[1]),
+                converged=bool(info[2]), Da=Da, Db=Db)
+
+
 def rhf(mol, params, aux=None):
     """Run the RHF SCF; returns dict(e_elec, e_total, iterations, fock_builds, converged, D)."""
     en = np.zeros(2)
@@ -246,5 +258,5 @@ def rhf(mol, params, aux=None):
                 converged=bool(info[2]), D=D)
 
 
-__all__ = ["FockEngine", "Parameters", "oneint", "rhf", "oneint_gradient", "nuclear_repulsion_gradient",
+__all__ = ["FockEngine", "Parameters", "oneint", "rhf", "uhf", "oneint_gradient", "nuclear_repulsion_gradient",
            "rhf_gradient", "PRIM_DTYPE"]

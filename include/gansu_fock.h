@@ -184,6 +184,14 @@ int gansu_oneint(const gansu_molecule* mol, double* d_S, double* d_T, double* d_
  * converged, 0}.                                                            */
 int gansu_rhf_run(const gansu_molecule* mol, const gansu_molecule* aux_mol, const gansu_params* p, double* energies,
                   double* D_out, int* info);
+/* UHF SCF loop (uhf.cu:118-300 via hf.cu:245-314): core guess, per-spin
+ * densities D_s = C_s,occ C_s,occ^T over the molecule's n_alpha / n_beta
+ * electrons, E = 1/2 sum_s D_s.(H + F_s), DIIS per spin
+ * (uhf.hpp:291-400; convergence_method=diis only, GANSU_E_UNSUPPORTED
+ * otherwise).  Da_out, Db_out (host, M*M) may be NULL; energies and info as
+ * gansu_rhf_run.                                                            */
+int gansu_uhf_run(const gansu_molecule* mol, const gansu_molecule* aux_mol, const gansu_params* p, double* energies,
+                  double* Da_out, double* Db_out, int* info);
 
 /* ------------------------------------------------------ nuclear gradient --
  * Analytic RHF gradient dE/dX (hartree/bohr, atom-major x,y,z).  No

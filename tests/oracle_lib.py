@@ -264,6 +264,63 @@ def rohf_fock(H, JKc, JKo, C, S, nc, no, coupling="roothaan"):
     return Fc, Fo, S @ C @ R @ C.T @ S
 
 
+def uhf(prims, types, norm, Z, xyz_bohr, na, nb, thr=1e-12, conv=1e-10, maxiter=200, diis_size=8):
+    """FP64 UHF on the host from the oracle's integrals and J/K (uhf.cu:118-300):
+    core guess, X = U s^-1/2 (eigenvalues of S below 1e-6 dropped), per-spin
+    D_s = C_s,occ C_s,occ^T, F_s = H + J[Da + Db] - K[D_s],
+    E = 1/2 sum_s D_s.(H + F_s), convergence on |dE|, DIIS per spin with the
+    error F_s D_s S - S D_s F_s.  Returns (E_elec, E_total, Da, Db, iterations)."""
+    S, T, V = oneint(prims, norm, Z, xyz_bohr)
+    H = T + V
+    Q = schwarz(prims, types)
+    w, U = np.linalg.eigh(S)
+    keep = w >= 1e-6
+    X = U[:, keep] / np.sqrt(w[keep])
+
+    def coeff(F):
+        e, Cp = np.linalg.eigh(X.T @ F @ X)
+        return X @ Cp
+
+    def dens(C, n):
+        return C[:, :n] @ C[:, :n].T
+
+    def fock(Da, Db):
+        Ja, Ka = jk(prims, types, norm, Q, Da, thr)
+        Jb, Kb = jk(prims, types, norm, Q, Db, thr)
+        return H + Ja + Jb - Ka, H + Ja + Jb - Kb
+
+    Da, Db = dens(coeff(H), na), dens(coeff(H), nb)
+    Fa, Fb = fock(Da, Db)
+    hist = ([], [])
+    prev, E, it = None, 0.0, 0
+    while it < maxiter:
+        Da, Db = dens(coeff(Fa), na), dens(coeff(Fb), nb)
+        Fa, Fb = fock(Da, Db)
+        E = 0.5 * (np.sum(Da * (H + Fa)) + np.sum(Db * (H + Fb)))
+        if prev is not None and abs(E - prev) < conv:
+            break
+        it += 1
+        prev = E
+        new = []
+        for F, D, hs in ((Fa, Da, hist[0]), (Fb, Db, hist[1])):
+            hs.append((F @ D @ S - S @ D @ F, F.copy()))
+            del hs[:-diis_size]
+            n = len(hs)
+            if n > 1:
+                B = -np.ones((n + 1, n + 1))
+                B[n, n] = 0.0
+                for a in range(n):
+                    for b in range(n):
+                        B[a, b] = np.sum(hs[a][0] * hs[b][0])
+                rhs = np.zeros(n + 1)
+                rhs[n] = -1.0
+                c = np.linalg.solve(B, rhs)
+                F = sum(c[k] * hs[k][1] for k in range(n))
+            new.append(F)
+        Fa, Fb = new
+    return E, E + nuclear_repulsion(Z, xyz_bohr), Da, Db, it
+
+
 def ri_jk(B, D):
     Ma, M = B.shape[0], B.shape[1]
     J, K = np.zeros((M, M)), np.zeros((M, M))

@@ -358,3 +358,33 @@ def test_rohf_fock_matches_oracle(method, cuda):
     assert np.abs(F.cpu().numpy() - F_rhf).max() <= 1e-9 * np.abs(F_rhf).max()
     with pytest.raises(RuntimeError):
         eng.compute_fock_matrix_rohf(*args, nc, no, "no-such-set")
+
+
+def test_uhf_scf_matches_oracle(cuda):
+    """UHF SCF (uhf.cu, per-spin DIIS): the Li atom and the water cation against
+    the oracle's UHF; closed-shell water UHF equals RHF on the stored and RI
+    engines."""
+    from dft_gansu_amd import uhf
+    prm = Parameters(convergence_energy_threshold=1e-10)
+    origin = np.zeros((1, 3))
+    Z = np.array([3])
+    mol, (p, t, n) = both(Z, origin, "sto-3g")
+    r = uhf(mol, prm)
+    assert r["converged"]
+    assert abs(r["e_total"] - O.uhf(p, t, n, Z, origin, 2, 1)[1]) < 1e-9
+    assert abs(r["e_total"] - (-7.315526)) < 2e-6
+    Z, xyz = gm.water(jitter=0.01, seed=11)
+    path = gm.basis_path("cc-pvdz")
+    cation = gm.Molecule.from_atoms(Z, xyz, path, charge=1)
+    p, t, n = O.build_molecule(path, Z, xyz)
+    r = uhf(cation, prm)
+    ref = O.uhf(p, t, n, Z, xyz, 5, 4)
+    assert r["converged"]
+    assert abs(r["e_total"] - ref[1]) < 1e-9
+    assert np.abs(r["Da"] - ref[2]).max() < 1e-6 and np.abs(r["Db"] - ref[3]).max() < 1e-6
+    mol, _ = both(Z, xyz, "cc-pvdz")
+    aux, _ = both(Z, xyz, "cc-pvdz-rifit")
+    for kw in (dict(eri_method="stored"), dict(eri_method="ri")):
+        p2 = Parameters(convergence_energy_threshold=1e-10, **kw)
+        a = aux if kw["eri_method"] == "ri" else None
+        assert abs(uhf(mol, p2, aux=a)["e_total"] - rhf(mol, p2, aux=a)["e_total"]) < 1e-9, kw

@@ -323,3 +323,18 @@ def test_rohf_couplings_only_change_the_diagonal_blocks(water_ccpvdz):
             assert np.abs(R[a, b] - ref[a, b]).max() < 1e-9 * np.abs(ref).max(), name
     assert np.abs(Rs["guest-saunders"][c, c] - ref[c, c]).max() > 1e-3
     assert np.abs(Rs["guest-saunders"][v, v] - ref[v, v]).max() > 1e-3
+
+
+# ------------------------------------------------------------------- UHF ----
+def test_uhf_oracle_known_answers():
+    """UHF/STO-3G of the H atom (-0.466582 Eh, the STO-3G 1s energy) and the Li
+    atom (-7.315526 Eh), and the closed-shell limit equal to RHF."""
+    p = gm.basis_path("sto-3g")
+    origin = np.zeros((1, 3))
+    for Zv, na, nb, ref in ((1, 1, 0, -0.466582), (3, 2, 1, -7.315526)):
+        Z = np.array([Zv])
+        pr, t, n = O.build_molecule(p, Z, origin)
+        assert abs(O.uhf(pr, t, n, Z, origin, na, nb)[1] - ref) < 2e-6, Zv
+    Z, xyz = gm.water(jitter=0.01, seed=11)
+    pr, t, n = O.build_molecule(p, Z, xyz)
+    assert abs(O.uhf(pr, t, n, Z, xyz, 5, 5)[1] - O.rhf(pr, t, n, Z, xyz, 5)[1]) < 1e-10
