@@ -291,6 +291,19 @@ int gansu_uhf_run(const gansu_molecule* mol, const gansu_molecule* aux_mol, cons
   });
 }
 
+int gansu_rohf_run(const gansu_molecule* mol, const gansu_molecule* aux_mol, const gansu_params* p, double* energies,
+                   double* Dc, double* Do, int* info) {
+  return guarded([&] {
+    if (!mol || !p || !energies) throw gansu::Error(GANSU_E_ARG, "null argument");
+    auto r = gansu::rohf_run(mol->m, aux_mol ? &aux_mol->m : nullptr, p->p);
+    energies[0] = r.e_elec;
+    energies[1] = r.e_total;
+    if (Dc) std::memcpy(Dc, r.Da.data(), r.Da.size() * sizeof(double));
+    if (Do) std::memcpy(Do, r.Db.data(), r.Db.size() * sizeof(double));
+    if (info) { info[0] = r.iterations; info[1] = r.fock_builds; info[2] = r.converged; info[3] = 0; }
+  });
+}
+
 int gansu_rhf_gradient(const gansu_molecule* mol, const gansu_molecule* aux_mol, const gansu_params* p,
                        double* energies, double* grad, int* info) {
   return guarded([&] {

@@ -536,4 +536,135 @@ UhfResult uhf_run(const Molecule& mol, const Molecule* aux, const Params& prm) {
   return res;
 }
 
+// ROHF (rohf.cu:59-300, rohf.hpp:330-360): n_beta closed and
+// n_alpha - n_beta open orbitals; core guess; C from the unified Fock matrix;
+// D_c = 2 C_c C_c^T, D_o = C_o C_o^T (gpu_kernels.cu:108-134); the engine's
+// ROHF build with the coupling set rohf_parameter_name;
+// E = 1/2 [D_c.(H + F_c) + D_o.H + 2 D_o.F_o] (gpu_manager.cu:871-882);
+// DIIS on the unified matrix with the total density.
+UhfResult rohf_run(const Molecule& mol, const Molecule* aux, const Params& prm) {
+  const int n = mol.nbasis, nc = mol.n_beta, no = mol.n_alpha - mol.n_beta;
+  if (no < 0 || nc + no > n) throw Error(GANSU_E_ELECTRONS, "invalid closed/open orbital counts");
+  if (prm.get("initial_guess") != "core")
+    throw Error(GANSU_E_UNSUPPORTED, "initial_guess=" + prm.get("initial_guess") + " is not provided (core only)");
+  if (prm.get("convergence_method") != "diis")
+    throw Error(GANSU_E_UNSUPPORTED, "ROHF is provided with convergence_method=diis");
+  RohfCoupling cp{};
+  if (!rohf_coupling(prm.get("rohf_parameter_name"), &cp))
+    throw Error(GANSU_E_PARAM, "unknown rohf_parameter_name: " + prm.get("rohf_parameter_name"));
+  const int maxiter = prm.geti("maxiter");
+  const double conv = prm.getd("convergence_energy_threshold");
+  const int diis_size = prm.geti("diis_size");
+  const bool diis_x = prm.geti("diis_include_transform") != 0;
+
+  GANSU_HIP(hipSetDevice(0));
+  Work w(n);
+  Blas blas;
+  const size_t nn = (size_t)n * n;
+  compute_oneint(mol, w.S.ptr, w.T.ptr, w.V.ptr, 0);
+  GANSU_HIP(launch_axpby(1.0, w.T.ptr, 1.0, w.V.ptr, nn, w.H.ptr, 0));
+  std::map<std::string, std::string> opts{{"eri_method", prm.get("eri_method")},
+                                          {"schwarz_screening_threshold", prm.get("schwarz_screening_threshold")},
+                                          {"precision", prm.get("precision")},
+                                          {"fp32_bound_threshold", prm.get("fp32_bound_threshold")},
+                                          {"ri_k", prm.get("ri_k")}};
+  const bool ri = prm.get("eri_method") == "ri";
+  if (ri && !aux) throw Error(GANSU_E_ARG, "eri_method=ri needs auxiliary_gbsfilename");
+  FockEngine eng(mol.prims.data(), mol.prims.size(), mol.types.data(), (int)mol.types.size(), mol.norm.data(), n,
+                 ri ? aux->prims.data() : nullptr, ri ? aux->prims.size() : 0, ri ? aux->types.data() : nullptr,
+                 ri ? (int)aux->types.size() : 0, ri ? aux->norm.data() : nullptr, ri ? aux->nbasis : 0, opts);
+  eng.precompute();
+  syevd(blas.h, n, w.S.ptr, w);
+  GANSU_HIP(launch_transform(w.evec.ptr, w.eval.ptr, n, 1e-6, w.X.ptr, 0));
+
+  DevBuf<double> Dc(nn), Do(nn), Fc(nn), Fo(nn);
+  auto coefficients = [&](const double* Fm) {
+    blas.gemm(true, false, n, w.X.ptr, Fm, w.tmp.ptr);
+    blas.gemm(false, false, n, w.tmp.ptr, w.X.ptr, w.tmp2.ptr);
+    syevd(blas.h, n, w.tmp2.ptr, w);
+    blas.gemm(false, true, n, w.X.ptr, w.evec.ptr, w.C.ptr);
+  };
+  auto density = [&]() {
+    // the column-major view of row-major C is C^T: rows 0..nc-1 closed, nc.. open
+    const double two = 2.0, one = 1.0, zero = 0.0;
+    if (nc > 0)
+      blas_check(rocblas_dgemm(blas.h, rocblas_operation_transpose, rocblas_operation_none, n, n, nc, &two, w.C.ptr,
+                               n, w.C.ptr, n, &zero, Dc.ptr, n),
+                 "rocblas_dgemm(D_c)");
+    else
+      GANSU_HIP(launch_zero(Dc.ptr, nn, 0));
+    if (no > 0)
+      blas_check(rocblas_dgemm(blas.h, rocblas_operation_transpose, rocblas_operation_none, n, n, no, &one,
+                               w.C.ptr + nc, n, w.C.ptr + nc, n, &zero, Do.ptr, n),
+                 "rocblas_dgemm(D_o)");
+    else
+      GANSU_HIP(launch_zero(Do.ptr, nn, 0));
+    GANSU_HIP(launch_axpby(1.0, Dc.ptr, 1.0, Do.ptr, nn, w.D.ptr, 0));
+  };
+  UhfResult res;
+  auto fock = [&]() {
+    eng.build_rohf(Dc.ptr, Do.ptr, w.H.ptr, w.C.ptr, w.S.ptr, cp, nc, no, Fc.ptr, Fo.ptr, w.F.ptr, 0);
+    ++res.fock_builds;
+  };
+  auto energy = [&]() {
+    return 0.5 * (dot(Dc.ptr, w.H.ptr, nn, w.part) + dot(Dc.ptr, Fc.ptr, nn, w.part) + dot(Do.ptr, w.H.ptr, nn, w.part) +
+                  2.0 * dot(Do.ptr, Fo.ptr, nn, w.part));
+  };
+  coefficients(w.H.ptr);
+  density();
+  fock();
+  DevBuf<double> errs(nn * diis_size), focks(nn * diis_size);
+  int diis_iter = 0;
+  double prev = std::numeric_limits<double>::max(), E = 0.0;
+  int it = 0;
+  while (it < maxiter) {
+    coefficients(w.F.ptr);
+    density();
+    fock();
+    E = energy();
+    if (std::fabs(E - prev) < conv) { res.converged = 1; break; }
+    ++it;
+    prev = E;
+    double* e = w.tmp2.ptr;
+    blas.gemm(false, false, n, w.F.ptr, w.D.ptr, w.tmp.ptr);
+    blas.gemm(false, false, n, w.tmp.ptr, w.S.ptr, e);
+    blas.gemm(false, false, n, w.S.ptr, w.D.ptr, w.tmp.ptr);
+    blas.gemm(false, false, n, w.tmp.ptr, w.F.ptr, e, -1.0, 1.0);
+    if (diis_x) {
+      blas.gemm(false, false, n, w.X.ptr, e, w.tmp.ptr);
+      blas.gemm(false, true, n, w.tmp.ptr, w.X.ptr, e);
+    }
+    const int slot = diis_iter % diis_size;
+    GANSU_HIP(hipMemcpy(errs.ptr + slot * nn, e, nn * sizeof(double), hipMemcpyDeviceToDevice));
+    GANSU_HIP(hipMemcpy(focks.ptr + slot * nn, w.F.ptr, nn * sizeof(double), hipMemcpyDeviceToDevice));
+    const int np = std::min(diis_iter + 1, diis_size);
+    if (np > 1) {
+      const int ns = np + 1;
+      std::vector<double> B((size_t)ns * ns, 0.0), rhs(ns, 0.0);
+      for (int i = 0; i < np; ++i) {
+        for (int j = i; j < np; ++j)
+          B[i * ns + j] = B[j * ns + i] = dot(errs.ptr + i * nn, errs.ptr + j * nn, nn, w.part);
+        B[i * ns + np] = B[np * ns + i] = -1.0;
+      }
+      rhs[np] = -1.0;
+      const auto c = solve_small(ns, B, rhs);
+      std::vector<const double*> mats(np);
+      for (int i = 0; i < np; ++i) mats[i] = focks.ptr + i * nn;
+      DevBuf<double> dc;
+      dc.upload(c.data(), np);
+      DevBuf<const double*> dm;
+      dm.upload(mats);
+      GANSU_HIP(launch_lincomb(np, dc.ptr, dm.ptr, nn, w.F.ptr, 0));
+    }
+    ++diis_iter;
+  }
+  GANSU_HIP(hipDeviceSynchronize());
+  res.e_elec = E;
+  res.e_total = E + mol.nuclear_repulsion();
+  res.iterations = it;
+  res.Da = Dc.download();  // closed-shell density
+  res.Db = Do.download();  // open-shell density
+  return res;
+}
+
 }  // namespace gansu

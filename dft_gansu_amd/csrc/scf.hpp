@@ -27,6 +27,9 @@ struct UhfResult {
 };
 // HF::single_point_energy for UHF (uhf.cu), core guess and per-spin DIIS.
 UhfResult uhf_run(const Molecule& mol, const Molecule* aux, const Params& prm);
+// HF::single_point_energy for ROHF (rohf.cu): core guess, DIIS on the
+// unified Fock matrix; Da / Db of the result hold the closed / open densities.
+UhfResult rohf_run(const Molecule& mol, const Molecule* aux, const Params& prm);
 
 // Device copies of a molecule's shell tables for the one-electron kernels.
 struct OneIntHost {

@@ -248,6 +248,18 @@ def uhf(mol, params, aux=None):
                 converged=bool(info[2]), Da=Da, Db=Db)
 
 
+def rohf(mol, params, aux=None):
+    """Run the ROHF SCF; returns dict(e_elec, e_total, iterations, fock_builds, converged, Dc, Do)."""
+    en = np.zeros(2)
+    Dc = np.zeros((mol.n_basis, mol.n_basis))
+    Do = np.zeros((mol.n_basis, mol.n_basis))
+    info = np.zeros(4, dtype=np.int32)
+    check(lib().gansu_rohf_run(mol.handle, aux.handle if aux else None, params.handle, ptr(en), ptr(Dc), ptr(Do),
+                               ptr(info)))
+    return dict(e_elec=float(en[0]), e_total=float(en[1]), iterations=int(info[0]), fock_builds=int(info[1]),
+                converged=bool(info[2]), Dc=Dc, Do=Do)
+
+
 def rhf(mol, params, aux=None):
     """Run the RHF SCF; returns dict(e_elec, e_total, iterations, fock_builds, converged, D)."""
     en = np.zeros(2)
@@ -258,5 +270,5 @@ def rhf(mol, params, aux=None):
                 converged=bool(info[2]), D=D)
 
 
-__all__ = ["FockEngine", "Parameters", "oneint", "rhf", "uhf", "oneint_gradient", "nuclear_repulsion_gradient",
+__all__ = ["FockEngine", "Parameters", "oneint", "rhf", "uhf", "rohf", "oneint_gradient", "nuclear_repulsion_gradient",
            "rhf_gradient", "PRIM_DTYPE"]

@@ -192,6 +192,14 @@ int gansu_rhf_run(const gansu_molecule* mol, const gansu_molecule* aux_mol, cons
  * gansu_rhf_run.                                                            */
 int gansu_uhf_run(const gansu_molecule* mol, const gansu_molecule* aux_mol, const gansu_params* p, double* energies,
                   double* Da_out, double* Db_out, int* info);
+/* ROHF SCF loop (rohf.cu:59-300 via hf.cu:245-314): n_beta closed and
+ * n_alpha - n_beta open orbitals, core guess, D_c = 2 C_c C_c^T,
+ * D_o = C_o C_o^T, the ROHF build with rohf_parameter_name,
+ * E = 1/2 [D_c.(H + F_c) + D_o.H + 2 D_o.F_o], DIIS on the unified Fock
+ * matrix (convergence_method=diis only).  Dc_out, Do_out (host, M*M) may be
+ * NULL; energies and info as gansu_rhf_run.                                 */
+int gansu_rohf_run(const gansu_molecule* mol, const gansu_molecule* aux_mol, const gansu_params* p, double* energies,
+                   double* Dc_out, double* Do_out, int* info);
 
 /* ------------------------------------------------------ nuclear gradient --
  * Analytic RHF gradient dE/dX (hartree/bohr, atom-major x,y,z).  No

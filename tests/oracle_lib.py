@@ -321,6 +321,59 @@ def uhf(prims, types, norm, Z, xyz_bohr, na, nb, thr=1e-12, conv=1e-10, maxiter=
     return E, E + nuclear_repulsion(Z, xyz_bohr), Da, Db, it
 
 
+def rohf(prims, types, norm, Z, xyz_bohr, nc, no, coupling="roothaan", thr=1e-12, conv=1e-10, maxiter=300,
+         diis_size=8):
+    """FP64 ROHF on the host (rohf.cu:59-300): core guess, C from the unified
+    Fock matrix, D_c = 2 C_c C_c^T, D_o = C_o C_o^T (gpu_kernels.cu:108-134),
+    rohf_fock for F_c, F_o, F, E = 1/2 [D_c.(H + F_c) + D_o.H + 2 D_o.F_o]
+    (gpu_manager.cu:871-882), DIIS on F with the total density.
+    Returns (E_elec, E_total, Dc, Do, iterations)."""
+    S, T, V = oneint(prims, norm, Z, xyz_bohr)
+    H = T + V
+    Q = schwarz(prims, types)
+    w, U = np.linalg.eigh(S)
+    keep = w >= 1e-6
+    X = U[:, keep] / np.sqrt(w[keep])
+
+    def coeff(F):
+        e, Cp = np.linalg.eigh(X.T @ F @ X)
+        return X @ Cp
+
+    def step(F):
+        C = coeff(F)
+        Dc = 2.0 * C[:, :nc] @ C[:, :nc].T
+        Do = C[:, nc:nc + no] @ C[:, nc:nc + no].T
+        Fc, Fo, Fu = rohf_fock(H, jk(prims, types, norm, Q, Dc, thr), jk(prims, types, norm, Q, Do, thr), C, S, nc, no,
+                               coupling)
+        E = 0.5 * (np.sum(Dc * (H + Fc)) + np.sum(Do * H) + 2.0 * np.sum(Do * Fo))
+        return Dc, Do, Fu, E
+
+    Dc, Do, F, E = step(H)
+    hist = []
+    prev, it = None, 0
+    while it < maxiter:
+        Dc, Do, F, E = step(F)
+        if prev is not None and abs(E - prev) < conv:
+            break
+        it += 1
+        prev = E
+        D = Dc + Do
+        hist.append((F @ D @ S - S @ D @ F, F.copy()))
+        del hist[:-diis_size]
+        n = len(hist)
+        if n > 1:
+            B = -np.ones((n + 1, n + 1))
+            B[n, n] = 0.0
+            for a in range(n):
+                for b in range(n):
+                    B[a, b] = np.sum(hist[a][0] * hist[b][0])
+            rhs = np.zeros(n + 1)
+            rhs[n] = -1.0
+            c = np.linalg.solve(B, rhs)
+            F = sum(c[k] * hist[k][1] for k in range(n))
+    return E, E + nuclear_repulsion(Z, xyz_bohr), Dc, Do, it
+
+
 def ri_jk(B, D):
     Ma, M = B.shape[0], B.shape[1]
     J, K = np.zeros((M, M)), np.zeros((M, M))

@@ -388,3 +388,29 @@ def test_uhf_scf_matches_oracle(cuda):
         p2 = Parameters(convergence_energy_threshold=1e-10, **kw)
         a = aux if kw["eri_method"] == "ri" else None
         assert abs(uhf(mol, p2, aux=a)["e_total"] - rhf(mol, p2, aux=a)["e_total"]) < 1e-9, kw
+
+
+def test_rohf_scf_matches_oracle(cuda):
+    """ROHF SCF (rohf.cu): the O triplet (STO-3G) and the water cation
+    (cc-pVDZ) against the oracle's ROHF; closed-shell water ROHF equals RHF."""
+    from dft_gansu_amd import rohf
+    prm = Parameters(convergence_energy_threshold=1e-10)
+    origin = np.zeros((1, 3))
+    Z = np.array([8])
+    path = gm.basis_path("sto-3g")
+    mol = gm.Molecule.from_atoms(Z, origin, path, beta_to_alpha=1)
+    p, t, n = O.build_molecule(path, Z, origin)
+    r = rohf(mol, prm)
+    assert r["converged"]
+    assert abs(r["e_total"] - O.rohf(p, t, n, Z, origin, 3, 2)[1]) < 1e-9
+    Z, xyz = gm.water(jitter=0.01, seed=11)
+    path = gm.basis_path("cc-pvdz")
+    cation = gm.Molecule.from_atoms(Z, xyz, path, charge=1)
+    p, t, n = O.build_molecule(path, Z, xyz)
+    for coupling in ("roothaan", "guest-saunders"):
+        r = rohf(cation, Parameters(convergence_energy_threshold=1e-10, rohf_parameter_name=coupling))
+        ref = O.rohf(p, t, n, Z, xyz, 4, 1, coupling)
+        assert r["converged"]
+        assert abs(r["e_total"] - ref[1]) < 1e-9, coupling
+    mol, _ = both(Z, xyz, "cc-pvdz")
+    assert abs(rohf(mol, prm)["e_total"] - rhf(mol, prm)["e_total"]) < 1e-9

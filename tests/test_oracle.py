@@ -338,3 +338,24 @@ def test_uhf_oracle_known_answers():
     Z, xyz = gm.water(jitter=0.01, seed=11)
     pr, t, n = O.build_molecule(p, Z, xyz)
     assert abs(O.uhf(pr, t, n, Z, xyz, 5, 5)[1] - O.rhf(pr, t, n, Z, xyz, 5)[1]) < 1e-10
+
+
+# ------------------------------------------------------------------ ROHF ----
+def test_rohf_oracle_pins():
+    """ROHF: the H atom at the STO-3G 1s energy; closed-shell water equal to
+    RHF; the converged energy of the Li doublet and the O triplet independent
+    of the coupling set (the sets differ only in the diagonal blocks of the
+    unified matrix, which do not enter the stationarity conditions)."""
+    p = gm.basis_path("sto-3g")
+    origin = np.zeros((1, 3))
+    Z = np.array([1])
+    pr, t, n = O.build_molecule(p, Z, origin)
+    assert abs(O.rohf(pr, t, n, Z, origin, 0, 1)[1] - (-0.466582)) < 2e-6
+    for Zv, nc, no in ((3, 1, 1), (8, 3, 2)):
+        Z = np.array([Zv])
+        pr, t, n = O.build_molecule(p, Z, origin)
+        e = [O.rohf(pr, t, n, Z, origin, nc, no, c)[1] for c in ("roothaan", "guest-saunders", "mcweeny-diercksen")]
+        assert max(e) - min(e) < 1e-9, (Zv, e)
+    Z, xyz = gm.water(jitter=0.01, seed=11)
+    pr, t, n = O.build_molecule(p, Z, xyz)
+    assert abs(O.rohf(pr, t, n, Z, xyz, 5, 0)[1] - O.rhf(pr, t, n, Z, xyz, 5)[1]) < 1e-10
