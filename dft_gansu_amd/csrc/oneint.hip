@@ -113,6 +113,17 @@ __global__ void k_axpby(double a, const double* __restrict__ x, double b, const 
   if (id < n) out[id] = a * x[id] + b * y[id];
 }
 
+// Generalized Wolfsberg-Helmholz guess (gpu_kernels.cu:535-545, cx = 1.75 by
+// gpu_manager.cu:1208): F_pq = cx S_pq (H_pp + H_qq) / 2 for every p, q, the
+// diagonal included.
+__global__ void k_gwh_guess(const double* __restrict__ H, const double* __restrict__ S, int M, double cx,
+                            double* __restrict__ F) {
+  const size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= (size_t)M * M) return;
+  const size_t p = id / M, q = id % M;
+  F[id] = cx * S[id] * (H[p * M + p] + H[q * M + q]) / 2.0;
+}
+
 // partial sums of sum_i x_i * y_i (deterministic two-pass reduction)
 __global__ void k_dot_partial(const double* __restrict__ x, const double* __restrict__ y, size_t n,
                               double* __restrict__ part) {
@@ -151,6 +162,12 @@ hipError_t launch_transform(const double* Ucol, const double* evals, int n, doub
 hipError_t launch_lincomb(int nmat, const double* d_coef, const double* const* d_mats, size_t n, double* out,
                           hipStream_t s) {
   hipLaunchKernelGGL(k_lincomb, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, nmat, d_coef, d_mats, n, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_gwh_guess(const double* H, const double* S, int M, double* F, hipStream_t s) {
+  const size_t n = (size_t)M * M;
+  hipLaunchKernelGGL(k_gwh_guess, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, H, S, M, 1.75, F);
   return hipGetLastError();
 }
 

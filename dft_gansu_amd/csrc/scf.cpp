@@ -203,7 +203,8 @@ ScfResult rhf_run(const Molecule& mol, const Molecule* aux, const Params& prm, b
   const int nocc = mol.n_electrons / 2;
   if (nocc > n) throw Error(GANSU_E_ELECTRONS, "more occupied orbitals than basis functions");
   const std::string guess = prm.get("initial_guess");
-  if (guess != "core") throw Error(GANSU_E_UNSUPPORTED, "initial_guess=" + guess + " is not provided (core only)");
+  if (guess != "core" && guess != "gwh")
+    throw Error(GANSU_E_UNSUPPORTED, "initial_guess=" + guess + " is not provided (core and gwh only)");
   const std::string conv_method = prm.get("convergence_method");
   if (conv_method != "diis" && conv_method != "damping" && conv_method != "optimaldamping")
     throw Error(GANSU_E_PARAM, "Unknown convergence method: " + conv_method);
@@ -284,7 +285,12 @@ ScfResult rhf_run(const Molecule& mol, const Molecule* aux, const Params& prm, b
   auto energy = [&]() { return 0.5 * (dot(w.D.ptr, w.H.ptr, nn, w.part) + dot(w.D.ptr, w.F.ptr, nn, w.part)); };
 
   const double ts3 = wall();
-  coefficients(w.H.ptr);
+  DevBuf<double> guess_F;
+  if (guess == "gwh") {
+    guess_F.alloc(nn);
+    GANSU_HIP(launch_gwh_guess(w.H.ptr, w.S.ptr, n, guess_F.ptr, 0));
+  }
+  coefficients(guess == "gwh" ? guess_F.ptr : w.H.ptr);
   density();
   fock();
   if (timing)
@@ -407,8 +413,9 @@ UhfResult uhf_run(const Molecule& mol, const Molecule* aux, const Params& prm) {
   const int n = mol.nbasis, na = mol.n_alpha, nb = mol.n_beta;
   if (na > n || nb > n)
     throw Error(GANSU_E_ELECTRONS, "more alpha or beta electrons than basis functions");
-  if (prm.get("initial_guess") != "core")
-    throw Error(GANSU_E_UNSUPPORTED, "initial_guess=" + prm.get("initial_guess") + " is not provided (core only)");
+  const std::string guess = prm.get("initial_guess");
+  if (guess != "core" && guess != "gwh")
+    throw Error(GANSU_E_UNSUPPORTED, "initial_guess=" + guess + " is not provided (core and gwh only)");
   if (prm.get("convergence_method") != "diis")
     throw Error(GANSU_E_UNSUPPORTED, "UHF is provided with convergence_method=diis");
   const int maxiter = prm.geti("maxiter");
@@ -469,8 +476,13 @@ UhfResult uhf_run(const Molecule& mol, const Molecule* aux, const Params& prm) {
     for (int sp = 0; sp < 2; ++sp) e += dot(D[sp], w.H.ptr, nn, w.part) + dot(D[sp], F[sp], nn, w.part);
     return 0.5 * e;
   };
+  DevBuf<double> guess_F;
+  if (guess == "gwh") {
+    guess_F.alloc(nn);
+    GANSU_HIP(launch_gwh_guess(w.H.ptr, w.S.ptr, n, guess_F.ptr, 0));
+  }
   for (int sp = 0; sp < 2; ++sp) {
-    coefficients(w.H.ptr, C[sp]);
+    coefficients(guess == "gwh" ? guess_F.ptr : w.H.ptr, C[sp]);
     density(sp);
   }
   fock();
@@ -545,8 +557,9 @@ UhfResult uhf_run(const Molecule& mol, const Molecule* aux, const Params& prm) {
 UhfResult rohf_run(const Molecule& mol, const Molecule* aux, const Params& prm) {
   const int n = mol.nbasis, nc = mol.n_beta, no = mol.n_alpha - mol.n_beta;
   if (no < 0 || nc + no > n) throw Error(GANSU_E_ELECTRONS, "invalid closed/open orbital counts");
-  if (prm.get("initial_guess") != "core")
-    throw Error(GANSU_E_UNSUPPORTED, "initial_guess=" + prm.get("initial_guess") + " is not provided (core only)");
+  const std::string guess = prm.get("initial_guess");
+  if (guess != "core" && guess != "gwh")
+    throw Error(GANSU_E_UNSUPPORTED, "initial_guess=" + guess + " is not provided (core and gwh only)");
   if (prm.get("convergence_method") != "diis")
     throw Error(GANSU_E_UNSUPPORTED, "ROHF is provided with convergence_method=diis");
   RohfCoupling cp{};
@@ -610,7 +623,12 @@ UhfResult rohf_run(const Molecule& mol, const Molecule* aux, const Params& prm) 
     return 0.5 * (dot(Dc.ptr, w.H.ptr, nn, w.part) + dot(Dc.ptr, Fc.ptr, nn, w.part) + dot(Do.ptr, w.H.ptr, nn, w.part) +
                   2.0 * dot(Do.ptr, Fo.ptr, nn, w.part));
   };
-  coefficients(w.H.ptr);
+  DevBuf<double> guess_F;
+  if (guess == "gwh") {
+    guess_F.alloc(nn);
+    GANSU_HIP(launch_gwh_guess(w.H.ptr, w.S.ptr, n, guess_F.ptr, 0));
+  }
+  coefficients(guess == "gwh" ? guess_F.ptr : w.H.ptr);
   density();
   fock();
   DevBuf<double> errs(nn * diis_size), focks(nn * diis_size);

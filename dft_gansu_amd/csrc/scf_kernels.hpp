@@ -27,6 +27,8 @@ hipError_t launch_transform(const double* Ucol, const double* evals, int n, doub
 hipError_t launch_lincomb(int nmat, const double* d_coef, const double* const* d_mats, size_t n, double* out,
                           hipStream_t s);
 hipError_t launch_axpby(double a, const double* x, double b, const double* y, size_t n, double* out, hipStream_t s);
+// Initial Fock matrix of the GWH guess: F_pq = 1.75 S_pq (H_pp + H_qq) / 2.
+hipError_t launch_gwh_guess(const double* H, const double* S, int M, double* F, hipStream_t s);
 hipError_t launch_dot_partial(const double* x, const double* y, size_t n, double* part, int nblocks, hipStream_t s);
 hipError_t launch_has_nan(const double* x, size_t n, int* flag, hipStream_t s);
 

@@ -178,13 +178,13 @@ void gansu_fock_destroy(gansu_fock_engine* e);
  * S, T, V on the device (M x M row-major, caller-owned device pointers).    */
 int gansu_oneint(const gansu_molecule* mol, double* d_S, double* d_T, double* d_V, void* stream);
 
-/* RHF SCF loop of hf.cu:245-314 (core/DIIS/damping as the recipe says) with
- * this engine as the ERI strategy.  energies[2] = {E_electronic, E_total};
+/* RHF SCF loop of hf.cu:245-314 (core or GWH guess, DIIS/damping as the recipe
+ * says) with this engine as the ERI strategy.  energies[2] = {E_electronic, E_total};
  * D_out (host, M*M) may be NULL.  info[4] = {iterations, fock builds,
  * converged, 0}.                                                            */
 int gansu_rhf_run(const gansu_molecule* mol, const gansu_molecule* aux_mol, const gansu_params* p, double* energies,
                   double* D_out, int* info);
-/* UHF SCF loop (uhf.cu:118-300 via hf.cu:245-314): core guess, per-spin
+/* UHF SCF loop (uhf.cu:118-300 via hf.cu:245-314): core or GWH guess, per-spin
  * densities D_s = C_s,occ C_s,occ^T over the molecule's n_alpha / n_beta
  * electrons, E = 1/2 sum_s D_s.(H + F_s), DIIS per spin
  * (uhf.hpp:291-400; convergence_method=diis only, GANSU_E_UNSUPPORTED
@@ -193,7 +193,7 @@ int gansu_rhf_run(const gansu_molecule* mol, const gansu_molecule* aux_mol, cons
 int gansu_uhf_run(const gansu_molecule* mol, const gansu_molecule* aux_mol, const gansu_params* p, double* energies,
                   double* Da_out, double* Db_out, int* info);
 /* ROHF SCF loop (rohf.cu:59-300 via hf.cu:245-314): n_beta closed and
- * n_alpha - n_beta open orbitals, core guess, D_c = 2 C_c C_c^T,
+ * n_alpha - n_beta open orbitals, core or GWH guess, D_c = 2 C_c C_c^T,
  * D_o = C_o C_o^T, the ROHF build with rohf_parameter_name,
  * E = 1/2 [D_c.(H + F_c) + D_o.H + 2 D_o.F_o], DIIS on the unified Fock
  * matrix (convergence_method=diis only).  Dc_out, Do_out (host, M*M) may be
