@@ -6,5 +6,5 @@ workload geometry generators.
 """
 from ._lib import GansuError, LIB_PATH, lib  # noqa: F401
 from .engine import (FockEngine, Parameters, nuclear_repulsion_gradient, oneint, oneint_gradient, rhf,  # noqa: F401
-                     rhf_gradient, rohf, uhf)
+                     rhf_gradient, rhf_orbitals, rohf, uhf, write_molden)
 from .molecule import CONFIGS, Molecule, basis_path, config_molecule  # noqa: F401

@@ -31,7 +31,7 @@ EXPORTS = [
     "gansu_fock_algorithm_name", "gansu_fock_schwarz", "gansu_fock_num_primitive_pairs", "gansu_fock_stats",
     "gansu_fock_set_profiling", "gansu_fock_last_profile", "gansu_fock_destroy", "gansu_oneint", "gansu_rhf_run",
     "gansu_fock_gradient_rhf", "gansu_fock_gradient_rhf_occ", "gansu_oneint_gradient", "gansu_molecule_nuclear_repulsion_gradient",
-    "gansu_rhf_gradient", "gansu_uhf_run", "gansu_rohf_run", "gansu_fock_build_uhf", "gansu_fock_build_rohf", "gansu_fock_update_geometry",
+    "gansu_rhf_gradient", "gansu_uhf_run", "gansu_rohf_run", "gansu_rhf_run_mo", "gansu_write_molden", "gansu_fock_build_uhf", "gansu_fock_build_rohf", "gansu_fock_update_geometry",
 ]
 
 
@@ -51,6 +51,13 @@ def lib():
         if not os.path.exists(LIB_PATH):
             raise ImportError(f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'`"
                               " (there is no CPU fallback)")
+        # torch ships its own HIP runtime; when the library's (system) runtime
+        # is loaded first, torch's later initialisation sees no device and
+        # the library's calls then fail, so torch is loaded first when present
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = ctypes.CDLL(LIB_PATH)
         vp, cp, ip, dp, sz = ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.c_size_t
         L.gansu_last_error.restype = cp
@@ -85,6 +92,8 @@ def lib():
         L.gansu_rhf_run.argtypes = [vp, vp, vp, vp, vp, vp]
         L.gansu_uhf_run.argtypes = [vp, vp, vp, vp, vp, vp, vp]
         L.gansu_rohf_run.argtypes = [vp, vp, vp, vp, vp, vp, vp]
+        L.gansu_rhf_run_mo.argtypes = [vp, vp, vp, vp, vp, vp, vp]
+        L.gansu_write_molden.argtypes = [vp, vp, vp, vp, ip, ctypes.c_char_p]
         L.gansu_fock_gradient_rhf.argtypes = [vp, vp, ip, vp, vp]
         L.gansu_fock_gradient_rhf_occ.argtypes = [vp, vp, vp, ip, ip, vp, vp]
         L.gansu_oneint_gradient.argtypes = [vp, vp, vp, vp, vp]

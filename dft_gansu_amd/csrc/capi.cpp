@@ -278,6 +278,27 @@ int gansu_molecule_nuclear_repulsion_gradient(const gansu_molecule* mol, double*
   });
 }
 
+int gansu_rhf_run_mo(const gansu_molecule* mol, const gansu_molecule* aux_mol, const gansu_params* p,
+                     double* energies, double* C_out, double* eps_out, int* info) {
+  return guarded([&] {
+    if (!mol || !p || !energies) throw gansu::Error(GANSU_E_ARG, "null argument");
+    auto r = gansu::rhf_run(mol->m, aux_mol ? &aux_mol->m : nullptr, p->p);
+    energies[0] = r.e_elec;
+    energies[1] = r.e_total;
+    if (C_out) std::memcpy(C_out, r.C.data(), r.C.size() * sizeof(double));
+    if (eps_out) std::memcpy(eps_out, r.eps.data(), r.eps.size() * sizeof(double));
+    if (info) { info[0] = r.iterations; info[1] = r.fock_builds; info[2] = r.converged; info[3] = 0; }
+  });
+}
+
+int gansu_write_molden(const gansu_molecule* mol, const double* C, const double* eps, const double* occ, int n_mo,
+                       const char* path) {
+  return guarded([&] {
+    if (!mol || !C || !eps || !occ || !path) throw gansu::Error(GANSU_E_ARG, "null argument");
+    gansu::write_molden(mol->m, C, eps, occ, n_mo, path);
+  });
+}
+
 int gansu_uhf_run(const gansu_molecule* mol, const gansu_molecule* aux_mol, const gansu_params* p, double* energies,
                   double* Da, double* Db, int* info) {
   return guarded([&] {

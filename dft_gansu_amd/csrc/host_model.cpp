@@ -137,6 +137,11 @@ double contracted_norm(int L, const std::vector<double>& e, const std::vector<do
 
 }  // namespace
 
+std::string element_symbol(int Z) {
+  if (Z < 1 || Z > kNumElements) throw Error(GANSU_E_ARG, "atomic number out of range");
+  return kElements[Z - 1];
+}
+
 double Molecule::nuclear_repulsion() const {
   double e = 0.0;
   for (size_t a = 0; a < atoms.size(); ++a)

@@ -388,6 +388,8 @@ ScfResult rhf_run(const Molecule& mol, const Molecule* aux, const Params& prm, b
   res.e_total = E + mol.nuclear_repulsion();
   res.iterations = it;
   res.D = w.D.download();
+  res.C = w.C.download();     // the orbitals of the last diagonalisation (those of D)
+  res.eps = w.eval.download();
   if (gradient) {
     // W = 2 C_occ eps C_occ^T = D F D / 2 for the converged pair (D, F(D))
     blas.gemm(false, false, n, w.F.ptr, w.D.ptr, w.tmp.ptr);

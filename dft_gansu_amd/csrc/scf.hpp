@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <string>
 #include <vector>
 
 #include "engine.hpp"
@@ -14,6 +15,8 @@ struct ScfResult {
   int iterations = 0, fock_builds = 0, converged = 0;
   std::vector<double> D;     // final density (M x M)
   std::vector<double> grad;  // dE/dX (3 natoms), when requested
+  std::vector<double> C;     // final MO coefficients (M x M row-major, MOs in columns)
+  std::vector<double> eps;   // their orbital energies (ascending)
 };
 
 // HF::single_point_energy (hf.cu:245-314) for RHF; with gradient = true the
@@ -30,6 +33,11 @@ UhfResult uhf_run(const Molecule& mol, const Molecule* aux, const Params& prm);
 // HF::single_point_energy for ROHF (rohf.cu): core guess, DIIS on the
 // unified Fock matrix; Da / Db of the result hold the closed / open densities.
 UhfResult rohf_run(const Molecule& mol, const Molecule* aux, const Params& prm);
+
+// Molden file of the orbitals C (M x n_mo row-major, MOs in columns) with
+// energies eps and occupations occ (rhf.cu:363-425 layout).
+void write_molden(const Molecule& mol, const double* C, const double* eps, const double* occ, int nmo,
+                  const std::string& path);
 
 // Device copies of a molecule's shell tables for the one-electron kernels.
 struct OneIntHost {

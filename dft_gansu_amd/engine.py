@@ -260,6 +260,27 @@ def rohf(mol, params, aux=None):
                 converged=bool(info[2]), Dc=Dc, Do=Do)
 
 
+def rhf_orbitals(mol, params, aux=None):
+    """RHF SCF returning dict(e_elec, e_total, iterations, fock_builds, converged, C, eps): the
+    final MO coefficients (M x M, MOs in columns) and orbital energies."""
+    en = np.zeros(2)
+    C = np.zeros((mol.n_basis, mol.n_basis))
+    eps = np.zeros(mol.n_basis)
+    info = np.zeros(4, dtype=np.int32)
+    check(lib().gansu_rhf_run_mo(mol.handle, aux.handle if aux else None, params.handle, ptr(en), ptr(C), ptr(eps),
+                                 ptr(info)))
+    return dict(e_elec=float(en[0]), e_total=float(en[1]), iterations=int(info[0]), fock_builds=int(info[1]),
+                converged=bool(info[2]), C=C, eps=eps)
+
+
+def write_molden(mol, C, eps, occ, path):
+    """Molden file (rhf.cu:363-425 layout) of orbitals C (M x n_mo, MOs in columns)."""
+    C = np.ascontiguousarray(C, dtype=np.float64)
+    eps = np.ascontiguousarray(eps, dtype=np.float64)
+    occ = np.ascontiguousarray(occ, dtype=np.float64)
+    check(lib().gansu_write_molden(mol.handle, ptr(C), ptr(eps), ptr(occ), C.shape[1], str(path).encode()))
+
+
 def rhf(mol, params, aux=None):
     """Run the RHF SCF; returns dict(e_elec, e_total, iterations, fock_builds, converged, D)."""
     en = np.zeros(2)
@@ -270,5 +291,5 @@ def rhf(mol, params, aux=None):
                 converged=bool(info[2]), D=D)
 
 
-__all__ = ["FockEngine", "Parameters", "oneint", "rhf", "uhf", "rohf", "oneint_gradient", "nuclear_repulsion_gradient",
+__all__ = ["FockEngine", "Parameters", "oneint", "rhf", "uhf", "rohf", "rhf_orbitals", "write_molden", "oneint_gradient", "nuclear_repulsion_gradient",
            "rhf_gradient", "PRIM_DTYPE"]

@@ -184,6 +184,17 @@ int gansu_oneint(const gansu_molecule* mol, double* d_S, double* d_T, double* d_
  * converged, 0}.                                                            */
 int gansu_rhf_run(const gansu_molecule* mol, const gansu_molecule* aux_mol, const gansu_params* p, double* energies,
                   double* D_out, int* info);
+/* gansu_rhf_run returning the final orbitals: C_out (host, M*M row-major,
+ * MOs in columns, those of the converged density) and eps_out (host, M,
+ * ascending orbital energies); either may be NULL.                          */
+int gansu_rhf_run_mo(const gansu_molecule* mol, const gansu_molecule* aux_mol, const gansu_params* p,
+                     double* energies, double* C_out, double* eps_out, int* info);
+/* Molden file of orbitals C (host, M x n_mo row-major, MOs in columns) with
+ * energies eps[n_mo] and occupations occ[n_mo], in the layout of
+ * RHF::export_molden_file (rhf.cu:363-425): [Atoms] (AU), [GTO] per atom and
+ * contracted shell (raw contraction coefficients), [MO] blocks.             */
+int gansu_write_molden(const gansu_molecule* mol, const double* C, const double* eps, const double* occ, int n_mo,
+                       const char* path);
 /* UHF SCF loop (uhf.cu:118-300 via hf.cu:245-314): core or GWH guess, per-spin
  * densities D_s = C_s,occ C_s,occ^T over the molecule's n_alpha / n_beta
  * electrons, E = 1/2 sum_s D_s.(H + F_s), DIIS per spin
