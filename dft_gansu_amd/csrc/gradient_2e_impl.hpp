@@ -210,11 +210,11 @@ __global__ __launch_bounds__(kGradThreads) void k_grad_2e(ClassLaunch c, Grad2eA
     const SPair ket = g.sp[ket_id];
     const bool active = in_block && (!KSPLIT || ikp < ket.nprim);
     double gA[3] = {0.0, 0.0, 0.0}, gB[3] = {0.0, 0.0, 0.0}, gC[3] = {0.0, 0.0, 0.0};
-    // runtime-indexed body: the bra primitive pair's E tables are wave-uniform,
-    // so three lanes build them once into the wave's LDS slot (read back by
-    // every lane instead of 64 private copies in scratch)
-    [[maybe_unused]] double* s_Eb = s_grad + 3 * g.natoms + wave * 3 * NEB;
-    if constexpr (!kGradUnrolled<LA, LB, LC, LD>) {
+    // the bra primitive pair's E tables are wave-uniform, so three lanes
+    // build them once into the wave's LDS slot and every lane reads them
+    // there (instead of 64 private copies in scratch or registers)
+    double* s_Eb = s_grad + 3 * g.natoms + wave * 3 * NEB;
+    {
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
       if (lane < 3) {
@@ -260,9 +260,7 @@ __global__ __launch_bounds__(kGradThreads) void k_grad_2e(ClassLaunch c, Grad2eA
       if constexpr (kGradUnrolled<LA, LB, LC, LD>) {
         const PairRec pb = g.pr[bra.prim_begin + ibp];
         const double exa = g.alpha[bra.prim_begin + ibp], exb = pb.p - exa;
-        double Eb[3][LA + 2][LB + 2][LAB + 3];
-#pragma unroll
-        for (int d = 0; d < 3; ++d) hermite_e_t<LA + 1, LB + 1, double>(pb.p, pb.PA[d], pb.PB[d], Eb[d]);
+        const auto& Eb = *reinterpret_cast<const double(*)[3][LA + 2][LB + 2][LAB + 3]>(s_Eb);
         for (int ik = 0; ik < ket.nprim; ++ik) {
           const PairRec pk = g.pr[ket.prim_begin + ik];
           if (pb.Q * pk.Q < g.thr) continue;
@@ -474,8 +472,7 @@ hipError_t launch_grad_2e_t(const ClassLaunch& c, const Grad2eArgs& a, int maxnp
                          kGradLanesPerQuartet<LA, LB, LC, LD>;
   const int grid = (int)std::min<long long>(2048, (nvirt + waves - 1) / waves);
   constexpr int NEB = (LA + 2) * (LB + 2) * (LA + LB + 3);
-  const size_t lds = (3 * (size_t)a.natoms + (kGradUnrolled<LA, LB, LC, LD> ? 0 : (size_t)waves * 3 * NEB)) *
-                     sizeof(double);
+  const size_t lds = (3 * (size_t)a.natoms + (size_t)waves * 3 * NEB) * sizeof(double);
   hipLaunchKernelGGL((k_grad_2e<LA, LB, LC, LD>), dim3(grid), dim3(kGradThreads), lds, s, c, a, maxnp, maxnk);
   return hipGetLastError();
 }
