@@ -92,6 +92,16 @@ int gansu_molecule_get(const gansu_molecule* mol, gansu_primitive_shell* prims, 
 double gansu_molecule_nuclear_repulsion(const gansu_molecule* mol) { return mol ? mol->m.nuclear_repulsion() : 0.0; }
 void gansu_molecule_destroy(gansu_molecule* mol) { delete mol; }
 
+int gansu_sad_density(const gansu_molecule* mol, double* Da, double* Db) {
+  return guarded([&] {
+    if (!mol || !Da || !Db) throw gansu::Error(GANSU_E_ARG, "null argument");
+    std::vector<double> a, b;
+    gansu::sad_density(mol->m, a, b);
+    std::memcpy(Da, a.data(), a.size() * sizeof(double));
+    std::memcpy(Db, b.data(), b.size() * sizeof(double));
+  });
+}
+
 int gansu_params_create(gansu_params** out) {
   return guarded([&] {
     if (!out) throw gansu::Error(GANSU_E_ARG, "null argument");

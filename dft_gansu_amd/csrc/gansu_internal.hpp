@@ -34,8 +34,15 @@ struct Molecule {
   std::vector<double> norm;                   // per basis function
   int nbasis = 0;
   int n_electrons = 0, n_alpha = 0, n_beta = 0;
+  std::string gbs_path;                       // basis file the molecule was built from
+  std::vector<int> atom_basis;                // first basis function of each atom, natoms + 1 entries
   double nuclear_repulsion() const;
 };
+
+// Superposition-of-atomic-densities guess (utils.cpp:21-58, rhf.hpp:551-584):
+// per-spin atomic densities read from the .sad file beside the basis file,
+// placed on the diagonal blocks of row-major M x M matrices Da and Db.
+void sad_density(const Molecule& m, std::vector<double>& Da, std::vector<double>& Db);
 
 Molecule build_molecule(const std::vector<Atom>& atoms, const std::string& gbs, int charge, int beta_to_alpha,
                         int normalization);

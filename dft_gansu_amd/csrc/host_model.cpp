@@ -193,9 +193,11 @@ Molecule build_molecule(const std::vector<Atom>& atoms, const std::string& gbs, 
   GbsLibrary lib = read_gbs(gbs);
   Molecule m;
   m.atoms = atoms;
+  m.gbs_path = gbs;
   size_t basis = 0;
   for (size_t ai = 0; ai < atoms.size(); ++ai) {
     const Atom& at = atoms[ai];
+    m.atom_basis.push_back((int)basis);
     if (at.Z < 1 || at.Z > kNumElements) throw Error(GANSU_E_ARG, "bad atomic number");
     auto it = lib.find(kElements[at.Z - 1]);
     if (it == lib.end()) throw Error(GANSU_E_BASIS, std::string("Element not in basis set: ") + kElements[at.Z - 1]);
@@ -217,6 +219,7 @@ Molecule build_molecule(const std::vector<Atom>& atoms, const std::string& gbs, 
     }
   }
   m.nbasis = (int)basis;
+  m.atom_basis.push_back((int)basis);
   // Same comparator and algorithm as molecular.hpp:173 (std::sort, not stable):
   // under libstdc++ the resulting order is the reference's order.
   std::sort(m.prims.begin(), m.prims.end(),
@@ -234,6 +237,73 @@ Molecule build_molecule(const std::vector<Atom>& atoms, const std::string& gbs, 
   m.n_beta = ne / 2 - beta_to_alpha;
   if (m.n_beta < 0) throw Error(GANSU_E_ELECTRONS, "The number of beta-spin electrons is less than zero.");
   return m;
+}
+
+namespace {
+
+// One element's block of a .sad file: a header line "<symbol> <n>" followed by
+// the n x n alpha and then the n x n beta density, row-major, whitespace-separated.
+bool read_sad_element(const std::string& path, const std::string& symbol, int& n, std::vector<double>& a,
+                      std::vector<double>& b) {
+  std::ifstream in(path);
+  if (!in) throw Error(GANSU_E_IO, "Failed to open SAD file: " + path);
+  std::string line;
+  while (std::getline(in, line)) {
+    std::istringstream hs(line);
+    std::string tag;
+    int cnt = 0;
+    if (!(hs >> tag >> cnt) || cnt <= 0) continue;
+    if (tag != symbol) {
+      for (int i = 0; i < 2 * cnt; ++i) std::getline(in, line);
+      continue;
+    }
+    n = cnt;
+    a.resize((size_t)n * n);
+    b.resize((size_t)n * n);
+    for (double& v : a)
+      if (!(in >> v)) throw Error(GANSU_E_IO, "truncated SAD block for " + symbol + " in " + path);
+    for (double& v : b)
+      if (!(in >> v)) throw Error(GANSU_E_IO, "truncated SAD block for " + symbol + " in " + path);
+    return true;
+  }
+  return false;
+}
+
+}  // namespace
+
+void sad_density(const Molecule& m, std::vector<double>& Da, std::vector<double>& Db) {
+  const size_t M = (size_t)m.nbasis;
+  if (m.atom_basis.size() != m.atoms.size() + 1) throw Error(GANSU_E_STATE, "molecule has no per-atom basis ranges");
+  // the .sad file sits beside the basis file, with the extension replaced
+  std::string path = m.gbs_path;
+  const size_t slash = path.find_last_of('/');
+  const size_t dot = path.find_last_of('.');
+  if (dot != std::string::npos && (slash == std::string::npos || dot > slash)) path.resize(dot);
+  path += ".sad";
+  Da.assign(M * M, 0.0);
+  Db.assign(M * M, 0.0);
+  struct Block { int n; std::vector<double> a, b; };
+  std::map<int, Block> cache;  // per element
+  for (size_t i = 0; i < m.atoms.size(); ++i) {
+    const int Z = m.atoms[i].Z;
+    auto it = cache.find(Z);
+    if (it == cache.end()) {
+      Block blk{0, {}, {}};
+      if (!read_sad_element(path, kElements[Z - 1], blk.n, blk.a, blk.b))
+        throw Error(GANSU_E_BASIS, std::string("Element not found in SAD file: ") + kElements[Z - 1]);
+      it = cache.emplace(Z, std::move(blk)).first;
+    }
+    const Block& blk = it->second;
+    const int start = m.atom_basis[i], nat = m.atom_basis[i + 1] - start;
+    if (blk.n != nat)
+      throw Error(GANSU_E_BASIS, std::string("SAD block of ") + kElements[Z - 1] + " has " + std::to_string(blk.n) +
+                                     " functions, the basis gives the atom " + std::to_string(nat));
+    for (int p = 0; p < nat; ++p)
+      for (int q = 0; q < nat; ++q) {
+        Da[(size_t)(start + p) * M + start + q] = blk.a[(size_t)p * nat + q];
+        Db[(size_t)(start + p) * M + start + q] = blk.b[(size_t)p * nat + q];
+      }
+  }
 }
 
 std::vector<CShell> contracted_shells(const gansu_primitive_shell* prims, size_t n, int max_l) {

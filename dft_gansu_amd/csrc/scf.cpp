@@ -104,6 +104,13 @@ std::vector<double> solve_small(int n, std::vector<double> A, std::vector<double
   return b;
 }
 
+// initial_guess values of rhf.cu:127-145 that the drivers provide ("density",
+// a restart from caller densities, is not)
+void check_guess(const std::string& guess) {
+  if (guess != "core" && guess != "gwh" && guess != "sad")
+    throw Error(GANSU_E_UNSUPPORTED, "initial_guess=" + guess + " is not provided (core, gwh and sad only)");
+}
+
 }  // namespace
 
 OneIntHost::OneIntHost(const Molecule& mol) : M(mol.nbasis), natoms((int)mol.atoms.size()) {
@@ -203,8 +210,7 @@ ScfResult rhf_run(const Molecule& mol, const Molecule* aux, const Params& prm, b
   const int nocc = mol.n_electrons / 2;
   if (nocc > n) throw Error(GANSU_E_ELECTRONS, "more occupied orbitals than basis functions");
   const std::string guess = prm.get("initial_guess");
-  if (guess != "core" && guess != "gwh")
-    throw Error(GANSU_E_UNSUPPORTED, "initial_guess=" + guess + " is not provided (core and gwh only)");
+  check_guess(guess);
   const std::string conv_method = prm.get("convergence_method");
   if (conv_method != "diis" && conv_method != "damping" && conv_method != "optimaldamping")
     throw Error(GANSU_E_PARAM, "Unknown convergence method: " + conv_method);
@@ -269,8 +275,10 @@ ScfResult rhf_run(const Molecule& mol, const Molecule* aux, const Params& prm, b
                              n, w.C.ptr, n, &zero, w.D.ptr, n),
                "rocblas_dgemm(D)");
   };
-  auto fock = [&]() {
-    eng.build_jk(w.D.ptr, nullptr, nullptr, w.H.ptr, w.F.ptr, 0, w.C.ptr, nocc);
+  // occ: D = 2 C_occ C_occ^T holds (every build but the SAD guess's first),
+  // so the RI exchange may take its occupied form
+  auto fock = [&](bool occ = true) {
+    eng.build_jk(w.D.ptr, nullptr, nullptr, w.H.ptr, w.F.ptr, 0, occ ? w.C.ptr : nullptr, occ ? nocc : 0);
     ++res.fock_builds;
     if (ri) {  // rhf.hpp:665-674 checks F for NaN on the RI path
       DevBuf<int> flag(1);
@@ -286,11 +294,22 @@ ScfResult rhf_run(const Molecule& mol, const Molecule* aux, const Params& prm, b
 
   const double ts3 = wall();
   DevBuf<double> guess_F;
-  if (guess == "gwh") {
-    guess_F.alloc(nn);
-    GANSU_HIP(launch_gwh_guess(w.H.ptr, w.S.ptr, n, guess_F.ptr, 0));
+  if (guess == "sad") {
+    // D = Da + Db of the atoms, F(D), then C, D and F as after any guess
+    // (InitialGuess_RHF_SAD::guess, rhf.hpp:551-584)
+    std::vector<double> da, db;
+    sad_density(mol, da, db);
+    for (size_t i = 0; i < nn; ++i) da[i] += db[i];
+    GANSU_HIP(hipMemcpy(w.D.ptr, da.data(), nn * sizeof(double), hipMemcpyHostToDevice));
+    fock(false);
+    coefficients(w.F.ptr);
+  } else {
+    if (guess == "gwh") {
+      guess_F.alloc(nn);
+      GANSU_HIP(launch_gwh_guess(w.H.ptr, w.S.ptr, n, guess_F.ptr, 0));
+    }
+    coefficients(guess == "gwh" ? guess_F.ptr : w.H.ptr);
   }
-  coefficients(guess == "gwh" ? guess_F.ptr : w.H.ptr);
   density();
   fock();
   if (timing)
@@ -416,8 +435,7 @@ UhfResult uhf_run(const Molecule& mol, const Molecule* aux, const Params& prm) {
   if (na > n || nb > n)
     throw Error(GANSU_E_ELECTRONS, "more alpha or beta electrons than basis functions");
   const std::string guess = prm.get("initial_guess");
-  if (guess != "core" && guess != "gwh")
-    throw Error(GANSU_E_UNSUPPORTED, "initial_guess=" + guess + " is not provided (core and gwh only)");
+  check_guess(guess);
   if (prm.get("convergence_method") != "diis")
     throw Error(GANSU_E_UNSUPPORTED, "UHF is provided with convergence_method=diis");
   const int maxiter = prm.geti("maxiter");
@@ -479,13 +497,30 @@ UhfResult uhf_run(const Molecule& mol, const Molecule* aux, const Params& prm) {
     return 0.5 * e;
   };
   DevBuf<double> guess_F;
-  if (guess == "gwh") {
-    guess_F.alloc(nn);
-    GANSU_HIP(launch_gwh_guess(w.H.ptr, w.S.ptr, n, guess_F.ptr, 0));
-  }
-  for (int sp = 0; sp < 2; ++sp) {
-    coefficients(guess == "gwh" ? guess_F.ptr : w.H.ptr, C[sp]);
-    density(sp);
+  if (guess == "sad") {
+    // atomic D_a, D_b, F(D), C per spin, then the symmetry break of
+    // InitialGuess_UHF_SAD (uhf.hpp:429-436, :668-705): the last row and the
+    // last column of C_beta are zeroed before the densities are formed
+    std::vector<double> dh[2];
+    sad_density(mol, dh[0], dh[1]);
+    for (int sp = 0; sp < 2; ++sp)
+      GANSU_HIP(hipMemcpy(D[sp], dh[sp].data(), nn * sizeof(double), hipMemcpyHostToDevice));
+    fock();
+    for (int sp = 0; sp < 2; ++sp) coefficients(F[sp], C[sp]);
+    std::vector<double> cb(nn);
+    GANSU_HIP(hipMemcpy(cb.data(), C[1], nn * sizeof(double), hipMemcpyDeviceToHost));
+    for (int i = 0; i < n; ++i) cb[(size_t)(n - 1) * n + i] = cb[(size_t)i * n + n - 1] = 0.0;
+    GANSU_HIP(hipMemcpy(C[1], cb.data(), nn * sizeof(double), hipMemcpyHostToDevice));
+    for (int sp = 0; sp < 2; ++sp) density(sp);
+  } else {
+    if (guess == "gwh") {
+      guess_F.alloc(nn);
+      GANSU_HIP(launch_gwh_guess(w.H.ptr, w.S.ptr, n, guess_F.ptr, 0));
+    }
+    for (int sp = 0; sp < 2; ++sp) {
+      coefficients(guess == "gwh" ? guess_F.ptr : w.H.ptr, C[sp]);
+      density(sp);
+    }
   }
   fock();
   DevBuf<double> errs[2], focks[2];
@@ -560,8 +595,11 @@ UhfResult rohf_run(const Molecule& mol, const Molecule* aux, const Params& prm) 
   const int n = mol.nbasis, nc = mol.n_beta, no = mol.n_alpha - mol.n_beta;
   if (no < 0 || nc + no > n) throw Error(GANSU_E_ELECTRONS, "invalid closed/open orbital counts");
   const std::string guess = prm.get("initial_guess");
-  if (guess != "core" && guess != "gwh")
-    throw Error(GANSU_E_UNSUPPORTED, "initial_guess=" + guess + " is not provided (core and gwh only)");
+  check_guess(guess);
+  // InitialGuess_ROHF_SAD (rohf.hpp:600-640) forms the unified Fock matrix
+  // S C R C^T S before anything has written C, so it has no defined result
+  if (guess == "sad")
+    throw Error(GANSU_E_UNSUPPORTED, "initial_guess=sad is not provided for ROHF (core and gwh only)");
   if (prm.get("convergence_method") != "diis")
     throw Error(GANSU_E_UNSUPPORTED, "ROHF is provided with convergence_method=diis");
   RohfCoupling cp{};

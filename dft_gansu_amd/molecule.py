@@ -77,6 +77,15 @@ class Molecule:
     def nuclear_repulsion(self):
         return lib().gansu_molecule_nuclear_repulsion(self._h)
 
+    def sad_density(self):
+        """(Da, Db): the atomic per-spin densities of the .sad file beside the
+        basis file on the diagonal atom blocks (read_density_from_sad,
+        utils.cpp:21-58; placement as rhf.hpp:551-584), M x M each."""
+        Da = np.zeros((self.n_basis, self.n_basis))
+        Db = np.zeros((self.n_basis, self.n_basis))
+        check(lib().gansu_sad_density(self._h, ptr(Da), ptr(Db)))
+        return Da, Db
+
     def __del__(self):
         h = getattr(self, "_h", None)
         if h is not None and h.value:

@@ -87,6 +87,15 @@ int gansu_molecule_get(const gansu_molecule* mol, gansu_primitive_shell* prims, 
                        double* norm, int* Z, double* xyz_bohr);
 double gansu_molecule_nuclear_repulsion(const gansu_molecule* mol);
 void gansu_molecule_destroy(gansu_molecule* mol);
+/* Superposition-of-atomic-densities guess densities (replaces
+ * read_density_from_sad, utils.cpp:21-58, and the block placement of
+ * InitialGuess_RHF_SAD::guess, rhf.hpp:551-584): the per-spin atomic
+ * densities of the .sad file beside the molecule's basis file (same name,
+ * extension .sad) on the diagonal atom blocks of Da and Db (host, M*M
+ * row-major, zero elsewhere).  Host only.  GANSU_E_IO when the file is
+ * missing, GANSU_E_BASIS when an element is absent from it or its block size
+ * differs from the atom's basis.                                           */
+int gansu_sad_density(const gansu_molecule* mol, double* Da, double* Db);
 
 /* ----------------------------------------------------------- parameters --
  * Same keys, defaults and recipe syntax as ParameterManager
@@ -178,7 +187,7 @@ void gansu_fock_destroy(gansu_fock_engine* e);
  * S, T, V on the device (M x M row-major, caller-owned device pointers).    */
 int gansu_oneint(const gansu_molecule* mol, double* d_S, double* d_T, double* d_V, void* stream);
 
-/* RHF SCF loop of hf.cu:245-314 (core or GWH guess, DIIS/damping as the recipe
+/* RHF SCF loop of hf.cu:245-314 (core, GWH or SAD guess, DIIS/damping as the recipe
  * says) with this engine as the ERI strategy.  energies[2] = {E_electronic, E_total};
  * D_out (host, M*M) may be NULL.  info[4] = {iterations, fock builds,
  * converged, 0}.                                                            */

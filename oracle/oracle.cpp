@@ -764,10 +764,12 @@ double orc_optimal_damping(int n, const double* F, const double* Fp, const doubl
 int orc_ri_jk(int Maux, int M, const double* B, const double* D, double* J, double* K);
 
 // riB (Maux x M x M, B^P_mn) selects RI-J/K Fock builds (ERI_RI_RHF, rhf.hpp:650).
-int orc_rhf(int M, int nocc, const double* S, const double* H, const double* eri, const void* vprims,
-            const void* vtypes, int ntypes, const double* norm, const double* Q, double thr, double enuc, int maxiter,
-            double conv, int diis_size, int diis_include_transform, double* out_energy, double* out_D,
-            double* out_F, const double* riB, int Maux) {
+// D0 (M x M, may be NULL) replaces the core guess: F(D0), then C, D and F
+// (InitialGuess_RHF_SAD::guess, rhf.hpp:551-584, with D0 = D_alpha + D_beta).
+static int rhf_impl(int M, int nocc, const double* S, const double* H, const double* eri, const void* vprims,
+                    const void* vtypes, int ntypes, const double* norm, const double* Q, double thr, double enuc,
+                    int maxiter, double conv, int diis_size, int diis_include_transform, double* out_energy,
+                    double* out_D, double* out_F, const double* riB, int Maux, const double* D0) {
   size_t MM = (size_t)M * M;
   std::vector<double> w(M), U(MM), X(MM), tmp(MM), Fp(MM), Cp(MM), C(MM), D(MM), F(MM), J(MM), K(MM);
   orc_eigh(M, S, w.data(), U.data());
@@ -805,8 +807,14 @@ int orc_rhf(int M, int nocc, const double* S, const double* H, const double* eri
     for (size_t i = 0; i < MM; ++i) f += D[i] * F[i];
     return 0.5 * (e + f);
   };
-  // core guess (rhf.hpp:384-396)
-  coeffs(H); density(); fock();
+  if (D0) {
+    std::copy(D0, D0 + MM, D.begin());
+    fock();
+    coeffs(F.data());
+  } else {
+    coeffs(H);  // core guess (rhf.hpp:384-396)
+  }
+  density(); fock();
   std::vector<double> errs((size_t)diis_size * MM), focks((size_t)diis_size * MM);
   int diis_iter = 0;
   double prev = 1.7976931348623157e308, E = 0;
@@ -853,6 +861,22 @@ int orc_rhf(int M, int nocc, const double* S, const double* H, const double* eri
   if (out_D) std::memcpy(out_D, D.data(), sizeof(double) * MM);
   if (out_F) std::memcpy(out_F, F.data(), sizeof(double) * MM);
   return it;
+}
+
+int orc_rhf(int M, int nocc, const double* S, const double* H, const double* eri, const void* vprims,
+            const void* vtypes, int ntypes, const double* norm, const double* Q, double thr, double enuc, int maxiter,
+            double conv, int diis_size, int diis_include_transform, double* out_energy, double* out_D,
+            double* out_F, const double* riB, int Maux) {
+  return rhf_impl(M, nocc, S, H, eri, vprims, vtypes, ntypes, norm, Q, thr, enuc, maxiter, conv, diis_size,
+                  diis_include_transform, out_energy, out_D, out_F, riB, Maux, nullptr);
+}
+
+int orc_rhf_guess(int M, int nocc, const double* S, const double* H, const double* eri, const void* vprims,
+                  const void* vtypes, int ntypes, const double* norm, const double* Q, double thr, double enuc,
+                  int maxiter, double conv, int diis_size, int diis_include_transform, double* out_energy,
+                  double* out_D, double* out_F, const double* riB, int Maux, const double* D0) {
+  return rhf_impl(M, nocc, S, H, eri, vprims, vtypes, ntypes, norm, Q, thr, enuc, maxiter, conv, diis_size,
+                  diis_include_transform, out_energy, out_D, out_F, riB, Maux, D0);
 }
 
 // ---- RI: (P|Q), (mn|P), B = L^-1 (P|mn), RI-J/K (gpu_manager.cu:1448-1632)

@@ -144,6 +144,26 @@ def optimal_damping(F, Fp, D, Dp):
     return lib().orc_optimal_damping(ctypes.c_int(arrs[0].shape[0]), *[_ptr(a) for a in arrs])
 
 
+def read_sad(path, symbol):
+    """(Da, Db) of one element of a .sad file: a header "<symbol> <n>", then the
+    n x n alpha and n x n beta densities as whitespace-separated rows
+    (the format read_density_from_sad parses, utils.cpp:21-58)."""
+    with open(path) as fh:
+        lines = fh.read().split("\n")
+    i = 0
+    while i < len(lines):
+        tok = lines[i].split()
+        if len(tok) == 2 and not tok[0][0].isdigit() and tok[0][0] != "-":
+            n = int(tok[1])
+            if tok[0] == symbol:
+                vals = np.array(" ".join(lines[i + 1:i + 1 + 2 * n]).split(), dtype=np.float64)
+                return vals[:n * n].reshape(n, n), vals[n * n:2 * n * n].reshape(n, n)
+            i += 1 + 2 * n
+            continue
+        i += 1
+    raise KeyError(symbol)
+
+
 def nuclear_repulsion(Z, xyz_bohr):
     xyz = np.asarray(xyz_bohr, dtype=np.float64).reshape(-1, 3)
     e = 0.0
@@ -154,9 +174,10 @@ def nuclear_repulsion(Z, xyz_bohr):
 
 
 def rhf(prims, types, norm, Z, xyz_bohr, nocc, thr=1e-12, conv=1e-10, maxiter=100, diis_size=8, dense=True,
-        diis_include_transform=False, riB=None):
+        diis_include_transform=False, riB=None, D0=None):
     """Full FP64 RHF on the host.  Returns (E_elec, E_total, D, F, iterations).
-    riB (Maux x M x M) selects RI-J/K Fock builds."""
+    riB (Maux x M x M) selects RI-J/K Fock builds; D0 (M x M) replaces the
+    core guess by F(D0) (the SAD guess with D0 = Da + Db, rhf.hpp:551-584)."""
     M = len(norm)
     S, T, V = oneint(prims, norm, Z, xyz_bohr)
     H = np.ascontiguousarray(T + V)
@@ -168,12 +189,15 @@ def rhf(prims, types, norm, Z, xyz_bohr, nocc, thr=1e-12, conv=1e-10, maxiter=10
     en = np.zeros(2)
     D = np.zeros((M, M))
     F = np.zeros((M, M))
-    it = lib().orc_rhf(ctypes.c_int(M), ctypes.c_int(nocc), _ptr(S), _ptr(H),
+    if D0 is not None:
+        D0 = np.ascontiguousarray(D0, dtype=np.float64)
+    it = lib().orc_rhf_guess(ctypes.c_int(M), ctypes.c_int(nocc), _ptr(S), _ptr(H),
                        _ptr(eri) if eri is not None else None, _ptr(prims), _ptr(types), ctypes.c_int(len(types)),
                        _ptr(norm), _ptr(Q), ctypes.c_double(thr), ctypes.c_double(enuc), ctypes.c_int(maxiter),
                        ctypes.c_double(conv), ctypes.c_int(diis_size), ctypes.c_int(int(diis_include_transform)),
                        _ptr(en), _ptr(D), _ptr(F), _ptr(riB) if riB is not None else None,
-                       ctypes.c_int(riB.shape[0] if riB is not None else 0))
+                       ctypes.c_int(riB.shape[0] if riB is not None else 0),
+                       _ptr(D0) if D0 is not None else None)
     return en[0], en[1], D, F, it
 
 
@@ -264,12 +288,15 @@ def rohf_fock(H, JKc, JKo, C, S, nc, no, coupling="roothaan"):
     return Fc, Fo, S @ C @ R @ C.T @ S
 
 
-def uhf(prims, types, norm, Z, xyz_bohr, na, nb, thr=1e-12, conv=1e-10, maxiter=200, diis_size=8):
+def uhf(prims, types, norm, Z, xyz_bohr, na, nb, thr=1e-12, conv=1e-10, maxiter=200, diis_size=8, sad=None):
     """FP64 UHF on the host from the oracle's integrals and J/K (uhf.cu:118-300):
     core guess, X = U s^-1/2 (eigenvalues of S below 1e-6 dropped), per-spin
     D_s = C_s,occ C_s,occ^T, F_s = H + J[Da + Db] - K[D_s],
     E = 1/2 sum_s D_s.(H + F_s), convergence on |dE|, DIIS per spin with the
-    error F_s D_s S - S D_s F_s.  Returns (E_elec, E_total, Da, Db, iterations)."""
+    error F_s D_s S - S D_s F_s.  Returns (E_elec, E_total, Da, Db, iterations).
+    sad = (Da0, Db0) replaces the core guess by InitialGuess_UHF_SAD
+    (uhf.hpp:668-705): F(Da0, Db0), C per spin, the last row and last column
+    of C_beta zeroed (break_symmetry, uhf.hpp:429-436), then D and F."""
     S, T, V = oneint(prims, norm, Z, xyz_bohr)
     H = T + V
     Q = schwarz(prims, types)
@@ -289,7 +316,14 @@ def uhf(prims, types, norm, Z, xyz_bohr, na, nb, thr=1e-12, conv=1e-10, maxiter=
         Jb, Kb = jk(prims, types, norm, Q, Db, thr)
         return H + Ja + Jb - Ka, H + Ja + Jb - Kb
 
-    Da, Db = dens(coeff(H), na), dens(coeff(H), nb)
+    if sad is None:
+        Da, Db = dens(coeff(H), na), dens(coeff(H), nb)
+    else:
+        Fa, Fb = fock(*sad)
+        Ca, Cb = coeff(Fa), coeff(Fb).copy()
+        Cb[-1, :] = 0.0
+        Cb[:, -1] = 0.0
+        Da, Db = dens(Ca, na), dens(Cb, nb)
     Fa, Fb = fock(Da, Db)
     hist = ([], [])
     prev, E, it = None, 0.0, 0

@@ -437,3 +437,41 @@ def test_gwh_guess_reaches_the_core_guess_solution(cuda):
     for run, m in ((rhf, mol), (uhf, cation)):
         e = [run(m, Parameters(convergence_energy_threshold=1e-10, initial_guess=g))["e_total"] for g in ("core", "gwh")]
         assert abs(e[0] - e[1]) < (1e-9 if run is rhf else 1e-6), (run.__name__, e)
+
+
+def test_sad_guess_scf_matches_oracle(cuda):
+    """initial_guess=sad (rhf.hpp:551-584, uhf.hpp:668-705): RHF of water
+    (cc-pVDZ) matches the oracle's RHF started from the same atomic densities,
+    energy within 1e-9 Eh and the iteration count within one, and reaches the
+    core-guess energy; the RI engine, whose first build takes a density that is
+    not 2 C_occ C_occ^T, does too.  UHF of H2O+ from the SAD guess (C_beta's
+    last row and column zeroed) matches the oracle's SAD UHF, both converged to
+    |dE| < 1e-12 as the per-spin DIIS needs (within 1e-8 Eh).  ROHF refuses
+    the guess (the reference's ROHF SAD reads C before anything writes it)."""
+    from dft_gansu_amd import rohf, uhf
+    from dft_gansu_amd._lib import GansuError
+    Z, xyz = gm.water(jitter=0.01, seed=11)
+    path = gm.basis_path("cc-pvdz")
+    mol = gm.Molecule.from_atoms(Z, xyz, path)
+    Da, Db = mol.sad_density()
+    p, t, n = O.build_molecule(path, Z, xyz)
+    ref = O.rhf(p, t, n, Z, xyz, 5, conv=1e-10, D0=Da + Db)
+    r = rhf(mol, Parameters(convergence_energy_threshold=1e-10, initial_guess="sad"))
+    assert r["converged"]
+    assert abs(r["e_total"] - ref[1]) < 1e-9
+    assert abs(r["iterations"] - ref[4]) <= 1, (r["iterations"], ref[4])
+    core = rhf(mol, Parameters(convergence_energy_threshold=1e-10))
+    assert abs(r["e_total"] - core["e_total"]) < 1e-9
+    aux, _ = both(Z, xyz, "cc-pvdz-rifit")
+    e_ri = [rhf(mol, Parameters(convergence_energy_threshold=1e-10, eri_method="ri", initial_guess=g), aux=aux)["e_total"]
+            for g in ("core", "sad")]
+    assert abs(e_ri[0] - e_ri[1]) < 1e-9, e_ri
+    cation = gm.Molecule.from_atoms(Z, xyz, path, charge=1)
+    ca, cb = cation.sad_density()
+    refu = O.uhf(p, t, n, Z, xyz, 5, 4, conv=1e-12, sad=(ca, cb))
+    ru = uhf(cation, Parameters(convergence_energy_threshold=1e-12, maxiter=200, initial_guess="sad"))
+    assert ru["converged"]
+    assert abs(ru["e_total"] - refu[1]) < 1e-8, (ru["e_total"], refu[1])
+    with pytest.raises(GansuError) as e:
+        rohf(cation, Parameters(initial_guess="sad"))
+    assert e.value.code == 9
