@@ -26,7 +26,7 @@ ERRORS = {1: "GANSU_E_ARG", 2: "GANSU_E_IO", 3: "GANSU_E_BASIS", 4: "GANSU_E_ELE
 EXPORTS = [
     "gansu_last_error", "gansu_version", "gansu_molecule_from_xyz", "gansu_molecule_from_atoms",
     "gansu_molecule_sizes", "gansu_molecule_get", "gansu_molecule_nuclear_repulsion", "gansu_molecule_destroy",
-    "gansu_sad_density",
+    "gansu_sad_density", "gansu_export_sad_cache",
     "gansu_params_create", "gansu_params_load_recipe", "gansu_params_set", "gansu_params_get", "gansu_params_destroy",
     "gansu_fock_create", "gansu_fock_precompute", "gansu_fock_build_rhf", "gansu_fock_build_jk", "gansu_fock_build_rhf_occ",
     "gansu_fock_algorithm_name", "gansu_fock_schwarz", "gansu_fock_num_primitive_pairs", "gansu_fock_stats",
@@ -71,6 +71,7 @@ def lib():
         L.gansu_molecule_nuclear_repulsion.restype = ctypes.c_double
         L.gansu_molecule_destroy.argtypes = [vp]
         L.gansu_sad_density.argtypes = [vp, vp, vp]
+        L.gansu_export_sad_cache.argtypes = [vp, vp, vp, ctypes.c_char_p]
         L.gansu_params_create.argtypes = [ctypes.POINTER(vp)]
         L.gansu_params_load_recipe.argtypes = [vp, cp]
         L.gansu_params_set.argtypes = [vp, cp, cp]

@@ -335,6 +335,36 @@ int gansu_rohf_run(const gansu_molecule* mol, const gansu_molecule* aux_mol, con
   });
 }
 
+int gansu_export_sad_cache(const gansu_molecule* mol, const gansu_molecule* aux_mol, const gansu_params* p,
+                           const char* path) {
+  return guarded([&] {
+    if (!mol || !p || !path) throw gansu::Error(GANSU_E_ARG, "null argument");
+    const std::string method = p->p.get("method");
+    const gansu::Molecule* aux = aux_mol ? &aux_mol->m : nullptr;
+    const size_t M = (size_t)mol->m.nbasis, nn = M * M;
+    std::vector<double> a(nn), b(nn);
+    // the per-spin densities of each method's export_density_matrix
+    // (rhf.cu:312-327, uhf.cu:331-343, rohf.cu:288-310)
+    if (method == "rhf") {
+      auto r = gansu::rhf_run(mol->m, aux, p->p);
+      for (size_t i = 0; i < nn; ++i) a[i] = b[i] = 0.5 * r.D[i];
+    } else if (method == "uhf") {
+      auto r = gansu::uhf_run(mol->m, aux, p->p);
+      a = r.Da;
+      b = r.Db;
+    } else if (method == "rohf") {
+      auto r = gansu::rohf_run(mol->m, aux, p->p);
+      for (size_t i = 0; i < nn; ++i) {
+        a[i] = 0.5 * r.Da[i] + r.Db[i];
+        b[i] = 0.5 * r.Da[i];
+      }
+    } else {
+      throw gansu::Error(GANSU_E_PARAM, "Unknown method: " + method);
+    }
+    gansu::write_sad_cache(path, (int)M, a, b);
+  });
+}
+
 int gansu_rhf_gradient(const gansu_molecule* mol, const gansu_molecule* aux_mol, const gansu_params* p,
                        double* energies, double* grad, int* info) {
   return guarded([&] {

@@ -43,6 +43,9 @@ struct Molecule {
 // per-spin atomic densities read from the .sad file beside the basis file,
 // placed on the diagonal blocks of row-major M x M matrices Da and Db.
 void sad_density(const Molecule& m, std::vector<double>& Da, std::vector<double>& Db);
+// The solved densities in the cache layout of HF::generate_sad_cache
+// (hf.hpp:372-402): "M", then the M x M alpha and beta rows.
+void write_sad_cache(const std::string& path, int M, const std::vector<double>& Da, const std::vector<double>& Db);
 
 Molecule build_molecule(const std::vector<Atom>& atoms, const std::string& gbs, int charge, int beta_to_alpha,
                         int normalization);

@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <cctype>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <fstream>
 #include <sstream>
@@ -304,6 +305,22 @@ void sad_density(const Molecule& m, std::vector<double>& Da, std::vector<double>
         Db[(size_t)(start + p) * M + start + q] = blk.b[(size_t)p * nat + q];
       }
   }
+}
+
+void write_sad_cache(const std::string& path, int M, const std::vector<double>& Da, const std::vector<double>& Db) {
+  std::ofstream out(path);
+  if (!out) throw Error(GANSU_E_IO, "Failed to open the SAD cache file: " + path);
+  out << M << '\n';
+  char buf[32];
+  for (const std::vector<double>* mat : {&Da, &Db})
+    for (int i = 0; i < M; ++i) {
+      for (int j = 0; j < M; ++j) {
+        std::snprintf(buf, sizeof buf, "%.10e ", (*mat)[(size_t)i * M + j]);
+        out << buf;
+      }
+      out << '\n';
+    }
+  if (!out) throw Error(GANSU_E_IO, "failed writing " + path);
 }
 
 std::vector<CShell> contracted_shells(const gansu_primitive_shell* prims, size_t n, int max_l) {

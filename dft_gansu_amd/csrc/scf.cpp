@@ -80,12 +80,14 @@ void syevd(rocblas_handle h, int n, const double* A, Work& w) {
 }
 
 // Gaussian elimination with partial pivoting (the getrf/getrs of the reference's DIIS).
+// Returns an empty vector when the system is singular, which happens when the
+// stored error vectors vanish exactly (a converged atom in a tiny basis).
 std::vector<double> solve_small(int n, std::vector<double> A, std::vector<double> b) {
   for (int c = 0; c < n; ++c) {
     int piv = c;
     for (int r = c + 1; r < n; ++r)
       if (std::fabs(A[r * n + c]) > std::fabs(A[piv * n + c])) piv = r;
-    if (A[piv * n + c] == 0.0) throw Error(GANSU_E_NUMERIC, "singular DIIS system");
+    if (A[piv * n + c] == 0.0) return {};  // singular: the caller skips the extrapolation
     if (piv != c) {
       for (int k = 0; k < n; ++k) std::swap(A[c * n + k], A[piv * n + k]);
       std::swap(b[c], b[piv]);
@@ -365,13 +367,15 @@ ScfResult rhf_run(const Molecule& mol, const Molecule* aux, const Params& prm, b
         }
         rhs[np] = -1.0;
         const auto c = solve_small(ns, B, rhs);
-        std::vector<const double*> mats(np);
-        for (int i = 0; i < np; ++i) mats[i] = focks.ptr + i * nn;
-        DevBuf<double> dc;
-        dc.upload(c.data(), np);
-        DevBuf<const double*> dm;
-        dm.upload(mats);
-        GANSU_HIP(launch_lincomb(np, dc.ptr, dm.ptr, nn, w.F.ptr, 0));
+        if (!c.empty()) {  // a singular system keeps F as it is
+          std::vector<const double*> mats(np);
+          for (int i = 0; i < np; ++i) mats[i] = focks.ptr + i * nn;
+          DevBuf<double> dc;
+          dc.upload(c.data(), np);
+          DevBuf<const double*> dm;
+          dm.upload(mats);
+          GANSU_HIP(launch_lincomb(np, dc.ptr, dm.ptr, nn, w.F.ptr, 0));
+        }
       }
       ++diis_iter;
     } else {
@@ -565,13 +569,15 @@ UhfResult uhf_run(const Molecule& mol, const Molecule* aux, const Params& prm) {
         }
         rhs[np] = -1.0;
         const auto c = solve_small(ns, B, rhs);
-        std::vector<const double*> mats(np);
-        for (int i = 0; i < np; ++i) mats[i] = focks[sp].ptr + i * nn;
-        DevBuf<double> dc;
-        dc.upload(c.data(), np);
-        DevBuf<const double*> dm;
-        dm.upload(mats);
-        GANSU_HIP(launch_lincomb(np, dc.ptr, dm.ptr, nn, F[sp], 0));
+        if (!c.empty()) {  // a singular system keeps F as it is
+          std::vector<const double*> mats(np);
+          for (int i = 0; i < np; ++i) mats[i] = focks[sp].ptr + i * nn;
+          DevBuf<double> dc;
+          dc.upload(c.data(), np);
+          DevBuf<const double*> dm;
+          dm.upload(mats);
+          GANSU_HIP(launch_lincomb(np, dc.ptr, dm.ptr, nn, F[sp], 0));
+        }
       }
     }
     ++diis_iter;
@@ -706,13 +712,15 @@ UhfResult rohf_run(const Molecule& mol, const Molecule* aux, const Params& prm) 
       }
       rhs[np] = -1.0;
       const auto c = solve_small(ns, B, rhs);
-      std::vector<const double*> mats(np);
-      for (int i = 0; i < np; ++i) mats[i] = focks.ptr + i * nn;
-      DevBuf<double> dc;
-      dc.upload(c.data(), np);
-      DevBuf<const double*> dm;
-      dm.upload(mats);
-      GANSU_HIP(launch_lincomb(np, dc.ptr, dm.ptr, nn, w.F.ptr, 0));
+      if (!c.empty()) {  // a singular system keeps F as it is
+        std::vector<const double*> mats(np);
+        for (int i = 0; i < np; ++i) mats[i] = focks.ptr + i * nn;
+        DevBuf<double> dc;
+        dc.upload(c.data(), np);
+        DevBuf<const double*> dm;
+        dm.upload(mats);
+        GANSU_HIP(launch_lincomb(np, dc.ptr, dm.ptr, nn, w.F.ptr, 0));
+      }
     }
     ++diis_iter;
   }

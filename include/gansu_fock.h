@@ -97,6 +97,7 @@ void gansu_molecule_destroy(gansu_molecule* mol);
  * differs from the atom's basis.                                           */
 int gansu_sad_density(const gansu_molecule* mol, double* Da, double* Db);
 
+
 /* ----------------------------------------------------------- parameters --
  * Same keys, defaults and recipe syntax as ParameterManager
  * (parameter_manager.cpp:42-88, :164-225); unknown keys are rejected.       */
@@ -187,6 +188,14 @@ void gansu_fock_destroy(gansu_fock_engine* e);
  * S, T, V on the device (M x M row-major, caller-owned device pointers).    */
 int gansu_oneint(const gansu_molecule* mol, double* d_S, double* d_T, double* d_V, void* stream);
 
+/* Solves the molecule with the params' method (rhf, uhf or rohf; for a SAD
+ * cache, a single atom) and writes its per-spin densities to path in the
+ * layout of HF::generate_sad_cache (hf.hpp:372-402, the --export_sad_cache
+ * option of HF_main.cu:55-59): a line "M", then M rows of alpha and M rows of
+ * beta values "%.10e ".  The spin densities are those of export_density_matrix:
+ * RHF D/2 and D/2; UHF Da, Db; ROHF Dc/2 + Do and Dc/2.                     */
+int gansu_export_sad_cache(const gansu_molecule* mol, const gansu_molecule* aux_mol, const gansu_params* p,
+                           const char* path);
 /* RHF SCF loop of hf.cu:245-314 (core, GWH or SAD guess, DIIS/damping as the recipe
  * says) with this engine as the ERI strategy.  energies[2] = {E_electronic, E_total};
  * D_out (host, M*M) may be NULL.  info[4] = {iterations, fock builds,

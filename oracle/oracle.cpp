@@ -848,11 +848,12 @@ static int rhf_impl(int M, int nocc, const double* S, const double* H, const dou
         B[i * ns + np] = B[np * ns + i] = -1.0;
       }
       rhs[np] = -1.0;
-      solve_lin(ns, B, rhs);
-      for (size_t k = 0; k < MM; ++k) {
-        double s = 0;
-        for (int i = 0; i < np; ++i) s += rhs[i] * focks[i * MM + k];
-        F[k] = s;
+      if (solve_lin(ns, B, rhs) == 0) {  // a singular system keeps F as it is
+        for (size_t k = 0; k < MM; ++k) {
+          double s = 0;
+          for (int i = 0; i < np; ++i) s += rhs[i] * focks[i * MM + k];
+          F[k] = s;
+        }
       }
     }
     ++diis_iter;

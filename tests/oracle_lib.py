@@ -348,8 +348,11 @@ def uhf(prims, types, norm, Z, xyz_bohr, na, nb, thr=1e-12, conv=1e-10, maxiter=
                         B[a, b] = np.sum(hs[a][0] * hs[b][0])
                 rhs = np.zeros(n + 1)
                 rhs[n] = -1.0
-                c = np.linalg.solve(B, rhs)
-                F = sum(c[k] * hs[k][1] for k in range(n))
+                try:  # a singular system keeps F as it is
+                    c = np.linalg.solve(B, rhs)
+                    F = sum(c[k] * hs[k][1] for k in range(n))
+                except np.linalg.LinAlgError:
+                    pass
             new.append(F)
         Fa, Fb = new
     return E, E + nuclear_repulsion(Z, xyz_bohr), Da, Db, it
@@ -403,8 +406,11 @@ def rohf(prims, types, norm, Z, xyz_bohr, nc, no, coupling="roothaan", thr=1e-12
                     B[a, b] = np.sum(hist[a][0] * hist[b][0])
             rhs = np.zeros(n + 1)
             rhs[n] = -1.0
-            c = np.linalg.solve(B, rhs)
-            F = sum(c[k] * hist[k][1] for k in range(n))
+            try:  # a singular system keeps F as it is
+                c = np.linalg.solve(B, rhs)
+                F = sum(c[k] * hist[k][1] for k in range(n))
+            except np.linalg.LinAlgError:
+                pass
     return E, E + nuclear_repulsion(Z, xyz_bohr), Dc, Do, it
 
 

@@ -475,3 +475,39 @@ def test_sad_guess_scf_matches_oracle(cuda):
     with pytest.raises(GansuError) as e:
         rohf(cation, Parameters(initial_guess="sad"))
     assert e.value.code == 9
+
+
+def test_export_sad_cache_reproduces_the_shipped_blocks(cuda, tmp_path):
+    """The SAD cache export (HF::generate_sad_cache after RHF for even Z, ROHF
+    for odd Z) regenerates the STO-3G blocks of the bundled .sad file (made in
+    unit normalisation): H and Li (ROHF) within 1e-6, O (RHF; its occupied p
+    pair is one of the degenerate choices) in its s block within 1e-6 (the
+    shipped Li beta block differs by 2.1e-7 from a run converged to 1e-12: the
+    file comes from a looser stop), with
+    tr(D_s S) = 4 per spin and idempotent spin densities (D_s S D_s = D_s).
+    The written file is what the SAD guess then reads."""
+    import shutil
+    from dft_gansu_amd import sad
+    gbs = tmp_path / "sto-3g.gbs"
+    shutil.copy(gm.basis_path("sto-3g"), gbs)
+    shipped = gm.basis_path("sto-3g")[:-4] + ".sad"
+    added = sad.generate_sad_file(str(gbs), elements=["H", "Li", "O"], normalization="unit",
+                                  convergence_energy_threshold=1e-12)
+    assert added == ["H", "Li", "O"]
+    assert sad.cached_elements(str(tmp_path / "sto-3g.sad")) == ["H", "Li", "O"]
+    for sym in ("H", "Li"):
+        a, b = O.read_sad(str(tmp_path / "sto-3g.sad"), sym)
+        ra, rb = O.read_sad(shipped, sym)
+        assert np.abs(a - ra).max() < 1e-6 and np.abs(b - rb).max() < 1e-6, sym
+    a, b = O.read_sad(str(tmp_path / "sto-3g.sad"), "O")
+    ra, rb = O.read_sad(shipped, "O")
+    assert np.abs(a[:2, :2] - ra[:2, :2]).max() < 1e-6 and np.abs(b[:2, :2] - rb[:2, :2]).max() < 1e-6
+    p, t, n = O.build_molecule(str(gbs), np.array([8]), np.zeros((1, 3)), quirk=False)
+    S = O.oneint(p, n, np.array([8]), np.zeros((1, 3)))[0]
+    for d in (a, b):
+        assert abs(np.trace(d @ S) - 4.0) < 1e-8
+        assert np.abs(d @ S @ d - d).max() < 1e-8
+    Z, xyz = gm.water(jitter=0.0)
+    mol = gm.Molecule.from_atoms(Z, xyz, str(gbs), normalization="unit")
+    Da, _ = mol.sad_density()
+    assert np.array_equal(Da[:5, :5], a)

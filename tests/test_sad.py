@@ -99,3 +99,16 @@ def test_sad_errors(tmp_path):
         mol.sad_density()
     assert e.value.code == 3 and "O" in str(e.value)
     assert os.path.exists(tmp_path / "sto-3g.sad")
+
+
+def test_sad_file_helpers():
+    """Element discovery of the .gbs and .sad files (generate_sad_cache.py's
+    extract_elements_from_basis_file / cache_has_element)."""
+    from dft_gansu_amd import sad
+    path = gm.basis_path("sto-3g")
+    in_basis = sad.basis_elements(path)
+    cached = sad.cached_elements(path[:-4] + ".sad")
+    assert in_basis[:3] == ["H", "He", "Li"]
+    assert len(cached) == 54 and set(cached) <= set(in_basis)
+    for sym in ("H", "O", "Li"):
+        assert O.read_sad(path[:-4] + ".sad", sym)[0].ndim == 2
