@@ -31,10 +31,17 @@ void blas_check(rocblas_status st, const char* what) {
   if (st != rocblas_status_success) throw Error(GANSU_E_HIP, std::string(what) + ": rocblas status " + std::to_string(st));
 }
 
+// The drivers' rocBLAS/rocSOLVER handle: one per host thread, created on
+// first use and kept for the life of the process.  A fresh handle costs its
+// first rocSOLVER call tens of milliseconds (workspace set-up), which a
+// handle per SCF run paid on every run.
 struct Blas {
   rocblas_handle h = nullptr;
-  Blas() { blas_check(rocblas_create_handle(&h), "rocblas_create_handle"); }
-  ~Blas() { if (h) rocblas_destroy_handle(h); }
+  Blas() {
+    thread_local rocblas_handle cached = nullptr;
+    if (!cached) blas_check(rocblas_create_handle(&cached), "rocblas_create_handle");
+    h = cached;
+  }
   // row-major C = op(A) op(B), all n x n
   void gemm(bool ta, bool tb, int n, const double* A, const double* B, double* C, double alpha = 1.0,
             double beta = 0.0) {
@@ -105,6 +112,62 @@ std::vector<double> solve_small(int n, std::vector<double> A, std::vector<double
   }
   return b;
 }
+
+// DIIS history (rhf.hpp:298-327): the last `size` error and Fock matrices in
+// rings of slots, and the Gram matrix of the stored errors.  A step computes
+// only the new error's row of it (one dgemv and one read-back, where forming
+// every pair again cost one synchronising dot per pair), then extrapolates F
+// from the stored Fock matrices once two are held.
+struct Diis {
+  int size = 0, count = 0;
+  size_t nn = 0;
+  DevBuf<double> errs, focks, row, coef;
+  DevBuf<const double*> mats;
+  std::vector<double> gram;  // size x size, by slot
+
+  void init(int size_, size_t nn_) {
+    size = size_;
+    nn = nn_;
+    count = 0;
+    errs.alloc(nn * size);
+    focks.alloc(nn * size);
+    row.alloc(size);
+    coef.alloc(size);
+    gram.assign((size_t)size * size, 0.0);
+    std::vector<const double*> m(size);
+    for (int i = 0; i < size; ++i) m[i] = focks.ptr + i * nn;
+    mats.upload(m.data(), m.size());
+  }
+
+  // stores (e, F) and overwrites F with the extrapolated matrix
+  void step(rocblas_handle h, const double* e, double* F) {
+    const int slot = count % size;
+    GANSU_HIP(hipMemcpy(errs.ptr + slot * nn, e, nn * sizeof(double), hipMemcpyDeviceToDevice));
+    GANSU_HIP(hipMemcpy(focks.ptr + slot * nn, F, nn * sizeof(double), hipMemcpyDeviceToDevice));
+    ++count;
+    const int np = std::min(count, size);
+    // row[i] = <e_i, e_slot> over the held slots (errs as an nn x np column-major matrix)
+    const double one = 1.0, zero = 0.0;
+    blas_check(rocblas_dgemv(h, rocblas_operation_transpose, (rocblas_int)nn, np, &one, errs.ptr, (rocblas_int)nn,
+                             errs.ptr + slot * nn, 1, &zero, row.ptr, 1),
+               "rocblas_dgemv(DIIS)");
+    std::vector<double> r(np);
+    GANSU_HIP(hipMemcpy(r.data(), row.ptr, np * sizeof(double), hipMemcpyDeviceToHost));
+    for (int i = 0; i < np; ++i) gram[(size_t)slot * size + i] = gram[(size_t)i * size + slot] = r[i];
+    if (np < 2) return;
+    const int ns = np + 1;
+    std::vector<double> B((size_t)ns * ns, 0.0), rhs(ns, 0.0);
+    for (int i = 0; i < np; ++i) {
+      for (int j = 0; j < np; ++j) B[i * ns + j] = gram[(size_t)i * size + j];
+      B[i * ns + np] = B[np * ns + i] = -1.0;
+    }
+    rhs[np] = -1.0;
+    const auto c = solve_small(ns, B, rhs);
+    if (c.empty()) return;  // a singular system keeps F as it is
+    GANSU_HIP(hipMemcpy(coef.ptr, c.data(), np * sizeof(double), hipMemcpyHostToDevice));
+    GANSU_HIP(launch_lincomb(np, coef.ptr, mats.ptr, nn, F, 0));
+  }
+};
 
 // initial_guess values of rhf.cu:127-145 that the drivers provide ("density",
 // a restart from caller densities, is not)
@@ -320,10 +383,10 @@ ScfResult rhf_run(const Molecule& mol, const Molecule* aux, const Params& prm, b
                  1e3 * (ts3 - ts2), 1e3 * (wall() - ts3));
 
   // convergence state
-  DevBuf<double> errs, focks, prevF, prevD;
-  if (conv_method == "diis") { errs.alloc(nn * diis_size); focks.alloc(nn * diis_size); }
+  Diis diis;
+  DevBuf<double> prevF, prevD;
+  if (conv_method == "diis") diis.init(diis_size, nn);
   else { prevF.alloc(nn); prevD.alloc(nn); }
-  int diis_iter = 0;
   bool first_damp = true;
   double prev = std::numeric_limits<double>::max(), E = 0.0;
   int it = 0;
@@ -353,31 +416,7 @@ ScfResult rhf_run(const Molecule& mol, const Molecule* aux, const Params& prm, b
         blas.gemm(false, false, n, w.X.ptr, e, w.tmp.ptr);
         blas.gemm(false, true, n, w.tmp.ptr, w.X.ptr, e);
       }
-      const int slot = diis_iter % diis_size;
-      GANSU_HIP(hipMemcpy(errs.ptr + slot * nn, e, nn * sizeof(double), hipMemcpyDeviceToDevice));
-      GANSU_HIP(hipMemcpy(focks.ptr + slot * nn, w.F.ptr, nn * sizeof(double), hipMemcpyDeviceToDevice));
-      const int np = std::min(diis_iter + 1, diis_size);
-      if (np > 1) {
-        const int ns = np + 1;
-        std::vector<double> B((size_t)ns * ns, 0.0), rhs(ns, 0.0);
-        for (int i = 0; i < np; ++i) {
-          for (int j = i; j < np; ++j)
-            B[i * ns + j] = B[j * ns + i] = dot(errs.ptr + i * nn, errs.ptr + j * nn, nn, w.part);
-          B[i * ns + np] = B[np * ns + i] = -1.0;
-        }
-        rhs[np] = -1.0;
-        const auto c = solve_small(ns, B, rhs);
-        if (!c.empty()) {  // a singular system keeps F as it is
-          std::vector<const double*> mats(np);
-          for (int i = 0; i < np; ++i) mats[i] = focks.ptr + i * nn;
-          DevBuf<double> dc;
-          dc.upload(c.data(), np);
-          DevBuf<const double*> dm;
-          dm.upload(mats);
-          GANSU_HIP(launch_lincomb(np, dc.ptr, dm.ptr, nn, w.F.ptr, 0));
-        }
-      }
-      ++diis_iter;
+      diis.step(blas.h, e, w.F.ptr);
     } else {
       if (first_damp) {
         first_damp = false;
@@ -527,12 +566,8 @@ UhfResult uhf_run(const Molecule& mol, const Molecule* aux, const Params& prm) {
     }
   }
   fock();
-  DevBuf<double> errs[2], focks[2];
-  for (int sp = 0; sp < 2; ++sp) {
-    errs[sp].alloc(nn * diis_size);
-    focks[sp].alloc(nn * diis_size);
-  }
-  int diis_iter = 0;
+  Diis diis[2];
+  for (int sp = 0; sp < 2; ++sp) diis[sp].init(diis_size, nn);
   double prev = std::numeric_limits<double>::max(), E = 0.0;
   int it = 0;
   while (it < maxiter) {
@@ -545,8 +580,6 @@ UhfResult uhf_run(const Molecule& mol, const Molecule* aux, const Params& prm) {
     if (std::fabs(E - prev) < conv) { res.converged = 1; break; }
     ++it;
     prev = E;
-    const int slot = diis_iter % diis_size;
-    const int np = std::min(diis_iter + 1, diis_size);
     for (int sp = 0; sp < 2; ++sp) {
       double* e = w.tmp2.ptr;
       blas.gemm(false, false, n, F[sp], D[sp], w.tmp.ptr);
@@ -557,30 +590,8 @@ UhfResult uhf_run(const Molecule& mol, const Molecule* aux, const Params& prm) {
         blas.gemm(false, false, n, w.X.ptr, e, w.tmp.ptr);
         blas.gemm(false, true, n, w.tmp.ptr, w.X.ptr, e);
       }
-      GANSU_HIP(hipMemcpy(errs[sp].ptr + slot * nn, e, nn * sizeof(double), hipMemcpyDeviceToDevice));
-      GANSU_HIP(hipMemcpy(focks[sp].ptr + slot * nn, F[sp], nn * sizeof(double), hipMemcpyDeviceToDevice));
-      if (np > 1) {
-        const int ns = np + 1;
-        std::vector<double> B((size_t)ns * ns, 0.0), rhs(ns, 0.0);
-        for (int i = 0; i < np; ++i) {
-          for (int j = i; j < np; ++j)
-            B[i * ns + j] = B[j * ns + i] = dot(errs[sp].ptr + i * nn, errs[sp].ptr + j * nn, nn, w.part);
-          B[i * ns + np] = B[np * ns + i] = -1.0;
-        }
-        rhs[np] = -1.0;
-        const auto c = solve_small(ns, B, rhs);
-        if (!c.empty()) {  // a singular system keeps F as it is
-          std::vector<const double*> mats(np);
-          for (int i = 0; i < np; ++i) mats[i] = focks[sp].ptr + i * nn;
-          DevBuf<double> dc;
-          dc.upload(c.data(), np);
-          DevBuf<const double*> dm;
-          dm.upload(mats);
-          GANSU_HIP(launch_lincomb(np, dc.ptr, dm.ptr, nn, F[sp], 0));
-        }
-      }
+      diis[sp].step(blas.h, e, F[sp]);
     }
-    ++diis_iter;
   }
   GANSU_HIP(hipDeviceSynchronize());
   res.e_elec = E;
@@ -677,8 +688,8 @@ UhfResult rohf_run(const Molecule& mol, const Molecule* aux, const Params& prm) 
   coefficients(guess == "gwh" ? guess_F.ptr : w.H.ptr);
   density();
   fock();
-  DevBuf<double> errs(nn * diis_size), focks(nn * diis_size);
-  int diis_iter = 0;
+  Diis diis;
+  diis.init(diis_size, nn);
   double prev = std::numeric_limits<double>::max(), E = 0.0;
   int it = 0;
   while (it < maxiter) {
@@ -698,31 +709,7 @@ UhfResult rohf_run(const Molecule& mol, const Molecule* aux, const Params& prm) 
       blas.gemm(false, false, n, w.X.ptr, e, w.tmp.ptr);
       blas.gemm(false, true, n, w.tmp.ptr, w.X.ptr, e);
     }
-    const int slot = diis_iter % diis_size;
-    GANSU_HIP(hipMemcpy(errs.ptr + slot * nn, e, nn * sizeof(double), hipMemcpyDeviceToDevice));
-    GANSU_HIP(hipMemcpy(focks.ptr + slot * nn, w.F.ptr, nn * sizeof(double), hipMemcpyDeviceToDevice));
-    const int np = std::min(diis_iter + 1, diis_size);
-    if (np > 1) {
-      const int ns = np + 1;
-      std::vector<double> B((size_t)ns * ns, 0.0), rhs(ns, 0.0);
-      for (int i = 0; i < np; ++i) {
-        for (int j = i; j < np; ++j)
-          B[i * ns + j] = B[j * ns + i] = dot(errs.ptr + i * nn, errs.ptr + j * nn, nn, w.part);
-        B[i * ns + np] = B[np * ns + i] = -1.0;
-      }
-      rhs[np] = -1.0;
-      const auto c = solve_small(ns, B, rhs);
-      if (!c.empty()) {  // a singular system keeps F as it is
-        std::vector<const double*> mats(np);
-        for (int i = 0; i < np; ++i) mats[i] = focks.ptr + i * nn;
-        DevBuf<double> dc;
-        dc.upload(c.data(), np);
-        DevBuf<const double*> dm;
-        dm.upload(mats);
-        GANSU_HIP(launch_lincomb(np, dc.ptr, dm.ptr, nn, w.F.ptr, 0));
-      }
-    }
-    ++diis_iter;
+    diis.step(blas.h, e, w.F.ptr);
   }
   GANSU_HIP(hipDeviceSynchronize());
   res.e_elec = E;
