@@ -855,13 +855,23 @@ void k_fock_slab(SlabLaunch c, EriArgs g) {
               for (int k = 0; k < NV; ++k) vals[k] = (double)src32[(size_t)k * 64];
             }
             digest([&](int aa, int bb, int cc, int d) -> double { return vals[((aa * NB + bb) * NC + cc) * ND + d]; });
-          } else if (lo) {
-            digest([&](int aa, int bb, int cc, int d) -> double {
-              return src[(size_t)(((aa * NB + bb) * NC + cc) * ND + d) * 64];
-            });
           } else {
+            // too many for registers: the NCD integrals of one (a, b) row are
+            // loaded together at the row's first use (digest visits a row's
+            // (c, d) in order, so the branch below resolves at compile time)
+            double row[NCD];
             digest([&](int aa, int bb, int cc, int d) -> double {
-              return (double)src32[(size_t)(((aa * NB + bb) * NC + cc) * ND + d) * 64];
+              if (cc == 0 && d == 0) {
+                const size_t r0 = (size_t)(aa * NB + bb) * NCD;
+                if (lo) {
+#pragma unroll
+                  for (int k = 0; k < NCD; ++k) row[k] = src[(r0 + k) * 64];
+                } else {
+#pragma unroll
+                  for (int k = 0; k < NCD; ++k) row[k] = (double)src32[(r0 + k) * 64];
+                }
+              }
+              return row[cc * ND + d];
             });
           }
         }
