@@ -21,6 +21,7 @@
 //   orc_oneint                        int1e (S, T, V), standard MD formulas
 //   orc_rhf                           hf.cu:171-314, rhf.hpp:298-327,
 //                                     gpu_manager.cu:529-620, :827-836, :1020-1194
+//   orc_rhf_guess                     orc_rhf from a guess density (rhf.hpp:551-584)
 //   orc_int2c2e/orc_int3c2e/orc_ri_*  int2c2e.cu:468-687, int3c2e.cu:1102-1336,
 //                                     gpu_manager.cu:1311-1632
 //
