@@ -268,14 +268,20 @@ std::vector<double> nuclear_repulsion_gradient(const Molecule& mol) {
   return g;
 }
 
-ScfResult rhf_run(const Molecule& mol, const Molecule* aux, const Params& prm, bool gradient) {
+ScfResult rhf_run(const Molecule& mol, const Molecule* aux, const Params& prm, bool gradient, const double* Da0,
+                  const double* Db0) {
   if (prm.get("method") != "rhf") throw Error(GANSU_E_PARAM, "only method=rhf is provided by this engine");
   if (mol.n_electrons % 2 != 0) throw Error(GANSU_E_ELECTRONS, "RHF requires an even number of electrons");
   const int n = mol.nbasis;
   const int nocc = mol.n_electrons / 2;
   if (nocc > n) throw Error(GANSU_E_ELECTRONS, "more occupied orbitals than basis functions");
   const std::string guess = prm.get("initial_guess");
-  check_guess(guess);
+  if (guess != "density") check_guess(guess);
+  // caller densities replace the guess (HF::solve with densities,
+  // rhf.cu:127-135); initial_guess=density without them is the core guess
+  const bool from_density = Da0 && Db0;
+  if (guess == "density" && !from_density)
+    std::fprintf(stderr, "initial_guess=density without density matrices: the core Hamiltonian guess is used\n");
   const std::string conv_method = prm.get("convergence_method");
   if (conv_method != "diis" && conv_method != "damping" && conv_method != "optimaldamping")
     throw Error(GANSU_E_PARAM, "Unknown convergence method: " + conv_method);
@@ -359,24 +365,32 @@ ScfResult rhf_run(const Molecule& mol, const Molecule* aux, const Params& prm, b
 
   const double ts3 = wall();
   DevBuf<double> guess_F;
-  if (guess == "sad") {
-    // D = Da + Db of the atoms, F(D), then C, D and F as after any guess
-    // (InitialGuess_RHF_SAD::guess, rhf.hpp:551-584)
-    std::vector<double> da, db;
-    sad_density(mol, da, db);
-    for (size_t i = 0; i < nn; ++i) da[i] += db[i];
-    GANSU_HIP(hipMemcpy(w.D.ptr, da.data(), nn * sizeof(double), hipMemcpyHostToDevice));
+  if (from_density) {
+    // D = Da + Db and F(D) only (InitialGuess_RHF_Density::guess, rhf.hpp:441-454)
+    std::vector<double> d(nn);
+    for (size_t i = 0; i < nn; ++i) d[i] = Da0[i] + Db0[i];
+    GANSU_HIP(hipMemcpy(w.D.ptr, d.data(), nn * sizeof(double), hipMemcpyHostToDevice));
     fock(false);
-    coefficients(w.F.ptr);
   } else {
-    if (guess == "gwh") {
-      guess_F.alloc(nn);
-      GANSU_HIP(launch_gwh_guess(w.H.ptr, w.S.ptr, n, guess_F.ptr, 0));
+    if (guess == "sad") {
+      // D = Da + Db of the atoms, F(D), then C, D and F as after any guess
+      // (InitialGuess_RHF_SAD::guess, rhf.hpp:551-584)
+      std::vector<double> da, db;
+      sad_density(mol, da, db);
+      for (size_t i = 0; i < nn; ++i) da[i] += db[i];
+      GANSU_HIP(hipMemcpy(w.D.ptr, da.data(), nn * sizeof(double), hipMemcpyHostToDevice));
+      fock(false);
+      coefficients(w.F.ptr);
+    } else {
+      if (guess == "gwh") {
+        guess_F.alloc(nn);
+        GANSU_HIP(launch_gwh_guess(w.H.ptr, w.S.ptr, n, guess_F.ptr, 0));
+      }
+      coefficients(guess == "gwh" ? guess_F.ptr : w.H.ptr);
     }
-    coefficients(guess == "gwh" ? guess_F.ptr : w.H.ptr);
+    density();
+    fock();
   }
-  density();
-  fock();
   if (timing)
     std::fprintf(stderr, "[scf] setup: one-electron integrals + engine %.2f ms, precompute %.2f ms, X %.2f ms, "
                          "core guess + first build %.2f ms\n", 1e3 * (ts1 - ts0), 1e3 * (ts2 - ts1),

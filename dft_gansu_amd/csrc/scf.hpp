@@ -21,7 +21,10 @@ struct ScfResult {
 
 // HF::single_point_energy (hf.cu:245-314) for RHF; with gradient = true the
 // analytic nuclear gradient of the converged energy follows.
-ScfResult rhf_run(const Molecule& mol, const Molecule* aux, const Params& prm, bool gradient = false);
+// Da0/Db0 (host, M x M), when both are given, replace the guess: F(Da0 + Db0)
+// starts the loop (HF::solve with densities, rhf.cu:127-135).
+ScfResult rhf_run(const Molecule& mol, const Molecule* aux, const Params& prm, bool gradient = false,
+                  const double* Da0 = nullptr, const double* Db0 = nullptr);
 
 struct UhfResult {
   double e_elec = 0, e_total = 0;

@@ -281,12 +281,22 @@ def write_molden(mol, C, eps, occ, path):
     check(lib().gansu_write_molden(mol.handle, ptr(C), ptr(eps), ptr(occ), C.shape[1], str(path).encode()))
 
 
-def rhf(mol, params, aux=None):
-    """Run the RHF SCF; returns dict(e_elec, e_total, iterations, fock_builds, converged, D)."""
+def rhf(mol, params, aux=None, density=None):
+    """Run the RHF SCF; returns dict(e_elec, e_total, iterations, fock_builds, converged, D).
+    density = (Da, Db) (M x M each, D/2 each for a closed shell) starts the
+    loop from F(Da + Db) (HF::solve with densities, rhf.cu:127-135)."""
     en = np.zeros(2)
     D = np.zeros((mol.n_basis, mol.n_basis))
     info = np.zeros(4, dtype=np.int32)
-    check(lib().gansu_rhf_run(mol.handle, aux.handle if aux else None, params.handle, ptr(en), ptr(D), ptr(info)))
+    if density is not None:
+        Da, Db = (np.ascontiguousarray(x, dtype=np.float64) for x in density)
+        if Da.shape != D.shape or Db.shape != D.shape:
+            raise ValueError(f"densities must be {D.shape}")
+        check(lib().gansu_rhf_run_guess(mol.handle, aux.handle if aux else None, params.handle, ptr(Da), ptr(Db),
+                                        ptr(en), ptr(D), ptr(info)))
+    else:
+        check(lib().gansu_rhf_run(mol.handle, aux.handle if aux else None, params.handle, ptr(en), ptr(D),
+                                  ptr(info)))
     return dict(e_elec=float(en[0]), e_total=float(en[1]), iterations=int(info[0]), fock_builds=int(info[1]),
                 converged=bool(info[2]), D=D)
 

@@ -202,6 +202,14 @@ int gansu_export_sad_cache(const gansu_molecule* mol, const gansu_molecule* aux_
  * converged, 0}.                                                            */
 int gansu_rhf_run(const gansu_molecule* mol, const gansu_molecule* aux_mol, const gansu_params* p, double* energies,
                   double* D_out, int* info);
+/* gansu_rhf_run started from caller densities (HF::solve with density
+ * matrices, rhf.cu:127-135, InitialGuess_RHF_Density, rhf.hpp:441-454): with
+ * Da0 and Db0 (host, M*M; for RHF each D/2) the loop starts from
+ * F(Da0 + Db0), whatever initial_guess says; with both NULL it is
+ * gansu_rhf_run (initial_guess=density then falls back to the core guess, as
+ * the reference does).  One of them NULL is GANSU_E_ARG.                    */
+int gansu_rhf_run_guess(const gansu_molecule* mol, const gansu_molecule* aux_mol, const gansu_params* p,
+                        const double* Da0, const double* Db0, double* energies, double* D_out, int* info);
 /* gansu_rhf_run returning the final orbitals: C_out (host, M*M row-major,
  * MOs in columns, those of the converged density) and eps_out (host, M,
  * ascending orbital energies); either may be NULL.                          */

@@ -511,3 +511,29 @@ def test_export_sad_cache_reproduces_the_shipped_blocks(cuda, tmp_path):
     mol = gm.Molecule.from_atoms(Z, xyz, str(gbs), normalization="unit")
     Da, _ = mol.sad_density()
     assert np.array_equal(Da[:5, :5], a)
+
+
+def test_rhf_restart_from_densities(cuda):
+    """HF::solve with density matrices (rhf.cu:127-135, InitialGuess_RHF_Density,
+    rhf.hpp:441-454): the loop starts from F(Da + Db).  From the converged
+    density (D/2, D/2) the SCF stops at once on the same energy; from the SAD
+    densities it reaches the oracle's energy; initial_guess=density without
+    densities is the core guess; UHF refuses the key."""
+    from dft_gansu_amd import uhf
+    from dft_gansu_amd._lib import GansuError
+    Z, xyz = gm.water(jitter=0.01, seed=11)
+    mol, (p, t, n) = both(Z, xyz, "cc-pvdz")
+    prm = Parameters(convergence_energy_threshold=1e-10)
+    r0 = rhf(mol, prm)
+    r1 = rhf(mol, prm, density=(0.5 * r0["D"], 0.5 * r0["D"]))
+    assert r1["converged"] and r1["iterations"] <= 2, r1["iterations"]
+    assert abs(r1["e_total"] - r0["e_total"]) < 1e-9
+    Da, Db = mol.sad_density()
+    r2 = rhf(mol, prm, density=(Da, Db))
+    assert abs(r2["e_total"] - O.rhf(p, t, n, Z, xyz, 5, conv=1e-10, D0=Da + Db)[1]) < 1e-9
+    r3 = rhf(mol, Parameters(convergence_energy_threshold=1e-10, initial_guess="density"))
+    assert abs(r3["iterations"] - r0["iterations"]) <= 1 and abs(r3["e_total"] - r0["e_total"]) < 1e-9
+    cation = gm.Molecule.from_atoms(Z, xyz, gm.basis_path("cc-pvdz"), charge=1)
+    with pytest.raises(GansuError) as e:
+        uhf(cation, Parameters(initial_guess="density"))
+    assert e.value.code == 9

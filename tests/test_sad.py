@@ -112,3 +112,16 @@ def test_sad_file_helpers():
     assert len(cached) == 54 and set(cached) <= set(in_basis)
     for sym in ("H", "O", "Li"):
         assert O.read_sad(path[:-4] + ".sad", sym)[0].ndim == 2
+
+
+def test_density_restart_needs_both_densities():
+    """gansu_rhf_run_guess takes both spin densities or neither (host-side
+    argument check, no device work)."""
+    from dft_gansu_amd import Parameters
+    from dft_gansu_amd._lib import lib, ptr
+    Z, xyz = gm.water(jitter=0.0)
+    mol = gm.Molecule.from_atoms(Z, xyz, gm.basis_path("sto-3g"))
+    Da = np.zeros((mol.n_basis, mol.n_basis))
+    en = np.zeros(2)
+    rc = lib().gansu_rhf_run_guess(mol.handle, None, Parameters().handle, ptr(Da), None, ptr(en), None, None)
+    assert rc == 1 and b"both" in lib().gansu_last_error()
