@@ -257,6 +257,21 @@ int gansu_rhf_run(const gansu_molecule* mol, const gansu_molecule* aux_mol, cons
   });
 }
 
+int gansu_rhf_gradient_guess(const gansu_molecule* mol, const gansu_molecule* aux_mol, const gansu_params* p,
+                             const double* Da0, const double* Db0, double* energies, double* grad, double* D_out,
+                             int* info) {
+  return guarded([&] {
+    if (!mol || !p || !energies || !grad) throw gansu::Error(GANSU_E_ARG, "null argument");
+    if ((Da0 == nullptr) != (Db0 == nullptr)) throw gansu::Error(GANSU_E_ARG, "give both densities or neither");
+    auto r = gansu::rhf_run(mol->m, aux_mol ? &aux_mol->m : nullptr, p->p, true, Da0, Db0);
+    energies[0] = r.e_elec;
+    energies[1] = r.e_total;
+    std::memcpy(grad, r.grad.data(), r.grad.size() * sizeof(double));
+    if (D_out) std::memcpy(D_out, r.D.data(), r.D.size() * sizeof(double));
+    if (info) { info[0] = r.iterations; info[1] = r.fock_builds; info[2] = r.converged; info[3] = 0; }
+  });
+}
+
 int gansu_rhf_run_guess(const gansu_molecule* mol, const gansu_molecule* aux_mol, const gansu_params* p,
                         const double* Da0, const double* Db0, double* energies, double* D_out, int* info) {
   return guarded([&] {

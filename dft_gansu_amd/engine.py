@@ -225,15 +225,24 @@ def nuclear_repulsion_gradient(mol):
     return g
 
 
-def rhf_gradient(mol, params, aux=None):
+def rhf_gradient(mol, params, aux=None, density=None):
     """SCF then the analytic nuclear gradient; returns dict(e_elec, e_total,
-    iterations, fock_builds, converged, grad (n_atoms x 3, hartree/bohr))."""
+    iterations, fock_builds, converged, grad (n_atoms x 3, hartree/bohr), D).
+    density = (Da, Db) starts the SCF from F(Da + Db) (a previous geometry's
+    D/2, D/2, say)."""
     en = np.zeros(2)
     g = np.zeros((mol.n_atoms, 3))
+    D = np.zeros((mol.n_basis, mol.n_basis))
     info = np.zeros(4, dtype=np.int32)
-    check(lib().gansu_rhf_gradient(mol.handle, aux.handle if aux else None, params.handle, ptr(en), ptr(g), ptr(info)))
+    Da = Db = None
+    if density is not None:
+        Da, Db = (np.ascontiguousarray(x, dtype=np.float64) for x in density)
+        if Da.shape != D.shape or Db.shape != D.shape:
+            raise ValueError(f"densities must be {D.shape}")
+    check(lib().gansu_rhf_gradient_guess(mol.handle, aux.handle if aux else None, params.handle, ptr(Da), ptr(Db),
+                                         ptr(en), ptr(g), ptr(D), ptr(info)))
     return dict(e_elec=float(en[0]), e_total=float(en[1]), iterations=int(info[0]), fock_builds=int(info[1]),
-                converged=bool(info[2]), grad=g)
+                converged=bool(info[2]), grad=g, D=D)
 
 
 def uhf(mol, params, aux=None):

@@ -45,22 +45,28 @@ def gradient_params(**kv):
     return Parameters(**kv)
 
 
-def optimize_geometry(mol, params=None, gtol=3e-4, max_steps=100, max_step=0.3, callback=None):
+def optimize_geometry(mol, params=None, gtol=3e-4, max_steps=100, max_step=0.3, callback=None, warm_start=True):
     """Minimise the RHF energy over Cartesian coordinates by BFGS with a
     backtracking line search.
 
     gtol: largest gradient component at convergence (hartree/bohr).
     max_step: largest displacement of any coordinate in one step (bohr).
-    Returns dict(mol, e_total, grad, steps, converged, history).
+    warm_start: start each SCF from the last converged density (D/2, D/2),
+    as the reference's optimisers hand densities to HF::solve.
+    Returns dict(mol, e_total, grad, steps, converged, history, scf_iterations).
     """
     params = params or gradient_params()
     x = np.array(mol.xyz, dtype=np.float64).reshape(-1)
+    last = {"D": None, "iterations": 0}
 
     def evaluate(xv):
         m = mol.with_coordinates(xv.reshape(-1, 3))
-        r = rhf_gradient(m, params)
+        D = last["D"] if warm_start else None
+        r = rhf_gradient(m, params, density=None if D is None else (0.5 * D, 0.5 * D))
         if not r["converged"]:
             raise RuntimeError("SCF did not converge during the optimisation")
+        last["D"] = r["D"]
+        last["iterations"] += r["iterations"]
         return m, r["e_total"], r["grad"].reshape(-1)
 
     m, e, g = evaluate(x)
@@ -99,7 +105,8 @@ def optimize_geometry(mol, params=None, gtol=3e-4, max_steps=100, max_step=0.3, 
             callback(steps, e, g)
     else:
         converged = np.max(np.abs(g)) < gtol
-    return dict(mol=m, e_total=e, grad=g.reshape(-1, 3), steps=steps, converged=bool(converged), history=history)
+    return dict(mol=m, e_total=e, grad=g.reshape(-1, 3), steps=steps, converged=bool(converged), history=history,
+                scf_iterations=last["iterations"])
 
 
 __all__ = ["RHFEnergy", "rhf_energy", "gradient_params", "optimize_geometry"]

@@ -210,6 +210,14 @@ int gansu_rhf_run(const gansu_molecule* mol, const gansu_molecule* aux_mol, cons
  * the reference does).  One of them NULL is GANSU_E_ARG.                    */
 int gansu_rhf_run_guess(const gansu_molecule* mol, const gansu_molecule* aux_mol, const gansu_params* p,
                         const double* Da0, const double* Db0, double* energies, double* D_out, int* info);
+/* gansu_rhf_gradient started from caller densities as gansu_rhf_run_guess
+ * (both NULL: the parameters' guess), also returning the converged density
+ * (D_out, host M*M, may be NULL): a geometry optimiser passes each step's D/2,
+ * D/2 to the next step, as the reference's optimisers pass densities to
+ * HF::solve (rhf.cu:127-135).                                               */
+int gansu_rhf_gradient_guess(const gansu_molecule* mol, const gansu_molecule* aux_mol, const gansu_params* p,
+                             const double* Da0, const double* Db0, double* energies, double* grad, double* D_out,
+                             int* info);
 /* gansu_rhf_run returning the final orbitals: C_out (host, M*M row-major,
  * MOs in columns, those of the converged density) and eps_out (host, M,
  * ascending orbital energies); either may be NULL.                          */

@@ -111,6 +111,11 @@ def test_geometry_optimiser_reaches_the_sto3g_water_minimum(cuda):
     roh = [np.linalg.norm(x[k] - x[0]) for k in (1, 2)]
     assert abs(roh[0] - roh[1]) < 1e-3
     assert abs(roh[0] / gm.ANGSTROM_TO_BOHR - 0.989) < 2e-3
+    # each SCF started from the previous converged density (the default) reaches
+    # the same minimum as cold starts, in fewer SCF iterations
+    cold = optimize_geometry(mol, gtol=1e-5, warm_start=False)
+    assert cold["converged"] and abs(cold["e_total"] - out["e_total"]) < 1e-8
+    assert out["scf_iterations"] < cold["scf_iterations"], (out["scf_iterations"], cold["scf_iterations"])
 
 
 # ------------------------------------------------------------ RI gradient --
