@@ -32,7 +32,7 @@ EXPORTS = [
     "gansu_fock_algorithm_name", "gansu_fock_schwarz", "gansu_fock_num_primitive_pairs", "gansu_fock_stats",
     "gansu_fock_set_profiling", "gansu_fock_last_profile", "gansu_fock_destroy", "gansu_oneint", "gansu_rhf_run",
     "gansu_fock_gradient_rhf", "gansu_fock_gradient_rhf_occ", "gansu_oneint_gradient", "gansu_molecule_nuclear_repulsion_gradient",
-    "gansu_rhf_gradient", "gansu_rhf_run_guess", "gansu_rhf_gradient_guess", "gansu_uhf_run", "gansu_rohf_run", "gansu_rhf_run_mo", "gansu_write_molden", "gansu_fock_build_uhf", "gansu_fock_build_rohf", "gansu_fock_update_geometry",
+    "gansu_rhf_gradient", "gansu_rhf_run_guess", "gansu_rhf_gradient_guess", "gansu_uhf_run_guess", "gansu_uhf_run", "gansu_rohf_run", "gansu_rhf_run_mo", "gansu_write_molden", "gansu_fock_build_uhf", "gansu_fock_build_rohf", "gansu_fock_update_geometry",
 ]
 
 
@@ -98,6 +98,7 @@ def lib():
         L.gansu_rhf_run_mo.argtypes = [vp, vp, vp, vp, vp, vp, vp]
         L.gansu_rhf_run_guess.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp]
         L.gansu_rhf_gradient_guess.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp]
+        L.gansu_uhf_run_guess.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp]
         L.gansu_write_molden.argtypes = [vp, vp, vp, vp, ip, ctypes.c_char_p]
         L.gansu_fock_gradient_rhf.argtypes = [vp, vp, ip, vp, vp]
         L.gansu_fock_gradient_rhf_occ.argtypes = [vp, vp, vp, ip, ip, vp, vp]

@@ -350,6 +350,20 @@ int gansu_uhf_run(const gansu_molecule* mol, const gansu_molecule* aux_mol, cons
   });
 }
 
+int gansu_uhf_run_guess(const gansu_molecule* mol, const gansu_molecule* aux_mol, const gansu_params* p,
+                        const double* Da0, const double* Db0, double* energies, double* Da, double* Db, int* info) {
+  return guarded([&] {
+    if (!mol || !p || !energies) throw gansu::Error(GANSU_E_ARG, "null argument");
+    if ((Da0 == nullptr) != (Db0 == nullptr)) throw gansu::Error(GANSU_E_ARG, "give both densities or neither");
+    auto r = gansu::uhf_run(mol->m, aux_mol ? &aux_mol->m : nullptr, p->p, Da0, Db0);
+    energies[0] = r.e_elec;
+    energies[1] = r.e_total;
+    if (Da) std::memcpy(Da, r.Da.data(), r.Da.size() * sizeof(double));
+    if (Db) std::memcpy(Db, r.Db.data(), r.Db.size() * sizeof(double));
+    if (info) { info[0] = r.iterations; info[1] = r.fock_builds; info[2] = r.converged; info[3] = 0; }
+  });
+}
+
 int gansu_rohf_run(const gansu_molecule* mol, const gansu_molecule* aux_mol, const gansu_params* p, double* energies,
                    double* Dc, double* Do, int* info) {
   return guarded([&] {

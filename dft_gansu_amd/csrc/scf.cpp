@@ -487,12 +487,15 @@ ScfResult rhf_run(const Molecule& mol, const Molecule* aux, const Params& prm, b
 // Fock pair from the engine's UHF build, E = 1/2 sum_s D_s.(H + F_s),
 // convergence on |dE|, and DIIS run separately for each spin with the error
 // F_s D_s S - S D_s F_s.
-UhfResult uhf_run(const Molecule& mol, const Molecule* aux, const Params& prm) {
+UhfResult uhf_run(const Molecule& mol, const Molecule* aux, const Params& prm, const double* Da0, const double* Db0) {
   const int n = mol.nbasis, na = mol.n_alpha, nb = mol.n_beta;
   if (na > n || nb > n)
     throw Error(GANSU_E_ELECTRONS, "more alpha or beta electrons than basis functions");
   const std::string guess = prm.get("initial_guess");
-  check_guess(guess);
+  if (guess != "density") check_guess(guess);
+  const bool from_density = Da0 && Db0;
+  if (guess == "density" && !from_density)
+    std::fprintf(stderr, "initial_guess=density without density matrices: the core Hamiltonian guess is used\n");
   if (prm.get("convergence_method") != "diis")
     throw Error(GANSU_E_UNSUPPORTED, "UHF is provided with convergence_method=diis");
   const int maxiter = prm.geti("maxiter");
@@ -554,7 +557,17 @@ UhfResult uhf_run(const Molecule& mol, const Molecule* aux, const Params& prm) {
     return 0.5 * e;
   };
   DevBuf<double> guess_F;
-  if (guess == "sad") {
+  if (from_density) {
+    // caller D_a, D_b, F(D), C and D per spin (InitialGuess_UHF_Density::guess,
+    // uhf.hpp:557-568)
+    GANSU_HIP(hipMemcpy(D[0], Da0, nn * sizeof(double), hipMemcpyHostToDevice));
+    GANSU_HIP(hipMemcpy(D[1], Db0, nn * sizeof(double), hipMemcpyHostToDevice));
+    fock();
+    for (int sp = 0; sp < 2; ++sp) {
+      coefficients(F[sp], C[sp]);
+      density(sp);
+    }
+  } else if (guess == "sad") {
     // atomic D_a, D_b, F(D), C per spin, then the symmetry break of
     // InitialGuess_UHF_SAD (uhf.hpp:429-436, :668-705): the last row and the
     // last column of C_beta are zeroed before the densities are formed

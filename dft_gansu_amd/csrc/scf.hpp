@@ -32,7 +32,10 @@ struct UhfResult {
   std::vector<double> Da, Db;  // final alpha and beta densities (M x M)
 };
 // HF::single_point_energy for UHF (uhf.cu), core guess and per-spin DIIS.
-UhfResult uhf_run(const Molecule& mol, const Molecule* aux, const Params& prm);
+// Da0/Db0 (host, M x M), when both are given, replace the guess: F(Da0, Db0),
+// then C and D per spin (HF::solve with densities).
+UhfResult uhf_run(const Molecule& mol, const Molecule* aux, const Params& prm, const double* Da0 = nullptr,
+                  const double* Db0 = nullptr);
 // HF::single_point_energy for ROHF (rohf.cu): core guess, DIIS on the
 // unified Fock matrix; Da / Db of the result hold the closed / open densities.
 UhfResult rohf_run(const Molecule& mol, const Molecule* aux, const Params& prm);

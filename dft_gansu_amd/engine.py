@@ -245,14 +245,22 @@ def rhf_gradient(mol, params, aux=None, density=None):
                 converged=bool(info[2]), grad=g, D=D)
 
 
-def uhf(mol, params, aux=None):
-    """Run the UHF SCF; returns dict(e_elec, e_total, iterations, fock_builds, converged, Da, Db)."""
+def uhf(mol, params, aux=None, density=None):
+    """Run the UHF SCF; returns dict(e_elec, e_total, iterations, fock_builds, converged, Da, Db).
+    density = (Da, Db) starts from F(Da, Db) (HF::solve with densities)."""
     en = np.zeros(2)
     Da = np.zeros((mol.n_basis, mol.n_basis))
     Db = np.zeros((mol.n_basis, mol.n_basis))
     info = np.zeros(4, dtype=np.int32)
-    check(lib().gansu_uhf_run(mol.handle, aux.handle if aux else None, params.handle, ptr(en), ptr(Da), ptr(Db),
-                              ptr(info)))
+    if density is not None:
+        a0, b0 = (np.ascontiguousarray(x, dtype=np.float64) for x in density)
+        if a0.shape != Da.shape or b0.shape != Da.shape:
+            raise ValueError(f"densities must be {Da.shape}")
+        check(lib().gansu_uhf_run_guess(mol.handle, aux.handle if aux else None, params.handle, ptr(a0), ptr(b0),
+                                        ptr(en), ptr(Da), ptr(Db), ptr(info)))
+    else:
+        check(lib().gansu_uhf_run(mol.handle, aux.handle if aux else None, params.handle, ptr(en), ptr(Da),
+                                  ptr(Db), ptr(info)))
     return dict(e_elec=float(en[0]), e_total=float(en[1]), iterations=int(info[0]), fock_builds=int(info[1]),
                 converged=bool(info[2]), Da=Da, Db=Db)
 

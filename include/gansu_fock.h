@@ -237,6 +237,11 @@ int gansu_write_molden(const gansu_molecule* mol, const double* C, const double*
  * gansu_rhf_run.                                                            */
 int gansu_uhf_run(const gansu_molecule* mol, const gansu_molecule* aux_mol, const gansu_params* p, double* energies,
                   double* Da_out, double* Db_out, int* info);
+/* gansu_uhf_run started from caller spin densities Da0, Db0 (host, M*M):
+ * F(Da0, Db0), then C and D per spin (InitialGuess_UHF_Density,
+ * uhf.hpp:557-568); both NULL is gansu_uhf_run, one NULL GANSU_E_ARG.       */
+int gansu_uhf_run_guess(const gansu_molecule* mol, const gansu_molecule* aux_mol, const gansu_params* p,
+                        const double* Da0, const double* Db0, double* energies, double* Da, double* Db, int* info);
 /* ROHF SCF loop (rohf.cu:59-300 via hf.cu:245-314): n_beta closed and
  * n_alpha - n_beta open orbitals, core or GWH guess, D_c = 2 C_c C_c^T,
  * D_o = C_o C_o^T, the ROHF build with rohf_parameter_name,

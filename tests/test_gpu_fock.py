@@ -518,8 +518,10 @@ def test_rhf_restart_from_densities(cuda):
     rhf.hpp:441-454): the loop starts from F(Da + Db).  From the converged
     density (D/2, D/2) the SCF stops at once on the same energy; from the SAD
     densities it reaches the oracle's energy; initial_guess=density without
-    densities is the core guess; UHF refuses the key."""
-    from dft_gansu_amd import uhf
+    densities is the core guess.  UHF restarts from (Da, Db) the same way
+    (F(Da, Db), then C and D per spin, uhf.hpp:557-568); ROHF refuses the key
+    (the reference's ROHF density guess forms S C R C^T S from an unwritten C)."""
+    from dft_gansu_amd import rohf, uhf
     from dft_gansu_amd._lib import GansuError
     Z, xyz = gm.water(jitter=0.01, seed=11)
     mol, (p, t, n) = both(Z, xyz, "cc-pvdz")
@@ -534,6 +536,12 @@ def test_rhf_restart_from_densities(cuda):
     r3 = rhf(mol, Parameters(convergence_energy_threshold=1e-10, initial_guess="density"))
     assert abs(r3["iterations"] - r0["iterations"]) <= 1 and abs(r3["e_total"] - r0["e_total"]) < 1e-9
     cation = gm.Molecule.from_atoms(Z, xyz, gm.basis_path("cc-pvdz"), charge=1)
+    pu = Parameters(convergence_energy_threshold=1e-12, maxiter=200)
+    u0 = uhf(cation, pu)
+    u1 = uhf(cation, pu, density=(u0["Da"], u0["Db"]))
+    # per-spin DIIS restarts with an empty history: 6 iterations against about 70 from the core guess
+    assert u1["converged"] and u1["iterations"] < u0["iterations"] // 4, (u1["iterations"], u0["iterations"])
+    assert abs(u1["e_total"] - u0["e_total"]) < 1e-9
     with pytest.raises(GansuError) as e:
-        uhf(cation, Parameters(initial_guess="density"))
+        rohf(cation, Parameters(initial_guess="density"))
     assert e.value.code == 9
