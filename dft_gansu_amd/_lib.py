@@ -96,9 +96,9 @@ def lib():
         L.gansu_uhf_run.argtypes = [vp, vp, vp, vp, vp, vp, vp]
         L.gansu_rohf_run.argtypes = [vp, vp, vp, vp, vp, vp, vp]
         L.gansu_rhf_run_mo.argtypes = [vp, vp, vp, vp, vp, vp, vp]
-        L.gansu_rhf_run_guess.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp]
-        L.gansu_rhf_gradient_guess.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp]
-        L.gansu_uhf_run_guess.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp]
+        L.gansu_rhf_run_guess.argtypes = [vp, vp, vp, vp, vp, ctypes.c_int, vp, vp, vp]
+        L.gansu_rhf_gradient_guess.argtypes = [vp, vp, vp, vp, vp, ctypes.c_int, vp, vp, vp, vp]
+        L.gansu_uhf_run_guess.argtypes = [vp, vp, vp, vp, vp, ctypes.c_int, vp, vp, vp, vp]
         L.gansu_write_molden.argtypes = [vp, vp, vp, vp, ip, ctypes.c_char_p]
         L.gansu_fock_gradient_rhf.argtypes = [vp, vp, ip, vp, vp]
         L.gansu_fock_gradient_rhf_occ.argtypes = [vp, vp, vp, ip, ip, vp, vp]

@@ -146,6 +146,7 @@ int gansu_fock_create(const gansu_primitive_shell* prims, size_t n_prims, const 
 int gansu_fock_precompute(gansu_fock_engine* e) {
   return guarded([&] {
     if (!e) throw gansu::Error(GANSU_E_ARG, "null engine");
+    gansu::DeviceScope ds(e->e->device());
     e->e->precompute();
   });
 }
@@ -153,6 +154,7 @@ int gansu_fock_precompute(gansu_fock_engine* e) {
 int gansu_fock_build_rhf(gansu_fock_engine* e, const double* d_D, const double* d_H, double* d_F, void* stream) {
   return guarded([&] {
     if (!e || !d_D || !d_H || !d_F) throw gansu::Error(GANSU_E_ARG, "null argument");
+    gansu::DeviceScope ds(e->e->device());
     e->e->build_jk(d_D, nullptr, nullptr, d_H, d_F, (hipStream_t)stream);
   });
 }
@@ -160,6 +162,7 @@ int gansu_fock_build_rhf(gansu_fock_engine* e, const double* d_D, const double* 
 int gansu_fock_build_jk(gansu_fock_engine* e, const double* d_D, double* d_J, double* d_K, void* stream) {
   return guarded([&] {
     if (!e || !d_D || !d_J || !d_K) throw gansu::Error(GANSU_E_ARG, "null argument");
+    gansu::DeviceScope ds(e->e->device());
     e->e->build_jk(d_D, d_J, d_K, nullptr, nullptr, (hipStream_t)stream);
   });
 }
@@ -168,6 +171,7 @@ int gansu_fock_build_rhf_occ(gansu_fock_engine* e, const double* d_D, const doub
                              double* d_F, void* stream) {
   return guarded([&] {
     if (!e || !d_D || !d_C || !d_H || !d_F) throw gansu::Error(GANSU_E_ARG, "null argument");
+    gansu::DeviceScope ds(e->e->device());
     e->e->build_jk(d_D, nullptr, nullptr, d_H, d_F, (hipStream_t)stream, d_C, nocc);
   });
 }
@@ -176,6 +180,7 @@ int gansu_fock_build_uhf(gansu_fock_engine* e, const double* d_Da, const double*
                          double* d_Fb, void* stream) {
   return guarded([&] {
     if (!e) throw gansu::Error(GANSU_E_ARG, "null engine");
+    gansu::DeviceScope ds(e->e->device());
     e->e->build_uhf(d_Da, d_Db, d_H, d_Fa, d_Fb, (hipStream_t)stream);
   });
 }
@@ -188,6 +193,7 @@ int gansu_fock_build_rohf(gansu_fock_engine* e, const double* d_Dc, const double
     const std::string name = coupling ? coupling : "roothaan";
     gansu::RohfCoupling cp{};
     if (!gansu::rohf_coupling(name, &cp)) throw gansu::Error(GANSU_E_PARAM, "unknown ROHF coupling set: " + name);
+    gansu::DeviceScope ds(e->e->device());
     e->e->build_rohf(d_Dc, d_Do, d_H, d_C, d_S, cp, num_closed, num_open, d_Fc, d_Fo, d_F, (hipStream_t)stream);
   });
 }
@@ -195,6 +201,7 @@ int gansu_fock_build_rohf(gansu_fock_engine* e, const double* d_Dc, const double
 int gansu_fock_update_geometry(gansu_fock_engine* e, const double* xyz_bohr, int n_atoms) {
   return guarded([&] {
     if (!e) throw gansu::Error(GANSU_E_ARG, "null engine");
+    gansu::DeviceScope ds(e->e->device());
     e->e->update_geometry(xyz_bohr, n_atoms);
   });
 }
@@ -204,6 +211,7 @@ const char* gansu_fock_algorithm_name(const gansu_fock_engine* e) { return e ? e
 int gansu_fock_schwarz(gansu_fock_engine* e, double* host_Q, size_t n) {
   return guarded([&] {
     if (!e || !host_Q) throw gansu::Error(GANSU_E_ARG, "null argument");
+    gansu::DeviceScope ds(e->e->device());
     auto q = e->e->schwarz_ref();
     if (n != q.size()) throw gansu::Error(GANSU_E_ARG, "size mismatch: engine has " + std::to_string(q.size()));
     std::memcpy(host_Q, q.data(), n * sizeof(double));
@@ -215,6 +223,7 @@ size_t gansu_fock_num_primitive_pairs(const gansu_fock_engine* e) { return e ? e
 int gansu_fock_stats(const gansu_fock_engine* e, double* stats) {
   return guarded([&] {
     if (!e || !stats) throw gansu::Error(GANSU_E_ARG, "null argument");
+    gansu::DeviceScope ds(e->e->device());
     e->e->stats(stats);
   });
 }
@@ -222,6 +231,7 @@ int gansu_fock_stats(const gansu_fock_engine* e, double* stats) {
 int gansu_fock_set_profiling(gansu_fock_engine* e, int enable) {
   return guarded([&] {
     if (!e) throw gansu::Error(GANSU_E_ARG, "null engine");
+    gansu::DeviceScope ds(e->e->device());
     e->e->set_profiling(enable != 0);
   });
 }
@@ -229,6 +239,7 @@ int gansu_fock_set_profiling(gansu_fock_engine* e, int enable) {
 int gansu_fock_last_profile(gansu_fock_engine* e, double* rows, int max_rows, int* n_rows) {
   return guarded([&] {
     if (!e || !n_rows) throw gansu::Error(GANSU_E_ARG, "null argument");
+    gansu::DeviceScope ds(e->e->device());
     auto v = e->e->last_profile();
     const int n = (int)(v.size() / 5);
     *n_rows = n;
@@ -236,7 +247,11 @@ int gansu_fock_last_profile(gansu_fock_engine* e, double* rows, int max_rows, in
   });
 }
 
-void gansu_fock_destroy(gansu_fock_engine* e) { delete e; }
+void gansu_fock_destroy(gansu_fock_engine* e) {
+  if (!e) return;
+  gansu::DeviceScope ds(e->e->device());  // buffers and streams live on the engine's device
+  delete e;
+}
 
 int gansu_oneint(const gansu_molecule* mol, double* d_S, double* d_T, double* d_V, void* stream) {
   return guarded([&] {
@@ -258,12 +273,12 @@ int gansu_rhf_run(const gansu_molecule* mol, const gansu_molecule* aux_mol, cons
 }
 
 int gansu_rhf_gradient_guess(const gansu_molecule* mol, const gansu_molecule* aux_mol, const gansu_params* p,
-                             const double* Da0, const double* Db0, double* energies, double* grad, double* D_out,
-                             int* info) {
+                             const double* Da0, const double* Db0, int force_density, double* energies, double* grad,
+                             double* D_out, int* info) {
   return guarded([&] {
     if (!mol || !p || !energies || !grad) throw gansu::Error(GANSU_E_ARG, "null argument");
     if ((Da0 == nullptr) != (Db0 == nullptr)) throw gansu::Error(GANSU_E_ARG, "give both densities or neither");
-    auto r = gansu::rhf_run(mol->m, aux_mol ? &aux_mol->m : nullptr, p->p, true, Da0, Db0);
+    auto r = gansu::rhf_run(mol->m, aux_mol ? &aux_mol->m : nullptr, p->p, true, Da0, Db0, force_density != 0);
     energies[0] = r.e_elec;
     energies[1] = r.e_total;
     std::memcpy(grad, r.grad.data(), r.grad.size() * sizeof(double));
@@ -273,11 +288,12 @@ int gansu_rhf_gradient_guess(const gansu_molecule* mol, const gansu_molecule* au
 }
 
 int gansu_rhf_run_guess(const gansu_molecule* mol, const gansu_molecule* aux_mol, const gansu_params* p,
-                        const double* Da0, const double* Db0, double* energies, double* D_out, int* info) {
+                        const double* Da0, const double* Db0, int force_density, double* energies, double* D_out,
+                        int* info) {
   return guarded([&] {
     if (!mol || !p || !energies) throw gansu::Error(GANSU_E_ARG, "null argument");
     if ((Da0 == nullptr) != (Db0 == nullptr)) throw gansu::Error(GANSU_E_ARG, "give both densities or neither");
-    auto r = gansu::rhf_run(mol->m, aux_mol ? &aux_mol->m : nullptr, p->p, false, Da0, Db0);
+    auto r = gansu::rhf_run(mol->m, aux_mol ? &aux_mol->m : nullptr, p->p, false, Da0, Db0, force_density != 0);
     energies[0] = r.e_elec;
     energies[1] = r.e_total;
     if (D_out) std::memcpy(D_out, r.D.data(), r.D.size() * sizeof(double));
@@ -288,6 +304,7 @@ int gansu_rhf_run_guess(const gansu_molecule* mol, const gansu_molecule* aux_mol
 int gansu_fock_gradient_rhf(gansu_fock_engine* e, const double* d_D, int n_atoms, double* d_grad, void* stream) {
   return guarded([&] {
     if (!e) throw gansu::Error(GANSU_E_ARG, "null engine");
+    gansu::DeviceScope ds(e->e->device());
     e->e->gradient_2e(d_D, n_atoms, d_grad, (hipStream_t)stream);
   });
 }
@@ -296,6 +313,7 @@ int gansu_fock_gradient_rhf_occ(gansu_fock_engine* e, const double* d_D, const d
                                 double* d_grad, void* stream) {
   return guarded([&] {
     if (!e) throw gansu::Error(GANSU_E_ARG, "null engine");
+    gansu::DeviceScope ds(e->e->device());
     e->e->gradient_2e(d_D, n_atoms, d_grad, (hipStream_t)stream, d_C, nocc);
   });
 }
@@ -351,11 +369,12 @@ int gansu_uhf_run(const gansu_molecule* mol, const gansu_molecule* aux_mol, cons
 }
 
 int gansu_uhf_run_guess(const gansu_molecule* mol, const gansu_molecule* aux_mol, const gansu_params* p,
-                        const double* Da0, const double* Db0, double* energies, double* Da, double* Db, int* info) {
+                        const double* Da0, const double* Db0, int force_density, double* energies, double* Da,
+                        double* Db, int* info) {
   return guarded([&] {
     if (!mol || !p || !energies) throw gansu::Error(GANSU_E_ARG, "null argument");
     if ((Da0 == nullptr) != (Db0 == nullptr)) throw gansu::Error(GANSU_E_ARG, "give both densities or neither");
-    auto r = gansu::uhf_run(mol->m, aux_mol ? &aux_mol->m : nullptr, p->p, Da0, Db0);
+    auto r = gansu::uhf_run(mol->m, aux_mol ? &aux_mol->m : nullptr, p->p, Da0, Db0, force_density != 0);
     energies[0] = r.e_elec;
     energies[1] = r.e_total;
     if (Da) std::memcpy(Da, r.Da.data(), r.Da.size() * sizeof(double));

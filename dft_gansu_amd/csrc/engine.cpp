@@ -55,6 +55,9 @@ FockEngine::FockEngine(const gansu_primitive_shell* prims, size_t n_prims, const
                        const std::map<std::string, std::string>& opts) {
   if (!prims || n_prims == 0 || !types || n_types <= 0 || !norm || num_basis <= 0)
     throw Error(GANSU_E_ARG, "empty primitive list or basis");
+  // the engine's device: the caller's current one (no device yet: 0, and the
+  // first call that needs one reports the HIP error)
+  if (hipGetDevice(&dev_) != hipSuccess) dev_ = 0;
   method_ = "stored";
   for (const auto& kv : opts) {
     if (kv.first == "eri_method") method_ = kv.second;
@@ -65,6 +68,7 @@ FockEngine::FockEngine(const gansu_primitive_shell* prims, size_t n_prims, const
     } else if (kv.first == "fp32_bound_threshold") fp32_thr_ = std::stod(kv.second);
     else if (kv.first == "autotune") autotune_ = kv.second != "0";
     else if (kv.first == "graph") use_graph_ = kv.second != "0";
+    else if (kv.first == "nan_check") nan_check_ = kv.second != "0";
     else if (kv.first == "ri_k") {
       if (kv.second != "density" && kv.second != "occupied") throw Error(GANSU_E_PARAM, "ri_k must be density|occupied");
       ri_k_occ_ = kv.second == "occupied";
@@ -177,7 +181,6 @@ const char* FockEngine::algorithm_name() const {
 }
 
 void FockEngine::precompute() {
-  GANSU_HIP(hipSetDevice(0));
   graph_release();  // a captured build refers to the buffers rebuilt below
   if (!boys_.ptr) boys_.upload(boys_grid_xn());
   d_w_.upload(w_h_);

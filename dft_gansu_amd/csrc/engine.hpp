@@ -15,6 +15,22 @@ namespace gansu {
 void hip_check(hipError_t e, const char* what);
 #define GANSU_HIP(x) ::gansu::hip_check((x), #x)
 
+// Makes `dev` the current device for a scope and restores the caller's
+// device afterwards, so that an engine created on device k runs there
+// whatever the caller's current device is at a later call.
+struct DeviceScope {
+  int prev = -1;
+  explicit DeviceScope(int dev) {
+    int cur = -1;
+    if (hipGetDevice(&cur) == hipSuccess && cur != dev && hipSetDevice(dev) == hipSuccess) prev = cur;
+  }
+  ~DeviceScope() {
+    if (prev >= 0) (void)hipSetDevice(prev);
+  }
+  DeviceScope(const DeviceScope&) = delete;
+  DeviceScope& operator=(const DeviceScope&) = delete;
+};
+
 // RAII device buffer.
 template <typename T>
 struct DevBuf {
@@ -112,6 +128,7 @@ class FockEngine {
   size_t num_primitive_pairs() const { return ref_pairs_; }
   void stats(double* out) const;
   int num_basis() const { return M_; }
+  int device() const { return dev_; }  // the device current when the engine was created
   void set_profiling(bool on);
   int graph_captures() const;  // graphs captured so far (the build replays the last one)
   std::vector<double> last_profile();  // 5 values per launch
@@ -139,7 +156,9 @@ class FockEngine {
   void build_ri(const double* d_D, double* d_J, double* d_K, const double* d_H, double* d_F, hipStream_t s,
                 const double* d_C, int nocc);
 
+  int dev_ = 0;
   std::string method_;  // stored | direct | ri
+  bool nan_check_ = true;  // eri_method=ri: a build whose F holds a NaN fails (rhf.hpp:665-674)
   double thr_ = 1e-12;
   bool mixed_ = false;
   double fp32_thr_ = 1e-6;

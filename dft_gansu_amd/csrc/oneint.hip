@@ -140,6 +140,8 @@ __global__ void k_dot_partial(const double* __restrict__ x, const double* __rest
   if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
 }
 
+__global__ void k_set_int(int* p, int v) { *p = v; }
+
 __global__ void k_has_nan(const double* __restrict__ x, size_t n, int* flag) {
   const size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (id < n && !(x[id] == x[id])) *flag = 1;
@@ -178,6 +180,11 @@ hipError_t launch_axpby(double a, const double* x, double b, const double* y, si
 
 hipError_t launch_dot_partial(const double* x, const double* y, size_t n, double* part, int nblocks, hipStream_t s) {
   hipLaunchKernelGGL(k_dot_partial, dim3(nblocks), dim3(256), 0, s, x, y, n, part);
+  return hipGetLastError();
+}
+
+hipError_t launch_set_int(int* p, int v, hipStream_t s) {
+  hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, s, p, v);
   return hipGetLastError();
 }
 

@@ -19,6 +19,7 @@
 
 #include "engine.hpp"
 #include "ri_kernels.hpp"
+#include "scf_kernels.hpp"
 
 namespace gansu {
 
@@ -60,9 +61,12 @@ struct FockEngine::RIState {
   DevBuf<int> jstate;  // {builds since FP64 J (-1: none yet), FP64 this build}
   std::vector<int> aux_atom_h;  // atom of each auxiliary shell
   DevBuf<int> aux_atom;
+  DevBuf<int> nanflag;       // NaN check of F (device flag, pinned host copy)
+  int* nan_host = nullptr;
   int pchunk = 0;
   ~RIState() {
     if (h) rocblas_destroy_handle(h);
+    if (nan_host) (void)hipHostFree(nan_host);
   }
 };
 
@@ -177,6 +181,8 @@ void FockEngine::precompute_ri() {
   r.W.alloc(Ma);
   r.J.alloc((size_t)M * M);
   r.K.alloc((size_t)M * M);
+  r.nanflag.alloc(1);
+  GANSU_HIP(hipHostMalloc((void**)&r.nan_host, sizeof(int), hipHostMallocDefault));
   // K workspace: a P-chunk of T[m][P][l], at most ~4 GB
   const size_t budget = size_t(4) << 30;
   r.pchunk = (int)std::max<size_t>(1, std::min<size_t>(Ma, budget / ((size_t)M * M * sizeof(double))));
@@ -236,7 +242,18 @@ void FockEngine::build_ri(const double* d_D, double* d_J, double* d_K, const dou
     prof_end(s);
   }
   }
-  if (d_F) GANSU_HIP(launch_fock_ri(d_H, J, K, (size_t)M * M, d_F, s));
+  if (d_F) {
+    GANSU_HIP(launch_fock_ri(d_H, J, K, (size_t)M * M, d_F, s));
+    if (nan_check_) {
+      // ERI_RI_RHF::compute_fock_matrix reads F back and fails on a NaN
+      // (rhf.hpp:665-674); here one flag word comes back instead of F
+      GANSU_HIP(launch_set_int(r.nanflag.ptr, 0, s));
+      GANSU_HIP(launch_has_nan(d_F, (size_t)M * M, r.nanflag.ptr, s));
+      GANSU_HIP(hipMemcpyAsync(r.nan_host, r.nanflag.ptr, sizeof(int), hipMemcpyDeviceToHost, s));
+      GANSU_HIP(hipStreamSynchronize(s));
+      if (*r.nan_host) throw Error(GANSU_E_NUMERIC, "NaN detected in the Fock matrix");
+    }
+  }
 }
 
 // K = sum_P (B^P X)(B^P X)^T with X = sqrt(2) C_occ (M x nocc, row-major,

@@ -176,6 +176,68 @@ void check_guess(const std::string& guess) {
     throw Error(GANSU_E_UNSUPPORTED, "initial_guess=" + guess + " is not provided (core, gwh and sad only)");
 }
 
+// Whether the guess comes from caller densities (guess_initial_fock_matrix,
+// rhf.cu:122-134, uhf.cu:118-130): when force_density is set or
+// initial_guess=density; without both densities the core guess is used then.
+bool density_guess(const std::string& guess, bool force_density, const double* Da0, const double* Db0) {
+  if (!force_density && guess != "density") return false;
+  if (Da0 && Db0) return true;
+  std::fprintf(stderr, "initial_guess=density or force_density without density matrices: the core Hamiltonian "
+                       "guess is used\n");
+  return false;
+}
+
+// UHF symmetry break of every UHF guess (InitialGuess_UHF::break_symmetry,
+// uhf.hpp:429-436): the last row and the last column of the row-major
+// C_beta are zeroed before the densities are formed.
+void break_symmetry(double* Cb, int n) {
+  GANSU_HIP(hipMemset(Cb + (size_t)(n - 1) * n, 0, sizeof(double) * n));
+  GANSU_HIP(hipMemset2D(Cb + (n - 1), sizeof(double) * n, 0, sizeof(double), n));
+}
+
+// Damping of a Fock matrix (and, for optimal damping, of the density) as
+// Convergence_RHF_Damping::get_new_fock_matrix (rhf.hpp:219-243) and
+// gpu::damping (gpu_manager.cu:989-1007): the first call stores F and D;
+// each later call sets F <- (1 - a) F_prev + a F and F_prev <- F, with a the
+// damping factor or, optimally (gpu_manager.cu:898-971),
+// a = min(1, -s / (2 c)), s = tr(F_prev dD), c = tr(dF dD); the optimal form
+// damps D the same way.
+struct Damper {
+  bool first = true, optimal = false;
+  double factor = 0.0;
+  size_t nn = 0;
+  DevBuf<double> prevF, prevD;
+  void init(size_t nn_, bool optimal_, double factor_) {
+    nn = nn_;
+    optimal = optimal_;
+    factor = factor_;
+    first = true;
+    prevF.alloc(nn);
+    prevD.alloc(nn);
+  }
+  void step(double* F, double* D, double* t1, double* t2, DevBuf<double>& part) {
+    if (first) {
+      first = false;
+      GANSU_HIP(hipMemcpy(prevD.ptr, D, nn * sizeof(double), hipMemcpyDeviceToDevice));
+      GANSU_HIP(hipMemcpy(prevF.ptr, F, nn * sizeof(double), hipMemcpyDeviceToDevice));
+      return;
+    }
+    double alpha = factor;
+    if (optimal) {
+      GANSU_HIP(launch_axpby(1.0, F, -1.0, prevF.ptr, nn, t1, 0));
+      GANSU_HIP(launch_axpby(1.0, D, -1.0, prevD.ptr, nn, t2, 0));
+      // for symmetric matrices tr(AB) = sum A .* B
+      const double sv = dot(prevF.ptr, t2, nn, part);
+      const double cv = dot(t1, t2, nn, part);
+      alpha = (cv <= -sv / 2.0) ? 1.0 : -0.5 * sv / cv;
+      GANSU_HIP(launch_axpby(1.0 - alpha, prevD.ptr, alpha, D, nn, prevD.ptr, 0));
+      GANSU_HIP(hipMemcpy(D, prevD.ptr, nn * sizeof(double), hipMemcpyDeviceToDevice));
+    }
+    GANSU_HIP(launch_axpby(1.0 - alpha, prevF.ptr, alpha, F, nn, prevF.ptr, 0));
+    GANSU_HIP(hipMemcpy(F, prevF.ptr, nn * sizeof(double), hipMemcpyDeviceToDevice));
+  }
+};
+
 }  // namespace
 
 OneIntHost::OneIntHost(const Molecule& mol) : M(mol.nbasis), natoms((int)mol.atoms.size()) {
@@ -269,7 +331,7 @@ std::vector<double> nuclear_repulsion_gradient(const Molecule& mol) {
 }
 
 ScfResult rhf_run(const Molecule& mol, const Molecule* aux, const Params& prm, bool gradient, const double* Da0,
-                  const double* Db0) {
+                  const double* Db0, bool force_density) {
   if (prm.get("method") != "rhf") throw Error(GANSU_E_PARAM, "only method=rhf is provided by this engine");
   if (mol.n_electrons % 2 != 0) throw Error(GANSU_E_ELECTRONS, "RHF requires an even number of electrons");
   const int n = mol.nbasis;
@@ -277,11 +339,9 @@ ScfResult rhf_run(const Molecule& mol, const Molecule* aux, const Params& prm, b
   if (nocc > n) throw Error(GANSU_E_ELECTRONS, "more occupied orbitals than basis functions");
   const std::string guess = prm.get("initial_guess");
   if (guess != "density") check_guess(guess);
-  // caller densities replace the guess (HF::solve with densities,
-  // rhf.cu:127-135); initial_guess=density without them is the core guess
-  const bool from_density = Da0 && Db0;
-  if (guess == "density" && !from_density)
-    std::fprintf(stderr, "initial_guess=density without density matrices: the core Hamiltonian guess is used\n");
+  // caller densities start the loop when force_density is set or
+  // initial_guess=density (rhf.cu:122-134); otherwise the named guess runs
+  const bool from_density = density_guess(guess, force_density, Da0, Db0);
   const std::string conv_method = prm.get("convergence_method");
   if (conv_method != "diis" && conv_method != "damping" && conv_method != "optimaldamping")
     throw Error(GANSU_E_PARAM, "Unknown convergence method: " + conv_method);
@@ -291,7 +351,6 @@ ScfResult rhf_run(const Molecule& mol, const Molecule* aux, const Params& prm, b
   const bool diis_x = prm.geti("diis_include_transform") != 0;
   const double damp = prm.getd("damping_factor");
 
-  GANSU_HIP(hipSetDevice(0));
   // GANSU_SCF_TIMING=1 prints the wall time of each phase (with a device
   // synchronisation at every phase boundary) to stderr.
   const char* tenv = std::getenv("GANSU_SCF_TIMING");
@@ -348,18 +407,10 @@ ScfResult rhf_run(const Molecule& mol, const Molecule* aux, const Params& prm, b
   };
   // occ: D = 2 C_occ C_occ^T holds (every build but the SAD guess's first),
   // so the RI exchange may take its occupied form
+  // (the RI build checks F for NaN itself, as ERI_RI_RHF does, rhf.hpp:665-674)
   auto fock = [&](bool occ = true) {
     eng.build_jk(w.D.ptr, nullptr, nullptr, w.H.ptr, w.F.ptr, 0, occ ? w.C.ptr : nullptr, occ ? nocc : 0);
     ++res.fock_builds;
-    if (ri) {  // rhf.hpp:665-674 checks F for NaN on the RI path
-      DevBuf<int> flag(1);
-      int zero = 0;
-      GANSU_HIP(hipMemcpy(flag.ptr, &zero, sizeof(int), hipMemcpyHostToDevice));
-      GANSU_HIP(launch_has_nan(w.F.ptr, nn, flag.ptr, 0));
-      int h = 0;
-      GANSU_HIP(hipMemcpy(&h, flag.ptr, sizeof(int), hipMemcpyDeviceToHost));
-      if (h) throw Error(GANSU_E_NUMERIC, "NaN detected in the Fock matrix");
-    }
   };
   auto energy = [&]() { return 0.5 * (dot(w.D.ptr, w.H.ptr, nn, w.part) + dot(w.D.ptr, w.F.ptr, nn, w.part)); };
 
@@ -398,10 +449,9 @@ ScfResult rhf_run(const Molecule& mol, const Molecule* aux, const Params& prm, b
 
   // convergence state
   Diis diis;
-  DevBuf<double> prevF, prevD;
+  Damper damper;
   if (conv_method == "diis") diis.init(diis_size, nn);
-  else { prevF.alloc(nn); prevD.alloc(nn); }
-  bool first_damp = true;
+  else damper.init(nn, conv_method == "optimaldamping", damp);
   double prev = std::numeric_limits<double>::max(), E = 0.0;
   int it = 0;
   while (it < maxiter) {
@@ -432,26 +482,7 @@ ScfResult rhf_run(const Molecule& mol, const Molecule* aux, const Params& prm, b
       }
       diis.step(blas.h, e, w.F.ptr);
     } else {
-      if (first_damp) {
-        first_damp = false;
-        GANSU_HIP(hipMemcpy(prevD.ptr, w.D.ptr, nn * sizeof(double), hipMemcpyDeviceToDevice));
-        GANSU_HIP(hipMemcpy(prevF.ptr, w.F.ptr, nn * sizeof(double), hipMemcpyDeviceToDevice));
-      } else {
-        double alpha = damp;
-        if (conv_method == "optimaldamping") {
-          // s = tr(Fp dD), c = tr(dF dD)
-          GANSU_HIP(launch_axpby(1.0, w.F.ptr, -1.0, prevF.ptr, nn, w.tmp.ptr, 0));
-          GANSU_HIP(launch_axpby(1.0, w.D.ptr, -1.0, prevD.ptr, nn, w.tmp2.ptr, 0));
-          // for symmetric matrices tr(AB) = sum A .* B
-          const double sv = dot(prevF.ptr, w.tmp2.ptr, nn, w.part);
-          const double cv = dot(w.tmp.ptr, w.tmp2.ptr, nn, w.part);
-          alpha = (cv <= -sv / 2.0) ? 1.0 : -0.5 * sv / cv;
-          GANSU_HIP(launch_axpby(1.0 - alpha, prevD.ptr, alpha, w.D.ptr, nn, prevD.ptr, 0));
-          GANSU_HIP(hipMemcpy(w.D.ptr, prevD.ptr, nn * sizeof(double), hipMemcpyDeviceToDevice));
-        }
-        GANSU_HIP(launch_axpby(1.0 - alpha, prevF.ptr, alpha, w.F.ptr, nn, prevF.ptr, 0));
-        GANSU_HIP(hipMemcpy(w.F.ptr, prevF.ptr, nn * sizeof(double), hipMemcpyDeviceToDevice));
-      }
+      damper.step(w.F.ptr, w.D.ptr, w.tmp.ptr, w.tmp2.ptr, w.part);
     }
     t_conv += now() - tc0;
   }
@@ -487,23 +518,24 @@ ScfResult rhf_run(const Molecule& mol, const Molecule* aux, const Params& prm, b
 // Fock pair from the engine's UHF build, E = 1/2 sum_s D_s.(H + F_s),
 // convergence on |dE|, and DIIS run separately for each spin with the error
 // F_s D_s S - S D_s F_s.
-UhfResult uhf_run(const Molecule& mol, const Molecule* aux, const Params& prm, const double* Da0, const double* Db0) {
+UhfResult uhf_run(const Molecule& mol, const Molecule* aux, const Params& prm, const double* Da0, const double* Db0,
+                  bool force_density) {
   const int n = mol.nbasis, na = mol.n_alpha, nb = mol.n_beta;
   if (na > n || nb > n)
     throw Error(GANSU_E_ELECTRONS, "more alpha or beta electrons than basis functions");
   const std::string guess = prm.get("initial_guess");
   if (guess != "density") check_guess(guess);
-  const bool from_density = Da0 && Db0;
-  if (guess == "density" && !from_density)
-    std::fprintf(stderr, "initial_guess=density without density matrices: the core Hamiltonian guess is used\n");
-  if (prm.get("convergence_method") != "diis")
-    throw Error(GANSU_E_UNSUPPORTED, "UHF is provided with convergence_method=diis");
+  const bool from_density = density_guess(guess, force_density, Da0, Db0);
+  // uhf.cu:57-69: damping (constant factor) or DIIS
+  const std::string conv_method = prm.get("convergence_method");
+  if (conv_method != "diis" && conv_method != "damping")
+    throw Error(GANSU_E_PARAM, "Invalid convergence_method: " + conv_method);
   const int maxiter = prm.geti("maxiter");
   const double conv = prm.getd("convergence_energy_threshold");
   const int diis_size = prm.geti("diis_size");
   const bool diis_x = prm.geti("diis_include_transform") != 0;
+  const double damp = prm.getd("damping_factor");
 
-  GANSU_HIP(hipSetDevice(0));
   Work w(n);
   Blas blas;
   const size_t nn = (size_t)n * n;
@@ -556,45 +588,37 @@ UhfResult uhf_run(const Molecule& mol, const Molecule* aux, const Params& prm, c
     for (int sp = 0; sp < 2; ++sp) e += dot(D[sp], w.H.ptr, nn, w.part) + dot(D[sp], F[sp], nn, w.part);
     return 0.5 * e;
   };
+  // Every guess ends with the symmetry break (the last row and column of
+  // C_beta zeroed), then D per spin and F (uhf.hpp:494-568, :668-705).
   DevBuf<double> guess_F;
-  if (from_density) {
-    // caller D_a, D_b, F(D), C and D per spin (InitialGuess_UHF_Density::guess,
-    // uhf.hpp:557-568)
-    GANSU_HIP(hipMemcpy(D[0], Da0, nn * sizeof(double), hipMemcpyHostToDevice));
-    GANSU_HIP(hipMemcpy(D[1], Db0, nn * sizeof(double), hipMemcpyHostToDevice));
-    fock();
-    for (int sp = 0; sp < 2; ++sp) {
-      coefficients(F[sp], C[sp]);
-      density(sp);
-    }
-  } else if (guess == "sad") {
-    // atomic D_a, D_b, F(D), C per spin, then the symmetry break of
-    // InitialGuess_UHF_SAD (uhf.hpp:429-436, :668-705): the last row and the
-    // last column of C_beta are zeroed before the densities are formed
+  if (from_density || guess == "sad") {
+    // caller (InitialGuess_UHF_Density, uhf.hpp:557-568) or atomic
+    // (InitialGuess_UHF_SAD) D_a, D_b, F(D), C per spin
     std::vector<double> dh[2];
-    sad_density(mol, dh[0], dh[1]);
+    if (!from_density) sad_density(mol, dh[0], dh[1]);
     for (int sp = 0; sp < 2; ++sp)
-      GANSU_HIP(hipMemcpy(D[sp], dh[sp].data(), nn * sizeof(double), hipMemcpyHostToDevice));
+      GANSU_HIP(hipMemcpy(D[sp], from_density ? (sp ? Db0 : Da0) : dh[sp].data(), nn * sizeof(double),
+                          hipMemcpyHostToDevice));
     fock();
     for (int sp = 0; sp < 2; ++sp) coefficients(F[sp], C[sp]);
-    std::vector<double> cb(nn);
-    GANSU_HIP(hipMemcpy(cb.data(), C[1], nn * sizeof(double), hipMemcpyDeviceToHost));
-    for (int i = 0; i < n; ++i) cb[(size_t)(n - 1) * n + i] = cb[(size_t)i * n + n - 1] = 0.0;
-    GANSU_HIP(hipMemcpy(C[1], cb.data(), nn * sizeof(double), hipMemcpyHostToDevice));
-    for (int sp = 0; sp < 2; ++sp) density(sp);
   } else {
+    // core or GWH: C_alpha from H (or the GWH matrix), C_beta a copy of it
     if (guess == "gwh") {
       guess_F.alloc(nn);
       GANSU_HIP(launch_gwh_guess(w.H.ptr, w.S.ptr, n, guess_F.ptr, 0));
     }
-    for (int sp = 0; sp < 2; ++sp) {
-      coefficients(guess == "gwh" ? guess_F.ptr : w.H.ptr, C[sp]);
-      density(sp);
-    }
+    coefficients(guess == "gwh" ? guess_F.ptr : w.H.ptr, C[0]);
+    GANSU_HIP(hipMemcpy(C[1], C[0], nn * sizeof(double), hipMemcpyDeviceToDevice));
   }
+  break_symmetry(C[1], n);
+  for (int sp = 0; sp < 2; ++sp) density(sp);
   fock();
   Diis diis[2];
-  for (int sp = 0; sp < 2; ++sp) diis[sp].init(diis_size, nn);
+  Damper damper[2];
+  for (int sp = 0; sp < 2; ++sp) {
+    if (conv_method == "diis") diis[sp].init(diis_size, nn);
+    else damper[sp].init(nn, false, damp);
+  }
   double prev = std::numeric_limits<double>::max(), E = 0.0;
   int it = 0;
   while (it < maxiter) {
@@ -608,6 +632,11 @@ UhfResult uhf_run(const Molecule& mol, const Molecule* aux, const Params& prm, c
     ++it;
     prev = E;
     for (int sp = 0; sp < 2; ++sp) {
+      if (conv_method == "damping") {
+        // Convergence_UHF_Damping (uhf.hpp:236-258), each spin on its own
+        damper[sp].step(F[sp], D[sp], w.tmp.ptr, w.tmp2.ptr, w.part);
+        continue;
+      }
       double* e = w.tmp2.ptr;
       blas.gemm(false, false, n, F[sp], D[sp], w.tmp.ptr);
       blas.gemm(false, false, n, w.tmp.ptr, w.S.ptr, e);
@@ -644,8 +673,10 @@ UhfResult rohf_run(const Molecule& mol, const Molecule* aux, const Params& prm) 
   // S C R C^T S before anything has written C, so it has no defined result
   if (guess == "sad")
     throw Error(GANSU_E_UNSUPPORTED, "initial_guess=sad is not provided for ROHF (core and gwh only)");
-  if (prm.get("convergence_method") != "diis")
-    throw Error(GANSU_E_UNSUPPORTED, "ROHF is provided with convergence_method=diis");
+  // rohf.cu:73-88: damping, optimal damping or DIIS
+  const std::string conv_method = prm.get("convergence_method");
+  if (conv_method != "diis" && conv_method != "damping" && conv_method != "optimaldamping")
+    throw Error(GANSU_E_PARAM, "Invalid convergence_method: " + conv_method);
   RohfCoupling cp{};
   if (!rohf_coupling(prm.get("rohf_parameter_name"), &cp))
     throw Error(GANSU_E_PARAM, "unknown rohf_parameter_name: " + prm.get("rohf_parameter_name"));
@@ -653,8 +684,8 @@ UhfResult rohf_run(const Molecule& mol, const Molecule* aux, const Params& prm) 
   const double conv = prm.getd("convergence_energy_threshold");
   const int diis_size = prm.geti("diis_size");
   const bool diis_x = prm.geti("diis_include_transform") != 0;
+  const double damp = prm.getd("damping_factor");
 
-  GANSU_HIP(hipSetDevice(0));
   Work w(n);
   Blas blas;
   const size_t nn = (size_t)n * n;
@@ -716,7 +747,9 @@ UhfResult rohf_run(const Molecule& mol, const Molecule* aux, const Params& prm) 
   density();
   fock();
   Diis diis;
-  diis.init(diis_size, nn);
+  Damper damper;
+  if (conv_method == "diis") diis.init(diis_size, nn);
+  else damper.init(nn, conv_method == "optimaldamping", damp);
   double prev = std::numeric_limits<double>::max(), E = 0.0;
   int it = 0;
   while (it < maxiter) {
@@ -727,6 +760,12 @@ UhfResult rohf_run(const Molecule& mol, const Molecule* aux, const Params& prm) 
     if (std::fabs(E - prev) < conv) { res.converged = 1; break; }
     ++it;
     prev = E;
+    if (conv_method != "diis") {
+      // Convergence_ROHF_Damping (rohf.hpp:254-276) on the unified Fock
+      // matrix and the total density
+      damper.step(w.F.ptr, w.D.ptr, w.tmp.ptr, w.tmp2.ptr, w.part);
+      continue;
+    }
     double* e = w.tmp2.ptr;
     blas.gemm(false, false, n, w.F.ptr, w.D.ptr, w.tmp.ptr);
     blas.gemm(false, false, n, w.tmp.ptr, w.S.ptr, e);

@@ -21,23 +21,26 @@ struct ScfResult {
 
 // HF::single_point_energy (hf.cu:245-314) for RHF; with gradient = true the
 // analytic nuclear gradient of the converged energy follows.
-// Da0/Db0 (host, M x M), when both are given, replace the guess: F(Da0 + Db0)
-// starts the loop (HF::solve with densities, rhf.cu:127-135).
+// Da0/Db0 (host, M x M) replace the guess when force_density is set or
+// initial_guess=density: F(Da0 + Db0) starts the loop (HF::solve with
+// densities, rhf.cu:122-134); without both the core guess runs then.
 ScfResult rhf_run(const Molecule& mol, const Molecule* aux, const Params& prm, bool gradient = false,
-                  const double* Da0 = nullptr, const double* Db0 = nullptr);
+                  const double* Da0 = nullptr, const double* Db0 = nullptr, bool force_density = false);
 
 struct UhfResult {
   double e_elec = 0, e_total = 0;
   int iterations = 0, fock_builds = 0, converged = 0;
   std::vector<double> Da, Db;  // final alpha and beta densities (M x M)
 };
-// HF::single_point_energy for UHF (uhf.cu), core guess and per-spin DIIS.
-// Da0/Db0 (host, M x M), when both are given, replace the guess: F(Da0, Db0),
-// then C and D per spin (HF::solve with densities).
+// HF::single_point_energy for UHF (uhf.cu): core, GWH or SAD guess with the
+// symmetry break, per-spin DIIS or damping.  Da0/Db0 (host, M x M) replace
+// the guess as in rhf_run: F(Da0, Db0), then C per spin, the symmetry break
+// and D (HF::solve with densities).
 UhfResult uhf_run(const Molecule& mol, const Molecule* aux, const Params& prm, const double* Da0 = nullptr,
-                  const double* Db0 = nullptr);
-// HF::single_point_energy for ROHF (rohf.cu): core guess, DIIS on the
-// unified Fock matrix; Da / Db of the result hold the closed / open densities.
+                  const double* Db0 = nullptr, bool force_density = false);
+// HF::single_point_energy for ROHF (rohf.cu): core or GWH guess, DIIS,
+// damping or optimal damping on the unified Fock matrix; Da / Db of the
+// result hold the closed / open densities.
 UhfResult rohf_run(const Molecule& mol, const Molecule* aux, const Params& prm);
 
 // Molden file of the orbitals C (M x n_mo row-major, MOs in columns) with

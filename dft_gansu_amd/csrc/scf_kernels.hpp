@@ -30,6 +30,8 @@ hipError_t launch_axpby(double a, const double* x, double b, const double* y, si
 // Initial Fock matrix of the GWH guess: F_pq = 1.75 S_pq (H_pp + H_qq) / 2.
 hipError_t launch_gwh_guess(const double* H, const double* S, int M, double* F, hipStream_t s);
 hipError_t launch_dot_partial(const double* x, const double* y, size_t n, double* part, int nblocks, hipStream_t s);
+// *flag = 1 if x holds a NaN (flag left as it is otherwise).
 hipError_t launch_has_nan(const double* x, size_t n, int* flag, hipStream_t s);
+hipError_t launch_set_int(int* p, int v, hipStream_t s);
 
 }  // namespace gansu

@@ -55,13 +55,13 @@ class FockEngine:
     """Fock-build engine over a molecule (and an auxiliary basis for RI)."""
 
     def __init__(self, mol, aux=None, eri_method="stored", schwarz_screening_threshold=1e-12, precision="fp64",
-                 fp32_bound_threshold=1e-6, ri_k="density", autotune=True, graph=True):
+                 fp32_bound_threshold=1e-6, ri_k="density", autotune=True, graph=True, nan_check=True):
         self.mol = mol
         self.aux = aux
         self.M = mol.n_basis
         opts = (f"eri_method={eri_method};schwarz_screening_threshold={schwarz_screening_threshold!r};"
                 f"precision={precision};fp32_bound_threshold={fp32_bound_threshold!r};ri_k={ri_k};"
-                f"autotune={int(bool(autotune))};graph={int(bool(graph))}")
+                f"autotune={int(bool(autotune))};graph={int(bool(graph))};nan_check={int(bool(nan_check))}")
         self._h = ctypes.c_void_p()
         a = aux
         check(lib().gansu_fock_create(
@@ -225,11 +225,11 @@ def nuclear_repulsion_gradient(mol):
     return g
 
 
-def rhf_gradient(mol, params, aux=None, density=None):
+def rhf_gradient(mol, params, aux=None, density=None, force_density=False):
     """SCF then the analytic nuclear gradient; returns dict(e_elec, e_total,
     iterations, fock_builds, converged, grad (n_atoms x 3, hartree/bohr), D).
-    density = (Da, Db) starts the SCF from F(Da + Db) (a previous geometry's
-    D/2, D/2, say)."""
+    density = (Da, Db) with force_density=True (or initial_guess=density)
+    starts the SCF from F(Da + Db) (a previous geometry's D/2, D/2, say)."""
     en = np.zeros(2)
     g = np.zeros((mol.n_atoms, 3))
     D = np.zeros((mol.n_basis, mol.n_basis))
@@ -240,14 +240,15 @@ def rhf_gradient(mol, params, aux=None, density=None):
         if Da.shape != D.shape or Db.shape != D.shape:
             raise ValueError(f"densities must be {D.shape}")
     check(lib().gansu_rhf_gradient_guess(mol.handle, aux.handle if aux else None, params.handle, ptr(Da), ptr(Db),
-                                         ptr(en), ptr(g), ptr(D), ptr(info)))
+                                         int(bool(force_density)), ptr(en), ptr(g), ptr(D), ptr(info)))
     return dict(e_elec=float(en[0]), e_total=float(en[1]), iterations=int(info[0]), fock_builds=int(info[1]),
                 converged=bool(info[2]), grad=g, D=D)
 
 
-def uhf(mol, params, aux=None, density=None):
+def uhf(mol, params, aux=None, density=None, force_density=False):
     """Run the UHF SCF; returns dict(e_elec, e_total, iterations, fock_builds, converged, Da, Db).
-    density = (Da, Db) starts from F(Da, Db) (HF::solve with densities)."""
+    density = (Da, Db) with force_density=True (or initial_guess=density)
+    starts from F(Da, Db) (HF::solve(Da, Db, force_density), uhf.cu:118-130)."""
     en = np.zeros(2)
     Da = np.zeros((mol.n_basis, mol.n_basis))
     Db = np.zeros((mol.n_basis, mol.n_basis))
@@ -257,7 +258,7 @@ def uhf(mol, params, aux=None, density=None):
         if a0.shape != Da.shape or b0.shape != Da.shape:
             raise ValueError(f"densities must be {Da.shape}")
         check(lib().gansu_uhf_run_guess(mol.handle, aux.handle if aux else None, params.handle, ptr(a0), ptr(b0),
-                                        ptr(en), ptr(Da), ptr(Db), ptr(info)))
+                                        int(bool(force_density)), ptr(en), ptr(Da), ptr(Db), ptr(info)))
     else:
         check(lib().gansu_uhf_run(mol.handle, aux.handle if aux else None, params.handle, ptr(en), ptr(Da),
                                   ptr(Db), ptr(info)))
@@ -298,10 +299,12 @@ def write_molden(mol, C, eps, occ, path):
     check(lib().gansu_write_molden(mol.handle, ptr(C), ptr(eps), ptr(occ), C.shape[1], str(path).encode()))
 
 
-def rhf(mol, params, aux=None, density=None):
+def rhf(mol, params, aux=None, density=None, force_density=False):
     """Run the RHF SCF; returns dict(e_elec, e_total, iterations, fock_builds, converged, D).
     density = (Da, Db) (M x M each, D/2 each for a closed shell) starts the
-    loop from F(Da + Db) (HF::solve with densities, rhf.cu:127-135)."""
+    loop from F(Da + Db) when force_density=True or initial_guess=density
+    (HF::solve(Da, Db, force_density), rhf.cu:122-134); otherwise the
+    parameters' guess runs, as in the reference."""
     en = np.zeros(2)
     D = np.zeros((mol.n_basis, mol.n_basis))
     info = np.zeros(4, dtype=np.int32)
@@ -310,7 +313,7 @@ def rhf(mol, params, aux=None, density=None):
         if Da.shape != D.shape or Db.shape != D.shape:
             raise ValueError(f"densities must be {D.shape}")
         check(lib().gansu_rhf_run_guess(mol.handle, aux.handle if aux else None, params.handle, ptr(Da), ptr(Db),
-                                        ptr(en), ptr(D), ptr(info)))
+                                        int(bool(force_density)), ptr(en), ptr(D), ptr(info)))
     else:
         check(lib().gansu_rhf_run(mol.handle, aux.handle if aux else None, params.handle, ptr(en), ptr(D),
                                   ptr(info)))

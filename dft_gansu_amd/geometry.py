@@ -62,7 +62,7 @@ def optimize_geometry(mol, params=None, gtol=3e-4, max_steps=100, max_step=0.3, 
     def evaluate(xv):
         m = mol.with_coordinates(xv.reshape(-1, 3))
         D = last["D"] if warm_start else None
-        r = rhf_gradient(m, params, density=None if D is None else (0.5 * D, 0.5 * D))
+        r = rhf_gradient(m, params, density=None if D is None else (0.5 * D, 0.5 * D), force_density=True)
         if not r["converged"]:
             raise RuntimeError("SCF did not converge during the optimisation")
         last["D"] = r["D"]

@@ -115,7 +115,13 @@ void gansu_params_destroy(gansu_params* p);
  * precision=fp64|mixed, fp32_bound_threshold=.., ri_k=density|occupied) and
  * engine keys autotune=1|0 (time digestion tiles at precompute, stored path)
  * and graph=1|0 (four-centre builds replay a HIP graph captured once per set
- * of D/J/K/H/F pointers; the graph is re-captured when the pointers change).
+ * of D/J/K/H/F pointers; the graph is re-captured when the pointers change),
+ * nan_check=1|0 (eri_method=ri: an RHF build that forms F reads one flag
+ * word back and fails with GANSU_E_NUMERIC when F holds a NaN, as
+ * ERI_RI_RHF::compute_fock_matrix does, rhf.hpp:665-674; this is the one
+ * host synchronisation inside a build; 0 skips it).
+ * The engine runs on the device that was current at gansu_fock_create; each
+ * call makes it current and restores the caller's device on return.
  * aux* may be NULL/0 unless eri_method=ri.                                  */
 typedef struct gansu_fock_engine gansu_fock_engine;
 
@@ -202,22 +208,25 @@ int gansu_export_sad_cache(const gansu_molecule* mol, const gansu_molecule* aux_
  * converged, 0}.                                                            */
 int gansu_rhf_run(const gansu_molecule* mol, const gansu_molecule* aux_mol, const gansu_params* p, double* energies,
                   double* D_out, int* info);
-/* gansu_rhf_run started from caller densities (HF::solve with density
- * matrices, rhf.cu:127-135, InitialGuess_RHF_Density, rhf.hpp:441-454): with
- * Da0 and Db0 (host, M*M; for RHF each D/2) the loop starts from
- * F(Da0 + Db0), whatever initial_guess says; with both NULL it is
- * gansu_rhf_run (initial_guess=density then falls back to the core guess, as
- * the reference does).  One of them NULL is GANSU_E_ARG.                    */
+/* HF::solve(Da0, Db0, force_density) for RHF (rhf.cu:122-134,
+ * InitialGuess_RHF_Density, rhf.hpp:441-454): when force_density is
+ * non-zero or initial_guess=density, the loop starts from F(Da0 + Db0)
+ * (Da0, Db0 host M*M, for RHF each D/2); otherwise initial_guess (core, gwh,
+ * sad) runs and the densities are not used.  With both NULL it is
+ * gansu_rhf_run (initial_guess=density or force_density then fall back to
+ * the core guess with a message, as the reference does).  One of them NULL
+ * is GANSU_E_ARG.                                                           */
 int gansu_rhf_run_guess(const gansu_molecule* mol, const gansu_molecule* aux_mol, const gansu_params* p,
-                        const double* Da0, const double* Db0, double* energies, double* D_out, int* info);
+                        const double* Da0, const double* Db0, int force_density, double* energies, double* D_out,
+                        int* info);
 /* gansu_rhf_gradient started from caller densities as gansu_rhf_run_guess
- * (both NULL: the parameters' guess), also returning the converged density
- * (D_out, host M*M, may be NULL): a geometry optimiser passes each step's D/2,
- * D/2 to the next step, as the reference's optimisers pass densities to
- * HF::solve (rhf.cu:127-135).                                               */
+ * (same force_density rule), also returning the converged density (D_out,
+ * host M*M, may be NULL): a geometry optimiser passes each step's D/2, D/2
+ * with force_density = 1 to the next step, as the reference's optimisers
+ * pass densities to HF::solve (rhf.cu:122-134).                             */
 int gansu_rhf_gradient_guess(const gansu_molecule* mol, const gansu_molecule* aux_mol, const gansu_params* p,
-                             const double* Da0, const double* Db0, double* energies, double* grad, double* D_out,
-                             int* info);
+                             const double* Da0, const double* Db0, int force_density, double* energies, double* grad,
+                             double* D_out, int* info);
 /* gansu_rhf_run returning the final orbitals: C_out (host, M*M row-major,
  * MOs in columns, those of the converged density) and eps_out (host, M,
  * ascending orbital energies); either may be NULL.                          */
@@ -229,25 +238,30 @@ int gansu_rhf_run_mo(const gansu_molecule* mol, const gansu_molecule* aux_mol, c
  * contracted shell (raw contraction coefficients), [MO] blocks.             */
 int gansu_write_molden(const gansu_molecule* mol, const double* C, const double* eps, const double* occ, int n_mo,
                        const char* path);
-/* UHF SCF loop (uhf.cu:118-300 via hf.cu:245-314): core or GWH guess, per-spin
- * densities D_s = C_s,occ C_s,occ^T over the molecule's n_alpha / n_beta
- * electrons, E = 1/2 sum_s D_s.(H + F_s), DIIS per spin
- * (uhf.hpp:291-400; convergence_method=diis only, GANSU_E_UNSUPPORTED
- * otherwise).  Da_out, Db_out (host, M*M) may be NULL; energies and info as
- * gansu_rhf_run.                                                            */
+/* UHF SCF loop (uhf.cu:118-300 via hf.cu:245-314): core, GWH or SAD guess,
+ * each followed by the symmetry break of uhf.hpp:429-436 (the last row and
+ * column of C_beta zeroed), per-spin densities D_s = C_s,occ C_s,occ^T over
+ * the molecule's n_alpha / n_beta electrons, E = 1/2 sum_s D_s.(H + F_s),
+ * DIIS per spin (uhf.hpp:291-400) or damping per spin (uhf.hpp:196-258;
+ * optimaldamping is GANSU_E_PARAM, as uhf.cu:57-69 rejects it).  Da_out,
+ * Db_out (host, M*M) may be NULL; energies and info as gansu_rhf_run.      */
 int gansu_uhf_run(const gansu_molecule* mol, const gansu_molecule* aux_mol, const gansu_params* p, double* energies,
                   double* Da_out, double* Db_out, int* info);
-/* gansu_uhf_run started from caller spin densities Da0, Db0 (host, M*M):
- * F(Da0, Db0), then C and D per spin (InitialGuess_UHF_Density,
- * uhf.hpp:557-568); both NULL is gansu_uhf_run, one NULL GANSU_E_ARG.       */
+/* HF::solve(Da0, Db0, force_density) for UHF (uhf.cu:118-130): when
+ * force_density is non-zero or initial_guess=density, F(Da0, Db0) (host,
+ * M*M), then C per spin, the symmetry break, D and F
+ * (InitialGuess_UHF_Density, uhf.hpp:544-568); otherwise initial_guess runs.
+ * Both NULL is gansu_uhf_run, one NULL GANSU_E_ARG.                        */
 int gansu_uhf_run_guess(const gansu_molecule* mol, const gansu_molecule* aux_mol, const gansu_params* p,
-                        const double* Da0, const double* Db0, double* energies, double* Da, double* Db, int* info);
+                        const double* Da0, const double* Db0, int force_density, double* energies, double* Da,
+                        double* Db, int* info);
 /* ROHF SCF loop (rohf.cu:59-300 via hf.cu:245-314): n_beta closed and
  * n_alpha - n_beta open orbitals, core or GWH guess, D_c = 2 C_c C_c^T,
  * D_o = C_o C_o^T, the ROHF build with rohf_parameter_name,
- * E = 1/2 [D_c.(H + F_c) + D_o.H + 2 D_o.F_o], DIIS on the unified Fock
- * matrix (convergence_method=diis only).  Dc_out, Do_out (host, M*M) may be
- * NULL; energies and info as gansu_rhf_run.                                 */
+ * E = 1/2 [D_c.(H + F_c) + D_o.H + 2 D_o.F_o], DIIS, damping or optimal
+ * damping (rohf.hpp:216-306) on the unified Fock matrix and the total
+ * density.  Dc_out, Do_out (host, M*M) may be NULL; energies and info as
+ * gansu_rhf_run.                                                            */
 int gansu_rohf_run(const gansu_molecule* mol, const gansu_molecule* aux_mol, const gansu_params* p, double* energies,
                    double* Dc_out, double* Do_out, int* info);
 

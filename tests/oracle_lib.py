@@ -288,15 +288,91 @@ def rohf_fock(H, JKc, JKo, C, S, nc, no, coupling="roothaan"):
     return Fc, Fo, S @ C @ R @ C.T @ S
 
 
-def uhf(prims, types, norm, Z, xyz_bohr, na, nb, thr=1e-12, conv=1e-10, maxiter=200, diis_size=8, sad=None):
+def _damp(state, F, D, factor, optimal):
+    """One damping step (rhf.hpp:219-243, gpu::damping gpu_manager.cu:989-1007):
+    the first call stores (F, D); later calls return the damped F (and D) and
+    store them, the factor optimal (orc_optimal_damping) or constant."""
+    if not state:
+        state.extend([F.copy(), D.copy()])
+        return F, D
+    Fp, Dp = state
+    a = optimal_damping(F, Fp, D, Dp) if optimal else factor
+    F = (1.0 - a) * Fp + a * F
+    if optimal:
+        D = (1.0 - a) * Dp + a * D
+        state[1] = D.copy()
+    state[0] = F.copy()
+    return F, D
+
+
+def _diis(hist, e, F, size):
+    """DIIS extrapolation of F from the stored (error, F) pairs (rhf.hpp:298-327)."""
+    hist.append((e, F.copy()))
+    del hist[:-size]
+    n = len(hist)
+    if n > 1:
+        B = -np.ones((n + 1, n + 1))
+        B[n, n] = 0.0
+        for a in range(n):
+            for b in range(n):
+                B[a, b] = np.sum(hist[a][0] * hist[b][0])
+        rhs = np.zeros(n + 1)
+        rhs[n] = -1.0
+        try:  # a singular system keeps F as it is
+            c = np.linalg.solve(B, rhs)
+            F = sum(c[k] * hist[k][1] for k in range(n))
+        except np.linalg.LinAlgError:
+            pass
+    return F
+
+
+def rhf_damped(prims, types, norm, Z, xyz_bohr, nocc, optimal=False, factor=0.9, thr=1e-12, conv=1e-10,
+               maxiter=300):
+    """FP64 RHF on the host with Convergence_RHF_Damping (rhf.hpp:182-262):
+    core guess, then per iteration C, D = 2 C_occ C_occ^T, F = H + J - K/2,
+    E = 1/2 D.(H + F), stop on |dE| < conv, else damp F (and D when
+    optimal).  Returns (E_elec, E_total, D, iterations)."""
+    S, T, V = oneint(prims, norm, Z, xyz_bohr)
+    H = T + V
+    Q = schwarz(prims, types)
+    w, U = np.linalg.eigh(S)
+    X = U[:, w >= 1e-6] / np.sqrt(w[w >= 1e-6])
+
+    def dens(F):
+        C = X @ np.linalg.eigh(X.T @ F @ X)[1]
+        return 2.0 * C[:, :nocc] @ C[:, :nocc].T
+
+    def fock(D):
+        J, K = jk(prims, types, norm, Q, D, thr)
+        return H + J - 0.5 * K
+
+    F = fock(dens(H))
+    state, prev, E, it = [], None, 0.0, 0
+    while it < maxiter:
+        D = dens(F)
+        F = fock(D)
+        E = 0.5 * np.sum(D * (H + F))
+        if prev is not None and abs(E - prev) < conv:
+            break
+        it += 1
+        prev = E
+        F, D = _damp(state, F, D, factor, optimal)
+    return E, E + nuclear_repulsion(Z, xyz_bohr), D, it
+
+
+def uhf(prims, types, norm, Z, xyz_bohr, na, nb, thr=1e-12, conv=1e-10, maxiter=200, diis_size=8, sad=None,
+        damping=None):
     """FP64 UHF on the host from the oracle's integrals and J/K (uhf.cu:118-300):
-    core guess, X = U s^-1/2 (eigenvalues of S below 1e-6 dropped), per-spin
-    D_s = C_s,occ C_s,occ^T, F_s = H + J[Da + Db] - K[D_s],
+    core guess (C_alpha from H, C_beta a copy of it), the symmetry break of
+    every UHF guess (break_symmetry, uhf.hpp:429-436: the last row and last
+    column of C_beta zeroed), X = U s^-1/2 (eigenvalues of S below 1e-6
+    dropped), per-spin D_s = C_s,occ C_s,occ^T, F_s = H + J[Da + Db] - K[D_s],
     E = 1/2 sum_s D_s.(H + F_s), convergence on |dE|, DIIS per spin with the
-    error F_s D_s S - S D_s F_s.  Returns (E_elec, E_total, Da, Db, iterations).
+    error F_s D_s S - S D_s F_s, or, with damping = factor, constant damping
+    per spin (uhf.hpp:236-258).  Returns (E_elec, E_total, Da, Db, iterations).
     sad = (Da0, Db0) replaces the core guess by InitialGuess_UHF_SAD
-    (uhf.hpp:668-705): F(Da0, Db0), C per spin, the last row and last column
-    of C_beta zeroed (break_symmetry, uhf.hpp:429-436), then D and F."""
+    (uhf.hpp:668-705) or InitialGuess_UHF_Density (uhf.hpp:557-568):
+    F(Da0, Db0), C per spin, the symmetry break, then D and F."""
     S, T, V = oneint(prims, norm, Z, xyz_bohr)
     H = T + V
     Q = schwarz(prims, types)
@@ -317,15 +393,17 @@ def uhf(prims, types, norm, Z, xyz_bohr, na, nb, thr=1e-12, conv=1e-10, maxiter=
         return H + Ja + Jb - Ka, H + Ja + Jb - Kb
 
     if sad is None:
-        Da, Db = dens(coeff(H), na), dens(coeff(H), nb)
+        Ca = coeff(H)
+        Cb = Ca.copy()
     else:
         Fa, Fb = fock(*sad)
         Ca, Cb = coeff(Fa), coeff(Fb).copy()
-        Cb[-1, :] = 0.0
-        Cb[:, -1] = 0.0
-        Da, Db = dens(Ca, na), dens(Cb, nb)
+    Cb[-1, :] = 0.0
+    Cb[:, -1] = 0.0
+    Da, Db = dens(Ca, na), dens(Cb, nb)
     Fa, Fb = fock(Da, Db)
     hist = ([], [])
+    damp_state = ([], [])
     prev, E, it = None, 0.0, 0
     while it < maxiter:
         Da, Db = dens(coeff(Fa), na), dens(coeff(Fb), nb)
@@ -335,35 +413,23 @@ def uhf(prims, types, norm, Z, xyz_bohr, na, nb, thr=1e-12, conv=1e-10, maxiter=
             break
         it += 1
         prev = E
-        new = []
-        for F, D, hs in ((Fa, Da, hist[0]), (Fb, Db, hist[1])):
-            hs.append((F @ D @ S - S @ D @ F, F.copy()))
-            del hs[:-diis_size]
-            n = len(hs)
-            if n > 1:
-                B = -np.ones((n + 1, n + 1))
-                B[n, n] = 0.0
-                for a in range(n):
-                    for b in range(n):
-                        B[a, b] = np.sum(hs[a][0] * hs[b][0])
-                rhs = np.zeros(n + 1)
-                rhs[n] = -1.0
-                try:  # a singular system keeps F as it is
-                    c = np.linalg.solve(B, rhs)
-                    F = sum(c[k] * hs[k][1] for k in range(n))
-                except np.linalg.LinAlgError:
-                    pass
-            new.append(F)
-        Fa, Fb = new
+        if damping is not None:
+            Fa = _damp(damp_state[0], Fa, Da, damping, False)[0]
+            Fb = _damp(damp_state[1], Fb, Db, damping, False)[0]
+        else:
+            Fa = _diis(hist[0], Fa @ Da @ S - S @ Da @ Fa, Fa, diis_size)
+            Fb = _diis(hist[1], Fb @ Db @ S - S @ Db @ Fb, Fb, diis_size)
     return E, E + nuclear_repulsion(Z, xyz_bohr), Da, Db, it
 
 
 def rohf(prims, types, norm, Z, xyz_bohr, nc, no, coupling="roothaan", thr=1e-12, conv=1e-10, maxiter=300,
-         diis_size=8):
+         diis_size=8, damping=None, optimal=False):
     """FP64 ROHF on the host (rohf.cu:59-300): core guess, C from the unified
     Fock matrix, D_c = 2 C_c C_c^T, D_o = C_o C_o^T (gpu_kernels.cu:108-134),
     rohf_fock for F_c, F_o, F, E = 1/2 [D_c.(H + F_c) + D_o.H + 2 D_o.F_o]
-    (gpu_manager.cu:871-882), DIIS on F with the total density.
+    (gpu_manager.cu:871-882), DIIS on F with the total density, or damping
+    of F (constant factor `damping`, or optimal, which damps the total
+    density as well: Convergence_ROHF_Damping, rohf.hpp:216-306).
     Returns (E_elec, E_total, Dc, Do, iterations)."""
     S, T, V = oneint(prims, norm, Z, xyz_bohr)
     H = T + V
@@ -386,7 +452,7 @@ def rohf(prims, types, norm, Z, xyz_bohr, nc, no, coupling="roothaan", thr=1e-12
         return Dc, Do, Fu, E
 
     Dc, Do, F, E = step(H)
-    hist = []
+    hist, damp_state = [], []
     prev, it = None, 0
     while it < maxiter:
         Dc, Do, F, E = step(F)
@@ -395,22 +461,10 @@ def rohf(prims, types, norm, Z, xyz_bohr, nc, no, coupling="roothaan", thr=1e-12
         it += 1
         prev = E
         D = Dc + Do
-        hist.append((F @ D @ S - S @ D @ F, F.copy()))
-        del hist[:-diis_size]
-        n = len(hist)
-        if n > 1:
-            B = -np.ones((n + 1, n + 1))
-            B[n, n] = 0.0
-            for a in range(n):
-                for b in range(n):
-                    B[a, b] = np.sum(hist[a][0] * hist[b][0])
-            rhs = np.zeros(n + 1)
-            rhs[n] = -1.0
-            try:  # a singular system keeps F as it is
-                c = np.linalg.solve(B, rhs)
-                F = sum(c[k] * hist[k][1] for k in range(n))
-            except np.linalg.LinAlgError:
-                pass
+        if damping is not None or optimal:
+            F = _damp(damp_state, F, D, damping, optimal)[0]
+        else:
+            F = _diis(hist, F @ D @ S - S @ D @ F, F, diis_size)
     return E, E + nuclear_repulsion(Z, xyz_bohr), Dc, Do, it
 
 

@@ -63,6 +63,59 @@ def test_quirk_only_changes_odd_l_norms():
     assert np.all(a.norm[lof == 1] != b.norm[lof == 1])
 
 
+def test_quirk_magnitude_nitrogen_2p_sto3g():
+    """The integer division of basis_set.hpp:123 (n_b uses L/2 with int L)
+    scales every odd-L contracted function.  For the N 2p shell of STO-3G the
+    self-overlap of a p function with that factor is 1.5587242377 (closed
+    form: N^2 sum_ab c_a c_b (2 sqrt(a b) / (a + b))^(L + 3/2) over normalised
+    primitives); with the division applied to both primitives it would be
+    2.3582, and with none 1.  Checked on the library's norm and on the
+    oracle's overlap matrix."""
+    e = np.array([3.780455879, 0.8784966449, 0.2857143744])
+    c = np.array([0.1559162750, 0.6076837186, 0.3919573931])
+    g = np.sum(np.outer(c, c) * (2.0 * np.sqrt(np.outer(e, e)) / np.add.outer(e, e)) ** 2.5)
+    Z, xyz = np.array([7]), np.zeros((1, 3))
+    mol = Molecule.from_atoms(Z, xyz, basis_path("sto-3g"), normalization="reference")
+    p, t, n = O.build_molecule(basis_path("sto-3g"), Z, xyz, quirk=True)
+    assert mol.n_basis == 5
+    for mu in (2, 3, 4):  # 1s, 2s, then the three 2p functions
+        assert abs(mol.norm[mu] ** 2 * g - 1.5587242377) < 1e-9, mu
+    S = O.oneint(p, n, Z, xyz)[0]
+    np.testing.assert_allclose(np.diag(S)[2:], 1.5587242377, rtol=0, atol=1e-9)
+    np.testing.assert_allclose(np.diag(S)[:2], 1.0, rtol=0, atol=1e-12)
+    unit = Molecule.from_atoms(Z, xyz, basis_path("sto-3g"), normalization="unit")
+    assert abs(unit.norm[2] ** 2 * g - 1.0) < 1e-12
+
+
+RECIPES = os.path.join(ROOT, "tests", "golden", "recipes")
+
+
+def test_reference_recipes_load():
+    """The reference's nine recipes (parameter_recipe/*.txt, committed as
+    fixtures) load with the reference's syntax: keys and values lower-cased
+    except file names, true/false as 1/0 (parameter_manager.cpp:164-225)."""
+    from dft_gansu_amd import Parameters
+    names = sorted(f for f in os.listdir(RECIPES) if f.endswith(".txt"))
+    assert len(names) == 9
+    seen = set()
+    for f in names:
+        p = Parameters(os.path.join(RECIPES, f))
+        method, conv = p["method"], p["convergence_method"]
+        assert method in ("rhf", "uhf", "rohf") and conv in ("diis", "damping", "optimaldamping"), f
+        assert method == f.split("_")[0].lower(), f
+        seen.add((method, conv))
+        if conv == "damping":
+            assert float(p["damping_factor"]) == 0.9
+        if conv == "diis":
+            assert p["diis_size"] == "8" and p["diis_include_transform"] == "0"
+        assert p["gbsfilename"].endswith(".gbs")
+    assert seen == {("rhf", "diis"), ("rhf", "damping"), ("rhf", "optimaldamping"), ("uhf", "diis"),
+                    ("uhf", "damping"), ("rohf", "diis"), ("rohf", "damping"), ("rohf", "optimaldamping")}
+    ev = Parameters(os.path.join(RECIPES, "RHF_eval.txt"))
+    assert ev["initial_guess"] == "sad" and float(ev["convergence_energy_threshold"]) == 1e-10
+    assert float(ev["schwarz_screening_threshold"]) == 1e-13
+
+
 def test_xyz_reader_and_errors(tmp_path):
     f = tmp_path / "w.xyz"
     f.write_text("3\nwater\nO 0.0 0.0 0.11779\nH 0.0 0.75545 -0.47116\n\nH 0.0 -0.75545 -0.47116\n")

@@ -337,7 +337,9 @@ def test_uhf_oracle_known_answers():
         assert abs(O.uhf(pr, t, n, Z, origin, na, nb)[1] - ref) < 2e-6, Zv
     Z, xyz = gm.water(jitter=0.01, seed=11)
     pr, t, n = O.build_molecule(p, Z, xyz)
-    assert abs(O.uhf(pr, t, n, Z, xyz, 5, 5)[1] - O.rhf(pr, t, n, Z, xyz, 5)[1]) < 1e-10
+    # from the symmetry-broken guess the closed shell returns to RHF; converged
+    # to 1e-12 so that the two stop points agree within 1e-10
+    assert abs(O.uhf(pr, t, n, Z, xyz, 5, 5, conv=1e-12)[1] - O.rhf(pr, t, n, Z, xyz, 5, conv=1e-12)[1]) < 1e-10
 
 
 # ------------------------------------------------------------------ ROHF ----
@@ -359,3 +361,210 @@ def test_rohf_oracle_pins():
     Z, xyz = gm.water(jitter=0.01, seed=11)
     pr, t, n = O.build_molecule(p, Z, xyz)
     assert abs(O.rohf(pr, t, n, Z, xyz, 5, 0)[1] - O.rhf(pr, t, n, Z, xyz, 5)[1]) < 1e-10
+
+
+# --------------------------------------------------------------- Schwarz ----
+# The primitive-pair factor of schwarz.txt:239-478 is
+#   Q_ab = sqrt(max over Cartesian components of c_a^2 c_b^2 N_a^2 N_b^2 [ab|ab])
+# with the contraction coefficients and primitive norms included and the
+# contracted norms left out.  These tests recompute it through the
+# Obara-Saika recursion (orc_eri_prim_os, which shares no code with the
+# McMurchie-Davidson path of orc_schwarz) and the textbook primitive norm,
+# and check it bounds every primitive quartet.
+
+def _components(L):
+    return [(l, m, L - l - m) for l in range(L, -1, -1) for m in range(L - l, -1, -1)]
+
+
+def _prim_norm(a, c):
+    """Textbook normalisation of x^l y^m z^n exp(-a r^2)."""
+    return (2 * a / math.pi) ** 0.75 * (4 * a) ** (sum(c) / 2) / math.sqrt(
+        dfact(2 * c[0] - 1) * dfact(2 * c[1] - 1) * dfact(2 * c[2] - 1))
+
+
+def _reference_pairs(types):
+    """(a, b) primitive indices in the reference's pair order
+    (gpu_manager.cu:413-432): pair types s0 <= s1 row-major over types; within
+    a type the upper triangle r1 <= r2 (r2 outer) for s0 = s1, row-major
+    (r1 outer) otherwise."""
+    out = []
+    for s0 in range(len(types)):
+        for s1 in range(s0, len(types)):
+            c0, c1 = int(types[s0]["count"]), int(types[s1]["count"])
+            b0, b1 = int(types[s0]["start_index"]), int(types[s1]["start_index"])
+            if s0 == s1:
+                out += [(b0 + r1, b0 + r2) for r2 in range(c0) for r1 in range(r2 + 1)]
+            else:
+                out += [(b0 + r1, b1 + r2) for r1 in range(c0) for r2 in range(c1)]
+    return out
+
+
+def _os_block(prs, idx, exps_override=None):
+    """c N [ab|cd] for all components, primitives idx = (a, b, c, d); an index
+    of None is an s function of exponent 0 at its partner's centre (the
+    missing index of a 2-centre integral), with coefficient and norm 1."""
+    cen, ex, ls, cf = [], [], [], []
+    for k, i in enumerate(idx):
+        if i is None:
+            j = idx[k - 1]
+            cen.append([prs[j]["x"], prs[j]["y"], prs[j]["z"]])
+            ex.append(0.0)
+            ls.append(None)
+            cf.append(1.0)
+        else:
+            cen.append([prs[i]["x"], prs[i]["y"], prs[i]["z"]])
+            ex.append(float(prs[i]["exponent"]))
+            ls.append(int(prs[i]["shell_type"]))
+            cf.append(float(prs[i]["coefficient"]))
+    comps = [[(0, 0, 0)] if L is None else _components(L) for L in ls]
+    out = {}
+    for ca in comps[0]:
+        for cb in comps[1]:
+            for cc in comps[2]:
+                for cd in comps[3]:
+                    v = O.eri_prim_os(np.array(cen), np.array(ex), np.array(ca + cb + cc + cd))
+                    nrm = 1.0
+                    for k, c in enumerate((ca, cb, cc, cd)):
+                        if ls[k] is not None:
+                            nrm *= _prim_norm(ex[k], c) * cf[k]
+                    out[(ca, cb, cc, cd)] = nrm * v
+    return out
+
+
+def test_schwarz_factor_is_the_tight_primitive_bound():
+    """Q_ab equals sqrt(max_comp c^2 N^2 [ab|ab]) recomputed by Obara-Saika,
+    for pairs of every pair type (s..d on H2O/cc-pVDZ, quirk normalisation:
+    Q does not involve the contracted norm)."""
+    Z, xyz = gm.water(jitter=0.01, seed=3)
+    p, t, n = O.build_molecule(gm.basis_path("cc-pvdz"), Z, xyz)
+    Q = O.schwarz(p, t)
+    pairs = _reference_pairs(t)
+    assert len(pairs) == len(Q) == O.n_pairs(t)
+    rng = np.random.default_rng(5)
+    sample = sorted(set(rng.choice(len(pairs), 160, replace=False).tolist()) | {0, len(pairs) - 1})
+    checked_types = set()
+    for k in sample:
+        a, b = pairs[k]
+        blk = _os_block(p, (a, b, a, b))
+        best = max(v for (ca, cb, cc, cd), v in blk.items() if ca == cc and cb == cd)
+        assert best > 0.0
+        assert abs(Q[k] - math.sqrt(best)) <= 1e-12 * math.sqrt(best), (k, Q[k], math.sqrt(best))
+        checked_types.add((int(p[a]["shell_type"]), int(p[b]["shell_type"])))
+    assert {(0, 0), (0, 1), (1, 1), (0, 2), (1, 2), (2, 2)} <= checked_types
+
+
+def test_schwarz_factor_bounds_every_primitive_quartet():
+    """|c N [ab|cd]| <= Q_ab Q_cd for every component of sampled primitive
+    quartets (the screening test of int2e.cu:1681 drops a quartet only when
+    this product is below the threshold), with equality reached on the
+    diagonal (ab|ab) at the maximising component."""
+    Z, xyz = gm.water(jitter=0.01, seed=3)
+    p, t, n = O.build_molecule(gm.basis_path("cc-pvdz"), Z, xyz)
+    Q = O.schwarz(p, t)
+    pairs = _reference_pairs(t)
+    rng = np.random.default_rng(9)
+    for _ in range(60):
+        x, y = rng.integers(len(pairs), size=2)
+        (a, b), (c, d) = pairs[x], pairs[y]
+        blk = _os_block(p, (a, b, c, d))
+        assert max(abs(v) for v in blk.values()) <= Q[x] * Q[y] * (1 + 1e-12)
+        diag = _os_block(p, (a, b, a, b))
+        assert abs(max(diag.values()) - Q[x] ** 2) <= 1e-12 * Q[x] ** 2
+
+
+def test_schwarz_aux_factor_closed_form_and_os():
+    """Auxiliary factor sqrt(max_comp c^2 N^2 (a|a)) (schwarz.txt:479+): for an
+    s primitive the closed form (a|a) = 2 pi^(5/2) / (a^2 sqrt(2a)) times
+    c^2 N^2; for every shell type Obara-Saika with exponent-0 partners."""
+    Z, xyz = gm.water(jitter=0.01, seed=3)
+    ap, at, an = O.build_molecule(gm.basis_path("cc-pvdz-rifit"), Z, xyz, quirk=False)
+    Qa = O.schwarz_aux(ap)
+    for i in range(len(ap)):
+        a, L, c = float(ap[i]["exponent"]), int(ap[i]["shell_type"]), float(ap[i]["coefficient"])
+        if L == 0:
+            ref = c * c * _prim_norm(a, (0, 0, 0)) ** 2 * 2 * math.pi ** 2.5 / (a * a * math.sqrt(2 * a))
+            assert abs(Qa[i] - math.sqrt(ref)) <= 1e-12 * math.sqrt(ref)
+    rng = np.random.default_rng(2)
+    for L in range(4):
+        idx = [i for i in range(len(ap)) if int(ap[i]["shell_type"]) == L]
+        for i in rng.choice(idx, min(5, len(idx)), replace=False):
+            blk = _os_block(ap, (int(i), None, int(i), None))
+            best = max(v for (ca, cb, cc, cd), v in blk.items() if ca == cc)
+            assert abs(Qa[i] - math.sqrt(best)) <= 1e-12 * math.sqrt(best), (L, i)
+
+
+def test_screened_jk_within_the_dropped_bound():
+    """J and K with the threshold raised to 1e-6 stay within the bound the
+    dropped primitive quartets allow: each holds |value| <= Q_ab Q_cd < thr
+    per component, so an element moves by at most
+    8 thr max(N)^4 max|D| x (number of dropped quartets).  A factor that is too
+    small (a missing coefficient, a wrong norm power) drops larger
+    integrals and breaks it."""
+    Z, xyz = gm.water(jitter=0.01, seed=3)
+    p, t, n = O.build_molecule(gm.basis_path("cc-pvdz"), Z, xyz)
+    Q = O.schwarz(p, t)
+    D = random_density(len(n), seed=4)
+    J0, K0 = O.jk(p, t, n, Q, D, thr=0.0)
+    thr = 1e-6
+    J1, K1 = O.jk(p, t, n, Q, D, thr=thr)
+    # S = sum of Q_x Q_y over unordered pairs of primitive pairs with
+    # Q_x Q_y < thr; each such quartet moves one element of J or K by at
+    # most 8 (permutations) x 36 (components of the other pair) x Q_x Q_y x
+    # max(N)^4 x max|D|
+    qs = np.sort(Q)
+    pre = np.concatenate(([0.0], np.cumsum(qs)))
+    below = np.searchsorted(qs, thr / np.maximum(qs, 1e-300), side="left")
+    k = np.minimum(below, np.arange(1, len(qs) + 1))
+    S = float(np.sum(qs * pre[k]))
+    assert S > 0.0
+    bound = 8.0 * 36.0 * np.abs(n).max() ** 4 * np.abs(D).max() * S
+    dj, dk = np.abs(J1 - J0).max(), np.abs(K1 - K0).max()
+    assert dj <= bound and dk <= bound, (dj, dk, bound)
+    # the screening does drop something visible at this threshold
+    assert dj > 0.0 and dk > 0.0
+
+
+def test_uhf_symmetry_break_reaches_the_broken_symmetry_state():
+    """Every UHF guess zeroes the last row and column of C_beta
+    (break_symmetry, uhf.hpp:429-436).  For H2/STO-3G stretched to 4 bohr
+    that takes the core guess off the restricted solution: the UHF energy
+    lies far below RHF and near two STO-3G hydrogen atoms (2 x -0.466582 Eh),
+    with unequal spin densities; at 1.4 bohr UHF returns to RHF
+    (Szabo-Ostlund -1.1167 Eh)."""
+    p = gm.basis_path("sto-3g")
+    Z = np.array([1, 1])
+    for R, below in ((4.0, True), (1.4, False)):
+        xyz = np.array([[0.0, 0.0, 0.0], [0.0, 0.0, R]])
+        pr, t, n = O.build_molecule(p, Z, xyz)
+        er = O.rhf(pr, t, n, Z, xyz, 1, conv=1e-12)[1]
+        eu, Da, Db = (lambda r: (r[1], r[2], r[3]))(O.uhf(pr, t, n, Z, xyz, 1, 1, conv=1e-12, maxiter=300))
+        if below:
+            assert eu < er - 0.1, (eu, er)
+            assert abs(eu - 2 * -0.466582) < 5e-3, eu
+            assert np.abs(Da - Db).max() > 0.1
+        else:
+            assert abs(eu - er) < 1e-8 and abs(er - (-1.1167)) < 1e-4, (eu, er)
+
+
+def test_damped_scf_reaches_the_diis_energy():
+    """Constant and optimal damping (rhf.hpp:182-262, rohf.hpp:216-306,
+    uhf.hpp:196-258) change the path, not the fixed point: each damped SCF
+    ends on the DIIS energy (RHF water, UHF Li, ROHF O triplet; STO-3G)."""
+    p = gm.basis_path("sto-3g")
+    Z, xyz = gm.water(jitter=0.01, seed=11)
+    pr, t, n = O.build_molecule(p, Z, xyz)
+    e_diis = O.rhf(pr, t, n, Z, xyz, 5, conv=1e-12)[1]
+    for optimal in (False, True):
+        e, _, _, it = (lambda r: (r[1], r[2], r[2], r[3]))(O.rhf_damped(pr, t, n, Z, xyz, 5, optimal=optimal,
+                                                                          conv=1e-12))
+        assert abs(e - e_diis) < 1e-9, (optimal, e, e_diis, it)
+    origin = np.zeros((1, 3))
+    Zli = np.array([3])
+    pr, t, n = O.build_molecule(p, Zli, origin)
+    assert abs(O.uhf(pr, t, n, Zli, origin, 2, 1, damping=0.9, conv=1e-12, maxiter=500)[1]
+               - O.uhf(pr, t, n, Zli, origin, 2, 1, conv=1e-12)[1]) < 1e-9
+    Zo = np.array([8])
+    pr, t, n = O.build_molecule(p, Zo, origin)
+    e_ref = O.rohf(pr, t, n, Zo, origin, 3, 2, conv=1e-12)[1]
+    for kw in (dict(damping=0.9), dict(optimal=True)):
+        assert abs(O.rohf(pr, t, n, Zo, origin, 3, 2, conv=1e-12, maxiter=500, **kw)[1] - e_ref) < 1e-9, kw

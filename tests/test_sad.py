@@ -123,5 +123,5 @@ def test_density_restart_needs_both_densities():
     mol = gm.Molecule.from_atoms(Z, xyz, gm.basis_path("sto-3g"))
     Da = np.zeros((mol.n_basis, mol.n_basis))
     en = np.zeros(2)
-    rc = lib().gansu_rhf_run_guess(mol.handle, None, Parameters().handle, ptr(Da), None, ptr(en), None, None)
+    rc = lib().gansu_rhf_run_guess(mol.handle, None, Parameters().handle, ptr(Da), None, 1, ptr(en), None, None)
     assert rc == 1 and b"both" in lib().gansu_last_error()
