@@ -597,7 +597,7 @@ def test_damping_drivers_match_oracle(cuda):
     r = uhf(mol, Parameters(convergence_energy_threshold=1e-10, maxiter=500, convergence_method="damping"))
     assert r["converged"] and abs(r["e_total"] - ref[1]) < 1e-9 and abs(r["iterations"] - ref[4]) <= 1
     Zo = np.array([8])
-    mol = gm.Molecule.from_atoms(Zo, origin, gm.basis_path("sto-3g"), beta_to_alpha=2)
+    mol = gm.Molecule.from_atoms(Zo, origin, gm.basis_path("sto-3g"), beta_to_alpha=1)
     p, t, n = O.build_molecule(gm.basis_path("sto-3g"), Zo, origin)
     for kw, cm in ((dict(damping=0.9), "damping"), (dict(optimal=True), "optimaldamping")):
         ref = O.rohf(p, t, n, Zo, origin, 3, 2, conv=1e-10, maxiter=500, **kw)
