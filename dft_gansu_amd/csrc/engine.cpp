@@ -69,6 +69,13 @@ FockEngine::FockEngine(const gansu_primitive_shell* prims, size_t n_prims, const
     else if (kv.first == "autotune") autotune_ = kv.second != "0";
     else if (kv.first == "graph") use_graph_ = kv.second != "0";
     else if (kv.first == "nan_check") nan_check_ = kv.second != "0";
+    else if (kv.first == "ri_refresh") {
+      rik_refresh_ = std::stoi(kv.second);
+      if (rik_refresh_ < 0) throw Error(GANSU_E_PARAM, "ri_refresh must be >= 0");
+    } else if (kv.first == "ri_tol") {
+      rik_tol_ = std::stod(kv.second);
+      if (!(rik_tol_ >= 0.0)) throw Error(GANSU_E_PARAM, "ri_tol must be >= 0");
+    }
     else if (kv.first == "ri_k") {
       if (kv.second != "density" && kv.second != "occupied") throw Error(GANSU_E_PARAM, "ri_k must be density|occupied");
       ri_k_occ_ = kv.second == "occupied";
@@ -613,6 +620,7 @@ void FockEngine::stats(double* out) const {
   const double P = double(M_) * (M_ + 1) / 2;
   out[4] = P * (P + 1) / 2;
   if (method_ == "ri") out[5] = double(M_) * Maux_ * M_ * sizeof(double);
+  out[7] = ri_incremental_builds();
 }
 
 }  // namespace gansu

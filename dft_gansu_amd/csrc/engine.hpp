@@ -131,6 +131,7 @@ class FockEngine {
   int device() const { return dev_; }  // the device current when the engine was created
   void set_profiling(bool on);
   int graph_captures() const;  // graphs captured so far (the build replays the last one)
+  int ri_incremental_builds() const;  // precision=mixed RI builds formed as increments
   std::vector<double> last_profile();  // 5 values per launch
   // Two-electron part of the RHF nuclear gradient for density D (device,
   // M x M): d_grad (device, 3 natoms) = sum Gamma(D) d(ab|cd)/dX, overwritten.
@@ -149,7 +150,12 @@ class FockEngine {
   void tune_slab(QuartetClass& qc, int mode);
   SlabLaunch slab_launch(QuartetClass& qc, int mode);
   void precompute_ri();
-  void build_ri_k_occ(const double* d_C, int nocc, double* K, hipStream_t s);
+  // seed: also write the T chunks into the incremental state (precision=mixed)
+  void build_ri_k_occ(const double* d_C, int nocc, double* K, hipStream_t s, bool seed = false);
+  // precision=mixed incremental exchange (ri_mfma.hip); false: a full build is due
+  bool build_ri_incremental(const double* d_C, int nocc, double* J, double* K, hipStream_t s);
+  void rik_prepare(int nocc);
+  static int rik_chunks(int M, long Kd);  // split-k depth of the dK product
   void gradient_ri(const double* d_D, const double* d_C, int nocc, int natoms, double* d_grad, hipStream_t s);
   void ri_k_product(const double* A, int kdim, bool accumulate, double* K, hipStream_t s);
   void build_four_center(const double* d_D, hipStream_t s);
@@ -163,6 +169,13 @@ class FockEngine {
   bool mixed_ = false;
   double fp32_thr_ = 1e-6;
   bool ri_k_occ_ = false;  // RI-K from occupied orbitals when they are given
+  // precision=mixed RI: a full FP64 build after rik_refresh_ increments or
+  // when the aligned orbital change |E| / |X| exceeds rik_tol_
+  int rik_refresh_ = 8;
+  double rik_tol_ = 0.1;
+  int rik_ns_iters_ = 40;  // Newton-Schulz steps of the orbital alignment
+  double last_rik_rel_ = 0.0;
+  static constexpr int kRikMaxBlocks = 10;  // nocc <= 320 for the incremental path
   bool autotune_ = true;   // time candidate digestion tiles at precompute
   int M_ = 0;
   std::vector<gansu_primitive_shell> prims_;

@@ -56,9 +56,20 @@ struct FockEngine::RIState {
   DevBuf<AuxShell> shells;
   DevBuf<AuxPrim> prims;
   DevBuf<double> waux, L, B, W, J, K, T;
-  DevBuf<float> B32;  // FP32 copy of B for the J increments of precision=mixed
-  DevBuf<double> Wprev, dW, Jprev, Kpart;
-  DevBuf<int> jstate;  // {builds since FP64 J (-1: none yet), FP64 this build}
+  DevBuf<double> Kpart;
+  // precision=mixed: incremental builds (ri_mfma.hip).  FP32 copy of B
+  // (scaled by sB for the fp16 split), the previous build's factor X_prev,
+  // K, J, W, and T_A, T_E (M Maux nocc each) with B X_prev = T_A + T_E / 2.
+  DevBuf<float> B32;
+  float sB = 1.0f;
+  int rik_nocc = 0;   // nocc of the held state (0: none)
+  int rik_since = 0;  // incremental builds since the last FP64 build
+  int rik_incremental = 0, rik_full = 0;  // counts, for the caller's statistics
+  DevBuf<float> TA, TE, Zpart;
+  DevBuf<double> Xprev, Xn, Ework, Y, Y2, G, Kprev, Jprev, Wprev, dW, dWrow, nrm;
+  DevBuf<uint16_t> Ehi, Elo;
+  DevBuf<RikState> rst;
+  double* nrm_host = nullptr;
   std::vector<int> aux_atom_h;  // atom of each auxiliary shell
   DevBuf<int> aux_atom;
   DevBuf<int> nanflag;       // NaN check of F (device flag, pinned host copy)
@@ -67,6 +78,7 @@ struct FockEngine::RIState {
   ~RIState() {
     if (h) rocblas_destroy_handle(h);
     if (nan_host) (void)hipHostFree(nan_host);
+    if (nrm_host) (void)hipHostFree(nrm_host);
   }
 };
 
@@ -171,12 +183,22 @@ void FockEngine::precompute_ri() {
   if (mixed_) {
     r.B32.alloc(r.B.n);
     GANSU_HIP(launch_to_f32(r.B.ptr, r.B.n, r.B32.ptr, 0));
+    DevBuf<unsigned> mx(1);
+    GANSU_HIP(launch_rik_absmax(r.B32.ptr, r.B32.n, mx.ptr, 0));
+    unsigned bits = 0;
+    GANSU_HIP(hipMemcpy(&bits, mx.ptr, sizeof bits, hipMemcpyDeviceToHost));
+    float bmax;
+    std::memcpy(&bmax, &bits, sizeof bmax);
+    int e = 0;
+    if (bmax > 0.0f) std::frexp(bmax, &e);
+    r.sB = bmax > 0.0f ? std::ldexp(1.0f, 14 - e) : 1.0f;
     r.Wprev.alloc(Ma);
     r.dW.alloc(Ma);
     r.Jprev.alloc((size_t)M * M);
-    GANSU_HIP(launch_zero(r.Wprev.ptr, Ma, 0));
-    const int init[2] = {-1, 1};
-    r.jstate.upload(init, 2);
+    r.Kprev.alloc((size_t)M * M);
+    r.rst.alloc(1);
+    r.nrm.alloc(4);
+    GANSU_HIP(hipHostMalloc((void**)&r.nrm_host, 4 * sizeof(double), hipHostMallocDefault));
   }
   r.W.alloc(Ma);
   r.J.alloc((size_t)M * M);
@@ -205,23 +227,33 @@ void FockEngine::build_ri(const double* d_D, double* d_J, double* d_K, const dou
   double* K = d_K ? d_K : r.K.ptr;
   const double bytesB = double(M) * Ma * M * sizeof(double);
   const bool occ = d_C && (ri_k_occ_ || mixed_);
-  if (occ) {
-    // K and W from the occupied-orbital intermediate T = B X
-    build_ri_k_occ(d_C, nocc, K, s);
+  if (mixed_ && occ && rik_blocks(nocc) <= kRikMaxBlocks && build_ri_incremental(d_C, nocc, J, K, s)) {
+    // J, K and W formed incrementally from the previous build
   } else {
-    prof_begin(s, 80001, bytesB, 2.0 * M * Ma * M, (double)Ma);
-    GANSU_HIP(launch_ri_w(r.B.ptr, d_D, M, Ma, r.W.ptr, s));
-    prof_end(s);
-  }
-  if (mixed_) {
-    prof_begin(s, 80012, bytesB / 2, 2.0 * M * Ma * M, (double)M);
-    GANSU_HIP(launch_ri_j_incremental(r.B.ptr, r.B32.ptr, r.W.ptr, r.Wprev.ptr, r.dW.ptr, M, Ma, 1e-2, 8,
-                                      r.jstate.ptr, r.Jprev.ptr, J, s));
-  } else {
+    if (mixed_) ri_->rik_nocc = 0;  // a build the increments cannot follow
+    if (occ) {
+      // K and W from the occupied-orbital intermediate T = B X
+      const bool seed = mixed_ && rik_blocks(nocc) <= kRikMaxBlocks;
+      if (seed) rik_prepare(nocc);
+      build_ri_k_occ(d_C, nocc, K, s, seed);
+    } else {
+      prof_begin(s, 80001, bytesB, 2.0 * M * Ma * M, (double)Ma);
+      GANSU_HIP(launch_ri_w(r.B.ptr, d_D, M, Ma, r.W.ptr, s));
+      prof_end(s);
+    }
     prof_begin(s, 80002, bytesB, 2.0 * M * Ma * M, (double)M);
     GANSU_HIP(launch_ri_j(r.B.ptr, r.W.ptr, M, Ma, J, s));
+    prof_end(s);
+    if (mixed_ && occ && r.rik_nocc == nocc) {
+      // the state the next increments start from
+      GANSU_HIP(hipMemcpyAsync(r.Kprev.ptr, K, sizeof(double) * M * M, hipMemcpyDeviceToDevice, s));
+      GANSU_HIP(hipMemcpyAsync(r.Jprev.ptr, J, sizeof(double) * M * M, hipMemcpyDeviceToDevice, s));
+      GANSU_HIP(hipMemcpyAsync(r.Wprev.ptr, r.W.ptr, sizeof(double) * Ma, hipMemcpyDeviceToDevice, s));
+      GANSU_HIP(launch_rik_xocc(d_C, M, nocc, r.Xprev.ptr, s));
+      r.rik_since = 0;
+      ++r.rik_full;
+    }
   }
-  prof_end(s);
   const double one = 1.0, zero = 0.0;
   if (!occ) {
   for (int p0 = 0; p0 < Ma; p0 += r.pchunk) {
@@ -256,10 +288,144 @@ void FockEngine::build_ri(const double* d_D, double* d_J, double* d_K, const dou
   }
 }
 
+// ---- precision=mixed: incremental builds --------------------------------
+// State for nocc occupied orbitals: T_A, T_E (M Maux nocc FP32 each), X_prev,
+// the fp16 split of E, the split-k partials of dK.  Allocated on the first
+// build with this nocc (the one allocation of the mixed path; later builds
+// reuse it).
+void FockEngine::rik_prepare(int nocc) {
+  RIState& r = *ri_;
+  const int M = M_, Ma = Maux_;
+  const size_t tn = (size_t)M * Ma * nocc;
+  if (r.TA.n != tn) {
+    r.TA.alloc(tn);
+    r.TE.alloc(tn);
+    r.dWrow.alloc((size_t)M * Ma);
+    r.Xprev.alloc((size_t)M * nocc);
+    r.Xn.alloc((size_t)M * nocc);
+    r.Ework.alloc((size_t)M * nocc);
+    r.Y.alloc((size_t)nocc * nocc);
+    r.Y2.alloc((size_t)nocc * nocc);
+    r.G.alloc((size_t)nocc * nocc);
+    const int kpad = (M + 31) / 32 * 32;
+    r.Ehi.alloc((size_t)rik_cols(nocc) * kpad);
+    r.Elo.alloc((size_t)rik_cols(nocc) * kpad);
+    r.Zpart.alloc((size_t)rik_chunks(M, (long)Ma * nocc) * M * M);
+  }
+  r.rik_nocc = nocc;
+}
+
+// split-k depth of the dK product: enough workgroups to fill the chip
+// several times over, and at most 32768 k per chunk, so that each FP32
+// partial sums no more than 2048 MFMA steps before the FP64 reduction
+int FockEngine::rik_chunks(int M, long Kd) {
+  const long nt = (M + 127) / 128;
+  const long fill = (4096 + nt * nt - 1) / (nt * nt);
+  const long acc = (Kd + 32767) / 32768;
+  return (int)std::max<long>(1, std::min<long>({std::max(fill, acc), 256, std::max<long>(1, Kd / 32)}));
+}
+
+// One incremental build (ri_mfma.hip): align the new occupied factor to the
+// previous one (orthogonal Procrustes by Newton-Schulz), E = X_n R - X_prev,
+// then T_E = B E with the T_A / T_E update and dW, J = J_prev + B32 dW,
+// K = K_prev + Z + Z^T with Z = T_A T_E^T.  Returns false (nothing changed)
+// when a full FP64 build is due: no state for this nocc, `rik_refresh`
+// increments since the last one, a relative change |E| / |X| above
+// `rik_tol`, or an alignment that did not converge.
+bool FockEngine::build_ri_incremental(const double* d_C, int nocc, double* J, double* K, hipStream_t s) {
+  RIState& r = *ri_;
+  const int M = M_, Ma = Maux_;
+  if (r.rik_nocc != nocc || r.rik_since >= rik_refresh_) return false;
+  const double one = 1.0, zero = 0.0, mone = -1.0;
+  const size_t mn = (size_t)M * nocc;
+  prof_begin(s, 80020, 0.0, 0.0, 1.0);
+  GANSU_HIP(launch_rik_xocc(d_C, M, nocc, r.Xn.ptr, s));
+  // column-major views: X (M x nocc row-major) is X^T (nocc x M, ld nocc)
+  // A = X_n^T X_prev, scaled by 1/|A|_F into Y
+  blas_ok(rocblas_dgemm(r.h, rocblas_operation_none, rocblas_operation_transpose, nocc, nocc, M, &one, r.Xn.ptr,
+                        nocc, r.Xprev.ptr, nocc, &zero, r.Y.ptr, nocc),
+          "rocblas_dgemm(A)");
+  double fro = 0.0;
+  blas_ok(rocblas_set_pointer_mode(r.h, rocblas_pointer_mode_host), "rocblas_set_pointer_mode");
+  blas_ok(rocblas_dnrm2(r.h, nocc * nocc, r.Y.ptr, 1, &fro), "rocblas_dnrm2");
+  if (!(fro > 0.0)) {
+    prof_end(s);
+    return false;
+  }
+  const double inv = 1.0 / fro;
+  blas_ok(rocblas_dscal(r.h, nocc * nocc, &inv, r.Y.ptr, 1), "rocblas_dscal");
+  // Newton-Schulz: Y <- 1.5 Y - 0.5 Y (Y^T Y) converges to the orthogonal
+  // polar factor R of A (singular values grow by 1.5 per step while small)
+  const double c15 = 1.5, cm05 = -0.5;
+  for (int it = 0; it < rik_ns_iters_; ++it) {
+    blas_ok(rocblas_dgemm(r.h, rocblas_operation_transpose, rocblas_operation_none, nocc, nocc, nocc, &one, r.Y.ptr,
+                          nocc, r.Y.ptr, nocc, &zero, r.G.ptr, nocc),
+            "rocblas_dgemm(NS G)");
+    GANSU_HIP(hipMemcpyAsync(r.Y2.ptr, r.Y.ptr, sizeof(double) * nocc * nocc, hipMemcpyDeviceToDevice, s));
+    blas_ok(rocblas_dgemm(r.h, rocblas_operation_none, rocblas_operation_none, nocc, nocc, nocc, &cm05, r.Y.ptr, nocc,
+                          r.G.ptr, nocc, &c15, r.Y2.ptr, nocc),
+            "rocblas_dgemm(NS Y)");
+    std::swap(r.Y.ptr, r.Y2.ptr);
+  }
+  // orthogonality residual of R, then E = X_n R - X_prev (column-major
+  // E^T = R^T X_n^T - X_prev^T)
+  blas_ok(rocblas_dgemm(r.h, rocblas_operation_transpose, rocblas_operation_none, nocc, nocc, nocc, &one, r.Y.ptr,
+                        nocc, r.Y.ptr, nocc, &zero, r.G.ptr, nocc),
+          "rocblas_dgemm(R^T R)");
+  GANSU_HIP(hipMemcpyAsync(r.Ework.ptr, r.Xprev.ptr, sizeof(double) * mn, hipMemcpyDeviceToDevice, s));
+  blas_ok(rocblas_dgemm(r.h, rocblas_operation_transpose, rocblas_operation_none, nocc, M, nocc, &one, r.Y.ptr, nocc,
+                        r.Xn.ptr, nocc, &mone, r.Ework.ptr, nocc),
+          "rocblas_dgemm(E)");
+  GANSU_HIP(launch_rik_norms(r.Ework.ptr, r.Xprev.ptr, mn, r.G.ptr, nocc, r.nrm.ptr, s));
+  GANSU_HIP(hipMemcpyAsync(r.nrm_host, r.nrm.ptr, 3 * sizeof(double), hipMemcpyDeviceToHost, s));
+  GANSU_HIP(hipStreamSynchronize(s));
+  prof_end(s);
+  const double rel = std::sqrt(r.nrm_host[0] / std::max(r.nrm_host[1], 1e-300));
+  last_rik_rel_ = rel;
+  if (!(std::sqrt(r.nrm_host[2]) < 1e-10) || !(rel <= rik_tol_)) return false;
+
+  const int kpad = (M + 31) / 32 * 32;
+  prof_begin(s, 80021, 0.0, 0.0, 1.0);
+  GANSU_HIP(launch_rik_split_e(r.Ework.ptr, M, nocc, kpad, r.rst.ptr, r.Ehi.ptr, r.Elo.ptr, s));
+  prof_end(s);
+  // T_E = B E (+ T_A update, dW rows): 2 R M nocc flops, three fp16 products each
+  const long R = (long)M * Ma;
+  RikTArgs ta{r.B32.ptr, R, M, Ma, nocc, kpad, r.sB, r.Ehi.ptr, r.Elo.ptr, r.Ework.ptr, r.rst.ptr,
+              r.TA.ptr, r.TE.ptr, r.dWrow.ptr};
+  prof_begin(s, 80022, double(R) * M * 4 + double(R) * nocc * 16, 2.0 * R * M * nocc * 3.0, (double)R);
+  GANSU_HIP(launch_rik_tstep(ta, s));
+  prof_end(s);
+  prof_begin(s, 80023, double(M) * Ma * 8, 0.0, (double)Ma);
+  GANSU_HIP(launch_rik_dw(r.dWrow.ptr, M, Ma, r.dW.ptr, r.W.ptr, r.Wprev.ptr, s));
+  prof_end(s);
+  prof_begin(s, 80024, double(M) * Ma * M * 4, 2.0 * M * Ma * M, (double)M);
+  GANSU_HIP(launch_rik_j(r.B32.ptr, r.dW.ptr, M, Ma, r.Jprev.ptr, J, s));
+  prof_end(s);
+  // Z = T_A T_E^T over Kd = Maux nocc: 2 M^2 Kd flops, three fp16 products each
+  const long Kd = (long)Ma * nocc;
+  const int nch = rik_chunks(M, Kd);
+  const long kchunk = ((Kd + nch - 1) / nch + 31) / 32 * 32;
+  RikDkArgs da{r.TA.ptr, r.TE.ptr, M, Kd, kchunk, r.rst.ptr, r.Zpart.ptr};
+  prof_begin(s, 80025, 2.0 * double(M) * Kd * 4, 2.0 * M * (double)M * Kd * 3.0, (double)nch);
+  GANSU_HIP(launch_rik_dk(da, nch, s));
+  prof_end(s);
+  prof_begin(s, 80026, 0.0, 0.0, 1.0);
+  GANSU_HIP(launch_rik_reduce(r.Zpart.ptr, nch, M, r.Kprev.ptr, K, s));
+  GANSU_HIP(hipMemcpyAsync(r.Kprev.ptr, K, sizeof(double) * M * M, hipMemcpyDeviceToDevice, s));
+  // X_prev <- X_prev + E = X_n R, the factor that B X_prev = T_A + T_E / 2 now holds
+  blas_ok(rocblas_daxpy(r.h, (int)mn, &one, r.Ework.ptr, 1, r.Xprev.ptr, 1), "rocblas_daxpy(X)");
+  prof_end(s);
+  ++r.rik_since;
+  ++r.rik_incremental;
+  return true;
+}
+
+int FockEngine::ri_incremental_builds() const { return ri_ ? ri_->rik_incremental : 0; }
+
 // K = sum_P (B^P X)(B^P X)^T with X = sqrt(2) C_occ (M x nocc, row-major,
 // leading dimension M).  Per P-chunk: T_m = B_m[p0:p0+pc] X for every m
 // (batched DGEMM), then K += T_cat T_cat^T with T_cat = M x (pc nocc).
-void FockEngine::build_ri_k_occ(const double* d_C, int nocc, double* K, hipStream_t s) {
+void FockEngine::build_ri_k_occ(const double* d_C, int nocc, double* K, hipStream_t s, bool seed) {
   RIState& r = *ri_;
   const int M = M_, Ma = Maux_;
   const int pc_max = (int)std::max<size_t>(1, std::min<size_t>(Ma, r.T.n / ((size_t)M * nocc)));
@@ -277,6 +443,7 @@ void FockEngine::build_ri_k_occ(const double* d_C, int nocc, double* K, hipStrea
     prof_begin(s, 80007, 0.0, 2.0 * M * (double)pc * nocc, (double)pc);
     GANSU_HIP(launch_ri_w_from_t(r.T.ptr, d_C, M, pc, nocc, p0, r.W.ptr, s));
     prof_end(s);
+    if (seed) GANSU_HIP(launch_rik_seed(r.T.ptr, M, pc, nocc, p0, Ma, r.TA.ptr, r.TE.ptr, s));
     ri_k_product(r.T.ptr, pc * nocc, p0 > 0, K, s);
   }
   GANSU_HIP(launch_mirror_upper(K, M, s));

@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <cstddef>
+#include <cstdint>
 
 #include "fock_kernels.hpp"
 
@@ -93,5 +94,49 @@ hipError_t launch_ri_j_incremental(const double* B, const float* B32, const doub
                                    int M, int Maux, double rel, int refresh, int* st, double* Jprev, double* J,
                                    hipStream_t s);
 hipError_t launch_fock_ri(const double* H, const double* J, const double* K, size_t n, double* F, hipStream_t s);
+
+// ---- incremental RI exchange on the fp16 matrix cores (ri_mfma.hip) ----
+// Device-side scalars of one incremental build.
+struct RikState {
+  float sE, emax;            // scale and max|E| of the orbital change
+  unsigned ta_max, te_max;   // max|T_A|, max|T_E| (float bits, atomicMax)
+};
+struct RikTArgs {
+  const float* B32;   // R x M (rows m Maux + P)
+  long R;
+  int M, Maux, nocc, kpad;
+  float sB;           // power-of-two scale of B32
+  const uint16_t* Ehi;  // fp16 bits, E^T [rik_cols(nocc)][kpad]
+  const uint16_t* Elo;
+  const double* E;    // M x nocc, row-major
+  RikState* st;
+  float* TA;          // R x nocc
+  float* TE;
+  double* dWrow;      // R
+};
+struct RikDkArgs {
+  const float* TA;    // M x Kd (Kd = Maux nocc)
+  const float* TE;
+  int M;
+  long Kd, kchunk;
+  const RikState* st;
+  float* Zpart;       // nchunk x M x M
+};
+int rik_blocks(int nocc);  // 32-column blocks of the orbital change (at most 10)
+int rik_cols(int nocc);    // columns of the fp16 E^T split (whole 64-column pairs of blocks)
+hipError_t launch_rik_split_e(const double* E, int M, int nocc, int kpad, RikState* st, uint16_t* Ehi, uint16_t* Elo,
+                              hipStream_t s);
+hipError_t launch_rik_tstep(const RikTArgs& a, hipStream_t s);
+hipError_t launch_rik_dk(const RikDkArgs& a, int nchunk, hipStream_t s);
+hipError_t launch_rik_reduce(const float* Zpart, int nchunk, int M, const double* Kprev, double* K, hipStream_t s);
+hipError_t launch_rik_dw(const double* dWrow, int M, int Maux, double* dW, double* W, double* Wprev, hipStream_t s);
+hipError_t launch_rik_j(const float* B32, const double* dW, int M, int Maux, double* Jprev, double* J, hipStream_t s);
+hipError_t launch_rik_seed(const double* T, int M, int pc, int nocc, int p0, int Maux, float* TA, float* TE,
+                           hipStream_t s);
+hipError_t launch_rik_xocc(const double* C, int M, int nocc, double* X, hipStream_t s);
+// out[0] = |x|^2, out[1] = |y|^2 (n entries), out[2] = |G - I|^2 (G no x no)
+hipError_t launch_rik_norms(const double* x, const double* y, size_t n, const double* G, int no, double* out,
+                            hipStream_t s);
+hipError_t launch_rik_absmax(const float* x, size_t n, unsigned* w, hipStream_t s);
 
 }  // namespace gansu

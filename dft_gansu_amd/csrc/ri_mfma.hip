@@ -1,0 +1,609 @@
+// Incremental RI exchange on the fp16 matrix cores of gfx950.
+//
+// For occupied factors X (D = X X^T, X = sqrt(2) C_occ) the RI exchange is
+// K(X) = sum_P (B^P X)(B^P X)^T.  Between two builds the occupied space
+// moves little: with X_n rotated onto the previous factor (orthogonal
+// Procrustes, R = polar(X_n^T X_prev)), E = X_n R - X_prev is small and
+//   D_n - D_prev = A E^T + E A^T,   A = (X_n R + X_prev) / 2,
+//   K_n - K_prev = T_A T_E^T + T_E T_A^T,  T_A = B A, T_E = B E,
+// with T_A = T_prev + T_E / 2 where T_prev = B X_prev is kept from the last
+// build.  Two kernels carry the flops:
+//   k_rik_tstep:  T_E = B E over all rows (m, P) of B (FP32 copy of B, E in
+//                 fp16 hi/lo), updating T_A, T_E in place and the fitted
+//                 density change dW_P = 2 sum_{n,i} T_A[n][P][i] E[n][i];
+//   k_rik_dk:     Z = T_A T_E^T (K dimension Maux nocc), split-k partials;
+//                 K_n = K_prev + Z + Z^T is summed in FP64 (k_rik_reduce).
+// Each product runs as three fp16 MFMAs on a hi/lo split of both operands
+// (x s = hi + lo, hi = fp16(x s), lo = fp16(x s - hi); hi*hi + hi*lo + lo*hi
+// carries 22 bits), with FP32 accumulation; operands are scaled by powers of
+// two so that their largest element sits at 2^14.  The rounding error of the
+// increment scales with |E|, so the FP64 Fock matrix accumulates K_prev plus
+// small, accurately formed increments; the caller refreshes K in FP64 from
+// time to time.
+//
+// MFMA v_mfma_f32_32x32x16_f16: lane l holds row (or column) l % 32 of an
+// operand and 8 consecutive k of k-group l / 32; the 16 results of lane l
+// are rows 8 (r / 4) + 4 (l / 32) + r % 4, column l % 32.  A and B share the
+// k mapping, so any hardware permutation of k within a group cancels.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+
+#include "ri_kernels.hpp"
+
+namespace gansu {
+
+namespace {
+
+typedef _Float16 h8 __attribute__((ext_vector_type(8)));
+typedef float f16x __attribute__((ext_vector_type(16)));
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+
+__device__ inline f16x mfma(const h8& a, const h8& b, const f16x& c) {
+  return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0);
+}
+
+// power-of-two scale taking max|x| to at most 2^14 (1 when max is 0)
+__device__ inline float op_scale(float mx) {
+  if (!(mx > 0.0f)) return 1.0f;
+  int e;
+  frexpf(mx, &e);  // mx = f 2^e, 0.5 <= f < 1
+  return ldexpf(1.0f, 14 - e);
+}
+
+__device__ inline void split1(float x, float s, _Float16& hi, _Float16& lo) {
+  const float y = x * s;
+  hi = (_Float16)y;
+  lo = (_Float16)(y - (float)hi);
+}
+
+// maximum of non-negative floats through their bit patterns
+__device__ inline void atomic_max_pos(unsigned* w, float v) { atomicMax(w, __float_as_uint(v)); }
+
+// ------------------------------------------------------------ E split -----
+// One workgroup: max|E| -> scale sE (st->sE), then E^T in fp16 hi/lo,
+// [c][k] with c < ncol_pad, k < kpad, zero outside nocc x M.
+__global__ __launch_bounds__(1024) void k_rik_split_e(const double* __restrict__ E, int M, int nocc, int ncol_pad,
+                                                      int kpad, RikState* st, uint16_t* __restrict__ Ehi_,
+                                                      uint16_t* __restrict__ Elo_) {
+  _Float16* Ehi = reinterpret_cast<_Float16*>(Ehi_);
+  _Float16* Elo = reinterpret_cast<_Float16*>(Elo_);
+  __shared__ float red[1024];
+  float mx = 0.0f;
+  const size_t n = (size_t)M * nocc;
+  for (size_t i = threadIdx.x; i < n; i += blockDim.x) mx = fmaxf(mx, fabsf((float)E[i]));
+  red[threadIdx.x] = mx;
+  __syncthreads();
+  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + o]);
+    __syncthreads();
+  }
+  const float s = op_scale(red[0]);
+  if (threadIdx.x == 0) {
+    st->sE = s;
+    st->emax = red[0];
+  }
+  const size_t tot = (size_t)ncol_pad * kpad;
+  for (size_t i = threadIdx.x; i < tot; i += blockDim.x) {
+    const int c = (int)(i / kpad), k = (int)(i % kpad);
+    _Float16 hi = 0, lo = 0;
+    if (c < nocc && k < M) split1((float)E[(size_t)k * nocc + c], s, hi, lo);
+    Ehi[i] = hi;
+    Elo[i] = lo;
+  }
+}
+
+// ------------------------------------------------------------ T update -----
+// Rows r = m Maux + P of B32 (R x M, row-major).  A workgroup of 4 waves
+// takes 64 rows and 64 NBW columns of E: wave w computes rows 32 (w & 1) and
+// column half w >> 1 (NBW blocks of 32).  Per stage of 32 k, the E^T chunk
+// (all the workgroup's columns, fp16 hi/lo) sits in LDS; the next stage's
+// E^T pieces and B32 values are loaded into registers while the MFMAs run
+// and reach LDS after a barrier, so no wave waits on a global load inside
+// the stage.
+template <int NBW>
+__global__ __launch_bounds__(256) void k_rik_tstep(RikTArgs a) {
+  constexpr int NCOL = 64 * NBW;           // columns of the workgroup
+  constexpr int NPIECE = 2 * NCOL * 4;     // 16-byte pieces of one E^T stage
+  constexpr int NPT = NPIECE / 256;        // per thread (= 2 NBW)
+  __shared__ __attribute__((aligned(16))) _Float16 lds[2][NCOL][40];  // [hi/lo][col][k] (+8 pad)
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int g = lane >> 5, li = lane & 31;
+  const int rg = wave & 1, ch = wave >> 1;
+  const long row0 = (long)blockIdx.x * 64 + rg * 32;
+  const long myrow = row0 + li;
+  const bool rowok = myrow < a.R;
+  const float* brow = a.B32 + (size_t)(rowok ? myrow : 0) * a.M;
+  const float sB = a.sB, sE = a.st->sE;
+  const int nk = a.kpad / 32;
+  const bool vec = (a.M & 3) == 0;  // rows start 16-byte aligned
+
+  f16x acc[NBW];
+#pragma unroll
+  for (int b = 0; b < NBW; ++b)
+#pragma unroll
+    for (int r = 0; r < 16; ++r) acc[b][r] = 0.0f;
+
+  // E^T stage pieces of this thread: piece p = t + 256 j -> (hi/lo, column, 8-k quarter);
+  // the source offset without the stage term and the LDS slot are fixed
+  u32x4 ereg[NPT];
+#define RIK_EPIECE(j) const int piece_ = t + 256 * (j), hl_ = piece_ / (NCOL * 4), rem_ = piece_ % (NCOL * 4)
+#define RIK_ESRC(kc) (*(const u32x4*)((hl_ ? a.Elo : a.Ehi) + (size_t)(rem_ >> 2) * a.kpad + (kc) * 32 + (rem_ & 3) * 8))
+#define RIK_EDST (*(u32x4*)&lds[hl_][rem_ >> 2][(rem_ & 3) * 8])
+  float areg[16];
+  auto load_a = [&](int kc) {
+    const int k0 = kc * 32;
+#pragma unroll
+    for (int ks = 0; ks < 2; ++ks) {
+      const int k = k0 + ks * 16 + g * 8;
+      if (vec && rowok && k + 8 <= a.M) {
+        const float4 v0 = *(const float4*)(brow + k), v1 = *(const float4*)(brow + k + 4);
+        areg[ks * 8 + 0] = v0.x; areg[ks * 8 + 1] = v0.y; areg[ks * 8 + 2] = v0.z; areg[ks * 8 + 3] = v0.w;
+        areg[ks * 8 + 4] = v1.x; areg[ks * 8 + 5] = v1.y; areg[ks * 8 + 6] = v1.z; areg[ks * 8 + 7] = v1.w;
+      } else {
+#pragma unroll
+        for (int e = 0; e < 8; ++e) areg[ks * 8 + e] = (rowok && k + e < a.M) ? brow[k + e] : 0.0f;
+      }
+    }
+  };
+
+#pragma unroll
+  for (int j = 0; j < NPT; ++j) {
+    RIK_EPIECE(j);
+    ereg[j] = RIK_ESRC(0);
+  }
+  load_a(0);
+#pragma unroll
+  for (int j = 0; j < NPT; ++j) {
+    RIK_EPIECE(j);
+    RIK_EDST = ereg[j];
+  }
+  __syncthreads();
+  for (int kc = 0; kc < nk; ++kc) {
+    h8 ahi[2], alo[2];
+#pragma unroll
+    for (int ks = 0; ks < 2; ++ks)
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        _Float16 h, l;
+        split1(areg[ks * 8 + e], sB, h, l);
+        ahi[ks][e] = h;
+        alo[ks][e] = l;
+      }
+    const bool more = kc + 1 < nk;
+    if (more) {
+#pragma unroll
+      for (int j = 0; j < NPT; ++j) {
+        RIK_EPIECE(j);
+        ereg[j] = RIK_ESRC(kc + 1);
+      }
+      load_a(kc + 1);
+    }
+#pragma unroll
+    for (int ks = 0; ks < 2; ++ks)
+#pragma unroll
+      for (int b = 0; b < NBW; ++b) {
+        const int col = ch * 32 * NBW + b * 32 + li;
+        const h8 bhi = *(const h8*)&lds[0][col][ks * 16 + g * 8];
+        const h8 blo = *(const h8*)&lds[1][col][ks * 16 + g * 8];
+        acc[b] = mfma(ahi[ks], bhi, acc[b]);
+        acc[b] = mfma(ahi[ks], blo, acc[b]);
+        acc[b] = mfma(alo[ks], bhi, acc[b]);
+      }
+    if (more) {
+      __syncthreads();
+#pragma unroll
+      for (int j = 0; j < NPT; ++j) {
+        RIK_EPIECE(j);
+        RIK_EDST = ereg[j];
+      }
+      __syncthreads();
+    }
+  }
+#undef RIK_EPIECE
+#undef RIK_ESRC
+#undef RIK_EDST
+
+  // epilogue: T_E, T_A = T_prev + T_E / 2 (T_prev = T_A_old + T_E_old / 2),
+  // row sums of T_A E for dW, maxima for the next product's scales
+  const float inv = 1.0f / (sB * sE);
+  float part[16];
+#pragma unroll
+  for (int r = 0; r < 16; ++r) part[r] = 0.0f;
+  float mta = 0.0f, mte = 0.0f;
+#pragma unroll
+  for (int b = 0; b < NBW; ++b) {
+    const int c = ch * 32 * NBW + b * 32 + li;
+    if (c >= a.nocc) continue;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const long row = row0 + 8 * (r >> 2) + 4 * g + (r & 3);
+      if (row >= a.R) continue;
+      const size_t idx = (size_t)row * a.nocc + c;
+      const float te = acc[b][r] * inv;
+      const float tp = a.TA[idx] + 0.5f * a.TE[idx];
+      const float ta = tp + 0.5f * te;
+      a.TA[idx] = ta;
+      a.TE[idx] = te;
+      mta = fmaxf(mta, fabsf(ta));
+      mte = fmaxf(mte, fabsf(te));
+      const int n = (int)(row / a.Maux);
+      part[r] += ta * (float)a.E[(size_t)n * a.nocc + c];
+    }
+  }
+  // rows are shared by the 32 lanes of a k-group (and by the two column
+  // halves): reduce over the lanes, add the halves through dWrow atomics
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    float v = part[r];
+#pragma unroll
+    for (int o = 16; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    const long row = row0 + 8 * (r >> 2) + 4 * g + (r & 3);
+    if (li == 0 && row < a.R) atomicAdd(&a.dWrow[row], 2.0 * (double)v);
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    mta = fmaxf(mta, __shfl_xor(mta, o, 64));
+    mte = fmaxf(mte, __shfl_xor(mte, o, 64));
+  }
+  if (lane == 0) {
+    atomic_max_pos(&a.st->ta_max, mta);
+    atomic_max_pos(&a.st->te_max, mte);
+  }
+}
+
+// --------------------------------------------------------------- dK -------
+// Zpart[chunk][m][n] = sum over the chunk's k of T_A[m][k] T_E[n][k]
+// (K dimension Kd = Maux nocc).  Workgroup tile 128 x 128, four waves of
+// 64 x 64 (2 x 2 MFMA blocks), 32-k stages: the stage in LDS feeds the
+// MFMAs while the next one is loaded into registers; it is split into
+// fp16 hi/lo on its way into LDS between two barriers.
+constexpr int kDkTile = 128;
+
+__global__ __launch_bounds__(256) void k_rik_dk(RikDkArgs a) {
+  __shared__ __attribute__((aligned(16))) _Float16 sa[2][kDkTile][40];  // [hi/lo][row][k]
+  __shared__ __attribute__((aligned(16))) _Float16 sb[2][kDkTile][40];
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int g = lane >> 5, li = lane & 31;
+  const int wm = wave >> 1, wn = wave & 1;
+  const int ntile = (a.M + kDkTile - 1) / kDkTile;
+  const int tm = blockIdx.x / ntile, tn = blockIdx.x % ntile;
+  const long k_begin = (long)blockIdx.y * a.kchunk;
+  const long k_end = k_begin + a.kchunk < a.Kd ? k_begin + a.kchunk : a.Kd;
+  const int nst = k_end > k_begin ? (int)((k_end - k_begin + 31) / 32) : 0;
+  const bool vec = (a.Kd & 3) == 0;  // rows start 16-byte aligned
+  const float sA = op_scale(__uint_as_float(a.st->ta_max));
+  const float sE = op_scale(__uint_as_float(a.st->te_max));
+
+  // a stage: 128 rows x 32 k of T_A (A side) and of T_E (B side), 1024
+  // float4 each, 4 per thread; row r = idx / 8, k offset 4 (idx % 8)
+  float4 va[4], vb[4];
+  auto fetch = [&](int s) {
+    const long k0 = k_begin + (long)s * 32;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int idx = t + 256 * j, r = idx >> 3, q = idx & 7;
+      const long k = k0 + q * 4;
+      const int ra = tm * kDkTile + r, rb = tn * kDkTile + r;
+      float4 x = make_float4(0.f, 0.f, 0.f, 0.f), y = x;
+      if (vec && k + 4 <= k_end) {
+        if (ra < a.M) x = *(const float4*)(a.TA + (size_t)ra * a.Kd + k);
+        if (rb < a.M) y = *(const float4*)(a.TE + (size_t)rb * a.Kd + k);
+      } else {
+        float xa[4] = {0.f, 0.f, 0.f, 0.f}, yb[4] = {0.f, 0.f, 0.f, 0.f};
+        for (int e = 0; e < 4; ++e)
+          if (k + e < k_end) {
+            if (ra < a.M) xa[e] = a.TA[(size_t)ra * a.Kd + k + e];
+            if (rb < a.M) yb[e] = a.TE[(size_t)rb * a.Kd + k + e];
+          }
+        x = make_float4(xa[0], xa[1], xa[2], xa[3]);
+        y = make_float4(yb[0], yb[1], yb[2], yb[3]);
+      }
+      va[j] = x;
+      vb[j] = y;
+    }
+  };
+  // split four floats into hi/lo halves and store each as one 8-byte word
+  auto put4 = [&](const float4& v, float s, _Float16* hi, _Float16* lo) {
+    typedef _Float16 h4 __attribute__((ext_vector_type(4)));
+    h4 h, l;
+    const float x[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      _Float16 p, q;
+      split1(x[e], s, p, q);
+      h[e] = p;
+      l[e] = q;
+    }
+    *(h4*)hi = h;
+    *(h4*)lo = l;
+  };
+  auto stash = [&]() {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int idx = t + 256 * j, r = idx >> 3, q = idx & 7;
+      put4(va[j], sA, &sa[0][r][q * 4], &sa[1][r][q * 4]);
+      put4(vb[j], sE, &sb[0][r][q * 4], &sb[1][r][q * 4]);
+    }
+  };
+
+  f16x acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
+
+  if (nst > 0) {
+    fetch(0);
+    stash();
+  }
+  __syncthreads();
+  for (int s = 0; s < nst; ++s) {
+    const bool more = s + 1 < nst;
+    if (more) fetch(s + 1);
+#pragma unroll
+    for (int ks = 0; ks < 2; ++ks) {
+      h8 ahi[2], alo[2], bhi[2], blo[2];
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+        const int r = wm * 64 + i * 32 + li;
+        ahi[i] = *(const h8*)&sa[0][r][ks * 16 + g * 8];
+        alo[i] = *(const h8*)&sa[1][r][ks * 16 + g * 8];
+      }
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        const int r = wn * 64 + j * 32 + li;
+        bhi[j] = *(const h8*)&sb[0][r][ks * 16 + g * 8];
+        blo[j] = *(const h8*)&sb[1][r][ks * 16 + g * 8];
+      }
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = mfma(ahi[i], bhi[j], acc[i][j]);
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = mfma(ahi[i], blo[j], acc[i][j]);
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = mfma(alo[i], bhi[j], acc[i][j]);
+    }
+    if (more) {
+      __syncthreads();
+      stash();
+      __syncthreads();
+    }
+  }
+
+  const float inv = 1.0f / (sA * sE);
+  float* Z = a.Zpart + (size_t)blockIdx.y * a.M * a.M;
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int col = tn * kDkTile + wn * 64 + j * 32 + li;
+      if (col >= a.M) continue;
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int row = tm * kDkTile + wm * 64 + i * 32 + 8 * (r >> 2) + 4 * g + (r & 3);
+        if (row < a.M) Z[(size_t)row * a.M + col] = acc[i][j][r] * inv;
+      }
+    }
+}
+
+// K = Kprev + S + S^T with S = sum over chunks of Zpart, in FP64; 32 x 32
+// tiles through LDS so that both S[m][n] and S[n][m] are read coalesced.
+__global__ __launch_bounds__(256) void k_rik_reduce(const float* __restrict__ Zpart, int nchunk, int M,
+                                                    const double* __restrict__ Kprev, double* __restrict__ K) {
+  __shared__ double st[32][33];
+  const int ntile = (M + 31) / 32;
+  const int bi = blockIdx.x / ntile, bj = blockIdx.x % ntile;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+  const size_t mm = (size_t)M * M;
+  // S^T tile: rows bj*32.., cols bi*32.. of S, stored transposed
+  for (int y = ty; y < 32; y += 8) {
+    const int r = bj * 32 + y, c = bi * 32 + tx;
+    double s = 0.0;
+    if (r < M && c < M)
+      for (int ch = 0; ch < nchunk; ++ch) s += (double)Zpart[ch * mm + (size_t)r * M + c];
+    st[tx][y] = s;
+  }
+  __syncthreads();
+  for (int y = ty; y < 32; y += 8) {
+    const int r = bi * 32 + y, c = bj * 32 + tx;
+    if (r >= M || c >= M) continue;
+    double s = 0.0;
+    for (int ch = 0; ch < nchunk; ++ch) s += (double)Zpart[ch * mm + (size_t)r * M + c];
+    K[(size_t)r * M + c] = Kprev[(size_t)r * M + c] + s + st[y][tx];
+  }
+}
+
+// dW[P] = sum_m dWrow[m Maux + P]; W = Wprev + dW; Wprev <- W
+__global__ void k_rik_dw(const double* __restrict__ dWrow, int M, int Maux, double* __restrict__ dW,
+                         double* __restrict__ W, double* __restrict__ Wprev) {
+  const int P = blockIdx.x * blockDim.x + threadIdx.x;
+  if (P >= Maux) return;
+  double s = 0.0;
+  for (int m = 0; m < M; ++m) s += dWrow[(size_t)m * Maux + P];
+  dW[P] = s;
+  const double w = Wprev[P] + s;
+  W[P] = w;
+  Wprev[P] = w;
+}
+
+// J[m][n] = Jprev[m][n] + sum_P dW_P B32[m][P][n]; Jprev <- J
+__global__ void k_rik_j(const float* __restrict__ B32, const double* __restrict__ dW, int M, int Maux,
+                        double* __restrict__ Jprev, double* __restrict__ J) {
+  const int m = blockIdx.x;
+  for (int n = threadIdx.x; n < M; n += blockDim.x) {
+    double s = 0.0;
+    const float* col = B32 + (size_t)m * Maux * M + n;
+    for (int P = 0; P < Maux; ++P) s += dW[P] * (double)col[(size_t)P * M];
+    const double v = Jprev[(size_t)m * M + n] + s;
+    J[(size_t)m * M + n] = v;
+    Jprev[(size_t)m * M + n] = v;
+  }
+}
+
+// T chunk of an FP64 build (T[m][p][i], m < M, p < pc, i < nocc) into the
+// incremental state: TA[m][p0 + p][i] = T, TE = 0.
+__global__ void k_rik_seed(const double* __restrict__ T, int M, int pc, int nocc, int p0, int Maux,
+                           float* __restrict__ TA, float* __restrict__ TE) {
+  const size_t n = (size_t)M * pc * nocc;
+  for (size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x; id < n; id += (size_t)gridDim.x * blockDim.x) {
+    const size_t m = id / ((size_t)pc * nocc), rem = id % ((size_t)pc * nocc);
+    const size_t dst = (m * Maux + p0) * nocc + rem;
+    TA[dst] = (float)T[id];
+    TE[dst] = 0.0f;
+  }
+}
+
+// X[m][i] = sqrt2 C[m][i] for i < nocc (C row-major M x M)
+__global__ void k_rik_xocc(const double* __restrict__ C, int M, int nocc, double* __restrict__ X) {
+  const size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= (size_t)M * nocc) return;
+  X[id] = 1.4142135623730951 * C[(id / nocc) * M + id % nocc];
+}
+
+// out[0] = sum x^2, out[1] = sum y^2 (n entries each), out[2] = sum over the
+// no x no matrix G of (G - I)^2 (one workgroup)
+__global__ __launch_bounds__(1024) void k_rik_norms(const double* __restrict__ x, const double* __restrict__ y,
+                                                    size_t n, const double* __restrict__ G, int no, double* out) {
+  __shared__ double sx[1024], sy[1024], sg[1024];
+  double a = 0.0, b = 0.0, c = 0.0;
+  for (size_t i = threadIdx.x; i < n; i += blockDim.x) {
+    a += x[i] * x[i];
+    b += y[i] * y[i];
+  }
+  for (size_t i = threadIdx.x; i < (size_t)no * no; i += blockDim.x) {
+    const double d = G[i] - ((i / no) == (i % no) ? 1.0 : 0.0);
+    c += d * d;
+  }
+  sx[threadIdx.x] = a;
+  sy[threadIdx.x] = b;
+  sg[threadIdx.x] = c;
+  __syncthreads();
+  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) {
+      sx[threadIdx.x] += sx[threadIdx.x + o];
+      sy[threadIdx.x] += sy[threadIdx.x + o];
+      sg[threadIdx.x] += sg[threadIdx.x + o];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[0] = sx[0];
+    out[1] = sy[0];
+    out[2] = sg[0];
+  }
+}
+
+// *w = max |x| as float bits (w zeroed by the caller)
+__global__ void k_rik_absmax(const float* __restrict__ x, size_t n, unsigned* w) {
+  float m = 0.0f;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    m = fmaxf(m, fabsf(x[i]));
+  for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
+  if ((threadIdx.x & 63) == 0) atomic_max_pos(w, m);
+}
+
+__global__ void k_set_u32(unsigned* p, unsigned v) { *p = v; }
+
+__global__ void k_rik_reset(RikState* st) {
+  st->ta_max = 0u;
+  st->te_max = 0u;
+}
+
+template <int NBW>
+hipError_t launch_tstep_nb(const RikTArgs& a, hipStream_t s) {
+  const unsigned grid = (unsigned)((a.R + 63) / 64);
+  hipLaunchKernelGGL(k_rik_tstep<NBW>, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+__global__ void k_zero_d(double* p, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    p[i] = 0.0;
+}
+
+}  // namespace
+
+int rik_blocks(int nocc) { return (nocc + 31) / 32; }
+int rik_cols(int nocc) { return 64 * ((rik_blocks(nocc) + 1) / 2); }
+
+hipError_t launch_rik_split_e(const double* E, int M, int nocc, int kpad, RikState* st, uint16_t* Ehi, uint16_t* Elo,
+                              hipStream_t s) {
+  hipLaunchKernelGGL(k_rik_split_e, dim3(1), dim3(1024), 0, s, E, M, nocc, rik_cols(nocc), kpad, st, Ehi, Elo);
+  return hipGetLastError();
+}
+
+hipError_t launch_rik_tstep(const RikTArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL(k_rik_reset, dim3(1), dim3(1), 0, s, a.st);
+  // dW rows gather the two column halves by atomics
+  const unsigned zg = (unsigned)std::min<long>(8192, (a.R + 255) / 256);
+  hipLaunchKernelGGL(k_zero_d, dim3(zg), dim3(256), 0, s, a.dWrow, (size_t)a.R);
+  switch ((rik_blocks(a.nocc) + 1) / 2) {  // 32-column blocks per wave
+    case 1: return launch_tstep_nb<1>(a, s);
+    case 2: return launch_tstep_nb<2>(a, s);
+    case 3: return launch_tstep_nb<3>(a, s);
+    case 4: return launch_tstep_nb<4>(a, s);
+    case 5: return launch_tstep_nb<5>(a, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_rik_dk(const RikDkArgs& a, int nchunk, hipStream_t s) {
+  const int ntile = (a.M + kDkTile - 1) / kDkTile;
+  hipLaunchKernelGGL(k_rik_dk, dim3((unsigned)(ntile * ntile), (unsigned)nchunk), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_rik_reduce(const float* Zpart, int nchunk, int M, const double* Kprev, double* K, hipStream_t s) {
+  const int nt = (M + 31) / 32;
+  hipLaunchKernelGGL(k_rik_reduce, dim3((unsigned)(nt * nt)), dim3(256), 0, s, Zpart, nchunk, M, Kprev, K);
+  return hipGetLastError();
+}
+
+hipError_t launch_rik_dw(const double* dWrow, int M, int Maux, double* dW, double* W, double* Wprev, hipStream_t s) {
+  hipLaunchKernelGGL(k_rik_dw, dim3((unsigned)((Maux + 255) / 256)), dim3(256), 0, s, dWrow, M, Maux, dW, W, Wprev);
+  return hipGetLastError();
+}
+
+hipError_t launch_rik_j(const float* B32, const double* dW, int M, int Maux, double* Jprev, double* J,
+                        hipStream_t s) {
+  hipLaunchKernelGGL(k_rik_j, dim3(M), dim3(256), 0, s, B32, dW, M, Maux, Jprev, J);
+  return hipGetLastError();
+}
+
+hipError_t launch_rik_seed(const double* T, int M, int pc, int nocc, int p0, int Maux, float* TA, float* TE,
+                           hipStream_t s) {
+  const size_t n = (size_t)M * pc * nocc;
+  const unsigned grid = (unsigned)std::min<size_t>(65536, (n + 255) / 256);
+  hipLaunchKernelGGL(k_rik_seed, dim3(grid), dim3(256), 0, s, T, M, pc, nocc, p0, Maux, TA, TE);
+  return hipGetLastError();
+}
+
+hipError_t launch_rik_xocc(const double* C, int M, int nocc, double* X, hipStream_t s) {
+  const size_t n = (size_t)M * nocc;
+  hipLaunchKernelGGL(k_rik_xocc, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, C, M, nocc, X);
+  return hipGetLastError();
+}
+
+hipError_t launch_rik_norms(const double* x, const double* y, size_t n, const double* G, int no, double* out,
+                            hipStream_t s) {
+  hipLaunchKernelGGL(k_rik_norms, dim3(1), dim3(1024), 0, s, x, y, n, G, no, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_rik_absmax(const float* x, size_t n, unsigned* w, hipStream_t s) {
+  hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, s, w, 0u);
+  const unsigned grid = (unsigned)std::min<size_t>(8192, (n + 255) / 256);
+  hipLaunchKernelGGL(k_rik_absmax, dim3(grid), dim3(256), 0, s, x, n, w);
+  return hipGetLastError();
+}
+
+}  // namespace gansu

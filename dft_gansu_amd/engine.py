@@ -55,13 +55,18 @@ class FockEngine:
     """Fock-build engine over a molecule (and an auxiliary basis for RI)."""
 
     def __init__(self, mol, aux=None, eri_method="stored", schwarz_screening_threshold=1e-12, precision="fp64",
-                 fp32_bound_threshold=1e-6, ri_k="density", autotune=True, graph=True, nan_check=True):
+                 fp32_bound_threshold=1e-6, ri_k="density", autotune=True, graph=True, nan_check=True,
+                 ri_refresh=None, ri_tol=None):
         self.mol = mol
         self.aux = aux
         self.M = mol.n_basis
         opts = (f"eri_method={eri_method};schwarz_screening_threshold={schwarz_screening_threshold!r};"
                 f"precision={precision};fp32_bound_threshold={fp32_bound_threshold!r};ri_k={ri_k};"
                 f"autotune={int(bool(autotune))};graph={int(bool(graph))};nan_check={int(bool(nan_check))}")
+        if ri_refresh is not None:
+            opts += f";ri_refresh={int(ri_refresh)}"
+        if ri_tol is not None:
+            opts += f";ri_tol={float(ri_tol)!r}"
         self._h = ctypes.c_void_p()
         a = aux
         check(lib().gansu_fock_create(
@@ -158,7 +163,7 @@ class FockEngine:
         s = np.zeros(8)
         check(lib().gansu_fock_stats(self._h, ptr(s)))
         return dict(contracted_quartets=s[0], primitive_quartets=s[1], stored_bytes=s[2], fp32_quartets=s[3],
-                    unique_quartets=s[4], ri_bytes=s[5], graph_captures=s[6])
+                    unique_quartets=s[4], ri_bytes=s[5], graph_captures=s[6], ri_incremental_builds=int(s[7]))
 
     def set_profiling(self, enable=True):
         check(lib().gansu_fock_set_profiling(self._h, int(enable)))

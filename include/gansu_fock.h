@@ -119,7 +119,17 @@ void gansu_params_destroy(gansu_params* p);
  * nan_check=1|0 (eri_method=ri: an RHF build that forms F reads one flag
  * word back and fails with GANSU_E_NUMERIC when F holds a NaN, as
  * ERI_RI_RHF::compute_fock_matrix does, rhf.hpp:665-674; this is the one
- * host synchronisation inside a build; 0 skips it).
+ * host synchronisation inside a build; 0 skips it), and for
+ * eri_method=ri precision=mixed ri_refresh=N (default 8) and ri_tol=t
+ * (default 0.1): an RHF build given the occupied orbitals
+ * (gansu_fock_build_rhf_occ) after a previous one with the same nocc
+ * (<= 320) forms K and J as increments on the fp16 matrix cores from the
+ * aligned orbital change E (K_n = K_prev + T_A T_E^T + T_E T_A^T,
+ * T = B X), unless N increments have been chained since the last FP64
+ * build or |E| / |X| > t, in which case the build is the FP64 one; the
+ * incremental build reads three numbers back (alignment residual, |E|,
+ * |X|).  State for the increments (about 8 M Maux nocc bytes) is
+ * allocated by the first such build.
  * The engine runs on the device that was current at gansu_fock_create; each
  * call makes it current and restores the caller's device on return.
  * aux* may be NULL/0 unless eri_method=ri.                                  */
@@ -178,7 +188,7 @@ int gansu_fock_schwarz(gansu_fock_engine* e, double* host_Q, size_t n);
 size_t gansu_fock_num_primitive_pairs(const gansu_fock_engine* e);
 /* stats[8] = {surviving contracted quartets, surviving primitive quartets,
  *             stored bytes, fp32 quartets, unique contracted quartets,
- *             ri B bytes, graphs captured, 0}                               */
+ *             ri B bytes, graphs captured, incremental RI builds}           */
 int gansu_fock_stats(const gansu_fock_engine* e, double* stats);
 /* Per-launch timing of the builds that follow, with HIP events recorded on the
  * build stream (no counterpart in the reference, whose profiler.hpp times host
