@@ -1,16 +1,24 @@
 #!/usr/bin/env python3
 """Fock-build benchmark on one MI355X.
 
-Workload (default): all-trans C10H22 (seeded +-0.02 A jitter), cc-pVDZ, 260
-Cartesian basis functions, RHF, conventional (stored) ERIs, Schwarz 1e-12.
-A step is one Fock build F = H + J[D] - K[D]/2 with the integrals already
-resident in HBM (the reference's ERI::compute_fock_matrix on the stored path).
+Workload (headline): 64 waters (seeded orientations in a 12.4 A box),
+cc-pVDZ (M = 1600 Cartesian) + cc-pVDZ-RIFIT (Maux = 6144), nocc = 320, RHF
+with RI-JK in precision=mixed.  A step is one Fock build F = H + J - K/2 on
+a new set of occupied orbitals (an SCF-like sequence whose occupied space
+moves by 1e-3 x 0.7^k), with B already resident in HBM: after the first
+build each one forms K and J as increments on the fp16 matrix cores, and
+a full FP64 build follows after at most 16 increments or once the summed
+orbital change reaches the default budget, so the timed loop carries the
+refreshes at their natural rate.
 
-Prints one JSON line with the measured Fock builds/s, the roofline figures of
-the dominant kernel (timed live with HIP events on the build stream), the SCF
-wall time, the direct (integral-recomputing) build rate, and a CPU baseline:
-the FP64 host oracle's direct J/K build on a bounded sample of the same
-workload, extrapolated.
+Prints one JSON line: builds/s, the roofline of the dominant kernel (timed
+live with HIP events on the build stream), the FP64 builds of the same
+workload (occupied-form RI, the reference's density-form RI, the
+four-centre direct build), mixed vs FP64 RI SCF (energy difference and
+wall time, the RI gradient), the C10H22/cc-pVDZ stored four-centre build,
+the C24H12 and C10H22/6-31G* gradient legs, and a CPU baseline: the FP64
+host oracle's direct J/K build of the same 64 waters on a bounded sample,
+extrapolated.
 """
 import argparse
 import json
@@ -29,6 +37,109 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
 MEASURED_TRAFFIC = {
     ("C10H22/cc-pVDZ", "fock_slab(pp|ps) stored-digest"): 336.8e6,
 }
+FP16_MFMA_PEAK_TFS = 2500.0  # MI355X dense fp16 MFMA, TFLOP/s
+# kernel ids of the engine's profile rows (incremental RI build)
+RIK_NAMES = {80020: "align (Procrustes)", 80021: "split E", 80022: "k_rik_tstep (T_E = B E, fp16 MFMA)",
+             80023: "dW", 80024: "J increment (FP32 B)", 80025: "k_rik_dk (Z = T_A T_E^T, fp16 MFMA)",
+             80026: "K reduce (FP64)", 80005: "T=B*X dgemm", 80007: "W from T", 80008: "K=T*T^T dgemm",
+             80002: "J (FP64 B)"}
+
+
+def orbital_sequence(M, n, seed, amp=1e-3, decay=0.7):
+    """n orthonormal M x M orbital matrices (torch, on the GPU) whose span
+    moves by amp decay^k per step, with a random rotation of the first
+    columns each step (what successive SCF diagonalisations hand the build)."""
+    import torch
+    g = torch.Generator(device="cuda").manual_seed(seed)
+    C = torch.linalg.qr(torch.randn(M, M, generator=g, device="cuda", dtype=torch.float64))[0].contiguous()
+    out = [C]
+    for k in range(1, n):
+        C = C + amp * decay ** (k - 1) * torch.randn(M, M, generator=g, device="cuda", dtype=torch.float64) / M ** 0.5
+        C = torch.linalg.qr(C)[0].contiguous()
+        out.append(C)
+    return out
+
+
+def ri_headline(steps, warmup, log_fn):
+    """The headline: precision=mixed RI-JK builds of the 64-water workload on
+    a moving orbital sequence (incremental fp16-MFMA builds plus the FP64
+    refreshes the default policy makes)."""
+    import numpy as np
+    import torch
+    from dft_gansu_amd import FockEngine
+    from dft_gansu_amd import molecule as gm
+    mol, aux = gm.config_molecule(4)
+    M, nocc = mol.n_basis, mol.n_electrons // 2
+    nprof = 3
+    seq = orbital_sequence(M, warmup + steps + nprof + 1, seed=5)
+    dens = [(2.0 * C[:, :nocc] @ C[:, :nocc].T).contiguous() for C in seq]
+    H = torch.zeros((M, M), dtype=torch.float64, device="cuda")
+    F = torch.empty_like(H)
+    stream = torch.cuda.current_stream()
+    e = FockEngine(mol, aux=aux, eri_method="ri", precision="mixed")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e.precomputation()
+    torch.cuda.synchronize()
+    pre = time.perf_counter() - t0
+    log_fn(f"[bench] 64 waters mixed RI: precompute {pre:.2f} s")
+    k = 0
+    for _ in range(warmup):
+        e.compute_fock_matrix_occ(dens[k], seq[k], nocc, H, F, stream=stream)
+        k += 1
+    n_inc0 = e.stats()["ri_incremental_builds"]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        e.compute_fock_matrix_occ(dens[k], seq[k], nocc, H, F, stream=stream)
+        k += 1
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    n_inc = e.stats()["ri_incremental_builds"] - n_inc0
+    log_fn(f"[bench] {steps} mixed builds in {elapsed:.3f} s ({n_inc} incremental, {steps - n_inc} FP64)")
+    # per-kernel rows of incremental builds (HIP events on the build stream)
+    agg = {}
+    nrows = 0
+    for _ in range(nprof):
+        before = e.stats()["ri_incremental_builds"]
+        e.set_profiling(True)
+        e.compute_fock_matrix_occ(dens[k], seq[k], nocc, H, F, stream=stream)
+        k += 1
+        rows = e.last_profile()
+        e.set_profiling(False)
+        if e.stats()["ri_incremental_builds"] == before:
+            continue  # a refresh build: not an incremental profile
+        nrows += 1
+        for kid, ms, by, fl, un in rows:
+            a = agg.setdefault(int(kid), [0.0, by, fl, un])
+            a[0] += ms
+    for v in agg.values():
+        v[0] /= max(1, nrows)
+    kernels = sorted(((RIK_NAMES.get(kid, str(kid)), v[0], v[1], v[2]) for kid, v in agg.items()), key=lambda r: -r[1])
+    dom = kernels[0]
+    tfs = dom[3] / (dom[1] * 1e-3) / 1e12
+    roof = dict(bound="mfma", kernel=dom[0], achieved=round(tfs, 1), peak=FP16_MFMA_PEAK_TFS, unit="TFLOP/s",
+                frac=round(tfs / FP16_MFMA_PEAK_TFS, 4), traffic=MEASURED_TRAFFIC.get(("64 waters", dom[0])),
+                flops_per_launch=dom[3], avg_launch_ms=round(dom[1], 3),
+                flops_note="three fp16 MFMA products (hi/lo split) per multiply-add of the algorithm")
+    per_kernel = []
+    for name, ms, by, fl in kernels:
+        row = dict(kernel=name, ms=round(ms, 3))
+        if fl > 0:
+            row.update(tflops=round(fl / (ms * 1e-3) / 1e12, 1), frac_fp16_peak=round(fl / (ms * 1e-3) / 1e12 /
+                                                                                     FP16_MFMA_PEAK_TFS, 3))
+        if by > 0:
+            row.update(GBs=round(by / (ms * 1e-3) / 1e9, 1))
+        per_kernel.append(row)
+    log_fn(f"[bench] dominant kernel {dom[0]}: {dom[1]:.3f} ms, {tfs:.0f} TFLOP/s")
+    del e, seq, dens
+    torch.cuda.empty_cache()
+    cfgd = dict(workload="64 H2O cc-pVDZ + cc-pVDZ-RIFIT RHF RI-JK, precision=mixed (fp16-MFMA incremental K/J, "
+                         "FP64 refresh by the default policy)", n_basis_cartesian=M, n_aux_cartesian=aux.n_basis,
+                nocc=nocc, eri_method="ri", precision="mixed", precompute_s=round(pre, 3),
+                incremental_builds_timed=n_inc, fp64_builds_timed=steps - n_inc)
+    return dict(value=round(steps / elapsed, 3), ms_per_step=round(elapsed / steps * 1e3, 3), roofline=roof,
+                kernels=per_kernel, config=cfgd)
 
 
 def log(msg):
@@ -61,7 +172,7 @@ def cpu_baseline(cfg, target_s):
     rng = np.random.default_rng(5)
     C = rng.normal(size=(len(n), max(1, len(n) // 3))) / np.sqrt(len(n))
     D = 2.0 * C @ C.T
-    stride = 4096
+    stride = max(64, O.n_pairs(t) // 32)
     while True:
         t0 = time.perf_counter()
         O.jk(p, t, n, Q, D, 1e-12, sample_stride=stride)
@@ -75,10 +186,11 @@ def cpu_baseline(cfg, target_s):
                        f"({dt:.1f} s measured, x{stride} extrapolated; full build ~{full_s:.0f} s)")
 
 
-def ri_leg(log_fn):
-    """64-water cc-pVDZ + cc-pVDZ-RIFIT: the plain FP64 GPU path (four-centre
-    direct build, the drop-in) against the RI-JK forms, and the mixed RI SCF
-    against the FP64 RI SCF."""
+def ri_leg(log_fn, mixed_rate, no_scf=False):
+    """64-water cc-pVDZ + cc-pVDZ-RIFIT: the FP64 builds the mixed headline is
+    measured against (four-centre direct build, the drop-in; the reference's
+    density-form RI-K; the occupied-form FP64 RI), the RI gradient, and the
+    mixed RI SCF (with its analytic gradient) against the FP64 RI SCF."""
     import numpy as np
     import torch
     from dft_gansu_amd import FockEngine, Parameters, rhf, rhf_gradient
@@ -106,8 +218,7 @@ def ri_leg(log_fn):
     del e
     log_fn(f"[bench] 64-water FP64 direct build {out['fp64_direct_build_s']} s")
     for name, kw in (("ri_fp64_density", dict(precision="fp64", ri_k="density")),
-                     ("ri_fp64_occupied", dict(precision="fp64", ri_k="occupied")),
-                     ("ri_mixed", dict(precision="mixed"))):
+                     ("ri_fp64_occupied", dict(precision="fp64", ri_k="occupied"))):
         e = FockEngine(mol, aux=aux, eri_method="ri", **kw)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -142,7 +253,7 @@ def ri_leg(log_fn):
                                          frac=round(a / HBM_PEAK_GBS, 3))
             out[name + "_phases"] = phases
             log_fn(f"[bench] 64-water {name} phases: {phases}")
-        if name == "ri_mixed":  # RI-JK energy gradient for the same density and orbitals
+        if name == "ri_fp64_occupied":  # RI-JK energy gradient for the same density and orbitals
             g = [None]
 
             def grad():
@@ -154,12 +265,17 @@ def ri_leg(log_fn):
         del e
         torch.cuda.empty_cache()
         log_fn(f"[bench] 64-water {name} build {t:.4f} s (precompute {pre:.2f} s)")
-    out["speedup_mixed_ri_vs_fp64_direct"] = round(out["fp64_direct_build_s"] / out["ri_mixed_build_s"], 2)
-    out["speedup_mixed_ri_vs_fp64_ri_density"] = round(out["ri_fp64_density_build_s"] / out["ri_mixed_build_s"], 2)
+    mixed_s = 1.0 / mixed_rate
+    out["ri_mixed_build_s"] = round(mixed_s, 4)
+    out["speedup_mixed_ri_vs_fp64_direct"] = round(out["fp64_direct_build_s"] / mixed_s, 2)
+    out["speedup_mixed_ri_vs_fp64_ri_density"] = round(out["ri_fp64_density_build_s"] / mixed_s, 2)
+    out["speedup_mixed_ri_vs_fp64_ri_occupied"] = round(out["ri_fp64_occupied_build_s"] / mixed_s, 2)
+    if no_scf:
+        return out
     scf = {}
     # the FP64 energy, then the mixed energy + analytic gradient (one SCF each)
     for name, kw in (("fp64", dict(ri_k="occupied")), ("mixed", dict(precision="mixed"))):
-        prm = Parameters(eri_method="ri", convergence_energy_threshold=1e-8, maxiter=60, **kw)
+        prm = Parameters(eri_method="ri", convergence_energy_threshold=1e-9, maxiter=60, **kw)
         t0 = time.perf_counter()
         r = rhf(mol, prm, aux=aux) if name == "fp64" else rhf_gradient(mol, prm, aux=aux)
         scf[name] = dict(wall_s=round(time.perf_counter() - t0, 3), energy=r["e_total"], iterations=r["iterations"],
@@ -273,50 +389,26 @@ def gradient_leg(log_fn):
     return out
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", type=int, default=1)
-    ap.add_argument("--method", default="stored", choices=["stored", "direct"])
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-scf", action="store_true")
-    ap.add_argument("--no-direct", action="store_true")
-    ap.add_argument("--no-ri", action="store_true", help="skip the 64-water RI-JK leg")
-    ap.add_argument("--no-grad", action="store_true", help="skip the C10H22/6-31G* gradient leg")
-    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision store leg")
-    ap.add_argument("--no-coronene", action="store_true", help="skip the C24H12 four-centre + RI-JK leg")
-    args = ap.parse_args()
-    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU: the Fock build runs on a single MI355X in this build "
-              "(multi-GPU sharding is described, not run); use --gpus 1", file=sys.stderr)
-        sys.exit(2)
-
+def stored_leg(cfg, method, precision, steps, warmup, no_direct, no_scf, no_mixed, log_fn):
+    """Four-centre leg (C10H22/cc-pVDZ by default): the stored (or direct)
+    build with the integrals resident in HBM, the live roofline of its
+    dominant digestion kernel, the direct build rate, the SCF, and the mixed
+    store (FP32 for quartets bounded below 1e-6)."""
     import numpy as np
     import torch
     from dft_gansu_amd import FockEngine, Parameters, oneint, rhf
     from dft_gansu_amd import molecule as gm
-
-    assert torch.cuda.is_available(), "bench.py needs a GPU"
     dev = torch.device("cuda:0")
-    cfgd = gm.CONFIGS[args.config]
-    mol, aux = gm.config_molecule(args.config)
+    cfgd = gm.CONFIGS[cfg]
+    mol, aux = gm.config_molecule(cfg)
     M = mol.n_basis
-    log(f"[bench] {cfgd['name']}: M={M} prims={mol.n_prims} method={args.method} precision={args.precision}")
-
-    eng = FockEngine(mol, eri_method=args.method, precision=args.precision)
+    eng = FockEngine(mol, eri_method=method, precision=precision)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     eng.precomputation()
     torch.cuda.synchronize()
     precompute_s = time.perf_counter() - t0
     st = eng.stats()
-    log(f"[bench] precompute {precompute_s:.3f} s, quartets {st['contracted_quartets']:.3e}, "
-        f"stored {st['stored_bytes'] / 1e9:.2f} GB")
-
     S, T, V = oneint(mol)
     H = (T + V).contiguous()
     rng = np.random.default_rng(5)
@@ -324,22 +416,18 @@ def main():
     D = torch.tensor(2.0 * Cm @ Cm.T, dtype=torch.float64, device=dev)
     F = torch.empty_like(D)
     stream = torch.cuda.current_stream()
-
-    for _ in range(args.warmup):
+    for _ in range(warmup):
         eng.compute_fock_matrix(D, H, F, stream=stream)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(steps):
         eng.compute_fock_matrix(D, H, F, stream=stream)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
-    ms_per_step = elapsed / args.steps * 1e3
-    log(f"[bench] {args.steps} builds in {elapsed:.3f} s -> {ms_per_step:.3f} ms/build")
-
     # live per-kernel timing (HIP events on the build stream), averaged over builds
     eng.set_profiling(True)
     agg = {}
-    nprof = max(3, min(10, args.steps))
+    nprof = max(3, min(10, steps))
     for _ in range(nprof):
         eng.compute_fock_matrix(D, H, F, stream=stream)
         for kid, ms, by, fl, un in eng.last_profile():
@@ -353,24 +441,25 @@ def main():
     # algorithmic bytes of a build: every stored integral once (the ket-major J
     # pass re-reads them; that is extra traffic, not algorithmic work)
     all_bytes = sum(r[2] for r in kernels if (int(r[0]) % 1000000) // 100000 != 3)
-    if args.method == "stored":
-        roof = dict(bound="hbm", kernel=kernel_name(dom[0]), achieved=round(dom[2] / (dom[1] * 1e-3) / 1e9, 1),
-                    peak=HBM_PEAK_GBS, unit="GB/s", frac=round(dom[2] / (dom[1] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-                    traffic=MEASURED_TRAFFIC.get((cfgd["name"], kernel_name(dom[0]))), bytes_per_launch=dom[2],
-                    avg_launch_ms=round(dom[1], 4),
+    if method == "stored":
+        gbs = dom[2] / (dom[1] * 1e-3) / 1e9
+        roof = dict(bound="hbm", kernel=kernel_name(dom[0]), achieved=round(gbs, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                    frac=round(gbs / HBM_PEAK_GBS, 4), traffic=MEASURED_TRAFFIC.get((cfgd["name"], kernel_name(dom[0]))),
+                    bytes_per_launch=dom[2], avg_launch_ms=round(dom[1], 4),
                     all_digest_kernels=dict(ms=round(build_kernel_ms, 4), bytes=all_bytes,
                                             GBs=round(all_bytes / (build_kernel_ms * 1e-3) / 1e9, 1)))
     else:
-        roof = dict(bound="fp64-valu", kernel=kernel_name(dom[0]), achieved=None, peak=78.6, unit="TFLOP/s",
-                    frac=None, traffic=None, avg_launch_ms=round(dom[1], 4))
-    per_kernel = [dict(kernel=kernel_name(k), ms=round(ms, 4), quartets=un) for k, ms, by, fl, un in kernels[:8]]
-    log(f"[bench] dominant kernel {roof['kernel']}: {dom[1]:.4f} ms")
-
-    extra = {}
+        roof = dict(bound="fp64-valu", kernel=kernel_name(dom[0]), avg_launch_ms=round(dom[1], 4))
+    out = dict(workload=f"{cfgd['name']} RHF {method} ERI, Schwarz 1e-12", n_basis_cartesian=M,
+               fock_builds_per_s=round(steps / elapsed, 3), ms_per_build=round(elapsed / steps * 1e3, 4),
+               precompute_s=round(precompute_s, 4), contracted_quartets=st["contracted_quartets"],
+               stored_bytes=st["stored_bytes"], roofline=roof,
+               kernels=[dict(kernel=kernel_name(k), ms=round(ms, 4), quartets=un) for k, ms, by, fl, un in kernels[:8]])
+    log_fn(f"[bench] {cfgd['name']} {method}: {out['ms_per_build']} ms/build, dominant {roof['kernel']}")
     del eng
     torch.cuda.empty_cache()
-    if not args.no_direct and args.method != "direct":
-        e2 = FockEngine(mol, eri_method="direct", precision=args.precision)
+    if not no_direct and method != "direct":
+        e2 = FockEngine(mol, eri_method="direct", precision=precision)
         e2.precomputation()
         e2.compute_fock_matrix(D, H, F, stream=stream)
         torch.cuda.synchronize()
@@ -379,82 +468,108 @@ def main():
         for _ in range(nd):
             e2.compute_fock_matrix(D, H, F, stream=stream)
         torch.cuda.synchronize()
-        extra["direct_fock_builds_per_s"] = round(nd / (time.perf_counter() - t0), 3)
-        log(f"[bench] direct build rate {extra['direct_fock_builds_per_s']} /s")
+        out["direct_fock_builds_per_s"] = round(nd / (time.perf_counter() - t0), 3)
+        log_fn(f"[bench] direct build rate {out['direct_fock_builds_per_s']} /s")
         del e2
-    if not args.no_scf:
-        # one small SCF first (water STO-3G), so that the timed SCFs below all
-        # run with rocBLAS / rocSOLVER already initialised in this process
-        rhf(gm.config_molecule(0)[0], Parameters(convergence_energy_threshold=1e-8))
+    if not no_scf:
         t0 = time.perf_counter()
-        r = rhf(mol, Parameters(eri_method=args.method, precision=args.precision,
-                                convergence_energy_threshold=1e-8, maxiter=100), aux=aux)
-        extra.update(scf_wall_s=round(time.perf_counter() - t0, 3), scf_energy=r["e_total"],
-                     scf_iterations=r["iterations"], scf_converged=r["converged"])
-        log(f"[bench] SCF {extra['scf_wall_s']} s, E = {r['e_total']:.10f} ({r['iterations']} it)")
-
-    if not args.no_mixed and args.precision == "fp64" and args.method == "stored":
+        r = rhf(mol, Parameters(eri_method=method, precision=precision, convergence_energy_threshold=1e-8,
+                                maxiter=100), aux=aux)
+        out.update(scf_wall_s=round(time.perf_counter() - t0, 3), scf_energy=r["e_total"],
+                   scf_iterations=r["iterations"], scf_converged=r["converged"])
+        log_fn(f"[bench] SCF {out['scf_wall_s']} s, E = {r['e_total']:.10f} ({r['iterations']} it)")
+    if not no_mixed and precision == "fp64" and method == "stored":
         em = FockEngine(mol, eri_method="stored", precision="mixed")
         em.precomputation()
-        for _ in range(args.warmup):
+        for _ in range(warmup):
             em.compute_fock_matrix(D, H, F, stream=stream)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        for _ in range(args.steps):
+        for _ in range(steps):
             em.compute_fock_matrix(D, H, F, stream=stream)
         torch.cuda.synchronize()
-        tm = (time.perf_counter() - t0) / args.steps
+        tm = (time.perf_counter() - t0) / steps
         sm = em.stats()
         del em
         torch.cuda.empty_cache()
         mixed = dict(fock_builds_per_s=round(1.0 / tm, 3), ms_per_build=round(tm * 1e3, 4),
-                     stored_bytes=sm["stored_bytes"], fp32_quartets=sm["fp32_quartets"],
-                     fp32_bound_threshold=1e-6)
-        if not args.no_scf:
+                     stored_bytes=sm["stored_bytes"], fp32_quartets=sm["fp32_quartets"], fp32_bound_threshold=1e-6)
+        if not no_scf:
             t0 = time.perf_counter()
             r = rhf(mol, Parameters(eri_method="stored", precision="mixed", convergence_energy_threshold=1e-8,
                                     maxiter=100))
             mixed.update(scf_wall_s=round(time.perf_counter() - t0, 3), scf_energy=r["e_total"],
                          scf_iterations=r["iterations"])
-            if "scf_energy" in extra:
-                mixed["dE_vs_fp64"] = r["e_total"] - extra["scf_energy"]
-        extra["mixed"] = mixed
-        log(f"[bench] mixed store: {mixed}")
+            if "scf_energy" in out:
+                mixed["dE_vs_fp64"] = r["e_total"] - out["scf_energy"]
+        out["mixed"] = mixed
+        log_fn(f"[bench] mixed store: {mixed}")
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=18)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-scf", action="store_true", help="skip the SCF runs of every leg")
+    ap.add_argument("--no-ri", action="store_true", help="skip the 64-water FP64 comparison leg")
+    ap.add_argument("--no-stored", action="store_true", help="skip the C10H22/cc-pVDZ four-centre leg")
+    ap.add_argument("--no-grad", action="store_true", help="skip the C10H22/6-31G* gradient leg")
+    ap.add_argument("--no-coronene", action="store_true", help="skip the C24H12 four-centre + RI-JK leg")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: the Fock build runs on a single MI355X in this build "
+              "(multi-GPU sharding is described, not run); use --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from dft_gansu_amd import Parameters, rhf
+    from dft_gansu_amd import molecule as gm
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    # one small SCF first (water STO-3G), so that everything below runs with
+    # rocBLAS / rocSOLVER already initialised in this process
+    rhf(gm.config_molecule(0)[0], Parameters(convergence_energy_threshold=1e-8))
+    head = ri_headline(args.steps, args.warmup, log)
+    extra = {}
+    if not args.no_ri:
+        extra["ri_64water"] = ri_leg(log, head["value"], no_scf=args.no_scf)
+        torch.cuda.empty_cache()
+    if not args.no_stored:
+        extra["c10h22_stored"] = stored_leg(1, "stored", "fp64", 20, 3, False, args.no_scf, False, log)
+        torch.cuda.empty_cache()
     if not args.no_coronene:
         extra["c24h12"] = coronene_leg(log)
         torch.cuda.empty_cache()
     if not args.no_grad:
         extra["gradient"] = gradient_leg(log)
         torch.cuda.empty_cache()
-    if not args.no_ri:
-        extra["ri_64water"] = ri_leg(log)
-        torch.cuda.empty_cache()
 
     cpu = None
     if not args.no_cpu:
         log("[bench] CPU baseline (oracle) ...")
-        cpu = cpu_baseline(args.config, args.cpu_seconds)
+        cpu = cpu_baseline(4, args.cpu_seconds)
         log(f"[bench] CPU baseline {cpu['value']:.3e} builds/s on {cpu['cores']} threads")
 
     out = {
         "metric": "Fock builds/s + SCF wall s at |dE|<=1uEh, mixed vs FP64 on GPU; gradient s/call",
-        "value": round(args.steps / elapsed, 3),
+        "value": head["value"],
         "unit": "Fock builds/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 4),
+        "ms_per_step": head["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if args.precision == "fp64" else "f64+f32",
-        "data": "synthetic (seeded geometry generator; random PSD density of rank nocc)",
-        "config": {"workload": f"{cfgd['name']} RHF {args.method} ERI, Schwarz 1e-12", "n_basis_cartesian": M,
-                   "n_primitives": mol.n_prims, "eri_method": args.method, "precision": args.precision,
-                   "precompute_s": round(precompute_s, 4), "contracted_quartets": st["contracted_quartets"],
-                   "stored_bytes": st["stored_bytes"]},
-        "roofline": roof,
-        "kernels": per_kernel,
+        "dtype": "f64+f16",
+        "data": "synthetic (seeded 64-water geometry; SCF-like sequence of random orthonormal orbitals)",
+        "config": head["config"],
+        "roofline": head["roofline"],
+        "kernels": head["kernels"],
         "cpu_baseline": cpu,
     }
     out.update(extra)

@@ -11,6 +11,7 @@
 #include <climits>
 #include <cmath>
 #include <cstring>
+#include <cstdlib>
 
 namespace gansu {
 
@@ -58,6 +59,10 @@ FockEngine::FockEngine(const gansu_primitive_shell* prims, size_t n_prims, const
   // the engine's device: the caller's current one (no device yet: 0, and the
   // first call that needs one reports the HIP error)
   if (hipGetDevice(&dev_) != hipSuccess) dev_ = 0;
+  {
+    const char* tr = std::getenv("GANSU_RI_TRACE");
+    rik_trace_ = tr && tr[0] == '1';
+  }
   method_ = "stored";
   for (const auto& kv : opts) {
     if (kv.first == "eri_method") method_ = kv.second;

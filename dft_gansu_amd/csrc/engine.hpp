@@ -169,10 +169,13 @@ class FockEngine {
   bool mixed_ = false;
   double fp32_thr_ = 1e-6;
   bool ri_k_occ_ = false;  // RI-K from occupied orbitals when they are given
-  // precision=mixed RI: a full FP64 build after rik_refresh_ increments or
-  // when the aligned orbital change |E| / |X| exceeds rik_tol_
-  int rik_refresh_ = 8;
-  double rik_tol_ = 0.1;
+  // precision=mixed RI: a full FP64 build after rik_refresh_ increments,
+  // for a change |E| / |X| above rik_accept_, and for the first small change
+  // after increments whose |E| / |X| summed to more than rik_tol_
+  int rik_refresh_ = 16;
+  double rik_tol_ = 0.01;
+  double rik_accept_ = 0.5;
+  bool rik_trace_ = false;  // GANSU_RI_TRACE=1: one stderr line per mixed RI build
   int rik_ns_iters_ = 40;  // Newton-Schulz steps of the orbital alignment
   double last_rik_rel_ = 0.0;
   static constexpr int kRikMaxBlocks = 10;  // nocc <= 320 for the incremental path

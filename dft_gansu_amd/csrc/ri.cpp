@@ -14,6 +14,8 @@
 #include <rocsolver/rocsolver.h>
 
 #include <algorithm>
+#include <cstdio>
+#include <cstdlib>
 #include <cmath>
 #include <cstring>
 
@@ -64,7 +66,8 @@ struct FockEngine::RIState {
   float sB = 1.0f;
   int rik_nocc = 0;   // nocc of the held state (0: none)
   int rik_since = 0;  // incremental builds since the last FP64 build
-  int rik_incremental = 0, rik_full = 0;  // counts, for the caller's statistics
+  double rik_drift = 0.0;  // sum of |E| / |X| over them
+  int rik_incremental = 0, rik_fresh = 0, rik_full = 0;  // counts, for the caller's statistics
   DevBuf<float> TA, TE, Zpart;
   DevBuf<double> Xprev, Xn, Ework, Y, Y2, G, Kprev, Jprev, Wprev, dW, dWrow, nrm;
   DevBuf<uint16_t> Ehi, Elo;
@@ -251,6 +254,7 @@ void FockEngine::build_ri(const double* d_D, double* d_J, double* d_K, const dou
       GANSU_HIP(hipMemcpyAsync(r.Wprev.ptr, r.W.ptr, sizeof(double) * Ma, hipMemcpyDeviceToDevice, s));
       GANSU_HIP(launch_rik_xocc(d_C, M, nocc, r.Xprev.ptr, s));
       r.rik_since = 0;
+      r.rik_drift = 0.0;
       ++r.rik_full;
     }
   }
@@ -316,22 +320,27 @@ void FockEngine::rik_prepare(int nocc) {
 }
 
 // split-k depth of the dK product: enough workgroups to fill the chip
-// several times over, and at most 32768 k per chunk, so that each FP32
-// partial sums no more than 2048 MFMA steps before the FP64 reduction
+// several times over, and at most 4096 k per chunk, so that each FP32
+// partial sums no more than 256 MFMA steps before the FP64 reduction
+// (a two-level FP32 sum inside the kernel was as accurate and, at one wave
+// per SIMD, half as fast)
 int FockEngine::rik_chunks(int M, long Kd) {
   const long nt = (M + 127) / 128;
   const long fill = (4096 + nt * nt - 1) / (nt * nt);
-  const long acc = (Kd + 32767) / 32768;
-  return (int)std::max<long>(1, std::min<long>({std::max(fill, acc), 256, std::max<long>(1, Kd / 32)}));
+  const long acc = (Kd + 4095) / 4096;
+  return (int)std::max<long>(1, std::min<long>({std::max(fill, acc), 1024, std::max<long>(1, Kd / 32)}));
 }
 
-// One incremental build (ri_mfma.hip): align the new occupied factor to the
+// One incremental or fresh build (ri_mfma.hip): align the new occupied factor to the
 // previous one (orthogonal Procrustes by Newton-Schulz), E = X_n R - X_prev,
 // then T_E = B E with the T_A / T_E update and dW, J = J_prev + B32 dW,
 // K = K_prev + Z + Z^T with Z = T_A T_E^T.  Returns false (nothing changed)
 // when a full FP64 build is due: no state for this nocc, `rik_refresh`
-// increments since the last one, a relative change |E| / |X| above
-// `rik_tol`, or an alignment that did not converge.
+// increments since the last one, an alignment that did not converge or a
+// change |E| / |X| above `rik_accept`, or a small change after increments
+// whose summed |E| / |X| exceeds `rik_tol`.  The rounding error of an
+// increment is proportional to its |E|, so the sum bounds the error K
+// carries between FP64 builds.
 bool FockEngine::build_ri_incremental(const double* d_C, int nocc, double* J, double* K, hipStream_t s) {
   RIState& r = *ri_;
   const int M = M_, Ma = Maux_;
@@ -382,7 +391,29 @@ bool FockEngine::build_ri_incremental(const double* d_C, int nocc, double* J, do
   prof_end(s);
   const double rel = std::sqrt(r.nrm_host[0] / std::max(r.nrm_host[1], 1e-300));
   last_rik_rel_ = rel;
-  if (!(std::sqrt(r.nrm_host[2]) < 1e-10) || !(rel <= rik_tol_)) return false;
+  // an increment is taken when the alignment converged and the change is at
+  // most rik_accept_; otherwise the build starts afresh on the same kernels
+  // (X_prev = 0, E = X_n: K = T T^T at the accuracy of the split products),
+  // counted as a change of 1.  Once the changes summed since the last FP64
+  // build exceed rik_tol_, the first build whose own change is below
+  // rik_tol_ / 4 is an FP64 one, so that the error of the early, large
+  // changes does not stay in K while the SCF converges.
+  const bool aligned = std::sqrt(r.nrm_host[2]) < 1e-10 && rel <= rik_accept_;
+  const double step = aligned ? rel : 1.0;
+  const bool wipe = r.rik_drift + step > rik_tol_ && step < 0.25 * rik_tol_;
+  if (rik_trace_)
+    std::fprintf(stderr, "[ri] |E|/|X| = %.3e, drift %.3e, orthogonality %.1e -> %s\n", rel, r.rik_drift + step,
+                 std::sqrt(r.nrm_host[2]), wipe ? "FP64" : (aligned ? "incremental" : "fresh (fp16 split)"));
+  if (wipe) return false;
+  r.rik_drift += step;
+  const bool fresh = !aligned;
+  if (fresh) {
+    // E = X_n, and the previous J, W, K are zero
+    GANSU_HIP(hipMemcpyAsync(r.Ework.ptr, r.Xn.ptr, sizeof(double) * mn, hipMemcpyDeviceToDevice, s));
+    GANSU_HIP(launch_zero(r.Xprev.ptr, mn, s));
+    GANSU_HIP(launch_zero(r.Jprev.ptr, (size_t)M * M, s));
+    GANSU_HIP(launch_zero(r.Wprev.ptr, (size_t)Ma, s));
+  }
 
   const int kpad = (M + 31) / 32 * 32;
   prof_begin(s, 80021, 0.0, 0.0, 1.0);
@@ -391,7 +422,7 @@ bool FockEngine::build_ri_incremental(const double* d_C, int nocc, double* J, do
   // T_E = B E (+ T_A update, dW rows): 2 R M nocc flops, three fp16 products each
   const long R = (long)M * Ma;
   RikTArgs ta{r.B32.ptr, R, M, Ma, nocc, kpad, r.sB, r.Ehi.ptr, r.Elo.ptr, r.Ework.ptr, r.rst.ptr,
-              r.TA.ptr, r.TE.ptr, r.dWrow.ptr};
+              r.TA.ptr, r.TE.ptr, r.dWrow.ptr, fresh ? 1 : 0};
   prof_begin(s, 80022, double(R) * M * 4 + double(R) * nocc * 16, 2.0 * R * M * nocc * 3.0, (double)R);
   GANSU_HIP(launch_rik_tstep(ta, s));
   prof_end(s);
@@ -410,13 +441,13 @@ bool FockEngine::build_ri_incremental(const double* d_C, int nocc, double* J, do
   GANSU_HIP(launch_rik_dk(da, nch, s));
   prof_end(s);
   prof_begin(s, 80026, 0.0, 0.0, 1.0);
-  GANSU_HIP(launch_rik_reduce(r.Zpart.ptr, nch, M, r.Kprev.ptr, K, s));
+  GANSU_HIP(launch_rik_reduce(r.Zpart.ptr, nch, M, fresh ? nullptr : r.Kprev.ptr, K, s));
   GANSU_HIP(hipMemcpyAsync(r.Kprev.ptr, K, sizeof(double) * M * M, hipMemcpyDeviceToDevice, s));
   // X_prev <- X_prev + E = X_n R, the factor that B X_prev = T_A + T_E / 2 now holds
   blas_ok(rocblas_daxpy(r.h, (int)mn, &one, r.Ework.ptr, 1, r.Xprev.ptr, 1), "rocblas_daxpy(X)");
   prof_end(s);
   ++r.rik_since;
-  ++r.rik_incremental;
+  ++(fresh ? r.rik_fresh : r.rik_incremental);
   return true;
 }
 

@@ -113,6 +113,7 @@ struct RikTArgs {
   float* TA;          // R x nocc
   float* TE;
   double* dWrow;      // R
+  int fresh;          // 1: the previous T_A, T_E count as zero (not read)
 };
 struct RikDkArgs {
   const float* TA;    // M x Kd (Kd = Maux nocc)

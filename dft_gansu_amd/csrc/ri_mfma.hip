@@ -19,7 +19,9 @@
 // two so that their largest element sits at 2^14.  The rounding error of the
 // increment scales with |E|, so the FP64 Fock matrix accumulates K_prev plus
 // small, accurately formed increments; the caller refreshes K in FP64 from
-// time to time.
+// time to time.  With the previous state taken as zero (X_prev = 0, a
+// "fresh" build) the same kernels give the whole of K = T T^T, T = B X,
+// at the accuracy of the split products.
 //
 // MFMA v_mfma_f32_32x32x16_f16: lane l holds row (or column) l % 32 of an
 // operand and 8 consecutive k of k-group l / 32; the 16 results of lane l
@@ -222,7 +224,7 @@ __global__ __launch_bounds__(256) void k_rik_tstep(RikTArgs a) {
       if (row >= a.R) continue;
       const size_t idx = (size_t)row * a.nocc + c;
       const float te = acc[b][r] * inv;
-      const float tp = a.TA[idx] + 0.5f * a.TE[idx];
+      const float tp = a.fresh ? 0.0f : a.TA[idx] + 0.5f * a.TE[idx];
       const float ta = tp + 0.5f * te;
       a.TA[idx] = ta;
       a.TE[idx] = te;
@@ -395,7 +397,7 @@ __global__ __launch_bounds__(256) void k_rik_dk(RikDkArgs a) {
     }
 }
 
-// K = Kprev + S + S^T with S = sum over chunks of Zpart, in FP64; 32 x 32
+// K = Kprev + S + S^T (Kprev null: 0) with S = sum over chunks of Zpart, in FP64; 32 x 32
 // tiles through LDS so that both S[m][n] and S[n][m] are read coalesced.
 __global__ __launch_bounds__(256) void k_rik_reduce(const float* __restrict__ Zpart, int nchunk, int M,
                                                     const double* __restrict__ Kprev, double* __restrict__ K) {
@@ -418,7 +420,7 @@ __global__ __launch_bounds__(256) void k_rik_reduce(const float* __restrict__ Zp
     if (r >= M || c >= M) continue;
     double s = 0.0;
     for (int ch = 0; ch < nchunk; ++ch) s += (double)Zpart[ch * mm + (size_t)r * M + c];
-    K[(size_t)r * M + c] = Kprev[(size_t)r * M + c] + s + st[y][tx];
+    K[(size_t)r * M + c] = (Kprev ? Kprev[(size_t)r * M + c] : 0.0) + s + st[y][tx];
   }
 }
 

@@ -120,13 +120,16 @@ void gansu_params_destroy(gansu_params* p);
  * word back and fails with GANSU_E_NUMERIC when F holds a NaN, as
  * ERI_RI_RHF::compute_fock_matrix does, rhf.hpp:665-674; this is the one
  * host synchronisation inside a build; 0 skips it), and for
- * eri_method=ri precision=mixed ri_refresh=N (default 8) and ri_tol=t
- * (default 0.1): an RHF build given the occupied orbitals
+ * eri_method=ri precision=mixed ri_refresh=N (default 16) and ri_tol=t
+ * (default 0.01): an RHF build given the occupied orbitals
  * (gansu_fock_build_rhf_occ) after a previous one with the same nocc
  * (<= 320) forms K and J as increments on the fp16 matrix cores from the
  * aligned orbital change E (K_n = K_prev + T_A T_E^T + T_E T_A^T,
- * T = B X), unless N increments have been chained since the last FP64
- * build or |E| / |X| > t, in which case the build is the FP64 one; the
+ * T = B X); the build is the FP64 one instead after N chained increments,
+ * when the alignment fails or |E| / |X| > 0.5, and, once the changes
+ * summed since the last FP64 build exceed t, for the first build whose
+ * change is below t / 4 (so that the error of early, large increments
+ * leaves K before the SCF converges); the
  * incremental build reads three numbers back (alignment residual, |E|,
  * |X|).  State for the increments (about 8 M Maux nocc bytes) is
  * allocated by the first such build.

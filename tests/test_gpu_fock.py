@@ -681,7 +681,8 @@ def test_ri_incremental_exchange_tracks_fp64(system, cuda):
     aligned orbital change; over a sequence of moving orbitals their Fock
     matrices stay within 1e-8 (relative to max|F|) of the FP64 occupied-form
     builds, and every build after the first is incremental (no refresh due
-    within ri_refresh).  C24H12 (nocc 78) runs three 32-column blocks, the
+    within ri_refresh, and ri_tol = 1 so that the summed change never asks
+    for one).  C24H12 (nocc 78) runs three 32-column blocks, the
     methane-water pair (nocc 10) one."""
     if system == "ch4h2o":
         Z, xyz = methane_water()
@@ -693,7 +694,7 @@ def test_ri_incremental_exchange_tracks_fp64(system, cuda):
     H = _torch(random_density(M, seed=2), cuda)
     e64 = FockEngine(mol, aux=aux, eri_method="ri", ri_k="occupied")
     e64.precomputation()
-    emx = FockEngine(mol, aux=aux, eri_method="ri", precision="mixed", ri_refresh=20)
+    emx = FockEngine(mol, aux=aux, eri_method="ri", precision="mixed", ri_refresh=20, ri_tol=1.0)
     emx.precomputation()
     worst = 0.0
     seq = _orbital_sequence(M, nocc, 7, seed=4)
@@ -708,3 +709,42 @@ def test_ri_incremental_exchange_tracks_fp64(system, cuda):
         assert err < 1e-8, (k, err)
     assert emx.stats()["ri_incremental_builds"] == len(seq) - 1
     print("%s worst %.2e" % (system, worst))
+
+
+def test_ri_mixed_fresh_build_and_refresh_policy(cuda):
+    """A mixed RI build whose orbitals are unrelated to the previous ones
+    (the alignment cannot follow them) starts afresh on the fp16 matrix cores
+    (K = T T^T from T = B X at the accuracy of the split products: within
+    1e-7 of max|F| of the FP64 build); small changes after it are increments,
+    and once the summed change is large and the next change small, the build
+    is an FP64 one (exact to rounding)."""
+    Z, xyz = methane_water()
+    mol, _ = both(Z, xyz, "cc-pvdz")
+    aux, _ = both(Z, xyz, "cc-pvdz-rifit")
+    M, nocc = mol.n_basis, mol.n_electrons // 2
+    H = _torch(random_density(M, seed=2), cuda)
+    e64 = FockEngine(mol, aux=aux, eri_method="ri", ri_k="occupied")
+    e64.precomputation()
+    emx = FockEngine(mol, aux=aux, eri_method="ri", precision="mixed")
+    emx.precomputation()
+    rng = np.random.default_rng(9)
+    seq = [np.linalg.qr(rng.normal(size=(M, M)))[0]]  # FP64 (no state yet)
+    seq.append(np.linalg.qr(rng.normal(size=(M, M)))[0])  # unrelated: fresh
+    seq += _orbital_sequence(M, nocc, 4, seed=5, amp=1e-5)[1:]  # small moves: increments
+    seq[2:] = [np.linalg.qr(seq[1] + 1e-5 * (k + 1) * rng.normal(size=(M, M)) / np.sqrt(M))[0] for k in range(3)]
+    kinds = []
+    for k, C in enumerate(seq):
+        Ct = _torch(C, cuda)
+        D = _torch(2.0 * C[:, :nocc] @ C[:, :nocc].T, cuda)
+        before = emx.stats()["ri_incremental_builds"]
+        F64 = e64.compute_fock_matrix_occ(D, Ct, nocc, H).cpu().numpy()
+        Fmx = emx.compute_fock_matrix_occ(D, Ct, nocc, H).cpu().numpy()
+        err = np.abs(Fmx - F64).max() / np.abs(F64).max()
+        kinds.append(emx.stats()["ri_incremental_builds"] - before)
+        print("build %d: incremental %d, max|F_mixed - F_fp64| / max|F| = %.2e" % (k, kinds[-1], err))
+        assert err < (1e-7 if k == 1 else 1e-12 if k == 0 else 1e-7), (k, err)
+    # build 0: FP64; build 1: fresh (not an increment); then increments until the
+    # summed change (1 from the fresh build) asks for the FP64 wipe on a small change
+    assert kinds[0] == 0 and kinds[1] == 0
+    assert kinds[2] == 0  # small change after a fresh build: the FP64 wipe
+    assert kinds[3] == 1  # then increments
