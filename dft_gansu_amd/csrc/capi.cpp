@@ -260,6 +260,13 @@ int gansu_oneint(const gansu_molecule* mol, double* d_S, double* d_T, double* d_
   });
 }
 
+int gansu_oneint_noalloc(const gansu_molecule* mol, double* d_S, double* d_T, double* d_V, void* stream) {
+  return guarded([&] {
+    if (!mol || !d_S || !d_T || !d_V) throw gansu::Error(GANSU_E_ARG, "null argument");
+    gansu::compute_oneint_noalloc(mol->m, d_S, d_T, d_V, (hipStream_t)stream);
+  });
+}
+
 int gansu_rhf_run(const gansu_molecule* mol, const gansu_molecule* aux_mol, const gansu_params* p, double* energies,
                   double* D_out, int* info) {
   return guarded([&] {

@@ -4,10 +4,22 @@
 // here one lane evaluates one (mu, nu) basis-function pair.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "gansu_internal.hpp"
 #include "md_device.hpp"
 #include "scf_kernels.hpp"
 
 namespace gansu {
+
+// Device array of the no-allocation entry point (gansu_oneint_noalloc).  In
+// host code its name is only a stand-in: the device address comes from
+// hipGetSymbolAddress, once per device.
+static __device__ __align__(256) unsigned char scratch[kOneIntScratchBytes];
+
+void hip_check(hipError_t e, const char* what);  // engine.cpp
 
 using namespace dev;
 
@@ -186,6 +198,71 @@ hipError_t launch_dot_partial(const double* x, const double* y, size_t n, double
 hipError_t launch_set_int(int* p, int v, hipStream_t s) {
   hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, s, p, v);
   return hipGetLastError();
+}
+
+namespace {
+// layout of the tables in the array: each one 256-byte aligned, in this order
+template <class F>
+size_t oneint_layout(const OneIntTables& t, F&& place) {
+  size_t at = 0;
+  auto put = [&](const void* src, size_t bytes) {
+    place(at, src, bytes);
+    at += (bytes + 255) / 256 * 256;
+  };
+  put(t.prims.data(), t.prims.size() * sizeof(gansu_primitive_shell));
+  put(t.off.data(), t.off.size() * sizeof(int));
+  put(t.sprims.data(), t.sprims.size() * sizeof(int));
+  put(t.sl.data(), t.sl.size() * sizeof(int));
+  put(t.sb.data(), t.sb.size() * sizeof(int));
+  put(t.bfs.data(), t.bfs.size() * sizeof(int));
+  put(t.Z.data(), t.Z.size() * sizeof(int));
+  put(t.w.data(), t.w.size() * sizeof(double));
+  put(t.xyz.data(), t.xyz.size() * sizeof(double));
+  put(t.boys.data(), t.boys.size() * sizeof(double));
+  return at;
+}
+
+// the array's device address on the current device
+unsigned char* scratch_ptr() {
+  static std::mutex mu;
+  static std::vector<unsigned char*> ptrs;
+  int dev = 0;
+  hip_check(hipGetDevice(&dev), "hipGetDevice");
+  std::lock_guard<std::mutex> lock(mu);
+  if ((int)ptrs.size() <= dev) ptrs.resize(dev + 1, nullptr);
+  if (!ptrs[dev]) {
+    void* p = nullptr;
+    hip_check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+    if (!p) throw Error(GANSU_E_HIP, "hipGetSymbolAddress(scratch) returned no address");
+    ptrs[dev] = static_cast<unsigned char*>(p);
+  }
+  return ptrs[dev];
+}
+}  // namespace
+
+size_t oneint_scratch_bytes(const OneIntTables& t) {
+  return oneint_layout(t, [](size_t, const void*, size_t) {});
+}
+
+void oneint_noalloc(const OneIntTables& t, double* S, double* T, double* V, hipStream_t s) {
+  const size_t need = oneint_scratch_bytes(t);
+  if (need > kOneIntScratchBytes)
+    throw Error(GANSU_E_ARG, "gansu_oneint_noalloc: the molecule's tables take " + std::to_string(need) +
+                                 " bytes, beyond the " + std::to_string(kOneIntScratchBytes) +
+                                 "-byte device array (use gansu_oneint)");
+  unsigned char* base = scratch_ptr();
+  const void* at[10];
+  int k = 0;
+  oneint_layout(t, [&](size_t off, const void* src, size_t bytes) {
+    at[k++] = base + off;
+    if (bytes) hip_check(hipMemcpyAsync(base + off, src, bytes, hipMemcpyHostToDevice, s), "hipMemcpyAsync(scratch)");
+  });
+  const OneIntArgs a{at[0], (const int*)at[1], (const int*)at[2], (const int*)at[3], (const int*)at[4],
+                     (const int*)at[5], (const double*)at[7], t.M, t.natoms, (const double*)at[8],
+                     (const int*)at[6], (const double*)at[9], S, T, V, t.zsum};
+  hip_check(launch_oneint(a, s), "launch_oneint");
+  // the host tables are released on return, and the next call reuses the array
+  hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
 }
 
 hipError_t launch_has_nan(const double* x, size_t n, int* flag, hipStream_t s) {

@@ -240,7 +240,10 @@ struct Damper {
 
 }  // namespace
 
-OneIntHost::OneIntHost(const Molecule& mol) : M(mol.nbasis), natoms((int)mol.atoms.size()) {
+OneIntTables make_oneint_tables(const Molecule& mol) {
+  OneIntTables t;
+  t.M = mol.nbasis;
+  t.natoms = (int)mol.atoms.size();
   const auto shells = contracted_shells(mol.prims.data(), mol.prims.size(), kMaxL);
   std::vector<int> o{0}, sp, l, bs, bf(mol.nbasis, -1);
   std::vector<double> wv(mol.nbasis);
@@ -262,23 +265,39 @@ OneIntHost::OneIntHost(const Molecule& mol) : M(mol.nbasis), natoms((int)mol.ato
     for (int d = 0; d < 3; ++d) df *= (e[d] == 2 ? 3.0 : 1.0);
     wv[mu] = mol.norm[mu] / std::sqrt(df);
   }
-  std::vector<double> x;
-  std::vector<int> z;
   for (const auto& a : mol.atoms) {
-    x.insert(x.end(), a.xyz, a.xyz + 3);
-    z.push_back(a.Z);
-    zsum += std::abs((double)a.Z);
+    t.xyz.insert(t.xyz.end(), a.xyz, a.xyz + 3);
+    t.Z.push_back(a.Z);
+    t.zsum += std::abs((double)a.Z);
   }
-  shell_L = l;
-  prims.upload(mol.prims);
-  off.upload(o); sprims.upload(sp); sl.upload(l); sb.upload(bs); bfs.upload(bf); Z.upload(z);
-  w.upload(wv); xyz.upload(x);
-  boys.upload(boys_grid_xn());
+  t.prims = mol.prims;
+  t.off = std::move(o);
+  t.sprims = std::move(sp);
+  t.sl = std::move(l);
+  t.sb = std::move(bs);
+  t.bfs = std::move(bf);
+  t.w = std::move(wv);
+  t.boys = boys_grid_xn();
+  return t;
+}
+
+OneIntHost::OneIntHost(const Molecule& mol) : M(mol.nbasis), natoms((int)mol.atoms.size()) {
+  const OneIntTables t = make_oneint_tables(mol);
+  zsum = t.zsum;
+  shell_L = t.sl;
+  prims.upload(t.prims);
+  off.upload(t.off); sprims.upload(t.sprims); sl.upload(t.sl); sb.upload(t.sb); bfs.upload(t.bfs); Z.upload(t.Z);
+  w.upload(t.w); xyz.upload(t.xyz);
+  boys.upload(t.boys);
 }
 
 OneIntArgs OneIntHost::args(double* S, double* T, double* V) const {
   return OneIntArgs{prims.ptr, off.ptr, sprims.ptr, sl.ptr, sb.ptr, bfs.ptr, w.ptr, M, natoms,
                     xyz.ptr, Z.ptr, boys.ptr, S, T, V, zsum};
+}
+
+void compute_oneint_noalloc(const Molecule& mol, double* dS, double* dT, double* dV, hipStream_t s) {
+  oneint_noalloc(make_oneint_tables(mol), dS, dT, dV, s);
 }
 
 // Build S, T, V of a molecule on the device.

@@ -61,6 +61,9 @@ struct OneIntHost {
 };
 
 void compute_oneint(const Molecule& mol, double* dS, double* dT, double* dV, hipStream_t s);
+OneIntTables make_oneint_tables(const Molecule& mol);
+// compute_oneint without device allocation (oneint_noalloc)
+void compute_oneint_noalloc(const Molecule& mol, double* dS, double* dT, double* dV, hipStream_t s);
 // d_grad (device, 3 natoms) = sum D d(T+V)/dX - sum W dS/dX, overwritten.
 void compute_oneint_gradient(const Molecule& mol, const double* dD, const double* dW, double* dgrad, hipStream_t s);
 // dV_nn/dX (host, 3 natoms)

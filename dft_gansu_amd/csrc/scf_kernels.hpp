@@ -2,6 +2,9 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <cstddef>
+#include <vector>
+
+#include "gansu_fock.h"
 
 namespace gansu {
 
@@ -23,6 +26,23 @@ struct OneIntArgs {
 };
 
 hipError_t launch_oneint(const OneIntArgs& a, hipStream_t s);
+
+// Host copies of the tables k_oneint reads (the device side of OneIntArgs).
+struct OneIntTables {
+  std::vector<gansu_primitive_shell> prims;
+  std::vector<int> off, sprims, sl, sb, bfs, Z;
+  std::vector<double> w, xyz, boys;
+  int M = 0, natoms = 0;
+  double zsum = 0.0;
+};
+// No-allocation one-electron integrals: the tables are copied into a
+// file-scope device array of kOneIntScratchBytes (one per GPU, found with
+// hipGetSymbolAddress on the first call on each device) and k_oneint runs
+// on them; no hipMalloc or hipFree.  Tables beyond the array are an error
+// (GANSU_E_ARG).  Calls that overlap on one GPU share the array and race.
+constexpr size_t kOneIntScratchBytes = size_t(4) << 20;
+size_t oneint_scratch_bytes(const OneIntTables& t);  // bytes the tables take in the array
+void oneint_noalloc(const OneIntTables& t, double* S, double* T, double* V, hipStream_t s);
 hipError_t launch_transform(const double* Ucol, const double* evals, int n, double thr, double* X, hipStream_t s);
 hipError_t launch_lincomb(int nmat, const double* d_coef, const double* const* d_mats, size_t n, double* out,
                           hipStream_t s);

@@ -206,6 +206,16 @@ void gansu_fock_destroy(gansu_fock_engine* e);
 /* ------------------------------------------------ one-electron / SCF -----
  * S, T, V on the device (M x M row-major, caller-owned device pointers).    */
 int gansu_oneint(const gansu_molecule* mol, double* d_S, double* d_T, double* d_V, void* stream);
+/* gansu_oneint without device allocation.  No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under graph capture,
+ * serving loops, harnesses that forbid hipMalloc).  The molecule's shell
+ * tables and the Boys grid are copied into a 4 MiB device array of the
+ * library (one per GPU, held until the process ends) and the same kernel
+ * as gansu_oneint runs on them; the call synchronises `stream` before it
+ * returns.  Molecules whose tables exceed the array (about 50,000
+ * primitives) are GANSU_E_ARG.  Calls that overlap on one GPU share the
+ * array and race.                                                          */
+int gansu_oneint_noalloc(const gansu_molecule* mol, double* d_S, double* d_T, double* d_V, void* stream);
 
 /* Solves the molecule with the params' method (rhf, uhf or rohf; for a SAD
  * cache, a single atom) and writes its per-spin densities to path in the
