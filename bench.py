@@ -36,7 +36,11 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
 # commands and raw counts are in profiles/stored_digest_traffic_c10h22_ccpvdz.md.
 MEASURED_TRAFFIC = {
     ("C10H22/cc-pVDZ", "fock_slab(pp|ps) stored-digest"): 336.8e6,
+    # profiles/ri_mixed_64water_ccpvdz.md (FETCH_SIZE x2.0, WRITE_SIZE x1.0 by a 2 GiB copy)
+    ("64 waters", "k_rik_tstep (T_E = B E, fp16 MFMA)"): 125.9e9,
+    ("64 waters", "k_rik_dk (Z = T_A T_E^T, fp16 MFMA)"): 279.5e9,
 }
+TRAFFIC_SOURCE = "stored constant: rocprofv3 FETCH_SIZE / WRITE_SIZE per launch, scaled by a 2 GiB copy (profiles/)"
 FP16_MFMA_PEAK_TFS = 2500.0  # MI355X dense fp16 MFMA, TFLOP/s
 # kernel ids of the engine's profile rows (incremental RI build)
 RIK_NAMES = {80020: "align (Procrustes)", 80021: "split E", 80022: "k_rik_tstep (T_E = B E, fp16 MFMA)",
@@ -46,16 +50,26 @@ RIK_NAMES = {80020: "align (Procrustes)", 80021: "split E", 80022: "k_rik_tstep 
 
 
 def orbital_sequence(M, n, seed, amp=1e-3, decay=0.7):
-    """n orthonormal M x M orbital matrices (torch, on the GPU) whose span
-    moves by amp decay^k per step, with a random rotation of the first
-    columns each step (what successive SCF diagonalisations hand the build)."""
+    """n orthonormal M x M orbital matrices (torch, on the GPU): a random
+    orthogonal C_0 (host QR), then C_k = C_(k-1) exp(A_k) with A_k a random
+    antisymmetric matrix of norm ~ amp decay^k (exponential by its Taylor
+    series, matrix products on the GPU), so that the occupied space moves
+    by about amp decay^k per step, as between SCF diagonalisations."""
+    import numpy as np
     import torch
+    rng = np.random.default_rng(seed)
+    C = torch.tensor(np.linalg.qr(rng.normal(size=(M, M)))[0], dtype=torch.float64, device="cuda").contiguous()
     g = torch.Generator(device="cuda").manual_seed(seed)
-    C = torch.linalg.qr(torch.randn(M, M, generator=g, device="cuda", dtype=torch.float64))[0].contiguous()
     out = [C]
     for k in range(1, n):
-        C = C + amp * decay ** (k - 1) * torch.randn(M, M, generator=g, device="cuda", dtype=torch.float64) / M ** 0.5
-        C = torch.linalg.qr(C)[0].contiguous()
+        A = torch.randn(M, M, generator=g, device="cuda", dtype=torch.float64) * (amp * decay ** (k - 1) / M)
+        A = A - A.T
+        R = torch.eye(M, dtype=torch.float64, device="cuda")
+        term = R.clone()
+        for j in range(1, 8):
+            term = term @ A / j
+            R = R + term
+        C = (C @ R).contiguous()
         out.append(C)
     return out
 
@@ -120,6 +134,7 @@ def ri_headline(steps, warmup, log_fn):
     tfs = dom[3] / (dom[1] * 1e-3) / 1e12
     roof = dict(bound="mfma", kernel=dom[0], achieved=round(tfs, 1), peak=FP16_MFMA_PEAK_TFS, unit="TFLOP/s",
                 frac=round(tfs / FP16_MFMA_PEAK_TFS, 4), traffic=MEASURED_TRAFFIC.get(("64 waters", dom[0])),
+                traffic_source=TRAFFIC_SOURCE,
                 flops_per_launch=dom[3], avg_launch_ms=round(dom[1], 3),
                 flops_note="three fp16 MFMA products (hi/lo split) per multiply-add of the algorithm")
     per_kernel = []
@@ -445,6 +460,7 @@ def stored_leg(cfg, method, precision, steps, warmup, no_direct, no_scf, no_mixe
         gbs = dom[2] / (dom[1] * 1e-3) / 1e9
         roof = dict(bound="hbm", kernel=kernel_name(dom[0]), achieved=round(gbs, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=round(gbs / HBM_PEAK_GBS, 4), traffic=MEASURED_TRAFFIC.get((cfgd["name"], kernel_name(dom[0]))),
+                    traffic_source=TRAFFIC_SOURCE,
                     bytes_per_launch=dom[2], avg_launch_ms=round(dom[1], 4),
                     all_digest_kernels=dict(ms=round(build_kernel_ms, 4), bytes=all_bytes,
                                             GBs=round(all_bytes / (build_kernel_ms * 1e-3) / 1e9, 1)))
@@ -519,6 +535,9 @@ def main():
     ap.add_argument("--no-stored", action="store_true", help="skip the C10H22/cc-pVDZ four-centre leg")
     ap.add_argument("--no-grad", action="store_true", help="skip the C10H22/6-31G* gradient leg")
     ap.add_argument("--no-coronene", action="store_true", help="skip the C24H12 four-centre + RI-JK leg")
+    ap.add_argument("--copy-probe-gib", type=float, default=0.0,
+                    help="run one device copy of this many GiB first (a known-size kernel that scales the "
+                         "rocprofv3 FETCH_SIZE / WRITE_SIZE counts of the other kernels)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the Fock build runs on a single MI355X in this build "
@@ -533,6 +552,15 @@ def main():
     # one small SCF first (water STO-3G), so that everything below runs with
     # rocBLAS / rocSOLVER already initialised in this process
     rhf(gm.config_molecule(0)[0], Parameters(convergence_energy_threshold=1e-8))
+    if args.copy_probe_gib > 0:
+        n = int(args.copy_probe_gib * 2 ** 30 / 8)
+        src = torch.ones(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+        torch.cuda.empty_cache()
+        log(f"[bench] copy probe: {args.copy_probe_gib} GiB read, {args.copy_probe_gib} GiB written")
     head = ri_headline(args.steps, args.warmup, log)
     extra = {}
     if not args.no_ri:
