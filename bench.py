@@ -196,9 +196,18 @@ def cpu_baseline(cfg, target_s):
             break
         stride = max(1, int(stride * max(dt, 1e-3) / target_s * 0.9)) if dt > 0.05 else max(1, stride // 8)
     full_s = dt * stride
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            model = next((ln.split(":", 1)[1].strip() for ln in fh if ln.startswith("model name")), "")
+    except OSError:
+        pass
     return dict(value=1.0 / full_s, unit="Fock builds/s", cores=O.lib().orc_num_threads(), kind="port",
-                sample=f"direct J/K (MD integrals + digestion) over every {stride}-th bra primitive-pair row "
-                       f"({dt:.1f} s measured, x{stride} extrapolated; full build ~{full_s:.0f} s)")
+                nproc=os.cpu_count(), cpu_model=model, omp_num_threads=os.environ.get("OMP_NUM_THREADS"),
+                sample=f"{c['name']}: the FP64 host oracle's four-centre direct J/K (MD integrals + digestion, "
+                       f"Schwarz 1e-12) over every {stride}-th bra primitive-pair row ({dt:.1f} s measured, "
+                       f"x{stride} extrapolated; full build ~{full_s:.0f} s); the host cannot hold the RI tensor "
+                       f"(126 GB FP64), so the GPU's RI build is set against the CPU's four-centre build")
 
 
 def ri_leg(log_fn, mixed_rate, no_scf=False):
