@@ -22,6 +22,7 @@
 // final J, K are the symmetrised accumulators (gpu_kernels.cu:230-287 defines
 // F = H + J - K/2 on the dense tensor).
 #include <hip/hip_runtime.h>
+#include <algorithm>
 
 #include "fock_kernels.hpp"
 #include "md_device.hpp"
@@ -394,148 +395,246 @@ __global__ __launch_bounds__(64) void k_eri_class(ClassLaunch c, EriArgs g) {
   }
 }
 
-// ------------------------------------------------------ generic kernel ----
-// Runtime angular momenta (d-containing classes with large blocks): the same
-// algorithm with private arrays.
-constexpr int kGenMaxComp = 36 * 36;
-constexpr int kGenMaxE = 3 * 3 * 5;  // (l+1)(l+1)(2l+1) for l <= 2
+// --------------------------------------------------- cooperative kernel ----
+// The classes whose blocks exceed a lane's registers ((dp|pp) up to (dd|dd)):
+// one workgroup of NT threads (64, 128 or 256 by block size) per contracted
+// shell quartet.  Per primitive quartet, the Boys values (one lane), the
+// Hermite R_tuv (level by level, the threads over the entries of a level,
+// ping-pong buffers in LDS), the ket-contracted
+// X[cd][tuv] = sum E^cd (-1)^(tau+nu+phi) R[t+tau, u+nu, v+phi] and the bra
+// contraction are split over the threads; a thread owns block components
+// tid, tid + NT, ... in registers.  Nothing is held in private arrays, so the
+// kernel needs no scratch memory.  blockIdx.y splits a block's kets over
+// gridDim.y workgroups.
+template <int LA, int LB, int LC, int LD>
+struct CoopShape {
+  static constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
+  static constexpr int NAB = NA * NB, NCD = NC * ND, NCOMP = NAB * NCD;
+  static constexpr int LAB = LA + LB, LCD = LC + LD, L = LAB + LCD;
+  static constexpr int NH = nherm(L), NHB = nherm(LAB);
+  static constexpr int NEB = (LA + 1) * (LB + 1) * (LAB + 1), NEK = (LC + 1) * (LD + 1) * (LCD + 1);
+  static constexpr int NT = NCOMP > 640 ? 256 : NCOMP > 192 ? 128 : 64;
+  static constexpr int NOWN = (NCOMP + NT - 1) / NT;
+  static constexpr int NXO = NCD * NHB > NCOMP ? NCD * NHB : NCOMP;
+};
 
-__global__ __launch_bounds__(64) void k_eri_generic(ClassLaunch c, EriArgs g, int mode) {
-  const int LA = c.LA, LB = c.LB, LC = c.LC, LD = c.LD;
-  const int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
-  const int NAB = NA * NB, NCD = NC * ND, NCOMP = NAB * NCD;
-  const int LAB = LA + LB, LCD = LC + LD, L = LAB + LCD;
+// Packed Cartesian exponents (x | y << 2 | z << 4) of component c of an l <= 2 shell.
+__device__ __forceinline__ int cart_pack(int l, int c) {
+  if (l == 0) return 0;
+  if (l == 1) return 1 << (2 * c);
+  return c < 3 ? 2 << (2 * c) : c == 3 ? (1 | 1 << 2) : c == 4 ? (1 | 1 << 4) : (1 << 2 | 1 << 4);
+}
+
+template <int LA, int LB, int LC, int LD, int MODE>
+__global__ __launch_bounds__((CoopShape<LA, LB, LC, LD>::NT)) void k_eri_coop(ClassLaunch c, EriArgs g) {
+  using S = CoopShape<LA, LB, LC, LD>;
+  constexpr int NA = S::NA, NB = S::NB, NC = S::NC, ND = S::ND, NAB = S::NAB, NCD = S::NCD;
+  constexpr int NCOMP = S::NCOMP, LAB = S::LAB, LCD = S::LCD, L = S::L;
+  constexpr int NH = S::NH, NHB = S::NHB, NT = S::NT, NOWN = S::NOWN;
+  __shared__ double sR[2][NH];        // R of two consecutive recursion levels
+  __shared__ double sXO[S::NXO];      // X[cd][tuv] while integrating, the block for digestion
+  __shared__ double sEb[3][S::NEB], sEk[3][S::NEK];
+  __shared__ double sF[L + 1];        // scaled Boys values
+  __shared__ int sTuv[NH];            // t | u << 4 | v << 8 of each level-ordered Hermite index
+  __shared__ int sAb[NAB], sCd[NCD];  // packed exponents of a, c (bits 0-5) and b, d (bits 6-11)
+  const int tid = threadIdx.x;
+  for (int h = tid; h < NH; h += NT) {
+    int n = 0;
+    while (nherm(n) <= h) ++n;
+    const int r = h - (n > 0 ? nherm(n - 1) : 0);
+    int s = 0;
+    while ((s + 1) * (s + 2) / 2 <= r) ++s;
+    const int v = r - s * (s + 1) / 2;
+    sTuv[h] = (n - s) | (s - v) << 4 | v << 8;
+  }
+  for (int ab = tid; ab < NAB; ab += NT) sAb[ab] = cart_pack(LA, ab / NB) | cart_pack(LB, ab % NB) << 6;
+  for (int cd = tid; cd < NCD; cd += NT) sCd[cd] = cart_pack(LC, cd / ND) | cart_pack(LD, cd % ND) << 6;
   const BlockRec blk = c.blocks[blockIdx.x];
-  const int lane = threadIdx.x;
-  const bool active = lane < blk.count;
   const int bra_id = c.bra_list[blk.bra_pos];
-  const int ket_id = c.ket_list[blk.ket_start + (active ? lane : 0)];
   const SPair bra = g.sp[bra_id];
-  const SPair ket = g.sp[ket_id];
-  double out[kGenMaxComp];
-  for (int k = 0; k < NCOMP; ++k) out[k] = 0.0;
   const bool f32 = blk.slot < 0;
-  const size_t at = (size_t)(f32 ? -blk.slot - 1 : blk.slot) * NCOMP * 64 + lane;
-  if (mode == kModeStoredDigest) {
-    for (int k = 0; k < NCOMP; ++k)
-      out[k] = active ? (f32 ? (double)c.store32[at + (size_t)k * 64] : c.store[at + (size_t)k * 64]) : 0.0;
-  } else {
-    const int ntb = LAB + 1, ntk = LCD + 1;
-    const int nk = active ? ket.nprim : 0;
-    for (int ib = 0; ib < bra.nprim; ++ib) {
-      const PairRec pb = g.pr[bra.prim_begin + ib];
-      double Eb[3][kGenMaxE];
-      for (int d = 0; d < 3; ++d) hermite_e_rt<double>(LA, LB, pb.p, pb.PA[d], pb.PB[d], Eb[d]);
-      for (int ik = 0; ik < nk; ++ik) {
-        const PairRec pk = g.pr[ket.prim_begin + ik];
-        if (pb.Q * pk.Q < g.thr) continue;
-        double Ek[3][kGenMaxE];
-        for (int d = 0; d < 3; ++d) hermite_e_rt<double>(LC, LD, pk.p, pk.PA[d], pk.PB[d], Ek[d]);
-        const double p = pb.p, q = pk.p;
-        const double alpha = p * q / (p + q);
-        const double pref = kTwoPi25 / (p * q * sqrt(p + q)) * pb.Kc * pk.Kc;
-        double R[nherm(8)];
-        hermite_r_rt<double>(L, alpha, pb.P[0] - pk.P[0], pb.P[1] - pk.P[1], pb.P[2] - pk.P[2], pref, g.boys, R);
-        for (int cd = 0; cd < NCD; ++cd) {
-          const int ci = cd / ND, di = cd % ND;
-          int ce[3], de[3];
-          for (int d = 0; d < 3; ++d) { ce[d] = cart_exp(LC, ci, d); de[d] = cart_exp(LD, di, d); }
-          const double* Ekx = Ek[0] + (ce[0] * (LD + 1) + de[0]) * ntk;
-          const double* Eky = Ek[1] + (ce[1] * (LD + 1) + de[1]) * ntk;
-          const double* Ekz = Ek[2] + (ce[2] * (LD + 1) + de[2]) * ntk;
-          double X[nherm(4)];
-          for (int t = 0; t <= LAB; ++t)
-            for (int u = 0; u + t <= LAB; ++u)
-              for (int v = 0; v + u + t <= LAB; ++v) {
-                double s = 0.0;
-                for (int tt = 0; tt <= ce[0] + de[0]; ++tt)
-                  for (int uu = 0; uu <= ce[1] + de[1]; ++uu) {
-                    const double exy = Ekx[tt] * Eky[uu];
-                    for (int vv = 0; vv <= ce[2] + de[2]; ++vv) {
-                      const double e = ((tt + uu + vv) & 1) ? -exy * Ekz[vv] : exy * Ekz[vv];
-                      s += e * R[hidx(t + tt, u + uu, v + vv)];
-                    }
-                  }
-                X[hidx(t, u, v)] = s;
-              }
-          for (int ab = 0; ab < NAB; ++ab) {
-            const int ai = ab / NB, bi_ = ab % NB;
-            int ae[3], be[3];
-            for (int d = 0; d < 3; ++d) { ae[d] = cart_exp(LA, ai, d); be[d] = cart_exp(LB, bi_, d); }
-            const double* Ebx = Eb[0] + (ae[0] * (LB + 1) + be[0]) * ntb;
-            const double* Eby = Eb[1] + (ae[1] * (LB + 1) + be[1]) * ntb;
-            const double* Ebz = Eb[2] + (ae[2] * (LB + 1) + be[2]) * ntb;
-            double s = 0.0;
-            for (int t = 0; t <= ae[0] + be[0]; ++t)
-              for (int u = 0; u <= ae[1] + be[1]; ++u) {
-                const double exy = Ebx[t] * Eby[u];
-                for (int v = 0; v <= ae[2] + be[2]; ++v) s += exy * Ebz[v] * X[hidx(t, u, v)];
-              }
-            out[ab * NCD + cd] += s;
+  const size_t slot_at = (size_t)(f32 ? -blk.slot - 1 : blk.slot) * NCOMP * 64;
+  __syncthreads();
+
+  for (int q = blockIdx.y; q < blk.count; q += gridDim.y) {
+    const int ket_id = c.ket_list[blk.ket_start + q];
+    const SPair ket = g.sp[ket_id];
+    double acc[NOWN];
+#pragma unroll
+    for (int j = 0; j < NOWN; ++j) acc[j] = 0.0;
+    if constexpr (MODE == kModeStoredDigest) {
+#pragma unroll
+      for (int j = 0; j < NOWN; ++j) {
+        const int k = tid + NT * j;
+        if (k < NCOMP)
+          acc[j] = f32 ? (double)c.store32[slot_at + (size_t)k * 64 + q] : c.store[slot_at + (size_t)k * 64 + q];
+      }
+    } else {
+      for (int ib = 0; ib < bra.nprim; ++ib) {
+        const PairRec* pb = g.pr + bra.prim_begin + ib;
+        if (tid < 3) hermite_e_rt<double>(LA, LB, pb->p, pb->PA[tid], pb->PB[tid], sEb[tid]);
+        for (int ik = 0; ik < ket.nprim; ++ik) {
+          const PairRec* pk = g.pr + ket.prim_begin + ik;
+          if (pb->Q * pk->Q < g.thr) continue;  // the same for every thread
+          const double p = pb->p, qe = pk->p;
+          const double alpha = p * qe / (p + qe);
+          const double X = pb->P[0] - pk->P[0], Y = pb->P[1] - pk->P[1], Z = pb->P[2] - pk->P[2];
+          if (tid >= 3 && tid < 6) {
+            hermite_e_rt<double>(LC, LD, qe, pk->PA[tid - 3], pk->PB[tid - 3], sEk[tid - 3]);
+          } else if (tid == 6) {
+            double F[L + 1];
+            boys_eval_t<L, double>(alpha * (X * X + Y * Y + Z * Z), g.boys, F);
+            double sc = kTwoPi25 / (p * qe * sqrt(p + qe)) * pb->Kc * pk->Kc;
+            const double m2a = -2.0 * alpha;
+#pragma unroll
+            for (int n = 0; n <= L; ++n) {
+              sF[n] = F[n] * sc;
+              sc *= m2a;
+            }
           }
+          __syncthreads();
+          // level n of the recursion holds R^(n)_tuv for t + u + v <= L - n
+#pragma unroll
+          for (int n = L - 1; n >= 0; --n) {
+            const double* src = sR[(n + 1) & 1];
+            double* dst = sR[n & 1];
+            for (int h = tid; h < nherm(L - n); h += NT) {
+              double val;
+              if (h == 0) {
+                val = sF[n];
+              } else if (n == L - 1) {  // level L holds only R^(L)_000 = F_L
+                const int w = sTuv[h];
+                val = ((w & 15) ? X : ((w >> 4) & 15) ? Y : Z) * sF[L];
+              } else {
+                const int w = sTuv[h], t = w & 15, u = (w >> 4) & 15, v = w >> 8;
+                if (t > 0) {
+                  val = X * src[hidx(t - 1, u, v)];
+                  if (t > 1) val += double(t - 1) * src[hidx(t - 2, u, v)];
+                } else if (u > 0) {
+                  val = Y * src[hidx(t, u - 1, v)];
+                  if (u > 1) val += double(u - 1) * src[hidx(t, u - 2, v)];
+                } else {
+                  val = Z * src[hidx(t, u, v - 1)];
+                  if (v > 1) val += double(v - 1) * src[hidx(t, u, v - 2)];
+                }
+              }
+              dst[h] = val;
+            }
+            __syncthreads();
+          }
+          const double* R = sR[0];
+          for (int it = tid; it < NCD * NHB; it += NT) {
+            const int cd = it / NHB, h = it % NHB;
+            const int w = sTuv[h], t = w & 15, u = (w >> 4) & 15, v = w >> 8;
+            const int e = sCd[cd];
+            const int cx = e & 3, cy = (e >> 2) & 3, cz = (e >> 4) & 3;
+            const int dx = (e >> 6) & 3, dy = (e >> 8) & 3, dz = (e >> 10) & 3;
+            const double* ex = &sEk[0][(cx * (LD + 1) + dx) * (LCD + 1)];
+            const double* ey = &sEk[1][(cy * (LD + 1) + dy) * (LCD + 1)];
+            const double* ez = &sEk[2][(cz * (LD + 1) + dz) * (LCD + 1)];
+            double s = 0.0;
+            for (int tt = 0; tt <= cx + dx; ++tt)
+              for (int uu = 0; uu <= cy + dy; ++uu) {
+                const double exy = ((tt + uu) & 1) ? -ex[tt] * ey[uu] : ex[tt] * ey[uu];
+                for (int vv = 0; vv <= cz + dz; ++vv) {
+                  const double e3 = (vv & 1) ? -exy * ez[vv] : exy * ez[vv];
+                  s += e3 * R[hidx(t + tt, u + uu, v + vv)];
+                }
+              }
+            sXO[cd * NHB + h] = s;
+          }
+          __syncthreads();
+#pragma unroll
+          for (int j = 0; j < NOWN; ++j) {
+            const int k = tid + NT * j;
+            if (k < NCOMP) {
+              const int ab = k / NCD;
+              const int e = sAb[ab];
+              const int ax = e & 3, ay = (e >> 2) & 3, az = (e >> 4) & 3;
+              const int bx = (e >> 6) & 3, by = (e >> 8) & 3, bz = (e >> 10) & 3;
+              const double* ex = &sEb[0][(ax * (LB + 1) + bx) * (LAB + 1)];
+              const double* ey = &sEb[1][(ay * (LB + 1) + by) * (LAB + 1)];
+              const double* ez = &sEb[2][(az * (LB + 1) + bz) * (LAB + 1)];
+              const double* xr = &sXO[(k - ab * NCD) * NHB];
+              double s = 0.0;
+              for (int t = 0; t <= ax + bx; ++t)
+                for (int u = 0; u <= ay + by; ++u) {
+                  const double exy = ex[t] * ey[u];
+                  for (int v = 0; v <= az + bz; ++v) s += exy * ez[v] * xr[hidx(t, u, v)];
+                }
+              acc[j] += s;
+            }
+          }
+          __syncthreads();  // the next primitive quartet rewrites sF, sR, sEk and sXO
         }
       }
     }
-  }
-  if (mode == kModeStore) {
-    if (f32)
-      for (int k = 0; k < NCOMP; ++k) c.store32[at + (size_t)k * 64] = (float)out[k];
-    else
-      for (int k = 0; k < NCOMP; ++k) c.store[at + (size_t)k * 64] = out[k];
-    return;
-  }
-  // digestion with runtime shapes
-  const int M = g.M;
-  const bool ij_diff = bra.si != bra.sj, kl_diff = ket.si != ket.sj, pair_diff = bra_id != ket_id;
-  const double fbk = (ij_diff ? 2.0 : 1.0) * (kl_diff ? 2.0 : 1.0);
-  const double fo = pair_diff ? 2.0 : 1.0;
-  const int bi = bra.bi, bj = bra.bj, bk = ket.bi, bl = ket.bj;
-  const double* D = g.Dp;
-  for (int ab = 0; ab < NAB; ++ab) {
-    double s = 0.0;
-    if (active)
-      for (int cd = 0; cd < NCD; ++cd) s += out[ab * NCD + cd] * D[(size_t)(bk + cd / ND) * M + bl + cd % ND];
-    s = wave_sum(s * fbk);
-    if (lane == 0) unsafeAtomicAdd(&g.Jr[(size_t)(bi + ab / NB) * M + bj + ab % NB], s);
-  }
-  if (!active) return;
-  if (pair_diff)
-    for (int cd = 0; cd < NCD; ++cd) {
-      double s = 0.0;
-      for (int ab = 0; ab < NAB; ++ab) s += out[ab * NCD + cd] * D[(size_t)(bi + ab / NB) * M + bj + ab % NB];
-      unsafeAtomicAdd(&g.Jr[(size_t)(bk + cd / ND) * M + bl + cd % ND], s * fbk);
+    if constexpr (MODE == kModeStore) {
+      // [slot][component][64 kets of the block]
+#pragma unroll
+      for (int j = 0; j < NOWN; ++j) {
+        const int k = tid + NT * j;
+        if (k < NCOMP) {
+          if (f32) c.store32[slot_at + (size_t)k * 64 + q] = (float)acc[j];
+          else c.store[slot_at + (size_t)k * 64 + q] = acc[j];
+        }
+      }
+    } else {
+      // digestion with the 8-fold weights: J of the bra and (for distinct
+      // pairs) ket blocks, and the four K updates
+#pragma unroll
+      for (int j = 0; j < NOWN; ++j)
+        if (tid + NT * j < NCOMP) sXO[tid + NT * j] = acc[j];
+      __syncthreads();
+      const int M = g.M;
+      const bool ij_diff = bra.si != bra.sj, kl_diff = ket.si != ket.sj, pair_diff = bra_id != ket_id;
+      const double fbk = (ij_diff ? 2.0 : 1.0) * (kl_diff ? 2.0 : 1.0);
+      const double fo = pair_diff ? 2.0 : 1.0;
+      const int bi = bra.bi, bj = bra.bj, bk = ket.bi, bl = ket.bj;
+      const double* D = g.Dp;
+      const double* O = sXO;
+      for (int ab = tid; ab < NAB; ab += NT) {
+        double s = 0.0;
+        for (int cd = 0; cd < NCD; ++cd) s += O[ab * NCD + cd] * D[(size_t)(bk + cd / ND) * M + bl + cd % ND];
+        unsafeAtomicAdd(&g.Jr[(size_t)(bi + ab / NB) * M + bj + ab % NB], s * fbk);
+      }
+      if (pair_diff)
+        for (int cd = tid; cd < NCD; cd += NT) {
+          double s = 0.0;
+          for (int ab = 0; ab < NAB; ++ab) s += O[ab * NCD + cd] * D[(size_t)(bi + ab / NB) * M + bj + ab % NB];
+          unsafeAtomicAdd(&g.Jr[(size_t)(bk + cd / ND) * M + bl + cd % ND], s * fbk);
+        }
+      constexpr int NT0 = NA * NC, NT1 = NB * NC, NT2 = NA * ND, NT3 = NB * ND;
+      for (int it = tid; it < NT0 + NT1 + NT2 + NT3; it += NT) {
+        double s = 0.0;
+        if (it < NT0) {  // K[a][c] += (ab|cd) D[b][d]
+          const int a = it / NC, cc = it % NC;
+          for (int b = 0; b < NB; ++b)
+            for (int d = 0; d < ND; ++d) s += O[(a * NB + b) * NCD + cc * ND + d] * D[(size_t)(bj + b) * M + bl + d];
+          unsafeAtomicAdd(&g.Kr[(size_t)(bi + a) * M + bk + cc], s * fo);
+        } else if (it < NT0 + NT1) {  // K[b][c] += (ab|cd) D[a][d]
+          if (!ij_diff) continue;
+          const int b = (it - NT0) / NC, cc = (it - NT0) % NC;
+          for (int a = 0; a < NA; ++a)
+            for (int d = 0; d < ND; ++d) s += O[(a * NB + b) * NCD + cc * ND + d] * D[(size_t)(bi + a) * M + bl + d];
+          unsafeAtomicAdd(&g.Kr[(size_t)(bj + b) * M + bk + cc], s * fo);
+        } else if (it < NT0 + NT1 + NT2) {  // K[a][d] += (ab|cd) D[b][c]
+          if (!kl_diff) continue;
+          const int a = (it - NT0 - NT1) / ND, d = (it - NT0 - NT1) % ND;
+          for (int b = 0; b < NB; ++b)
+            for (int cc = 0; cc < NC; ++cc) s += O[(a * NB + b) * NCD + cc * ND + d] * D[(size_t)(bj + b) * M + bk + cc];
+          unsafeAtomicAdd(&g.Kr[(size_t)(bi + a) * M + bl + d], s * fo);
+        } else {  // K[b][d] += (ab|cd) D[a][c]
+          if (!(ij_diff && kl_diff)) continue;
+          const int b = (it - NT0 - NT1 - NT2) / ND, d = (it - NT0 - NT1 - NT2) % ND;
+          for (int a = 0; a < NA; ++a)
+            for (int cc = 0; cc < NC; ++cc) s += O[(a * NB + b) * NCD + cc * ND + d] * D[(size_t)(bi + a) * M + bk + cc];
+          unsafeAtomicAdd(&g.Kr[(size_t)(bj + b) * M + bl + d], s * fo);
+        }
+      }
+      __syncthreads();  // the next ket rewrites sXO
     }
-  for (int a = 0; a < NA; ++a)
-    for (int cc = 0; cc < NC; ++cc) {
-      double s = 0.0;
-      for (int b = 0; b < NB; ++b)
-        for (int d = 0; d < ND; ++d) s += out[(a * NB + b) * NCD + cc * ND + d] * D[(size_t)(bj + b) * M + bl + d];
-      unsafeAtomicAdd(&g.Kr[(size_t)(bi + a) * M + bk + cc], s * fo);
-    }
-  if (ij_diff)
-    for (int b = 0; b < NB; ++b)
-      for (int cc = 0; cc < NC; ++cc) {
-        double s = 0.0;
-        for (int a = 0; a < NA; ++a)
-          for (int d = 0; d < ND; ++d) s += out[(a * NB + b) * NCD + cc * ND + d] * D[(size_t)(bi + a) * M + bl + d];
-        unsafeAtomicAdd(&g.Kr[(size_t)(bj + b) * M + bk + cc], s * fo);
-      }
-  if (kl_diff)
-    for (int a = 0; a < NA; ++a)
-      for (int d = 0; d < ND; ++d) {
-        double s = 0.0;
-        for (int b = 0; b < NB; ++b)
-          for (int cc = 0; cc < NC; ++cc) s += out[(a * NB + b) * NCD + cc * ND + d] * D[(size_t)(bj + b) * M + bk + cc];
-        unsafeAtomicAdd(&g.Kr[(size_t)(bi + a) * M + bl + d], s * fo);
-      }
-  if (ij_diff && kl_diff)
-    for (int b = 0; b < NB; ++b)
-      for (int d = 0; d < ND; ++d) {
-        double s = 0.0;
-        for (int a = 0; a < NA; ++a)
-          for (int cc = 0; cc < NC; ++cc) s += out[(a * NB + b) * NCD + cc * ND + d] * D[(size_t)(bi + a) * M + bk + cc];
-        unsafeAtomicAdd(&g.Kr[(size_t)(bj + b) * M + bl + d], s * fo);
-      }
+  }
 }
 
 // ---------------------------------------------------------- elementwise ----
@@ -1022,13 +1121,30 @@ hipError_t launch_pair_data(const void* prims, const int* pa, const int* pb, con
   return hipGetLastError();
 }
 
+// One wavefront per quartet; a block's kets are split over enough
+// wavefronts (blockIdx.y) to fill the device.
+template <int LA, int LB, int LC, int LD>
+hipError_t launch_coop_t(const ClassLaunch& c, const EriArgs& a, int mode, hipStream_t s) {
+  const unsigned split = (unsigned)std::min(64, std::max(1, (8192 + c.nblocks - 1) / c.nblocks));
+  const dim3 grid((unsigned)c.nblocks, split), block(CoopShape<LA, LB, LC, LD>::NT);
+  if (mode == kModeDirect)
+    hipLaunchKernelGGL((k_eri_coop<LA, LB, LC, LD, kModeDirect>), grid, block, 0, s, c, a);
+  else if (mode == kModeStore)
+    hipLaunchKernelGGL((k_eri_coop<LA, LB, LC, LD, kModeStore>), grid, block, 0, s, c, a);
+  else
+    hipLaunchKernelGGL((k_eri_coop<LA, LB, LC, LD, kModeStoredDigest>), grid, block, 0, s, c, a);
+  return hipGetLastError();
+}
+
 // Classes whose block and Hermite tables fit a lane's registers run the
-// templated kernel; the rest the generic one.
+// templated one-lane-per-quartet kernel; the d-rich rest the cooperative one.
 hipError_t launch_eri_class(const ClassLaunch& c, const EriArgs& a, int mode, hipStream_t s) {
   if (c.nblocks == 0) return hipSuccess;
   const int key = ((c.LA * 3 + c.LB) * 3 + c.LC) * 3 + c.LD;
 #define GANSU_CASE(a_, b_, c_, d_) \
   case ((a_ * 3 + b_) * 3 + c_) * 3 + d_: return launch_t<a_, b_, c_, d_>(c, a, mode, s);
+#define GANSU_COOP(a_, b_, c_, d_) \
+  case ((a_ * 3 + b_) * 3 + c_) * 3 + d_: return launch_coop_t<a_, b_, c_, d_>(c, a, mode, s);
   switch (key) {
     GANSU_CASE(0, 0, 0, 0)
     GANSU_CASE(1, 0, 0, 0)
@@ -1043,11 +1159,18 @@ hipError_t launch_eri_class(const ClassLaunch& c, const EriArgs& a, int mode, hi
     GANSU_CASE(2, 1, 0, 0)
     GANSU_CASE(2, 1, 1, 0)
     GANSU_CASE(2, 2, 0, 0)
-    default:
-      hipLaunchKernelGGL(k_eri_generic, dim3(c.nblocks), dim3(64), 0, s, c, a, mode);
-      return hipGetLastError();
+    GANSU_COOP(2, 1, 1, 1)
+    GANSU_COOP(2, 1, 2, 0)
+    GANSU_COOP(2, 1, 2, 1)
+    GANSU_COOP(2, 2, 1, 0)
+    GANSU_COOP(2, 2, 1, 1)
+    GANSU_COOP(2, 2, 2, 0)
+    GANSU_COOP(2, 2, 2, 1)
+    GANSU_COOP(2, 2, 2, 2)
+    default: return hipErrorInvalidValue;
   }
 #undef GANSU_CASE
+#undef GANSU_COOP
 }
 
 
