@@ -413,6 +413,46 @@ def gradient_leg(log_fn):
     return out
 
 
+def direct_leg(cfgs, log_fn, steps=3):
+    """Direct (integral-recomputing) FP64 four-centre builds of the given
+    workloads: wall time per build, then one profiled build whose per-class
+    kernel rows (HIP events, the classes run in sequence) are reported."""
+    import numpy as np
+    import torch
+    from dft_gansu_amd import FockEngine
+    from dft_gansu_amd import molecule as gm
+    out = {}
+    for cfg in cfgs:
+        mol, _ = gm.config_molecule(cfg)
+        M, nocc = mol.n_basis, mol.n_electrons // 2
+        rng = np.random.default_rng(7)
+        C = torch.tensor(np.linalg.qr(rng.normal(size=(M, M)))[0], dtype=torch.float64, device="cuda")
+        D = (2.0 * C[:, :nocc] @ C[:, :nocc].T).contiguous()
+        H = torch.zeros_like(D)
+        e = FockEngine(mol, eri_method="direct")
+        e.precomputation()
+        e.compute_fock_matrix(D, H)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            e.compute_fock_matrix(D, H)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) / steps * 1e3
+        e.set_profiling(True)
+        e.compute_fock_matrix(D, H)
+        torch.cuda.synchronize()
+        rows = e.last_profile()
+        e.set_profiling(False)
+        top = sorted(rows, key=lambda r: -r[1])[:12]
+        name = {1: "C10H22/cc-pVDZ", 2: "C24H12/cc-pVDZ"}.get(cfg, f"config {cfg}")
+        out[name] = dict(M=M, ms_per_build=round(ms, 2), sequential_kernel_ms=round(float(rows[:, 1].sum()), 2),
+                         classes={kernel_name(r[0]): dict(ms=round(float(r[1]), 3), quartets=int(r[4])) for r in top})
+        log_fn(f"[bench] direct {name} (M={M}): {ms:.2f} ms/build")
+        del e
+        torch.cuda.empty_cache()
+    return out
+
+
 def stored_leg(cfg, method, precision, steps, warmup, no_direct, no_scf, no_mixed, log_fn):
     """Four-centre leg (C10H22/cc-pVDZ by default): the stored (or direct)
     build with the integrals resident in HBM, the live roofline of its
@@ -547,6 +587,9 @@ def main():
     ap.add_argument("--copy-probe-gib", type=float, default=0.0,
                     help="run one device copy of this many GiB first (a known-size kernel that scales the "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE counts of the other kernels)")
+    ap.add_argument("--direct-only", action="store_true",
+                    help="only the direct FP64 four-centre builds of C10H22 and C24H12 (cc-pVDZ), "
+                         "with per-class kernel times")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the Fock build runs on a single MI355X in this build "
@@ -561,6 +604,9 @@ def main():
     # one small SCF first (water STO-3G), so that everything below runs with
     # rocBLAS / rocSOLVER already initialised in this process
     rhf(gm.config_molecule(0)[0], Parameters(convergence_energy_threshold=1e-8))
+    if args.direct_only:
+        print(json.dumps({"direct_fp64": direct_leg([1, 2], log)}), flush=True)
+        return
     if args.copy_probe_gib > 0:
         n = int(args.copy_probe_gib * 2 ** 30 / 8)
         src = torch.ones(n, dtype=torch.float64, device="cuda")

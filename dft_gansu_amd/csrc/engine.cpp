@@ -232,7 +232,7 @@ void FockEngine::precompute_pairs() {
     dpa.upload(pa);
     dpb.upload(pb);
     d_Qref_.alloc(ref_pairs_);
-    GANSU_HIP(launch_schwarz_ref(d_prims_.ptr, dpa.ptr, dpb.ptr, ref_pairs_, boys_.ptr, 0, d_Qref_.ptr, 0));
+    GANSU_HIP(launch_schwarz_ref(d_prims_.ptr, dpa.ptr, dpb.ptr, ref_pairs_, d_Qref_.ptr, 0));
     GANSU_HIP(hipDeviceSynchronize());
   }
   Qref_h_ = d_Qref_.download();
@@ -475,19 +475,14 @@ void FockEngine::build_four_center(const double* d_D, hipStream_t s) {
       st.push_back(side_[i]);
     }
   }
-  // per-lane (non-slab) class kernels can need a large private segment;
-  // they stay on the build stream so that only its queue holds such scratch
+  // longest first onto the least-loaded stream (no class kernel needs a
+  // private segment, so any queue can take any of them)
   std::vector<double> load(st.size(), 0.0);
-  for (const Item& it : items)
-    if (it.kind == 0 && !it.qc->slab_ok[mode]) load[0] += it.cost;
   for (const Item& it : items) {
     QuartetClass* qc = it.qc;
     const bool slab = qc->slab_ok[mode];
-    size_t k = 0;
-    if (it.kind == 1 || slab) {
-      k = std::min_element(load.begin(), load.end()) - load.begin();
-      load[k] += it.cost;
-    }
+    const size_t k = std::min_element(load.begin(), load.end()) - load.begin();
+    load[k] += it.cost;
     hipStream_t q = st[k];
     const double ncomp = double(ncart(qc->LA) * ncart(qc->LB) * ncart(qc->LC) * ncart(qc->LD));
     // algorithmic traffic of the stored digestion: each surviving integral read once

@@ -39,11 +39,11 @@ __device__ __forceinline__ double dfact2(int n) {  // (n)!! for n >= -1
   return r;
 }
 
-// primitive norm of component (l,m,n), int2e.hpp:127-130
-__device__ double prim_norm(double a, int l, int m, int n) {
-  const int L = l + m + n;
-  return double(1 << L) / sqrt(dfact2(2 * l - 1) * dfact2(2 * m - 1) * dfact2(2 * n - 1)) *
-         0.71270547035499016035 * pow(a, (2.0 * L + 3.0) / 4.0);
+// primitive norm of component (l,m,n), int2e.hpp:127-130:
+// prim_norm_shape(l, m, n) * a^((2L+3)/4)
+__device__ __forceinline__ double prim_norm_shape(int l, int m, int n) {
+  return double(1 << (l + m + n)) / sqrt(dfact2(2 * l - 1) * dfact2(2 * m - 1) * dfact2(2 * n - 1)) *
+         0.71270547035499016035;
 }
 
 struct DPrim {  // == gansu_primitive_shell layout
@@ -56,61 +56,110 @@ struct DPrim {  // == gansu_primitive_shell layout
 // ---------------------------------------------------------------- Schwarz --
 // Q = sqrt(max_comp c_a^2 c_b^2 N_a^2 N_b^2 [ab|ab]) per primitive pair in the
 // reference's pair order; pair_b < 0 marks an auxiliary primitive (a|a).
-__global__ void k_schwarz_ref(const DPrim* __restrict__ prims, const int* __restrict__ pa,
-                              const int* __restrict__ pb, size_t n, const double* __restrict__ boys, int aux,
-                              double* __restrict__ Q) {
-  const size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (id >= n) return;
-  const DPrim a = prims[pa[id]];
-  const bool single = pb[id] < 0;
-  const DPrim b = single ? a : prims[pb[id]];
-  const int la = a.shell_type, lb = single ? 0 : b.shell_type;
-  const double ea = a.exponent, eb = single ? 0.0 : b.exponent;
-  const double p = ea + eb;
-  const double A[3] = {a.x, a.y, a.z}, B[3] = {b.x, b.y, b.z};
-  double PA[3], PB[3], r2 = 0.0;
+// [ab|ab] = K^2 I with K = exp(-ab/p |AB|^2); the kernel returns sqrt(I) K.
+// At P = Q the Hermite integrals are closed-form: R_{2i,2j,2k} =
+// (2i-1)!! (2j-1)!! (2k-1)!! (-2 alpha)^n F_n(0), n = i + j + k, with
+// F_n(0) = 1/(2n+1) and alpha = p/2, and R vanishes at odd indices.  So
+// [ab|ab] = pref sum_n (-p)^n / (2n+1) [g_x g_y g_z]_n with one polynomial
+// per direction, g[m] = (2m-1)!! sum_{t+tau=2m} (-1)^tau E_t E_tau.  Every
+// loop bound is a template constant, so the tables stay in registers.
+template <int LA, int LB>
+__device__ __forceinline__ double schwarz_max_t(double p, const double (&PA)[3], const double (&PB)[3], double pref, double ea,
+                                double eb, bool single, double ca2) {
+  constexpr int LAB = LA + LB;
+  // g[d][i][j][m] for every exponent pair (i, j) of direction d; entries
+  // with m > i + j stay unused (zero)
+  double g[3][LA + 1][LB + 1][LAB + 1];
+#pragma unroll
   for (int d = 0; d < 3; ++d) {
-    const double P = (ea * A[d] + eb * B[d]) / p;
-    PA[d] = P - A[d];
-    PB[d] = P - B[d];
-    r2 += (A[d] - B[d]) * (A[d] - B[d]);
-  }
-  const double K = single ? 1.0 : exp(-ea * eb / p * r2);
-  double E[3][4 * 4 * 7];
-  for (int d = 0; d < 3; ++d) hermite_e_rt<double>(la, lb, p, PA[d], PB[d], E[d]);
-  const int L = 2 * (la + lb);
-  double R[nherm(12)];
-  // [ab|ab]: q = p, PQ = 0, reduced exponent p/2
-  const double pref = kTwoPi25 / (p * p * sqrt(2.0 * p)) * K * K;
-  hermite_r_rt<double>(L, 0.5 * p, 0.0, 0.0, 0.0, pref, boys, R);
-  const int nt = la + lb + 1;
-  double best = 0.0;
-  const double ca2 = a.coefficient * a.coefficient * (single ? 1.0 : b.coefficient * b.coefficient);
-  for (int ia = 0; ia < ncart(la); ++ia)
-    for (int ib = 0; ib < ncart(lb); ++ib) {
-      int ax[3], bx[3];
-      for (int d = 0; d < 3; ++d) { ax[d] = cart_exp(la, ia, d, aux); bx[d] = cart_exp(lb, ib, d, aux); }
-      const double* Ex = E[0] + (ax[0] * (lb + 1) + bx[0]) * nt;
-      const double* Ey = E[1] + (ax[1] * (lb + 1) + bx[1]) * nt;
-      const double* Ez = E[2] + (ax[2] * (lb + 1) + bx[2]) * nt;
-      double s = 0.0;
-      for (int t = 0; t <= ax[0] + bx[0]; ++t)
-        for (int u = 0; u <= ax[1] + bx[1]; ++u)
-          for (int v = 0; v <= ax[2] + bx[2]; ++v) {
-            const double e1 = Ex[t] * Ey[u] * Ez[v];
-            for (int t2 = 0; t2 <= ax[0] + bx[0]; ++t2)
-              for (int u2 = 0; u2 <= ax[1] + bx[1]; ++u2)
-                for (int v2 = 0; v2 <= ax[2] + bx[2]; ++v2) {
-                  const double sg = ((t2 + u2 + v2) & 1) ? -1.0 : 1.0;
-                  s += e1 * sg * Ex[t2] * Ey[u2] * Ez[v2] * R[hidx(t + t2, u + u2, v + v2)];
-                }
+    double E[LA + 1][LB + 1][LAB + 1];
+    hermite_e_t<LA, LB, double>(p, PA[d], PB[d], E);
+#pragma unroll
+    for (int i = 0; i <= LA; ++i)
+#pragma unroll
+      for (int j = 0; j <= LB; ++j)
+#pragma unroll
+        for (int m = 0; m <= LAB; ++m) {
+          double s = 0.0;
+#pragma unroll
+          for (int t = 0; t <= i + j; ++t) {
+            const int tau = 2 * m - t;
+            if (tau < 0 || tau > i + j) continue;
+            s += (tau & 1) ? -E[i][j][t] * E[i][j][tau] : E[i][j][t] * E[i][j][tau];
           }
-      double nrm = prim_norm(ea, ax[0], ax[1], ax[2]);
-      if (!single) nrm *= prim_norm(eb, bx[0], bx[1], bx[2]);
-      const double val = nrm * nrm * ca2 * s;
+          g[d][i][j][m] = s * dfact2(2 * m - 1);
+        }
+  }
+  // the component-independent part of the primitive norms, once
+  const double pa_pow = pow(ea, (2.0 * LA + 3.0) / 4.0), pb_pow = single ? 1.0 : pow(eb, (2.0 * LB + 3.0) / 4.0);
+  double best = 0.0;
+#pragma unroll
+  for (int ia = 0; ia < ncart(LA); ++ia)
+#pragma unroll
+    for (int ib = 0; ib < ncart(LB); ++ib) {
+      const int ax = cart_exp(LA, ia, 0), ay = cart_exp(LA, ia, 1), az = cart_exp(LA, ia, 2);
+      const int bx = cart_exp(LB, ib, 0), by = cart_exp(LB, ib, 1), bz = cart_exp(LB, ib, 2);
+      double tot = 0.0, pw = 1.0;
+#pragma unroll
+      for (int n = 0; n <= LAB; ++n) {
+        double cn = 0.0;
+#pragma unroll
+        for (int mx = 0; mx <= ax + bx; ++mx)
+#pragma unroll
+          for (int my = 0; my <= ay + by; ++my) {
+            const int mz = n - mx - my;
+            if (mz < 0 || mz > az + bz) continue;
+            cn += g[0][ax][bx][mx] * g[1][ay][by][my] * g[2][az][bz][mz];
+          }
+        tot += cn * pw / double(2 * n + 1);
+        pw *= -p;
+      }
+      double nrm = prim_norm_shape(ax, ay, az) * pa_pow;
+      if (!single) nrm *= prim_norm_shape(bx, by, bz) * pb_pow;
+      const double val = nrm * nrm * ca2 * pref * tot;
       if (fabs(best) < fabs(val)) best = val;
     }
-  Q[id] = sqrt(best);
+  return best;
+}
+
+// The maximum over components does not depend on their order, so the
+// auxiliary f order needs no separate table here.
+__global__ __launch_bounds__(256) void k_schwarz_ref(const DPrim* __restrict__ prims, const int* __restrict__ pa,
+                              const int* __restrict__ pb, size_t n, double* __restrict__ Q) {
+  const size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= n) return;
+  const DPrim p0 = prims[pa[id]];
+  const bool single = pb[id] < 0;
+  const DPrim p1 = single ? p0 : prims[pb[id]];
+  const bool sw = !single && p1.shell_type > p0.shell_type;  // [ab|ab] = [ba|ba]
+  const int la = sw ? p1.shell_type : p0.shell_type, lb = single ? 0 : (sw ? p0.shell_type : p1.shell_type);
+  const double ea = sw ? p1.exponent : p0.exponent, eb = single ? 0.0 : (sw ? p0.exponent : p1.exponent);
+  const double ca = sw ? p1.coefficient : p0.coefficient, cb = sw ? p0.coefficient : p1.coefficient;
+  const double p = ea + eb;
+  double PA[3], PB[3], r2 = 0.0;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const double xa = d == 0 ? p0.x : d == 1 ? p0.y : p0.z, xb = d == 0 ? p1.x : d == 1 ? p1.y : p1.z;
+    const double A = sw ? xb : xa, B = sw ? xa : xb;
+    const double P = (ea * A + eb * B) / p;
+    PA[d] = P - A;
+    PB[d] = P - B;
+    r2 += (A - B) * (A - B);
+  }
+  // K^2 enters as a common factor after the square root: K^2 alone can be
+  // subnormal (|AB| large), K is not
+  const double K = single ? 1.0 : exp(-ea * eb / p * r2);
+  const double pref = kTwoPi25 / (p * p * sqrt(2.0 * p));
+  const double ca2 = ca * ca * (single ? 1.0 : cb * cb);
+  double best = 0.0;
+  switch (la * 4 + lb) {
+#define GANSU_SQ(x_, y_) \
+  case x_ * 4 + y_: best = schwarz_max_t<x_, y_>(p, PA, PB, pref, ea, eb, single, ca2); break;
+    GANSU_SQ(0, 0) GANSU_SQ(1, 0) GANSU_SQ(1, 1) GANSU_SQ(2, 0) GANSU_SQ(2, 1) GANSU_SQ(2, 2) GANSU_SQ(3, 0)
+#undef GANSU_SQ
+    default: best = __builtin_nan("");  // orbital shells are at most d, auxiliary ones at most f
+  }
+  Q[id] = sqrt(best) * K;
 }
 
 // --------------------------------------------------------- pair records ----
@@ -355,7 +404,6 @@ template <int LA, int LB, int LC, int LD, int MODE>
 __global__ __launch_bounds__(64) void k_eri_class(ClassLaunch c, EriArgs g) {
   constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
   constexpr int NAB = NA * NB, NCD = NC * ND, NCOMP = NAB * NCD;
-  constexpr int LAB = LA + LB, LCD = LC + LD, L = LAB + LCD;
   const BlockRec blk = c.blocks[blockIdx.x];
   const int lane = threadIdx.x;
   const bool active = lane < blk.count;
@@ -397,15 +445,19 @@ __global__ __launch_bounds__(64) void k_eri_class(ClassLaunch c, EriArgs g) {
 
 // --------------------------------------------------- cooperative kernel ----
 // The classes whose blocks exceed a lane's registers ((dp|pp) up to (dd|dd)):
-// one workgroup of NT threads (64, 128 or 256 by block size) per contracted
-// shell quartet.  Per primitive quartet, the Boys values (one lane), the
-// Hermite R_tuv (level by level, the threads over the entries of a level,
-// ping-pong buffers in LDS), the ket-contracted
-// X[cd][tuv] = sum E^cd (-1)^(tau+nu+phi) R[t+tau, u+nu, v+phi] and the bra
-// contraction are split over the threads; a thread owns block components
-// tid, tid + NT, ... in registers.  Nothing is held in private arrays, so the
-// kernel needs no scratch memory.  blockIdx.y splits a block's kets over
-// gridDim.y workgroups.
+// one workgroup of NT threads per contracted shell quartet.  Per group of up
+// to NBG bra and KG ket primitive pairs, the pairs' data and Hermite E tables
+// go to LDS once; the (bra, ket) primitive quartets that pass the screen are
+// then taken KC at a time:
+//   1. their Boys values, a thread each;
+//   2. their Hermite R_tuv, level by level of the downward recursion, the
+//      threads over (quartet, tuv) entries, ping-pong buffers in LDS;
+//   3. X_b[cd][tuv] += sum E^cd (-1)^(tau+nu+phi) R[t+tau, u+nu, v+phi] for
+//      the bra primitive b of each quartet, a thread per (cd, tuv) item.
+// The bra contraction sum_b sum_tuv E^ab_b,tuv X_b[cd][tuv] then runs once
+// per bra group, a thread per block component (tid, tid + NT, ...), into
+// registers.  Nothing is held in private arrays, so the kernel needs no
+// scratch memory.  blockIdx.y splits a block's kets over gridDim.y workgroups.
 template <int LA, int LB, int LC, int LD>
 struct CoopShape {
   static constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
@@ -413,9 +465,15 @@ struct CoopShape {
   static constexpr int LAB = LA + LB, LCD = LC + LD, L = LAB + LCD;
   static constexpr int NH = nherm(L), NHB = nherm(LAB);
   static constexpr int NEB = (LA + 1) * (LB + 1) * (LAB + 1), NEK = (LC + 1) * (LD + 1) * (LCD + 1);
-  static constexpr int NT = NCOMP > 640 ? 256 : NCOMP > 192 ? 128 : 64;
-  static constexpr int NOWN = (NCOMP + NT - 1) / NT;
-  static constexpr int NXO = NCD * NHB > NCOMP ? NCD * NHB : NCOMP;
+  static constexpr int NT = NCOMP > 640 ? 256 : 128;
+  static constexpr int NOWN = (NCOMP + NT - 1) / NT;             // block components per thread
+  static constexpr int NX = NCD * NHB;                            // X entries per bra primitive
+  static constexpr int NXI = (NX + NT - 1) / NT;                  // X items per thread
+  static constexpr int clampi(int v, int lo, int hi) { return v < lo ? lo : v > hi ? hi : v; }
+  static constexpr int KC = clampi(512 / NH, 2, 32);             // primitive quartets per chunk
+  static constexpr int NBG = clampi(768 / NX, 1, 4);             // bra primitive pairs per group
+  static constexpr int KG = clampi(256 / NEK, 4, 64);            // ket primitive pairs per group
+  static constexpr int NXO = NBG * NX > NCOMP ? NBG * NX : NCOMP;
 };
 
 // Packed Cartesian exponents (x | y << 2 | z << 4) of component c of an l <= 2 shell.
@@ -429,14 +487,18 @@ template <int LA, int LB, int LC, int LD, int MODE>
 __global__ __launch_bounds__((CoopShape<LA, LB, LC, LD>::NT)) void k_eri_coop(ClassLaunch c, EriArgs g) {
   using S = CoopShape<LA, LB, LC, LD>;
   constexpr int NA = S::NA, NB = S::NB, NC = S::NC, ND = S::ND, NAB = S::NAB, NCD = S::NCD;
-  constexpr int NCOMP = S::NCOMP, LAB = S::LAB, LCD = S::LCD, L = S::L;
-  constexpr int NH = S::NH, NHB = S::NHB, NT = S::NT, NOWN = S::NOWN;
-  __shared__ double sR[2][NH];        // R of two consecutive recursion levels
-  __shared__ double sXO[S::NXO];      // X[cd][tuv] while integrating, the block for digestion
-  __shared__ double sEb[3][S::NEB], sEk[3][S::NEK];
-  __shared__ double sF[L + 1];        // scaled Boys values
-  __shared__ int sTuv[NH];            // t | u << 4 | v << 8 of each level-ordered Hermite index
-  __shared__ int sAb[NAB], sCd[NCD];  // packed exponents of a, c (bits 0-5) and b, d (bits 6-11)
+  constexpr int NCOMP = S::NCOMP, LAB = S::LAB, LCD = S::LCD, L = S::L, NEB = S::NEB, NEK = S::NEK;
+  constexpr int NH = S::NH, NHB = S::NHB, NT = S::NT, NOWN = S::NOWN, NX = S::NX, NXI = S::NXI;
+  constexpr int KC = S::KC, NBG = S::NBG, KG = S::KG;
+  __shared__ double sR[2][KC * NH];       // R of two consecutive recursion levels, per quartet
+  __shared__ double sXO[S::NXO];          // X_b[cd][tuv] of the bra group, then the block for digestion
+  __shared__ double sEb[NBG][3][NEB], sEk[KG][3][NEK];
+  __shared__ double sBp[NBG][6], sKp[KG][6];  // p, P[3], Kc, Q of the group's primitive pairs
+  __shared__ double sF[KC][L + 1];        // scaled Boys values
+  __shared__ double sPQ[KC][3];           // P - Q
+  __shared__ int sPair[KC], sN;           // surviving quartets of the chunk: bra << 8 | ket
+  __shared__ int sTuv[NH];                // t | u << 4 | v << 8 of each level-ordered Hermite index
+  __shared__ int sAb[NAB], sCd[NCD];      // packed exponents of a, c (bits 0-5) and b, d (bits 6-11)
   const int tid = threadIdx.x;
   for (int h = tid; h < NH; h += NT) {
     int n = 0;
@@ -470,103 +532,169 @@ __global__ __launch_bounds__((CoopShape<LA, LB, LC, LD>::NT)) void k_eri_coop(Cl
           acc[j] = f32 ? (double)c.store32[slot_at + (size_t)k * 64 + q] : c.store[slot_at + (size_t)k * 64 + q];
       }
     } else {
-      for (int ib = 0; ib < bra.nprim; ++ib) {
-        const PairRec* pb = g.pr + bra.prim_begin + ib;
-        if (tid < 3) hermite_e_rt<double>(LA, LB, pb->p, pb->PA[tid], pb->PB[tid], sEb[tid]);
-        for (int ik = 0; ik < ket.nprim; ++ik) {
-          const PairRec* pk = g.pr + ket.prim_begin + ik;
-          if (pb->Q * pk->Q < g.thr) continue;  // the same for every thread
-          const double p = pb->p, qe = pk->p;
-          const double alpha = p * qe / (p + qe);
-          const double X = pb->P[0] - pk->P[0], Y = pb->P[1] - pk->P[1], Z = pb->P[2] - pk->P[2];
-          if (tid >= 3 && tid < 6) {
-            hermite_e_rt<double>(LC, LD, qe, pk->PA[tid - 3], pk->PB[tid - 3], sEk[tid - 3]);
-          } else if (tid == 6) {
-            double F[L + 1];
-            boys_eval_t<L, double>(alpha * (X * X + Y * Y + Z * Z), g.boys, F);
-            double sc = kTwoPi25 / (p * qe * sqrt(p + qe)) * pb->Kc * pk->Kc;
-            const double m2a = -2.0 * alpha;
-#pragma unroll
-            for (int n = 0; n <= L; ++n) {
-              sF[n] = F[n] * sc;
-              sc *= m2a;
+      for (int b0 = 0; b0 < bra.nprim; b0 += NBG) {
+        const int nbl = min(NBG, bra.nprim - b0);
+        for (int i = tid; i < nbl * NX; i += NT) sXO[i] = 0.0;
+        bool any = false;
+        for (int k0 = 0; k0 < ket.nprim; k0 += KG) {
+          const int kn = min(KG, ket.nprim - k0);
+          // primitive data and E tables of the group (the bra's only on its first ket group)
+          const int nbi = k0 == 0 ? nbl : 0;
+          for (int it = tid; it < 4 * (nbi + kn); it += NT) {
+            const bool isb = it < 4 * nbi;
+            const int i = isb ? it : it - 4 * nbi, n = isb ? nbi : kn;
+            const int e = i / n, r = i - e * n;
+            const PairRec* pp = g.pr + (isb ? bra.prim_begin + b0 + r : ket.prim_begin + k0 + r);
+            if (e == 3) {
+              double* dst = isb ? sBp[r] : sKp[r];
+              dst[0] = pp->p;
+              dst[1] = pp->P[0];
+              dst[2] = pp->P[1];
+              dst[3] = pp->P[2];
+              dst[4] = pp->Kc;
+              dst[5] = pp->Q;
+            } else if (isb) {
+              hermite_e_rt<double>(LA, LB, pp->p, pp->PA[e], pp->PB[e], sEb[r][e]);
+            } else {
+              hermite_e_rt<double>(LC, LD, pp->p, pp->PA[e], pp->PB[e], sEk[r][e]);
             }
           }
           __syncthreads();
-          // level n of the recursion holds R^(n)_tuv for t + u + v <= L - n
-#pragma unroll
-          for (int n = L - 1; n >= 0; --n) {
-            const double* src = sR[(n + 1) & 1];
-            double* dst = sR[n & 1];
-            for (int h = tid; h < nherm(L - n); h += NT) {
-              double val;
-              if (h == 0) {
-                val = sF[n];
-              } else if (n == L - 1) {  // level L holds only R^(L)_000 = F_L
-                const int w = sTuv[h];
-                val = ((w & 15) ? X : ((w >> 4) & 15) ? Y : Z) * sF[L];
-              } else {
-                const int w = sTuv[h], t = w & 15, u = (w >> 4) & 15, v = w >> 8;
-                if (t > 0) {
-                  val = X * src[hidx(t - 1, u, v)];
-                  if (t > 1) val += double(t - 1) * src[hidx(t - 2, u, v)];
-                } else if (u > 0) {
-                  val = Y * src[hidx(t, u - 1, v)];
-                  if (u > 1) val += double(u - 1) * src[hidx(t, u - 2, v)];
-                } else {
-                  val = Z * src[hidx(t, u, v - 1)];
-                  if (v > 1) val += double(v - 1) * src[hidx(t, u, v - 2)];
-                }
-              }
-              dst[h] = val;
+          const int npair = nbl * kn;
+          for (int p0 = 0; p0 < npair; p0 += KC) {
+            if (tid < 64) {  // wave 0 compacts the surviving quartets
+              const int pr = p0 + tid, ib = pr / kn, ik = pr - ib * kn;
+              const bool keep = tid < KC && pr < npair && sBp[ib][5] * sKp[ik][5] >= g.thr;
+              const unsigned long long m = __ballot(keep);
+              if (keep) sPair[__popcll(m & ((1ull << tid) - 1ull))] = ib << 8 | ik;
+              if (tid == 0) sN = (int)__popcll(m);
             }
             __syncthreads();
-          }
-          const double* R = sR[0];
-          for (int it = tid; it < NCD * NHB; it += NT) {
-            const int cd = it / NHB, h = it % NHB;
-            const int w = sTuv[h], t = w & 15, u = (w >> 4) & 15, v = w >> 8;
-            const int e = sCd[cd];
-            const int cx = e & 3, cy = (e >> 2) & 3, cz = (e >> 4) & 3;
-            const int dx = (e >> 6) & 3, dy = (e >> 8) & 3, dz = (e >> 10) & 3;
-            const double* ex = &sEk[0][(cx * (LD + 1) + dx) * (LCD + 1)];
-            const double* ey = &sEk[1][(cy * (LD + 1) + dy) * (LCD + 1)];
-            const double* ez = &sEk[2][(cz * (LD + 1) + dz) * (LCD + 1)];
-            double s = 0.0;
-            for (int tt = 0; tt <= cx + dx; ++tt)
-              for (int uu = 0; uu <= cy + dy; ++uu) {
-                const double exy = ((tt + uu) & 1) ? -ex[tt] * ey[uu] : ex[tt] * ey[uu];
-                for (int vv = 0; vv <= cz + dz; ++vv) {
-                  const double e3 = (vv & 1) ? -exy * ez[vv] : exy * ez[vv];
-                  s += e3 * R[hidx(t + tt, u + uu, v + vv)];
-                }
-              }
-            sXO[cd * NHB + h] = s;
-          }
-          __syncthreads();
+            const int nkc = sN;
+            __syncthreads();  // sN is rewritten by the next chunk
+            if (nkc == 0) continue;
+            any = true;
+            // 1. Boys values
+            if (tid < nkc) {
+              const int w = sPair[tid];
+              const double* B = sBp[w >> 8];
+              const double* K = sKp[w & 255];
+              const double p = B[0], qe = K[0];
+              const double alpha = p * qe / (p + qe);
+              const double X = B[1] - K[1], Y = B[2] - K[2], Z = B[3] - K[3];
+              double F[L + 1];
+              boys_eval_t<L, double>(alpha * (X * X + Y * Y + Z * Z), g.boys, F);
+              double sc = kTwoPi25 / (p * qe * sqrt(p + qe)) * B[4] * K[4];
+              const double m2a = -2.0 * alpha;
 #pragma unroll
-          for (int j = 0; j < NOWN; ++j) {
-            const int k = tid + NT * j;
-            if (k < NCOMP) {
-              const int ab = k / NCD;
-              const int e = sAb[ab];
-              const int ax = e & 3, ay = (e >> 2) & 3, az = (e >> 4) & 3;
-              const int bx = (e >> 6) & 3, by = (e >> 8) & 3, bz = (e >> 10) & 3;
-              const double* ex = &sEb[0][(ax * (LB + 1) + bx) * (LAB + 1)];
-              const double* ey = &sEb[1][(ay * (LB + 1) + by) * (LAB + 1)];
-              const double* ez = &sEb[2][(az * (LB + 1) + bz) * (LAB + 1)];
-              const double* xr = &sXO[(k - ab * NCD) * NHB];
-              double s = 0.0;
+              for (int n = 0; n <= L; ++n) {
+                sF[tid][n] = F[n] * sc;
+                sc *= m2a;
+              }
+              sPQ[tid][0] = X;
+              sPQ[tid][1] = Y;
+              sPQ[tid][2] = Z;
+            }
+            __syncthreads();
+            // 2. R^(n) for n = L-1 .. 0; level n holds t + u + v <= L - n
+#pragma unroll
+            for (int n = L - 1; n >= 0; --n) {
+              const int nh = nherm(L - n);
+              const double* src = sR[(n + 1) & 1];
+              double* dst = sR[n & 1];
+              for (int it = tid; it < nkc * nh; it += NT) {
+                const int t = it / nh, h = it - t * nh;
+                double val;
+                if (h == 0) {
+                  val = sF[t][n];
+                } else {
+                  const int w = sTuv[h], tx = w & 15, u = (w >> 4) & 15, v = w >> 8;
+                  const double* r = src + t * NH;
+                  if (n == L - 1) {  // R^(L) holds only R_000 = F_L
+                    val = sPQ[t][tx ? 0 : u ? 1 : 2] * sF[t][L];
+                  } else if (tx > 0) {
+                    val = sPQ[t][0] * r[hidx(tx - 1, u, v)];
+                    if (tx > 1) val += double(tx - 1) * r[hidx(tx - 2, u, v)];
+                  } else if (u > 0) {
+                    val = sPQ[t][1] * r[hidx(tx, u - 1, v)];
+                    if (u > 1) val += double(u - 1) * r[hidx(tx, u - 2, v)];
+                  } else {
+                    val = sPQ[t][2] * r[hidx(tx, u, v - 1)];
+                    if (v > 1) val += double(v - 1) * r[hidx(tx, u, v - 2)];
+                  }
+                }
+                dst[t * NH + h] = val;
+              }
+              __syncthreads();
+            }
+            // 3. X_b[cd][tuv]; the chunk's quartets are in bra-major order
+#pragma unroll
+            for (int j = 0; j < NXI; ++j) {
+              const int it = tid + NT * j;
+              if (it < NX) {
+                const int cd = it / NHB, h = it - cd * NHB;
+                const int w = sTuv[h], t = w & 15, u = (w >> 4) & 15, v = w >> 8;
+                const int e = sCd[cd];
+                const int cx = e & 3, cy = (e >> 2) & 3, cz = (e >> 4) & 3;
+                const int dx = (e >> 6) & 3, dy = (e >> 8) & 3, dz = (e >> 10) & 3;
+                const int ox = (cx * (LD + 1) + dx) * (LCD + 1), oy = (cy * (LD + 1) + dy) * (LCD + 1),
+                          oz = (cz * (LD + 1) + dz) * (LCD + 1);
+                double s = 0.0;
+                int cur = sPair[0] >> 8;
+                for (int k = 0; k < nkc; ++k) {
+                  const int pw = sPair[k], ib = pw >> 8, ik = pw & 255;
+                  if (ib != cur) {
+                    sXO[cur * NX + it] += s;
+                    s = 0.0;
+                    cur = ib;
+                  }
+                  const double* ex = &sEk[ik][0][ox];
+                  const double* ey = &sEk[ik][1][oy];
+                  const double* ez = &sEk[ik][2][oz];
+                  const double* R = sR[0] + k * NH;
+                  for (int tt = 0; tt <= cx + dx; ++tt)
+                    for (int uu = 0; uu <= cy + dy; ++uu) {
+                      const double exy = ((tt + uu) & 1) ? -ex[tt] * ey[uu] : ex[tt] * ey[uu];
+                      for (int vv = 0; vv <= cz + dz; ++vv) {
+                        const double e3 = (vv & 1) ? -exy * ez[vv] : exy * ez[vv];
+                        s += e3 * R[hidx(t + tt, u + uu, v + vv)];
+                      }
+                    }
+                }
+                sXO[cur * NX + it] += s;
+              }
+            }
+            __syncthreads();  // the next chunk rewrites sPair, sF, sPQ and sR
+          }
+        }
+        if (!any) continue;  // the same for every thread
+        // bra contraction of the group
+#pragma unroll
+        for (int j = 0; j < NOWN; ++j) {
+          const int k = tid + NT * j;
+          if (k < NCOMP) {
+            const int ab = k / NCD;
+            const int e = sAb[ab];
+            const int ax = e & 3, ay = (e >> 2) & 3, az = (e >> 4) & 3;
+            const int bx = (e >> 6) & 3, by = (e >> 8) & 3, bz = (e >> 10) & 3;
+            const int ox = (ax * (LB + 1) + bx) * (LAB + 1), oy = (ay * (LB + 1) + by) * (LAB + 1),
+                      oz = (az * (LB + 1) + bz) * (LAB + 1);
+            double s = 0.0;
+            for (int ib = 0; ib < nbl; ++ib) {
+              const double* ex = &sEb[ib][0][ox];
+              const double* ey = &sEb[ib][1][oy];
+              const double* ez = &sEb[ib][2][oz];
+              const double* xr = &sXO[ib * NX + (k - ab * NCD) * NHB];
               for (int t = 0; t <= ax + bx; ++t)
                 for (int u = 0; u <= ay + by; ++u) {
                   const double exy = ex[t] * ey[u];
                   for (int v = 0; v <= az + bz; ++v) s += exy * ez[v] * xr[hidx(t, u, v)];
                 }
-              acc[j] += s;
             }
+            acc[j] += s;
           }
-          __syncthreads();  // the next primitive quartet rewrites sF, sR, sEk and sXO
         }
+        __syncthreads();  // the next bra group rewrites sEb, sBp and sXO
       }
     }
     if constexpr (MODE == kModeStore) {
@@ -721,6 +849,16 @@ hipError_t launch_t(const ClassLaunch& c, const EriArgs& a, int mode, hipStream_
   return hipGetLastError();
 }
 
+
+// Classes whose direct (integral-recomputing) slab tile fits a lane's
+// registers: the integral block, R and both E tables.  The d-rich rest,
+// (dd|ss) included (its d-d bra tables alone are 135 values), run the
+// cooperative kernel.
+constexpr bool slab_direct_class(int LA, int LB, int LC, int LD) {
+  const int L = LA + LB + LC + LD;
+  const int ncomp = ncart(LA) * ncart(LB) * ncart(LC) * ncart(LD);
+  return L <= 4 && ncomp <= 81 && !(LA == 2 && LB == 2);
+}
 
 // --------------------------------------------------- slab digestion kernel --
 // A 256-thread workgroup owns BPW consecutive bra shell pairs (i j) of one
@@ -1019,10 +1157,15 @@ template <int LA, int LB, int LC, int LD>
 hipError_t launch_slab_t(const SlabLaunch& c, const EriArgs& a, int mode, size_t max_lds, hipStream_t s) {
   if (c.nwork == 0) return hipSuccess;
   if (mode == kModeDirect) {
-    if (c.bpw < 1 || c.bpw > SlabShape<LA, LB, LC, LD, kModeDirect>::BPW) return hipErrorInvalidValue;
-    const size_t lds = slab_lds_t<LA, LB, LC, LD, kModeDirect>(a.M, c.bpw);
-    if (lds > max_lds) return hipErrorInvalidValue;
-    hipLaunchKernelGGL((k_fock_slab<LA, LB, LC, LD, kModeDirect>), dim3(c.nwork), dim3(kSlabThreads), lds, s, c, a);
+    if constexpr (slab_direct_class(LA, LB, LC, LD)) {
+      if (c.bpw < 1 || c.bpw > SlabShape<LA, LB, LC, LD, kModeDirect>::BPW) return hipErrorInvalidValue;
+      const size_t lds = slab_lds_t<LA, LB, LC, LD, kModeDirect>(a.M, c.bpw);
+      if (lds > max_lds) return hipErrorInvalidValue;
+      hipLaunchKernelGGL((k_fock_slab<LA, LB, LC, LD, kModeDirect>), dim3(c.nwork), dim3(kSlabThreads), lds, s, c,
+                         a);
+    } else {
+      return hipErrorInvalidValue;
+    }
   } else {
     if (c.bpw < 1 || c.bpw > SlabShape<LA, LB, LC, LD, kModeStoredDigest>::BPW) return hipErrorInvalidValue;
     const size_t lds = slab_lds_t<LA, LB, LC, LD, kModeStoredDigest>(a.M, c.bpw);
@@ -1104,12 +1247,11 @@ size_t g_max_lds = 0;
 }  // namespace
 
 // --------------------------------------------------------------- launchers --
-hipError_t launch_schwarz_ref(const void* prims, const int* pair_a, const int* pair_b, size_t n, const double* boys,
-                              int aux_order, double* Q, hipStream_t s) {
+hipError_t launch_schwarz_ref(const void* prims, const int* pair_a, const int* pair_b, size_t n, double* Q,
+                              hipStream_t s) {
   if (n == 0) return hipSuccess;
   const unsigned nb = (unsigned)((n + 255) / 256);
-  hipLaunchKernelGGL(k_schwarz_ref, dim3(nb), dim3(256), 0, s, (const DPrim*)prims, pair_a, pair_b, n, boys, aux_order,
-                     Q);
+  hipLaunchKernelGGL(k_schwarz_ref, dim3(nb), dim3(256), 0, s, (const DPrim*)prims, pair_a, pair_b, n, Q);
   return hipGetLastError();
 }
 
@@ -1158,7 +1300,7 @@ hipError_t launch_eri_class(const ClassLaunch& c, const EriArgs& a, int mode, hi
     GANSU_CASE(2, 0, 2, 0)
     GANSU_CASE(2, 1, 0, 0)
     GANSU_CASE(2, 1, 1, 0)
-    GANSU_CASE(2, 2, 0, 0)
+    GANSU_COOP(2, 2, 0, 0)
     GANSU_COOP(2, 1, 1, 1)
     GANSU_COOP(2, 1, 2, 0)
     GANSU_COOP(2, 1, 2, 1)
@@ -1206,9 +1348,7 @@ hipError_t launch_jket(const SlabLaunch& c, const EriArgs& a, hipStream_t s) {
 // from stored blocks.
 bool slab_available(int LA, int LB, int LC, int LD, int mode) {
   if (mode == kModeStoredDigest) return true;
-  const int L = LA + LB + LC + LD;
-  const int ncomp = ncart(LA) * ncart(LB) * ncart(LC) * ncart(LD);
-  return mode == kModeDirect && L <= 4 && ncomp <= 81;
+  return mode == kModeDirect && slab_direct_class(LA, LB, LC, LD);
 }
 
 // LDS bytes of the slab kernel of a class at basis size M (0: no variant).
@@ -1256,7 +1396,6 @@ hipError_t launch_fock_slab(const SlabLaunch& c, const EriArgs& a, int mode, hip
       GANSU_SLAB(0, 0, 0, 0) GANSU_SLAB(1, 0, 0, 0) GANSU_SLAB(1, 0, 1, 0) GANSU_SLAB(1, 1, 0, 0)
       GANSU_SLAB(1, 1, 1, 0) GANSU_SLAB(1, 1, 1, 1) GANSU_SLAB(2, 0, 0, 0) GANSU_SLAB(2, 0, 1, 0)
       GANSU_SLAB(2, 0, 1, 1) GANSU_SLAB(2, 0, 2, 0) GANSU_SLAB(2, 1, 0, 0) GANSU_SLAB(2, 1, 1, 0)
-      GANSU_SLAB(2, 2, 0, 0)
       default: return hipErrorInvalidValue;
     }
   }

@@ -94,8 +94,9 @@ struct EriArgs {
 };
 
 // Host-callable launchers (fock_kernels.hip).
-hipError_t launch_schwarz_ref(const void* prims, const int* pair_a, const int* pair_b, size_t n, const double* boys,
-                              int aux_order, double* Q, hipStream_t s);
+// Shell types up to f (3); a pair_b entry < 0 marks an auxiliary primitive (a|a).
+hipError_t launch_schwarz_ref(const void* prims, const int* pair_a, const int* pair_b, size_t n, double* Q,
+                              hipStream_t s);
 hipError_t launch_pair_data(const void* prims, const int* pa, const int* pb, const double* Qg, size_t n, PairRec* out,
                             hipStream_t s);
 hipError_t launch_eri_class(const ClassLaunch& c, const EriArgs& a, int mode, hipStream_t s);

@@ -121,7 +121,7 @@ void FockEngine::precompute_ri() {
     DevBuf<double> dq(aux_prims_.size());
     dpa.upload(pa);
     dpb.upload(pb);
-    GANSU_HIP(launch_schwarz_ref(dprims.ptr, dpa.ptr, dpb.ptr, pa.size(), boys_.ptr, 1, dq.ptr, 0));
+    GANSU_HIP(launch_schwarz_ref(dprims.ptr, dpa.ptr, dpb.ptr, pa.size(), dq.ptr, 0));
     GANSU_HIP(hipDeviceSynchronize());
     qaux = dq.download();
   }

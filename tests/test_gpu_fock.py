@@ -42,7 +42,15 @@ def test_schwarz_factors_and_masks(name, cuda):
     q_gpu = eng.schwarz_factors()
     q_ref = O.schwarz(p, t)
     assert q_gpu.shape == q_ref.shape
-    np.testing.assert_allclose(q_gpu, q_ref, rtol=1e-13, atol=1e-300)
+    # Q = sqrt(K^2 I): where Q^2 is below the smallest normal double, the
+    # oracle's products (K^2, pref K^2, ...) pass through the subnormal range
+    # and keep only a few significant digits (measured: 1e-4 relative at
+    # Q = 1e-161); the kernel takes K out of the root.  Those pairs lie more
+    # than 140 orders of magnitude below the screening threshold, so there
+    # both must merely vanish at that scale
+    normal = q_ref ** 2 >= np.finfo(np.float64).tiny
+    np.testing.assert_allclose(q_gpu[normal], q_ref[normal], rtol=1e-13, atol=1e-300)
+    assert np.all(np.abs(q_gpu[~normal] - q_ref[~normal]) <= 1e-150)
     # screening mask Q_x Q_y >= thr: bit-exact outside a 1e-12 relative band
     prod_g = np.outer(q_gpu, q_gpu)
     prod_r = np.outer(q_ref, q_ref)
