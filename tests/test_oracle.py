@@ -337,9 +337,15 @@ def test_uhf_oracle_known_answers():
         assert abs(O.uhf(pr, t, n, Z, origin, na, nb)[1] - ref) < 2e-6, Zv
     Z, xyz = gm.water(jitter=0.01, seed=11)
     pr, t, n = O.build_molecule(p, Z, xyz)
-    # from the symmetry-broken guess the closed shell returns to RHF; converged
-    # to 1e-12 so that the two stop points agree within 1e-10
-    assert abs(O.uhf(pr, t, n, Z, xyz, 5, 5, conv=1e-12)[1] - O.rhf(pr, t, n, Z, xyz, 5, conv=1e-12)[1]) < 1e-10
+    # from the symmetry-broken guess the closed shell returns to RHF along a
+    # soft spin mode: the per-iteration energy change falls below the stop
+    # threshold while the energy still lies 1e-11..5e-10 above the RHF one
+    # (the stop point moves with the BLAS summation order), so the UHF energy
+    # is bounded from both sides and the spin densities must have merged
+    _, e_u, Da, Db, _ = O.uhf(pr, t, n, Z, xyz, 5, 5, conv=1e-12)
+    e_r = O.rhf(pr, t, n, Z, xyz, 5, conv=1e-12)[1]
+    assert -1e-11 < e_u - e_r < 2e-9
+    assert np.abs(Da - Db).max() < 1e-4
 
 
 # ------------------------------------------------------------------ ROHF ----
