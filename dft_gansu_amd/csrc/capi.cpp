@@ -275,7 +275,7 @@ int gansu_rhf_run(const gansu_molecule* mol, const gansu_molecule* aux_mol, cons
     energies[0] = r.e_elec;
     energies[1] = r.e_total;
     if (D_out) std::memcpy(D_out, r.D.data(), r.D.size() * sizeof(double));
-    if (info) { info[0] = r.iterations; info[1] = r.fock_builds; info[2] = r.converged; info[3] = 0; }
+    if (info) { info[0] = r.iterations; info[1] = r.fock_builds; info[2] = r.converged; info[3] = r.graph_launches; }
   });
 }
 
@@ -290,7 +290,7 @@ int gansu_rhf_gradient_guess(const gansu_molecule* mol, const gansu_molecule* au
     energies[1] = r.e_total;
     std::memcpy(grad, r.grad.data(), r.grad.size() * sizeof(double));
     if (D_out) std::memcpy(D_out, r.D.data(), r.D.size() * sizeof(double));
-    if (info) { info[0] = r.iterations; info[1] = r.fock_builds; info[2] = r.converged; info[3] = 0; }
+    if (info) { info[0] = r.iterations; info[1] = r.fock_builds; info[2] = r.converged; info[3] = r.graph_launches; }
   });
 }
 
@@ -304,7 +304,7 @@ int gansu_rhf_run_guess(const gansu_molecule* mol, const gansu_molecule* aux_mol
     energies[0] = r.e_elec;
     energies[1] = r.e_total;
     if (D_out) std::memcpy(D_out, r.D.data(), r.D.size() * sizeof(double));
-    if (info) { info[0] = r.iterations; info[1] = r.fock_builds; info[2] = r.converged; info[3] = 0; }
+    if (info) { info[0] = r.iterations; info[1] = r.fock_builds; info[2] = r.converged; info[3] = r.graph_launches; }
   });
 }
 
@@ -350,7 +350,7 @@ int gansu_rhf_run_mo(const gansu_molecule* mol, const gansu_molecule* aux_mol, c
     energies[1] = r.e_total;
     if (C_out) std::memcpy(C_out, r.C.data(), r.C.size() * sizeof(double));
     if (eps_out) std::memcpy(eps_out, r.eps.data(), r.eps.size() * sizeof(double));
-    if (info) { info[0] = r.iterations; info[1] = r.fock_builds; info[2] = r.converged; info[3] = 0; }
+    if (info) { info[0] = r.iterations; info[1] = r.fock_builds; info[2] = r.converged; info[3] = r.graph_launches; }
   });
 }
 
@@ -441,7 +441,7 @@ int gansu_rhf_gradient(const gansu_molecule* mol, const gansu_molecule* aux_mol,
     energies[0] = r.e_elec;
     energies[1] = r.e_total;
     std::memcpy(grad, r.grad.data(), r.grad.size() * sizeof(double));
-    if (info) { info[0] = r.iterations; info[1] = r.fock_builds; info[2] = r.converged; info[3] = 0; }
+    if (info) { info[0] = r.iterations; info[1] = r.fock_builds; info[2] = r.converged; info[3] = r.graph_launches; }
   });
 }
 

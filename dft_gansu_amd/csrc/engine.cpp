@@ -531,6 +531,14 @@ void FockEngine::build_jk(const double* d_D, double* d_J, double* d_K, const dou
   GANSU_HIP(hipGraphLaunch(graph_for(GraphKey{d_D, d_J, d_K, d_H, d_F}), s));
 }
 
+void FockEngine::record_build(const double* d_D, const double* d_H, double* d_F, hipStream_t s) {
+  if (!precomputed_) throw Error(GANSU_E_STATE, "build before precompute");
+  if (method_ == "ri") throw Error(GANSU_E_UNSUPPORTED, "a captured build needs eri_method stored or direct");
+  if (!d_D || !d_H || !d_F) throw Error(GANSU_E_ARG, "null device pointer");
+  build_four_center(d_D, s);
+  GANSU_HIP(launch_finalize(d_Jr_.ptr, d_Kr_.ptr, d_w_.ptr, M_, d_H, nullptr, nullptr, d_F, s));
+}
+
 void FockEngine::graph_release() {
   for (auto& e : gcache_) {
     if (e.exec) (void)hipGraphExecDestroy(e.exec);

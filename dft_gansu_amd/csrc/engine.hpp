@@ -113,6 +113,10 @@ class FockEngine {
   // d_C/nocc (optional): occupied orbitals, D = 2 C_occ C_occ^T
   void build_jk(const double* d_D, double* d_J, double* d_K, const double* d_H, double* d_F, hipStream_t s,
                 const double* d_C = nullptr, int nocc = 0);
+  // Four-centre RHF build F = H + J - K/2 launched on s without the engine's
+  // own graph, so that a caller capturing s records it into its own graph
+  // (stored and direct engines; RI builds call rocBLAS and are refused).
+  void record_build(const double* d_D, const double* d_H, double* d_F, hipStream_t s);
   // UHF (gpu_kernels.cu:289-340): F_a = H + J[D_a + D_b] - K[D_a], F_b likewise.
   void build_uhf(const double* d_Da, const double* d_Db, const double* d_H, double* d_Fa, double* d_Fb, hipStream_t s);
   // ROHF (gpu_manager.cu:733-817): F_c, F_o from J/K of the closed and open

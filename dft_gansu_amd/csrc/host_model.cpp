@@ -380,7 +380,8 @@ Params::Params() {
         {"precision", "fp64"},
         {"fp32_bound_threshold", "1e-6"},
         {"normalization", "reference"},
-        {"ri_k", "density"}};
+        {"ri_k", "density"},
+        {"scf_solver", "diagonalize"}};
 }
 
 static std::string lower(std::string s) {

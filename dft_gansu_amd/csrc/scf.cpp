@@ -22,6 +22,7 @@
 
 #include "gradient_kernels.hpp"
 #include "scf.hpp"
+#include "scf_graph.hpp"
 
 namespace gansu {
 
@@ -238,6 +239,138 @@ struct Damper {
   }
 };
 
+// The RHF iteration of the loop below (hf.cu:278-307) as one captured HIP
+// graph: X^T F X, SP2 purification instead of the eigensolver, D = 2 X P X^T,
+// the engine's four-centre build, E = 1/2 D.(H + F) and the convergence
+// test, then (unless converged) the DIIS error F D S - S D F and the DIIS
+// step, all reading their control words from a device state.  The host
+// launches the graph once per iteration and reads the state back (40
+// bytes) to decide whether to go on.  The purification's step count is
+// captured with a margin over what the first iteration needed; an
+// iteration that runs out of steps doubles it (re-captured), and a
+// convergence reached on such an iteration is not accepted.
+struct GraphLoop {
+  double E = 0.0, seconds = 0.0;
+  int iterations = 0, converged = 0, launches = 0;
+};
+
+GraphLoop graph_loop(FockEngine& eng, Work& w, int nocc, int maxiter, double conv, int diis_size, bool diis_x,
+                     bool timing, bool trace) {
+  const int n = w.n;
+  const size_t nn = (size_t)n * n;
+  DevBuf<double> Fo, A, e, errs(nn * diis_size), focks(nn * diis_size), dpart((size_t)kDiisBlocks * diis_size);
+  for (auto* b : {&Fo, &A, &e}) b->alloc(nn);
+  DevBuf<double> P0(nn), P1(nn), part(sp2_part_doubles(n)), rows(2 * (size_t)n), epart(kEnergyBlocks);
+  DevBuf<ScfDevState> st(1);
+  ScfDevState h{};
+  h.Eprev = std::numeric_limits<double>::max();
+  GANSU_HIP(hipMemcpy(st.ptr, &h, sizeof h, hipMemcpyHostToDevice));
+  const int* conv_flag = &st.ptr->converged;
+
+  hipStream_t cs = nullptr;
+  GANSU_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  DevBuf<double> eps;
+  DevBuf<int> done;
+  auto release = [&]() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+  };
+  auto capture = [&](int steps) {
+    release();
+    eps.alloc(steps + 1);
+    done.alloc(steps + 1);
+    const Sp2Work sw{{P0.ptr, P1.ptr}, A.ptr, rows.ptr, part.ptr, eps.ptr, done.ptr, sp2_workgroups(n)};
+    const double* P = A.ptr;
+    GANSU_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+    try {
+      GANSU_HIP(launch_gemm64(true, false, n, 1.0, w.X.ptr, w.F.ptr, 0.0, w.tmp.ptr, nullptr, cs));
+      GANSU_HIP(launch_gemm64(false, false, n, 1.0, w.tmp.ptr, w.X.ptr, 0.0, Fo.ptr, nullptr, cs));
+      GANSU_HIP(launch_sp2(Fo.ptr, n, nocc, steps, sw, st.ptr, cs));
+      GANSU_HIP(launch_gemm64(false, false, n, 1.0, w.X.ptr, P, 0.0, w.tmp.ptr, nullptr, cs));
+      GANSU_HIP(launch_gemm64(false, true, n, 2.0, w.tmp.ptr, w.X.ptr, 0.0, w.D.ptr, nullptr, cs));
+      eng.record_build(w.D.ptr, w.H.ptr, w.F.ptr, cs);
+      GANSU_HIP(launch_scf_energy(w.D.ptr, w.H.ptr, w.F.ptr, nn, conv, epart.ptr, st.ptr, cs));
+      // e = F D S - S D F = A - A^T with A = F D S (F, D, S symmetric)
+      GANSU_HIP(launch_gemm64(false, false, n, 1.0, w.F.ptr, w.D.ptr, 0.0, w.tmp.ptr, conv_flag, cs));
+      GANSU_HIP(launch_gemm64(false, false, n, 1.0, w.tmp.ptr, w.S.ptr, 0.0, A.ptr, conv_flag, cs));
+      GANSU_HIP(launch_antisym(A.ptr, n, e.ptr, st.ptr, cs));
+      if (diis_x) {
+        GANSU_HIP(launch_gemm64(false, false, n, 1.0, w.X.ptr, e.ptr, 0.0, w.tmp.ptr, conv_flag, cs));
+        GANSU_HIP(launch_gemm64(false, true, n, 1.0, w.tmp.ptr, w.X.ptr, 0.0, e.ptr, conv_flag, cs));
+      }
+      GANSU_HIP(launch_diis(e.ptr, w.F.ptr, errs.ptr, focks.ptr, dpart.ptr, nn, diis_size, st.ptr, cs));
+    } catch (...) {
+      hipGraph_t g = nullptr;
+      (void)hipStreamEndCapture(cs, &g);
+      if (g) (void)hipGraphDestroy(g);
+      throw;
+    }
+    GANSU_HIP(hipStreamEndCapture(cs, &graph));
+    GANSU_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+  };
+
+  GraphLoop r;
+  GANSU_HIP(hipDeviceSynchronize());  // the guess ran on the null stream
+  int steps = 128;
+  bool first = true;
+  try {
+    capture(steps);
+    while (true) {
+      const auto t0 = std::chrono::steady_clock::now();
+      GANSU_HIP(hipGraphLaunch(exec, cs));
+      GANSU_HIP(hipMemcpyAsync(&h, st.ptr, sizeof h, hipMemcpyDeviceToHost, cs));
+      GANSU_HIP(hipStreamSynchronize(cs));
+      r.seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+      ++r.launches;
+      if (trace)
+        std::fprintf(stderr, "[scf] graph iteration %d: E = %.12f, purification steps %d (bounds %.4f %.4f), "
+                             "DIIS steps %d\n", r.launches, h.E, h.sp2_used, h.bounds[0], h.bounds[1], h.diis_count);
+      if (!std::isfinite(h.E)) throw Error(GANSU_E_NUMERIC, "SCF energy is not finite");
+      if (h.sp2_used < 0) {
+        // the purification ran out of steps: more steps, and an energy
+        // reached with this density does not count as converged
+        if (steps >= 1024)
+          throw Error(GANSU_E_NUMERIC, "density purification did not reach idempotency (no gap at the Fermi level?)");
+        steps *= 2;
+        if (timing) std::fprintf(stderr, "[scf] purification out of steps; re-captured with %d\n", steps);
+        if (h.converged) {
+          const int zero = 0;
+          GANSU_HIP(hipMemcpy(&st.ptr->converged, &zero, sizeof zero, hipMemcpyHostToDevice));
+          h.converged = 0;
+        }
+        capture(steps);
+      } else if (first) {
+        // size the step count to the first density's need with a margin
+        // (steps after the stop cost a launch each)
+        const int want = std::max(48, (h.sp2_used * 3 / 2 + 16 + 7) / 8 * 8);
+        if (want < steps) {
+          steps = want;
+          capture(steps);
+        }
+        first = false;
+      }
+      if (h.converged || h.iter >= maxiter) break;
+    }
+  } catch (...) {
+    release();
+    (void)hipStreamDestroy(cs);
+    throw;
+  }
+  release();
+  GANSU_HIP(hipStreamSynchronize(cs));
+  (void)hipStreamDestroy(cs);
+  // X^T F X of the last density's Fock matrix, for the final orbitals
+  GANSU_HIP(hipMemcpy(w.tmp2.ptr, Fo.ptr, nn * sizeof(double), hipMemcpyDeviceToDevice));
+  r.E = h.E;
+  r.iterations = h.iter;
+  r.converged = h.converged;
+  return r;
+}
+
 }  // namespace
 
 OneIntTables make_oneint_tables(const Molecule& mol) {
@@ -369,11 +502,22 @@ ScfResult rhf_run(const Molecule& mol, const Molecule* aux, const Params& prm, b
   const int diis_size = prm.geti("diis_size");
   const bool diis_x = prm.geti("diis_include_transform") != 0;
   const double damp = prm.getd("damping_factor");
+  const std::string solver = prm.get("scf_solver");
+  if (solver != "diagonalize" && solver != "purify") throw Error(GANSU_E_PARAM, "Unknown scf_solver: " + solver);
+  const bool purify = solver == "purify";
+  if (purify && prm.get("eri_method") == "ri")
+    throw Error(GANSU_E_UNSUPPORTED, "scf_solver=purify needs eri_method stored or direct");
+  if (purify && conv_method != "diis") throw Error(GANSU_E_UNSUPPORTED, "scf_solver=purify runs DIIS only");
+  if (purify && (diis_size < 1 || diis_size > kGraphDiisMax))
+    throw Error(GANSU_E_PARAM, "scf_solver=purify needs 1 <= diis_size <= " + std::to_string(kGraphDiisMax));
 
   // GANSU_SCF_TIMING=1 prints the wall time of each phase (with a device
   // synchronisation at every phase boundary) to stderr.
   const char* tenv = std::getenv("GANSU_SCF_TIMING");
   const bool timing = tenv && tenv[0] == '1';
+  // GANSU_SCF_TRACE=1 prints the energy of every iteration to stderr.
+  const char* trenv = std::getenv("GANSU_SCF_TRACE");
+  const bool trace = trenv && trenv[0] == '1';
   auto wall = [&]() {
     if (timing) GANSU_HIP(hipDeviceSynchronize());
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
@@ -469,11 +613,23 @@ ScfResult rhf_run(const Molecule& mol, const Molecule* aux, const Params& prm, b
   // convergence state
   Diis diis;
   Damper damper;
-  if (conv_method == "diis") diis.init(diis_size, nn);
-  else damper.init(nn, conv_method == "optimaldamping", damp);
+  if (conv_method == "diis" && !purify) diis.init(diis_size, nn);
+  else if (!purify) damper.init(nn, conv_method == "optimaldamping", damp);
   double prev = std::numeric_limits<double>::max(), E = 0.0;
   int it = 0;
-  while (it < maxiter) {
+  if (purify) {
+    GraphLoop gl = graph_loop(eng, w, nocc, maxiter, conv, diis_size, diis_x, timing, trace);
+    E = gl.E;
+    it = gl.iterations;
+    res.converged = gl.converged;
+    res.fock_builds += gl.launches;
+    res.graph_launches = gl.launches;
+    t_fock = gl.seconds;
+    // orbitals of the last density: eigenvectors of its X^T F X (left in w.tmp2)
+    syevd(blas.h, n, w.tmp2.ptr, w);
+    blas.gemm(false, true, n, w.X.ptr, w.evec.ptr, w.C.ptr);
+  }
+  while (!purify && it < maxiter) {
     double t0 = now();
     coefficients(w.F.ptr);
     double t1 = now();
@@ -484,6 +640,7 @@ ScfResult rhf_run(const Molecule& mol, const Molecule* aux, const Params& prm, b
     E = energy();
     double t4 = now();
     t_diag += t1 - t0; t_dens += t2 - t1; t_fock += t3 - t2; t_energy += t4 - t3;
+    if (trace) std::fprintf(stderr, "[scf] iteration %d: E = %.12f\n", it + 1, E);
     if (std::fabs(E - prev) < conv) { res.converged = 1; break; }
     ++it;
     prev = E;
@@ -505,7 +662,10 @@ ScfResult rhf_run(const Molecule& mol, const Molecule* aux, const Params& prm, b
     }
     t_conv += now() - tc0;
   }
-  if (timing)
+  if (timing && purify)
+    std::fprintf(stderr, "[scf] %d iterations in %d graph launches: %.3f ms per iteration\n", it + 1,
+                 res.graph_launches, 1e3 * t_fock / std::max(1, res.graph_launches));
+  else if (timing)
     std::fprintf(stderr, "[scf] %d iterations: diag %.2f ms, density %.2f ms, fock %.2f ms, energy %.2f ms, "
                          "convergence %.2f ms (per iteration)\n", it + 1, 1e3 * t_diag / (it + 1),
                  1e3 * t_dens / (it + 1), 1e3 * t_fock / (it + 1), 1e3 * t_energy / (it + 1), 1e3 * t_conv / (it + 1));

@@ -13,6 +13,7 @@ namespace gansu {
 struct ScfResult {
   double e_elec = 0, e_total = 0;
   int iterations = 0, fock_builds = 0, converged = 0;
+  int graph_launches = 0;    // scf_solver=purify: captured iterations launched
   std::vector<double> D;     // final density (M x M)
   std::vector<double> grad;  // dE/dX (3 natoms), when requested
   std::vector<double> C;     // final MO coefficients (M x M row-major, MOs in columns)

@@ -305,7 +305,8 @@ def write_molden(mol, C, eps, occ, path):
 
 
 def rhf(mol, params, aux=None, density=None, force_density=False):
-    """Run the RHF SCF; returns dict(e_elec, e_total, iterations, fock_builds, converged, D).
+    """Run the RHF SCF; returns dict(e_elec, e_total, iterations, fock_builds, converged, graph_launches, D)
+    (graph_launches: captured iterations launched with scf_solver=purify, else 0).
     density = (Da, Db) (M x M each, D/2 each for a closed shell) starts the
     loop from F(Da + Db) when force_density=True or initial_guess=density
     (HF::solve(Da, Db, force_density), rhf.cu:122-134); otherwise the
@@ -323,7 +324,7 @@ def rhf(mol, params, aux=None, density=None, force_density=False):
         check(lib().gansu_rhf_run(mol.handle, aux.handle if aux else None, params.handle, ptr(en), ptr(D),
                                   ptr(info)))
     return dict(e_elec=float(en[0]), e_total=float(en[1]), iterations=int(info[0]), fock_builds=int(info[1]),
-                converged=bool(info[2]), D=D)
+                converged=bool(info[2]), graph_launches=int(info[3]), D=D)
 
 
 __all__ = ["FockEngine", "Parameters", "oneint", "rhf", "uhf", "rohf", "rhf_orbitals", "write_molden", "oneint_gradient", "nuclear_repulsion_gradient",

@@ -228,7 +228,15 @@ int gansu_export_sad_cache(const gansu_molecule* mol, const gansu_molecule* aux_
 /* RHF SCF loop of hf.cu:245-314 (core, GWH or SAD guess, DIIS/damping as the recipe
  * says) with this engine as the ERI strategy.  energies[2] = {E_electronic, E_total};
  * D_out (host, M*M) may be NULL.  info[4] = {iterations, fock builds,
- * converged, 0}.                                                            */
+ * converged, graph launches}.
+ * Engine key scf_solver (no reference counterpart): "diagonalize" (default,
+ * the reference's eigensolver step) or "purify" (eri_method stored or
+ * direct, DIIS): each iteration after the guess is one captured HIP graph
+ * -- X^T F X on the FP64 matrix cores, SP2 density purification in place of
+ * the eigensolver, D, the Fock build, the energy and convergence test and
+ * the DIIS step as device kernels -- launched once per iteration with one
+ * 40-byte read-back; the final orbitals come from one eigensolve of the
+ * last iteration's X^T F X.  info[3] counts those launches (0 otherwise).   */
 int gansu_rhf_run(const gansu_molecule* mol, const gansu_molecule* aux_mol, const gansu_params* p, double* energies,
                   double* D_out, int* info);
 /* HF::solve(Da0, Db0, force_density) for RHF (rhf.cu:122-134,
