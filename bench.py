@@ -363,6 +363,7 @@ def coronene_leg(log_fn):
     out["speedup_ri_mixed_vs_direct_fp64"] = round(out["direct_fp64"]["build_s"] / out["ri_mixed"]["build_s"], 2)
     scf = {}
     for name, kw, ri in (("stored_fp64", dict(eri_method="stored"), False),
+                         ("stored_fp64_purify", dict(eri_method="stored", scf_solver="purify"), False),
                          ("stored_mixed", dict(eri_method="stored", precision="mixed"), False),
                          ("ri_fp64", dict(eri_method="ri", ri_k="occupied"), True),
                          ("ri_mixed", dict(eri_method="ri", precision="mixed"), True)):
@@ -372,6 +373,7 @@ def coronene_leg(log_fn):
                          converged=r["converged"])
         log_fn(f"[bench] C24H12 SCF {name}: {scf[name]}")
     scf["dE_stored_mixed_vs_fp64"] = scf["stored_mixed"]["energy"] - scf["stored_fp64"]["energy"]
+    scf["dE_stored_purify_vs_diagonalize"] = scf["stored_fp64_purify"]["energy"] - scf["stored_fp64"]["energy"]
     scf["dE_ri_mixed_vs_fp64"] = scf["ri_mixed"]["energy"] - scf["ri_fp64"]["energy"]
     out["scf"] = scf
     return out
@@ -543,6 +545,15 @@ def stored_leg(cfg, method, precision, steps, warmup, no_direct, no_scf, no_mixe
         out.update(scf_wall_s=round(time.perf_counter() - t0, 3), scf_energy=r["e_total"],
                    scf_iterations=r["iterations"], scf_converged=r["converged"])
         log_fn(f"[bench] SCF {out['scf_wall_s']} s, E = {r['e_total']:.10f} ({r['iterations']} it)")
+        if method != "ri":
+            # the device-resident iteration: one captured graph per iteration
+            t0 = time.perf_counter()
+            rp = rhf(mol, Parameters(eri_method=method, precision=precision, convergence_energy_threshold=1e-8,
+                                     maxiter=100, scf_solver="purify"), aux=aux)
+            out["scf_purify"] = dict(wall_s=round(time.perf_counter() - t0, 3), energy=rp["e_total"],
+                                     iterations=rp["iterations"], graph_launches=rp["graph_launches"],
+                                     converged=rp["converged"], dE_vs_diagonalize=rp["e_total"] - r["e_total"])
+            log_fn(f"[bench] SCF purify (graph per iteration): {out['scf_purify']}")
     if not no_mixed and precision == "fp64" and method == "stored":
         em = FockEngine(mol, eri_method="stored", precision="mixed")
         em.precomputation()

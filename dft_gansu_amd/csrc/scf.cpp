@@ -14,6 +14,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cstddef>
 #include <cstdio>
 #include <cstdlib>
 #include <cmath>
@@ -244,8 +245,8 @@ struct Damper {
 // the engine's four-centre build, E = 1/2 D.(H + F) and the convergence
 // test, then (unless converged) the DIIS error F D S - S D F and the DIIS
 // step, all reading their control words from a device state.  The host
-// launches the graph once per iteration and reads the state back (40
-// bytes) to decide whether to go on.  The purification's step count is
+// launches the graph once per iteration and reads the control words back
+// (56 bytes) to decide whether to go on.  The purification's step count is
 // captured with a margin over what the first iteration needed; an
 // iteration that runs out of steps doubles it (re-captured), and a
 // convergence reached on such an iteration is not accepted.
@@ -322,7 +323,8 @@ GraphLoop graph_loop(FockEngine& eng, Work& w, int nocc, int maxiter, double con
     while (true) {
       const auto t0 = std::chrono::steady_clock::now();
       GANSU_HIP(hipGraphLaunch(exec, cs));
-      GANSU_HIP(hipMemcpyAsync(&h, st.ptr, sizeof h, hipMemcpyDeviceToHost, cs));
+      // the control words only (not the DIIS Gram matrix behind them)
+      GANSU_HIP(hipMemcpyAsync(&h, st.ptr, offsetof(ScfDevState, gram), hipMemcpyDeviceToHost, cs));
       GANSU_HIP(hipStreamSynchronize(cs));
       r.seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
       ++r.launches;

@@ -235,7 +235,7 @@ int gansu_export_sad_cache(const gansu_molecule* mol, const gansu_molecule* aux_
  * -- X^T F X on the FP64 matrix cores, SP2 density purification in place of
  * the eigensolver, D, the Fock build, the energy and convergence test and
  * the DIIS step as device kernels -- launched once per iteration with one
- * 40-byte read-back; the final orbitals come from one eigensolve of the
+ * 56-byte read-back; the final orbitals come from one eigensolve of the
  * last iteration's X^T F X.  info[3] counts those launches (0 otherwise).   */
 int gansu_rhf_run(const gansu_molecule* mol, const gansu_molecule* aux_mol, const gansu_params* p, double* energies,
                   double* D_out, int* info);
