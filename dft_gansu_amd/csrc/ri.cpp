@@ -436,7 +436,7 @@ bool FockEngine::build_ri_incremental(const double* d_C, int nocc, double* J, do
   const long Kd = (long)Ma * nocc;
   const int nch = rik_chunks(M, Kd);
   const long kchunk = ((Kd + nch - 1) / nch + 31) / 32 * 32;
-  RikDkArgs da{r.TA.ptr, r.TE.ptr, M, Kd, kchunk, r.rst.ptr, r.Zpart.ptr};
+  RikDkArgs da{r.TA.ptr, r.TE.ptr, M, Kd, kchunk, nch, r.rst.ptr, r.Zpart.ptr};
   prof_begin(s, 80025, 2.0 * double(M) * Kd * 4, 2.0 * M * (double)M * Kd * 3.0, (double)nch);
   GANSU_HIP(launch_rik_dk(da, nch, s));
   prof_end(s);

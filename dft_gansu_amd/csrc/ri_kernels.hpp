@@ -120,6 +120,7 @@ struct RikDkArgs {
   const float* TE;
   int M;
   long Kd, kchunk;
+  int nchunk;         // k chunks (split-k partials)
   const RikState* st;
   float* Zpart;       // nchunk x M x M
 };

@@ -258,40 +258,60 @@ __global__ __launch_bounds__(256) void k_rik_tstep(RikTArgs a) {
 // --------------------------------------------------------------- dK -------
 // Zpart[chunk][m][n] = sum over the chunk's k of T_A[m][k] T_E[n][k]
 // (K dimension Kd = Maux nocc).  Workgroup tile 128 x 128, four waves of
-// 64 x 64 (2 x 2 MFMA blocks), 32-k stages: the stage in LDS feeds the
-// MFMAs while the next one is loaded into registers; it is split into
-// fp16 hi/lo on its way into LDS between two barriers.
+// 64 x 64 (2 x 2 MFMA blocks), 32-k stages.  Stages are loaded into
+// registers two ahead of the one the MFMAs use (two register sets) and
+// split into fp16 hi/lo on their way into LDS, which holds two stages, so
+// one barrier per stage separates the writes of stage s + 1 from the reads
+// of stage s - 1.
 constexpr int kDkTile = 128;
 
+struct DkStage {
+  float4 a[4], b[4];
+  float fa[4], fb[4];  // 1 for loads inside the matrices, 0 for clamped ones
+};
+
+template <bool VEC>
 __global__ __launch_bounds__(256) void k_rik_dk(RikDkArgs a) {
-  __shared__ __attribute__((aligned(16))) _Float16 sa[2][kDkTile][40];  // [hi/lo][row][k]
-  __shared__ __attribute__((aligned(16))) _Float16 sb[2][kDkTile][40];
+  __shared__ __attribute__((aligned(16))) _Float16 sa[2][2][kDkTile][40];  // [buf][hi/lo][row][k]
+  __shared__ __attribute__((aligned(16))) _Float16 sb[2][2][kDkTile][40];
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int g = lane >> 5, li = lane & 31;
   const int wm = wave >> 1, wn = wave & 1;
   const int ntile = (a.M + kDkTile - 1) / kDkTile;
-  const int tm = blockIdx.x / ntile, tn = blockIdx.x % ntile;
-  const long k_begin = (long)blockIdx.y * a.kchunk;
+  // Workgroups reach the eight XCDs round-robin (XCD = linear id % 8): XCD x
+  // walks the tiles of chunks x, x + 8, ... in row-major tile order, so the
+  // workgroups resident on one XCD share the k stages of one chunk in its L2.
+  const int nt2 = ntile * ntile;
+  const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
+  const int chunk = (slot / nt2) * 8 + xcd, tile = slot % nt2;
+  if (chunk >= a.nchunk) return;
+  const int tm = tile / ntile, tn = tile % ntile;
+  const long k_begin = (long)chunk * a.kchunk;
   const long k_end = k_begin + a.kchunk < a.Kd ? k_begin + a.kchunk : a.Kd;
   const int nst = k_end > k_begin ? (int)((k_end - k_begin + 31) / 32) : 0;
-  const bool vec = (a.Kd & 3) == 0;  // rows start 16-byte aligned
   const float sA = op_scale(__uint_as_float(a.st->ta_max));
   const float sE = op_scale(__uint_as_float(a.st->te_max));
 
   // a stage: 128 rows x 32 k of T_A (A side) and of T_E (B side), 1024
-  // float4 each, 4 per thread; row r = idx / 8, k offset 4 (idx % 8)
-  float4 va[4], vb[4];
-  auto fetch = [&](int s) {
+  // float4 each, 4 per thread; row r = idx / 8, k offset 4 (idx % 8).  With
+  // VEC (Kd % 4 == 0, so chunk ends are multiples of 4) the loads are
+  // unconditional from clamped addresses and masked afterwards, which keeps
+  // the loop free of divergent branches (and the load counters exact).
+  auto fetch = [&](int s, DkStage& v) {
     const long k0 = k_begin + (long)s * 32;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const int idx = t + 256 * j, r = idx >> 3, q = idx & 7;
       const long k = k0 + q * 4;
       const int ra = tm * kDkTile + r, rb = tn * kDkTile + r;
-      float4 x = make_float4(0.f, 0.f, 0.f, 0.f), y = x;
-      if (vec && k + 4 <= k_end) {
-        if (ra < a.M) x = *(const float4*)(a.TA + (size_t)ra * a.Kd + k);
-        if (rb < a.M) y = *(const float4*)(a.TE + (size_t)rb * a.Kd + k);
+      if constexpr (VEC) {
+        const long kc = k + 4 <= k_end ? k : k_end - 4;
+        const int rac = ra < a.M ? ra : a.M - 1, rbc = rb < a.M ? rb : a.M - 1;
+        v.a[j] = *(const float4*)(a.TA + (size_t)rac * a.Kd + kc);
+        v.b[j] = *(const float4*)(a.TE + (size_t)rbc * a.Kd + kc);
+        // the masks are applied in stash, once the loads have landed
+        v.fa[j] = (ra < a.M && k < k_end) ? 1.0f : 0.0f;
+        v.fb[j] = (rb < a.M && k < k_end) ? 1.0f : 0.0f;
       } else {
         float xa[4] = {0.f, 0.f, 0.f, 0.f}, yb[4] = {0.f, 0.f, 0.f, 0.f};
         for (int e = 0; e < 4; ++e)
@@ -299,11 +319,11 @@ __global__ __launch_bounds__(256) void k_rik_dk(RikDkArgs a) {
             if (ra < a.M) xa[e] = a.TA[(size_t)ra * a.Kd + k + e];
             if (rb < a.M) yb[e] = a.TE[(size_t)rb * a.Kd + k + e];
           }
-        x = make_float4(xa[0], xa[1], xa[2], xa[3]);
-        y = make_float4(yb[0], yb[1], yb[2], yb[3]);
+        v.a[j] = make_float4(xa[0], xa[1], xa[2], xa[3]);
+        v.b[j] = make_float4(yb[0], yb[1], yb[2], yb[3]);
+        v.fa[j] = 1.0f;
+        v.fb[j] = 1.0f;
       }
-      va[j] = x;
-      vb[j] = y;
     }
   };
   // split four floats into hi/lo halves and store each as one 8-byte word
@@ -321,12 +341,12 @@ __global__ __launch_bounds__(256) void k_rik_dk(RikDkArgs a) {
     *(h4*)hi = h;
     *(h4*)lo = l;
   };
-  auto stash = [&]() {
+  auto stash = [&](const DkStage& v, int buf) {
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const int idx = t + 256 * j, r = idx >> 3, q = idx & 7;
-      put4(va[j], sA, &sa[0][r][q * 4], &sa[1][r][q * 4]);
-      put4(vb[j], sE, &sb[0][r][q * 4], &sb[1][r][q * 4]);
+      put4(v.a[j], sA * v.fa[j], &sa[buf][0][r][q * 4], &sa[buf][1][r][q * 4]);
+      put4(v.b[j], sE * v.fb[j], &sb[buf][0][r][q * 4], &sb[buf][1][r][q * 4]);
     }
   };
 
@@ -338,28 +358,21 @@ __global__ __launch_bounds__(256) void k_rik_dk(RikDkArgs a) {
 #pragma unroll
       for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
 
-  if (nst > 0) {
-    fetch(0);
-    stash();
-  }
-  __syncthreads();
-  for (int s = 0; s < nst; ++s) {
-    const bool more = s + 1 < nst;
-    if (more) fetch(s + 1);
+  auto compute = [&](int buf) {
 #pragma unroll
     for (int ks = 0; ks < 2; ++ks) {
       h8 ahi[2], alo[2], bhi[2], blo[2];
 #pragma unroll
       for (int i = 0; i < 2; ++i) {
         const int r = wm * 64 + i * 32 + li;
-        ahi[i] = *(const h8*)&sa[0][r][ks * 16 + g * 8];
-        alo[i] = *(const h8*)&sa[1][r][ks * 16 + g * 8];
+        ahi[i] = *(const h8*)&sa[buf][0][r][ks * 16 + g * 8];
+        alo[i] = *(const h8*)&sa[buf][1][r][ks * 16 + g * 8];
       }
 #pragma unroll
       for (int j = 0; j < 2; ++j) {
         const int r = wn * 64 + j * 32 + li;
-        bhi[j] = *(const h8*)&sb[0][r][ks * 16 + g * 8];
-        blo[j] = *(const h8*)&sb[1][r][ks * 16 + g * 8];
+        bhi[j] = *(const h8*)&sb[buf][0][r][ks * 16 + g * 8];
+        blo[j] = *(const h8*)&sb[buf][1][r][ks * 16 + g * 8];
       }
 #pragma unroll
       for (int i = 0; i < 2; ++i)
@@ -374,15 +387,38 @@ __global__ __launch_bounds__(256) void k_rik_dk(RikDkArgs a) {
 #pragma unroll
         for (int j = 0; j < 2; ++j) acc[i][j] = mfma(alo[i], bhi[j], acc[i][j]);
     }
-    if (more) {
-      __syncthreads();
-      stash();
-      __syncthreads();
-    }
+  };
+
+  // stage s sits in LDS buffer s % 2 and, before that, in register set s % 2;
+  // the loop takes stages in pairs, and every fetch is issued (stages past
+  // the end fetch the last one again), so the loop has no branches and each
+  // stash waits only for its own stage's loads
+  DkStage r0, r1;
+  const int last = nst - 1;
+  if (nst > 0) {
+    fetch(0, r0);
+    stash(r0, 0);
+    fetch(1 < last ? 1 : last, r1);
+    fetch(2 < last ? 2 : last, r0);
   }
+  __syncthreads();
+  int s = 0;
+  for (; s + 2 <= nst; s += 2) {
+    // stage s in buffer 0, stage s + 1 in r1, stage s + 2 in r0
+    stash(r1, 1);
+    fetch(s + 3 < last ? s + 3 : last, r1);
+    compute(0);
+    __syncthreads();
+    // stage s + 1 in buffer 1, stage s + 2 in r0, stage s + 3 in r1
+    stash(r0, 0);
+    fetch(s + 4 < last ? s + 4 : last, r0);
+    compute(1);
+    __syncthreads();
+  }
+  if (s < nst) compute(0);  // an odd last stage, already in buffer 0
 
   const float inv = 1.0f / (sA * sE);
-  float* Z = a.Zpart + (size_t)blockIdx.y * a.M * a.M;
+  float* Z = a.Zpart + (size_t)chunk * a.M * a.M;
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
@@ -437,17 +473,65 @@ __global__ void k_rik_dw(const double* __restrict__ dWrow, int M, int Maux, doub
   Wprev[P] = w;
 }
 
-// J[m][n] = Jprev[m][n] + sum_P dW_P B32[m][P][n]; Jprev <- J
-__global__ void k_rik_j(const float* __restrict__ B32, const double* __restrict__ dW, int M, int Maux,
-                        double* __restrict__ Jprev, double* __restrict__ J) {
-  const int m = blockIdx.x;
-  for (int n = threadIdx.x; n < M; n += blockDim.x) {
-    double s = 0.0;
-    const float* col = B32 + (size_t)m * Maux * M + n;
-    for (int P = 0; P < Maux; ++P) s += dW[P] * (double)col[(size_t)P * M];
-    const double v = Jprev[(size_t)m * M + n] + s;
-    J[(size_t)m * M + n] = v;
-    Jprev[(size_t)m * M + n] = v;
+// J[m][n] = Jprev[m][n] + sum_P dW_P B32[m][P][n]; Jprev <- J.  A workgroup
+// takes one m and 256 consecutive n: each lane four columns (16-byte loads
+// when M % 4 == 0), the four waves a quarter of P each, four P per round in
+// flight; the wave partial sums meet in LDS in wave order.
+__global__ __launch_bounds__(256) void k_rik_j(const float* __restrict__ B32, const double* __restrict__ dW, int M,
+                                               int Maux, double* __restrict__ Jprev, double* __restrict__ J) {
+  __shared__ double red[3][64][4];
+  const int m = blockIdx.x, w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int n0 = blockIdx.y * 256 + lane * 4;
+  const int P0 = (int)((long)w * Maux / 4), P1 = (int)((long)(w + 1) * Maux / 4);
+  const float* slab = B32 + (size_t)m * Maux * M;
+  double s[4] = {0.0, 0.0, 0.0, 0.0};
+  if (n0 < M) {
+    if ((M & 3) == 0) {
+      int P = P0;
+      for (; P + 4 <= P1; P += 4) {
+        float4 v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) v[u] = *(const float4*)(slab + (size_t)(P + u) * M + n0);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const double d = dW[P + u];
+          s[0] += d * (double)v[u].x;
+          s[1] += d * (double)v[u].y;
+          s[2] += d * (double)v[u].z;
+          s[3] += d * (double)v[u].w;
+        }
+      }
+      for (; P < P1; ++P) {
+        const float4 v = *(const float4*)(slab + (size_t)P * M + n0);
+        const double d = dW[P];
+        s[0] += d * (double)v.x;
+        s[1] += d * (double)v.y;
+        s[2] += d * (double)v.z;
+        s[3] += d * (double)v.w;
+      }
+    } else {
+      for (int P = P0; P < P1; ++P) {
+        const double d = dW[P];
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+          if (n0 + e < M) s[e] += d * (double)slab[(size_t)P * M + n0 + e];
+      }
+    }
+  }
+  if (w > 0)
+#pragma unroll
+    for (int e = 0; e < 4; ++e) red[w - 1][lane][e] = s[e];
+  __syncthreads();
+  if (w != 0 || n0 >= M) return;
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    const int n = n0 + e;
+    if (n >= M) continue;
+    const double tot = ((s[e] + red[0][lane][e]) + red[1][lane][e]) + red[2][lane][e];
+    const size_t id = (size_t)m * M + n;
+    const double v = Jprev[id] + tot;
+    J[id] = v;
+    Jprev[id] = v;
   }
 }
 
@@ -559,8 +643,11 @@ hipError_t launch_rik_tstep(const RikTArgs& a, hipStream_t s) {
 }
 
 hipError_t launch_rik_dk(const RikDkArgs& a, int nchunk, hipStream_t s) {
+  if (a.nchunk != nchunk) return hipErrorInvalidValue;
   const int ntile = (a.M + kDkTile - 1) / kDkTile;
-  hipLaunchKernelGGL(k_rik_dk, dim3((unsigned)(ntile * ntile), (unsigned)nchunk), dim3(256), 0, s, a);
+  const long nwg = (long)ntile * ntile * ((nchunk + 7) / 8) * 8;
+  if ((a.Kd & 3) == 0) hipLaunchKernelGGL(k_rik_dk<true>, dim3((unsigned)nwg), dim3(256), 0, s, a);
+  else hipLaunchKernelGGL(k_rik_dk<false>, dim3((unsigned)nwg), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 
@@ -577,7 +664,8 @@ hipError_t launch_rik_dw(const double* dWrow, int M, int Maux, double* dW, doubl
 
 hipError_t launch_rik_j(const float* B32, const double* dW, int M, int Maux, double* Jprev, double* J,
                         hipStream_t s) {
-  hipLaunchKernelGGL(k_rik_j, dim3(M), dim3(256), 0, s, B32, dW, M, Maux, Jprev, J);
+  hipLaunchKernelGGL(k_rik_j, dim3((unsigned)M, (unsigned)((M + 255) / 256)), dim3(256), 0, s, B32, dW, M, Maux,
+                     Jprev, J);
   return hipGetLastError();
 }
 
