@@ -72,6 +72,7 @@ struct FockEngine::RIState {
   DevBuf<double> Xprev, Xn, Ework, Y, Y2, G, Kprev, Jprev, Wprev, dW, dWrow, nrm;
   DevBuf<uint16_t> Ehi, Elo;
   DevBuf<RikState> rst;
+  DevBuf<unsigned> wgmax;  // per-workgroup maxima of the T update
   double* nrm_host = nullptr;
   std::vector<int> aux_atom_h;  // atom of each auxiliary shell
   DevBuf<int> aux_atom;
@@ -305,6 +306,7 @@ void FockEngine::rik_prepare(int nocc) {
     r.TA.alloc(tn);
     r.TE.alloc(tn);
     r.dWrow.alloc((size_t)M * Ma);
+    r.wgmax.alloc(2 * rik_tstep_workgroups((long)M * Ma));
     r.Xprev.alloc((size_t)M * nocc);
     r.Xn.alloc((size_t)M * nocc);
     r.Ework.alloc((size_t)M * nocc);
@@ -422,7 +424,7 @@ bool FockEngine::build_ri_incremental(const double* d_C, int nocc, double* J, do
   // T_E = B E (+ T_A update, dW rows): 2 R M nocc flops, three fp16 products each
   const long R = (long)M * Ma;
   RikTArgs ta{r.B32.ptr, R, M, Ma, nocc, kpad, r.sB, r.Ehi.ptr, r.Elo.ptr, r.Ework.ptr, r.rst.ptr,
-              r.TA.ptr, r.TE.ptr, r.dWrow.ptr, fresh ? 1 : 0};
+              r.TA.ptr, r.TE.ptr, r.dWrow.ptr, fresh ? 1 : 0, r.wgmax.ptr};
   prof_begin(s, 80022, double(R) * M * 4 + double(R) * nocc * 16, 2.0 * R * M * nocc * 3.0, (double)R);
   GANSU_HIP(launch_rik_tstep(ta, s));
   prof_end(s);

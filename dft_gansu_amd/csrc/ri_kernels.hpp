@@ -114,7 +114,9 @@ struct RikTArgs {
   float* TE;
   double* dWrow;      // R
   int fresh;          // 1: the previous T_A, T_E count as zero (not read)
+  unsigned* wgmax;    // 2 per workgroup: max|T_A|, max|T_E| (float bits)
 };
+size_t rik_tstep_workgroups(long R);
 struct RikDkArgs {
   const float* TA;    // M x Kd (Kd = Maux nocc)
   const float* TE;

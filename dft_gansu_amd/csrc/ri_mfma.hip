@@ -104,7 +104,7 @@ __global__ __launch_bounds__(1024) void k_rik_split_e(const double* __restrict__
 // E^T pieces and B32 values are loaded into registers while the MFMAs run
 // and reach LDS after a barrier, so no wave waits on a global load inside
 // the stage.
-template <int NBW>
+template <int NBW, bool VEC>
 __global__ __launch_bounds__(256) void k_rik_tstep(RikTArgs a) {
   constexpr int NCOL = 64 * NBW;           // columns of the workgroup
   constexpr int NPIECE = 2 * NCOL * 4;     // 16-byte pieces of one E^T stage
@@ -116,10 +116,9 @@ __global__ __launch_bounds__(256) void k_rik_tstep(RikTArgs a) {
   const long row0 = (long)blockIdx.x * 64 + rg * 32;
   const long myrow = row0 + li;
   const bool rowok = myrow < a.R;
-  const float* brow = a.B32 + (size_t)(rowok ? myrow : 0) * a.M;
+  const float* brow = a.B32 + (size_t)(rowok ? myrow : a.R - 1) * a.M;
   const float sB = a.sB, sE = a.st->sE;
   const int nk = a.kpad / 32;
-  const bool vec = (a.M & 3) == 0;  // rows start 16-byte aligned
 
   f16x acc[NBW];
 #pragma unroll
@@ -133,19 +132,34 @@ __global__ __launch_bounds__(256) void k_rik_tstep(RikTArgs a) {
 #define RIK_EPIECE(j) const int piece_ = t + 256 * (j), hl_ = piece_ / (NCOL * 4), rem_ = piece_ % (NCOL * 4)
 #define RIK_ESRC(kc) (*(const u32x4*)((hl_ ? a.Elo : a.Ehi) + (size_t)(rem_ >> 2) * a.kpad + (kc) * 32 + (rem_ & 3) * 8))
 #define RIK_EDST (*(u32x4*)&lds[hl_][rem_ >> 2][(rem_ & 3) * 8])
+  // B32 row pieces of a stage: 2 x 8 consecutive k per lane.  With VEC
+  // (M % 4 == 0) as four unconditional 16-byte loads from clamped addresses,
+  // masked in the split (areg_f), so the loop has no divergent branches and
+  // each wait covers only the loads it needs.
   float areg[16];
+  float areg_f[4];
   auto load_a = [&](int kc) {
     const int k0 = kc * 32;
 #pragma unroll
     for (int ks = 0; ks < 2; ++ks) {
       const int k = k0 + ks * 16 + g * 8;
-      if (vec && rowok && k + 8 <= a.M) {
-        const float4 v0 = *(const float4*)(brow + k), v1 = *(const float4*)(brow + k + 4);
-        areg[ks * 8 + 0] = v0.x; areg[ks * 8 + 1] = v0.y; areg[ks * 8 + 2] = v0.z; areg[ks * 8 + 3] = v0.w;
-        areg[ks * 8 + 4] = v1.x; areg[ks * 8 + 5] = v1.y; areg[ks * 8 + 6] = v1.z; areg[ks * 8 + 7] = v1.w;
+      if constexpr (VEC) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const int kk = k + 4 * h;
+          const int kcl = kk + 4 <= a.M ? kk : a.M - 4;
+          const float4 v = *(const float4*)(brow + kcl);
+          areg[ks * 8 + 4 * h + 0] = v.x;
+          areg[ks * 8 + 4 * h + 1] = v.y;
+          areg[ks * 8 + 4 * h + 2] = v.z;
+          areg[ks * 8 + 4 * h + 3] = v.w;
+          areg_f[ks * 2 + h] = (rowok && kk + 4 <= a.M) ? 1.0f : 0.0f;
+        }
       } else {
 #pragma unroll
         for (int e = 0; e < 8; ++e) areg[ks * 8 + e] = (rowok && k + e < a.M) ? brow[k + e] : 0.0f;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) areg_f[ks * 2 + h] = 1.0f;
       }
     }
   };
@@ -169,7 +183,7 @@ __global__ __launch_bounds__(256) void k_rik_tstep(RikTArgs a) {
 #pragma unroll
       for (int e = 0; e < 8; ++e) {
         _Float16 h, l;
-        split1(areg[ks * 8 + e], sB, h, l);
+        split1(areg[ks * 8 + e], sB * areg_f[ks * 2 + (e >> 2)], h, l);
         ahi[ks][e] = h;
         alo[ks][e] = l;
       }
@@ -182,17 +196,24 @@ __global__ __launch_bounds__(256) void k_rik_tstep(RikTArgs a) {
       }
       load_a(kc + 1);
     }
+    // the three products of a block are dependent: issue them for all
+    // blocks in turn, so that NBW independent accumulators separate them
 #pragma unroll
-    for (int ks = 0; ks < 2; ++ks)
+    for (int ks = 0; ks < 2; ++ks) {
+      h8 bhi[NBW], blo[NBW];
 #pragma unroll
       for (int b = 0; b < NBW; ++b) {
         const int col = ch * 32 * NBW + b * 32 + li;
-        const h8 bhi = *(const h8*)&lds[0][col][ks * 16 + g * 8];
-        const h8 blo = *(const h8*)&lds[1][col][ks * 16 + g * 8];
-        acc[b] = mfma(ahi[ks], bhi, acc[b]);
-        acc[b] = mfma(ahi[ks], blo, acc[b]);
-        acc[b] = mfma(alo[ks], bhi, acc[b]);
+        bhi[b] = *(const h8*)&lds[0][col][ks * 16 + g * 8];
+        blo[b] = *(const h8*)&lds[1][col][ks * 16 + g * 8];
       }
+#pragma unroll
+      for (int b = 0; b < NBW; ++b) acc[b] = mfma(ahi[ks], bhi[b], acc[b]);
+#pragma unroll
+      for (int b = 0; b < NBW; ++b) acc[b] = mfma(ahi[ks], blo[b], acc[b]);
+#pragma unroll
+      for (int b = 0; b < NBW; ++b) acc[b] = mfma(alo[ks], bhi[b], acc[b]);
+    }
     if (more) {
       __syncthreads();
 #pragma unroll
@@ -249,9 +270,42 @@ __global__ __launch_bounds__(256) void k_rik_tstep(RikTArgs a) {
     mta = fmaxf(mta, __shfl_xor(mta, o, 64));
     mte = fmaxf(mte, __shfl_xor(mte, o, 64));
   }
+  // the workgroup's maxima go to its own slot (k_rik_maxred reduces them):
+  // one atomic per wave on a single word serialised every workgroup
+  __shared__ float wmax[2][4];
   if (lane == 0) {
-    atomic_max_pos(&a.st->ta_max, mta);
-    atomic_max_pos(&a.st->te_max, mte);
+    wmax[0][wave] = mta;
+    wmax[1][wave] = mte;
+  }
+  __syncthreads();
+  if (t == 0) {
+    a.wgmax[2 * blockIdx.x] = __float_as_uint(fmaxf(fmaxf(wmax[0][0], wmax[0][1]), fmaxf(wmax[0][2], wmax[0][3])));
+    a.wgmax[2 * blockIdx.x + 1] =
+        __float_as_uint(fmaxf(fmaxf(wmax[1][0], wmax[1][1]), fmaxf(wmax[1][2], wmax[1][3])));
+  }
+}
+
+// max|T_A|, max|T_E| over the workgroups of k_rik_tstep (one workgroup)
+__global__ __launch_bounds__(1024) void k_rik_maxred(const unsigned* __restrict__ wgmax, long nwg, RikState* st) {
+  __shared__ unsigned m[2][1024];
+  unsigned a0 = 0u, a1 = 0u;
+  for (long i = threadIdx.x; i < nwg; i += 1024) {
+    a0 = max(a0, wgmax[2 * i]);
+    a1 = max(a1, wgmax[2 * i + 1]);
+  }
+  m[0][threadIdx.x] = a0;
+  m[1][threadIdx.x] = a1;
+  __syncthreads();
+  for (int o = 512; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) {
+      m[0][threadIdx.x] = max(m[0][threadIdx.x], m[0][threadIdx.x + o]);
+      m[1][threadIdx.x] = max(m[1][threadIdx.x], m[1][threadIdx.x + o]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    st->ta_max = m[0][0];
+    st->te_max = m[1][0];
   }
 }
 
@@ -606,8 +660,9 @@ __global__ void k_rik_reset(RikState* st) {
 
 template <int NBW>
 hipError_t launch_tstep_nb(const RikTArgs& a, hipStream_t s) {
-  const unsigned grid = (unsigned)((a.R + 63) / 64);
-  hipLaunchKernelGGL(k_rik_tstep<NBW>, dim3(grid), dim3(256), 0, s, a);
+  const unsigned grid = (unsigned)((a.R + 63) / 64);  // rik_tstep_workgroups
+  if ((a.M & 3) == 0) hipLaunchKernelGGL((k_rik_tstep<NBW, true>), dim3(grid), dim3(256), 0, s, a);
+  else hipLaunchKernelGGL((k_rik_tstep<NBW, false>), dim3(grid), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 
@@ -627,19 +682,25 @@ hipError_t launch_rik_split_e(const double* E, int M, int nocc, int kpad, RikSta
   return hipGetLastError();
 }
 
+size_t rik_tstep_workgroups(long R) { return (size_t)((R + 63) / 64); }
+
 hipError_t launch_rik_tstep(const RikTArgs& a, hipStream_t s) {
   hipLaunchKernelGGL(k_rik_reset, dim3(1), dim3(1), 0, s, a.st);
   // dW rows gather the two column halves by atomics
   const unsigned zg = (unsigned)std::min<long>(8192, (a.R + 255) / 256);
   hipLaunchKernelGGL(k_zero_d, dim3(zg), dim3(256), 0, s, a.dWrow, (size_t)a.R);
+  hipError_t e = hipErrorInvalidValue;
   switch ((rik_blocks(a.nocc) + 1) / 2) {  // 32-column blocks per wave
-    case 1: return launch_tstep_nb<1>(a, s);
-    case 2: return launch_tstep_nb<2>(a, s);
-    case 3: return launch_tstep_nb<3>(a, s);
-    case 4: return launch_tstep_nb<4>(a, s);
-    case 5: return launch_tstep_nb<5>(a, s);
-    default: return hipErrorInvalidValue;
+    case 1: e = launch_tstep_nb<1>(a, s); break;
+    case 2: e = launch_tstep_nb<2>(a, s); break;
+    case 3: e = launch_tstep_nb<3>(a, s); break;
+    case 4: e = launch_tstep_nb<4>(a, s); break;
+    case 5: e = launch_tstep_nb<5>(a, s); break;
+    default: break;
   }
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_rik_maxred, dim3(1), dim3(1024), 0, s, a.wgmax, (long)rik_tstep_workgroups(a.R), a.st);
+  return hipGetLastError();
 }
 
 hipError_t launch_rik_dk(const RikDkArgs& a, int nchunk, hipStream_t s) {
