@@ -605,8 +605,7 @@ void FockEngine::update_geometry(const double* xyz, int n_atoms) {
   graph_release();
   if (precomputed_) {
     precomputed_ = false;
-    ri_destroy(ri_);
-    ri_ = nullptr;
+    // the RI state is replaced by precompute_ri, which keeps its large buffers
     d_alpha_.release();
     d_shell_atom_.release();
     precompute();

@@ -14,6 +14,7 @@
 #include <rocsolver/rocsolver.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cstdio>
 #include <cstdlib>
 #include <cmath>
@@ -105,10 +106,34 @@ static int ri_k_max_split(int nr, int nc) {
 }
 
 void FockEngine::precompute_ri() {
-  ri_destroy(ri_);
+  // GANSU_RI_TRACE=1: wall time of each precompute phase (device-synchronised)
+  auto t_last = std::chrono::steady_clock::now();
+  auto mark = [&](const char* what) {
+    if (!rik_trace_) return;
+    GANSU_HIP(hipDeviceSynchronize());
+    const auto now = std::chrono::steady_clock::now();
+    std::fprintf(stderr, "[ri] precompute %s: %.1f ms\n", what,
+                 1e3 * std::chrono::duration<double>(now - t_last).count());
+    t_last = now;
+  };
+  // A repeated precompute (a new geometry) keeps the large tensors of the
+  // previous state when their sizes match: reallocating a freed buffer of
+  // this size costs about 2 s (the driver clears the memory again), the
+  // recomputation fills every element anyway.
+  RIState* old = ri_;
   ri_ = new RIState();
   RIState& r = *ri_;
+  if (old) {
+    r.B = std::move(old->B);
+    r.B32 = std::move(old->B32);
+    r.TA = std::move(old->TA);
+    r.TE = std::move(old->TE);
+    r.T = std::move(old->T);
+    r.Zpart = std::move(old->Zpart);
+    ri_destroy(old);
+  }
   blas_ok(rocblas_create_handle(&r.h), "rocblas_create_handle");
+  mark("handle");
   const int M = M_, Ma = Maux_;
   // auxiliary shells and primitive Schwarz factors (schwarz.txt:479+)
   const auto ash = contracted_shells(aux_prims_.data(), aux_prims_.size(), kMaxLAux);
@@ -159,6 +184,7 @@ void FockEngine::precompute_ri() {
 
   // (P|Q) and its Cholesky factor.  Row-major L (lower) is the column-major
   // upper factor of dpotrf (the reference's convention, gpu_manager.cu:1311-1349).
+  mark("aux shells");
   r.L.alloc((size_t)Ma * Ma);
   GANSU_HIP(launch_zero(r.L.ptr, r.L.n, 0));
   const size_t nsh = r.shells_h.size();
@@ -169,23 +195,29 @@ void FockEngine::precompute_ri() {
   int hinfo = 0;
   GANSU_HIP(hipMemcpy(&hinfo, info.ptr, sizeof(int), hipMemcpyDeviceToHost));
   if (hinfo != 0) throw Error(GANSU_E_NUMERIC, "(P|Q) is not positive definite (potrf info " + std::to_string(hinfo) + ")");
+  mark("metric + Cholesky");
 
   // (mn|P) into B[m][P][n], then B_m = L^-1 T_m for every m
-  r.B.alloc((size_t)M * Ma * M);
+  if (r.B.n != (size_t)M * Ma * M) r.B.alloc((size_t)M * Ma * M);
+  mark("B allocation");
   GANSU_HIP(launch_zero(r.B.ptr, r.B.n, 0));
+  mark("B fill");
   DevBuf<double> dw;
   dw.upload(w_h_);
   RI3cArgs a3{d_sp_.ptr, d_pr_.ptr, (int)sp_h_.size(), r.shells.ptr, r.prims.ptr, (int)nsh, dw.ptr, r.waux.ptr,
               M, Ma, thr_, boys_.ptr, r.B.ptr};
   GANSU_HIP(launch_int3c2e(a3, 0));
+  mark("3-centre integrals");
   const double one = 1.0;
   // column-major view of slab m is T_m^T (M x Maux, ld M): T_m^T <- T_m^T U^-1, U = L^T
   blas_ok(rocblas_dtrsm_strided_batched(r.h, rocblas_side_right, rocblas_fill_upper, rocblas_operation_none,
                                         rocblas_diagonal_non_unit, M, Ma, &one, r.L.ptr, Ma, 0, r.B.ptr, M,
                                         (rocblas_stride)Ma * M, M),
           "rocblas_dtrsm_strided_batched");
+  mark("B = L^-1 (mn|P)");
   if (mixed_) {
-    r.B32.alloc(r.B.n);
+    if (r.B32.n != r.B.n) r.B32.alloc(r.B.n);
+    mark("B32 allocation");
     GANSU_HIP(launch_to_f32(r.B.ptr, r.B.n, r.B32.ptr, 0));
     DevBuf<unsigned> mx(1);
     GANSU_HIP(launch_rik_absmax(r.B32.ptr, r.B32.n, mx.ptr, 0));
@@ -196,6 +228,7 @@ void FockEngine::precompute_ri() {
     int e = 0;
     if (bmax > 0.0f) std::frexp(bmax, &e);
     r.sB = bmax > 0.0f ? std::ldexp(1.0f, 14 - e) : 1.0f;
+    mark("B32 copy and scale");
     r.Wprev.alloc(Ma);
     r.dW.alloc(Ma);
     r.Jprev.alloc((size_t)M * M);
@@ -212,13 +245,14 @@ void FockEngine::precompute_ri() {
   // K workspace: a P-chunk of T[m][P][l], at most ~4 GB
   const size_t budget = size_t(4) << 30;
   r.pchunk = (int)std::max<size_t>(1, std::min<size_t>(Ma, budget / ((size_t)M * M * sizeof(double))));
-  r.T.alloc((size_t)M * r.pchunk * M);
+  if (r.T.n != (size_t)M * r.pchunk * M) r.T.alloc((size_t)M * r.pchunk * M);
   {
     int nb, bs;
     ri_k_blocking(M, &nb, &bs);
     r.Kpart.alloc((size_t)(ri_k_max_split(bs, bs) + 1) * bs * bs);
   }
   GANSU_HIP(hipDeviceSynchronize());
+  mark("work buffers");
 }
 
 void FockEngine::build_ri(const double* d_D, double* d_J, double* d_K, const double* d_H, double* d_F,
@@ -302,9 +336,11 @@ void FockEngine::rik_prepare(int nocc) {
   RIState& r = *ri_;
   const int M = M_, Ma = Maux_;
   const size_t tn = (size_t)M * Ma * nocc;
-  if (r.TA.n != tn) {
-    r.TA.alloc(tn);
-    r.TE.alloc(tn);
+  if (r.dWrow.n != (size_t)M * Ma || r.TA.n != tn) {
+    if (r.TA.n != tn) {
+      r.TA.alloc(tn);
+      r.TE.alloc(tn);
+    }
     r.dWrow.alloc((size_t)M * Ma);
     r.wgmax.alloc(2 * rik_tstep_workgroups((long)M * Ma));
     r.Xprev.alloc((size_t)M * nocc);
@@ -316,7 +352,8 @@ void FockEngine::rik_prepare(int nocc) {
     const int kpad = (M + 31) / 32 * 32;
     r.Ehi.alloc((size_t)rik_cols(nocc) * kpad);
     r.Elo.alloc((size_t)rik_cols(nocc) * kpad);
-    r.Zpart.alloc((size_t)rik_chunks(M, (long)Ma * nocc) * M * M);
+    const size_t zn = (size_t)rik_chunks(M, (long)Ma * nocc) * M * M;
+    if (r.Zpart.n != zn) r.Zpart.alloc(zn);
   }
   r.rik_nocc = nocc;
 }

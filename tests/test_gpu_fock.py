@@ -331,6 +331,40 @@ def test_update_geometry_equals_fresh_engine(method, cuda):
     assert float((F2 - F1).abs().max()) > 1e-6  # the geometry really moved
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_ri_update_geometry_equals_fresh_engine(precision, cuda):
+    """RI engines keep B (and the FP32 copy, the increment state) across a
+    new geometry: the builds after update_geometry equal a fresh engine's,
+    for the mixed path over a sequence that starts with the FP64 build and
+    continues with increments on the reused buffers."""
+    Z, xyz = methane_water()
+    xyz2 = xyz + np.random.default_rng(5).normal(scale=0.05, size=xyz.shape)
+    mol, _ = both(Z, xyz, "cc-pvdz")
+    aux, _ = both(Z, xyz, "cc-pvdz-rifit")
+    mol2, _ = both(Z, xyz2, "cc-pvdz")
+    aux2, _ = both(Z, xyz2, "cc-pvdz-rifit")
+    M, nocc = mol.n_basis, mol.n_electrons // 2
+    H = _torch(np.zeros((M, M)), cuda)
+    kw = dict(eri_method="ri", ri_k="occupied") if precision == "fp64" else dict(eri_method="ri", precision="mixed")
+    seq = [_torch(C, cuda) for C in _orbital_sequence(M, nocc, 3, seed=6)]
+    dens = [(2.0 * C[:, :nocc] @ C[:, :nocc].T).contiguous() for C in seq]
+    eng = FockEngine(mol, aux=aux, **kw)
+    eng.precomputation()
+    for C, D in zip(seq, dens):
+        F1 = eng.compute_fock_matrix_occ(D, C, nocc, H).clone()
+    eng.update_geometry(xyz2)
+    fresh = FockEngine(mol2, aux=aux2, **kw)
+    fresh.precomputation()
+    for C, D in zip(seq, dens):
+        F2 = eng.compute_fock_matrix_occ(D, C, nocc, H)
+        Fr = fresh.compute_fock_matrix_occ(D, C, nocc, H)
+        assert float((F2 - Fr).abs().max()) <= 1e-12 * float(Fr.abs().max())
+    assert float((F2 - F1).abs().max()) > 1e-6  # the geometry really moved
+    if precision == "mixed":
+        # the counts restart with the new geometry's state
+        assert eng.stats()["ri_incremental_builds"] == fresh.stats()["ri_incremental_builds"] == 2
+
+
 @pytest.mark.parametrize("method", ["stored", "direct", "ri"])
 def test_rohf_fock_matches_oracle(method, cuda):
     """ROHF build (gpu_manager.cu:733-817, RI :1800-1961): F_c, F_o and the
