@@ -60,10 +60,11 @@ struct FockEngine::RIState {
   DevBuf<AuxPrim> prims;
   DevBuf<double> waux, L, B, W, J, K, T;
   DevBuf<double> Kpart;
-  // precision=mixed: incremental builds (ri_mfma.hip).  FP32 copy of B
-  // (scaled by sB for the fp16 split), the previous build's factor X_prev,
-  // K, J, W, and T_A, T_E (M Maux nocc each) with B X_prev = T_A + T_E / 2.
-  DevBuf<float> B32;
+  // precision=mixed: incremental builds (ri_mfma.hip).  B as fp16 hi/lo
+  // planes (B sB = hi + lo, the size of an FP32 copy), the previous build's
+  // factor X_prev, K, J, W, and T_A, T_E (M Maux nocc each) with
+  // B X_prev = T_A + T_E / 2.
+  DevBuf<uint16_t> Bp;
   float sB = 1.0f;
   int rik_nocc = 0;   // nocc of the held state (0: none)
   int rik_since = 0;  // incremental builds since the last FP64 build
@@ -125,7 +126,7 @@ void FockEngine::precompute_ri() {
   RIState& r = *ri_;
   if (old) {
     r.B = std::move(old->B);
-    r.B32 = std::move(old->B32);
+    r.Bp = std::move(old->Bp);
     r.TA = std::move(old->TA);
     r.TE = std::move(old->TE);
     r.T = std::move(old->T);
@@ -216,11 +217,12 @@ void FockEngine::precompute_ri() {
           "rocblas_dtrsm_strided_batched");
   mark("B = L^-1 (mn|P)");
   if (mixed_) {
-    if (r.B32.n != r.B.n) r.B32.alloc(r.B.n);
-    mark("B32 allocation");
-    GANSU_HIP(launch_to_f32(r.B.ptr, r.B.n, r.B32.ptr, 0));
+    const int kpad = rik_kpad(M);
+    const size_t np = (size_t)M * Ma * kpad * 2;
+    if (r.Bp.n != np) r.Bp.alloc(np);
+    mark("B planes allocation");
     DevBuf<unsigned> mx(1);
-    GANSU_HIP(launch_rik_absmax(r.B32.ptr, r.B32.n, mx.ptr, 0));
+    GANSU_HIP(launch_rik_absmax(r.B.ptr, r.B.n, mx.ptr, 0));
     unsigned bits = 0;
     GANSU_HIP(hipMemcpy(&bits, mx.ptr, sizeof bits, hipMemcpyDeviceToHost));
     float bmax;
@@ -228,7 +230,8 @@ void FockEngine::precompute_ri() {
     int e = 0;
     if (bmax > 0.0f) std::frexp(bmax, &e);
     r.sB = bmax > 0.0f ? std::ldexp(1.0f, 14 - e) : 1.0f;
-    mark("B32 copy and scale");
+    GANSU_HIP(launch_rik_planes(r.B.ptr, (long)M * Ma, M, kpad, r.sB, r.Bp.ptr, 0));
+    mark("B planes (fp16 hi/lo)");
     r.Wprev.alloc(Ma);
     r.dW.alloc(Ma);
     r.Jprev.alloc((size_t)M * M);
@@ -349,7 +352,7 @@ void FockEngine::rik_prepare(int nocc) {
     r.Y.alloc((size_t)nocc * nocc);
     r.Y2.alloc((size_t)nocc * nocc);
     r.G.alloc((size_t)nocc * nocc);
-    const int kpad = (M + 31) / 32 * 32;
+    const int kpad = rik_kpad(M);
     r.Ehi.alloc((size_t)rik_cols(nocc) * kpad);
     r.Elo.alloc((size_t)rik_cols(nocc) * kpad);
     const size_t zn = (size_t)rik_chunks(M, (long)Ma * nocc) * M * M;
@@ -372,7 +375,7 @@ int FockEngine::rik_chunks(int M, long Kd) {
 
 // One incremental or fresh build (ri_mfma.hip): align the new occupied factor to the
 // previous one (orthogonal Procrustes by Newton-Schulz), E = X_n R - X_prev,
-// then T_E = B E with the T_A / T_E update and dW, J = J_prev + B32 dW,
+// then T_E = B E with the T_A / T_E update and dW, J = J_prev + B dW,
 // K = K_prev + Z + Z^T with Z = T_A T_E^T.  Returns false (nothing changed)
 // when a full FP64 build is due: no state for this nocc, `rik_refresh`
 // increments since the last one, an alignment that did not converge or a
@@ -454,22 +457,23 @@ bool FockEngine::build_ri_incremental(const double* d_C, int nocc, double* J, do
     GANSU_HIP(launch_zero(r.Wprev.ptr, (size_t)Ma, s));
   }
 
-  const int kpad = (M + 31) / 32 * 32;
+  const int kpad = rik_kpad(M);
   prof_begin(s, 80021, 0.0, 0.0, 1.0);
   GANSU_HIP(launch_rik_split_e(r.Ework.ptr, M, nocc, kpad, r.rst.ptr, r.Ehi.ptr, r.Elo.ptr, s));
   prof_end(s);
   // T_E = B E (+ T_A update, dW rows): 2 R M nocc flops, three fp16 products each
   const long R = (long)M * Ma;
-  RikTArgs ta{r.B32.ptr, R, M, Ma, nocc, kpad, r.sB, r.Ehi.ptr, r.Elo.ptr, r.Ework.ptr, r.rst.ptr,
-              r.TA.ptr, r.TE.ptr, r.dWrow.ptr, fresh ? 1 : 0, r.wgmax.ptr};
-  prof_begin(s, 80022, double(R) * M * 4 + double(R) * nocc * 16, 2.0 * R * M * nocc * 3.0, (double)R);
+  RikTArgs ta{r.Bp.ptr, R, M, Ma, nocc, kpad, r.sB, r.Ehi.ptr, r.Elo.ptr, r.Ework.ptr, r.rst.ptr,
+              r.TA.ptr, r.TE.ptr, r.dWrow.ptr, fresh ? 1 : 0, r.wgmax.ptr, 0, 0};
+  prof_begin(s, 80022, double(R) * kpad * 4 + double(R) * nocc * (fresh ? 8 : 16), 2.0 * R * M * nocc * 3.0,
+             (double)R);
   GANSU_HIP(launch_rik_tstep(ta, s));
   prof_end(s);
   prof_begin(s, 80023, double(M) * Ma * 8, 0.0, (double)Ma);
   GANSU_HIP(launch_rik_dw(r.dWrow.ptr, M, Ma, r.dW.ptr, r.W.ptr, r.Wprev.ptr, s));
   prof_end(s);
-  prof_begin(s, 80024, double(M) * Ma * M * 4, 2.0 * M * Ma * M, (double)M);
-  GANSU_HIP(launch_rik_j(r.B32.ptr, r.dW.ptr, M, Ma, r.Jprev.ptr, J, s));
+  prof_begin(s, 80024, double(M) * Ma * kpad * 4, 2.0 * M * Ma * M, (double)M);
+  GANSU_HIP(launch_rik_j(r.Bp.ptr, r.sB, kpad, r.dW.ptr, M, Ma, r.Jprev.ptr, J, s));
   prof_end(s);
   // Z = T_A T_E^T over Kd = Maux nocc: 2 M^2 Kd flops, three fp16 products each
   const long Kd = (long)Ma * nocc;

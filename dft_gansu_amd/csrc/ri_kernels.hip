@@ -305,74 +305,10 @@ __global__ void k_sum_parts(const double* __restrict__ part, int nparts, int nr,
   *k = s;
 }
 
-// Incremental J state for precision=mixed, kept on the device so a build never
-// waits on the host.  st[0] = builds since the last FP64 J, st[1] = 1 when
-// this build takes the FP64 path.
-// dW = W - Wprev; the FP32 increment is used while max|dW| <= rel * max|W|
-// and fewer than `refresh` increments have been chained.
-__global__ void k_ri_j_mode(const double* __restrict__ W, const double* __restrict__ Wprev, int Maux, double rel,
-                            int refresh, int* st, double* __restrict__ dW) {
-  __shared__ double mw[256], md[256];
-  double a = 0.0, b = 0.0;
-  for (int p = threadIdx.x; p < Maux; p += blockDim.x) {
-    const double d = W[p] - Wprev[p];
-    dW[p] = d;
-    a = fmax(a, fabs(W[p]));
-    b = fmax(b, fabs(d));
-  }
-  mw[threadIdx.x] = a;
-  md[threadIdx.x] = b;
-  __syncthreads();
-  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
-    if ((int)threadIdx.x < o) {
-      mw[threadIdx.x] = fmax(mw[threadIdx.x], mw[threadIdx.x + o]);
-      md[threadIdx.x] = fmax(md[threadIdx.x], md[threadIdx.x + o]);
-    }
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    const bool full = st[0] < 0 || st[0] >= refresh || md[0] > rel * mw[0];
-    st[1] = full ? 1 : 0;
-    st[0] = full ? 0 : st[0] + 1;
-  }
-}
-
-// FP64 J (full) or Jprev + FP32-B increment, as st[1] selects; J -> Jprev.
-template <int FULL>
-__global__ void k_ri_j_step(const double* __restrict__ B, const float* __restrict__ B32,
-                            const double* __restrict__ W, const double* __restrict__ dW, int M, int Maux,
-                            const int* __restrict__ st, double* __restrict__ Jprev, double* __restrict__ J) {
-  if (st[1] != FULL) return;
-  const int m = blockIdx.x;
-  for (int n = threadIdx.x; n < M; n += blockDim.x) {
-    double s = 0.0;
-    if constexpr (FULL) {
-      const double* col = B + (size_t)m * Maux * M + n;
-      for (int P = 0; P < Maux; ++P) s += W[P] * col[(size_t)P * M];
-    } else {
-      const float* col = B32 + (size_t)m * Maux * M + n;
-      for (int P = 0; P < Maux; ++P) s += dW[P] * (double)col[(size_t)P * M];
-      s += Jprev[(size_t)m * M + n];
-    }
-    J[(size_t)m * M + n] = s;
-    Jprev[(size_t)m * M + n] = s;
-  }
-}
-
-__global__ void k_copy(const double* __restrict__ x, size_t n, double* __restrict__ y) {
-  const size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (id < n) y[id] = x[id];
-}
-
 __global__ void k_fock_ri(const double* __restrict__ H, const double* __restrict__ J, const double* __restrict__ K,
                           size_t n, double* __restrict__ F) {
   const size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (id < n) F[id] = H[id] + J[id] - 0.5 * K[id];
-}
-
-__global__ void k_to_f32(const double* __restrict__ x, size_t n, float* __restrict__ y) {
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
-    y[i] = (float)x[i];
 }
 
 }  // namespace
@@ -417,11 +353,6 @@ hipError_t launch_ri_j(const double* B, const double* W, int M, int Maux, double
   return hipGetLastError();
 }
 
-hipError_t launch_ri_j_f32(const float* B, const double* W, int M, int Maux, double* J, hipStream_t s) {
-  hipLaunchKernelGGL(k_ri_j<float>, dim3(M), dim3(256), 0, s, B, W, M, Maux, J);
-  return hipGetLastError();
-}
-
 hipError_t launch_ri_w_from_t(const double* T, const double* C, int M, int pc, int nocc, int p0, double* W,
                               hipStream_t s) {
   hipLaunchKernelGGL(k_ri_w_from_t, dim3(pc), dim3(256), 0, s, T, C, M, pc, nocc, p0, W);
@@ -434,28 +365,11 @@ hipError_t launch_mirror_upper(double* K, int M, hipStream_t s) {
   return hipGetLastError();
 }
 
-hipError_t launch_to_f32(const double* x, size_t n, float* y, hipStream_t s) {
-  size_t nb = (n + 255) / 256;
-  if (nb > 65536) nb = 65536;
-  hipLaunchKernelGGL(k_to_f32, dim3((unsigned)nb), dim3(256), 0, s, x, n, y);
-  return hipGetLastError();
-}
-
 hipError_t launch_sum_parts(const double* part, int nparts, int nr, int nc, int accumulate, double* K, int ldk,
                             hipStream_t s) {
   const size_t n = (size_t)nr * nc;
   hipLaunchKernelGGL(k_sum_parts, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, part, nparts, nr, nc, accumulate,
                      K, ldk);
-  return hipGetLastError();
-}
-
-hipError_t launch_ri_j_incremental(const double* B, const float* B32, const double* W, double* Wprev, double* dW,
-                                   int M, int Maux, double rel, int refresh, int* st, double* Jprev, double* J,
-                                   hipStream_t s) {
-  hipLaunchKernelGGL(k_ri_j_mode, dim3(1), dim3(256), 0, s, W, Wprev, Maux, rel, refresh, st, dW);
-  hipLaunchKernelGGL(k_ri_j_step<1>, dim3(M), dim3(256), 0, s, B, B32, W, dW, M, Maux, st, Jprev, J);
-  hipLaunchKernelGGL(k_ri_j_step<0>, dim3(M), dim3(256), 0, s, B, B32, W, dW, M, Maux, st, Jprev, J);
-  hipLaunchKernelGGL(k_copy, dim3((unsigned)((Maux + 255) / 256)), dim3(256), 0, s, W, (size_t)Maux, Wprev);
   return hipGetLastError();
 }
 

@@ -81,18 +81,11 @@ hipError_t launch_int2c2e(const RI2cArgs& a, hipStream_t s);
 hipError_t launch_int3c2e(const RI3cArgs& a, hipStream_t s);
 hipError_t launch_ri_w(const double* B, const double* D, int M, int Maux, double* W, hipStream_t s);
 hipError_t launch_ri_j(const double* B, const double* W, int M, int Maux, double* J, hipStream_t s);
-hipError_t launch_ri_j_f32(const float* B, const double* W, int M, int Maux, double* J, hipStream_t s);
 hipError_t launch_ri_w_from_t(const double* T, const double* C, int M, int pc, int nocc, int p0, double* W,
                               hipStream_t s);
 hipError_t launch_mirror_upper(double* K, int M, hipStream_t s);
-hipError_t launch_to_f32(const double* x, size_t n, float* y, hipStream_t s);
 hipError_t launch_sum_parts(const double* part, int nparts, int nr, int nc, int accumulate, double* K, int ldk,
                             hipStream_t s);
-// precision=mixed J: FP64 J when the fitted density moved by more than rel
-// (max-norm of W) or after `refresh` increments, else Jprev + FP32-B increment.
-hipError_t launch_ri_j_incremental(const double* B, const float* B32, const double* W, double* Wprev, double* dW,
-                                   int M, int Maux, double rel, int refresh, int* st, double* Jprev, double* J,
-                                   hipStream_t s);
 hipError_t launch_fock_ri(const double* H, const double* J, const double* K, size_t n, double* F, hipStream_t s);
 
 // ---- incremental RI exchange on the fp16 matrix cores (ri_mfma.hip) ----
@@ -102,10 +95,10 @@ struct RikState {
   unsigned ta_max, te_max;   // max|T_A|, max|T_E| (float bits, atomicMax)
 };
 struct RikTArgs {
-  const float* B32;   // R x M (rows m Maux + P)
+  const uint16_t* Bp; // fp16 hi/lo planes of B, [R][kpad / 8][hi 8 | lo 8] (rows m Maux + P)
   long R;
   int M, Maux, nocc, kpad;
-  float sB;           // power-of-two scale of B32
+  float sB;           // power-of-two scale of the planes (B sB = hi + lo)
   const uint16_t* Ehi;  // fp16 bits, E^T [rik_cols(nocc)][kpad]
   const uint16_t* Elo;
   const double* E;    // M x nocc, row-major
@@ -115,8 +108,12 @@ struct RikTArgs {
   double* dWrow;      // R
   int fresh;          // 1: the previous T_A, T_E count as zero (not read)
   unsigned* wgmax;    // 2 per workgroup: max|T_A|, max|T_E| (float bits)
+  long row0, wg0;     // first row and workgroup of one launch (set by launch_rik_tstep)
 };
 size_t rik_tstep_workgroups(long R);
+int rik_kpad(int M);  // k extent of the planes and of E^T: M rounded up to 64
+// Bp from the FP64 B (R x M): the hi/lo split of (float)B sB
+hipError_t launch_rik_planes(const double* B, long R, int M, int kpad, float sB, uint16_t* Bp, hipStream_t s);
 struct RikDkArgs {
   const float* TA;    // M x Kd (Kd = Maux nocc)
   const float* TE;
@@ -134,13 +131,14 @@ hipError_t launch_rik_tstep(const RikTArgs& a, hipStream_t s);
 hipError_t launch_rik_dk(const RikDkArgs& a, int nchunk, hipStream_t s);
 hipError_t launch_rik_reduce(const float* Zpart, int nchunk, int M, const double* Kprev, double* K, hipStream_t s);
 hipError_t launch_rik_dw(const double* dWrow, int M, int Maux, double* dW, double* W, double* Wprev, hipStream_t s);
-hipError_t launch_rik_j(const float* B32, const double* dW, int M, int Maux, double* Jprev, double* J, hipStream_t s);
+hipError_t launch_rik_j(const uint16_t* Bp, float sB, int kpad, const double* dW, int M, int Maux, double* Jprev,
+                        double* J, hipStream_t s);
 hipError_t launch_rik_seed(const double* T, int M, int pc, int nocc, int p0, int Maux, float* TA, float* TE,
                            hipStream_t s);
 hipError_t launch_rik_xocc(const double* C, int M, int nocc, double* X, hipStream_t s);
 // out[0] = |x|^2, out[1] = |y|^2 (n entries), out[2] = |G - I|^2 (G no x no)
 hipError_t launch_rik_norms(const double* x, const double* y, size_t n, const double* G, int no, double* out,
                             hipStream_t s);
-hipError_t launch_rik_absmax(const float* x, size_t n, unsigned* w, hipStream_t s);
+hipError_t launch_rik_absmax(const double* x, size_t n, unsigned* w, hipStream_t s);  // max |(float)x|
 
 }  // namespace gansu

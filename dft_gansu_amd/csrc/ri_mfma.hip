@@ -96,106 +96,110 @@ __global__ __launch_bounds__(1024) void k_rik_split_e(const double* __restrict__
   }
 }
 
-// ------------------------------------------------------------ T update -----
-// Rows r = m Maux + P of B32 (R x M, row-major).  A workgroup of 4 waves
-// takes 64 rows and 64 NBW columns of E: wave w computes rows 32 (w & 1) and
-// column half w >> 1 (NBW blocks of 32).  Per stage of 32 k, the E^T chunk
-// (all the workgroup's columns, fp16 hi/lo) sits in LDS; the next stage's
-// E^T pieces and B32 values are loaded into registers while the MFMAs run
-// and reach LDS after a barrier, so no wave waits on a global load inside
-// the stage.
-template <int NBW, bool VEC>
-__global__ __launch_bounds__(256) void k_rik_tstep(RikTArgs a) {
-  constexpr int NCOL = 64 * NBW;           // columns of the workgroup
-  constexpr int NPIECE = 2 * NCOL * 4;     // 16-byte pieces of one E^T stage
-  constexpr int NPT = NPIECE / 256;        // per thread (= 2 NBW)
-  __shared__ __attribute__((aligned(16))) _Float16 lds[2][NCOL][40];  // [hi/lo][col][k] (+8 pad)
-  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-  const int g = lane >> 5, li = lane & 31;
-  const int rg = wave & 1, ch = wave >> 1;
-  const long row0 = (long)blockIdx.x * 64 + rg * 32;
-  const long myrow = row0 + li;
-  const bool rowok = myrow < a.R;
-  const float* brow = a.B32 + (size_t)(rowok ? myrow : a.R - 1) * a.M;
-  const float sB = a.sB, sE = a.st->sE;
-  const int nk = a.kpad / 32;
+// ------------------------------------------------------------ B planes -----
+// B (R x M, FP64, rows m Maux + P) as the fp16 hi/lo split of its FP32
+// rounding scaled by sB, Bp[r][k / 8][hi 8 | lo 8] for k < kpad (zero for
+// k >= M): the 8 consecutive k of both planes that one MFMA lane takes are
+// one 32-byte piece.  Built once per geometry; the T update reads it as its
+// A operand and the J increment as B = (hi + lo) / sB (22 bits).
+__global__ void k_rik_planes(const double* __restrict__ B, long R, int M, int kpad, float sB,
+                             uint16_t* __restrict__ Bp_) {
+  _Float16* Bp = reinterpret_cast<_Float16*>(Bp_);
+  const int ng = kpad / 8;
+  const size_t n = (size_t)R * ng;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    const size_t r = i / ng;
+    const int k0 = (int)(i % ng) * 8;
+    h8 hi, lo;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      _Float16 h = 0, l = 0;
+      if (k0 + e < M) split1((float)B[r * M + k0 + e], sB, h, l);
+      hi[e] = h;
+      lo[e] = l;
+    }
+    *(h8*)(Bp + i * 16) = hi;
+    *(h8*)(Bp + i * 16 + 8) = lo;
+  }
+}
 
+// ------------------------------------------------------------ T update -----
+// Rows r = m Maux + P of B.  A workgroup of four waves takes 128 rows; each
+// wave 32 rows and all NBW 32-column blocks of E (16 NBW accumulators per
+// lane), so E^T is read once per 128 rows and each B row by one wave.  Per
+// 32-k stage: the wave's A operand (hi/lo planes of its rows) is loaded from
+// global memory two stages ahead into a two-slot register ring; E^T (fp16
+// hi/lo, all columns) goes through registers into one of two LDS buffers one
+// stage ahead, so one barrier per stage separates the writes of stage s + 1
+// from the reads of stage s - 1.  The loop body is unrolled over the two
+// buffer parities (nk is even), which keeps the ring in fixed registers.
+// TAIL: the last, partial tile (per-element row checks in the epilogue).
+constexpr int kTRows = 128;  // rows per workgroup
+
+template <int NBW, bool TAIL>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, NBW > 5 ? 1 : 8))) void k_rik_tstep(
+    RikTArgs a) {
+  constexpr int NCOL = NBW * 32;
+  constexpr int NPIECE = 2 * NCOL * 4;            // 16-byte pieces of one E^T stage
+  constexpr int NPT = (NPIECE + 255) / 256;        // per thread
+  __shared__ __attribute__((aligned(16))) _Float16 lds[2][2][NCOL][40];  // [buffer][hi/lo][col][k] (+8 pad)
+  const int t = threadIdx.x, lane = t & 63;
+  const int g = lane >> 5, li = lane & 31;
+  const int wv = __builtin_amdgcn_readfirstlane(t >> 6);
+  const long rw = a.row0 + (long)blockIdx.x * kTRows + wv * 32;  // first row of the wave
+  const int kq = a.kpad / 8;
+  const int nk = a.kpad / 32;
+  const long myrow = rw + li < a.R ? rw + li : a.R - 1;  // rows past R compute, never stored
+  const u32x4* arow = reinterpret_cast<const u32x4*>(a.Bp) + (size_t)myrow * kq * 2;
+  const float sB = a.sB, sE = a.st->sE;
   f16x acc[NBW];
 #pragma unroll
   for (int b = 0; b < NBW; ++b)
 #pragma unroll
     for (int r = 0; r < 16; ++r) acc[b][r] = 0.0f;
-
-  // E^T stage pieces of this thread: piece p = t + 256 j -> (hi/lo, column, 8-k quarter);
-  // the source offset without the stage term and the LDS slot are fixed
+  // E^T stage pieces of this thread: piece p = t + 256 j -> (hi/lo, column, 8-k quarter)
   u32x4 ereg[NPT];
 #define RIK_EPIECE(j) const int piece_ = t + 256 * (j), hl_ = piece_ / (NCOL * 4), rem_ = piece_ % (NCOL * 4)
 #define RIK_ESRC(kc) (*(const u32x4*)((hl_ ? a.Elo : a.Ehi) + (size_t)(rem_ >> 2) * a.kpad + (kc) * 32 + (rem_ & 3) * 8))
-#define RIK_EDST (*(u32x4*)&lds[hl_][rem_ >> 2][(rem_ & 3) * 8])
-  // B32 row pieces of a stage: 2 x 8 consecutive k per lane.  With VEC
-  // (M % 4 == 0) as four unconditional 16-byte loads from clamped addresses,
-  // masked in the split (areg_f), so the loop has no divergent branches and
-  // each wait covers only the loads it needs.
-  float areg[16];
-  float areg_f[4];
-  auto load_a = [&](int kc) {
-    const int k0 = kc * 32;
+#define RIK_EDST(buf) (*(u32x4*)&lds[buf][hl_][rem_ >> 2][(rem_ & 3) * 8])
+  auto eload = [&](int kc) {
 #pragma unroll
-    for (int ks = 0; ks < 2; ++ks) {
-      const int k = k0 + ks * 16 + g * 8;
-      if constexpr (VEC) {
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-          const int kk = k + 4 * h;
-          const int kcl = kk + 4 <= a.M ? kk : a.M - 4;
-          const float4 v = *(const float4*)(brow + kcl);
-          areg[ks * 8 + 4 * h + 0] = v.x;
-          areg[ks * 8 + 4 * h + 1] = v.y;
-          areg[ks * 8 + 4 * h + 2] = v.z;
-          areg[ks * 8 + 4 * h + 3] = v.w;
-          areg_f[ks * 2 + h] = (rowok && kk + 4 <= a.M) ? 1.0f : 0.0f;
-        }
-      } else {
-#pragma unroll
-        for (int e = 0; e < 8; ++e) areg[ks * 8 + e] = (rowok && k + e < a.M) ? brow[k + e] : 0.0f;
-#pragma unroll
-        for (int h = 0; h < 2; ++h) areg_f[ks * 2 + h] = 1.0f;
-      }
+    for (int j = 0; j < NPT; ++j) {
+      RIK_EPIECE(j);
+      if (NPIECE % 256 == 0 || piece_ < NPIECE) ereg[j] = RIK_ESRC(kc);
     }
   };
-
+  auto estore = [&](int buf) {
 #pragma unroll
-  for (int j = 0; j < NPT; ++j) {
-    RIK_EPIECE(j);
-    ereg[j] = RIK_ESRC(0);
-  }
-  load_a(0);
+    for (int j = 0; j < NPT; ++j) {
+      RIK_EPIECE(j);
+      if (NPIECE % 256 == 0 || piece_ < NPIECE) RIK_EDST(buf) = ereg[j];
+    }
+  };
+  // A operand ring: [slot][k-half][hi/lo]
+  u32x4 ap[2][2][2];
+  auto load_a = [&](int slot, int kc) {
 #pragma unroll
-  for (int j = 0; j < NPT; ++j) {
-    RIK_EPIECE(j);
-    RIK_EDST = ereg[j];
-  }
+    for (int ks = 0; ks < 2; ++ks) {
+      const u32x4* p = arow + (size_t)(kc * 4 + ks * 2 + g) * 2;
+      ap[slot][ks][0] = p[0];
+      ap[slot][ks][1] = p[1];
+    }
+  };
+  eload(0);
+  load_a(0, 0);
+  load_a(1, 1);
+  estore(0);
+  eload(1);
   __syncthreads();
-  for (int kc = 0; kc < nk; ++kc) {
+  auto stage = [&]<int P>(int kc) {
     h8 ahi[2], alo[2];
 #pragma unroll
-    for (int ks = 0; ks < 2; ++ks)
-#pragma unroll
-      for (int e = 0; e < 8; ++e) {
-        _Float16 h, l;
-        split1(areg[ks * 8 + e], sB * areg_f[ks * 2 + (e >> 2)], h, l);
-        ahi[ks][e] = h;
-        alo[ks][e] = l;
-      }
-    const bool more = kc + 1 < nk;
-    if (more) {
-#pragma unroll
-      for (int j = 0; j < NPT; ++j) {
-        RIK_EPIECE(j);
-        ereg[j] = RIK_ESRC(kc + 1);
-      }
-      load_a(kc + 1);
+    for (int ks = 0; ks < 2; ++ks) {
+      ahi[ks] = __builtin_bit_cast(h8, ap[P][ks][0]);
+      alo[ks] = __builtin_bit_cast(h8, ap[P][ks][1]);
     }
+    if (kc + 2 < nk) load_a(P, kc + 2);
     // the three products of a block are dependent: issue them for all
     // blocks in turn, so that NBW independent accumulators separate them
 #pragma unroll
@@ -203,9 +207,8 @@ __global__ __launch_bounds__(256) void k_rik_tstep(RikTArgs a) {
       h8 bhi[NBW], blo[NBW];
 #pragma unroll
       for (int b = 0; b < NBW; ++b) {
-        const int col = ch * 32 * NBW + b * 32 + li;
-        bhi[b] = *(const h8*)&lds[0][col][ks * 16 + g * 8];
-        blo[b] = *(const h8*)&lds[1][col][ks * 16 + g * 8];
+        bhi[b] = *(const h8*)&lds[P][0][b * 32 + li][ks * 16 + g * 8];
+        blo[b] = *(const h8*)&lds[P][1][b * 32 + li][ks * 16 + g * 8];
       }
 #pragma unroll
       for (int b = 0; b < NBW; ++b) acc[b] = mfma(ahi[ks], bhi[b], acc[b]);
@@ -214,74 +217,116 @@ __global__ __launch_bounds__(256) void k_rik_tstep(RikTArgs a) {
 #pragma unroll
       for (int b = 0; b < NBW; ++b) acc[b] = mfma(alo[ks], bhi[b], acc[b]);
     }
-    if (more) {
-      __syncthreads();
-#pragma unroll
-      for (int j = 0; j < NPT; ++j) {
-        RIK_EPIECE(j);
-        RIK_EDST = ereg[j];
-      }
-      __syncthreads();
+    if (kc + 1 < nk) {
+      estore(P ^ 1);
+      if (kc + 2 < nk) eload(kc + 2);
     }
+    __syncthreads();
+  };
+  for (int kc = 0; kc < nk; kc += 2) {
+    stage.template operator()<0>(kc);
+    stage.template operator()<1>(kc + 1);
   }
 #undef RIK_EPIECE
 #undef RIK_ESRC
 #undef RIK_EDST
 
   // epilogue: T_E, T_A = T_prev + T_E / 2 (T_prev = T_A_old + T_E_old / 2),
-  // row sums of T_A E for dW, maxima for the next product's scales
+  // dW rows = 2 sum_i T_A E (each row belongs to one wave: stored, not
+  // accumulated), maxima for the next product's scales.  The rows of a wave
+  // span at most two m (32 <= Maux), so E is read for m_A and m_A + 1 only.
   const float inv = 1.0f / (sB * sE);
+  float mta = 0.0f, mte = 0.0f;
+  const int nocc = a.nocc;
+  const long nA = rw / a.Maux, nbnd = (nA + 1) * a.Maux;
+  const long nB = nA + 1 < a.M ? nA + 1 : nA;
   float part[16];
 #pragma unroll
   for (int r = 0; r < 16; ++r) part[r] = 0.0f;
-  float mta = 0.0f, mte = 0.0f;
+  if constexpr (!TAIL) {
+    // wave-uniform row base, per-lane 32-bit byte offsets; all 32 loads of
+    // a block are issued before its stores
+    const unsigned nb4 = (unsigned)nocc * 4u;
+    const unsigned lane_off = (unsigned)(4 * g) * nb4 + (unsigned)li * 4u;
+    unsigned offr[16];
 #pragma unroll
-  for (int b = 0; b < NBW; ++b) {
-    const int c = ch * 32 * NBW + b * 32 + li;
-    if (c >= a.nocc) continue;
+    for (int r = 0; r < 16; ++r) offr[r] = lane_off + (unsigned)(8 * (r >> 2) + (r & 3)) * nb4;
+    char* bA = (char*)(a.TA + (size_t)rw * nocc);
+    char* bE = (char*)(a.TE + (size_t)rw * nocc);
+    const double* eAp = a.E + (size_t)nA * nocc + li;
+    const double* eBp = a.E + (size_t)nB * nocc + li;
 #pragma unroll
-    for (int r = 0; r < 16; ++r) {
-      const long row = row0 + 8 * (r >> 2) + 4 * g + (r & 3);
-      if (row >= a.R) continue;
-      const size_t idx = (size_t)row * a.nocc + c;
-      const float te = acc[b][r] * inv;
-      const float tp = a.fresh ? 0.0f : a.TA[idx] + 0.5f * a.TE[idx];
-      const float ta = tp + 0.5f * te;
-      a.TA[idx] = ta;
-      a.TE[idx] = te;
-      mta = fmaxf(mta, fabsf(ta));
-      mte = fmaxf(mte, fabsf(te));
-      const int n = (int)(row / a.Maux);
-      part[r] += ta * (float)a.E[(size_t)n * a.nocc + c];
+    for (int b = 0; b < NBW; ++b) {
+      if (b * 32 + li < nocc) {
+        const float eA = (float)eAp[b * 32], eB = (float)eBp[b * 32];
+        float tao[16], teo[16];
+        if (!a.fresh) {
+#pragma unroll
+          for (int r = 0; r < 16; ++r) {
+            tao[r] = *(const float*)(bA + offr[r] + b * 128);
+            teo[r] = *(const float*)(bE + offr[r] + b * 128);
+          }
+        }
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          const long row = rw + 4 * g + 8 * (r >> 2) + (r & 3);
+          const float te = acc[b][r] * inv;
+          const float tp = a.fresh ? 0.0f : tao[r] + 0.5f * teo[r];
+          const float ta = tp + 0.5f * te;
+          *(float*)(bA + offr[r] + b * 128) = ta;
+          *(float*)(bE + offr[r] + b * 128) = te;
+          mta = fmaxf(mta, fabsf(ta));
+          mte = fmaxf(mte, fabsf(te));
+          part[r] += ta * (row < nbnd ? eA : eB);
+        }
+      }
+      __asm__ volatile("" ::: "memory");  // one block's loads in flight at a time
+    }
+  } else {
+#pragma unroll
+    for (int b = 0; b < NBW; ++b) {
+      const int c = b * 32 + li;
+      if (c >= nocc) continue;
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const long row = rw + 4 * g + 8 * (r >> 2) + (r & 3);
+        if (row >= a.R) continue;
+        const size_t idx = (size_t)row * nocc + c;
+        const float te = acc[b][r] * inv;
+        const float tp = a.fresh ? 0.0f : a.TA[idx] + 0.5f * a.TE[idx];
+        const float ta = tp + 0.5f * te;
+        a.TA[idx] = ta;
+        a.TE[idx] = te;
+        mta = fmaxf(mta, fabsf(ta));
+        mte = fmaxf(mte, fabsf(te));
+        part[r] += ta * (float)a.E[(size_t)(row < nbnd ? nA : nB) * nocc + c];
+      }
     }
   }
-  // rows are shared by the 32 lanes of a k-group (and by the two column
-  // halves): reduce over the lanes, add the halves through dWrow atomics
 #pragma unroll
   for (int r = 0; r < 16; ++r) {
     float v = part[r];
 #pragma unroll
     for (int o = 16; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    const long row = row0 + 8 * (r >> 2) + 4 * g + (r & 3);
-    if (li == 0 && row < a.R) atomicAdd(&a.dWrow[row], 2.0 * (double)v);
+    const long row = rw + 8 * (r >> 2) + 4 * g + (r & 3);
+    if (li == 0 && (!TAIL || row < a.R)) a.dWrow[row] = 2.0 * (double)v;
   }
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) {
     mta = fmaxf(mta, __shfl_xor(mta, o, 64));
     mte = fmaxf(mte, __shfl_xor(mte, o, 64));
   }
-  // the workgroup's maxima go to its own slot (k_rik_maxred reduces them):
-  // one atomic per wave on a single word serialised every workgroup
+  // the workgroup's maxima go to its own slot (k_rik_maxred reduces them)
   __shared__ float wmax[2][4];
   if (lane == 0) {
-    wmax[0][wave] = mta;
-    wmax[1][wave] = mte;
+    wmax[0][wv] = mta;
+    wmax[1][wv] = mte;
   }
   __syncthreads();
   if (t == 0) {
-    a.wgmax[2 * blockIdx.x] = __float_as_uint(fmaxf(fmaxf(wmax[0][0], wmax[0][1]), fmaxf(wmax[0][2], wmax[0][3])));
-    a.wgmax[2 * blockIdx.x + 1] =
-        __float_as_uint(fmaxf(fmaxf(wmax[1][0], wmax[1][1]), fmaxf(wmax[1][2], wmax[1][3])));
+    const size_t w = (size_t)a.wg0 + blockIdx.x;
+    a.wgmax[2 * w] = __float_as_uint(fmaxf(fmaxf(wmax[0][0], wmax[0][1]), fmaxf(wmax[0][2], wmax[0][3])));
+    a.wgmax[2 * w + 1] = __float_as_uint(fmaxf(fmaxf(wmax[1][0], wmax[1][1]), fmaxf(wmax[1][2], wmax[1][3])));
   }
 }
 
@@ -527,61 +572,59 @@ __global__ void k_rik_dw(const double* __restrict__ dWrow, int M, int Maux, doub
   Wprev[P] = w;
 }
 
-// J[m][n] = Jprev[m][n] + sum_P dW_P B32[m][P][n]; Jprev <- J.  A workgroup
-// takes one m and 256 consecutive n: each lane four columns (16-byte loads
-// when M % 4 == 0), the four waves a quarter of P each, four P per round in
-// flight; the wave partial sums meet in LDS in wave order.
-__global__ __launch_bounds__(256) void k_rik_j(const float* __restrict__ B32, const double* __restrict__ dW, int M,
-                                               int Maux, double* __restrict__ Jprev, double* __restrict__ J) {
-  __shared__ double red[3][64][4];
+// J[m][n] = Jprev[m][n] + sum_P dW_P B[m][P][n]; Jprev <- J, with B from
+// the planes as (hi + lo) / sB (exact sums in FP64; the 1 / sB power of two
+// is applied to the total).  A workgroup takes one m and 64 consecutive
+// 8-column groups: each lane one group (two 16-byte loads per P), the four
+// waves a quarter of P each, four P per round in flight; the wave partial
+// sums meet in LDS in wave order.
+__global__ __launch_bounds__(256) void k_rik_j(const uint16_t* __restrict__ Bp, float sB, int kpad,
+                                               const double* __restrict__ dW, int M, int Maux,
+                                               double* __restrict__ Jprev, double* __restrict__ J) {
+  __shared__ double red[3][64][8];
   const int m = blockIdx.x, w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int n0 = blockIdx.y * 256 + lane * 4;
+  const int ng = kpad / 8;
+  const int q = blockIdx.y * 64 + lane;  // 8-column group
   const int P0 = (int)((long)w * Maux / 4), P1 = (int)((long)(w + 1) * Maux / 4);
-  const float* slab = B32 + (size_t)m * Maux * M;
-  double s[4] = {0.0, 0.0, 0.0, 0.0};
-  if (n0 < M) {
-    if ((M & 3) == 0) {
-      int P = P0;
-      for (; P + 4 <= P1; P += 4) {
-        float4 v[4];
+  const h8* slab = reinterpret_cast<const h8*>(Bp) + ((size_t)m * Maux * ng + q) * 2;
+  const size_t pstride = (size_t)ng * 2;
+  double s[8];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) v[u] = *(const float4*)(slab + (size_t)(P + u) * M + n0);
+  for (int e = 0; e < 8; ++e) s[e] = 0.0;
+  if (q < ng) {
+    int P = P0;
+    for (; P + 4 <= P1; P += 4) {
+      h8 hi[4], lo[4];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const double d = dW[P + u];
-          s[0] += d * (double)v[u].x;
-          s[1] += d * (double)v[u].y;
-          s[2] += d * (double)v[u].z;
-          s[3] += d * (double)v[u].w;
-        }
+      for (int u = 0; u < 4; ++u) {
+        hi[u] = slab[(size_t)(P + u) * pstride];
+        lo[u] = slab[(size_t)(P + u) * pstride + 1];
       }
-      for (; P < P1; ++P) {
-        const float4 v = *(const float4*)(slab + (size_t)P * M + n0);
-        const double d = dW[P];
-        s[0] += d * (double)v.x;
-        s[1] += d * (double)v.y;
-        s[2] += d * (double)v.z;
-        s[3] += d * (double)v.w;
-      }
-    } else {
-      for (int P = P0; P < P1; ++P) {
-        const double d = dW[P];
 #pragma unroll
-        for (int e = 0; e < 4; ++e)
-          if (n0 + e < M) s[e] += d * (double)slab[(size_t)P * M + n0 + e];
+      for (int u = 0; u < 4; ++u) {
+        const double d = dW[P + u];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) s[e] += d * ((double)hi[u][e] + (double)lo[u][e]);
       }
+    }
+    for (; P < P1; ++P) {
+      const h8 hi = slab[(size_t)P * pstride], lo = slab[(size_t)P * pstride + 1];
+      const double d = dW[P];
+#pragma unroll
+      for (int e = 0; e < 8; ++e) s[e] += d * ((double)hi[e] + (double)lo[e]);
     }
   }
   if (w > 0)
 #pragma unroll
-    for (int e = 0; e < 4; ++e) red[w - 1][lane][e] = s[e];
+    for (int e = 0; e < 8; ++e) red[w - 1][lane][e] = s[e];
   __syncthreads();
-  if (w != 0 || n0 >= M) return;
+  if (w != 0 || q >= ng) return;
+  const double inv = 1.0 / (double)sB;
 #pragma unroll
-  for (int e = 0; e < 4; ++e) {
-    const int n = n0 + e;
+  for (int e = 0; e < 8; ++e) {
+    const int n = q * 8 + e;
     if (n >= M) continue;
-    const double tot = ((s[e] + red[0][lane][e]) + red[1][lane][e]) + red[2][lane][e];
+    const double tot = (((s[e] + red[0][lane][e]) + red[1][lane][e]) + red[2][lane][e]) * inv;
     const size_t id = (size_t)m * M + n;
     const double v = Jprev[id] + tot;
     J[id] = v;
@@ -642,11 +685,11 @@ __global__ __launch_bounds__(1024) void k_rik_norms(const double* __restrict__ x
   }
 }
 
-// *w = max |x| as float bits (w zeroed by the caller)
-__global__ void k_rik_absmax(const float* __restrict__ x, size_t n, unsigned* w) {
+// *w = max |(float)x| as float bits (w zeroed by the caller)
+__global__ void k_rik_absmax(const double* __restrict__ x, size_t n, unsigned* w) {
   float m = 0.0f;
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
-    m = fmaxf(m, fabsf(x[i]));
+    m = fmaxf(m, fabsf((float)x[i]));
   for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
   if ((threadIdx.x & 63) == 0) atomic_max_pos(w, m);
 }
@@ -660,15 +703,17 @@ __global__ void k_rik_reset(RikState* st) {
 
 template <int NBW>
 hipError_t launch_tstep_nb(const RikTArgs& a, hipStream_t s) {
-  const unsigned grid = (unsigned)((a.R + 63) / 64);  // rik_tstep_workgroups
-  if ((a.M & 3) == 0) hipLaunchKernelGGL((k_rik_tstep<NBW, true>), dim3(grid), dim3(256), 0, s, a);
-  else hipLaunchKernelGGL((k_rik_tstep<NBW, false>), dim3(grid), dim3(256), 0, s, a);
+  const long full = a.R / kTRows;
+  RikTArgs b = a;
+  b.row0 = 0;
+  b.wg0 = 0;
+  if (full > 0) hipLaunchKernelGGL((k_rik_tstep<NBW, false>), dim3((unsigned)full), dim3(256), 0, s, b);
+  if (a.R % kTRows) {
+    b.row0 = full * kTRows;
+    b.wg0 = full;
+    hipLaunchKernelGGL((k_rik_tstep<NBW, true>), dim3(1), dim3(256), 0, s, b);
+  }
   return hipGetLastError();
-}
-
-__global__ void k_zero_d(double* p, size_t n) {
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
-    p[i] = 0.0;
 }
 
 }  // namespace
@@ -682,20 +727,32 @@ hipError_t launch_rik_split_e(const double* E, int M, int nocc, int kpad, RikSta
   return hipGetLastError();
 }
 
-size_t rik_tstep_workgroups(long R) { return (size_t)((R + 63) / 64); }
+size_t rik_tstep_workgroups(long R) { return (size_t)((R + kTRows - 1) / kTRows); }
+
+int rik_kpad(int M) { return (M + 63) / 64 * 64; }
+
+hipError_t launch_rik_planes(const double* B, long R, int M, int kpad, float sB, uint16_t* Bp, hipStream_t s) {
+  const size_t n = (size_t)R * (kpad / 8);
+  const unsigned grid = (unsigned)std::min<size_t>(65536, (n + 255) / 256);
+  hipLaunchKernelGGL(k_rik_planes, dim3(grid), dim3(256), 0, s, B, R, M, kpad, sB, Bp);
+  return hipGetLastError();
+}
 
 hipError_t launch_rik_tstep(const RikTArgs& a, hipStream_t s) {
+  if (a.kpad % 64 != 0 || a.kpad < a.M || a.R <= 0) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_rik_reset, dim3(1), dim3(1), 0, s, a.st);
-  // dW rows gather the two column halves by atomics
-  const unsigned zg = (unsigned)std::min<long>(8192, (a.R + 255) / 256);
-  hipLaunchKernelGGL(k_zero_d, dim3(zg), dim3(256), 0, s, a.dWrow, (size_t)a.R);
   hipError_t e = hipErrorInvalidValue;
-  switch ((rik_blocks(a.nocc) + 1) / 2) {  // 32-column blocks per wave
+  switch (rik_blocks(a.nocc)) {  // 32-column blocks of E (all in each wave)
     case 1: e = launch_tstep_nb<1>(a, s); break;
     case 2: e = launch_tstep_nb<2>(a, s); break;
     case 3: e = launch_tstep_nb<3>(a, s); break;
     case 4: e = launch_tstep_nb<4>(a, s); break;
     case 5: e = launch_tstep_nb<5>(a, s); break;
+    case 6: e = launch_tstep_nb<6>(a, s); break;
+    case 7: e = launch_tstep_nb<7>(a, s); break;
+    case 8: e = launch_tstep_nb<8>(a, s); break;
+    case 9: e = launch_tstep_nb<9>(a, s); break;
+    case 10: e = launch_tstep_nb<10>(a, s); break;
     default: break;
   }
   if (e != hipSuccess) return e;
@@ -723,10 +780,11 @@ hipError_t launch_rik_dw(const double* dWrow, int M, int Maux, double* dW, doubl
   return hipGetLastError();
 }
 
-hipError_t launch_rik_j(const float* B32, const double* dW, int M, int Maux, double* Jprev, double* J,
-                        hipStream_t s) {
-  hipLaunchKernelGGL(k_rik_j, dim3((unsigned)M, (unsigned)((M + 255) / 256)), dim3(256), 0, s, B32, dW, M, Maux,
-                     Jprev, J);
+hipError_t launch_rik_j(const uint16_t* Bp, float sB, int kpad, const double* dW, int M, int Maux, double* Jprev,
+                        double* J, hipStream_t s) {
+  const int ng = kpad / 8;
+  hipLaunchKernelGGL(k_rik_j, dim3((unsigned)M, (unsigned)((ng + 63) / 64)), dim3(256), 0, s, Bp, sB, kpad, dW, M,
+                     Maux, Jprev, J);
   return hipGetLastError();
 }
 
@@ -750,7 +808,7 @@ hipError_t launch_rik_norms(const double* x, const double* y, size_t n, const do
   return hipGetLastError();
 }
 
-hipError_t launch_rik_absmax(const float* x, size_t n, unsigned* w, hipStream_t s) {
+hipError_t launch_rik_absmax(const double* x, size_t n, unsigned* w, hipStream_t s) {
   hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, s, w, 0u);
   const unsigned grid = (unsigned)std::min<size_t>(8192, (n + 255) / 256);
   hipLaunchKernelGGL(k_rik_absmax, dim3(grid), dim3(256), 0, s, x, n, w);
