@@ -38,14 +38,14 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
 MEASURED_TRAFFIC = {
     ("C10H22/cc-pVDZ", "fock_slab(pp|ps) stored-digest"): 274.7e6,
     # profiles/ri_mixed_64water_ccpvdz.md (FETCH_SIZE x2.0, WRITE_SIZE x1.0 by a 2 GiB copy)
-    ("64 waters", "k_rik_tstep (T_E = B E, fp16 MFMA)"): 126.4e9,
-    ("64 waters", "k_rik_dk (Z = T_A T_E^T, fp16 MFMA)"): 217.1e9,
+    ("64 waters", "k_rik_dk (Z = T_A T_E^T, fp16 MFMA)"): 216.0e9,
+    ("64 waters", "k_rik_tstep (T_E = B E, fp16 MFMA)"): 118.4e9,
 }
 TRAFFIC_SOURCE = "stored constant: rocprofv3 FETCH_SIZE / WRITE_SIZE per launch, scaled by a 2 GiB copy (profiles/)"
 FP16_MFMA_PEAK_TFS = 2500.0  # MI355X dense fp16 MFMA, TFLOP/s
 # kernel ids of the engine's profile rows (incremental RI build)
 RIK_NAMES = {80020: "align (Procrustes)", 80021: "split E", 80022: "k_rik_tstep (T_E = B E, fp16 MFMA)",
-             80023: "dW", 80024: "J increment (FP32 B)", 80025: "k_rik_dk (Z = T_A T_E^T, fp16 MFMA)",
+             80023: "dW", 80024: "J increment (B planes)", 80025: "k_rik_dk (Z = T_A T_E^T, fp16 MFMA)",
              80026: "K reduce (FP64)", 80005: "T=B*X dgemm", 80007: "W from T", 80008: "K=T*T^T dgemm",
              80002: "J (FP64 B)"}
 
@@ -263,7 +263,7 @@ def ri_leg(log_fn, mixed_rate, no_scf=False):
             e.set_profiling(False)
             phases = {}
             for kid, label, unit in ((80005, "T=B*X dgemm", "TFLOP/s"), (80008, "K=T*T^T dgemm", "TFLOP/s"),
-                                     (80012, "J incremental (FP32 B)", "GB/s"), (80002, "J (FP64 B)", "GB/s")):
+                                     (80002, "J (FP64 B)", "GB/s")):
                 sel = rows[rows[:, 0] == kid]
                 if len(sel) == 0:
                     continue
