@@ -367,8 +367,8 @@ void FockEngine::rik_prepare(int nocc) {
 // (a two-level FP32 sum inside the kernel was as accurate and, at one wave
 // per SIMD, half as fast)
 int FockEngine::rik_chunks(int M, long Kd) {
-  const long nt = (M + 127) / 128;
-  const long fill = (4096 + nt * nt - 1) / (nt * nt);
+  const long nt2 = rik_dk_tiles(M);
+  const long fill = (4096 + nt2 - 1) / nt2;
   const long acc = (Kd + 4095) / 4096;
   return (int)std::max<long>(1, std::min<long>({std::max(fill, acc), 1024, std::max<long>(1, Kd / 32)}));
 }

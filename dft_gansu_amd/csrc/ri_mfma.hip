@@ -356,73 +356,76 @@ __global__ __launch_bounds__(1024) void k_rik_maxred(const unsigned* __restrict_
 
 // --------------------------------------------------------------- dK -------
 // Zpart[chunk][m][n] = sum over the chunk's k of T_A[m][k] T_E[n][k]
-// (K dimension Kd = Maux nocc).  Workgroup tile 128 x 128, four waves of
-// 64 x 64 (2 x 2 MFMA blocks), 32-k stages.  Stages are loaded into
-// registers two ahead of the one the MFMAs use (two register sets) and
-// split into fp16 hi/lo on their way into LDS, which holds two stages, so
-// one barrier per stage separates the writes of stage s + 1 from the reads
-// of stage s - 1.
-constexpr int kDkTile = 128;
-
-struct DkStage {
-  float4 a[4], b[4];
-  float fa[4], fb[4];  // 1 for loads inside the matrices, 0 for clamped ones
-};
+// (K dimension Kd = Maux nocc).  Workgroup tile 256 x 128 (T_A rows x T_E
+// rows), eight waves of 64 x 64 (2 x 2 MFMA blocks), 32-k stages: the T_A
+// panel is re-read by 13 column tiles and the T_E panel by 7 row tiles at
+// M = 1600 (26 panel reads per pair with 128 x 128 tiles, 20 here).  Stages
+// are loaded into registers two ahead of the one the MFMAs use (two register
+// sets) and split into fp16 hi/lo on their way into LDS, which holds two
+// stages, so one barrier per stage separates the writes of stage s + 1 from
+// the reads of stage s - 1.
+constexpr int kDkWM = 4, kDkWN = 2;                 // waves along T_A and T_E rows
+constexpr int kDkTM = 64 * kDkWM, kDkTN = 64 * kDkWN;  // tile rows of T_A, T_E
+constexpr int kDkThreads = 64 * kDkWM * kDkWN;
 
 template <bool VEC>
-__global__ __launch_bounds__(256) void k_rik_dk(RikDkArgs a) {
-  __shared__ __attribute__((aligned(16))) _Float16 sa[2][2][kDkTile][40];  // [buf][hi/lo][row][k]
-  __shared__ __attribute__((aligned(16))) _Float16 sb[2][2][kDkTile][40];
+__global__ __launch_bounds__(kDkThreads) void k_rik_dk(RikDkArgs a) {
+  constexpr int NT = kDkThreads, NA = kDkTM * 8 / NT, NB = kDkTN * 8 / NT;  // float4 per thread and stage
+  static_assert(NA * NT == kDkTM * 8 && NB * NT == kDkTN * 8, "staging");
+  __shared__ __attribute__((aligned(16))) _Float16 sa[2][2][kDkTM][40];  // [buf][hi/lo][row][k]
+  __shared__ __attribute__((aligned(16))) _Float16 sb[2][2][kDkTN][40];
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int g = lane >> 5, li = lane & 31;
-  const int wm = wave >> 1, wn = wave & 1;
-  const int ntile = (a.M + kDkTile - 1) / kDkTile;
+  const int wm = wave / kDkWN, wn = wave % kDkWN;
+  const int ntm = (a.M + kDkTM - 1) / kDkTM, ntn = (a.M + kDkTN - 1) / kDkTN;
   // Workgroups reach the eight XCDs round-robin (XCD = linear id % 8): XCD x
   // walks the tiles of chunks x, x + 8, ... in row-major tile order, so the
   // workgroups resident on one XCD share the k stages of one chunk in its L2.
-  const int nt2 = ntile * ntile;
+  const int nt2 = ntm * ntn;
   const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
   const int chunk = (slot / nt2) * 8 + xcd, tile = slot % nt2;
   if (chunk >= a.nchunk) return;
-  const int tm = tile / ntile, tn = tile % ntile;
+  const int tm = tile / ntn, tn = tile % ntn;
   const long k_begin = (long)chunk * a.kchunk;
   const long k_end = k_begin + a.kchunk < a.Kd ? k_begin + a.kchunk : a.Kd;
   const int nst = k_end > k_begin ? (int)((k_end - k_begin + 31) / 32) : 0;
   const float sA = op_scale(__uint_as_float(a.st->ta_max));
   const float sE = op_scale(__uint_as_float(a.st->te_max));
 
-  // a stage: 128 rows x 32 k of T_A (A side) and of T_E (B side), 1024
-  // float4 each, 4 per thread; row r = idx / 8, k offset 4 (idx % 8).  With
-  // VEC (Kd % 4 == 0, so chunk ends are multiples of 4) the loads are
+  // a stage: 32 k of the tile's T_A rows (A side) and T_E rows (B side) as
+  // float4; piece idx -> row idx / 8, k offset 4 (idx % 8).  With VEC
+  // (Kd % 4 == 0, so chunk ends are multiples of 4) the loads are
   // unconditional from clamped addresses and masked afterwards, which keeps
   // the loop free of divergent branches (and the load counters exact).
-  auto fetch = [&](int s, DkStage& v) {
+  struct Stage {
+    float4 a[NA], b[NB];
+    float fa[NA], fb[NB];  // 1 for loads inside the matrices, 0 for clamped ones
+  };
+  auto load1 = [&](const float* base, int row, long k, float4& v, float& f) {
+    if constexpr (VEC) {
+      const long kc = k + 4 <= k_end ? k : k_end - 4;
+      const int rc = row < a.M ? row : a.M - 1;
+      v = *(const float4*)(base + (size_t)rc * a.Kd + kc);
+      f = (row < a.M && k < k_end) ? 1.0f : 0.0f;  // applied in stash, once the load has landed
+    } else {
+      float x[4] = {0.f, 0.f, 0.f, 0.f};
+      for (int e = 0; e < 4; ++e)
+        if (k + e < k_end && row < a.M) x[e] = base[(size_t)row * a.Kd + k + e];
+      v = make_float4(x[0], x[1], x[2], x[3]);
+      f = 1.0f;
+    }
+  };
+  auto fetch = [&](int s, Stage& v) {
     const long k0 = k_begin + (long)s * 32;
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int idx = t + 256 * j, r = idx >> 3, q = idx & 7;
-      const long k = k0 + q * 4;
-      const int ra = tm * kDkTile + r, rb = tn * kDkTile + r;
-      if constexpr (VEC) {
-        const long kc = k + 4 <= k_end ? k : k_end - 4;
-        const int rac = ra < a.M ? ra : a.M - 1, rbc = rb < a.M ? rb : a.M - 1;
-        v.a[j] = *(const float4*)(a.TA + (size_t)rac * a.Kd + kc);
-        v.b[j] = *(const float4*)(a.TE + (size_t)rbc * a.Kd + kc);
-        // the masks are applied in stash, once the loads have landed
-        v.fa[j] = (ra < a.M && k < k_end) ? 1.0f : 0.0f;
-        v.fb[j] = (rb < a.M && k < k_end) ? 1.0f : 0.0f;
-      } else {
-        float xa[4] = {0.f, 0.f, 0.f, 0.f}, yb[4] = {0.f, 0.f, 0.f, 0.f};
-        for (int e = 0; e < 4; ++e)
-          if (k + e < k_end) {
-            if (ra < a.M) xa[e] = a.TA[(size_t)ra * a.Kd + k + e];
-            if (rb < a.M) yb[e] = a.TE[(size_t)rb * a.Kd + k + e];
-          }
-        v.a[j] = make_float4(xa[0], xa[1], xa[2], xa[3]);
-        v.b[j] = make_float4(yb[0], yb[1], yb[2], yb[3]);
-        v.fa[j] = 1.0f;
-        v.fb[j] = 1.0f;
-      }
+    for (int j = 0; j < NA; ++j) {
+      const int idx = t + NT * j;
+      load1(a.TA, tm * kDkTM + (idx >> 3), k0 + (idx & 7) * 4, v.a[j], v.fa[j]);
+    }
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+      const int idx = t + NT * j;
+      load1(a.TE, tn * kDkTN + (idx >> 3), k0 + (idx & 7) * 4, v.b[j], v.fb[j]);
     }
   };
   // split four floats into hi/lo halves and store each as one 8-byte word
@@ -440,11 +443,15 @@ __global__ __launch_bounds__(256) void k_rik_dk(RikDkArgs a) {
     *(h4*)hi = h;
     *(h4*)lo = l;
   };
-  auto stash = [&](const DkStage& v, int buf) {
+  auto stash = [&](const Stage& v, int buf) {
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int idx = t + 256 * j, r = idx >> 3, q = idx & 7;
+    for (int j = 0; j < NA; ++j) {
+      const int idx = t + NT * j, r = idx >> 3, q = idx & 7;
       put4(v.a[j], sA * v.fa[j], &sa[buf][0][r][q * 4], &sa[buf][1][r][q * 4]);
+    }
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+      const int idx = t + NT * j, r = idx >> 3, q = idx & 7;
       put4(v.b[j], sE * v.fb[j], &sb[buf][0][r][q * 4], &sb[buf][1][r][q * 4]);
     }
   };
@@ -492,7 +499,7 @@ __global__ __launch_bounds__(256) void k_rik_dk(RikDkArgs a) {
   // the loop takes stages in pairs, and every fetch is issued (stages past
   // the end fetch the last one again), so the loop has no branches and each
   // stash waits only for its own stage's loads
-  DkStage r0, r1;
+  Stage r0, r1;
   const int last = nst - 1;
   if (nst > 0) {
     fetch(0, r0);
@@ -522,11 +529,11 @@ __global__ __launch_bounds__(256) void k_rik_dk(RikDkArgs a) {
   for (int i = 0; i < 2; ++i)
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
-      const int col = tn * kDkTile + wn * 64 + j * 32 + li;
+      const int col = tn * kDkTN + wn * 64 + j * 32 + li;
       if (col >= a.M) continue;
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
-        const int row = tm * kDkTile + wm * 64 + i * 32 + 8 * (r >> 2) + 4 * g + (r & 3);
+        const int row = tm * kDkTM + wm * 64 + i * 32 + 8 * (r >> 2) + 4 * g + (r & 3);
         if (row < a.M) Z[(size_t)row * a.M + col] = acc[i][j][r] * inv;
       }
     }
@@ -762,12 +769,14 @@ hipError_t launch_rik_tstep(const RikTArgs& a, hipStream_t s) {
 
 hipError_t launch_rik_dk(const RikDkArgs& a, int nchunk, hipStream_t s) {
   if (a.nchunk != nchunk) return hipErrorInvalidValue;
-  const int ntile = (a.M + kDkTile - 1) / kDkTile;
-  const long nwg = (long)ntile * ntile * ((nchunk + 7) / 8) * 8;
-  if ((a.Kd & 3) == 0) hipLaunchKernelGGL(k_rik_dk<true>, dim3((unsigned)nwg), dim3(256), 0, s, a);
-  else hipLaunchKernelGGL(k_rik_dk<false>, dim3((unsigned)nwg), dim3(256), 0, s, a);
+  const long nt2 = (long)((a.M + kDkTM - 1) / kDkTM) * ((a.M + kDkTN - 1) / kDkTN);
+  const long nwg = nt2 * ((nchunk + 7) / 8) * 8;
+  if ((a.Kd & 3) == 0) hipLaunchKernelGGL(k_rik_dk<true>, dim3((unsigned)nwg), dim3(kDkThreads), 0, s, a);
+  else hipLaunchKernelGGL(k_rik_dk<false>, dim3((unsigned)nwg), dim3(kDkThreads), 0, s, a);
   return hipGetLastError();
 }
+
+int rik_dk_tiles(int M) { return ((M + kDkTM - 1) / kDkTM) * ((M + kDkTN - 1) / kDkTN); }
 
 hipError_t launch_rik_reduce(const float* Zpart, int nchunk, int M, const double* Kprev, double* K, hipStream_t s) {
   const int nt = (M + 31) / 32;
