@@ -537,6 +537,18 @@ def stored_leg(cfg, method, precision, steps, warmup, no_direct, no_scf, no_mixe
         out["direct_fock_builds_per_s"] = round(nd / (time.perf_counter() - t0), 3)
         log_fn(f"[bench] direct build rate {out['direct_fock_builds_per_s']} /s")
         del e2
+        if not no_scf:
+            # direct SCF with full builds and with delta-density builds (a full one every 8)
+            dscf = {}
+            for name, kw in (("full_builds", {}), ("incremental_fock_8", dict(incremental_fock=8))):
+                t0 = time.perf_counter()
+                rd = rhf(mol, Parameters(eri_method="direct", precision=precision, convergence_energy_threshold=1e-8,
+                                         maxiter=100, **kw))
+                dscf[name] = dict(wall_s=round(time.perf_counter() - t0, 3), energy=rd["e_total"],
+                                  iterations=rd["iterations"], converged=rd["converged"])
+            dscf["dE_incremental_vs_full"] = dscf["incremental_fock_8"]["energy"] - dscf["full_builds"]["energy"]
+            out["direct_scf"] = dscf
+            log_fn(f"[bench] direct SCF: {dscf}")
     if not no_scf:
         t0 = time.perf_counter()
         r = rhf(mol, Parameters(eri_method=method, precision=precision, convergence_energy_threshold=1e-8,

@@ -6,6 +6,7 @@
 // (gpu_manager.cu:413-432), computeERIMatrix (gpu_manager.cu:434-517) and
 // computeFockMatrix_RHF (gpu_manager.cu:678-691).
 #include "engine.hpp"
+#include "scf_kernels.hpp"
 
 #include <algorithm>
 #include <climits>
@@ -81,6 +82,13 @@ FockEngine::FockEngine(const gansu_primitive_shell* prims, size_t n_prims, const
       rik_tol_ = std::stod(kv.second);
       if (!(rik_tol_ >= 0.0)) throw Error(GANSU_E_PARAM, "ri_tol must be >= 0");
     }
+    else if (kv.first == "incremental_fock") {
+      incr_period_ = std::stoi(kv.second);
+      if (incr_period_ < 0) throw Error(GANSU_E_PARAM, "incremental_fock must be >= 0");
+    } else if (kv.first == "incremental_threshold") {
+      incr_thr_ = std::stod(kv.second);
+      if (!(incr_thr_ > 0.0)) throw Error(GANSU_E_PARAM, "incremental_threshold must be > 0");
+    }
     else if (kv.first == "ri_k") {
       if (kv.second != "density" && kv.second != "occupied") throw Error(GANSU_E_PARAM, "ri_k must be density|occupied");
       ri_k_occ_ = kv.second == "occupied";
@@ -89,6 +97,8 @@ FockEngine::FockEngine(const gansu_primitive_shell* prims, size_t n_prims, const
   }
   if (method_ != "stored" && method_ != "direct" && method_ != "ri")
     throw Error(GANSU_E_PARAM, "Unknown eri_method: " + method_ + " (stored|direct|ri)");
+  if (incr_period_ > 0 && method_ != "direct")
+    throw Error(GANSU_E_PARAM, "incremental_fock needs eri_method=direct");
   prims_.assign(prims, prims + n_prims);
   types_.assign(types, types + n_types);
   norm_.assign(norm, norm + num_basis);
@@ -202,6 +212,21 @@ void FockEngine::precompute() {
   precompute_pairs();
   if (method_ == "ri") precompute_ri();
   else precompute_quartets();
+  if (incr_period_ > 0) {
+    const int nsh = (int)shells_.size();
+    std::vector<int> bf(nsh), nf(nsh);
+    for (int i = 0; i < nsh; ++i) {
+      bf[i] = shells_[i].basis;
+      nf[i] = ncart(shells_[i].L);
+    }
+    d_shell_bf_.upload(bf);
+    d_shell_nf_.upload(nf);
+    d_dmax_.alloc((size_t)nsh * nsh + 1);
+    d_Dprev_.alloc((size_t)M_ * M_);
+    d_dDp_.alloc((size_t)M_ * M_);
+    d_incr_st_.alloc(2);
+    reset_incremental(0);
+  }
   streams_create();
   GANSU_HIP(hipDeviceSynchronize());
   precomputed_ = true;
@@ -444,12 +469,31 @@ void FockEngine::tune_slab(QuartetClass& qc, int mode) {
   (void)hipEventDestroy(e1);
 }
 
-void FockEngine::build_four_center(const double* d_D, hipStream_t s) {
+void FockEngine::reset_incremental(hipStream_t s) {
+  if (incr_period_ > 0 && d_incr_st_.ptr) GANSU_HIP(launch_set_int(d_incr_st_.ptr, 0, s));
+}
+
+void FockEngine::build_four_center(const double* d_D, hipStream_t s, bool incremental) {
   prof_clear();
-  GANSU_HIP(launch_scale_density(d_D, d_w_.ptr, M_, d_Dp_.ptr, s));
-  GANSU_HIP(launch_zero(d_Jr_.ptr, (size_t)M_ * M_, s));
-  GANSU_HIP(launch_zero(d_Kr_.ptr, (size_t)M_ * M_, s));
   EriArgs g{d_sp_.ptr, d_pr_.ptr, boys_.ptr, thr_, d_Dp_.ptr, M_, d_Jr_.ptr, d_Kr_.ptr};
+  if (incremental) {
+    // G(D) = G(D_prev) + G(D - D_prev): Jr, Kr keep the previous build's sums
+    // (cleared on the full builds), the kernels digest the density change and
+    // skip the quartets whose bound times the change is below the threshold
+    const int nsh = (int)shells_.size();
+    GANSU_HIP(launch_incr_step(d_incr_st_.ptr, incr_period_, s));
+    GANSU_HIP(launch_delta_density(d_D, d_w_.ptr, M_, d_incr_st_.ptr, d_Dp_.ptr, d_Dprev_.ptr, d_dDp_.ptr, s));
+    GANSU_HIP(launch_zero_if_full(d_Jr_.ptr, d_Kr_.ptr, (size_t)M_ * M_, d_incr_st_.ptr, s));
+    GANSU_HIP(launch_shell_dmax(d_dDp_.ptr, M_, d_shell_bf_.ptr, d_shell_nf_.ptr, nsh, d_dmax_.ptr, s));
+    g.Dp = d_dDp_.ptr;
+    g.dmax = d_dmax_.ptr;
+    g.nsh = nsh;
+    g.dthr = incr_thr_;
+  } else {
+    GANSU_HIP(launch_scale_density(d_D, d_w_.ptr, M_, d_Dp_.ptr, s));
+    GANSU_HIP(launch_zero(d_Jr_.ptr, (size_t)M_ * M_, s));
+    GANSU_HIP(launch_zero(d_Kr_.ptr, (size_t)M_ * M_, s));
+  }
   const int mode = method_ == "stored" ? kModeStoredDigest : kModeDirect;
   // launch items: (class, 0 = digestion, 1 = ket-J pass), costed by the
   // integrals they visit; profiling runs them in order on the build stream
@@ -520,8 +564,9 @@ void FockEngine::build_jk(const double* d_D, double* d_J, double* d_K, const dou
   if (!d_D) throw Error(GANSU_E_ARG, "null density");
   if (d_C && (nocc <= 0 || nocc > M_)) throw Error(GANSU_E_ARG, "nocc out of range");
   if (method_ == "ri") { build_ri(d_D, d_J, d_K, d_H, d_F, s, d_C, nocc); return; }
-  if (!use_graph_ || profiling_) {
-    build_four_center(d_D, s);
+  const bool incr = incr_period_ > 0 && !multi_density_;
+  if (!use_graph_ || profiling_ || incr) {
+    build_four_center(d_D, s, incr);
     GANSU_HIP(launch_finalize(d_Jr_.ptr, d_Kr_.ptr, d_w_.ptr, M_, d_H, d_J, d_K, d_F, s));
     return;
   }
@@ -535,7 +580,7 @@ void FockEngine::record_build(const double* d_D, const double* d_H, double* d_F,
   if (!precomputed_) throw Error(GANSU_E_STATE, "build before precompute");
   if (method_ == "ri") throw Error(GANSU_E_UNSUPPORTED, "a captured build needs eri_method stored or direct");
   if (!d_D || !d_H || !d_F) throw Error(GANSU_E_ARG, "null device pointer");
-  build_four_center(d_D, s);
+  build_four_center(d_D, s, incr_period_ > 0);
   GANSU_HIP(launch_finalize(d_Jr_.ptr, d_Kr_.ptr, d_w_.ptr, M_, d_H, nullptr, nullptr, d_F, s));
 }
 
@@ -584,8 +629,15 @@ void FockEngine::build_uhf(const double* d_Da, const double* d_Db, const double*
   const size_t nn = (size_t)M_ * M_;
   if (uhf_.n < 4 * nn) uhf_.alloc(4 * nn);
   double *Ja = uhf_.ptr, *Ka = Ja + nn, *Jb = Ka + nn, *Kb = Jb + nn;
-  build_jk(d_Da, Ja, Ka, nullptr, nullptr, s);
-  build_jk(d_Db, Jb, Kb, nullptr, nullptr, s);
+  multi_density_ = true;  // two densities per build: full builds
+  try {
+    build_jk(d_Da, Ja, Ka, nullptr, nullptr, s);
+    build_jk(d_Db, Jb, Kb, nullptr, nullptr, s);
+  } catch (...) {
+    multi_density_ = false;
+    throw;
+  }
+  multi_density_ = false;
   GANSU_HIP(launch_uhf_combine(d_H, Ja, Ka, Jb, Kb, nn, d_Fa, d_Fb, s));
 }
 

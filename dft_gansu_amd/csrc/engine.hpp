@@ -127,6 +127,8 @@ class FockEngine {
   // Moves every primitive (and auxiliary primitive) to the position of its
   // atom and repeats the precompute if it had been done.
   void update_geometry(const double* xyz_bohr, int n_atoms);
+  // incremental_fock: the next build is a full one (a new SCF starts)
+  void reset_incremental(hipStream_t s);
   const char* algorithm_name() const;
   std::vector<double> schwarz_ref() const;  // host copy, reference pair order
   size_t num_primitive_pairs() const { return ref_pairs_; }
@@ -162,7 +164,9 @@ class FockEngine {
   static int rik_chunks(int M, long Kd);  // split-k depth of the dK product
   void gradient_ri(const double* d_D, const double* d_C, int nocc, int natoms, double* d_grad, hipStream_t s);
   void ri_k_product(const double* A, int kdim, bool accumulate, double* K, hipStream_t s);
-  void build_four_center(const double* d_D, hipStream_t s);
+  // incremental: a delta-density build (incremental_fock), Jr and Kr
+  // accumulate on top of the previous build's
+  void build_four_center(const double* d_D, hipStream_t s, bool incremental = false);
   void build_ri(const double* d_D, double* d_J, double* d_K, const double* d_H, double* d_F, hipStream_t s,
                 const double* d_C, int nocc);
 
@@ -184,6 +188,14 @@ class FockEngine {
   double last_rik_rel_ = 0.0;
   static constexpr int kRikMaxBlocks = 10;  // nocc <= 320 for the incremental path
   bool autotune_ = true;   // time candidate digestion tiles at precompute
+  // eri_method=direct, incremental_fock=N > 0: builds from the change of the
+  // density with density-weighted screening, a full build every N builds
+  int incr_period_ = 0;
+  double incr_thr_ = 1e-14;  // density-weighted screen: bound x |density element| below this is skipped
+  bool multi_density_ = false;  // inside a UHF/ROHF build (two densities): no increments
+  DevBuf<double> d_Dprev_, d_dDp_;
+  DevBuf<float> d_dmax_;
+  DevBuf<int> d_shell_bf_, d_shell_nf_, d_incr_st_;
   int M_ = 0;
   std::vector<gansu_primitive_shell> prims_;
   std::vector<gansu_shell_type_info> types_;

@@ -397,6 +397,33 @@ __device__ __forceinline__ void eri_block_t(const SPair& bra, const SPair& ket, 
     }
   }
 
+// Density-weighted screen of a contracted quartet on the direct path (see
+// EriArgs::dmax): the bound of its integrals times the largest density
+// element of the six shell blocks its J and K contributions use.
+__device__ inline bool density_screened(const EriArgs& g, const SPair& bra, const SPair& ket) {
+  if (!g.dmax) return false;
+  const int n = g.nsh;
+  const float* d = g.dmax;
+  const float m = fmaxf(fmaxf(fmaxf(d[bra.si * n + bra.sj], d[ket.si * n + ket.sj]),
+                              fmaxf(d[bra.si * n + ket.si], d[bra.si * n + ket.sj])),
+                        fmaxf(d[bra.sj * n + ket.si], d[bra.sj * n + ket.sj]));
+  return bra.qmax * ket.qmax * (double)m < g.dthr;
+}
+
+// Kets at positions >= the returned count of a bra's qmax-sorted ket list
+// fall below the density screen for any shell block (bound times the largest
+// density element): a binary search for the first such position in [0, n).
+template <class QAt>
+__device__ inline int density_ket_cut(const EriArgs& g, double qb, int n, QAt qat) {
+  const double t = g.dthr / (qb * (double)g.dmax[g.nsh * g.nsh]);
+  int lo = 0, hi = n;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (qat(mid) >= t) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+
 // ------------------------------------------------- class-templated kernel --
 // One lane = one contracted shell quartet; full register tiles.  Used for the
 // classes whose block and Hermite tables fit in registers.
@@ -406,11 +433,11 @@ __global__ __launch_bounds__(64) void k_eri_class(ClassLaunch c, EriArgs g) {
   constexpr int NAB = NA * NB, NCD = NC * ND, NCOMP = NAB * NCD;
   const BlockRec blk = c.blocks[blockIdx.x];
   const int lane = threadIdx.x;
-  const bool active = lane < blk.count;
   const int bra_id = c.bra_list[blk.bra_pos];
-  const int ket_id = c.ket_list[blk.ket_start + (active ? lane : 0)];
+  const int ket_id = c.ket_list[blk.ket_start + (lane < blk.count ? lane : 0)];
   const SPair bra = g.sp[bra_id];
   const SPair ket = g.sp[ket_id];
+  const bool active = lane < blk.count && !(MODE == kModeDirect && density_screened(g, bra, ket));
 
   double out[NCOMP];
 #pragma unroll
@@ -521,6 +548,7 @@ __global__ __launch_bounds__((CoopShape<LA, LB, LC, LD>::NT)) void k_eri_coop(Cl
   for (int q = blockIdx.y; q < blk.count; q += gridDim.y) {
     const int ket_id = c.ket_list[blk.ket_start + q];
     const SPair ket = g.sp[ket_id];
+    if (MODE == kModeDirect && density_screened(g, bra, ket)) continue;  // uniform over the workgroup
     double acc[NOWN];
 #pragma unroll
     for (int j = 0; j < NOWN; ++j) acc[j] = 0.0;
@@ -824,6 +852,57 @@ __global__ void k_rohf_unified(const double* __restrict__ Fc, const double* __re
   R[id] = d;
 }
 
+__global__ void k_incr_step(int* st, int period) {
+  const int n = st[0];
+  st[1] = (period <= 0 || n % period == 0) ? 1 : 0;
+  st[0] = n + 1;
+}
+
+__global__ void k_delta_density(const double* __restrict__ D, const double* __restrict__ w, int M,
+                                const int* __restrict__ st, double* __restrict__ Dp, double* __restrict__ Dprev,
+                                double* __restrict__ dDp) {
+  const size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= (size_t)M * M) return;
+  const double v = D[id] * w[id / M] * w[id % M];
+  const double prev = st[1] ? 0.0 : Dprev[id];
+  Dp[id] = v;
+  dDp[id] = v - prev;
+  Dprev[id] = v;
+}
+
+__global__ void k_zero_if_full(double* __restrict__ Jr, double* __restrict__ Kr, size_t n, const int* __restrict__ st) {
+  if (!st[1]) return;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    Jr[i] = 0.0;
+    Kr[i] = 0.0;
+  }
+}
+
+__global__ void k_shell_dmax(const double* __restrict__ dDp, int M, const int* __restrict__ bf,
+                             const int* __restrict__ nf, int nsh, float* __restrict__ dmax) {
+  const int id = blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= nsh * nsh) return;
+  const int i = id / nsh, j = id % nsh;
+  double m = 0.0;
+  for (int a = 0; a < nf[i]; ++a)
+    for (int b = 0; b < nf[j]; ++b) m = fmax(m, fabs(dDp[(size_t)(bf[i] + a) * M + bf[j] + b]));
+  dmax[id] = (float)m;
+}
+
+// dmax[nsh * nsh] = max over the nsh * nsh block maxima (one workgroup)
+__global__ __launch_bounds__(1024) void k_dmax_reduce(float* dmax, int nsh) {
+  __shared__ float red[1024];
+  float m = 0.0f;
+  for (int i = threadIdx.x; i < nsh * nsh; i += 1024) m = fmaxf(m, dmax[i]);
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int o = 512; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + o]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) dmax[nsh * nsh] = red[0];
+}
+
 __global__ void k_finalize(const double* __restrict__ Jr, const double* __restrict__ Kr, const double* __restrict__ w,
                            int M, const double* __restrict__ H, double* J, double* K, double* F) {
   const size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -921,6 +1000,8 @@ void k_fock_slab(SlabLaunch c, EriArgs g) {
       const SPair bra = g.sp[bid[b]];
       bbi[b] = bra.bi; bbj[b] = bra.bj; bdiff[b] = bra.si != bra.sj;
       cnt[b] = min(br.count, wk.k_end); fblk[b] = br.first_block;
+      if (MODE == kModeDirect && g.dmax)
+        cnt[b] = min(cnt[b], density_ket_cut(g, bra.qmax, cnt[b], [&](int k) { return g.sp[c.ket_meta[k].w].qmax; }));
       spl[b] = br.split; fblk32[b] = br.first_block32;
       maxc = max(maxc, cnt[b]);
     } else {
@@ -1062,10 +1143,12 @@ void k_fock_slab(SlabLaunch c, EriArgs g) {
             }
         };
         if constexpr (MODE == kModeDirect) {
+          const SPair ket = g.sp[ket_id];
+          const SPair bra = g.sp[bid[b]];
+          if (density_screened(g, bra, ket)) continue;
 #pragma unroll
           for (int k = 0; k < NCOMP; ++k) out[k] = 0.0;
-          const SPair ket = g.sp[ket_id];
-          eri_block_t<LA, LB, LC, LD>(g.sp[bid[b]], ket, ket.nprim, g, out);
+          eri_block_t<LA, LB, LC, LD>(bra, ket, ket.nprim, g, out);
           digest([&](int aa, int bb, int cc, int d) -> double { return out[((aa * NB + bb) * NC + cc) * ND + d]; });
         } else {
           // the lane's integrals of this quartet: rows a of the lane, all (b, c, d)
@@ -1426,6 +1509,32 @@ hipError_t launch_fill(double* p, size_t n, double v, hipStream_t s) {
   size_t nb = (n + 255) / 256;
   if (nb > 4096) nb = 4096;
   hipLaunchKernelGGL(k_fill, dim3((unsigned)nb), dim3(256), 0, s, p, n, v);
+  return hipGetLastError();
+}
+
+hipError_t launch_incr_step(int* st, int period, hipStream_t s) {
+  hipLaunchKernelGGL(k_incr_step, dim3(1), dim3(1), 0, s, st, period);
+  return hipGetLastError();
+}
+
+hipError_t launch_delta_density(const double* D, const double* w, int M, const int* st, double* Dp, double* Dprev,
+                                double* dDp, hipStream_t s) {
+  const size_t n = (size_t)M * M;
+  hipLaunchKernelGGL(k_delta_density, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, D, w, M, st, Dp, Dprev, dDp);
+  return hipGetLastError();
+}
+
+hipError_t launch_zero_if_full(double* Jr, double* Kr, size_t n, const int* st, hipStream_t s) {
+  const unsigned grid = (unsigned)std::min<size_t>(4096, (n + 255) / 256);
+  hipLaunchKernelGGL(k_zero_if_full, dim3(grid), dim3(256), 0, s, Jr, Kr, n, st);
+  return hipGetLastError();
+}
+
+hipError_t launch_shell_dmax(const double* dDp, int M, const int* bf, const int* nf, int nsh, float* dmax,
+                             hipStream_t s) {
+  const int n = nsh * nsh;
+  hipLaunchKernelGGL(k_shell_dmax, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, dDp, M, bf, nf, nsh, dmax);
+  hipLaunchKernelGGL(k_dmax_reduce, dim3(1), dim3(1024), 0, s, dmax, nsh);
   return hipGetLastError();
 }
 

@@ -91,6 +91,13 @@ struct EriArgs {
   int M;
   double* Jr;  // raw-basis J accumulator (M x M)
   double* Kr;  // raw-basis K accumulator (M x M)
+  // density-weighted screening of the direct path (null: off): max |Dp| over
+  // each shell block, [nsh][nsh], then the largest of them at [nsh * nsh]; a
+  // quartet (ij|kl) is skipped when
+  // qmax_ij qmax_kl max(d_ij, d_kl, d_ik, d_il, d_jk, d_jl) < dthr
+  const float* dmax = nullptr;
+  int nsh = 0;
+  double dthr = 0.0;
 };
 
 // Host-callable launchers (fock_kernels.hip).
@@ -124,5 +131,21 @@ hipError_t launch_uhf_combine(const double* H, const double* Ja, const double* K
                               size_t n, double* Fa, double* Fb, hipStream_t s);
 hipError_t launch_finalize(const double* Jr, const double* Kr, const double* w, int M, const double* H, double* J,
                            double* K, double* F, hipStream_t s);
+
+// Incremental (delta-density) direct builds.  st[0] counts builds since the
+// last full one; launch_incr_step sets st[1] = 1 (full build: the previous
+// density counts as zero and Jr, Kr are cleared) when st[0] % period == 0,
+// else 0, and advances st[0].  launch_delta_density: Dp = scaled D,
+// dDp = Dp - Dprev (Dp when st[1]), Dprev = Dp.  launch_zero_if_full clears
+// Jr and Kr when st[1].  launch_shell_dmax: dmax[i][j] = max |dDp| over the
+// basis functions of shells i, j (first function bf[i], nf[i] functions) and
+// dmax[nsh * nsh] = the largest of them.  All device-side, so a captured
+// build graph replays them.
+hipError_t launch_incr_step(int* st, int period, hipStream_t s);
+hipError_t launch_delta_density(const double* D, const double* w, int M, const int* st, double* Dp, double* Dprev,
+                                double* dDp, hipStream_t s);
+hipError_t launch_zero_if_full(double* Jr, double* Kr, size_t n, const int* st, hipStream_t s);
+hipError_t launch_shell_dmax(const double* dDp, int M, const int* bf, const int* nf, int nsh, float* dmax,
+                             hipStream_t s);
 
 }  // namespace gansu

@@ -381,7 +381,9 @@ Params::Params() {
         {"fp32_bound_threshold", "1e-6"},
         {"normalization", "reference"},
         {"ri_k", "density"},
-        {"scf_solver", "diagonalize"}};
+        {"scf_solver", "diagonalize"},
+        {"incremental_fock", "0"},
+        {"incremental_threshold", "1e-14"}};
 }
 
 static std::string lower(std::string s) {

@@ -59,8 +59,15 @@ void FockEngine::build_rohf(const double* d_Dc, const double* d_Do, const double
   if (uhf_.n < 4 * nn) uhf_.alloc(4 * nn);
   if (rohf_.n < 3 * nn) rohf_.alloc(3 * nn);
   double *Jc = uhf_.ptr, *Kc = Jc + nn, *Jo = Kc + nn, *Ko = Jo + nn;
-  build_jk(d_Dc, Jc, Kc, nullptr, nullptr, s);
-  build_jk(d_Do, Jo, Ko, nullptr, nullptr, s);
+  multi_density_ = true;  // two densities per build: full builds
+  try {
+    build_jk(d_Dc, Jc, Kc, nullptr, nullptr, s);
+    build_jk(d_Do, Jo, Ko, nullptr, nullptr, s);
+  } catch (...) {
+    multi_density_ = false;
+    throw;
+  }
+  multi_density_ = false;
   GANSU_HIP(launch_rohf_combine(d_H, Jc, Kc, Jo, Ko, nn, d_Fc, d_Fo, s));
   if (!rohf_blas_) {
     rocblas_handle h = nullptr;

@@ -535,7 +535,9 @@ ScfResult rhf_run(const Molecule& mol, const Molecule* aux, const Params& prm, b
                                           {"schwarz_screening_threshold", prm.get("schwarz_screening_threshold")},
                                           {"precision", prm.get("precision")},
                                           {"fp32_bound_threshold", prm.get("fp32_bound_threshold")},
-                                          {"ri_k", prm.get("ri_k")}};
+                                          {"ri_k", prm.get("ri_k")},
+                                          {"incremental_fock", prm.get("incremental_fock")},
+                                          {"incremental_threshold", prm.get("incremental_threshold")}};
   const bool ri = prm.get("eri_method") == "ri";
   if (ri && !aux) throw Error(GANSU_E_ARG, "eri_method=ri needs auxiliary_gbsfilename");
   FockEngine eng(mol.prims.data(), mol.prims.size(), mol.types.data(), (int)mol.types.size(), mol.norm.data(), n,

@@ -56,7 +56,7 @@ class FockEngine:
 
     def __init__(self, mol, aux=None, eri_method="stored", schwarz_screening_threshold=1e-12, precision="fp64",
                  fp32_bound_threshold=1e-6, ri_k="density", autotune=True, graph=True, nan_check=True,
-                 ri_refresh=None, ri_tol=None):
+                 ri_refresh=None, ri_tol=None, incremental_fock=None):
         self.mol = mol
         self.aux = aux
         self.M = mol.n_basis
@@ -67,6 +67,8 @@ class FockEngine:
             opts += f";ri_refresh={int(ri_refresh)}"
         if ri_tol is not None:
             opts += f";ri_tol={float(ri_tol)!r}"
+        if incremental_fock is not None:
+            opts += f";incremental_fock={int(incremental_fock)}"
         self._h = ctypes.c_void_p()
         a = aux
         check(lib().gansu_fock_create(

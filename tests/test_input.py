@@ -174,6 +174,11 @@ def test_engine_option_errors():
         FockEngine(mol, precision="fp16")
     with pytest.raises(GansuError, match="auxiliary"):
         FockEngine(mol, eri_method="ri")
+    # delta-density builds exist for the direct path only, with a positive period
+    with pytest.raises(GansuError, match="incremental_fock needs eri_method=direct"):
+        FockEngine(mol, eri_method="stored", incremental_fock=4)
+    with pytest.raises(GansuError, match="incremental_fock must be"):
+        FockEngine(mol, eri_method="direct", incremental_fock=-1)
     f = mol.prims.copy()
     f["shell_type"][-1] = 3  # an f primitive: beyond the compiled orbital classes
     mol.prims = f
