@@ -132,7 +132,15 @@ void gansu_params_destroy(gansu_params* p);
  * leaves K before the SCF converges); the
  * incremental build reads three numbers back (alignment residual, |E|,
  * |X|).  State for the increments (about 8 M Maux nocc bytes) is
- * allocated by the first such build.
+ * allocated by the first such build.  For eri_method=direct,
+ * incremental_fock=N (default 0: off) makes RHF builds (gansu_fock_build and
+ * the RHF drivers; UHF and ROHF builds stay full) digest the change of the
+ * density since the previous build and add it to that build's J and K,
+ * skipping quartets whose bound times the largest density-change element of
+ * their six shell blocks is below incremental_threshold (default 1e-14); every
+ * N-th build is a full one.  The state lives on the device, so a captured
+ * build replays it; any other eri_method with incremental_fock > 0 is
+ * GANSU_E_PARAM.
  * The engine runs on the device that was current at gansu_fock_create; each
  * call makes it current and restores the caller's device on return.
  * aux* may be NULL/0 unless eri_method=ri.                                  */
