@@ -50,6 +50,29 @@ RIK_NAMES = {80020: "align (Procrustes)", 80021: "split E", 80022: "k_rik_tstep 
              80002: "J (FP64 B)"}
 
 
+FP64_VECTOR_PEAK_TFS = 78.6  # MI355X FP64 vector, TFLOP/s
+
+
+def md_algorithmic_rate(eng, build_s):
+    """Algorithmic FP64 work of a direct build: per quartet class the primitive
+    quartets that pass the Schwarz screen (the engine's count) times the FP64
+    operations of one primitive quartet in the oracle's McMurchie-Davidson
+    routine (tests/golden/md_flops.json, counted by running that routine on
+    op-counting numbers), over the build time.  The kernels factor the
+    Hermite contraction (E tables and bra intermediates shared across
+    components), so they execute fewer operations than this count; the rate
+    is the reference algorithm's work per second."""
+    fx = json.load(open(os.path.join(ROOT, "tests", "golden", "md_flops.json")))["classes"]
+    cs = eng.class_stats()
+    flops = sum(c["primitive"] * fx["%d %d %d %d" % c["L"]][0] for c in cs)
+    prim = sum(c["primitive"] for c in cs)
+    tfs = flops / build_s / 1e12
+    return dict(primitive_quartets=prim, contracted_quartets=sum(c["contracted"] for c in cs),
+                reference_md_flops=flops, reference_md_tflops=round(tfs, 2),
+                frac_of_fp64_vector_peak=round(tfs / FP64_VECTOR_PEAK_TFS, 3),
+                primitive_quartets_per_s=round(prim / build_s, 1), unit_note="reference-MD operation count")
+
+
 def orbital_sequence(M, n, seed, amp=1e-3, decay=0.7):
     """n orthonormal M x M orbital matrices (torch, on the GPU): a random
     orthogonal C_0 (host QR), then C_k = C_(k-1) exp(A_k) with A_k a random
@@ -356,6 +379,8 @@ def coronene_leg(log_fn):
             st = e.stats()
             out[name]["stored_bytes"] = st["stored_bytes"]
             out[name]["contracted_quartets"] = st["contracted_quartets"]
+            if name == "direct_fp64":
+                out[name]["md"] = md_algorithmic_rate(e, t)
         del e
         torch.cuda.empty_cache()
         log_fn(f"[bench] C24H12 {name}: {out[name]}")
@@ -536,6 +561,8 @@ def stored_leg(cfg, method, precision, steps, warmup, no_direct, no_scf, no_mixe
         torch.cuda.synchronize()
         out["direct_fock_builds_per_s"] = round(nd / (time.perf_counter() - t0), 3)
         log_fn(f"[bench] direct build rate {out['direct_fock_builds_per_s']} /s")
+        out["direct_md"] = md_algorithmic_rate(e2, 1.0 / out["direct_fock_builds_per_s"])
+        log_fn(f"[bench] direct build, algorithmic MD work: {out['direct_md']}")
         del e2
         if not no_scf:
             # direct SCF with full builds and with delta-density builds (a full one every 8)

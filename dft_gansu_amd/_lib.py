@@ -30,6 +30,7 @@ EXPORTS = [
     "gansu_params_create", "gansu_params_load_recipe", "gansu_params_set", "gansu_params_get", "gansu_params_destroy",
     "gansu_fock_create", "gansu_fock_precompute", "gansu_fock_build_rhf", "gansu_fock_build_jk", "gansu_fock_build_rhf_occ",
     "gansu_fock_algorithm_name", "gansu_fock_schwarz", "gansu_fock_num_primitive_pairs", "gansu_fock_stats",
+    "gansu_fock_class_stats",
     "gansu_fock_set_profiling", "gansu_fock_last_profile", "gansu_fock_destroy", "gansu_oneint", "gansu_rhf_run",
     "gansu_fock_gradient_rhf", "gansu_fock_gradient_rhf_occ", "gansu_oneint_gradient", "gansu_molecule_nuclear_repulsion_gradient",
     "gansu_rhf_gradient", "gansu_oneint_noalloc", "gansu_rhf_run_guess", "gansu_rhf_gradient_guess", "gansu_uhf_run_guess", "gansu_uhf_run", "gansu_rohf_run", "gansu_rhf_run_mo", "gansu_write_molden", "gansu_fock_build_uhf", "gansu_fock_build_rohf", "gansu_fock_update_geometry",
@@ -88,6 +89,7 @@ def lib():
         L.gansu_fock_num_primitive_pairs.argtypes = [vp]
         L.gansu_fock_num_primitive_pairs.restype = sz
         L.gansu_fock_stats.argtypes = [vp, vp]
+        L.gansu_fock_class_stats.argtypes = [vp, ctypes.c_int, vp, vp, vp, vp]
         L.gansu_fock_set_profiling.argtypes = [vp, ip]
         L.gansu_fock_last_profile.argtypes = [vp, vp, ip, vp]
         L.gansu_fock_destroy.argtypes = [vp]

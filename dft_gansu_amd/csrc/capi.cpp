@@ -228,6 +228,22 @@ int gansu_fock_stats(const gansu_fock_engine* e, double* stats) {
   });
 }
 
+int gansu_fock_class_stats(gansu_fock_engine* e, int max_classes, int* n_classes, int* lab, double* n_contracted,
+                           double* n_primitive) {
+  return guarded([&] {
+    if (!e || !n_classes) throw gansu::Error(GANSU_E_ARG, "null argument");
+    if (max_classes > 0 && (!lab || !n_contracted || !n_primitive)) throw gansu::Error(GANSU_E_ARG, "null output");
+    gansu::DeviceScope ds(e->e->device());
+    const auto& cc = e->e->class_counts();
+    *n_classes = (int)cc.size();
+    for (int c = 0; c < (int)cc.size() && c < max_classes; ++c) {
+      for (int k = 0; k < 4; ++k) lab[4 * c + k] = cc[c].L[k];
+      n_contracted[c] = cc[c].contracted;
+      n_primitive[c] = cc[c].primitive;
+    }
+  });
+}
+
 int gansu_fock_set_profiling(gansu_fock_engine* e, int enable) {
   return guarded([&] {
     if (!e) throw gansu::Error(GANSU_E_ARG, "null engine");

@@ -308,6 +308,8 @@ void FockEngine::precompute_pairs() {
     GANSU_HIP(launch_pair_data(d_prims_.ptr, dpa.ptr, dpb.ptr, dq.ptr, ppa.size(), d_pr_.ptr, 0));
     GANSU_HIP(hipDeviceSynchronize());
   }
+  pair_q_h_ = pq;
+  class_counts_.clear();
   pair_alpha_h_.resize(ppa.size());
   for (size_t i = 0; i < ppa.size(); ++i) pair_alpha_h_[i] = prims_[ppa[i]].exponent;
   d_sp_.upload(sp_h_);
@@ -667,6 +669,45 @@ void FockEngine::update_geometry(const double* xyz, int n_atoms) {
 std::vector<double> FockEngine::schwarz_ref() const {
   if (!precomputed_) throw Error(GANSU_E_STATE, "schwarz factors before precompute");
   return Qref_h_;
+}
+
+const std::vector<FockEngine::ClassCount>& FockEngine::class_counts() {
+  if (!precomputed_) throw Error(GANSU_E_STATE, "class counts before precompute");
+  if (method_ == "ri") throw Error(GANSU_E_UNSUPPORTED, "class counts of the four-centre paths (stored, direct)");
+  if (!class_counts_.empty()) return class_counts_;
+  // each shell pair's primitive-pair Q, descending
+  std::vector<std::vector<double>> qs(sp_h_.size());
+  for (size_t i = 0; i < sp_h_.size(); ++i) {
+    const SPair& p = sp_h_[i];
+    qs[i].assign(pair_q_h_.begin() + p.prim_begin, pair_q_h_.begin() + p.prim_begin + p.nprim);
+    std::sort(qs[i].begin(), qs[i].end(), std::greater<double>());
+  }
+  double total = 0.0;
+  for (const auto& qc : qclass_) {
+    ClassCount cc{{qc->LA, qc->LB, qc->LC, qc->LD}, qc->n_quartets, 0.0};
+    const auto& bl = pclass_[qc->A].sorted;
+    const auto& kl = pclass_[qc->B].sorted;
+    for (const BraRec& br : qc->bras_h) {
+      const auto& X = qs[bl[br.bra_pos]];
+      for (int k = 0; k < br.count; ++k) {
+        const auto& Y = qs[kl[k]];
+        // pairs with X_i Y_j >= thr: for descending X the admissible prefix of
+        // Y shrinks, so one pointer walks Y once
+        size_t j = Y.size();
+        double n = 0.0;
+        for (double x : X) {
+          while (j > 0 && x * Y[j - 1] < thr_) --j;
+          if (j == 0) break;
+          n += (double)j;
+        }
+        cc.primitive += n;
+      }
+    }
+    total += cc.primitive;
+    class_counts_.push_back(cc);
+  }
+  n_prim_quartets_ = total;
+  return class_counts_;
 }
 
 void FockEngine::stats(double* out) const {

@@ -133,6 +133,12 @@ class FockEngine {
   std::vector<double> schwarz_ref() const;  // host copy, reference pair order
   size_t num_primitive_pairs() const { return ref_pairs_; }
   void stats(double* out) const;
+  // Per quartet class of the stored/direct paths: (LA LB LC LD), the
+  // contracted quartets the builds visit and the primitive quartets among
+  // them that pass the primitive screen Q_ab Q_cd >= thr (the MD unit of
+  // work).  Counted on the host once, on first use.
+  struct ClassCount { int L[4]; double contracted, primitive; };
+  const std::vector<ClassCount>& class_counts();
   int num_basis() const { return M_; }
   int device() const { return dev_; }  // the device current when the engine was created
   void set_profiling(bool on);
@@ -212,6 +218,8 @@ class FockEngine {
   std::vector<double> Qref_h_;
   DevBuf<PairRec> d_pr_;
   std::vector<double> pair_alpha_h_;  // exponent of the first primitive of each pair record
+  std::vector<double> pair_q_h_;      // Q of each pair record (host copy)
+  std::vector<ClassCount> class_counts_;
   DevBuf<double> d_alpha_;
   DevBuf<int> d_shell_atom_;
   DevBuf<SPair> d_sp_;

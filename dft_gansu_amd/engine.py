@@ -167,6 +167,18 @@ class FockEngine:
         return dict(contracted_quartets=s[0], primitive_quartets=s[1], stored_bytes=s[2], fp32_quartets=s[3],
                     unique_quartets=s[4], ri_bytes=s[5], graph_captures=s[6], ri_incremental_builds=int(s[7]))
 
+    def class_stats(self):
+        """Per quartet class: dict(L=(LA, LB, LC, LD), contracted=..., primitive=...): the
+        contracted quartets the builds visit and the primitive quartets among them that pass
+        the primitive Schwarz screen (stored and direct engines, after precomputation)."""
+        n = ctypes.c_int(0)
+        check(lib().gansu_fock_class_stats(self._h, 0, ctypes.byref(n), None, None, None))
+        lab = np.zeros(4 * n.value, dtype=np.int32)
+        nc, npq = np.zeros(n.value), np.zeros(n.value)
+        check(lib().gansu_fock_class_stats(self._h, n.value, ctypes.byref(n), ptr(lab), ptr(nc), ptr(npq)))
+        return [dict(L=tuple(int(x) for x in lab[4 * c:4 * c + 4]), contracted=float(nc[c]), primitive=float(npq[c]))
+                for c in range(n.value)]
+
     def set_profiling(self, enable=True):
         check(lib().gansu_fock_set_profiling(self._h, int(enable)))
 

@@ -201,6 +201,16 @@ size_t gansu_fock_num_primitive_pairs(const gansu_fock_engine* e);
  *             stored bytes, fp32 quartets, unique contracted quartets,
  *             ri B bytes, graphs captured, incremental RI builds}           */
 int gansu_fock_stats(const gansu_fock_engine* e, double* stats);
+/* Per quartet class of a stored or direct engine (after precompute): lab[4c
+ * .. 4c+3] = LA LB LC LD, n_contracted[c] = the contracted quartets the
+ * builds visit, n_primitive[c] = the primitive quartets among them that pass
+ * the primitive screen Q_ab Q_cd >= schwarz_screening_threshold (the unit of
+ * work of the MD integrals, eri.cu:160-196 / int2e.cu:1646-1947).  At most
+ * max_classes entries are written; *n_classes = the number of classes.  The
+ * first call counts on the host (and fills stats[1]); RI engines:
+ * GANSU_E_UNSUPPORTED.  No counterpart in the reference (measurement).      */
+int gansu_fock_class_stats(gansu_fock_engine* e, int max_classes, int* n_classes, int* lab, double* n_contracted,
+                           double* n_primitive);
 /* Per-launch timing of the builds that follow, with HIP events recorded on the
  * build stream (no counterpart in the reference, whose profiler.hpp times host
  * wall clock only).  After a build, rows[5*i + k] for launch i hold

@@ -24,6 +24,8 @@
 //   orc_rhf_guess                     orc_rhf from a guess density (rhf.hpp:551-584)
 //   orc_int2c2e/orc_int3c2e/orc_ri_*  int2c2e.cu:468-687, int3c2e.cu:1102-1336,
 //                                     gpu_manager.cu:1311-1632
+//   orc_md_flops                      FP64 op count of prim_quartet per class
+//                                     (int2e.cu:1646-1947 structure)
 //
 // Pinning (see tests/test_oracle.py):
 //   Boys           closed forms F_n(0)=1/(2n+1), large-x asymptote, scipy 1F1,
@@ -79,6 +81,36 @@ inline const int* cart(int l, int c, bool aux) {
 
 double dfact(int n) { double r = 1; for (int k = n; k > 1; k -= 2) r *= k; return r; }
 
+// ------------------------------------------------------ FP64 op counting --
+// Op: a double that counts the arithmetic performed on it (+, -, *, /; unary
+// minus is a sign flip and is not counted) and, separately, the special
+// functions (exp, sqrt, erf, pow, floor).  The MD routines below are templates
+// over the number type: with double they are the oracle itself, with Op the
+// same code counts its own floating-point work (orc_md_flops).
+struct Op {
+  double v;
+  Op() : v(0.0) {}
+  Op(double x) : v(x) {}  // constants enter free
+  static thread_local long long n_arith, n_special;
+};
+thread_local long long Op::n_arith = 0, Op::n_special = 0;
+inline Op operator+(Op a, Op b) { ++Op::n_arith; return Op(a.v + b.v); }
+inline Op operator-(Op a, Op b) { ++Op::n_arith; return Op(a.v - b.v); }
+inline Op operator*(Op a, Op b) { ++Op::n_arith; return Op(a.v * b.v); }
+inline Op operator/(Op a, Op b) { ++Op::n_arith; return Op(a.v / b.v); }
+inline Op operator-(Op a) { return Op(-a.v); }
+inline Op& operator+=(Op& a, Op b) { ++Op::n_arith; a.v += b.v; return a; }
+inline Op& operator*=(Op& a, Op b) { ++Op::n_arith; a.v *= b.v; return a; }
+inline bool operator<(Op a, Op b) { return a.v < b.v; }
+inline bool operator>(Op a, Op b) { return a.v > b.v; }
+inline bool operator==(Op a, Op b) { return a.v == b.v; }
+inline Op exp(Op a) { ++Op::n_special; return Op(std::exp(a.v)); }
+inline Op sqrt(Op a) { ++Op::n_special; return Op(std::sqrt(a.v)); }
+inline Op erf(Op a) { ++Op::n_special; return Op(std::erf(a.v)); }
+inline Op pow(Op a, double b) { ++Op::n_special; return Op(std::pow(a.v, b)); }
+inline double floor_of(double x) { return std::floor(x); }
+inline double floor_of(Op x) { ++Op::n_special; return std::floor(x.v); }
+
 // ---------------------------------------------------------------- Boys ----
 // Exact Boys function by the convergent series
 //   F_n(x) = exp(-x) * sum_k (2x)^k / ((2n+1)(2n+3)...(2n+2k+1))
@@ -104,27 +136,29 @@ constexpr double A_RS = 0.064048916778075, B_RS = 28.487431543672;
 constexpr int LUT_NUM_XI = 1024, LUT_K_MAX = 5;
 constexpr double LUT_XI_INTERVAL = 0.03125;
 
-void boys_ref(int n, double x, const double* table, double* out) {
-  if (x == 0.0) {
-    for (int j = 0; j <= n; ++j) out[j] = 1.0 / (2 * j + 1);
-  } else if (x < A_TR * n + B_TR) {
-    int xi = (int)std::floor(x / LUT_XI_INTERVAL + 0.5);
-    double dx = x - LUT_XI_INTERVAL * xi;
+template <class T>
+void boys_ref(int n, T x, const double* table, T* out) {
+  using std::exp; using std::sqrt; using std::erf;
+  if (x == T(0.0)) {
+    for (int j = 0; j <= n; ++j) out[j] = T(1.0) / T(2 * j + 1);
+  } else if (x < T(A_TR * n + B_TR)) {
+    int xi = (int)floor_of(x / T(LUT_XI_INTERVAL) + T(0.5));
+    T dx = x - T(LUT_XI_INTERVAL * xi);
     for (int j = 0; j <= n; ++j) {
-      double num = 1.0; int fac = 1;
-      double f = table[LUT_NUM_XI * j + xi];
+      T num = 1.0; int fac = 1;
+      T f = table[LUT_NUM_XI * j + xi];
       for (int k = 1; k <= LUT_K_MAX; ++k) {
         num *= -dx; fac *= k;
-        f += table[LUT_NUM_XI * (j + k) + xi] * num / fac;
+        f += T(table[LUT_NUM_XI * (j + k) + xi]) * num / T(fac);
       }
       out[j] = f;
     }
   } else {
-    double em = 0.0, r2x = 1.0 / (2 * x);
-    double f = 0.5 * std::sqrt(M_PI / x);
-    if (x < A_RS * n + B_RS) { em = std::exp(-x); f *= std::erf(std::sqrt(x)); }
+    T em = 0.0, r2x = T(1.0) / (T(2) * x);
+    T f = T(0.5) * sqrt(T(M_PI) / x);
+    if (x < T(A_RS * n + B_RS)) { em = exp(-x); f *= erf(sqrt(x)); }
     out[0] = f;
-    for (int j = 1; j <= n; ++j) { f = ((2 * j - 1) * f - em) * r2x; out[j] = f; }
+    for (int j = 1; j <= n; ++j) { f = (T(2 * j - 1) * f - em) * r2x; out[j] = f; }
   }
 }
 
@@ -138,6 +172,8 @@ void boys(int n, double x, double* out) {
   double ex = std::exp(-x);
   for (int j = n; j > 0; --j) out[j - 1] = (2 * x * out[j] + ex) / (2 * j - 1);
 }
+// the op-counted evaluation uses the reference-style Boys function
+void boys(int n, Op x, Op* out) { boys_ref(n, x, g_boys.table, out); }
 
 // ------------------------------------------------------ MD building blocks
 // Hermite expansion coefficients E^{ij}_t for one Cartesian direction, with
@@ -167,27 +203,29 @@ void hermite_e(int la, int lb, double p, double PA, double PB, Etab& E) {
 
 // Hermite Coulomb integrals R_{tuv} (n=0) for t+u+v <= L.
 // R^n_{000} = (-2a)^n F_n(a |PQ|^2);  R^n_{t+1,u,v} = t R^{n+1}_{t-1,u,v} + X R^{n+1}_{t,u,v}.
-struct Rtab { double r[13][13][13]; };
-void hermite_r(int L, double a, double X, double Y, double Z, Rtab& R) {
-  double F[32];
+template <class T> struct RtabT { T r[13][13][13]; };
+using Rtab = RtabT<double>;
+template <class T>
+void hermite_r(int L, T a, T X, T Y, T Z, RtabT<T>& R) {
+  T F[32];
   boys(L, a * (X * X + Y * Y + Z * Z), F);
   // tmp[n][t][u][v]
-  static thread_local std::vector<double> tmp;
+  static thread_local std::vector<T> tmp;
   const int S = 13;
-  tmp.assign((size_t)(L + 1) * S * S * S, 0.0);
-  auto at = [&](int n, int t, int u, int v) -> double& { return tmp[(((size_t)n * S + t) * S + u) * S + v]; };
-  double m = 1.0;
-  for (int n = 0; n <= L; ++n) { at(n, 0, 0, 0) = m * F[n]; m *= -2.0 * a; }
+  tmp.assign((size_t)(L + 1) * S * S * S, T(0.0));
+  auto at = [&](int n, int t, int u, int v) -> T& { return tmp[(((size_t)n * S + t) * S + u) * S + v]; };
+  T m = 1.0;
+  for (int n = 0; n <= L; ++n) { at(n, 0, 0, 0) = m * F[n]; m *= T(-2.0) * a; }
   for (int n = L - 1; n >= 0; --n) {
     int K = L - n;
     for (int t = 0; t <= K; ++t)
       for (int u = 0; u + t <= K; ++u)
         for (int v = 0; v + u + t <= K; ++v) {
           if (t + u + v == 0) continue;
-          double val;
-          if (t > 0) val = X * at(n + 1, t - 1, u, v) + (t > 1 ? (t - 1) * at(n + 1, t - 2, u, v) : 0.0);
-          else if (u > 0) val = Y * at(n + 1, t, u - 1, v) + (u > 1 ? (u - 1) * at(n + 1, t, u - 2, v) : 0.0);
-          else val = Z * at(n + 1, t, u, v - 1) + (v > 1 ? (v - 1) * at(n + 1, t, u, v - 2) : 0.0);
+          T val;
+          if (t > 0) val = t > 1 ? X * at(n + 1, t - 1, u, v) + T(t - 1) * at(n + 1, t - 2, u, v) : X * at(n + 1, t - 1, u, v);
+          else if (u > 0) val = u > 1 ? Y * at(n + 1, t, u - 1, v) + T(u - 1) * at(n + 1, t, u - 2, v) : Y * at(n + 1, t, u - 1, v);
+          else val = v > 1 ? Z * at(n + 1, t, u, v - 1) + T(v - 1) * at(n + 1, t, u, v - 2) : Z * at(n + 1, t, u, v - 1);
           at(n, t, u, v) = val;
         }
   }
@@ -220,13 +258,15 @@ void make_pair(const Prim& a, int la, const Prim& b, int lb, PairData& pd) {
 
 // Primitive [ab|cd] for all Cartesian components, raw (no coefficients, no
 // norms).  out[((ia*nb+ib)*nc+ic)*nd+id].
+template <class T>
 void prim_quartet(const PairData& ab, int la, int lb, bool auxa, bool auxb, const PairData& cd, int lc, int ld,
-                  bool auxc, bool auxd, double* out) {
-  double p = ab.p, q = cd.p, alpha = p * q / (p + q);
+                  bool auxc, bool auxd, T* out) {
+  using std::sqrt;
+  T p = ab.p, q = cd.p, alpha = p * q / (p + q);
   int L = la + lb + lc + ld;
-  Rtab R;
-  hermite_r(L, alpha, ab.P[0] - cd.P[0], ab.P[1] - cd.P[1], ab.P[2] - cd.P[2], R);
-  double pref = 2.0 * std::pow(M_PI, 2.5) / (p * q * std::sqrt(p + q)) * ab.K * cd.K;
+  RtabT<T> R;
+  hermite_r(L, alpha, T(ab.P[0]) - T(cd.P[0]), T(ab.P[1]) - T(cd.P[1]), T(ab.P[2]) - T(cd.P[2]), R);
+  T pref = T(2.0 * std::pow(M_PI, 2.5)) / (p * q * sqrt(p + q)) * T(ab.K) * T(cd.K);
   int na = ncart(la), nb = ncart(lb), nc = ncart(lc), nd = ncart(ld);
   for (int ia = 0; ia < na; ++ia)
     for (int ib = 0; ib < nb; ++ib) {
@@ -234,17 +274,17 @@ void prim_quartet(const PairData& ab, int la, int lb, bool auxa, bool auxb, cons
       for (int ic = 0; ic < nc; ++ic)
         for (int id = 0; id < nd; ++id) {
           const int *C = cart(lc, ic, auxc), *D = cart(ld, id, auxd);
-          double s = 0;
+          T s = 0;
           for (int t = 0; t <= A[0] + B[0]; ++t)
             for (int u = 0; u <= A[1] + B[1]; ++u)
               for (int v = 0; v <= A[2] + B[2]; ++v) {
-                double eab = ab.E[0].e[A[0]][B[0]][t] * ab.E[1].e[A[1]][B[1]][u] * ab.E[2].e[A[2]][B[2]][v];
-                if (eab == 0.0) continue;
+                T eab = T(ab.E[0].e[A[0]][B[0]][t]) * T(ab.E[1].e[A[1]][B[1]][u]) * T(ab.E[2].e[A[2]][B[2]][v]);
+                if (eab == T(0.0)) continue;
                 for (int tt = 0; tt <= C[0] + D[0]; ++tt)
                   for (int uu = 0; uu <= C[1] + D[1]; ++uu)
                     for (int vv = 0; vv <= C[2] + D[2]; ++vv) {
-                      double ecd = cd.E[0].e[C[0]][D[0]][tt] * cd.E[1].e[C[1]][D[1]][uu] * cd.E[2].e[C[2]][D[2]][vv];
-                      double sg = ((tt + uu + vv) & 1) ? -1.0 : 1.0;
+                      T ecd = T(cd.E[0].e[C[0]][D[0]][tt]) * T(cd.E[1].e[C[1]][D[1]][uu]) * T(cd.E[2].e[C[2]][D[2]][vv]);
+                      T sg = ((tt + uu + vv) & 1) ? -1.0 : 1.0;
                       s += eab * sg * ecd * R.r[t + tt][u + uu][v + vv];
                     }
               }
@@ -1111,6 +1151,47 @@ double orc_eri_prim_md(const double* ctr, const double* ex, const int* ang) {
     }
   }
   return out[((idx[0] * ncart(L[1]) + idx[1]) * ncart(L[2]) + idx[2]) * ncart(L[3]) + idx[3]];
+}
+
+// FP64 work of one primitive quartet of class (la lb | lc ld) in the MD
+// routine above (prim_quartet: Boys, R recursion, E products, contraction
+// sum, prefactor), counted by running it on Op numbers, plus the
+// accumulation of the primitive block into the contracted one (one multiply
+// by the coefficient product and one add per component).  The reference-style
+// Boys function takes the branch of the argument x_boys (table Taylor
+// expansion when x < 0.3529 n + 0.0155, else the closed form and upward
+// recursion).  out[0] = arithmetic operations, out[1] = special-function
+// calls (exp, sqrt, erf, pow, floor).  A generic geometry: no E coefficient
+// vanishes, so nothing is skipped.
+int orc_md_flops(int la, int lb, int lc, int ld, double x_boys, const double* boys_table, double* out) {
+  if (la < 0 || lb < 0 || lc < 0 || ld < 0 || la + lb + lc + ld > 12 || !boys_table) return 1;
+  Prim q[4];
+  const double ex[4] = {1.3, 0.7, 0.9, 0.45};
+  const double c[4][3] = {{0.1, 0.2, 0.3}, {1.1, -0.4, 0.6}, {-0.5, 0.8, 1.7}, {0.7, 1.3, -0.9}};
+  for (int k = 0; k < 4; ++k) {
+    q[k] = Prim{};
+    q[k].exponent = ex[k]; q[k].x = c[k][0]; q[k].y = c[k][1]; q[k].z = c[k][2];
+  }
+  PairData ab, cd;
+  make_pair(q[0], la, q[1], lb, ab);
+  make_pair(q[2], lc, q[3], ld, cd);
+  // place the pair centres so that alpha |PQ|^2 = x_boys
+  const double p = ab.p, qq = cd.p, alpha = p * qq / (p + qq);
+  const double d[3] = {ab.P[0] - cd.P[0], ab.P[1] - cd.P[1], ab.P[2] - cd.P[2]};
+  const double r = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+  const double want = std::sqrt(x_boys / alpha);
+  for (int k = 0; k < 3; ++k) cd.P[k] = ab.P[k] - d[k] * (want / r);
+  const int ncomp = ncart(la) * ncart(lb) * ncart(lc) * ncart(ld);
+  std::vector<Op> blk(ncomp);
+  const BoysMode saved = g_boys;
+  g_boys = BoysMode{1, boys_table};
+  Op::n_arith = 0;
+  Op::n_special = 0;
+  prim_quartet(ab, la, lb, false, false, cd, lc, ld, false, false, blk.data());
+  g_boys = saved;
+  out[0] = (double)Op::n_arith + 2.0 * ncomp;
+  out[1] = (double)Op::n_special;
+  return 0;
 }
 
 int orc_num_threads() { return omp_get_max_threads(); }

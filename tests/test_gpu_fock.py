@@ -286,6 +286,29 @@ def test_graph_replay_matches_eager(method, cuda):
     assert graph.stats()["graph_captures"] == 2
 
 
+def test_class_stats_counts_primitive_quartets(cuda):
+    """gansu_fock_class_stats: per quartet class, the contracted quartets the
+    builds visit and the primitive quartets among them that pass the
+    primitive Schwarz screen.  Water/STO-3G has three primitives in every
+    shell, so with no screening (threshold 1e-30) each contracted quartet
+    holds 81 primitive quartets; at 1e-12 at most that; the class totals
+    match the engine statistics."""
+    mol = gm.Molecule.from_atoms(*gm.water(), gm.basis_path("sto-3g"))
+    for thr in (1e-30, 1e-12):
+        eng = FockEngine(mol, eri_method="direct", schwarz_screening_threshold=thr)
+        eng.precomputation()
+        cs = eng.class_stats()
+        st = eng.stats()
+        assert len(cs) == 21 and {c["L"] for c in cs} >= {(0, 0, 0, 0), (1, 1, 1, 1), (1, 0, 0, 0)}
+        assert sum(c["contracted"] for c in cs) == st["contracted_quartets"]
+        assert sum(c["primitive"] for c in cs) == st["primitive_quartets"] > 0
+        for c in cs:
+            if thr < 1e-20:
+                assert c["primitive"] == 81 * c["contracted"], c
+            else:
+                assert c["primitive"] <= 81 * c["contracted"], c
+
+
 def test_direct_incremental_fock_sequence(cuda):
     """incremental_fock=N (direct path): each build digests D - D_prev with
     density-weighted screening and adds to the previous J, K; every N-th is

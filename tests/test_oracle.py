@@ -574,3 +574,35 @@ def test_damped_scf_reaches_the_diis_energy():
     e_ref = O.rohf(pr, t, n, Zo, origin, 3, 2, conv=1e-12)[1]
     for kw in (dict(damping=0.9), dict(optimal=True)):
         assert abs(O.rohf(pr, t, n, Zo, origin, 3, 2, conv=1e-12, maxiter=500, **kw)[1] - e_ref) < 1e-9, kw
+
+
+def test_md_flop_counter_hand_counts():
+    """The op counter of the MD routine (orc_md_flops), pinned by hand counts
+    of the routine's operations.  (ss|ss) at x = 1 (Boys closed form with
+    exp and erf): alpha 3, P - Q 3, Boys argument 6, Boys 5, R^0 seed 3,
+    prefactor 6, the contraction term 8 and its scaling 1, accumulation 2:
+    37 operations; special calls sqrt, exp, sqrt, erf in Boys and sqrt in the
+    prefactor: 5.  At x = 30 (beyond the exp/erf range) one multiply and
+    exp/erf/one sqrt drop out: 36 and 2.  (ps|ss) at x = 1: Boys 8, R^0..1
+    seeds 6, three first-order R entries 3, per p component two Hermite
+    terms of 8 plus the scaling: 92."""
+    tab = O.boys_table()
+    assert O.md_flops(0, 0, 0, 0, 1.0, tab) == (37, 5)
+    assert O.md_flops(0, 0, 0, 0, 30.0, tab) == (36, 2)
+    assert O.md_flops(1, 0, 0, 0, 1.0, tab) == (92, 5)
+
+
+def test_md_flops_fixture_is_current():
+    """tests/golden/md_flops.json (bench.py's per-class algorithmic flops)
+    holds what the oracle counts now, for all 21 classes up to (dd|dd), and
+    the count grows with the class: adding angular momentum on any centre
+    never lowers it."""
+    fx = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "md_flops.json")))
+    tab = O.boys_table()
+    assert len(fx["classes"]) == 21
+    for key, (arith, special) in fx["classes"].items():
+        la, lb, lc, ld = map(int, key.split())
+        assert O.md_flops(la, lb, lc, ld, fx["x_boys"], tab) == (arith, special), key
+    c = {tuple(map(int, k.split())): v[0] for k, v in fx["classes"].items()}
+    assert c[(1, 1, 1, 1)] > c[(1, 1, 1, 0)] > c[(1, 0, 1, 0)] > c[(1, 0, 0, 0)] > c[(0, 0, 0, 0)]
+    assert c[(2, 2, 2, 2)] > c[(2, 2, 2, 1)] > c[(2, 2, 2, 0)]
