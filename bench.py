@@ -53,7 +53,7 @@ RIK_NAMES = {80020: "align (Procrustes)", 80021: "split E", 80022: "k_rik_tstep 
 FP64_VECTOR_PEAK_TFS = 78.6  # MI355X FP64 vector, TFLOP/s
 
 
-def md_algorithmic_rate(eng, build_s):
+def md_algorithmic_rate(eng, build_s, table="classes"):
     """Algorithmic FP64 work of a direct build: per quartet class the primitive
     quartets that pass the Schwarz screen (the engine's count) times the FP64
     operations of one primitive quartet in the oracle's McMurchie-Davidson
@@ -61,8 +61,9 @@ def md_algorithmic_rate(eng, build_s):
     op-counting numbers), over the build time.  The kernels factor the
     Hermite contraction (E tables and bra intermediates shared across
     components), so they execute fewer operations than this count; the rate
-    is the reference algorithm's work per second."""
-    fx = json.load(open(os.path.join(ROOT, "tests", "golden", "md_flops.json")))["classes"]
+    is the reference algorithm's work per second.  table="gradient_classes":
+    the nuclear-derivative routine's count, for the two-electron gradient."""
+    fx = json.load(open(os.path.join(ROOT, "tests", "golden", "md_flops.json")))[table]
     cs = eng.class_stats()
     flops = sum(c["primitive"] * fx["%d %d %d %d" % c["L"]][0] for c in cs)
     prim = sum(c["primitive"] for c in cs)
@@ -430,8 +431,17 @@ def gradient_leg(log_fn):
         oneint_gradient(mol, D, W)
     torch.cuda.synchronize()
     per = (time.perf_counter() - t0) / n
+    # two-electron part alone, for the algorithmic derivative-MD rate
+    t0 = time.perf_counter()
+    for _ in range(n):
+        eng.gradient_2e(D)
+    torch.cuda.synchronize()
+    per2 = (time.perf_counter() - t0) / n
+    md = md_algorithmic_rate(eng, per2, table="gradient_classes")
+    del eng
     g = r["grad"]
     out = dict(workload=f"C10H22 6-31G* RHF (M={mol.n_basis}, {mol.n_atoms} atoms)", gradient_s_per_call=round(per, 4),
+               gradient_2e_s=round(per2, 4), gradient_2e_md=md,
                scf_plus_gradient_wall_s=round(wall, 3), energy=r["e_total"],
                max_abs_gradient=float(np.max(np.abs(g))),
                translational_residual=float(np.max(np.abs(g.sum(axis=0)))))

@@ -101,6 +101,7 @@ inline Op operator/(Op a, Op b) { ++Op::n_arith; return Op(a.v / b.v); }
 inline Op operator-(Op a) { return Op(-a.v); }
 inline Op& operator+=(Op& a, Op b) { ++Op::n_arith; a.v += b.v; return a; }
 inline Op& operator*=(Op& a, Op b) { ++Op::n_arith; a.v *= b.v; return a; }
+inline Op& operator-=(Op& a, Op b) { ++Op::n_arith; a.v -= b.v; return a; }
 inline bool operator<(Op a, Op b) { return a.v < b.v; }
 inline bool operator>(Op a, Op b) { return a.v > b.v; }
 inline bool operator==(Op a, Op b) { return a.v == b.v; }
@@ -1194,6 +1195,14 @@ int orc_md_flops(int la, int lb, int lc, int ld, double x_boys, const double* bo
   return 0;
 }
 
+// The same count for the nuclear-derivative routine (grad_prim_quartet, the
+// body of orc_grad_2e): the R table one order higher, Gamma(D) per component
+// from a density of ones, and the 12 centre derivatives of every component
+// (one or two shifted Hermite contractions each), accumulated.  Norms enter
+// as per-component products (the primitive norm itself is a per-primitive
+// constant and is not counted).
+int orc_grad_md_flops(int la, int lb, int lc, int ld, double x_boys, const double* boys_table, double* out);
+
 int orc_num_threads() { return omp_get_max_threads(); }
 
 }  // extern "C"
@@ -1213,23 +1222,74 @@ int orc_num_threads() { return omp_get_max_threads(); }
 
 // Raw primitive (ab|cd) for explicit exponent vectors (E tables of ab/cd must
 // reach the requested orders; R must reach sum of all orders).
-static double md_raw4(const PairData& ab, const PairData& cd, const Rtab& R, const int* A, const int* B, const int* C,
-                      const int* D) {
-  double s = 0;
+template <class T>
+static T md_raw4(const PairData& ab, const PairData& cd, const RtabT<T>& R, const int* A, const int* B, const int* C,
+                 const int* D) {
+  T s = 0;
   for (int t = 0; t <= A[0] + B[0]; ++t)
     for (int u = 0; u <= A[1] + B[1]; ++u)
       for (int v = 0; v <= A[2] + B[2]; ++v) {
-        const double eab = ab.E[0].e[A[0]][B[0]][t] * ab.E[1].e[A[1]][B[1]][u] * ab.E[2].e[A[2]][B[2]][v];
-        if (eab == 0.0) continue;
+        const T eab = T(ab.E[0].e[A[0]][B[0]][t]) * T(ab.E[1].e[A[1]][B[1]][u]) * T(ab.E[2].e[A[2]][B[2]][v]);
+        if (eab == T(0.0)) continue;
         for (int tt = 0; tt <= C[0] + D[0]; ++tt)
           for (int uu = 0; uu <= C[1] + D[1]; ++uu)
             for (int vv = 0; vv <= C[2] + D[2]; ++vv) {
-              const double ecd = cd.E[0].e[C[0]][D[0]][tt] * cd.E[1].e[C[1]][D[1]][uu] * cd.E[2].e[C[2]][D[2]][vv];
-              const double sg = ((tt + uu + vv) & 1) ? -1.0 : 1.0;
+              const T ecd = T(cd.E[0].e[C[0]][D[0]][tt]) * T(cd.E[1].e[C[1]][D[1]][uu]) * T(cd.E[2].e[C[2]][D[2]][vv]);
+              const T sg = ((tt + uu + vv) & 1) ? -1.0 : 1.0;
               s += eab * sg * ecd * R.r[t + tt][u + uu][v + vv];
             }
       }
   return s;
+}
+
+// Gradient contribution of one primitive quartet f[0..3] (shell types l):
+// Gamma(D) times the centre derivatives of every component, added to gl.
+// deg: the permutational weight of the quartet.
+template <class T>
+static void grad_prim_quartet(const Prim* const f[4], const int l[4], const double* norm, size_t MM, const double* D,
+                              double deg, T* gl) {
+  using std::sqrt;
+  PairData ab, cd;
+  make_pair(*f[0], l[0] + 1, *f[1], l[1] + 1, ab);
+  make_pair(*f[2], l[2] + 1, *f[3], l[3] + 1, cd);
+  const T p = ab.p, q = cd.p;
+  RtabT<T> R;
+  hermite_r(l[0] + l[1] + l[2] + l[3] + 1, p * q / (p + q), T(ab.P[0]) - T(cd.P[0]), T(ab.P[1]) - T(cd.P[1]),
+            T(ab.P[2]) - T(cd.P[2]), R);
+  const T pref = T(2.0 * std::pow(M_PI, 2.5)) / (p * q * sqrt(p + q)) * T(ab.K) * T(cd.K);
+  const T cf = T(f[0]->coefficient) * T(f[1]->coefficient) * T(f[2]->coefficient) * T(f[3]->coefficient);
+  for (int i0 = 0; i0 < ncart(l[0]); ++i0)
+    for (int i1 = 0; i1 < ncart(l[1]); ++i1)
+      for (int i2 = 0; i2 < ncart(l[2]); ++i2)
+        for (int i3 = 0; i3 < ncart(l[3]); ++i3) {
+          const int ic[4] = {i0, i1, i2, i3};
+          size_t bf[4];
+          int e[4][3];
+          T nrm = 1.0;
+          for (int k = 0; k < 4; ++k) {
+            const int* c = cart(l[k], ic[k], false);
+            for (int d = 0; d < 3; ++d) e[k][d] = c[d];
+            bf[k] = f[k]->basis_index + ic[k];
+            nrm *= T(prim_norm(f[k]->exponent, c[0], c[1], c[2])) * T(norm[bf[k]]);
+          }
+          const size_t pp = bf[0], qq = bf[1], rr = bf[2], ss = bf[3];
+          const T gam = T(0.5) * T(D[pp * MM + qq]) * T(D[rr * MM + ss]) -
+                        T(0.125) * (T(D[pp * MM + rr]) * T(D[qq * MM + ss]) + T(D[pp * MM + ss]) * T(D[qq * MM + rr]));
+          if (gam == T(0.0)) continue;
+          const T fac = T(deg) * cf * nrm * gam * pref;
+          for (int k = 0; k < 4; ++k)
+            for (int d = 0; d < 3; ++d) {
+              const int e0 = e[k][d];
+              e[k][d] = e0 + 1;
+              T v = T(2.0 * f[k]->exponent) * md_raw4(ab, cd, R, e[0], e[1], e[2], e[3]);
+              if (e0 > 0) {
+                e[k][d] = e0 - 1;
+                v -= T(e0) * md_raw4(ab, cd, R, e[0], e[1], e[2], e[3]);
+              }
+              e[k][d] = e0;
+              gl[3 * f[k]->atom_index + d] += fac * v;
+            }
+        }
 }
 
 extern "C" {
@@ -1254,49 +1314,9 @@ int orc_grad_2e(const void* vprims, const void* vtypes, int ntypes, const double
         const Prim* f[4] = {&P[pl[x].a], &P[pl[x].b], &P[pl[y].a], &P[pl[y].b]};
         int l[4];
         for (int k = 0; k < 4; ++k) l[k] = f[k]->shell_type;
-        PairData ab, cd;
-        make_pair(*f[0], l[0] + 1, *f[1], l[1] + 1, ab);
-        make_pair(*f[2], l[2] + 1, *f[3], l[3] + 1, cd);
-        const double p = ab.p, q = cd.p;
-        Rtab R;
-        hermite_r(l[0] + l[1] + l[2] + l[3] + 1, p * q / (p + q), ab.P[0] - cd.P[0], ab.P[1] - cd.P[1],
-                  ab.P[2] - cd.P[2], R);
-        const double pref = 2.0 * std::pow(M_PI, 2.5) / (p * q * std::sqrt(p + q)) * ab.K * cd.K;
         const bool sb = pl[x].a == pl[x].b, sk = pl[y].a == pl[y].b, sbk = x == y;
         const double deg = (sb ? 1.0 : 2.0) * (sk ? 1.0 : 2.0) * (sbk ? 1.0 : 2.0);
-        const double cf = f[0]->coefficient * f[1]->coefficient * f[2]->coefficient * f[3]->coefficient;
-        for (int i0 = 0; i0 < ncart(l[0]); ++i0)
-          for (int i1 = 0; i1 < ncart(l[1]); ++i1)
-            for (int i2 = 0; i2 < ncart(l[2]); ++i2)
-              for (int i3 = 0; i3 < ncart(l[3]); ++i3) {
-                const int ic[4] = {i0, i1, i2, i3};
-                size_t bf[4];
-                int e[4][3];
-                double nrm = 1.0;
-                for (int k = 0; k < 4; ++k) {
-                  const int* c = cart(l[k], ic[k], false);
-                  for (int d = 0; d < 3; ++d) e[k][d] = c[d];
-                  bf[k] = f[k]->basis_index + ic[k];
-                  nrm *= prim_norm(f[k]->exponent, c[0], c[1], c[2]) * norm[bf[k]];
-                }
-                const size_t pp = bf[0], qq = bf[1], rr = bf[2], ss = bf[3];
-                const double gam = 0.5 * D[pp * MM + qq] * D[rr * MM + ss] -
-                                   0.125 * (D[pp * MM + rr] * D[qq * MM + ss] + D[pp * MM + ss] * D[qq * MM + rr]);
-                if (gam == 0.0) continue;
-                const double fac = deg * cf * nrm * gam * pref;
-                for (int k = 0; k < 4; ++k)
-                  for (int d = 0; d < 3; ++d) {
-                    const int e0 = e[k][d];
-                    e[k][d] = e0 + 1;
-                    double v = 2.0 * f[k]->exponent * md_raw4(ab, cd, R, e[0], e[1], e[2], e[3]);
-                    if (e0 > 0) {
-                      e[k][d] = e0 - 1;
-                      v -= e0 * md_raw4(ab, cd, R, e[0], e[1], e[2], e[3]);
-                    }
-                    e[k][d] = e0;
-                    gl[3 * f[k]->atom_index + d] += fac * v;
-                  }
-              }
+        grad_prim_quartet(f, l, norm, MM, D, deg, gl);
       }
     }
   }
@@ -1533,3 +1553,46 @@ int orc_grad_2c(const void* vaux, int naux_prims, const double* aux_norm, int Ma
 }
 
 }  // extern "C"
+
+extern "C" int orc_grad_md_flops(int la, int lb, int lc, int ld, double x_boys, const double* boys_table, double* out) {
+  if (la < 0 || lb < 0 || lc < 0 || ld < 0 || la + lb + lc + ld > 11 || !boys_table) return 1;
+  const int L[4] = {la, lb, lc, ld};
+  Prim q[4];
+  const double ex[4] = {1.3, 0.7, 0.9, 0.45};
+  const double c[4][3] = {{0.1, 0.2, 0.3}, {1.1, -0.4, 0.6}, {-0.5, 0.8, 1.7}, {0.7, 1.3, -0.9}};
+  size_t off = 0;
+  for (int k = 0; k < 4; ++k) {
+    q[k] = Prim{};
+    q[k].exponent = ex[k]; q[k].coefficient = 1.0; q[k].shell_type = L[k];
+    q[k].x = c[k][0]; q[k].y = c[k][1]; q[k].z = c[k][2];
+    q[k].basis_index = off; q[k].atom_index = k;
+    off += ncart(L[k]);
+  }
+  // pair centres moved along P - Q so that alpha |PQ|^2 = x_boys (as orc_md_flops)
+  PairData ab, cd;
+  make_pair(q[0], la + 1, q[1], lb + 1, ab);
+  make_pair(q[2], lc + 1, q[3], ld + 1, cd);
+  const double alpha = ab.p * cd.p / (ab.p + cd.p);
+  const double d[3] = {ab.P[0] - cd.P[0], ab.P[1] - cd.P[1], ab.P[2] - cd.P[2]};
+  const double r = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+  const double scale = std::sqrt(x_boys / alpha) / r;
+  // shift the ket primitives rigidly: P_cd moves by (scale - 1) (Q - P)
+  for (int k = 2; k < 4; ++k) {
+    q[k].x += (scale - 1.0) * -d[0];
+    q[k].y += (scale - 1.0) * -d[1];
+    q[k].z += (scale - 1.0) * -d[2];
+  }
+  const size_t M = off;
+  std::vector<double> D(M * M, 1.0), nrm(M, 1.0);
+  const Prim* f[4] = {&q[0], &q[1], &q[2], &q[3]};
+  std::vector<Op> gl(12);
+  const BoysMode saved = g_boys;
+  g_boys = BoysMode{1, boys_table};
+  Op::n_arith = 0;
+  Op::n_special = 0;
+  grad_prim_quartet(f, L, nrm.data(), M, D.data(), 1.0, gl.data());
+  g_boys = saved;
+  out[0] = (double)Op::n_arith;
+  out[1] = (double)Op::n_special;
+  return 0;
+}

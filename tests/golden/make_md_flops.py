@@ -1,6 +1,8 @@
 """Writes md_flops.json: the FP64 work of one primitive quartet of each
 four-centre class (orbital shells up to d) in the oracle's McMurchie-Davidson
-routine, counted by running that routine on op-counting numbers
+routine (`classes`) and in its nuclear-derivative routine
+(`gradient_classes`: the R table one order higher, Gamma(D) and the 12 centre
+derivatives of every component), counted by running them on op-counting numbers
 (oracle orc_md_flops: Boys, R recursion, E products, the Hermite contraction
 sum, the prefactor, and the multiply-add that accumulates the primitive block
 into the contracted one).  The reference-style Boys function is counted at
@@ -30,12 +32,14 @@ def classes(lmax=2):
 
 def main():
     tab = O.boys_table()
-    out = {}
+    out, grad = {}, {}
     for c in classes():
         arith, special = O.md_flops(*c, x_boys=X_BOYS, table=tab)
         out["%d %d %d %d" % c] = [arith, special]
+        arith, special = O.grad_md_flops(*c, x_boys=X_BOYS, table=tab)
+        grad["%d %d %d %d" % c] = [arith, special]
     doc = dict(x_boys=X_BOYS, unit="FP64 operations per primitive quartet (arithmetic, special-function calls)",
-               classes=out)
+               classes=out, gradient_classes=grad)
     with open(os.path.join(HERE, "md_flops.json"), "w") as f:
         json.dump(doc, f, indent=1, sort_keys=True)
         f.write("\n")

@@ -220,6 +220,18 @@ def md_flops(la, lb, lc, ld, x_boys=1.0, table=None):
     return int(out[0]), int(out[1])
 
 
+def grad_md_flops(la, lb, lc, ld, x_boys=1.0, table=None):
+    """(arithmetic ops, special-function calls) of one primitive quartet of
+    class (la lb | lc ld) in the oracle's derivative routine (orc_grad_md_flops)."""
+    tab = boys_table() if table is None else table
+    out = np.zeros(2)
+    rc = lib().orc_grad_md_flops(ctypes.c_int(la), ctypes.c_int(lb), ctypes.c_int(lc), ctypes.c_int(ld),
+                                 ctypes.c_double(x_boys), _ptr(tab), _ptr(out))
+    if rc:
+        raise ValueError("orc_grad_md_flops: bad class")
+    return int(out[0]), int(out[1])
+
+
 def eri_prim_md(ctr, ex, ang):
     ctr = np.ascontiguousarray(ctr, dtype=np.float64).reshape(-1)
     ex = np.ascontiguousarray(ex, dtype=np.float64)

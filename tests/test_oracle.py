@@ -590,6 +590,12 @@ def test_md_flop_counter_hand_counts():
     assert O.md_flops(0, 0, 0, 0, 1.0, tab) == (37, 5)
     assert O.md_flops(0, 0, 0, 0, 30.0, tab) == (36, 2)
     assert O.md_flops(1, 0, 0, 0, 1.0, tab) == (92, 5)
+    # derivative routine, (ss|ss): the (ps|ss)-order R table 3 + 3 + 23,
+    # prefactor 6, coefficient product 3, norms 8, Gamma 7, factor 4, and 12
+    # derivatives: a shifted bra index gives 2 Hermite terms of 8 ops, a
+    # shifted ket index 2 + 2 x 6, each plus 3 to scale and accumulate:
+    # 6 x 19 + 6 x 17 = 216; 273 in all
+    assert O.grad_md_flops(0, 0, 0, 0, 1.0, tab) == (273, 5)
 
 
 def test_md_flops_fixture_is_current():
@@ -599,10 +605,12 @@ def test_md_flops_fixture_is_current():
     never lowers it."""
     fx = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "md_flops.json")))
     tab = O.boys_table()
-    assert len(fx["classes"]) == 21
+    assert len(fx["classes"]) == 21 and len(fx["gradient_classes"]) == 21
     for key, (arith, special) in fx["classes"].items():
         la, lb, lc, ld = map(int, key.split())
         assert O.md_flops(la, lb, lc, ld, fx["x_boys"], tab) == (arith, special), key
+        assert O.grad_md_flops(la, lb, lc, ld, fx["x_boys"], tab) == tuple(fx["gradient_classes"][key]), key
+        assert fx["gradient_classes"][key][0] > 6 * arith  # 12 derivatives of every component
     c = {tuple(map(int, k.split())): v[0] for k, v in fx["classes"].items()}
     assert c[(1, 1, 1, 1)] > c[(1, 1, 1, 0)] > c[(1, 0, 1, 0)] > c[(1, 0, 0, 0)] > c[(0, 0, 0, 0)]
     assert c[(2, 2, 2, 2)] > c[(2, 2, 2, 1)] > c[(2, 2, 2, 0)]
