@@ -8,9 +8,10 @@
 //   K_n - K_prev = T_A T_E^T + T_E T_A^T,  T_A = B A, T_E = B E,
 // with T_A = T_prev + T_E / 2 where T_prev = B X_prev is kept from the last
 // build.  Two kernels carry the flops:
-//   k_rik_tstep:  T_E = B E over all rows (m, P) of B (FP32 copy of B, E in
-//                 fp16 hi/lo), updating T_A, T_E in place and the fitted
-//                 density change dW_P = 2 sum_{n,i} T_A[n][P][i] E[n][i];
+//   k_rik_tstep:  T_E = B E over all rows (m, P) of B (B as pre-split fp16
+//                 hi/lo planes, E^T in fp16 hi/lo), updating T_A, T_E in
+//                 place and the fitted density change
+//                 dW_P = 2 sum_{n,i} T_A[n][P][i] E[n][i];
 //   k_rik_dk:     Z = T_A T_E^T (K dimension Maux nocc), split-k partials;
 //                 K_n = K_prev + Z + Z^T is summed in FP64 (k_rik_reduce).
 // Each product runs as three fp16 MFMAs on a hi/lo split of both operands

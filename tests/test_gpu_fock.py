@@ -242,9 +242,9 @@ def test_ri_scf_occupied_form_energy(cuda):
 
 
 def test_ri_mixed_precision_energy(cuda):
-    """precision=mixed (occupied-form FP64 K; J as FP64 J of an earlier build
-    plus an FP32-B increment over the fitted-density change) stays within
-    1 micro-Eh of the FP64 density-form RI energy."""
+    """precision=mixed (K and J as increments on the fp16 matrix cores from
+    the orbital change, over FP64 builds) stays within 1 micro-Eh of the
+    FP64 density-form RI energy."""
     Z, xyz = methane_water()
     mol, _ = both(Z, xyz, "cc-pvdz")
     aux, _ = both(Z, xyz, "cc-pvdz-rifit")
@@ -406,7 +406,7 @@ def test_update_geometry_equals_fresh_engine(method, cuda):
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_ri_update_geometry_equals_fresh_engine(precision, cuda):
-    """RI engines keep B (and the FP32 copy, the increment state) across a
+    """RI engines keep B (and its fp16 planes, the increment state) across a
     new geometry: the builds after update_geometry equal a fresh engine's,
     for the mixed path over a sequence that starts with the FP64 build and
     continues with increments on the reused buffers."""
