@@ -71,7 +71,7 @@ struct FockEngine::RIState {
   double rik_drift = 0.0;  // sum of |E| / |X| over them
   int rik_incremental = 0, rik_fresh = 0, rik_full = 0;  // counts, for the caller's statistics
   DevBuf<float> TA, TE, Zpart;
-  DevBuf<double> Xprev, Xn, Ework, Y, Y2, G, Kprev, Jprev, Wprev, dW, dWrow, nrm;
+  DevBuf<double> Xprev, Xn, Ework, Y, Y2, G, Kprev, Jprev, Wprev, dW, dWrow, nrm, Zsum;
   DevBuf<uint16_t> Ehi, Elo;
   DevBuf<RikState> rst;
   DevBuf<unsigned> wgmax;  // per-workgroup maxima of the T update
@@ -357,6 +357,7 @@ void FockEngine::rik_prepare(int nocc) {
     r.Elo.alloc((size_t)rik_cols(nocc) * kpad);
     const size_t zn = (size_t)rik_chunks(M, (long)Ma * nocc) * M * M;
     if (r.Zpart.n != zn) r.Zpart.alloc(zn);
+    if (r.Zsum.n != (size_t)M * M) r.Zsum.alloc((size_t)M * M);
   }
   r.rik_nocc = nocc;
 }
@@ -484,7 +485,7 @@ bool FockEngine::build_ri_incremental(const double* d_C, int nocc, double* J, do
   GANSU_HIP(launch_rik_dk(da, nch, s));
   prof_end(s);
   prof_begin(s, 80026, 0.0, 0.0, 1.0);
-  GANSU_HIP(launch_rik_reduce(r.Zpart.ptr, nch, M, fresh ? nullptr : r.Kprev.ptr, K, s));
+  GANSU_HIP(launch_rik_reduce(r.Zpart.ptr, nch, M, r.Zsum.ptr, fresh ? nullptr : r.Kprev.ptr, K, s));
   GANSU_HIP(hipMemcpyAsync(r.Kprev.ptr, K, sizeof(double) * M * M, hipMemcpyDeviceToDevice, s));
   // X_prev <- X_prev + E = X_n R, the factor that B X_prev = T_A + T_E / 2 now holds
   blas_ok(rocblas_daxpy(r.h, (int)mn, &one, r.Ework.ptr, 1, r.Xprev.ptr, 1), "rocblas_daxpy(X)");

@@ -130,7 +130,9 @@ hipError_t launch_rik_split_e(const double* E, int M, int nocc, int kpad, RikSta
 hipError_t launch_rik_tstep(const RikTArgs& a, hipStream_t s);
 hipError_t launch_rik_dk(const RikDkArgs& a, int nchunk, hipStream_t s);
 int rik_dk_tiles(int M);  // output tiles of one k chunk of the dK product
-hipError_t launch_rik_reduce(const float* Zpart, int nchunk, int M, const double* Kprev, double* K, hipStream_t s);
+// S = sum of the Zpart chunks (FP64, M x M scratch), then K = Kprev + S + S^T
+hipError_t launch_rik_reduce(const float* Zpart, int nchunk, int M, double* S, const double* Kprev, double* K,
+                             hipStream_t s);
 hipError_t launch_rik_dw(const double* dWrow, int M, int Maux, double* dW, double* W, double* Wprev, hipStream_t s);
 hipError_t launch_rik_j(const uint16_t* Bp, float sB, int kpad, const double* dW, int M, int Maux, double* Jprev,
                         double* J, hipStream_t s);

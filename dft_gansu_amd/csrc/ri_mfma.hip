@@ -540,30 +540,62 @@ __global__ __launch_bounds__(kDkThreads) void k_rik_dk(RikDkArgs a) {
     }
 }
 
-// K = Kprev + S + S^T (Kprev null: 0) with S = sum over chunks of Zpart, in FP64; 32 x 32
-// tiles through LDS so that both S[m][n] and S[n][m] are read coalesced.
-__global__ __launch_bounds__(256) void k_rik_reduce(const float* __restrict__ Zpart, int nchunk, int M,
-                                                    const double* __restrict__ Kprev, double* __restrict__ K) {
+// S = sum over chunks of Zpart in FP64, chunk by chunk (the order fixes the
+// rounding): each thread four consecutive elements, four chunks' loads in
+// flight.  Every partial is read once.
+__global__ __launch_bounds__(256) void k_rik_sum(const float* __restrict__ Zpart, int nchunk, size_t mm,
+                                                 double* __restrict__ S) {
+  const size_t i0 = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) * 4;
+  if (i0 >= mm) return;
+  double s[4] = {0.0, 0.0, 0.0, 0.0};
+  if (i0 + 4 <= mm && (mm & 3) == 0) {
+    int ch = 0;
+    for (; ch + 4 <= nchunk; ch += 4) {
+      float4 v[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) v[u] = *(const float4*)(Zpart + (size_t)(ch + u) * mm + i0);
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        s[0] += (double)v[u].x;
+        s[1] += (double)v[u].y;
+        s[2] += (double)v[u].z;
+        s[3] += (double)v[u].w;
+      }
+    }
+    for (; ch < nchunk; ++ch) {
+      const float4 v = *(const float4*)(Zpart + (size_t)ch * mm + i0);
+      s[0] += (double)v.x;
+      s[1] += (double)v.y;
+      s[2] += (double)v.z;
+      s[3] += (double)v.w;
+    }
+  } else {
+    for (int ch = 0; ch < nchunk; ++ch)
+      for (int e = 0; e < 4; ++e)
+        if (i0 + e < mm) s[e] += (double)Zpart[(size_t)ch * mm + i0 + e];
+  }
+  for (int e = 0; e < 4; ++e)
+    if (i0 + e < mm) S[i0 + e] = s[e];
+}
+
+// K = Kprev + S + S^T (Kprev null: 0); 32 x 32 tiles through LDS so that
+// both S[m][n] and S[n][m] are read coalesced.
+__global__ __launch_bounds__(256) void k_rik_symm(const double* __restrict__ S, int M, const double* __restrict__ Kprev,
+                                                  double* __restrict__ K) {
   __shared__ double st[32][33];
   const int ntile = (M + 31) / 32;
   const int bi = blockIdx.x / ntile, bj = blockIdx.x % ntile;
   const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
-  const size_t mm = (size_t)M * M;
-  // S^T tile: rows bj*32.., cols bi*32.. of S, stored transposed
   for (int y = ty; y < 32; y += 8) {
     const int r = bj * 32 + y, c = bi * 32 + tx;
-    double s = 0.0;
-    if (r < M && c < M)
-      for (int ch = 0; ch < nchunk; ++ch) s += (double)Zpart[ch * mm + (size_t)r * M + c];
-    st[tx][y] = s;
+    st[tx][y] = (r < M && c < M) ? S[(size_t)r * M + c] : 0.0;
   }
   __syncthreads();
   for (int y = ty; y < 32; y += 8) {
     const int r = bi * 32 + y, c = bj * 32 + tx;
     if (r >= M || c >= M) continue;
-    double s = 0.0;
-    for (int ch = 0; ch < nchunk; ++ch) s += (double)Zpart[ch * mm + (size_t)r * M + c];
-    K[(size_t)r * M + c] = (Kprev ? Kprev[(size_t)r * M + c] : 0.0) + s + st[y][tx];
+    const size_t id = (size_t)r * M + c;
+    K[id] = (Kprev ? Kprev[id] : 0.0) + S[id] + st[y][tx];
   }
 }
 
@@ -779,9 +811,12 @@ hipError_t launch_rik_dk(const RikDkArgs& a, int nchunk, hipStream_t s) {
 
 int rik_dk_tiles(int M) { return ((M + kDkTM - 1) / kDkTM) * ((M + kDkTN - 1) / kDkTN); }
 
-hipError_t launch_rik_reduce(const float* Zpart, int nchunk, int M, const double* Kprev, double* K, hipStream_t s) {
+hipError_t launch_rik_reduce(const float* Zpart, int nchunk, int M, double* S, const double* Kprev, double* K,
+                             hipStream_t s) {
+  const size_t mm = (size_t)M * M;
+  hipLaunchKernelGGL(k_rik_sum, dim3((unsigned)((mm / 4 + 256) / 256)), dim3(256), 0, s, Zpart, nchunk, mm, S);
   const int nt = (M + 31) / 32;
-  hipLaunchKernelGGL(k_rik_reduce, dim3((unsigned)(nt * nt)), dim3(256), 0, s, Zpart, nchunk, M, Kprev, K);
+  hipLaunchKernelGGL(k_rik_symm, dim3((unsigned)(nt * nt)), dim3(256), 0, s, S, M, Kprev, K);
   return hipGetLastError();
 }
 
