@@ -369,6 +369,31 @@ constexpr int kDkWM = 4, kDkWN = 2;                 // waves along T_A and T_E r
 constexpr int kDkTM = 64 * kDkWM, kDkTN = 64 * kDkWN;  // tile rows of T_A, T_E
 constexpr int kDkThreads = 64 * kDkWM * kDkWN;
 
+// Instruction order of one loop half of k_rik_dk (per wave: 24 MFMAs, about
+// 140 VALU of the hi/lo split, 16 LDS reads, 8 LDS writes, 6 global loads).
+// Every wave of the workgroup runs the same phase between two barriers, so
+// the split of the next stage must be threaded between the MFMAs of this
+// one inside each wave: the first stage operands are read, then each MFMA
+// is followed by six VALU and one LDS read (first eight) or write, and the
+// second twelve also by a global load.
+__device__ __forceinline__ void dk_interleave() {
+  __builtin_amdgcn_sched_group_barrier(0x100, 8, 0);  // LDS reads, first k half
+#pragma unroll
+  for (int i = 0; i < 12; ++i) {
+    __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // MFMA
+    __builtin_amdgcn_sched_group_barrier(0x002, 6, 0);  // VALU
+    if (i < 8) __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);  // LDS read
+    else __builtin_amdgcn_sched_group_barrier(0x200, 1, 0);        // LDS write
+  }
+#pragma unroll
+  for (int i = 0; i < 12; ++i) {
+    __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+    __builtin_amdgcn_sched_group_barrier(0x002, 6, 0);
+    __builtin_amdgcn_sched_group_barrier(0x200, 1, 0);  // LDS write
+    __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);  // global load
+  }
+}
+
 template <bool VEC>
 __global__ __launch_bounds__(kDkThreads) void k_rik_dk(RikDkArgs a) {
   constexpr int NT = kDkThreads, NA = kDkTM * 8 / NT, NB = kDkTN * 8 / NT;  // float4 per thread and stage
@@ -512,14 +537,16 @@ __global__ __launch_bounds__(kDkThreads) void k_rik_dk(RikDkArgs a) {
   int s = 0;
   for (; s + 2 <= nst; s += 2) {
     // stage s in buffer 0, stage s + 1 in r1, stage s + 2 in r0
+    compute(0);
     stash(r1, 1);
     fetch(s + 3 < last ? s + 3 : last, r1);
-    compute(0);
+    dk_interleave();
     __syncthreads();
     // stage s + 1 in buffer 1, stage s + 2 in r0, stage s + 3 in r1
+    compute(1);
     stash(r0, 0);
     fetch(s + 4 < last ? s + 4 : last, r0);
-    compute(1);
+    dk_interleave();
     __syncthreads();
   }
   if (s < nst) compute(0);  // an odd last stage, already in buffer 0
