@@ -210,12 +210,22 @@ void FockEngine::precompute_ri() {
   GANSU_HIP(launch_int3c2e(a3, 0));
   mark("3-centre integrals");
   const double one = 1.0;
-  // column-major view of slab m is T_m^T (M x Maux, ld M): T_m^T <- T_m^T U^-1, U = L^T
-  blas_ok(rocblas_dtrsm_strided_batched(r.h, rocblas_side_right, rocblas_fill_upper, rocblas_operation_none,
-                                        rocblas_diagonal_non_unit, M, Ma, &one, r.L.ptr, Ma, 0, r.B.ptr, M,
-                                        (rocblas_stride)Ma * M, M),
-          "rocblas_dtrsm_strided_batched");
-  mark("B = L^-1 (mn|P)");
+  // column-major view of slab m is T_m^T (M x Maux, ld M): T_m^T <- T_m^T U^-1,
+  // U = L^T, row n by row n.  (mn|P) is symmetric in m and n, so slab m is
+  // solved only for its rows n >= m0, the first m of its block of kSymBlock
+  // slabs, and the rows below are copied from the slabs solved for them:
+  // 0.54 of the solve of all M^2 rows at M = 1600.
+  constexpr int kSymBlock = 128;
+  for (int m0 = 0; m0 < M; m0 += kSymBlock) {
+    const int cnt = std::min(kSymBlock, M - m0);
+    blas_ok(rocblas_dtrsm_strided_batched(r.h, rocblas_side_right, rocblas_fill_upper, rocblas_operation_none,
+                                          rocblas_diagonal_non_unit, M - m0, Ma, &one, r.L.ptr, Ma, 0,
+                                          r.B.ptr + (size_t)m0 * Ma * M + m0, M, (rocblas_stride)Ma * M, cnt),
+            "rocblas_dtrsm_strided_batched");
+  }
+  mark("B = L^-1 (mn|P), rows n >= m block");
+  GANSU_HIP(launch_ri_mirror(r.B.ptr, M, Ma, kSymBlock, 0));
+  mark("B mirror (n < m block)");
   if (mixed_) {
     const int kpad = rik_kpad(M);
     const size_t np = (size_t)M * Ma * kpad * 2;

@@ -343,6 +343,40 @@ hipError_t launch_int3c2e(const RI3cArgs& a, hipStream_t s) {
   return hipGetLastError();
 }
 
+// B[m][P][n] = B[n][P][m] where n lies in an earlier block of bs (a multiple
+// of 32) than m: 32 x 32 tiles of (m, n) through LDS, 64 P per workgroup.  A
+// 32-tile lies inside one block, so a tile is copied whole or not at all;
+// the copied entries are never read here (reads are at n-blocks above the
+// m-block of their slab, writes below), so the workgroups do not race.
+__global__ __launch_bounds__(256) void k_ri_mirror(double* __restrict__ B, int M, int Maux, int bs) {
+  __shared__ double t[32][33];
+  const int nt = (M + 31) / 32;
+  const int tm = blockIdx.x / nt, tn = blockIdx.x % nt;
+  if ((tn * 32) / bs >= (tm * 32) / bs) return;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  const int P1 = min(Maux, (int)blockIdx.y * 64 + 64);
+  for (int P = blockIdx.y * 64; P < P1; ++P) {
+    for (int y = ty; y < 32; y += 8) {
+      const int n = tn * 32 + y, m = tm * 32 + tx;
+      if (n < M && m < M) t[y][tx] = B[((size_t)n * Maux + P) * M + m];
+    }
+    __syncthreads();
+    for (int y = ty; y < 32; y += 8) {
+      const int m = tm * 32 + y, n = tn * 32 + tx;
+      if (n < M && m < M) B[((size_t)m * Maux + P) * M + n] = t[tx][y];
+    }
+    __syncthreads();
+  }
+}
+
+hipError_t launch_ri_mirror(double* B, int M, int Maux, int bs, hipStream_t s) {
+  if (bs <= 0 || bs % 32 != 0) return hipErrorInvalidValue;
+  const int nt = (M + 31) / 32;
+  hipLaunchKernelGGL(k_ri_mirror, dim3((unsigned)(nt * nt), (unsigned)((Maux + 63) / 64)), dim3(256), 0, s, B, M,
+                     Maux, bs);
+  return hipGetLastError();
+}
+
 hipError_t launch_ri_w(const double* B, const double* D, int M, int Maux, double* W, hipStream_t s) {
   hipLaunchKernelGGL(k_ri_w<double>, dim3(Maux), dim3(256), 0, s, B, D, M, Maux, W);
   return hipGetLastError();

@@ -79,6 +79,8 @@ hipError_t launch_fill_cD(const double* c, const double* D, int P0, int pc, int 
 
 hipError_t launch_int2c2e(const RI2cArgs& a, hipStream_t s);
 hipError_t launch_int3c2e(const RI3cArgs& a, hipStream_t s);
+// B[m][P][n] = B[n][P][m] for n in an earlier block of bs (bs % 32 == 0) than m
+hipError_t launch_ri_mirror(double* B, int M, int Maux, int bs, hipStream_t s);
 hipError_t launch_ri_w(const double* B, const double* D, int M, int Maux, double* W, hipStream_t s);
 hipError_t launch_ri_j(const double* B, const double* W, int M, int Maux, double* J, hipStream_t s);
 hipError_t launch_ri_w_from_t(const double* T, const double* C, int M, int pc, int nocc, int p0, double* W,
