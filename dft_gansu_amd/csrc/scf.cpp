@@ -507,8 +507,6 @@ ScfResult rhf_run(const Molecule& mol, const Molecule* aux, const Params& prm, b
   const std::string solver = prm.get("scf_solver");
   if (solver != "diagonalize" && solver != "purify") throw Error(GANSU_E_PARAM, "Unknown scf_solver: " + solver);
   const bool purify = solver == "purify";
-  if (purify && prm.get("eri_method") == "ri")
-    throw Error(GANSU_E_UNSUPPORTED, "scf_solver=purify needs eri_method stored or direct");
   if (purify && conv_method != "diis") throw Error(GANSU_E_UNSUPPORTED, "scf_solver=purify runs DIIS only");
   if (purify && (diis_size < 1 || diis_size > kGraphDiisMax))
     throw Error(GANSU_E_PARAM, "scf_solver=purify needs 1 <= diis_size <= " + std::to_string(kGraphDiisMax));
@@ -564,6 +562,89 @@ ScfResult rhf_run(const Molecule& mol, const Molecule* aux, const Params& prm, b
     // evec column-major: row view is V^T, so C = X V = X (row view)^T
     blas.gemm(false, true, n, w.X.ptr, w.evec.ptr, w.C.ptr);
   };
+  // RI engines with scf_solver=purify: the occupied orbitals of an iteration
+  // come from SP2 purification of Fo = X^T F X instead of the eigensolver
+  // (their builds need an occupied factor, so the captured loop of the
+  // four-centre engines does not apply).  The projector P takes the previous
+  // iteration's occupied orbitals Z (orthogonal basis, column-major n x nocc)
+  // into its range, Q0 = P Z, and a Cholesky QR (Q0^T Q0 = R^T R, Q = Q0 R^-1)
+  // makes them orthonormal: C_occ = X Q.  Any orthonormal basis of the
+  // occupied space gives the same D and RI exchange (the incremental RI-K
+  // aligns successive factors itself).  The first iteration, one whose
+  // purification does not reach idempotency and one whose Q0 is
+  // ill-conditioned (sigma_min / sigma_max < 1e-4) diagonalise instead.
+  const bool ri_purify = purify && ri;
+  DevBuf<double> sFo, sP0, sP1, sP, srows, spart, seps, sZ, sG;
+  DevBuf<int> sdone;
+  DevBuf<ScfDevState> sst;
+  int sp2_steps = 128, sp2_iters = 0, sp2_fallbacks = 0;
+  bool have_z = false, sp2_sized = false, last_sp2 = false;
+  if (ri_purify) {
+    for (auto* b : {&sFo, &sP0, &sP1, &sP}) b->alloc(nn);
+    srows.alloc(2 * (size_t)n);
+    spart.alloc(sp2_part_doubles(n));
+    sZ.alloc((size_t)n * nocc);
+    sG.alloc((size_t)nocc * nocc);
+    sst.alloc(1);
+    ScfDevState h0{};
+    GANSU_HIP(hipMemcpy(sst.ptr, &h0, sizeof h0, hipMemcpyHostToDevice));
+  }
+  auto keep_z = [&]() {  // the occupied eigenvectors of the last diagonalisation
+    GANSU_HIP(hipMemcpy(sZ.ptr, w.evec.ptr, sizeof(double) * n * nocc, hipMemcpyDeviceToDevice));
+    have_z = true;
+  };
+  auto coefficients_sp2 = [&](const double* Fm) -> bool {
+    blas.gemm(true, false, n, w.X.ptr, Fm, w.tmp.ptr);
+    blas.gemm(false, false, n, w.tmp.ptr, w.X.ptr, sFo.ptr);
+    if (seps.n != (size_t)sp2_steps + 1) {
+      seps.alloc(sp2_steps + 1);
+      sdone.alloc(sp2_steps + 1);
+    }
+    const Sp2Work sw{{sP0.ptr, sP1.ptr}, sP.ptr, srows.ptr, spart.ptr, seps.ptr, sdone.ptr, sp2_workgroups(n)};
+    GANSU_HIP(launch_sp2(sFo.ptr, n, nocc, sp2_steps, sw, sst.ptr, 0));
+    int used = -1;
+    GANSU_HIP(hipMemcpy(&used, &sst.ptr->sp2_used, sizeof used, hipMemcpyDeviceToHost));
+    if (used < 0) {
+      if (sp2_steps >= 1024)
+        throw Error(GANSU_E_NUMERIC, "density purification did not reach idempotency (no gap at the Fermi level?)");
+      sp2_steps *= 2;
+      return false;
+    }
+    if (!sp2_sized) {  // the first density's need with a margin (later steps cost a launch each)
+      sp2_steps = std::min(sp2_steps, std::max(48, (used * 3 / 2 + 16 + 7) / 8 * 8));
+      sp2_sized = true;
+    }
+    const double one = 1.0, zero = 0.0;
+    double* Q = w.tmp.ptr;  // n x nocc, column-major
+    blas_check(rocblas_dgemm(blas.h, rocblas_operation_none, rocblas_operation_none, n, nocc, n, &one, sP.ptr, n,
+                             sZ.ptr, n, &zero, Q, n),
+               "rocblas_dgemm(P Z)");
+    blas_check(rocblas_dgemm(blas.h, rocblas_operation_transpose, rocblas_operation_none, nocc, nocc, n, &one, Q, n,
+                             Q, n, &zero, sG.ptr, nocc),
+               "rocblas_dgemm(Q0^T Q0)");
+    blas_check(rocsolver_dpotrf(blas.h, rocblas_fill_upper, nocc, sG.ptr, nocc, w.info.ptr), "rocsolver_dpotrf");
+    int info = 0;
+    GANSU_HIP(hipMemcpy(&info, w.info.ptr, sizeof info, hipMemcpyDeviceToHost));
+    if (info != 0) return false;
+    std::vector<double> rd(nocc);
+    GANSU_HIP(hipMemcpy2D(rd.data(), sizeof(double), sG.ptr, (size_t)(nocc + 1) * sizeof(double), sizeof(double),
+                          nocc, hipMemcpyDeviceToHost));
+    double rmin = std::numeric_limits<double>::max(), rmax = 0.0;
+    for (double v : rd) {
+      rmin = std::min(rmin, std::fabs(v));
+      rmax = std::max(rmax, std::fabs(v));
+    }
+    if (!(rmin > 1e-4 * rmax)) return false;
+    blas_check(rocblas_dtrsm(blas.h, rocblas_side_right, rocblas_fill_upper, rocblas_operation_none,
+                             rocblas_diagonal_non_unit, n, nocc, &one, sG.ptr, nocc, Q, n),
+               "rocblas_dtrsm(Q0 R^-1)");
+    GANSU_HIP(hipMemcpy(sZ.ptr, Q, sizeof(double) * n * nocc, hipMemcpyDeviceToDevice));
+    // rows i < nocc of the column-major view of row-major C: C^T[i][mu] = (Q^T X^T)[i][mu]
+    blas_check(rocblas_dgemm(blas.h, rocblas_operation_transpose, rocblas_operation_none, nocc, n, n, &one, sZ.ptr,
+                             n, w.X.ptr, n, &zero, w.C.ptr, n),
+               "rocblas_dgemm(C_occ)");
+    return true;
+  };
   auto density = [&]() {
     // D = 2 C_occ C_occ^T (gpu_kernels.cu:54-69).  The column-major view of
     // row-major C is C^T, whose first nocc rows are the occupied orbitals.
@@ -606,6 +687,7 @@ ScfResult rhf_run(const Molecule& mol, const Molecule* aux, const Params& prm, b
       }
       coefficients(guess == "gwh" ? guess_F.ptr : w.H.ptr);
     }
+    if (ri_purify) keep_z();
     density();
     fock();
   }
@@ -617,11 +699,12 @@ ScfResult rhf_run(const Molecule& mol, const Molecule* aux, const Params& prm, b
   // convergence state
   Diis diis;
   Damper damper;
-  if (conv_method == "diis" && !purify) diis.init(diis_size, nn);
-  else if (!purify) damper.init(nn, conv_method == "optimaldamping", damp);
+  const bool graph = purify && !ri;  // the captured loop of the four-centre engines
+  if (conv_method == "diis" && !graph) diis.init(diis_size, nn);
+  else if (!graph) damper.init(nn, conv_method == "optimaldamping", damp);
   double prev = std::numeric_limits<double>::max(), E = 0.0;
   int it = 0;
-  if (purify) {
+  if (graph) {
     GraphLoop gl = graph_loop(eng, w, nocc, maxiter, conv, diis_size, diis_x, timing, trace);
     E = gl.E;
     it = gl.iterations;
@@ -633,9 +716,19 @@ ScfResult rhf_run(const Molecule& mol, const Molecule* aux, const Params& prm, b
     syevd(blas.h, n, w.tmp2.ptr, w);
     blas.gemm(false, true, n, w.X.ptr, w.evec.ptr, w.C.ptr);
   }
-  while (!purify && it < maxiter) {
+  while (!graph && it < maxiter) {
     double t0 = now();
-    coefficients(w.F.ptr);
+    if (ri_purify && have_z && coefficients_sp2(w.F.ptr)) {
+      ++sp2_iters;
+      last_sp2 = true;
+    } else {
+      coefficients(w.F.ptr);
+      if (ri_purify) {
+        keep_z();
+        ++sp2_fallbacks;
+        last_sp2 = false;
+      }
+    }
     double t1 = now();
     density();
     double t2 = now();
@@ -666,10 +759,18 @@ ScfResult rhf_run(const Molecule& mol, const Molecule* aux, const Params& prm, b
     }
     t_conv += now() - tc0;
   }
-  if (timing && purify)
+  if (ri_purify && last_sp2) {
+    // canonical orbitals and energies of the Fock matrix that gave the last D
+    syevd(blas.h, n, sFo.ptr, w);
+    blas.gemm(false, true, n, w.X.ptr, w.evec.ptr, w.C.ptr);
+  }
+  if (timing && ri_purify)
+    std::fprintf(stderr, "[scf] purification for %d iterations, eigensolver for %d (%d purification steps)\n",
+                 sp2_iters, sp2_fallbacks, sp2_steps);
+  if (timing && graph)
     std::fprintf(stderr, "[scf] %d iterations in %d graph launches: %.3f ms per iteration\n", it + 1,
                  res.graph_launches, 1e3 * t_fock / std::max(1, res.graph_launches));
-  else if (timing)
+  if (timing && !graph)
     std::fprintf(stderr, "[scf] %d iterations: diag %.2f ms, density %.2f ms, fock %.2f ms, energy %.2f ms, "
                          "convergence %.2f ms (per iteration)\n", it + 1, 1e3 * t_diag / (it + 1),
                  1e3 * t_dens / (it + 1), 1e3 * t_fock / (it + 1), 1e3 * t_energy / (it + 1), 1e3 * t_conv / (it + 1));

@@ -248,13 +248,18 @@ int gansu_export_sad_cache(const gansu_molecule* mol, const gansu_molecule* aux_
  * D_out (host, M*M) may be NULL.  info[4] = {iterations, fock builds,
  * converged, graph launches}.
  * Engine key scf_solver (no reference counterpart): "diagonalize" (default,
- * the reference's eigensolver step) or "purify" (eri_method stored or
- * direct, DIIS): each iteration after the guess is one captured HIP graph
- * -- X^T F X on the FP64 matrix cores, SP2 density purification in place of
- * the eigensolver, D, the Fock build, the energy and convergence test and
- * the DIIS step as device kernels -- launched once per iteration with one
- * 56-byte read-back; the final orbitals come from one eigensolve of the
- * last iteration's X^T F X.  info[3] counts those launches (0 otherwise).   */
+ * the reference's eigensolver step) or "purify" (DIIS only).  With
+ * eri_method stored or direct each iteration after the guess is one captured
+ * HIP graph -- X^T F X on the FP64 matrix cores, SP2 density purification in
+ * place of the eigensolver, D, the Fock build, the energy and convergence
+ * test and the DIIS step as device kernels -- launched once per iteration
+ * with one 56-byte read-back; info[3] counts those launches (0 otherwise).
+ * With eri_method=ri the loop stays eager (the RI builds take an occupied
+ * factor): each iteration after the first purifies X^T F X by SP2 and takes
+ * the occupied orbitals as a Cholesky QR of the previous ones projected by
+ * the result, falling back to the eigensolver for an iteration whose
+ * purification or projection fails.  Either way the final orbitals come
+ * from one eigensolve of the last iteration's X^T F X.                      */
 int gansu_rhf_run(const gansu_molecule* mol, const gansu_molecule* aux_mol, const gansu_params* p, double* energies,
                   double* D_out, int* info);
 /* HF::solve(Da0, Db0, force_density) for RHF (rhf.cu:122-134,

@@ -246,15 +246,14 @@ def test_water_box_geometry():
 
 def test_scf_solver_parameter_validation():
     """scf_solver is checked before any device work: an unknown value, the
-    purification loop with damping, with RI or with a DIIS history beyond
-    the device solver's 16 slots are refused with their error codes."""
+    purification loop with damping or with a DIIS history beyond the device
+    solver's 16 slots are refused with their error codes."""
     from dft_gansu_amd import Molecule, Parameters, basis_path, rhf
     from dft_gansu_amd._lib import GansuError
     Z, xyz = gm.water()
     mol = Molecule.from_atoms(Z, xyz, basis_path("sto-3g"))
     cases = [(dict(scf_solver="bogus"), "GANSU_E_PARAM"),
              (dict(scf_solver="purify", convergence_method="damping"), "GANSU_E_UNSUPPORTED"),
-             (dict(scf_solver="purify", eri_method="ri"), "GANSU_E_UNSUPPORTED"),
              (dict(scf_solver="purify", diis_size=17), "GANSU_E_PARAM")]
     for kw, code in cases:
         with pytest.raises(GansuError, match=code):
