@@ -574,21 +574,42 @@ ScfResult rhf_run(const Molecule& mol, const Molecule* aux, const Params& prm, b
   // purification does not reach idempotency and one whose Q0 is
   // ill-conditioned (sigma_min / sigma_max < 1e-4) diagonalise instead.
   const bool ri_purify = purify && ri;
-  DevBuf<double> sFo, sP0, sP1, sP, srows, spart, seps, sZ, sG;
-  DevBuf<int> sdone;
-  DevBuf<ScfDevState> sst;
-  int sp2_steps = 128, sp2_iters = 0, sp2_fallbacks = 0;
-  bool have_z = false, sp2_sized = false, last_sp2 = false;
+  DevBuf<double> sFo, sP, sP2, srows, spart, strfr, sZ, sG;
+  int sp2_iters = 0, sp2_fallbacks = 0, sp2_steps = 0;
+  bool have_z = false, last_sp2 = false;
   if (ri_purify) {
-    for (auto* b : {&sFo, &sP0, &sP1, &sP}) b->alloc(nn);
+    for (auto* b : {&sFo, &sP, &sP2}) b->alloc(nn);
     srows.alloc(2 * (size_t)n);
-    spart.alloc(sp2_part_doubles(n));
+    spart.alloc(sp2e_part_doubles());
+    strfr.alloc(2);
     sZ.alloc((size_t)n * nocc);
     sG.alloc((size_t)nocc * nocc);
-    sst.alloc(1);
-    ScfDevState h0{};
-    GANSU_HIP(hipMemcpy(sst.ptr, &h0, sizeof h0, hipMemcpyHostToDevice));
   }
+  // SP2 (Niklasson, Phys. Rev. B 66, 155115 (2002)) of sFo into sP, the
+  // products on rocBLAS, the branch and stop decisions on the host from
+  // (tr P, tr P^2) as in the captured loop's kernels: P <- P^2 while tr P >
+  // nocc, else 2P - P^2, until the idempotency error tr P - tr P^2 is below
+  // 1e-4 and no longer falls between every second step.  Returns the steps
+  // taken, -1 if 256 did not reach that floor.
+  auto sp2_eager = [&]() -> int {
+    double h[2];
+    GANSU_HIP(launch_sp2e_init(sFo.ptr, n, srows.ptr, sP.ptr, spart.ptr, strfr.ptr, 0));
+    GANSU_HIP(hipMemcpy(h, strfr.ptr, sizeof h, hipMemcpyDeviceToHost));
+    std::vector<double> eps;
+    const double one = 1.0, zero = 0.0;
+    for (int k = 1; k <= 256; ++k) {
+      const double e = h[0] - h[1];  // idempotency error of P_(k-1)
+      const bool stop = !(e > 0.0) || (k >= 3 && e < 1e-4 && e >= eps[k - 3]);
+      eps.push_back(e);
+      if (stop) return k - 1;
+      blas_check(rocblas_dgemm(blas.h, rocblas_operation_none, rocblas_operation_none, n, n, n, &one, sP.ptr, n,
+                               sP.ptr, n, &zero, sP2.ptr, n),
+                 "rocblas_dgemm(P P)");
+      GANSU_HIP(launch_sp2e_apply(sP.ptr, sP2.ptr, n, h[0] > nocc, spart.ptr, strfr.ptr, 0));
+      GANSU_HIP(hipMemcpy(h, strfr.ptr, sizeof h, hipMemcpyDeviceToHost));
+    }
+    return -1;
+  };
   auto keep_z = [&]() {  // the occupied eigenvectors of the last diagonalisation
     GANSU_HIP(hipMemcpy(sZ.ptr, w.evec.ptr, sizeof(double) * n * nocc, hipMemcpyDeviceToDevice));
     have_z = true;
@@ -596,24 +617,9 @@ ScfResult rhf_run(const Molecule& mol, const Molecule* aux, const Params& prm, b
   auto coefficients_sp2 = [&](const double* Fm) -> bool {
     blas.gemm(true, false, n, w.X.ptr, Fm, w.tmp.ptr);
     blas.gemm(false, false, n, w.tmp.ptr, w.X.ptr, sFo.ptr);
-    if (seps.n != (size_t)sp2_steps + 1) {
-      seps.alloc(sp2_steps + 1);
-      sdone.alloc(sp2_steps + 1);
-    }
-    const Sp2Work sw{{sP0.ptr, sP1.ptr}, sP.ptr, srows.ptr, spart.ptr, seps.ptr, sdone.ptr, sp2_workgroups(n)};
-    GANSU_HIP(launch_sp2(sFo.ptr, n, nocc, sp2_steps, sw, sst.ptr, 0));
-    int used = -1;
-    GANSU_HIP(hipMemcpy(&used, &sst.ptr->sp2_used, sizeof used, hipMemcpyDeviceToHost));
-    if (used < 0) {
-      if (sp2_steps >= 1024)
-        throw Error(GANSU_E_NUMERIC, "density purification did not reach idempotency (no gap at the Fermi level?)");
-      sp2_steps *= 2;
-      return false;
-    }
-    if (!sp2_sized) {  // the first density's need with a margin (later steps cost a launch each)
-      sp2_steps = std::min(sp2_steps, std::max(48, (used * 3 / 2 + 16 + 7) / 8 * 8));
-      sp2_sized = true;
-    }
+    const int used = sp2_eager();
+    if (used < 0) return false;
+    sp2_steps = std::max(sp2_steps, used);
     const double one = 1.0, zero = 0.0;
     double* Q = w.tmp.ptr;  // n x nocc, column-major
     blas_check(rocblas_dgemm(blas.h, rocblas_operation_none, rocblas_operation_none, n, nocc, n, &one, sP.ptr, n,
@@ -765,7 +771,7 @@ ScfResult rhf_run(const Molecule& mol, const Molecule* aux, const Params& prm, b
     blas.gemm(false, true, n, w.X.ptr, w.evec.ptr, w.C.ptr);
   }
   if (timing && ri_purify)
-    std::fprintf(stderr, "[scf] purification for %d iterations, eigensolver for %d (%d purification steps)\n",
+    std::fprintf(stderr, "[scf] purification for %d iterations, eigensolver for %d (at most %d SP2 steps)\n",
                  sp2_iters, sp2_fallbacks, sp2_steps);
   if (timing && graph)
     std::fprintf(stderr, "[scf] %d iterations in %d graph launches: %.3f ms per iteration\n", it + 1,

@@ -239,6 +239,83 @@ __global__ __launch_bounds__(256) void k_sp2_step(int n, double nocc, int k, con
   }
 }
 
+// Host-driven SP2 (the eager loop of the RI engines; the products run on
+// rocBLAS).  Both kernels write (trace, sum of squares) partials per
+// workgroup of a fixed grid, which k_sp2e_sum adds in a fixed order, so the
+// stop decisions are reproducible.
+constexpr int kSp2eBlocks = 512;
+
+// P0 = (emax I - Fo) / (emax - emin) from the Gershgorin rows of Fo.
+__global__ __launch_bounds__(256) void k_sp2e_init(const double* __restrict__ Fo, int n,
+                                                   const double* __restrict__ rows, double* __restrict__ P,
+                                                   double* __restrict__ part) {
+  __shared__ double lo[256], hi[256], sh[8];
+  const int t = threadIdx.x;
+  double l = DBL_MAX, h = -DBL_MAX;
+  for (int i = t; i < n; i += 256) {
+    l = fmin(l, rows[2 * i]);
+    h = fmax(h, rows[2 * i + 1]);
+  }
+  lo[t] = l;
+  hi[t] = h;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (t < o) {
+      lo[t] = fmin(lo[t], lo[t + o]);
+      hi[t] = fmax(hi[t], hi[t + o]);
+    }
+    __syncthreads();
+  }
+  const double emax = hi[0], width = hi[0] > lo[0] ? hi[0] - lo[0] : 1.0;
+  double tr = 0.0, fr = 0.0;
+  const size_t nn = (size_t)n * n;
+  for (size_t id = (size_t)blockIdx.x * 256 + t; id < nn; id += (size_t)gridDim.x * 256) {
+    const size_t i = id / n, j = id - i * n;
+    const double v = ((i == j ? emax : 0.0) - Fo[id]) / width;
+    P[id] = v;
+    if (i == j) tr += v;
+    fr += v * v;
+  }
+  block_sum256x2(tr, fr, sh);
+  if (t == 0) {
+    part[2 * blockIdx.x] = tr;
+    part[2 * blockIdx.x + 1] = fr;
+  }
+}
+
+// P <- P2 (square) or 2 P - P2, in place.
+__global__ __launch_bounds__(256) void k_sp2e_apply(double* __restrict__ P, const double* __restrict__ P2, int n,
+                                                    int square, double* __restrict__ part) {
+  __shared__ double sh[8];
+  double tr = 0.0, fr = 0.0;
+  const size_t nn = (size_t)n * n;
+  for (size_t id = (size_t)blockIdx.x * 256 + threadIdx.x; id < nn; id += (size_t)gridDim.x * 256) {
+    const double v = square ? P2[id] : 2.0 * P[id] - P2[id];
+    P[id] = v;
+    if (id % (size_t)(n + 1) == 0) tr += v;
+    fr += v * v;
+  }
+  block_sum256x2(tr, fr, sh);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = tr;
+    part[2 * blockIdx.x + 1] = fr;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_sp2e_sum(const double* __restrict__ part, double* __restrict__ out) {
+  __shared__ double sh[8];
+  double tr = 0.0, fr = 0.0;
+  for (int i = threadIdx.x; i < kSp2eBlocks; i += 256) {
+    tr += part[2 * i];
+    fr += part[2 * i + 1];
+  }
+  block_sum256x2(tr, fr, sh);
+  if (threadIdx.x == 0) {
+    out[0] = tr;
+    out[1] = fr;
+  }
+}
+
 // The projector of the stopped steps (P_used, in buffer used % 2) copied to
 // P_out; the last step's buffer if the steps never stopped.
 __global__ void k_sp2_final(int n, int steps, const double* __restrict__ b0, const double* __restrict__ b1,
@@ -463,6 +540,23 @@ hipError_t launch_sp2(const double* Fo, int n, int nocc, int steps, const Sp2Wor
   const size_t m = (size_t)n * n;
   hipLaunchKernelGGL(k_sp2_final, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, n, steps, w.buf[0], w.buf[1],
                      w.out, st);
+  return hipGetLastError();
+}
+
+size_t sp2e_part_doubles() { return 2 * (size_t)kSp2eBlocks; }
+
+hipError_t launch_sp2e_init(const double* Fo, int n, double* rows, double* P, double* part, double* trfr,
+                            hipStream_t s) {
+  hipLaunchKernelGGL(k_sp2_rows, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, s, Fo, n, rows);
+  hipLaunchKernelGGL(k_sp2e_init, dim3(kSp2eBlocks), dim3(256), 0, s, Fo, n, rows, P, part);
+  hipLaunchKernelGGL(k_sp2e_sum, dim3(1), dim3(256), 0, s, part, trfr);
+  return hipGetLastError();
+}
+
+hipError_t launch_sp2e_apply(double* P, const double* P2, int n, bool square, double* part, double* trfr,
+                             hipStream_t s) {
+  hipLaunchKernelGGL(k_sp2e_apply, dim3(kSp2eBlocks), dim3(256), 0, s, P, P2, n, square ? 1 : 0, part);
+  hipLaunchKernelGGL(k_sp2e_sum, dim3(1), dim3(256), 0, s, part, trfr);
   return hipGetLastError();
 }
 

@@ -54,6 +54,16 @@ int sp2_workgroups(int n);
 hipError_t launch_sp2(const double* Fo, int n, int nocc, int steps, const Sp2Work& w, ScfDevState* st,
                       hipStream_t s);
 
+// Host-driven SP2 for the eager loop (the products P P on rocBLAS by the
+// caller): init forms P0 from the Gershgorin interval of Fo (rows: 2n
+// doubles) and apply replaces P by P2 (square) or 2 P - P2; both leave
+// (tr P, sum of squares of P) in trfr[2].  part: sp2e_part_doubles().
+size_t sp2e_part_doubles();
+hipError_t launch_sp2e_init(const double* Fo, int n, double* rows, double* P, double* part, double* trfr,
+                            hipStream_t s);
+hipError_t launch_sp2e_apply(double* P, const double* P2, int n, bool square, double* part, double* trfr,
+                             hipStream_t s);
+
 // E = 1/2 sum D.(H + F); converged = |E - Eprev| < conv, else iter += 1 and
 // Eprev = E (the loop tail of hf.cu:278-307).  part: kEnergyBlocks doubles.
 constexpr int kEnergyBlocks = 32;
