@@ -38,8 +38,8 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
 MEASURED_TRAFFIC = {
     ("C10H22/cc-pVDZ", "fock_slab(pp|ps) stored-digest"): 274.7e6,
     # profiles/ri_mixed_64water_ccpvdz.md (FETCH_SIZE x2.0, WRITE_SIZE x1.0 by a 2 GiB copy)
-    ("64 waters", "k_rik_dk (Z = T_A T_E^T, fp16 MFMA)"): 139.6e9,
-    ("64 waters", "k_rik_tstep (T_E = B E, fp16 MFMA)"): 118.4e9,
+    ("64 waters", "k_rik_dk (Z = T_A T_E^T, fp16 MFMA)"): 143.0e9,
+    ("64 waters", "k_rik_tstep (T_E = B E, fp16 MFMA)"): 118.3e9,
 }
 TRAFFIC_SOURCE = "stored constant: rocprofv3 FETCH_SIZE / WRITE_SIZE per launch, scaled by a 2 GiB copy (profiles/)"
 FP16_MFMA_PEAK_TFS = 2500.0  # MI355X dense fp16 MFMA, TFLOP/s
