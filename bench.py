@@ -150,10 +150,11 @@ def ri_headline(steps, warmup, log_fn):
             continue  # a refresh build: not an incremental profile
         nrows += 1
         for kid, ms, by, fl, un in rows:
-            a = agg.setdefault(int(kid), [0.0, by, fl, un])
+            a = agg.setdefault(int(kid), [0.0, by, fl, un, 0])
             a[0] += ms
+            a[4] += 1
     for v in agg.values():
-        v[0] /= max(1, nrows)
+        v[0] /= max(1, v[4])  # per launch
     kernels = sorted(((RIK_NAMES.get(kid, str(kid)), v[0], v[1], v[2]) for kid, v in agg.items()), key=lambda r: -r[1])
     dom = kernels[0]
     tfs = dom[3] / (dom[1] * 1e-3) / 1e12
