@@ -292,7 +292,8 @@ void FockEngine::build_ri(const double* d_D, double* d_J, double* d_K, const dou
       GANSU_HIP(launch_ri_w(r.B.ptr, d_D, M, Ma, r.W.ptr, s));
       prof_end(s);
     }
-    prof_begin(s, 80002, bytesB, 2.0 * M * Ma * M, (double)M);
+    // J reads the rows of B up to the diagonal (B is symmetric in m, n)
+    prof_begin(s, 80002, 0.5 * double(M) * (M + 1) * Ma * sizeof(double), 2.0 * M * Ma * M, (double)M);
     GANSU_HIP(launch_ri_j(r.B.ptr, r.W.ptr, M, Ma, J, s));
     prof_end(s);
     if (mixed_ && occ && r.rik_nocc == nocc) {
@@ -483,7 +484,9 @@ bool FockEngine::build_ri_incremental(const double* d_C, int nocc, double* J, do
   prof_begin(s, 80023, double(M) * Ma * 8, 0.0, (double)Ma);
   GANSU_HIP(launch_rik_dw(r.dWrow.ptr, M, Ma, r.dW.ptr, r.W.ptr, r.Wprev.ptr, s));
   prof_end(s);
-  prof_begin(s, 80024, double(M) * Ma * kpad * 4, 2.0 * M * Ma * M, (double)M);
+  double jgroups = 0.0;  // 8-column groups the J increment reads per P (rows to their diagonal group)
+  for (int m = 0; m < M; ++m) jgroups += std::min(kpad / 8, (m >> 3) + 1);
+  prof_begin(s, 80024, jgroups * Ma * 32.0, 2.0 * M * Ma * M, (double)M);
   GANSU_HIP(launch_rik_j(r.Bp.ptr, r.sB, kpad, r.dW.ptr, M, Ma, r.Jprev.ptr, J, s));
   prof_end(s);
   // Z = T_A T_E^T over Kd = Maux nocc: 2 M^2 Kd flops, three fp16 products each

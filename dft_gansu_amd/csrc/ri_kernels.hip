@@ -246,11 +246,12 @@ __global__ void k_ri_w(const T* __restrict__ B, const double* __restrict__ D, in
   if (threadIdx.x == 0) W[P] = sh[0];
 }
 
-// J[m][n] = sum_P W_P B[m][P][n]: one workgroup per m.
+// J[m][n] = sum_P W_P B[m][P][n] for n <= m (B is symmetric in m and n; the
+// caller mirrors the rest): one workgroup per m.
 template <typename T>
 __global__ void k_ri_j(const T* __restrict__ B, const double* __restrict__ W, int M, int Maux, double* __restrict__ J) {
   const int m = blockIdx.x;
-  for (int n = threadIdx.x; n < M; n += blockDim.x) {
+  for (int n = threadIdx.x; n <= m; n += blockDim.x) {
     double s = 0.0;
     const T* col = B + (size_t)m * Maux * M + n;
     for (int P = 0; P < Maux; ++P) s += W[P] * (double)col[(size_t)P * M];
@@ -384,7 +385,7 @@ hipError_t launch_ri_w(const double* B, const double* D, int M, int Maux, double
 
 hipError_t launch_ri_j(const double* B, const double* W, int M, int Maux, double* J, hipStream_t s) {
   hipLaunchKernelGGL(k_ri_j<double>, dim3(M), dim3(256), 0, s, B, W, M, Maux, J);
-  return hipGetLastError();
+  return launch_mirror_upper(J, M, s);
 }
 
 hipError_t launch_ri_w_from_t(const double* T, const double* C, int M, int pc, int nocc, int p0, double* W,

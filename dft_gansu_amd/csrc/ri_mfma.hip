@@ -641,24 +641,28 @@ __global__ void k_rik_dw(const double* __restrict__ dWrow, int M, int Maux, doub
 
 // J[m][n] = Jprev[m][n] + sum_P dW_P B[m][P][n]; Jprev <- J, with B from
 // the planes as (hi + lo) / sB (exact sums in FP64; the 1 / sB power of two
-// is applied to the total).  A workgroup takes one m and 64 consecutive
-// 8-column groups: each lane one group (two 16-byte loads per P), the four
-// waves a quarter of P each, four P per round in flight; the wave partial
-// sums meet in LDS in wave order.
+// is applied to the total).  B and the planes are symmetric in m and n, so
+// row m forms only its 8-column groups up to the one holding n = m (half of
+// the planes' bytes); the caller mirrors J and Jprev.  A workgroup takes one
+// m and 64 consecutive 8-column groups: each lane one group (two 16-byte
+// loads per P), the four waves a quarter of P each, four P per round in
+// flight; the wave partial sums meet in LDS in wave order.
 __global__ __launch_bounds__(256) void k_rik_j(const uint16_t* __restrict__ Bp, float sB, int kpad,
                                                const double* __restrict__ dW, int M, int Maux,
                                                double* __restrict__ Jprev, double* __restrict__ J) {
   __shared__ double red[3][64][8];
   const int m = blockIdx.x, w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int ng = kpad / 8;
-  const int q = blockIdx.y * 64 + lane;  // 8-column group
+  if ((int)blockIdx.y * 64 > (m >> 3)) return;  // every group of the workgroup lies above the diagonal
+  const int q = blockIdx.y * 64 + lane;          // 8-column group
+  const int qmax = min(ng - 1, m >> 3);          // the groups of row m that are formed
   const int P0 = (int)((long)w * Maux / 4), P1 = (int)((long)(w + 1) * Maux / 4);
   const h8* slab = reinterpret_cast<const h8*>(Bp) + ((size_t)m * Maux * ng + q) * 2;
   const size_t pstride = (size_t)ng * 2;
   double s[8];
 #pragma unroll
   for (int e = 0; e < 8; ++e) s[e] = 0.0;
-  if (q < ng) {
+  if (q <= qmax) {
     int P = P0;
     for (; P + 4 <= P1; P += 4) {
       h8 hi[4], lo[4];
@@ -685,7 +689,7 @@ __global__ __launch_bounds__(256) void k_rik_j(const uint16_t* __restrict__ Bp, 
 #pragma unroll
     for (int e = 0; e < 8; ++e) red[w - 1][lane][e] = s[e];
   __syncthreads();
-  if (w != 0 || q >= ng) return;
+  if (w != 0 || q > qmax) return;
   const double inv = 1.0 / (double)sB;
 #pragma unroll
   for (int e = 0; e < 8; ++e) {
@@ -857,7 +861,10 @@ hipError_t launch_rik_j(const uint16_t* Bp, float sB, int kpad, const double* dW
   const int ng = kpad / 8;
   hipLaunchKernelGGL(k_rik_j, dim3((unsigned)M, (unsigned)((ng + 63) / 64)), dim3(256), 0, s, Bp, sB, kpad, dW, M,
                      Maux, Jprev, J);
-  return hipGetLastError();
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = launch_mirror_upper(J, M, s);
+  if (e == hipSuccess) e = launch_mirror_upper(Jprev, M, s);
+  return e;
 }
 
 hipError_t launch_rik_seed(const double* T, int M, int pc, int nocc, int p0, int Maux, float* TA, float* TE,
