@@ -523,7 +523,7 @@ __global__ __launch_bounds__((CoopShape<LA, LB, LC, LD>::NT)) void k_eri_coop(Cl
   __shared__ double sBp[NBG][6], sKp[KG][6];  // p, P[3], Kc, Q of the group's primitive pairs
   __shared__ double sF[KC][L + 1];        // scaled Boys values
   __shared__ double sPQ[KC][3];           // P - Q
-  __shared__ int sPair[KC], sN;           // surviving quartets of the chunk: bra << 8 | ket
+  __shared__ int sPair[KC];               // surviving quartets of the chunk: bra << 8 | ket
   __shared__ int sTuv[NH];                // t | u << 4 | v << 8 of each level-ordered Hermite index
   __shared__ int sAb[NAB], sCd[NCD];      // packed exponents of a, c (bits 0-5) and b, d (bits 6-11)
   const int tid = threadIdx.x;
@@ -590,17 +590,23 @@ __global__ __launch_bounds__((CoopShape<LA, LB, LC, LD>::NT)) void k_eri_coop(Cl
           __syncthreads();
           const int npair = nbl * kn;
           for (int p0 = 0; p0 < npair; p0 += KC) {
-            if (tid < 64) {  // wave 0 compacts the surviving quartets
-              const int pr = p0 + tid, ib = pr / kn, ik = pr - ib * kn;
-              const bool keep = tid < KC && pr < npair && sBp[ib][5] * sKp[ik][5] >= g.thr;
+            // every wave takes the same ballot over the chunk's quartets, so
+            // the survivor count needs no workgroup barrier; wave 0 writes
+            // the compacted list, and only wave 0 (the Boys step) reads it
+            // before the next barrier
+            int nkc;
+            {
+              const int lane = tid & 63;
+              const int pr = p0 + lane, ib = pr / kn, ik = pr - ib * kn;
+              const bool keep = lane < KC && pr < npair && sBp[ib][5] * sKp[ik][5] >= g.thr;
               const unsigned long long m = __ballot(keep);
-              if (keep) sPair[__popcll(m & ((1ull << tid) - 1ull))] = ib << 8 | ik;
-              if (tid == 0) sN = (int)__popcll(m);
+              if (tid < 64 && keep) sPair[__popcll(m & ((1ull << lane) - 1ull))] = ib << 8 | ik;
+              nkc = (int)__popcll(m);
+              __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+              __builtin_amdgcn_wave_barrier();
+              __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
             }
-            __syncthreads();
-            const int nkc = sN;
-            __syncthreads();  // sN is rewritten by the next chunk
-            if (nkc == 0) continue;
+            if (nkc == 0) continue;  // the same in every wave
             any = true;
             // 1. Boys values
             if (tid < nkc) {
