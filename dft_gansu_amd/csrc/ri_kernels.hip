@@ -247,16 +247,47 @@ __global__ void k_ri_w(const T* __restrict__ B, const double* __restrict__ D, in
 }
 
 // J[m][n] = sum_P W_P B[m][P][n] for n <= m (B is symmetric in m and n; the
-// caller mirrors the rest): one workgroup per m.
-template <typename T>
-__global__ void k_ri_j(const T* __restrict__ B, const double* __restrict__ W, int M, int Maux, double* __restrict__ J) {
-  const int m = blockIdx.x;
-  for (int n = threadIdx.x; n <= m; n += blockDim.x) {
-    double s = 0.0;
-    const T* col = B + (size_t)m * Maux * M + n;
-    for (int P = 0; P < Maux; ++P) s += W[P] * (double)col[(size_t)P * M];
-    J[(size_t)m * M + n] = s;
+// caller mirrors the rest).  A workgroup takes one m and 128 consecutive n
+// (two per lane), the four waves a quarter of P each with four P in flight;
+// the wave partial sums meet in LDS in wave order.
+__global__ __launch_bounds__(256) void k_ri_j(const double* __restrict__ B, const double* __restrict__ W, int M,
+                                              int Maux, double* __restrict__ J) {
+  __shared__ double red[3][64][2];
+  const int m = blockIdx.x, w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if ((int)blockIdx.y * 128 > m) return;  // every column of the workgroup lies above the diagonal
+  const int n0 = blockIdx.y * 128 + 2 * lane;
+  const int P0 = (int)((long)w * Maux / 4), P1 = (int)((long)(w + 1) * Maux / 4);
+  const double* col = B + (size_t)m * Maux * M + n0;
+  double s0 = 0.0, s1 = 0.0;
+  if (n0 <= m) {
+    const bool two = n0 + 1 < M;
+    int P = P0;
+    for (; P + 4 <= P1; P += 4) {
+      double a[4], b[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        a[u] = col[(size_t)(P + u) * M];
+        b[u] = two ? col[(size_t)(P + u) * M + 1] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        s0 += W[P + u] * a[u];
+        s1 += W[P + u] * b[u];
+      }
+    }
+    for (; P < P1; ++P) {
+      s0 += W[P] * col[(size_t)P * M];
+      if (two) s1 += W[P] * col[(size_t)P * M + 1];
+    }
   }
+  if (w > 0) {
+    red[w - 1][lane][0] = s0;
+    red[w - 1][lane][1] = s1;
+  }
+  __syncthreads();
+  if (w != 0 || n0 > m) return;
+  J[(size_t)m * M + n0] = ((s0 + red[0][lane][0]) + red[1][lane][0]) + red[2][lane][0];
+  if (n0 + 1 < M) J[(size_t)m * M + n0 + 1] = ((s1 + red[0][lane][1]) + red[1][lane][1]) + red[2][lane][1];
 }
 
 // W[p0 + p] += sum_m sum_i T[m][p][i] X[m][i] with X = sqrt(2) C_occ, i.e.
@@ -384,8 +415,9 @@ hipError_t launch_ri_w(const double* B, const double* D, int M, int Maux, double
 }
 
 hipError_t launch_ri_j(const double* B, const double* W, int M, int Maux, double* J, hipStream_t s) {
-  hipLaunchKernelGGL(k_ri_j<double>, dim3(M), dim3(256), 0, s, B, W, M, Maux, J);
-  return launch_mirror_upper(J, M, s);
+  hipLaunchKernelGGL(k_ri_j, dim3((unsigned)M, (unsigned)((M + 127) / 128)), dim3(256), 0, s, B, W, M, Maux, J);
+  const hipError_t e = hipGetLastError();
+  return e == hipSuccess ? launch_mirror_upper(J, M, s) : e;
 }
 
 hipError_t launch_ri_w_from_t(const double* T, const double* C, int M, int pc, int nocc, int p0, double* W,
