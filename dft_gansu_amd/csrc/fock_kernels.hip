@@ -686,14 +686,43 @@ __global__ __launch_bounds__((CoopShape<LA, LB, LC, LD>::NT)) void k_eri_coop(Cl
                   const double* ey = &sEk[ik][1][oy];
                   const double* ez = &sEk[ik][2][oz];
                   const double* R = sR[0] + k * NH;
-                  for (int tt = 0; tt <= cx + dx; ++tt)
-                    for (int uu = 0; uu <= cy + dy; ++uu) {
-                      const double exy = ((tt + uu) & 1) ? -ex[tt] * ey[uu] : ex[tt] * ey[uu];
-                      for (int vv = 0; vv <= cz + dz; ++vv) {
-                        const double e3 = (vv & 1) ? -exy * ez[vv] : exy * ez[vv];
-                        s += e3 * R[hidx(t + tt, u + uu, v + vv)];
+                  if constexpr (LCD <= 2) {
+                    // the quartet's ket E factors of this (c, d) into registers
+                    // once, so that each multiply-add reads only R from LDS
+                    // (the loops run to the class maximum, masked per lane).
+                    // (dp|pp) and (dp|ds) 29 -> 24 ms at C24H12; with LCD = 3
+                    // the registers cost a resident wave and (dp|dp) lost 15 %.
+                    double fx[LCD + 1], fy[LCD + 1], fz[LCD + 1];
+#pragma unroll
+                    for (int i = 0; i <= LCD; ++i) {
+                      fx[i] = i <= cx + dx ? ((i & 1) ? -ex[i] : ex[i]) : 0.0;
+                      fy[i] = i <= cy + dy ? ((i & 1) ? -ey[i] : ey[i]) : 0.0;
+                      fz[i] = i <= cz + dz ? ((i & 1) ? -ez[i] : ez[i]) : 0.0;
+                    }
+#pragma unroll
+                    for (int tt = 0; tt <= LCD; ++tt) {
+                      if (tt > cx + dx) continue;
+#pragma unroll
+                      for (int uu = 0; uu <= LCD - tt; ++uu) {
+                        if (uu > cy + dy) continue;
+                        const double exy = fx[tt] * fy[uu];
+#pragma unroll
+                        for (int vv = 0; vv <= LCD - tt - uu; ++vv) {
+                          if (vv > cz + dz) continue;
+                          s += exy * fz[vv] * R[hidx(t + tt, u + uu, v + vv)];
+                        }
                       }
                     }
+                  } else {
+                    for (int tt = 0; tt <= cx + dx; ++tt)
+                      for (int uu = 0; uu <= cy + dy; ++uu) {
+                        const double exy = ((tt + uu) & 1) ? -ex[tt] * ey[uu] : ex[tt] * ey[uu];
+                        for (int vv = 0; vv <= cz + dz; ++vv) {
+                          const double e3 = (vv & 1) ? -exy * ez[vv] : exy * ez[vv];
+                          s += e3 * R[hidx(t + tt, u + uu, v + vv)];
+                        }
+                      }
+                  }
                 }
                 sXO[cur * NX + it] += s;
               }
