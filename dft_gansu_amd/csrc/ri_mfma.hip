@@ -112,12 +112,31 @@ __global__ void k_rik_planes(const double* __restrict__ B, long R, int M, int kp
     const size_t r = i / ng;
     const int k0 = (int)(i % ng) * 8;
     h8 hi, lo;
+    if ((M & 1) == 0 && k0 + 8 <= M) {
+      // a whole group inside the row: four 16-byte loads (rows start 16-byte aligned for even M)
+      const double2* src = reinterpret_cast<const double2*>(B + r * M + k0);
+      double x[8];
 #pragma unroll
-    for (int e = 0; e < 8; ++e) {
-      _Float16 h = 0, l = 0;
-      if (k0 + e < M) split1((float)B[r * M + k0 + e], sB, h, l);
-      hi[e] = h;
-      lo[e] = l;
+      for (int j = 0; j < 4; ++j) {
+        const double2 v = src[j];
+        x[2 * j] = v.x;
+        x[2 * j + 1] = v.y;
+      }
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        _Float16 h, l;
+        split1((float)x[e], sB, h, l);
+        hi[e] = h;
+        lo[e] = l;
+      }
+    } else {
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        _Float16 h = 0, l = 0;
+        if (k0 + e < M) split1((float)B[r * M + k0 + e], sB, h, l);
+        hi[e] = h;
+        lo[e] = l;
+      }
     }
     *(h8*)(Bp + i * 16) = hi;
     *(h8*)(Bp + i * 16 + 8) = lo;
