@@ -1,9 +1,10 @@
 // One-electron integrals S, T, V (McMurchie-Davidson) and small dense helpers
 // used by the RHF driver.  The reference computes S/T by MD and V by
 // Obara-Saika per shell-type pair (int1e.cu, MD_kernel.txt, OS_kernel.txt);
-// here one lane evaluates one (mu, nu) basis-function pair.
+// here one wave evaluates one shell pair, its lanes splitting the nuclei.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -32,70 +33,144 @@ struct DPrim {
   int atom_index;
 };
 
-__global__ void k_oneint(OneIntArgs a) {
-  const size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+// S, T and V per shell-pair class (LA, LB): a wave takes one shell pair
+// (s1 >= s2, decoded from the triangular index; the pairs of other classes
+// are passed over) and every nucleus, 64 per round, one per lane.  Per
+// primitive pair three lanes build the E tables (two orders higher in b, for
+// the kinetic term) into the wave's LDS slot; lane ab < NAB forms S and T of
+// component ab, every lane R for its nucleus and V of all components, summed
+// over the wave at the end.  Every (mu, nu) of the pair is written once,
+// with its mirror, so no output needs clearing and the sums are in a fixed
+// order.  McMurchie-Davidson: S and T from the overlap E coefficients
+// (T by the j +- 2 recurrence), V = -sum_C Z_C 2 pi / p sum_tuv E E E R_tuv.
+constexpr int kOneIntWaves = 4;
+
+__device__ __forceinline__ double oneint_wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+template <int LA, int LB>
+__global__ __launch_bounds__(64 * kOneIntWaves) void k_oneint_t(OneIntArgs a) {
+  constexpr int NA = ncart(LA), NB = ncart(LB), NAB = NA * NB, LAB = LA + LB;
+  constexpr int NJ = LB + 3, NTT = LAB + 3, NE = (LA + 1) * NJ * NTT;
+  __shared__ double sE[kOneIntWaves][3][NE];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const long long npair = (long long)a.nshell * (a.nshell + 1) / 2;
+  const DPrim* prims = (const DPrim*)a.prims;
   const int M = a.M;
-  if (id >= (size_t)M * M) return;
-  const int mu = (int)(id / M), nu = (int)(id % M);
-  if (nu > mu) return;  // lower triangle, mirrored at the end
-  const int sa = a.bf_shell[mu], sb = a.bf_shell[nu];
-  const int ca = mu - a.shell_basis[sa], cb = nu - a.shell_basis[sb];
-  const int la = a.shell_L[sa], lb = a.shell_L[sb];
-  int ea[3], eb[3];
-  for (int d = 0; d < 3; ++d) { ea[d] = cart_exp(la, ca, d); eb[d] = cart_exp(lb, cb, d); }
-  double S = 0.0, T = 0.0, V = 0.0;
-  for (int ia = a.shell_off[sa]; ia < a.shell_off[sa + 1]; ++ia) {
-    const DPrim pa = ((const DPrim*)a.prims)[a.shell_prims[ia]];
-    for (int ib = a.shell_off[sb]; ib < a.shell_off[sb + 1]; ++ib) {
-      const DPrim pb = ((const DPrim*)a.prims)[a.shell_prims[ib]];
-      const double al = pa.exponent, be = pb.exponent, p = al + be;
-      const double A[3] = {pa.x, pa.y, pa.z}, B[3] = {pb.x, pb.y, pb.z};
-      double P[3], r2 = 0.0;
-      double E[3][3 * 5 * 8];  // (la+1) x (lb+3) x (la+lb+3), la, lb <= 2
-      const int nj = lb + 3, ntt = la + lb + 3;
-      for (int d = 0; d < 3; ++d) {
-        P[d] = (al * A[d] + be * B[d]) / p;
-        r2 += (A[d] - B[d]) * (A[d] - B[d]);
-        hermite_e_rt<double>(la, lb + 2, p, P[d] - A[d], P[d] - B[d], E[d]);
+  for (long long pid = (long long)blockIdx.x * kOneIntWaves + wave; pid < npair;
+       pid += (long long)gridDim.x * kOneIntWaves) {
+    long long r = (long long)((sqrt(8.0 * (double)pid + 1.0) - 1.0) / 2.0);
+    while ((r + 1) * (r + 2) / 2 <= pid) ++r;
+    while (r * (r + 1) / 2 > pid) --r;
+    const int s1 = (int)r, s2 = (int)(pid - r * (r + 1) / 2);
+    if (a.shell_L[s1] != LA || a.shell_L[s2] != LB) continue;  // wave-uniform
+    const int ab = lane < NAB ? lane : 0;
+    const int ea[3] = {cart_exp(LA, ab / NB, 0), cart_exp(LA, ab / NB, 1), cart_exp(LA, ab / NB, 2)};
+    const int eb[3] = {cart_exp(LB, ab % NB, 0), cart_exp(LB, ab % NB, 1), cart_exp(LB, ab % NB, 2)};
+    double S = 0.0, T = 0.0, vacc[NAB];
+#pragma unroll
+    for (int k = 0; k < NAB; ++k) vacc[k] = 0.0;
+    for (int ia = a.shell_off[s1]; ia < a.shell_off[s1 + 1]; ++ia)
+      for (int ib = a.shell_off[s2]; ib < a.shell_off[s2 + 1]; ++ib) {  // wave-uniform
+        const DPrim pa = prims[a.shell_prims[ia]], pb = prims[a.shell_prims[ib]];
+        const double al = pa.exponent, be = pb.exponent, p = al + be;
+        const double A[3] = {pa.x, pa.y, pa.z}, B[3] = {pb.x, pb.y, pb.z};
+        double P[3], r2 = 0.0;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          P[d] = (al * A[d] + be * B[d]) / p;
+          r2 += (A[d] - B[d]) * (A[d] - B[d]);
+        }
+        const double K = exp(-al * be / p * r2);
+        const double na = 0.71270547035499016035 * double(1 << LA) * pow(al, (2.0 * LA + 3.0) / 4.0);
+        const double nb = 0.71270547035499016035 * double(1 << LB) * pow(be, (2.0 * LB + 3.0) / 4.0);
+        const double pf = pa.coefficient * pb.coefficient * na * nb * K;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();  // the previous primitive pair's readers are done
+        if (lane < 3) {
+          double E[LA + 1][NJ][NTT];
+          const double Pd = lane == 0 ? P[0] : lane == 1 ? P[1] : P[2];
+          const double Ad = lane == 0 ? A[0] : lane == 1 ? A[1] : A[2];
+          const double Bd = lane == 0 ? B[0] : lane == 1 ? B[1] : B[2];
+          hermite_e_t<LA, LB + 2, double>(p, Pd - Ad, Pd - Bd, E);
+          double* dst = &sE[wave][lane][0];
+#pragma unroll
+          for (int i = 0; i <= LA; ++i)
+#pragma unroll
+            for (int j = 0; j < NJ; ++j)
+#pragma unroll
+              for (int t = 0; t < NTT; ++t) dst[(i * NJ + j) * NTT + t] = E[i][j][t];
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        const auto& E = *reinterpret_cast<const double(*)[3][LA + 1][NJ][NTT]>(&sE[wave][0][0]);
+        if (lane < NAB) {
+          double s1d[3], t1d[3];
+#pragma unroll
+          for (int d = 0; d < 3; ++d) {
+            const int i = ea[d], j = eb[d];
+            const double sij = E[d][i][j][0], sp2 = E[d][i][j + 2][0];
+            const double sm2 = j >= 2 ? E[d][i][j - 2][0] : 0.0;
+            s1d[d] = sij;
+            t1d[d] = -2.0 * be * be * sp2 + be * (2 * j + 1) * sij - 0.5 * j * (j - 1) * sm2;
+          }
+          const double ovl = pow(kPi / p, 1.5);
+          S += pf * ovl * s1d[0] * s1d[1] * s1d[2];
+          T += pf * ovl * (t1d[0] * s1d[1] * s1d[2] + s1d[0] * t1d[1] * s1d[2] + s1d[0] * s1d[1] * t1d[2]);
+        }
+        const double vpf = pf * 2.0 * kPi / p;
+        for (int c0 = 0; c0 < a.natoms; c0 += 64) {
+          const int c = c0 + lane;
+          if (c >= a.natoms) continue;
+          double R[nherm(LAB)];
+          hermite_r_t<LAB, double>(p, P[0] - a.atom_xyz[3 * c], P[1] - a.atom_xyz[3 * c + 1],
+                                   P[2] - a.atom_xyz[3 * c + 2], 1.0, a.boys, R);
+          const double zf = -(double)a.Z[c] * vpf;
+#pragma unroll
+          for (int k = 0; k < NAB; ++k) {
+            const int x0 = cart_exp(LA, k / NB, 0), x1 = cart_exp(LA, k / NB, 1), x2 = cart_exp(LA, k / NB, 2);
+            const int y0 = cart_exp(LB, k % NB, 0), y1 = cart_exp(LB, k % NB, 1), y2 = cart_exp(LB, k % NB, 2);
+            double acc = 0.0;
+#pragma unroll
+            for (int t = 0; t <= LAB; ++t) {
+              if (t > x0 + y0) continue;
+#pragma unroll
+              for (int u = 0; u <= LAB; ++u) {
+                if (u > x1 + y1) continue;
+                const double exy = E[0][x0][y0][t] * E[1][x1][y1][u];
+#pragma unroll
+                for (int v = 0; v <= LAB; ++v) {
+                  if (v > x2 + y2) continue;
+                  acc += exy * E[2][x2][y2][v] * R[hidx(t, u, v)];
+                }
+              }
+            }
+            vacc[k] += zf * acc;
+          }
+        }
       }
-      const double K = exp(-al * be / p * r2);
-      const double na = 0.71270547035499016035 * double(1 << la) * pow(al, (2.0 * la + 3.0) / 4.0);
-      const double nb = 0.71270547035499016035 * double(1 << lb) * pow(be, (2.0 * lb + 3.0) / 4.0);
-      const double pf = pa.coefficient * pb.coefficient * na * nb * K;
-      double s1[3], t1[3];
-      for (int d = 0; d < 3; ++d) {
-        const int i = ea[d], j = eb[d];
-        const double sij = E[d][(i * nj + j) * ntt];
-        const double sp2 = E[d][(i * nj + j + 2) * ntt];
-        const double sm2 = j >= 2 ? E[d][(i * nj + j - 2) * ntt] : 0.0;
-        s1[d] = sij;
-        t1[d] = -2.0 * be * be * sp2 + be * (2 * j + 1) * sij - 0.5 * j * (j - 1) * sm2;
+    const int b1 = a.shell_basis[s1], b2 = a.shell_basis[s2];
+#pragma unroll
+    for (int k = 0; k < NAB; ++k) {
+      const double v = oneint_wave_sum(vacc[k]);  // every lane reaches this point
+      if (lane == k) {
+        const int mu = b1 + k / NB, nu = b2 + k % NB;
+        const double ww = a.w[mu] * a.w[nu];
+        a.S[(size_t)mu * M + nu] = a.S[(size_t)nu * M + mu] = S * ww;
+        a.T[(size_t)mu * M + nu] = a.T[(size_t)nu * M + mu] = T * ww;
+        a.V[(size_t)mu * M + nu] = a.V[(size_t)nu * M + mu] = v * ww;
       }
-      const double ovl = pow(kPi / p, 1.5);
-      S += pf * ovl * s1[0] * s1[1] * s1[2];
-      T += pf * ovl * (t1[0] * s1[1] * s1[2] + s1[0] * t1[1] * s1[2] + s1[0] * s1[1] * t1[2]);
-      const int L = la + lb;
-      const double* Ex = E[0] + (ea[0] * nj + eb[0]) * ntt;
-      const double* Ey = E[1] + (ea[1] * nj + eb[1]) * ntt;
-      const double* Ez = E[2] + (ea[2] * nj + eb[2]) * ntt;
-      double vsum = 0.0;
-      for (int c = 0; c < a.natoms; ++c) {
-        double R[nherm(4)];
-        hermite_r_rt<double>(L, p, P[0] - a.atom_xyz[3 * c], P[1] - a.atom_xyz[3 * c + 1],
-                             P[2] - a.atom_xyz[3 * c + 2], 1.0, a.boys, R);
-        double acc = 0.0;
-        for (int t = 0; t <= ea[0] + eb[0]; ++t)
-          for (int u = 0; u <= ea[1] + eb[1]; ++u)
-            for (int v = 0; v <= ea[2] + eb[2]; ++v) acc += Ex[t] * Ey[u] * Ez[v] * R[hidx(t, u, v)];
-        vsum -= a.Z[c] * acc;
-      }
-      V += pf * 2.0 * kPi / p * vsum;
     }
   }
-  const double ww = a.w[mu] * a.w[nu];
-  a.S[(size_t)mu * M + nu] = a.S[(size_t)nu * M + mu] = S * ww;
-  a.T[(size_t)mu * M + nu] = a.T[(size_t)nu * M + mu] = T * ww;
-  a.V[(size_t)mu * M + nu] = a.V[(size_t)nu * M + mu] = V * ww;
+}
+
+template <int LA, int LB>
+void launch_oneint_class(const OneIntArgs& a, long long npair, hipStream_t s) {
+  const unsigned grid = (unsigned)std::min<long long>(16384, (npair + kOneIntWaves - 1) / kOneIntWaves);
+  hipLaunchKernelGGL((k_oneint_t<LA, LB>), dim3(grid), dim3(64 * kOneIntWaves), 0, s, a);
 }
 
 // X[i][k] = U_col[i + k n] * (s_k < thr ? 0 : s_k^-1/2)   (hf.cu:171-242,
@@ -162,8 +237,18 @@ __global__ void k_has_nan(const double* __restrict__ x, size_t n, int* flag) {
 }  // namespace
 
 hipError_t launch_oneint(const OneIntArgs& a, hipStream_t s) {
-  const size_t n = (size_t)a.M * a.M;
-  hipLaunchKernelGGL(k_oneint, dim3((unsigned)((n + 127) / 128)), dim3(128), 0, s, a);
+  // one launch per shell-pair class; each writes the blocks of its pairs
+  const long long npair = (long long)a.nshell * (a.nshell + 1) / 2;
+  if (npair <= 0) return hipSuccess;
+  launch_oneint_class<0, 0>(a, npair, s);
+  launch_oneint_class<0, 1>(a, npair, s);
+  launch_oneint_class<0, 2>(a, npair, s);
+  launch_oneint_class<1, 0>(a, npair, s);
+  launch_oneint_class<1, 1>(a, npair, s);
+  launch_oneint_class<1, 2>(a, npair, s);
+  launch_oneint_class<2, 0>(a, npair, s);
+  launch_oneint_class<2, 1>(a, npair, s);
+  launch_oneint_class<2, 2>(a, npair, s);
   return hipGetLastError();
 }
 
@@ -259,7 +344,7 @@ void oneint_noalloc(const OneIntTables& t, double* S, double* T, double* V, hipS
   });
   const OneIntArgs a{at[0], (const int*)at[1], (const int*)at[2], (const int*)at[3], (const int*)at[4],
                      (const int*)at[5], (const double*)at[7], t.M, t.natoms, (const double*)at[8],
-                     (const int*)at[6], (const double*)at[9], S, T, V, t.zsum};
+                     (const int*)at[6], (const double*)at[9], S, T, V, t.zsum, (int)t.sl.size()};
   hip_check(launch_oneint(a, s), "launch_oneint");
   // the host tables are released on return, and the next call reuses the array
   hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");

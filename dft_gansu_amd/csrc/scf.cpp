@@ -428,7 +428,7 @@ OneIntHost::OneIntHost(const Molecule& mol) : M(mol.nbasis), natoms((int)mol.ato
 
 OneIntArgs OneIntHost::args(double* S, double* T, double* V) const {
   return OneIntArgs{prims.ptr, off.ptr, sprims.ptr, sl.ptr, sb.ptr, bfs.ptr, w.ptr, M, natoms,
-                    xyz.ptr, Z.ptr, boys.ptr, S, T, V, zsum};
+                    xyz.ptr, Z.ptr, boys.ptr, S, T, V, zsum, (int)shell_L.size()};
 }
 
 void compute_oneint_noalloc(const Molecule& mol, double* dS, double* dT, double* dV, hipStream_t s) {

@@ -23,6 +23,7 @@ struct OneIntArgs {
   const double* boys;
   double *S, *T, *V;
   double zsum;             // sum of the nuclear charges
+  int nshell;              // contracted shells (entries of shell_L)
 };
 
 hipError_t launch_oneint(const OneIntArgs& a, hipStream_t s);
