@@ -714,6 +714,28 @@ __global__ __launch_bounds__((CoopShape<LA, LB, LC, LD>::NT)) void k_eri_coop(Cl
                         }
                       }
                     }
+                  } else if constexpr (LCD == 3) {
+                    // x and y factors in registers, z from LDS (the register
+                    // budget of three resident waves)
+                    double fx[LCD + 1], fy[LCD + 1];
+#pragma unroll
+                    for (int i = 0; i <= LCD; ++i) {
+                      fx[i] = i <= cx + dx ? ((i & 1) ? -ex[i] : ex[i]) : 0.0;
+                      fy[i] = i <= cy + dy ? ((i & 1) ? -ey[i] : ey[i]) : 0.0;
+                    }
+#pragma unroll
+                    for (int tt = 0; tt <= LCD; ++tt) {
+                      if (tt > cx + dx) continue;
+#pragma unroll
+                      for (int uu = 0; uu <= LCD - tt; ++uu) {
+                        if (uu > cy + dy) continue;
+                        const double exy = fx[tt] * fy[uu];
+                        for (int vv = 0; vv <= cz + dz; ++vv) {
+                          const double e3 = (vv & 1) ? -exy * ez[vv] : exy * ez[vv];
+                          s += e3 * R[hidx(t + tt, u + uu, v + vv)];
+                        }
+                      }
+                    }
                   } else {
                     for (int tt = 0; tt <= cx + dx; ++tt)
                       for (int uu = 0; uu <= cy + dy; ++uu) {
