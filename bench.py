@@ -331,6 +331,9 @@ def ri_leg(log_fn, mixed_rate, no_scf=False):
         r = scf_in_fresh_process(4, dict(eri_method="ri", convergence_energy_threshold=1e-9, maxiter=60, **kw), grad)
         scf[name] = r
         log_fn(f"[bench] 64-water RI SCF {name} (fresh process): {scf[name]}")
+    scf["note"] = ("each leg a fresh process, in this order; a process started right after one that freed B "
+                   "(126 GB) and its planes waits for the driver to clear that memory at its first allocation "
+                   "(precompute 5.0 s instead of 1.5 s, measured), so the later legs include seconds of clearing")
     scf["dE_mixed_vs_fp64"] = scf["mixed"]["energy"] - scf["fp64"]["energy"]
     scf["speedup_mixed_vs_fp64"] = round(scf["fp64"]["wall_s"] / scf["mixed"]["wall_s"], 2)
     out["scf"] = scf
