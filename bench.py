@@ -326,14 +326,22 @@ def ri_leg(log_fn, mixed_rate, no_scf=False):
     # each SCF in a fresh process, as a user's run would be: in a process
     # that has already freed a buffer of B's size (126 GB), allocating one
     # again costs ~2 s while the driver clears it
-    for name, kw, grad in (("fp64", dict(ri_k="occupied"), False), ("mixed", dict(precision="mixed"), False),
-                           ("mixed_with_gradient", dict(precision="mixed"), True)):
+    # (and a process started right after one that freed B and its planes
+    # waits for the driver to clear them: measured 5.0 s instead of 1.5 s of
+    # precompute; after a 30 s pause the next one precomputed in 1.4 s, so
+    # the legs are spaced to time each on a device the last one has left)
+    for i, (name, kw, grad) in enumerate((("fp64", dict(ri_k="occupied"), False),
+                                          ("mixed", dict(precision="mixed"), False),
+                                          ("mixed_with_gradient", dict(precision="mixed"), True))):
+        if i > 0:
+            log_fn("[bench] 15 s pause: the driver clears the memory the previous SCF process freed")
+            time.sleep(15.0)
         r = scf_in_fresh_process(4, dict(eri_method="ri", convergence_energy_threshold=1e-9, maxiter=60, **kw), grad)
         scf[name] = r
         log_fn(f"[bench] 64-water RI SCF {name} (fresh process): {scf[name]}")
-    scf["note"] = ("each leg a fresh process, in this order; a process started right after one that freed B "
+    scf["note"] = ("each leg a fresh process, 15 s apart: a process started right after one that freed B "
                    "(126 GB) and its planes waits for the driver to clear that memory at its first allocation "
-                   "(precompute 5.0 s instead of 1.5 s, measured), so the later legs include seconds of clearing")
+                   "(precompute 5.0 s instead of 1.5 s, measured); the pause lets the clearing finish first")
     scf["dE_mixed_vs_fp64"] = scf["mixed"]["energy"] - scf["fp64"]["energy"]
     scf["speedup_mixed_vs_fp64"] = round(scf["fp64"]["wall_s"] / scf["mixed"]["wall_s"], 2)
     out["scf"] = scf
